@@ -1,0 +1,476 @@
+// C ABI of the MI355X MCTDHX path: context, setup, right-hand side orchestration,
+// operator pieces and measurement.  Declared in include/cody_mctdhx.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cody_mctdhx.h"
+#include "host_setup.hpp"
+#include "mctdhx_kernels.hpp"
+#include "cody_ctx.hpp"
+
+using namespace cody;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(CODY_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+#define CHK(expr)           \
+  do {                      \
+    int r_ = (expr);        \
+    if (r_) return r_;      \
+  } while (0)
+
+namespace cody {
+
+int ctx_alloc(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(CODY_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) return fail(CODY_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  return CODY_OK;
+}
+
+int set_error(int code, const char* msg) { return fail(code, msg); }
+
+}  // namespace cody
+
+extern "C" {
+
+const char* cody_last_error(void) { return g_err.c_str(); }
+
+int cody_fedvr_npoints(int nE, int nLoc, int xminExcl, int xmaxExcl) {
+  if (nE < 1 || nLoc < 2) return -1;
+  return fedvr_npoints(nE, nLoc, xminExcl != 0, xmaxExcl != 0);
+}
+
+int cody_fedvr_grid(double xmin, double xmax, int nE, int nLoc, int xminExcl, int xmaxExcl,
+                    double* x, double* w) {
+  if (nE < 1 || nLoc < 2 || !(xmax > xmin) || !x || !w) return fail(CODY_ERR_ARG, "cody_fedvr_grid: bad arguments");
+  fedvr_grid(xmin, xmax, nE, nLoc, xminExcl != 0, xmaxExcl != 0, x, w);
+  return CODY_OK;
+}
+
+int cody_fedvr_kref(int nLoc, double* kref) {
+  if (nLoc < 2 || !kref) return fail(CODY_ERR_ARG, "cody_fedvr_kref: bad arguments");
+  fedvr_kref(nLoc, kref);
+  return CODY_OK;
+}
+
+int cody_create(cody_ctx** out, int device, int nG, int nBodies, int nOrbs, double reg) {
+  if (!out) return fail(CODY_ERR_ARG, "cody_create: null output");
+  *out = nullptr;
+  if (nG < 2 || nOrbs < 1 || nOrbs > kMaxOrbs || nBodies < 1 || nBodies > nOrbs)
+    return fail(CODY_ERR_UNSUPPORTED, "cody_create: need nG >= 2 and 1 <= nBodies <= nOrbs <= 12");
+  if (!(reg >= 0.0)) return fail(CODY_ERR_ARG, "cody_create: regularization must be >= 0");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(CODY_ERR_ARG, "cody_create: no such device");
+  HIPCHK(hipSetDevice(device));
+  auto* c = new cody_ctx();
+  c->device = device;
+  c->nG = nG;
+  c->nGp = 64 * ((nG + 63) / 64);
+  c->N = nBodies;
+  c->nO = nOrbs;
+  c->reg = reg;
+  c->P = nOrbs * (nOrbs + 1) / 2;
+  c->Pc = 16 * ((2 * c->P + 15) / 16);
+  std::string err;
+  if (!build_ci_tables(nOrbs, nBodies, c->ci, err)) {
+    delete c;
+    return fail(CODY_ERR_UNSUPPORTED, err);
+  }
+  c->nC = c->ci.nConf;
+  c->nS = c->nC + nG * nOrbs;
+  if (c->nC > 1024) {
+    delete c;
+    return fail(CODY_ERR_UNSUPPORTED, "cody_create: more than 1024 configurations");
+  }
+  // split-K factor of the mean-field contraction and x-split of the h2 Gram kernel
+  const int k16 = c->nGp / 16;
+  c->nsplit = 1;
+  for (int s : {8, 7, 6, 5, 4, 3, 2, 1})
+    if (k16 % s == 0) {
+      c->nsplit = s;
+      break;
+    }
+  const int k64 = c->nGp / 64;
+  c->nxs = 1;
+  for (int s : {8, 7, 6, 5, 4, 3, 2, 1})
+    if (k64 % s == 0) {
+      c->nxs = s;
+      break;
+    }
+  c->nPart = c->nGp / 64;
+  int r = c->allocate();
+  if (r) {
+    delete c;
+    return r;
+  }
+  *out = c;
+  return CODY_OK;
+}
+
+int cody_destroy(cody_ctx* c) {
+  if (!c) return CODY_OK;
+  hipSetDevice(c->device);
+  c->release();
+  delete c;
+  return CODY_OK;
+}
+
+int cody_sizes(const cody_ctx* c, int* nC, int* nState) {
+  if (!c) return fail(CODY_ERR_ARG, "cody_sizes: null context");
+  if (nC) *nC = c->nC;
+  if (nState) *nState = c->nS;
+  return CODY_OK;
+}
+
+int cody_get_stream(const cody_ctx* c, void** stream) {
+  if (!c || !stream) return fail(CODY_ERR_ARG, "cody_get_stream: null argument");
+  *stream = c->stream;
+  return CODY_OK;
+}
+
+int cody_set_grid(cody_ctx* c, const double* x, const double* w) {
+  if (!c || !x || !w) return fail(CODY_ERR_ARG, "cody_set_grid: null argument");
+  for (int i = 0; i < c->nG; ++i)
+    if (!(w[i] > 0.0)) return fail(CODY_ERR_ARG, "cody_set_grid: weights must be positive");
+  HIPCHK(hipSetDevice(c->device));
+  c->hx.assign(x, x + c->nG);
+  c->hw.assign(w, w + c->nG);
+  HIPCHK(hipMemcpy(c->d_x, x, sizeof(double) * c->nG, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_w, w, sizeof(double) * c->nG, hipMemcpyHostToDevice));
+  c->grid_set = true;
+  c->int_set = false;  // the interaction matrix depends on the grid
+  return CODY_OK;
+}
+
+int cody_set_kinetic_fedvr(cody_ctx* c, double xmin, double xmax, int nE, int nLoc, int xminExcl,
+                           int xmaxExcl, double mass) {
+  if (!c) return fail(CODY_ERR_ARG, "cody_set_kinetic_fedvr: null context");
+  if (nE < 1 || nLoc < 2 || nLoc > 32 || !(xmax > xmin) || !(mass > 0.0))
+    return fail(CODY_ERR_ARG, "cody_set_kinetic_fedvr: bad arguments");
+  if (fedvr_npoints(nE, nLoc, xminExcl != 0, xmaxExcl != 0) != c->nG)
+    return fail(CODY_ERR_ARG, "cody_set_kinetic_fedvr: element layout does not match nG");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<double> kref(nLoc * nLoc);
+  fedvr_kref(nLoc, kref.data());
+  HIPCHK(hipMemcpy(c->d_kref, kref.data(), sizeof(double) * nLoc * nLoc, hipMemcpyHostToDevice));
+  c->kin = KinLaunch{c->d_kref, nE, nLoc, xminExcl != 0 ? 1 : 0, nE / (xmax - xmin), -0.5 / mass};
+  c->kin_set = true;
+  return CODY_OK;
+}
+
+int cody_set_potential(cody_ctx* c, const double* vext) {
+  if (!c || !vext) return fail(CODY_ERR_ARG, "cody_set_potential: null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(c->d_vext, vext, sizeof(double) * c->nG, hipMemcpyHostToDevice));
+  c->pot_set = true;
+  return CODY_OK;
+}
+
+int cody_set_potential_softyukawa(cody_ctx* c, int n, const double* pos, const double* q,
+                                  const double* a1, const double* a2, const double* d) {
+  if (!c || n < 0 || (n > 0 && (!pos || !q || !a1 || !a2 || !d)))
+    return fail(CODY_ERR_ARG, "cody_set_potential_softyukawa: bad arguments");
+  if (!c->grid_set) return fail(CODY_ERR_STATE, "cody_set_potential_softyukawa: set the grid first");
+  std::vector<double> v(c->nG, 0.0);
+  for (int i = 0; i < c->nG; ++i)
+    for (int k = 0; k < n; ++k) {
+      const double r = std::fabs(c->hx[i] - pos[k]);
+      v[i] -= q[k] * std::exp(-r * d[k]) / (std::sqrt(r * r + a1[k] * a1[k]) + a2[k]);
+    }
+  return cody_set_potential(c, v.data());
+}
+
+int cody_set_interaction_softyukawa(cody_ctx* c, double s, double a1, double a2, double damp) {
+  if (!c) return fail(CODY_ERR_ARG, "cody_set_interaction_softyukawa: null context");
+  if (!c->grid_set) return fail(CODY_ERR_STATE, "cody_set_interaction_softyukawa: set the grid first");
+  if (a1 == 0.0 && a2 <= 0.0)
+    return fail(CODY_ERR_ARG, "cody_set_interaction_softyukawa: kernel is singular at r = 0");
+  HIPCHK(hipSetDevice(c->device));
+  launch_fill_softyukawa(c->d_x, c->nG, c->nGp, s, a1, a2, damp, c->d_V, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->int_set = true;
+  return CODY_OK;
+}
+
+int cody_set_laser(cody_ctx* c, int nK, const double* coef, double T) {
+  if (!c || nK < 0 || nK > 64 || (nK > 0 && (!coef || !(T > 0.0))))
+    return fail(CODY_ERR_ARG, "cody_set_laser: bad arguments");
+  c->nLaser = nK;
+  c->laserT = T;
+  for (int k = 0; k < nK; ++k) c->laserC[k] = coef[k];
+  return CODY_OK;
+}
+
+int cody_set_precision(cody_ctx* c, int mode) {
+  if (!c) return fail(CODY_ERR_ARG, "cody_set_precision: null context");
+  if (mode != CODY_PREC_FP64) return fail(CODY_ERR_UNSUPPORTED, "cody_set_precision: mode not built");
+  c->precision = mode;
+  return CODY_OK;
+}
+
+int cody_upload_state(cody_ctx* c, const double* state) {
+  if (!c || !state) return fail(CODY_ERR_ARG, "cody_upload_state: null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(c->d_state, state, sizeof(double2) * c->nS, hipMemcpyHostToDevice));
+  return CODY_OK;
+}
+
+int cody_download_state(cody_ctx* c, double* state) {
+  if (!c || !state) return fail(CODY_ERR_ARG, "cody_download_state: null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(state, c->d_state, sizeof(double2) * c->nS, hipMemcpyDeviceToHost));
+  return CODY_OK;
+}
+
+int cody_state_device_ptr(cody_ctx* c, void** d_state) {
+  if (!c || !d_state) return fail(CODY_ERR_ARG, "cody_state_device_ptr: null argument");
+  *d_state = c->d_state;
+  return CODY_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// RHS orchestration
+// ------------------------------------------------------------------------------------------
+namespace cody {
+
+double laser_field(const cody_ctx* c, double t) {
+  if (c->nLaser <= 0 || t < 0.0 || t > c->laserT) return 0.0;
+  double env = std::sin(M_PI * t / c->laserT);
+  env *= env;
+  double e = 0.0;
+  for (int k = 0; k < c->nLaser; ++k) e += c->laserC[k] * std::sin((k + 1) * M_PI * t / c->laserT);
+  return e * env;
+}
+
+int check_ready(const cody_ctx* c) {
+  if (!c->grid_set) return set_error(CODY_ERR_STATE, "grid not set");
+  if (!c->kin_set) return set_error(CODY_ERR_STATE, "kinetic operator not set");
+  if (!c->pot_set) return set_error(CODY_ERR_STATE, "external potential not set");
+  if (!c->int_set) return set_error(CODY_ERR_STATE, "interaction not set");
+  return CODY_OK;
+}
+
+// Enqueue one right-hand side on `st`.  mode: kModeFull / kModeNonLin / kModeLin.
+// When `ev` is non-null, an event is recorded after each launch (profiling).
+int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode,
+                hipStream_t st, double* energy, std::vector<hipEvent_t>* ev) {
+  const double2* C = state;
+  const double2* orbs = state + c->nC;
+  double2* dC = dstate;
+  double2* dorbs = dstate + c->nC;
+  auto mark = [&]() {
+    if (ev) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      hipEventRecord(e, st);
+      ev->push_back(e);
+    }
+  };
+  mark();
+  const double efield = laser_field(c, t);
+  if (mode == kModeLin) {
+    // TimeDerivativeOrbsLin: dphi = -i (T + V) phi, CI frozen (S_Method_Mb_OrbBased_Mctdhx.f90:152-177)
+    launch_orbital(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield, c->d_WT, c->d_rho2p, c->nG,
+                   c->nGp, c->nO, kModeLin, c->d_draw, c->d_part, st);
+    mark();
+    launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 0, dorbs, st);
+    mark();
+    launch_zero(dC, c->nC, st);
+    mark();
+    return CODY_OK;
+  }
+  launch_ci_density(C, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p, c->d_reginv, st);
+  mark();
+  launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, st);
+  mark();
+  launch_meanfield_f64(c->d_V, c->d_ST, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+  mark();
+  launch_wsum(c->d_Wpart, c->nsplit, c->nGp, c->Pc, c->d_WT, st);
+  mark();
+  launch_orbital(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield, c->d_WT, c->d_rho2p, c->nG, c->nGp,
+                 c->nO, mode, c->d_draw, c->d_part, st);
+  mark();
+  launch_h2_gram(c->d_ST, c->d_WT, c->nGp, c->Pc, c->nxs, c->d_Gpart, st);
+  mark();
+  launch_finalize(C, c->nC, c->nO, c->Pc, c->d_part, c->nPart, c->d_Gpart, c->nxs, c->dci, c->d_rho1,
+                  c->d_rho2, c->d_h1, c->d_h2, c->d_inp, c->d_G, dC, 1, energy, st);
+  mark();
+  launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 1, dorbs, st);
+  mark();
+  return CODY_OK;
+}
+
+}  // namespace cody
+
+extern "C" {
+
+static int rhs_dev(cody_ctx* c, const void* d_state, void* d_dstate, double t, void* stream,
+                   int mode) {
+  if (!c || !d_state || !d_dstate) return fail(CODY_ERR_ARG, "time derivative: null argument");
+  if (d_state == d_dstate) return fail(CODY_ERR_ARG, "time derivative: state and dState alias");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  CHK(enqueue_rhs(c, static_cast<const double2*>(d_state), static_cast<double2*>(d_dstate), t, mode,
+                  st, nullptr, nullptr));
+  HIPCHK(hipGetLastError());
+  return CODY_OK;
+}
+
+int cody_time_derivative_dev(cody_ctx* c, const void* s, void* d, double t, void* stream) {
+  return rhs_dev(c, s, d, t, stream, kModeFull);
+}
+int cody_time_derivative_orbs_lin_dev(cody_ctx* c, const void* s, void* d, double t, void* stream) {
+  return rhs_dev(c, s, d, t, stream, kModeLin);
+}
+int cody_time_derivative_nonlin_dev(cody_ctx* c, const void* s, void* d, double t, void* stream) {
+  return rhs_dev(c, s, d, t, stream, kModeNonLin);
+}
+
+int cody_time_derivative(cody_ctx* c, const double* state, double* dstate, double t) {
+  if (!c || !state || !dstate) return fail(CODY_ERR_ARG, "cody_time_derivative: null argument");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->d_tmp, state, sizeof(double2) * c->nS, hipMemcpyHostToDevice, c->stream));
+  CHK(enqueue_rhs(c, c->d_tmp, c->d_tmp2, t, kModeFull, c->stream, nullptr, nullptr));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dstate, c->d_tmp2, sizeof(double2) * c->nS, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CODY_OK;
+}
+
+int cody_energy(cody_ctx* c, const double* state, double t, double* energy) {
+  if (!c || !state || !energy) return fail(CODY_ERR_ARG, "cody_energy: null argument");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->d_tmp, state, sizeof(double2) * c->nS, hipMemcpyHostToDevice, c->stream));
+  CHK(enqueue_rhs(c, c->d_tmp, c->d_tmp2, t, kModeFull, c->stream, c->d_energy, nullptr));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(energy, c->d_energy, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CODY_OK;
+}
+
+int cody_meanfield_w(cody_ctx* c, const double* orbs, double* W) {
+  if (!c || !orbs || !W) return fail(CODY_ERR_ARG, "cody_meanfield_w: null argument");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  double2* d_orbs = c->d_tmp + c->nC;
+  HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
+  launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, c->stream);
+  launch_meanfield_f64(c->d_V, c->d_ST, c->d_Wpart, c->nGp, c->Pc, c->nsplit, c->stream);
+  launch_wsum(c->d_Wpart, c->nsplit, c->nGp, c->Pc, c->d_WT, c->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<double> wt((size_t)c->Pc * c->nGp);
+  HIPCHK(hipMemcpyAsync(wt.data(), c->d_WT, sizeof(double) * wt.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int p = 0; p < c->P; ++p)
+    for (int x = 0; x < c->nG; ++x) {
+      W[2 * ((size_t)p * c->nG + x)] = wt[(size_t)(2 * p) * c->nGp + x];
+      W[2 * ((size_t)p * c->nG + x) + 1] = wt[(size_t)(2 * p + 1) * c->nGp + x];
+    }
+  return CODY_OK;
+}
+
+int cody_apply_single_body(cody_ctx* c, const double* orbs, double t, double* dorbs, double* h1) {
+  if (!c || !orbs || !dorbs) return fail(CODY_ERR_ARG, "cody_apply_single_body: null argument");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  double2* d_orbs = c->d_tmp + c->nC;
+  HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
+  launch_orbital(d_orbs, c->d_w, c->d_x, c->d_vext, c->kin, laser_field(c, t), c->d_WT, c->d_rho2p,
+                 c->nG, c->nGp, c->nO, kModeLin, c->d_draw, c->d_part, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dorbs, c->d_draw, sizeof(double2) * c->nG * c->nO, hipMemcpyDeviceToHost, c->stream));
+  if (h1) {
+    std::vector<double2> part((size_t)c->nPart * 2 * c->nO * c->nO);
+    HIPCHK(hipMemcpyAsync(part.data(), c->d_part, sizeof(double2) * part.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int n2 = c->nO * c->nO;
+    for (int e = 0; e < n2; ++e) {
+      double re = 0, im = 0;
+      for (int b = 0; b < c->nPart; ++b) {
+        re += part[(size_t)b * 2 * n2 + e].x;
+        im += part[(size_t)b * 2 * n2 + e].y;
+      }
+      h1[2 * e] = re;
+      h1[2 * e + 1] = im;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CODY_OK;
+}
+
+int cody_densities(cody_ctx* c, const double* coeffs, double* rdm1, double* rdm2, double* reginv) {
+  if (!c || !coeffs) return fail(CODY_ERR_ARG, "cody_densities: null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->d_tmp, coeffs, sizeof(double2) * c->nC, hipMemcpyHostToDevice, c->stream));
+  launch_ci_density(c->d_tmp, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p, c->d_reginv,
+                    c->stream);
+  HIPCHK(hipGetLastError());
+  const size_t n2 = (size_t)c->nO * c->nO, n4 = n2 * n2;
+  if (rdm1) HIPCHK(hipMemcpyAsync(rdm1, c->d_rho1, sizeof(double2) * n2, hipMemcpyDeviceToHost, c->stream));
+  if (rdm2) HIPCHK(hipMemcpyAsync(rdm2, c->d_rho2, sizeof(double2) * n4, hipMemcpyDeviceToHost, c->stream));
+  if (reginv) HIPCHK(hipMemcpyAsync(reginv, c->d_reginv, sizeof(double2) * n2, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CODY_OK;
+}
+
+int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* names, int namesLen,
+                     int* nKernels) {
+  static const char* kNames =
+      "k_ci_density;k_pair_src;k_meanfield_f64;k_wsum;k_orbital;k_h2_gram;k_finalize;k_project";
+  if (!c || iters < 1 || !avg_ms || nMax < 1) return fail(CODY_ERR_ARG, "cody_profile_rhs: bad arguments");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<double> acc;
+  int nk = 0;
+  for (int it = 0; it < iters + 1; ++it) {
+    std::vector<hipEvent_t> ev;
+    CHK(enqueue_rhs(c, c->d_state, c->d_tmp2, 0.0, kModeFull, c->stream, nullptr, &ev));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    nk = static_cast<int>(ev.size()) - 1;
+    if (acc.empty()) acc.assign(nk, 0.0);
+    for (int k = 0; k < nk; ++k) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      if (it > 0) acc[k] += ms;  // first pass is a warm-up
+    }
+    for (auto e : ev) hipEventDestroy(e);
+  }
+  for (int k = 0; k < nk && k < nMax; ++k) avg_ms[k] = acc[k] / iters;
+  if (nKernels) *nKernels = nk;
+  if (names && namesLen > 0) {
+    std::strncpy(names, kNames, namesLen - 1);
+    names[namesLen - 1] = '\0';
+  }
+  return CODY_OK;
+}
+
+}  // extern "C"

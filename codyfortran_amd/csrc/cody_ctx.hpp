@@ -1,0 +1,145 @@
+// Device context of one MCTDHX system (one GPU, one fermionic body type).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "host_setup.hpp"
+#include "mctdhx_kernels.hpp"
+
+namespace cody {
+int ctx_alloc(void** p, size_t bytes);
+int set_error(int code, const char* msg);
+}  // namespace cody
+
+// Work buffers of the device-side propagators (allocated on first use).
+struct PropBuffers {
+  int kdim = 0;
+  double2 *basis = nullptr, *w = nullptr, *deriv = nullptr, *partial = nullptr, *dots = nullptr;
+  double2 *phi = nullptr, *nrm2 = nullptr, *trial = nullptr, *work = nullptr;
+  double *alpha = nullptr, *beta = nullptr, *flags = nullptr, *h_flags = nullptr;
+  double2 *rk_inc = nullptr, *rk_tmp = nullptr, *rk_d = nullptr;
+  void release() {
+    void* ptrs[] = {basis, w, deriv, partial, dots, phi, nrm2, trial, work,
+                    alpha, beta, flags, rk_inc, rk_tmp, rk_d};
+    for (void* p : ptrs)
+      if (p) hipFree(p);
+    if (h_flags) hipHostFree(h_flags);
+    *this = PropBuffers();
+  }
+};
+
+struct cody_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // sizes
+  int nG = 0, nGp = 0, N = 0, nO = 0, nC = 0, nS = 0, P = 0, Pc = 0;
+  int nsplit = 1, nxs = 1, nPart = 1;
+  double reg = 0.0;
+  int precision = 0;
+  // setup flags
+  bool grid_set = false, kin_set = false, pot_set = false, int_set = false;
+  std::vector<double> hx, hw;
+  cody::KinLaunch kin{};
+  int nLaser = 0;
+  double laserT = 0.0;
+  double laserC[64] = {};
+  // CI tables
+  cody::CiTables ci;
+  cody::DevCiTables dci{};
+  void* ci_blob = nullptr;
+  // device buffers
+  double *d_x = nullptr, *d_w = nullptr, *d_vext = nullptr, *d_kref = nullptr;
+  double *d_V = nullptr, *d_ST = nullptr, *d_Wpart = nullptr, *d_WT = nullptr;
+  double *d_Gpart = nullptr, *d_G = nullptr, *d_energy = nullptr;
+  double2 *d_state = nullptr, *d_tmp = nullptr, *d_tmp2 = nullptr;
+  double2 *d_rho1 = nullptr, *d_rho2 = nullptr, *d_rho2p = nullptr, *d_reginv = nullptr;
+  double2 *d_h1 = nullptr, *d_h2 = nullptr, *d_inp = nullptr;
+  double2 *d_draw = nullptr, *d_part = nullptr;
+  PropBuffers prop;
+
+  int allocate() {
+    using cody::ctx_alloc;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+      return cody::set_error(2, "hipStreamCreate failed");
+    const size_t n2 = (size_t)nO * nO, n4 = n2 * n2;
+    int r = 0;
+#define ALLOC(ptr, bytes)                                   \
+  if ((r = ctx_alloc((void**)&(ptr), (bytes))) != 0) return r;
+    ALLOC(d_x, sizeof(double) * nG);
+    ALLOC(d_w, sizeof(double) * nG);
+    ALLOC(d_vext, sizeof(double) * nG);
+    ALLOC(d_kref, sizeof(double) * 32 * 32);
+    ALLOC(d_V, sizeof(double) * nGp * nGp);
+    ALLOC(d_ST, sizeof(double) * Pc * nGp);
+    ALLOC(d_Wpart, sizeof(double) * nsplit * nGp * Pc);
+    ALLOC(d_WT, sizeof(double) * Pc * nGp);
+    ALLOC(d_Gpart, sizeof(double) * nxs * Pc * Pc);
+    ALLOC(d_G, sizeof(double) * Pc * Pc);
+    ALLOC(d_energy, sizeof(double) * 8);
+    ALLOC(d_state, sizeof(double2) * nS);
+    ALLOC(d_tmp, sizeof(double2) * nS);
+    ALLOC(d_tmp2, sizeof(double2) * nS);
+    ALLOC(d_rho1, sizeof(double2) * n2);
+    ALLOC(d_rho2, sizeof(double2) * n4);
+    ALLOC(d_rho2p, sizeof(double2) * n4);
+    ALLOC(d_reginv, sizeof(double2) * n2);
+    ALLOC(d_h1, sizeof(double2) * n2);
+    ALLOC(d_h2, sizeof(double2) * n4);
+    ALLOC(d_inp, sizeof(double2) * n2);
+    ALLOC(d_draw, sizeof(double2) * nG * nO);
+    ALLOC(d_part, sizeof(double2) * nPart * 2 * n2);
+    // CI tables in one blob: ptr arrays then entry arrays
+    const size_t ns = ci.s_ptr.size(), nd = ci.d_ptr.size(), n1 = ci.r1_ptr.size(), nr2 = ci.r2_ptr.size();
+    const size_t es = ci.singles.size(), ed = ci.doubles.size(), e1 = ci.r1.size(), e2 = ci.r2.size();
+    const size_t ib = sizeof(int) * (ns + nd + n1 + nr2);
+    const size_t ibp = 256 * ((ib + 255) / 256);
+    const size_t eb = sizeof(cody::CiEntryDev) * (es + ed + e1 + e2);
+    ALLOC(ci_blob, ibp + eb);
+#undef ALLOC
+    std::vector<char> host(ibp + eb, 0);
+    int* ip = reinterpret_cast<int*>(host.data());
+    auto cpyi = [&](const std::vector<int32_t>& v) {
+      std::copy(v.begin(), v.end(), ip);
+      ip += v.size();
+    };
+    cpyi(ci.s_ptr);
+    cpyi(ci.d_ptr);
+    cpyi(ci.r1_ptr);
+    cpyi(ci.r2_ptr);
+    auto* ep = reinterpret_cast<cody::CiEntryDev*>(host.data() + ibp);
+    auto cpye = [&](const std::vector<cody::CiEntry>& v) {
+      for (const auto& e : v) *ep++ = cody::CiEntryDev{e.ic, e.jc, e.code, e.f};
+    };
+    cpye(ci.singles);
+    cpye(ci.doubles);
+    cpye(ci.r1);
+    cpye(ci.r2);
+    if (hipMemcpy(ci_blob, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cody::set_error(2, "hipMemcpy of CI tables failed");
+    const int* ib0 = reinterpret_cast<const int*>(ci_blob);
+    const auto* eb0 = reinterpret_cast<const cody::CiEntryDev*>(static_cast<char*>(ci_blob) + ibp);
+    dci.s_ptr = ib0;
+    dci.d_ptr = ib0 + ns;
+    dci.r1_ptr = ib0 + ns + nd;
+    dci.r2_ptr = ib0 + ns + nd + n1;
+    dci.singles = eb0;
+    dci.doubles = eb0 + es;
+    dci.r1 = eb0 + es + ed;
+    dci.r2 = eb0 + es + ed + e1;
+    return 0;
+  }
+
+  void release() {
+    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_Wpart, d_WT,
+                    d_Gpart, d_G,  d_energy, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
+                    d_rho2p, d_reginv, d_h1, d_h2,   d_inp,   d_draw,  d_part,  ci_blob};
+    for (void* p : ptrs)
+      if (p) hipFree(p);
+    prop.release();
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};

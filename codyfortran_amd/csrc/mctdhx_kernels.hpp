@@ -1,0 +1,59 @@
+// Kernel launchers of the MCTDHX right-hand side (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cody {
+
+constexpr int kMaxOrbs = 12;   // orbitals per body type handled by the kernels
+constexpr int kMaxPc = 160;    // 16*ceil(M(M+1)/16) for M <= 12
+enum { kModeFull = 0, kModeNonLin = 1, kModeLin = 2 };
+
+struct CiEntryDev {
+  int32_t ic, jc, code;
+  float f;
+};
+
+struct DevCiTables {
+  const int* s_ptr;
+  const CiEntryDev* singles;
+  const int* d_ptr;
+  const CiEntryDev* doubles;
+  const int* r1_ptr;
+  const CiEntryDev* r1;
+  const int* r2_ptr;
+  const CiEntryDev* r2;
+};
+
+struct KinLaunch {
+  const double* kref;
+  int nE, nLoc, xminExcl;
+  double invh, fac;
+};
+
+void launch_zero(double2* p, int n, hipStream_t st);
+void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a1, double a2,
+                            double damp, double* V, hipStream_t st);
+void launch_ci_density(const double2* C, int nO, double reg, const DevCiTables& t, double2* rho1,
+                       double2* rho2, double2* rho2p, double2* reginv, hipStream_t st);
+void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
+                     double* ST, hipStream_t st);
+void launch_meanfield_f64(const double* V, const double* ST, double* Wpart, int nGp, int Pc,
+                          int nsplit, hipStream_t st);
+void launch_wsum(const double* Wpart, int nsplit, int nGp, int Pc, double* WT, hipStream_t st);
+void launch_orbital(const double2* orbs, const double* w, const double* x, const double* vext,
+                    const KinLaunch& kl, double efield, const double* WT, const double2* rho2p,
+                    int nG, int nGp, int nO, int mode, double2* draw, double2* part,
+                    hipStream_t st);
+void launch_h2_gram(const double* ST, const double* WT, int nGp, int Pc, int nxs, double* Gpart,
+                    hipStream_t st);
+void launch_finalize(const double2* C, int nC, int nO, int Pc, const double2* part, int nPart,
+                     const double* Gpart, int nG2, const DevCiTables& t, const double2* rho1,
+                     const double2* rho2, double2* h1, double2* h2, double2* inp, double* G,
+                     double2* dC, int ciMode, double* energy, hipStream_t st);
+void launch_project(const double2* orbs, const double2* draw, const double2* inp, int nG, int nO,
+                    int project, double2* dorbs, hipStream_t st);
+
+}  // namespace cody

@@ -1,0 +1,259 @@
+"""ctypes binding of ``libcody_hip.so`` (the C ABI declared in ``include/cody_mctdhx.h``).
+
+The Python side is plumbing for tests and measurement; the product is the HIP library,
+which a Fortran host binds through iso_c_binding (see ``codyfortran_amd/fortran``).
+Loading fails loudly when the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcody_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cody_mctdhx.h")
+
+PREC_FP64, PREC_SPLIT_FP32, PREC_BF16X3 = 0, 1, 2
+
+_lib = None
+
+
+class CodyError(RuntimeError):
+    pass
+
+
+def header_symbols() -> list[str]:
+    """Every function declared in include/cody_mctdhx.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*|void)\s+(cody_\w+)\s*\(", txt, re.M)))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CodyError(f"{LIB_PATH} is missing: build it with `python -m codyfortran_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    dp, ip, vp, i = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p, C.c_int
+    d = C.c_double
+    sig = {
+        "cody_last_error": (C.c_char_p, []),
+        "cody_fedvr_npoints": (i, [i, i, i, i]),
+        "cody_fedvr_grid": (i, [d, d, i, i, i, i, dp, dp]),
+        "cody_fedvr_kref": (i, [i, dp]),
+        "cody_create": (i, [C.POINTER(vp), i, i, i, i, d]),
+        "cody_destroy": (i, [vp]),
+        "cody_sizes": (i, [vp, ip, ip]),
+        "cody_get_stream": (i, [vp, C.POINTER(vp)]),
+        "cody_set_grid": (i, [vp, dp, dp]),
+        "cody_set_kinetic_fedvr": (i, [vp, d, d, i, i, i, i, d]),
+        "cody_set_potential": (i, [vp, dp]),
+        "cody_set_potential_softyukawa": (i, [vp, i, dp, dp, dp, dp, dp]),
+        "cody_set_interaction_softyukawa": (i, [vp, d, d, d, d]),
+        "cody_set_laser": (i, [vp, i, dp, d]),
+        "cody_set_precision": (i, [vp, i]),
+        "cody_upload_state": (i, [vp, vp]),
+        "cody_download_state": (i, [vp, vp]),
+        "cody_state_device_ptr": (i, [vp, C.POINTER(vp)]),
+        "cody_time_derivative_dev": (i, [vp, vp, vp, d, vp]),
+        "cody_time_derivative": (i, [vp, vp, vp, d]),
+        "cody_time_derivative_orbs_lin_dev": (i, [vp, vp, vp, d, vp]),
+        "cody_time_derivative_nonlin_dev": (i, [vp, vp, vp, d, vp]),
+        "cody_energy": (i, [vp, vp, d, dp]),
+        "cody_meanfield_w": (i, [vp, vp, vp]),
+        "cody_apply_single_body": (i, [vp, vp, d, vp, vp]),
+        "cody_densities": (i, [vp, vp, vp, vp, vp]),
+        "cody_sil": (i, [vp, d, d, i, d, i, i, ip]),
+        "cody_rk4": (i, [vp, d, d, i]),
+        "cody_strang": (i, [vp, d, d, i, d, i, i, ip]),
+        "cody_profile_rhs": (i, [vp, i, i, dp, C.c_char_p, i, ip]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = load().cody_last_error().decode(errors="replace")
+        raise CodyError(f"cody error {rc}: {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _vp(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _c128(a):
+    return np.require(np.asarray(a, dtype=np.complex128), requirements=["F", "A"])
+
+
+# ----------------------------------------------------------------------------- grid helpers
+def fedvr_grid(xmin, xmax, nE, nLoc, xmin_excl=False, xmax_excl=False):
+    lib = load()
+    n = lib.cody_fedvr_npoints(nE, nLoc, int(xmin_excl), int(xmax_excl))
+    if n < 1:
+        raise CodyError("bad FEDVR layout")
+    x = np.zeros(n)
+    w = np.zeros(n)
+    _check(lib.cody_fedvr_grid(xmin, xmax, nE, nLoc, int(xmin_excl), int(xmax_excl), _dp(x), _dp(w)))
+    return x, w
+
+
+def fedvr_kref(nLoc):
+    K = np.zeros((nLoc, nLoc), order="F")
+    _check(load().cody_fedvr_kref(nLoc, _dp(K)))
+    return K
+
+
+class Context:
+    """One MCTDHX system on one GPU (one spin-polarised fermionic body type)."""
+
+    def __init__(self, nG, n_bodies, n_orbs, reg=1e-10, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        _check(self.lib.cody_create(C.byref(h), device, nG, n_bodies, n_orbs, float(reg)))
+        self.h = h
+        nC, nS = C.c_int(), C.c_int()
+        _check(self.lib.cody_sizes(h, C.byref(nC), C.byref(nS)))
+        self.nG, self.nO, self.N = nG, n_orbs, n_bodies
+        self.nC, self.n_state = nC.value, nS.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cody_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _check(self.lib.cody_get_stream(self.h, C.byref(s)))
+        return s.value or 0
+
+    # ---------------------------------------------------------------- setup
+    def set_grid(self, x, w):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.w = np.ascontiguousarray(w, dtype=np.float64)
+        _check(self.lib.cody_set_grid(self.h, _dp(self.x), _dp(self.w)))
+
+    def set_kinetic_fedvr(self, xmin, xmax, nE, nLoc, xmin_excl=False, xmax_excl=False, mass=1.0):
+        _check(self.lib.cody_set_kinetic_fedvr(self.h, xmin, xmax, nE, nLoc, int(xmin_excl),
+                                               int(xmax_excl), mass))
+
+    def set_potential(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(self.lib.cody_set_potential(self.h, _dp(v)))
+
+    def set_potential_softyukawa(self, position, charge, soft1, soft2, damp):
+        arrs = [np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)
+                for a in (position, charge, soft1, soft2, damp)]
+        _check(self.lib.cody_set_potential_softyukawa(self.h, len(arrs[0]), *[_dp(a) for a in arrs]))
+
+    def set_interaction_softyukawa(self, strength=1.0, soft1=1.0, soft2=0.0, damp=0.0):
+        _check(self.lib.cody_set_interaction_softyukawa(self.h, strength, soft1, soft2, damp))
+
+    def set_laser(self, c, T):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        _check(self.lib.cody_set_laser(self.h, len(c), _dp(c), float(T)))
+
+    def set_precision(self, mode):
+        _check(self.lib.cody_set_precision(self.h, int(mode)))
+
+    # ---------------------------------------------------------------- state
+    def upload_state(self, state):
+        s = _c128(state)
+        assert s.size == self.n_state
+        _check(self.lib.cody_upload_state(self.h, _vp(s)))
+
+    def download_state(self):
+        s = np.zeros(self.n_state, dtype=np.complex128)
+        _check(self.lib.cody_download_state(self.h, _vp(s)))
+        return s
+
+    def state_device_ptr(self) -> int:
+        p = C.c_void_p()
+        _check(self.lib.cody_state_device_ptr(self.h, C.byref(p)))
+        return p.value
+
+    # ---------------------------------------------------------------- method
+    def time_derivative(self, state, t=0.0):
+        s = _c128(state)
+        d = np.zeros(self.n_state, dtype=np.complex128)
+        _check(self.lib.cody_time_derivative(self.h, _vp(s), _vp(d), float(t)))
+        return d
+
+    def time_derivative_dev(self, d_state: int, d_dstate: int, t=0.0, stream: int = 0, mode="full"):
+        fn = {"full": self.lib.cody_time_derivative_dev,
+              "lin": self.lib.cody_time_derivative_orbs_lin_dev,
+              "nonlin": self.lib.cody_time_derivative_nonlin_dev}[mode]
+        _check(fn(self.h, C.c_void_p(d_state), C.c_void_p(d_dstate), float(t), C.c_void_p(stream or None)))
+
+    def energy(self, state, t=0.0):
+        s = _c128(state)
+        e = C.c_double()
+        _check(self.lib.cody_energy(self.h, _vp(s), float(t), C.byref(e)))
+        return e.value
+
+    # ---------------------------------------------------------------- pieces
+    def meanfield_w(self, orbs):
+        o = _c128(orbs)
+        P = self.nO * (self.nO + 1) // 2
+        W = np.zeros((self.nG, P), dtype=np.complex128, order="F")
+        _check(self.lib.cody_meanfield_w(self.h, _vp(o), _vp(W)))
+        return W
+
+    def apply_single_body(self, orbs, t=0.0):
+        o = _c128(orbs)
+        d = np.zeros((self.nG, self.nO), dtype=np.complex128, order="F")
+        h1 = np.zeros((self.nO, self.nO), dtype=np.complex128, order="F")
+        _check(self.lib.cody_apply_single_body(self.h, _vp(o), float(t), _vp(d), _vp(h1)))
+        return d, h1
+
+    def densities(self, coeffs):
+        c = _c128(coeffs)
+        n = self.nO
+        r1 = np.zeros((n, n), dtype=np.complex128, order="F")
+        r2 = np.zeros((n, n, n, n), dtype=np.complex128, order="F")
+        ri = np.zeros((n, n), dtype=np.complex128, order="F")
+        _check(self.lib.cody_densities(self.h, _vp(c), _vp(r1), _vp(r2), _vp(ri)))
+        return r1, r2, ri
+
+    # ---------------------------------------------------------------- propagation
+    def sil(self, t0, t1, krylov_dim=32, tol=1e-10, max_restarts=8, metric=True):
+        ok = C.c_int()
+        _check(self.lib.cody_sil(self.h, t0, t1, krylov_dim, tol, max_restarts, int(metric), C.byref(ok)))
+        return bool(ok.value)
+
+    def rk4(self, t0, t1, full=False):
+        _check(self.lib.cody_rk4(self.h, t0, t1, int(full)))
+
+    def strang(self, t0, t1, krylov_dim=32, tol=1e-10, max_restarts=8, metric=True):
+        ok = C.c_int()
+        _check(self.lib.cody_strang(self.h, t0, t1, krylov_dim, tol, max_restarts, int(metric), C.byref(ok)))
+        return bool(ok.value)
+
+    # ---------------------------------------------------------------- measurement
+    def profile_rhs(self, iters=20):
+        n = 32
+        ms = (C.c_double * n)()
+        names = C.create_string_buffer(1024)
+        nk = C.c_int()
+        _check(self.lib.cody_profile_rhs(self.h, iters, n, ms, names, 1024, C.byref(nk)))
+        labels = names.value.decode().split(";")
+        return {labels[k] if k < len(labels) else f"k{k}": ms[k] for k in range(nk.value)}

@@ -1,0 +1,153 @@
+/*
+ * cody_mctdhx.h — C ABI of the MI355X (gfx950) MCTDHX right-hand side and propagation.
+ *
+ * Drop-in boundary for CodyFortran's Fabricate/Setup + procedure-pointer API.  A Fortran
+ * program binds these entry points through iso_c_binding (the pattern of
+ * src/Utils/M_Utils_CombinationGslLib.f90:20-48) and assigns them to the procedure
+ * pointers it already uses (Method_TimeDerivative, Propagator_Propagate, ...).
+ *
+ * Conventions (identical to the reference):
+ *   - complex(R64) arrays are interleaved (re, im) doubles, column-major;
+ *   - the packed state is [C(nC) ; orbs(nG, nO)] (S_Method_Mb_OrbBased_Mctdhx.f90:229-249);
+ *   - CI configurations are the lexicographic N-of-M combinations of one fermionic body type;
+ *   - every entry point returns 0 on success and a CODY_ERR_* code otherwise; the message of
+ *     the last failure is available from cody_last_error() (the Fortran wrapper turns a
+ *     non-zero status into `error stop`, the reference's only error mechanism).
+ * Pointers named d_* are device (HBM) pointers; all other pointers are host pointers.
+ * A `stream` argument is a hipStream_t (NULL selects the context's own stream).
+ * Contexts are not thread-safe; one context drives one GPU.
+ */
+#ifndef CODY_MCTDHX_H
+#define CODY_MCTDHX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CODY_OK 0
+#define CODY_ERR_ARG 1         /* invalid argument or size */
+#define CODY_ERR_HIP 2         /* HIP runtime failure */
+#define CODY_ERR_STATE 3       /* call out of order (e.g. RHS before grid/potentials are set) */
+#define CODY_ERR_UNSUPPORTED 4 /* configuration outside what the kernels implement */
+
+/* precision of the mean-field contraction W = V . S (the dominant kernel) */
+#define CODY_PREC_FP64 0       /* FP64 MFMA, FP64 accumulation */
+#define CODY_PREC_SPLIT_FP32 1 /* V and S split into FP32 hi+lo, 3 FP32-MFMA products, FP64 chunk sums */
+#define CODY_PREC_BF16X3 2     /* V and S split into 3 BF16 terms, 6 BF16-MFMA products, FP32 accumulate */
+
+typedef struct cody_ctx cody_ctx;
+
+const char* cody_last_error(void);
+
+/* ---------------------------------------------------------------- grid helpers (host) */
+/* Number of FEDVR grid points: src/Utils/M_Utils_Fedvr.f90:108-111. */
+int cody_fedvr_npoints(int nE, int nLoc, int xminExcl, int xmaxExcl);
+/* FEDVR nodes and summed weights: replaces Fedvr_CreateCtx, src/Utils/M_Utils_Fedvr.f90:76-149. */
+int cody_fedvr_grid(double xmin, double xmax, int nE, int nLoc, int xminExcl, int xmaxExcl,
+                    double* x, double* w);
+/* Reference-element stiffness K(i,j) = -sum_k w_k D(k,i) D(k,j) on [-1/2,1/2]:
+ * replaces DerivativeFedvr_CreateCtx, src/Utils/M_Utils_DerivativeFedvr.f90:29-76. */
+int cody_fedvr_kref(int nLoc, double* kref);
+
+/* ---------------------------------------------------------------- context (Setup) */
+/* One spin-polarised fermionic body type: nBodies particles in nOrbs orbitals on nG points.
+ * reg is method.mb.orbBased.regularizationParameter (S_Method_Mb_OrbBased.f90:79).
+ * Replaces the allocations of Method_Setup (S_Method_Mb_OrbBased_Mctdhx.f90:40-68),
+ * ConfigList_Setup (S_ConfigList_AllActive.f90:105-117) and Coeffs_Fabricate
+ * (S_Coeffs_Generic.f90:55-82). */
+int cody_create(cody_ctx** ctx, int device, int nG, int nBodies, int nOrbs, double reg);
+int cody_destroy(cody_ctx* ctx);
+int cody_sizes(const cody_ctx* ctx, int* nC, int* nState);
+int cody_get_stream(const cody_ctx* ctx, void** stream);
+
+/* Grid coordinates and quadrature weights: Grid_Linear_xCoord / Grid_Linear_weights
+ * (src/Grid/Linear/S_Grid_Linear.f90:53-61 uses them in Grid_InnerProduct). */
+int cody_set_grid(cody_ctx* ctx, const double* x, const double* w);
+/* Linear FEDVR kinetic energy -(1/2m) d2/dx2 in weak form on nE equal elements of nLoc GLL
+ * points (the FEDVR operator of src/Utils/M_Utils_DerivativeFedvr.f90:99-152 behind the
+ * SysKinetic_MultiplyWithKineticOp pointer, src/SysKinetic/M_SysKinetic.f90:141-160). */
+int cody_set_kinetic_fedvr(cody_ctx* ctx, double xmin, double xmax, int nE, int nLoc,
+                           int xminExcl, int xmaxExcl, double mass);
+/* External potential values V_ext(x_i) (SysPotential_FillExternalPotential,
+ * src/SysPotential/M_SysPotential.f90:71-101). */
+int cody_set_potential(cody_ctx* ctx, const double* vext);
+/* Soft-Yukawa centres -sum_c q_c e^{-d_c r}/(sqrt(r^2+a1_c^2)+a2_c):
+ * S_SysPotential_Linear_SoftYukawa_StdImpl.f90:45-84. */
+int cody_set_potential_softyukawa(cody_ctx* ctx, int nCenters, const double* position,
+                                  const double* charge, const double* soft1, const double* soft2,
+                                  const double* damp);
+/* Two-body kernel K(r) = s e^{-d r}/(sqrt(r^2+a1^2)+a2) at true distances r = |x_i - x_j|,
+ * tabulated on the device as an nG x nG matrix: replaces SysInteraction_Setup +
+ * FillInteractionPotential (src/SysInteraction/Linear/SoftYukawa/S_SysInteraction_Linear_SoftYukawa.f90:67-73,
+ * src/Utils/M_Utils_ConvolutionIntegral.f90:71-107). */
+int cody_set_interaction_softyukawa(cody_ctx* ctx, double strength, double soft1, double soft2,
+                                    double damp);
+/* Length-gauge dipole field V(x,t) = E(t) x, E(t) = sum_k c_k sin(k pi t/T) sin^2(pi t/T)
+ * (a 1D leaf in the pattern of src/SysPotential/Ylm/CoulombLaser; nK = 0 switches it off). */
+int cody_set_laser(cody_ctx* ctx, int nK, const double* c, double T);
+/* Precision of the mean-field contraction (CODY_PREC_*). */
+int cody_set_precision(cody_ctx* ctx, int mode);
+
+/* ---------------------------------------------------------------- state transfer */
+/* The context's device-resident state (Method_state, src/Method/M_Method.f90:47). */
+int cody_upload_state(cody_ctx* ctx, const double* state);
+int cody_download_state(cody_ctx* ctx, double* state);
+int cody_state_device_ptr(cody_ctx* ctx, void** d_state);
+
+/* ---------------------------------------------------------------- Method */
+/* Full MCTDHX right-hand side  dState = -i [ (1-P)(h phi + corr) ; (H - <H>) C ]:
+ * replaces Method_TimeDerivative (I_Method_TimeDerivative, src/Method/M_Method.f90:82-90),
+ * MCTDHX implementation S_Method_Mb_OrbBased_Mctdhx.f90:101-149.  Overwrites d_dstate. */
+int cody_time_derivative_dev(cody_ctx* ctx, const void* d_state, void* d_dstate, double t,
+                             void* stream);
+/* Same, host arrays in and out (copies through the context's buffers; synchronous). */
+int cody_time_derivative(cody_ctx* ctx, const double* state, double* dstate, double t);
+/* Split-step vector fields: Method_Mb_OrbBased_TimeDerivativeOrbsLin (:152-177) and
+ * Method_Mb_OrbBased_TimeDerivativeCoeffsPlusOrbsNonLin (:180-226). */
+int cody_time_derivative_orbs_lin_dev(cody_ctx* ctx, const void* d_state, void* d_dstate, double t,
+                                      void* stream);
+int cody_time_derivative_nonlin_dev(cody_ctx* ctx, const void* d_state, void* d_dstate, double t,
+                                    void* stream);
+/* Energy E = Tr[rho1 h1] + Tr[rho2 h2] of a host state: replaces Method_GetEnergy
+ * (src/Method/Mb/OrbBased/S_Method_Mb_OrbBased.f90:132-161). */
+int cody_energy(cody_ctx* ctx, const double* state, double t, double* energy);
+
+/* ---------------------------------------------------------------- operator pieces (host in/out) */
+/* Mean-field potentials of the unordered pairs l <= k (pair p = l*nO - l(l-1)/2 + k-l):
+ * W(:,p) = FillInteractionPotential(FillInteractionSrc(phi_l, phi_k)) (S_SysInteraction_Linear.f90:57-72).
+ * orbs is nG x nO, W is nG x P (complex, column-major). */
+int cody_meanfield_w(cody_ctx* ctx, const double* orbs, double* W);
+/* dOrbs = (T + V) orbs and h1 = <phi_i|(T+V) phi_j> (ApplySingleBodyOp, S_Method_Mb_OrbBased.f90:976-1034). */
+int cody_apply_single_body(cody_ctx* ctx, const double* orbs, double t, double* dorbs, double* h1);
+/* rho1, rho2 of a CI vector (Coeffs_ApplyH1FillRdm1 / Coeffs_ApplyH2FillRdm2 in density mode,
+ * S_Coeffs_Generic.f90:103-319 with the unpack of :325-434) and the regularised inverse
+ * rho1 (rho1^2 + eps)^-1 (S_Method_Mb_OrbBased.f90:733-744). */
+int cody_densities(cody_ctx* ctx, const double* coeffs, double* rdm1, double* rdm2, double* reginv);
+
+/* ---------------------------------------------------------------- propagation (device state) */
+/* Short iterative Lanczos on the linear split part with full re-orthogonalisation and step
+ * halving (S_IntegratorList_Sil.f90:114-312), metric = unit CI weights + grid weights per
+ * orbital block when useMetric != 0.  Propagates the context's state from t0 to t1.
+ * *converged receives 1 if every sub-step met the tolerance. */
+int cody_sil(cody_ctx* ctx, double t0, double t1, int krylovDim, double tol, int maxRestarts,
+             int useMetric, int* converged);
+/* Classical RK4 on the nonlinear split part (S_IntegratorList_Rk_O4Expl.f90:90-141) or, with
+ * full != 0, on the full right-hand side. */
+int cody_rk4(cody_ctx* ctx, double t0, double t1, int full);
+/* Strang step A(dt/2) B(dt) A(dt/2), A = SIL on the linear part, B = RK4 on the nonlinear part:
+ * replaces Propagator_Propagate (src/Propagator/M_Propagator.f90:75-99) for splitStep.order2
+ * (S_Propagator_SplitStep_Order2.f90:52-77). */
+int cody_strang(cody_ctx* ctx, double t0, double t1, int krylovDim, double tol, int maxRestarts,
+                int useMetric, int* converged);
+
+/* ---------------------------------------------------------------- measurement */
+/* Runs `iters` right-hand sides of the context's state and returns the average duration of each
+ * launch of the sequence, timed with HIP events on the context's stream (names in `names`,
+ * ';'-separated, at most nMax entries).  Returns the number of launches per RHS in *nKernels. */
+int cody_profile_rhs(cody_ctx* ctx, int iters, int nMax, double* avg_ms, char* names, int namesLen,
+                     int* nKernels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
