@@ -39,6 +39,39 @@ def _compile(src: str) -> str:
     return out
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_SOURCES = ["M_Utils_Types.f90", "M_CodyHip.f90", "M_Method.f90", "M_Propagator.f90",
+                   "M_Grid.f90", "M_Method_Mb_OrbBased_MctdhxHip.f90",
+                   "S_Method_Mb_OrbBased_MctdhxHip.f90"]
+FORTRAN_DRIVER = os.path.join(ROOT, "tests", "fortran", "T_MctdhxHip_Rhs.f90")
+BIN = os.path.join(HERE, "bin")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Fortran host API (mirror modules + GPU leaf) and the driver test program."""
+    mod = os.path.join(OBJ, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    objs = []
+    for src in FORTRAN_SOURCES + [FORTRAN_DRIVER]:
+        path = src if os.path.isabs(src) else os.path.join(FORTRAN_DIR, src)
+        out = os.path.join(mod, os.path.splitext(os.path.basename(path))[0] + ".o")
+        cmd = [FC, "-O2", "-fPIC", "-module-dir", mod, "-I", mod, "-c", path, "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {path}:\n{r.stdout}\n{r.stderr}")
+        objs.append(out)
+    exe = os.path.join(BIN, "t_mctdhx_hip")
+    cmd = [FC, *objs, "-o", exe, "-L", HERE, "-lcody_hip", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran link failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 def build_library(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
@@ -55,4 +88,5 @@ def build_library(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_fortran(verbose=True)
     sys.exit(0)

@@ -1,0 +1,158 @@
+! iso_c_binding interfaces of the MI355X MCTDHX library (include/cody_mctdhx.h), in the
+! bind(C) pattern the reference uses for its native libraries
+! (src/Utils/M_Utils_CombinationGslLib.f90:20-48).  Every function returns 0 on success;
+! CodyHip_Check turns a non-zero status into `error stop`, the reference's error mechanism.
+module M_CodyHip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: CODY_OK = 0
+  integer(c_int), parameter :: CODY_PREC_FP64 = 0
+
+  interface
+    function cody_last_error() bind(C, name="cody_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+
+    function cody_fedvr_npoints(nE, nLoc, xminExcl, xmaxExcl) bind(C, name="cody_fedvr_npoints") result(r)
+      import :: c_int
+      integer(c_int), value :: nE, nLoc, xminExcl, xmaxExcl
+      integer(c_int) :: r
+    end function
+
+    function cody_fedvr_grid(xmin, xmax, nE, nLoc, xminExcl, xmaxExcl, x, w) &
+        bind(C, name="cody_fedvr_grid") result(r)
+      import :: c_int, c_double
+      real(c_double), value :: xmin, xmax
+      integer(c_int), value :: nE, nLoc, xminExcl, xmaxExcl
+      real(c_double) :: x(*), w(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_create(ctx, device, nG, nBodies, nOrbs, reg) bind(C, name="cody_create") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr) :: ctx
+      integer(c_int), value :: device, nG, nBodies, nOrbs
+      real(c_double), value :: reg
+      integer(c_int) :: r
+    end function
+
+    function cody_destroy(ctx) bind(C, name="cody_destroy") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: r
+    end function
+
+    function cody_sizes(ctx, nC, nState) bind(C, name="cody_sizes") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: nC, nState
+      integer(c_int) :: r
+    end function
+
+    function cody_set_grid(ctx, x, w) bind(C, name="cody_set_grid") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: x(*), w(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_set_kinetic_fedvr(ctx, xmin, xmax, nE, nLoc, xminExcl, xmaxExcl, mass) &
+        bind(C, name="cody_set_kinetic_fedvr") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: xmin, xmax, mass
+      integer(c_int), value :: nE, nLoc, xminExcl, xmaxExcl
+      integer(c_int) :: r
+    end function
+
+    function cody_set_potential_softyukawa(ctx, n, pos, q, a1, a2, d) &
+        bind(C, name="cody_set_potential_softyukawa") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      real(c_double) :: pos(*), q(*), a1(*), a2(*), d(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_set_interaction_softyukawa(ctx, s, a1, a2, d) &
+        bind(C, name="cody_set_interaction_softyukawa") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: s, a1, a2, d
+      integer(c_int) :: r
+    end function
+
+    function cody_set_laser(ctx, nK, c, T) bind(C, name="cody_set_laser") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nK
+      real(c_double) :: c(*)
+      real(c_double), value :: T
+      integer(c_int) :: r
+    end function
+
+    function cody_upload_state(ctx, state) bind(C, name="cody_upload_state") result(r)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: state(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_download_state(ctx, state) bind(C, name="cody_download_state") result(r)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: state(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_time_derivative(ctx, state, dstate, t) bind(C, name="cody_time_derivative") result(r)
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: state(*), dstate(*)
+      real(c_double), value :: t
+      integer(c_int) :: r
+    end function
+
+    function cody_energy(ctx, state, t, e) bind(C, name="cody_energy") result(r)
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: state(*)
+      real(c_double), value :: t
+      real(c_double) :: e
+      integer(c_int) :: r
+    end function
+
+    function cody_strang(ctx, t0, t1, krylovDim, tol, maxRestarts, useMetric, converged) &
+        bind(C, name="cody_strang") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t0, t1, tol
+      integer(c_int), value :: krylovDim, maxRestarts, useMetric
+      integer(c_int) :: converged
+      integer(c_int) :: r
+    end function
+  end interface
+
+contains
+
+  !> Turn a non-zero status into `error stop` with the library's message.
+  subroutine CodyHip_Check(status, where)
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: i
+    if (status == CODY_OK) return
+    text = ""
+    call c_f_pointer(cody_last_error(), msg, [512])
+    do i = 1, 512
+      if (msg(i) == c_null_char) exit
+      text(i:i) = msg(i)
+    end do
+    write (*, '(A,I0,A)') "cody error ", status, " in "//where//": "//trim(text)
+    error stop "MI355X MCTDHX library call failed"
+  end subroutine
+
+end module

@@ -1,0 +1,47 @@
+! GPU leaf of the MCTDHX method: fabricating it binds the reference's Method and Propagator
+! procedure pointers to the MI355X implementation behind the C ABI (include/cody_mctdhx.h).
+! The input mirrors the JSON keys the reference reads for this system
+! (grid.linear.fedvr, sysKinetic, sysPotential.linear.softYukawa,
+! sysInteraction.linear.softYukawa, method.mb.orbBased, integratorList.sil).
+module M_Method_Mb_OrbBased_MctdhxHip
+  use iso_c_binding
+  use M_Utils_Types
+  implicit none
+
+  type :: T_MctdhxHip_Input
+    real(R64) :: xmin = -10.0_R64, xmax = 10.0_R64
+    integer(I32) :: nElements = 10, nLocals = 5
+    logical :: xminExcludedQ = .false., xmaxExcludedQ = .false.
+    real(R64) :: bodyMass = 1.0_R64
+    real(R64) :: position(1) = [0.0_R64], charge(1) = [1.0_R64]
+    real(R64) :: softening1(1) = [1.0_R64], softening2(1) = [0.0_R64], dampening(1) = [0.0_R64]
+    real(R64) :: strength = 1.0_R64, intSoftening1 = 1.0_R64, intSoftening2 = 0.0_R64
+    real(R64) :: intDampening = 0.0_R64
+    integer(I32) :: nBodies = 2, nOrbs = 4
+    real(R64) :: regularizationParameter = 1.0e-10_R64
+    integer(I32) :: krylovDim = 32, maxRestarts = 8
+    real(R64) :: tolerance = 1.0e-10_R64
+    logical :: metricQ = .true.
+    integer(I32) :: device = 0
+  end type
+
+  type(T_MctdhxHip_Input), target :: MctdhxHip_input
+  type(c_ptr) :: MctdhxHip_ctx = c_null_ptr
+  integer(I32) :: MctdhxHip_nCoeffs = 0
+  integer(I32) :: MctdhxHip_nState = 0
+
+  interface
+    !> Binds Method_Setup, Method_TimeDerivative and Method_GetEnergy to the GPU leaf.
+    module subroutine Method_Mb_OrbBased_MctdhxHip_Fabricate(input)
+      type(T_MctdhxHip_Input), intent(in) :: input
+    end subroutine
+    !> Binds Propagator_Setup / Propagator_Propagate to the device split step order 2
+    !> (SIL on the linear part, RK4 on the nonlinear part).
+    module subroutine Propagator_SplitStepHip_Fabricate
+    end subroutine
+    !> Releases the device context.
+    module subroutine Method_Mb_OrbBased_MctdhxHip_Finalize
+    end subroutine
+  end interface
+
+end module

@@ -1,0 +1,104 @@
+! Implementation of the GPU MCTDHX leaf.
+submodule(M_Method_Mb_OrbBased_MctdhxHip) S_Method_Mb_OrbBased_MctdhxHip
+  use M_CodyHip
+  implicit none
+
+contains
+
+  module subroutine Method_Mb_OrbBased_MctdhxHip_Fabricate(input)
+    use M_Method
+    type(T_MctdhxHip_Input), intent(in) :: input
+    MctdhxHip_input = input
+    Method_Setup => Setup
+    Method_TimeDerivative => TimeDerivative
+    Method_GetEnergy => GetEnergy
+  end subroutine
+
+  module subroutine Propagator_SplitStepHip_Fabricate
+    use M_Propagator
+    Propagator_Setup => PropagatorSetup
+    Propagator_Propagate => Propagate
+  end subroutine
+
+  module subroutine Method_Mb_OrbBased_MctdhxHip_Finalize
+    integer(c_int) :: r
+    if (c_associated(MctdhxHip_ctx)) r = cody_destroy(MctdhxHip_ctx)
+    MctdhxHip_ctx = c_null_ptr
+  end subroutine
+
+  !> Creates the device context, the FEDVR grid, kinetic energy, potentials and the packed
+  !> state (the allocations of Method_Setup, S_Method_Mb_OrbBased_Mctdhx.f90:40-68).
+  subroutine Setup
+    use M_Method
+    use M_Grid
+    type(T_MctdhxHip_Input), pointer :: in
+    integer(c_int) :: lo, hi, nG, nC, nS
+    in => MctdhxHip_input
+    lo = merge(1, 0, in%xminExcludedQ)
+    hi = merge(1, 0, in%xmaxExcludedQ)
+    nG = cody_fedvr_npoints(in%nElements, in%nLocals, lo, hi)
+    if (nG < 2) error stop "MctdhxHip Setup: invalid FEDVR layout"
+    Grid_nPoints = nG
+    if (allocated(Grid_Linear_xCoord)) deallocate (Grid_Linear_xCoord, Grid_Linear_weights)
+    allocate (Grid_Linear_xCoord(nG), Grid_Linear_weights(nG))
+    call CodyHip_Check(cody_fedvr_grid(in%xmin, in%xmax, in%nElements, in%nLocals, lo, hi, &
+                                       Grid_Linear_xCoord, Grid_Linear_weights), "fedvr_grid")
+    call CodyHip_Check(cody_create(MctdhxHip_ctx, in%device, nG, in%nBodies, in%nOrbs, &
+                                   in%regularizationParameter), "create")
+    call CodyHip_Check(cody_sizes(MctdhxHip_ctx, nC, nS), "sizes")
+    MctdhxHip_nCoeffs = nC
+    MctdhxHip_nState = nS
+    call CodyHip_Check(cody_set_grid(MctdhxHip_ctx, Grid_Linear_xCoord, Grid_Linear_weights), "set_grid")
+    call CodyHip_Check(cody_set_kinetic_fedvr(MctdhxHip_ctx, in%xmin, in%xmax, in%nElements, in%nLocals, &
+                                              lo, hi, in%bodyMass), "set_kinetic_fedvr")
+    call CodyHip_Check(cody_set_potential_softyukawa(MctdhxHip_ctx, 1, in%position, in%charge, &
+                                                     in%softening1, in%softening2, in%dampening), &
+                       "set_potential_softyukawa")
+    call CodyHip_Check(cody_set_interaction_softyukawa(MctdhxHip_ctx, in%strength, in%intSoftening1, &
+                                                       in%intSoftening2, in%intDampening), &
+                       "set_interaction_softyukawa")
+    if (allocated(Method_state)) deallocate (Method_state)
+    allocate (Method_state(nS))
+    Method_state(:) = (0.0_R64, 0.0_R64)
+  end subroutine
+
+  !> Method_TimeDerivative (S_Method_Mb_OrbBased_Mctdhx.f90:101-149) on the GPU.
+  subroutine TimeDerivative(dState, state, time)
+    complex(R64), intent(out), contiguous, target :: dState(:)
+    complex(R64), intent(in), contiguous, target :: state(:)
+    real(R64), intent(in) :: time
+    if (size(state) /= MctdhxHip_nState .or. size(dState) /= MctdhxHip_nState) &
+      error stop "MctdhxHip TimeDerivative: state size mismatch"
+    call CodyHip_Check(cody_time_derivative(MctdhxHip_ctx, state, dState, time), "time_derivative")
+  end subroutine
+
+  !> Method_GetEnergy (S_Method_Mb_OrbBased.f90:132-161) on the GPU.
+  function GetEnergy(time) result(res)
+    use M_Method
+    real(R64) :: res
+    real(R64), intent(in) :: time
+    real(c_double) :: e
+    call CodyHip_Check(cody_energy(MctdhxHip_ctx, Method_state, time, e), "energy")
+    res = e
+  end function
+
+  subroutine PropagatorSetup
+  end subroutine
+
+  !> Propagator_Propagate for splitStep.order2 (S_Propagator_SplitStep_Order2.f90:52-77) with
+  !> SIL on the linear part and RK4 on the nonlinear part, device resident between the copies.
+  subroutine Propagate(state, t0, t1)
+    complex(R64), intent(inout), contiguous :: state(:)
+    real(R64), intent(in) :: t0
+    real(R64), intent(in) :: t1
+    integer(c_int) :: conv
+    type(T_MctdhxHip_Input), pointer :: in
+    in => MctdhxHip_input
+    call CodyHip_Check(cody_upload_state(MctdhxHip_ctx, state), "upload_state")
+    call CodyHip_Check(cody_strang(MctdhxHip_ctx, t0, t1, in%krylovDim, in%tolerance, in%maxRestarts, &
+                                   merge(1, 0, in%metricQ), conv), "strang")
+    call CodyHip_Check(cody_download_state(MctdhxHip_ctx, state), "download_state")
+    if (conv == 0) write (*, '(A,ES13.6)') "Warning: Short iterative Lanczos step exceeded tolerance at t=", t0
+  end subroutine
+
+end submodule

@@ -1,0 +1,74 @@
+! Fortran driver of the GPU MCTDHX leaf, written like the reference's simulation tests
+! (test/simulationTests/He1d/T_He1d_06_TimePropagationMctdhxSil.f90:49-107): fabricate, set up,
+! evaluate Method_TimeDerivative and Method_GetEnergy, propagate with Propagator_Propagate.
+! Usage: t_mctdhx_hip <initial state file> <output directory>
+! State files are raw complex(R64) streams (the format of src/Utils/M_Utils_DataStorage.f90:20-65).
+program T_MctdhxHip_Rhs
+  use M_Utils_Types
+  use M_Method
+  use M_Propagator
+  use M_Grid
+  use M_Method_Mb_OrbBased_MctdhxHip
+  implicit none
+
+  type(T_MctdhxHip_Input) :: input
+  character(len=1024) :: inFile, outDir
+  complex(R64), allocatable :: dState(:)
+  real(R64) :: t, dt, e0, e1
+  integer :: io, iStep
+
+  call get_command_argument(1, inFile)
+  call get_command_argument(2, outDir)
+  if (len_trim(inFile) == 0 .or. len_trim(outDir) == 0) error stop "usage: t_mctdhx_hip <state> <outdir>"
+
+  ! 1D FEDVR MCTDHF: 100 elements x 5 GLL points on [-100, 100], N=2 fermions in M=4 orbitals
+  input%xmin = -100.0_R64
+  input%xmax = 100.0_R64
+  input%nElements = 100
+  input%nLocals = 5
+  input%charge = [2.0_R64]
+  input%softening1 = [1.0_R64]
+  input%softening2 = [0.0_R64]
+  input%strength = 1.0_R64
+  input%intSoftening1 = 1.0_R64
+  input%intSoftening2 = 0.0_R64
+  input%nBodies = 2
+  input%nOrbs = 4
+  input%regularizationParameter = 1.0e-8_R64
+  input%krylovDim = 16
+  input%tolerance = 1.0e-12_R64
+
+  call Method_Mb_OrbBased_MctdhxHip_Fabricate(input)
+  call Propagator_SplitStepHip_Fabricate
+  call Method_Setup
+  call Propagator_Setup
+
+  open (newunit=io, file=trim(inFile), access="stream", form="unformatted", status="old", action="read")
+  read (io) Method_state
+  close (io)
+
+  allocate (dState(size(Method_state)))
+  call Method_TimeDerivative(dState, Method_state, 0.0_R64)
+  open (newunit=io, file=trim(outDir)//"/dstate.bin", access="stream", form="unformatted", status="replace")
+  write (io) dState
+  close (io)
+
+  e0 = Method_GetEnergy(0.0_R64)
+  write (*, '(A,ES24.16)') "energy0 ", e0
+
+  t = 0.0_R64
+  dt = 0.01_R64
+  do iStep = 1, 5
+    call Propagator_Propagate(Method_state, t, t + dt)
+    t = t + dt
+  end do
+  e1 = Method_GetEnergy(t)
+  write (*, '(A,ES24.16)') "energy5 ", e1
+  write (*, '(A,ES24.16)') "gridsum ", sum(Grid_Linear_weights)
+  open (newunit=io, file=trim(outDir)//"/state5.bin", access="stream", form="unformatted", status="replace")
+  write (io) Method_state
+  close (io)
+
+  call Method_Mb_OrbBased_MctdhxHip_Finalize
+  write (*, '(A)') "done"
+end program
