@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -49,6 +50,74 @@ int ctx_alloc(void** p, size_t bytes) {
 }
 
 int set_error(int code, const char* msg) { return fail(code, msg); }
+
+// Allocates the operands of the selected contraction precision and splits V (once per V).
+int prepare_precision(cody_ctx* c) {
+  const size_t nv = (size_t)c->nGp * c->nGp, ns = (size_t)c->Pc * c->nGp;
+  int r = 0;
+  switch (c->precision) {
+    case CODY_PREC_INT8_SLICED:
+      if (!c->d_Vi8) {
+        if ((r = ctx_alloc((void**)&c->d_Vi8, nv * kInt8Slices)) || (r = ctx_alloc((void**)&c->d_Si8, ns * kInt8Slices)) ||
+            (r = ctx_alloc((void**)&c->d_eV, sizeof(int) * c->nGp)) || (r = ctx_alloc((void**)&c->d_eS, sizeof(int) * c->Pc)))
+          return r;
+      }
+      if (c->int_set && !c->vprec_ready)
+        launch_slice_rows(c->d_V, c->nGp, c->nGp, kInt8Slices, c->d_Vi8, c->d_eV, c->stream);
+      break;
+    case CODY_PREC_SPLIT_FP32:
+      if (!c->d_Vhi) {
+        if ((r = ctx_alloc((void**)&c->d_Vhi, nv * sizeof(float))) || (r = ctx_alloc((void**)&c->d_Vlo, nv * sizeof(float))) ||
+            (r = ctx_alloc((void**)&c->d_Shi, ns * sizeof(float))) || (r = ctx_alloc((void**)&c->d_Slo, ns * sizeof(float))))
+          return r;
+      }
+      if (c->int_set && !c->vprec_ready) launch_split_f32(c->d_V, nv, c->d_Vhi, c->d_Vlo, c->stream);
+      break;
+    case CODY_PREC_BF16X3:
+      if (!c->d_Vbf) {
+        if ((r = ctx_alloc((void**)&c->d_Vbf, 3 * nv * sizeof(__bf16))) || (r = ctx_alloc((void**)&c->d_Sbf, 3 * ns * sizeof(__bf16))))
+          return r;
+      }
+      if (c->int_set && !c->vprec_ready) launch_split_bf16x3(c->d_V, nv, c->d_Vbf, c->stream);
+      break;
+    default:
+      break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(CODY_ERR_HIP, std::string("prepare_precision: ") + hipGetErrorString(e));
+  c->vprec_ready = c->int_set;
+  return CODY_OK;
+}
+
+// Enqueue W = V S for the context's precision (ST already formed); returns the kernel name.
+const char* enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark) {
+  const size_t ns = (size_t)c->Pc * c->nGp;
+  switch (c->precision) {
+    case CODY_PREC_INT8_SLICED:
+      launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
+      mark("k_slice_rows");
+      launch_meanfield_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+      mark("k_meanfield_i8");
+      return "k_meanfield_i8";
+    case CODY_PREC_SPLIT_FP32:
+      launch_split_f32(c->d_ST, ns, c->d_Shi, c->d_Slo, st);
+      mark("k_split_f32");
+      launch_meanfield_f32x3(c->d_Vhi, c->d_Vlo, c->d_Shi, c->d_Slo, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+      mark("k_meanfield_f32x3");
+      return "k_meanfield_f32x3";
+    case CODY_PREC_BF16X3:
+      launch_split_bf16x3(c->d_ST, ns, c->d_Sbf, st);
+      mark("k_split_bf16x3");
+      launch_meanfield_bf16x3(c->d_Vbf, c->d_Sbf, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+      mark("k_meanfield_bf16x3");
+      return "k_meanfield_bf16x3";
+    default:
+      launch_meanfield_f64(c->d_V, c->d_ST, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+      mark("k_meanfield_f64");
+      return "k_meanfield_f64";
+  }
+}
 
 }  // namespace cody
 
@@ -212,7 +281,8 @@ int cody_set_interaction_softyukawa(cody_ctx* c, double s, double a1, double a2,
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   c->int_set = true;
-  return CODY_OK;
+  c->vprec_ready = false;
+  return prepare_precision(c);
 }
 
 int cody_set_laser(cody_ctx* c, int nK, const double* coef, double T) {
@@ -226,9 +296,12 @@ int cody_set_laser(cody_ctx* c, int nK, const double* coef, double T) {
 
 int cody_set_precision(cody_ctx* c, int mode) {
   if (!c) return fail(CODY_ERR_ARG, "cody_set_precision: null context");
-  if (mode != CODY_PREC_FP64) return fail(CODY_ERR_UNSUPPORTED, "cody_set_precision: mode not built");
+  if (mode < CODY_PREC_FP64 || mode > CODY_PREC_INT8_SLICED)
+    return fail(CODY_ERR_ARG, "cody_set_precision: unknown mode");
+  HIPCHK(hipSetDevice(c->device));
   c->precision = mode;
-  return CODY_OK;
+  c->vprec_ready = false;
+  return prepare_precision(c);
 }
 
 int cody_upload_state(cody_ctx* c, const double* state) {
@@ -279,50 +352,50 @@ int check_ready(const cody_ctx* c) {
 // Enqueue one right-hand side on `st`.  mode: kModeFull / kModeNonLin / kModeLin.
 // When `ev` is non-null, an event is recorded after each launch (profiling).
 int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode,
-                hipStream_t st, double* energy, std::vector<hipEvent_t>* ev) {
+                hipStream_t st, double* energy, RhsTrace* tr) {
   const double2* C = state;
   const double2* orbs = state + c->nC;
   double2* dC = dstate;
   double2* dorbs = dstate + c->nC;
-  auto mark = [&]() {
-    if (ev) {
+  auto mark = [&](const char* name) {
+    if (tr) {
       hipEvent_t e;
       hipEventCreate(&e);
       hipEventRecord(e, st);
-      ev->push_back(e);
+      tr->ev.push_back(e);
+      if (name) tr->names.emplace_back(name);
     }
   };
-  mark();
+  mark(nullptr);
   const double efield = laser_field(c, t);
   if (mode == kModeLin) {
     // TimeDerivativeOrbsLin: dphi = -i (T + V) phi, CI frozen (S_Method_Mb_OrbBased_Mctdhx.f90:152-177)
     launch_orbital(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield, c->d_WT, c->d_rho2p, c->nG,
                    c->nGp, c->nO, kModeLin, c->d_draw, c->d_part, st);
-    mark();
+    mark("k_orbital");
     launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 0, dorbs, st);
-    mark();
+    mark("k_project");
     launch_zero(dC, c->nC, st);
-    mark();
+    mark("k_zero");
     return CODY_OK;
   }
   launch_ci_density(C, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p, c->d_reginv, st);
-  mark();
+  mark("k_ci_density");
   launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, st);
-  mark();
-  launch_meanfield_f64(c->d_V, c->d_ST, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
-  mark();
+  mark("k_pair_src");
+  enqueue_contraction(c, st, mark);
   launch_wsum(c->d_Wpart, c->nsplit, c->nGp, c->Pc, c->d_WT, st);
-  mark();
+  mark("k_wsum");
   launch_orbital(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield, c->d_WT, c->d_rho2p, c->nG, c->nGp,
                  c->nO, mode, c->d_draw, c->d_part, st);
-  mark();
+  mark("k_orbital");
   launch_h2_gram(c->d_ST, c->d_WT, c->nGp, c->Pc, c->nxs, c->d_Gpart, st);
-  mark();
+  mark("k_h2_gram");
   launch_finalize(C, c->nC, c->nO, c->Pc, c->d_part, c->nPart, c->d_Gpart, c->nxs, c->dci, c->d_rho1,
                   c->d_rho2, c->d_h1, c->d_h2, c->d_inp, c->d_G, dC, 1, energy, st);
-  mark();
+  mark("k_finalize");
   launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 1, dorbs, st);
-  mark();
+  mark("k_project");
   return CODY_OK;
 }
 
@@ -384,7 +457,7 @@ int cody_meanfield_w(cody_ctx* c, const double* orbs, double* W) {
   double2* d_orbs = c->d_tmp + c->nC;
   HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
   launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, c->stream);
-  launch_meanfield_f64(c->d_V, c->d_ST, c->d_Wpart, c->nGp, c->Pc, c->nsplit, c->stream);
+  enqueue_contraction(c, c->stream, [](const char*) {});
   launch_wsum(c->d_Wpart, c->nsplit, c->nGp, c->Pc, c->d_WT, c->stream);
   HIPCHK(hipGetLastError());
   std::vector<double> wt((size_t)c->Pc * c->nGp);
@@ -444,30 +517,32 @@ int cody_densities(cody_ctx* c, const double* coeffs, double* rdm1, double* rdm2
 
 int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* names, int namesLen,
                      int* nKernels) {
-  static const char* kNames =
-      "k_ci_density;k_pair_src;k_meanfield_f64;k_wsum;k_orbital;k_h2_gram;k_finalize;k_project";
   if (!c || iters < 1 || !avg_ms || nMax < 1) return fail(CODY_ERR_ARG, "cody_profile_rhs: bad arguments");
   CHK(check_ready(c));
   HIPCHK(hipSetDevice(c->device));
   std::vector<double> acc;
+  std::string labels;
   int nk = 0;
   for (int it = 0; it < iters + 1; ++it) {
-    std::vector<hipEvent_t> ev;
-    CHK(enqueue_rhs(c, c->d_state, c->d_tmp2, 0.0, kModeFull, c->stream, nullptr, &ev));
+    RhsTrace tr;
+    CHK(enqueue_rhs(c, c->d_state, c->d_tmp2, 0.0, kModeFull, c->stream, nullptr, &tr));
     HIPCHK(hipStreamSynchronize(c->stream));
-    nk = static_cast<int>(ev.size()) - 1;
-    if (acc.empty()) acc.assign(nk, 0.0);
+    nk = static_cast<int>(tr.ev.size()) - 1;
+    if (acc.empty()) {
+      acc.assign(nk, 0.0);
+      for (int k = 0; k < nk; ++k) labels += (k ? ";" : "") + tr.names[k];
+    }
     for (int k = 0; k < nk; ++k) {
       float ms = 0.f;
-      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      hipEventElapsedTime(&ms, tr.ev[k], tr.ev[k + 1]);
       if (it > 0) acc[k] += ms;  // first pass is a warm-up
     }
-    for (auto e : ev) hipEventDestroy(e);
+    for (auto e : tr.ev) hipEventDestroy(e);
   }
   for (int k = 0; k < nk && k < nMax; ++k) avg_ms[k] = acc[k] / iters;
   if (nKernels) *nKernels = nk;
   if (names && namesLen > 0) {
-    std::strncpy(names, kNames, namesLen - 1);
+    std::strncpy(names, labels.c_str(), namesLen - 1);
     names[namesLen - 1] = '\0';
   }
   return CODY_OK;

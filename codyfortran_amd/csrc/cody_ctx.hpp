@@ -14,6 +14,12 @@ int ctx_alloc(void** p, size_t bytes);
 int set_error(int code, const char* msg);
 }  // namespace cody
 
+// Events recorded after each launch of a right-hand side (profiling).
+struct RhsTrace {
+  std::vector<hipEvent_t> ev;
+  std::vector<std::string> names;
+};
+
 // Work buffers of the device-side propagators (allocated on first use).
 struct PropBuffers {
   int kdim = 0;
@@ -59,6 +65,12 @@ struct cody_ctx {
   double2 *d_h1 = nullptr, *d_h2 = nullptr, *d_inp = nullptr;
   double2 *d_draw = nullptr, *d_part = nullptr;
   PropBuffers prop;
+  // reduced-precision operands of the mean-field contraction (allocated by cody_set_precision)
+  int8_t *d_Vi8 = nullptr, *d_Si8 = nullptr;
+  int *d_eV = nullptr, *d_eS = nullptr;
+  float *d_Vhi = nullptr, *d_Vlo = nullptr, *d_Shi = nullptr, *d_Slo = nullptr;
+  __bf16 *d_Vbf = nullptr, *d_Sbf = nullptr;
+  bool vprec_ready = false;  // V operands of the current precision are up to date
 
   int allocate() {
     using cody::ctx_alloc;
@@ -139,6 +151,9 @@ struct cody_ctx {
     for (void* p : ptrs)
       if (p) hipFree(p);
     prop.release();
+    void* lp[] = {d_Vi8, d_Si8, d_eV, d_eS, d_Vhi, d_Vlo, d_Shi, d_Slo, d_Vbf, d_Sbf};
+    for (void* p : lp)
+      if (p) hipFree(p);
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
   }

@@ -15,7 +15,7 @@
 
 namespace cody {
 int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode,
-                hipStream_t st, double* energy, std::vector<hipEvent_t>* ev);
+                hipStream_t st, double* energy, RhsTrace* tr);
 int check_ready(const cody_ctx* c);
 }  // namespace cody
 

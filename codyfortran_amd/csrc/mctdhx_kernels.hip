@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_pair_src(const double2* __restrict__ or
 // ------------------------------------------------------------------ mean-field contraction (FP64 MFMA)
 // Wpart[s][r][c] = sum_{k in split s} V[r][k] S[k][c], S[k][c] = ST[c][k].
 // v_mfma_f64_16x16x4f64: lane l holds A(row l%16, k l/16), B(k l/16, col l%16) and
-// D(row 4(l/16)+i, col l%16), i = 0..3.  The contraction index is permuted inside each
+// D(row 4i + l/16, col l%16), i = 0..3.  The contraction index is permuted inside each
 // 16-wide k block (MFMA step s of lane group q uses k = kb + 4q + s) so that every lane
 // fetches its four A and four B values with one 32-byte load each.
 template <int NT>
@@ -252,12 +252,13 @@ __global__ __launch_bounds__(256) void k_meanfield_f64(const double* __restrict_
       for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[t][s], acc[t], 0, 0, 0);
     }
   }
+  // FP64 16x16 result layout: accumulator element i of lane l is row 4i + l/16, column l%16
   double* out = Wpart + (size_t)split * nGp * Pc;
-  const int orow = blockIdx.x * 64 + wave * 16 + 4 * q;
+  const int orow = blockIdx.x * 64 + wave * 16 + q;
 #pragma unroll
   for (int t = 0; t < NT; ++t)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[(size_t)(orow + i) * Pc + t * 16 + rr] = acc[t][i];
+    for (int i = 0; i < 4; ++i) out[(size_t)(orow + 4 * i) * Pc + t * 16 + rr] = acc[t][i];
 }
 
 // Sum the split-K partials in fixed order and transpose to WT[c][x].

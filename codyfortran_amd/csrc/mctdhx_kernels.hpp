@@ -9,6 +9,7 @@ namespace cody {
 
 constexpr int kMaxOrbs = 12;   // orbitals per body type handled by the kernels
 constexpr int kMaxPc = 160;    // 16*ceil(M(M+1)/16) for M <= 12
+constexpr int kInt8Slices = 6; // 7-bit slices per operand of the int8-sliced contraction
 enum { kModeFull = 0, kModeNonLin = 1, kModeLin = 2 };
 
 struct CiEntryDev {
@@ -53,6 +54,16 @@ void launch_finalize(const double2* C, int nC, int nO, int Pc, const double2* pa
                      const double* Gpart, int nG2, const DevCiTables& t, const double2* rho1,
                      const double2* rho2, double2* h1, double2* h2, double2* inp, double* G,
                      double2* dC, int ciMode, double* energy, hipStream_t st);
+void launch_slice_rows(const double* X, int len, int nlines, int ns, int8_t* out, int* expo,
+                       hipStream_t st);
+void launch_meanfield_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB,
+                         double* Wpart, int nGp, int Pc, int nsplit, hipStream_t st);
+void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream_t st);
+void launch_meanfield_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl,
+                            double* Wpart, int nGp, int Pc, int nsplit, hipStream_t st);
+void launch_split_bf16x3(const double* X, size_t n, __bf16* out, hipStream_t st);
+void launch_meanfield_bf16x3(const __bf16* A, const __bf16* B, double* Wpart, int nGp, int Pc,
+                             int nsplit, hipStream_t st);
 void launch_project(const double2* orbs, const double2* draw, const double2* inp, int nG, int nO,
                     int project, double2* dorbs, hipStream_t st);
 
