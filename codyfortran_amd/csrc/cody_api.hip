@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <functional>
 #include <string>
 #include <vector>
@@ -189,6 +190,17 @@ int cody_create(cody_ctx** out, int device, int nG, int nBodies, int nOrbs, doub
       break;
     }
   c->nPart = c->nGp / 64;
+  {
+    const size_t lds = std::max(ci_density_lds_bytes(c->nC, nOrbs), ci_finalize_lds_bytes(c->nC, nOrbs));
+    if (lds > 160 * 1024) {
+      delete c;
+      return fail(CODY_ERR_UNSUPPORTED, "cody_create: CI workspace exceeds the 160 KiB of LDS");
+    }
+    if (lds > 64 * 1024 && ci_kernels_allow_lds(lds)) {
+      delete c;
+      return fail(CODY_ERR_HIP, "cody_create: could not raise the LDS limit of the CI kernels");
+    }
+  }
   int r = c->allocate();
   if (r) {
     delete c;
@@ -379,7 +391,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
     mark("k_zero");
     return CODY_OK;
   }
-  launch_ci_density(C, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p, c->d_reginv, st);
+  launch_ci_density(C, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p, c->d_reginv, st);
   mark("k_ci_density");
   launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, st);
   mark("k_pair_src");
@@ -391,9 +403,12 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   mark("k_orbital");
   launch_h2_gram(c->d_ST, c->d_WT, c->nGp, c->Pc, c->nxs, c->d_Gpart, st);
   mark("k_h2_gram");
-  launch_finalize(C, c->nC, c->nO, c->Pc, c->d_part, c->nPart, c->d_Gpart, c->nxs, c->dci, c->d_rho1,
-                  c->d_rho2, c->d_h1, c->d_h2, c->d_inp, c->d_G, dC, 1, energy, st);
-  mark("k_finalize");
+  launch_reduce_partials(c->d_part, c->nPart, c->nO * c->nO, c->d_Gpart, c->nxs, c->Pc * c->Pc, c->d_h1,
+                         c->d_inp, c->d_G, st);
+  mark("k_reduce_partials");
+  launch_ci_finalize(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_rho1, c->d_rho2, c->d_h2, dC,
+                     energy, st);
+  mark("k_ci_finalize");
   launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 1, dorbs, st);
   mark("k_project");
   return CODY_OK;
@@ -504,8 +519,8 @@ int cody_densities(cody_ctx* c, const double* coeffs, double* rdm1, double* rdm2
   if (!c || !coeffs) return fail(CODY_ERR_ARG, "cody_densities: null argument");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpyAsync(c->d_tmp, coeffs, sizeof(double2) * c->nC, hipMemcpyHostToDevice, c->stream));
-  launch_ci_density(c->d_tmp, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p, c->d_reginv,
-                    c->stream);
+  launch_ci_density(c->d_tmp, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p,
+                    c->d_reginv, c->stream);
   HIPCHK(hipGetLastError());
   const size_t n2 = (size_t)c->nO * c->nO, n4 = n2 * n2;
   if (rdm1) HIPCHK(hipMemcpyAsync(rdm1, c->d_rho1, sizeof(double2) * n2, hipMemcpyDeviceToHost, c->stream));
@@ -545,6 +560,43 @@ int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* nam
     std::strncpy(names, labels.c_str(), namesLen - 1);
     names[namesLen - 1] = '\0';
   }
+  return CODY_OK;
+}
+
+int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
+  if (!c || iters < 1 || !avg_ms) return fail(CODY_ERR_ARG, "cody_profile_contraction: bad arguments");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, st);
+  auto none = [](const char*) {};
+  enqueue_contraction(c, st, none);  // warm-up
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  for (int i = 0; i < iters; ++i) {
+    switch (c->precision) {
+      case CODY_PREC_INT8_SLICED:
+        launch_meanfield_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+        break;
+      case CODY_PREC_SPLIT_FP32:
+        launch_meanfield_f32x3(c->d_Vhi, c->d_Vlo, c->d_Shi, c->d_Slo, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+        break;
+      case CODY_PREC_BF16X3:
+        launch_meanfield_bf16x3(c->d_Vbf, c->d_Sbf, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+        break;
+      default:
+        launch_meanfield_f64(c->d_V, c->d_ST, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+    }
+  }
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *avg_ms = ms / iters;
   return CODY_OK;
 }
 

@@ -65,151 +65,26 @@ __global__ void k_fill_softyukawa(const double* __restrict__ x, int nG, int nGp,
   V[(size_t)i * nGp + j] = v;
 }
 
-// ------------------------------------------------------------------ CI densities
-// One workgroup.  rho1(i,j) = <a+_i a_j>, rho2(i1,i2,j1,j2) = <a+_i1 a+_i2 a_j2 a_j1> from the
-// gather lists (S_Coeffs_Generic.f90:103-319, unpack :325-434), then the regularised inverse
-// R = rho1 (rho1^2 + eps)^{-1} (equal to regInvMat, S_Method_Mb_OrbBased.f90:733-744) by a
-// Cholesky solve with one step of iterative refinement, then rho2'(:,:,j1,:) = sum_k rho2(:,:,k,:) R(k,j1).
-__global__ __launch_bounds__(256) void k_ci_density(
-    const double2* __restrict__ C, int nO, double reg, const int* __restrict__ r1_ptr,
-    const CiEntryDev* __restrict__ r1, const int* __restrict__ r2_ptr,
-    const CiEntryDev* __restrict__ r2, double2* __restrict__ rho1, double2* __restrict__ rho2,
-    double2* __restrict__ rho2p, double2* __restrict__ reginv) {
-  __shared__ double2 s_r1[kMaxOrbs * kMaxOrbs];
-  __shared__ double2 s_A[kMaxOrbs * kMaxOrbs];
-  __shared__ double2 s_L[kMaxOrbs * kMaxOrbs];
-  __shared__ double2 s_X[kMaxOrbs * kMaxOrbs];
-  __shared__ double2 s_Rs[kMaxOrbs * kMaxOrbs];
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int n2 = nO * nO, n4 = n2 * n2;
-  for (int e = t; e < n2; e += nt) {
-    const int i = e % nO, j = e / nO;  // column-major (i, j)
-    const int code = i * nO + j;
-    double2 acc = cmk(0, 0);
-    for (int q = r1_ptr[code]; q < r1_ptr[code + 1]; ++q) {
-      const CiEntryDev en = r1[q];
-      double2 v = cmul(cconj(C[en.jc]), C[en.ic]);
-      acc = cadd(acc, cscale(v, en.f));
-    }
-    s_r1[e] = acc;
-    rho1[e] = acc;
-  }
-  for (int e = t; e < n4; e += nt) {
-    int a = e % nO, b = (e / nO) % nO, c = (e / n2) % nO, d = e / (n2 * nO);
-    double2 acc = cmk(0, 0);
-    if (a != b && c != d) {
-      double sg = 1.0;
-      if (a > b) {
-        int tmp = a;
-        a = b;
-        b = tmp;
-        sg = -sg;
-      }
-      if (c > d) {
-        int tmp = c;
-        c = d;
-        d = tmp;
-        sg = -sg;
-      }
-      const int code = ((a * nO + c) * nO + b) * nO + d;
-      for (int q = r2_ptr[code]; q < r2_ptr[code + 1]; ++q) {
-        const CiEntryDev en = r2[q];
-        double2 v = cmul(cconj(C[en.jc]), C[en.ic]);
-        acc = cadd(acc, cscale(v, en.f));
-      }
-      acc = cscale(acc, sg);
-    }
-    rho2[e] = acc;
-  }
-  __syncthreads();
-  // A = rho1 rho1 + eps I
-  for (int e = t; e < n2; e += nt) {
-    const int i = e % nO, j = e / nO;
-    double2 acc = cmk(i == j ? reg : 0.0, 0.0);
-    for (int k = 0; k < nO; ++k) cfma(acc, s_r1[i + k * nO], s_r1[k + j * nO]);
-    s_A[e] = acc;
-  }
-  __syncthreads();
-  if (t == 0) {
-    // Cholesky A = L L^H (lower), in LDS
-    for (int j = 0; j < nO; ++j) {
-      double d = s_A[j + j * nO].x;
-      for (int k = 0; k < j; ++k) d -= s_L[j + k * nO].x * s_L[j + k * nO].x + s_L[j + k * nO].y * s_L[j + k * nO].y;
-      d = sqrt(d);
-      s_L[j + j * nO] = cmk(d, 0.0);
-      for (int i = j + 1; i < nO; ++i) {
-        double2 acc = s_A[i + j * nO];
-        for (int k = 0; k < j; ++k) {
-          // acc -= L(i,k) conj(L(j,k))
-          double2 lik = s_L[i + k * nO], ljk = s_L[j + k * nO];
-          acc.x -= lik.x * ljk.x + lik.y * ljk.y;
-          acc.y -= lik.y * ljk.x - lik.x * ljk.y;
-        }
-        s_L[i + j * nO] = cscale(acc, 1.0 / d);
-      }
-      for (int i = 0; i < j; ++i) s_L[i + j * nO] = cmk(0, 0);
-    }
-  }
-  __syncthreads();
-  // two passes: X = A^-1 rho1, then X += A^-1 (rho1 - A X)
-  for (int pass = 0; pass < 2; ++pass) {
-    if (t < nO) {
-      const int col = t;
-      double2 y[kMaxOrbs];
-      for (int i = 0; i < nO; ++i) {
-        double2 b;
-        if (pass == 0) {
-          b = s_r1[i + col * nO];
-        } else {
-          b = s_r1[i + col * nO];
-          for (int k = 0; k < nO; ++k) b = csub(b, cmul(s_A[i + k * nO], s_X[k + col * nO]));
-        }
-        for (int k = 0; k < i; ++k) b = csub(b, cmul(s_L[i + k * nO], y[k]));
-        y[i] = cscale(b, 1.0 / s_L[i + i * nO].x);
-      }
-      for (int i = nO - 1; i >= 0; --i) {
-        double2 b = y[i];
-        // L^H(i,k) = conj(L(k,i)), k > i
-        for (int k = i + 1; k < nO; ++k) b = csub(b, cmul(cconj(s_L[k + i * nO]), y[k]));
-        y[i] = cscale(b, 1.0 / s_L[i + i * nO].x);
-      }
-      if (pass == 0) {
-        for (int i = 0; i < nO; ++i) s_Rs[i + col * nO] = y[i];
-      } else {
-        for (int i = 0; i < nO; ++i) s_Rs[i + col * nO] = cadd(s_X[i + col * nO], y[i]);
-      }
-    }
-    __syncthreads();
-    for (int e = t; e < n2; e += nt) s_X[e] = s_Rs[e];
-    __syncthreads();
-  }
-  for (int e = t; e < n2; e += nt) reginv[e] = s_X[e];
-  __threadfence_block();
-  __syncthreads();
-  for (int e = t; e < n4; e += nt) {
-    const int i12 = e % n2, j1 = (e / n2) % nO, l2 = e / (n2 * nO);
-    double2 acc = cmk(0, 0);
-    for (int k = 0; k < nO; ++k) cfma(acc, rho2[i12 + n2 * (k + nO * l2)], s_X[k + j1 * nO]);
-    rho2p[e] = acc;
-  }
-}
-
 // ------------------------------------------------------------------ pair sources
 // ST[2p][x] + i ST[2p+1][x] = conj(phi_l(x)) phi_k(x) w(x), p = (l <= k)
 // (FillInteractionSrc, src/SysInteraction/Linear/S_SysInteraction_Linear.f90:57-72).
+template <int NO>
 __global__ __launch_bounds__(256) void k_pair_src(const double2* __restrict__ orbs,
                                                   const double* __restrict__ w, int nG, int nGp,
-                                                  int nO, int Pc, double* __restrict__ ST) {
+                                                  int Pc, double* __restrict__ ST) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nGp) return;
-  double2 phi[kMaxOrbs];
+  double2 phi[NO];
   const bool in = x < nG;
   const double wx = in ? w[x] : 0.0;
-  for (int k = 0; k < nO; ++k) phi[k] = in ? orbs[(size_t)k * nG + x] : cmk(0, 0);
+#pragma unroll
+  for (int k = 0; k < NO; ++k) phi[k] = in ? orbs[(size_t)k * nG + x] : cmk(0, 0);
   int c = 0;
-  for (int l = 0; l < nO; ++l)
-    for (int k = l; k < nO; ++k) {
-      double2 s = cscale(cmul(cconj(phi[l]), phi[k]), wx);
+#pragma unroll
+  for (int l = 0; l < NO; ++l)
+#pragma unroll
+    for (int k = l; k < NO; ++k) {
+      const double2 s = cscale(cmul(cconj(phi[l]), phi[k]), wx);
       ST[(size_t)c * nGp + x] = s.x;
       ST[(size_t)(c + 1) * nGp + x] = s.y;
       c += 2;
@@ -324,43 +199,53 @@ __device__ __forceinline__ double2 kinetic_at(const double2* __restrict__ f, int
 //   corr_j   = sum_{k1,(l2,k2)} W_{l2k2} phi_k1 rho2'(k1,k2,j,l2) (ApplyCorrelationOp :762-838)
 //   draw_j   = [hphi_j] + [corr_j]  (mode)
 // and per-workgroup partial sums of h1(i,j) = <phi_i|hphi_j>_w and inp(i,j) = <phi_i|draw_j>_w.
+// The rho2' coefficients depend only on the wave's orbital j and the loop indices, so their
+// loads are uniform across the wave.
+template <int NO>
 __global__ __launch_bounds__(256) void k_orbital(
     const double2* __restrict__ orbs, const double* __restrict__ w,
     const double* __restrict__ xg, const double* __restrict__ vext, KinParams kp, double efield,
-    const double* __restrict__ WT, const double2* __restrict__ rho2p, int nG, int nGp, int nO,
-    int mode, double2* __restrict__ draw, double2* __restrict__ part) {
+    const double* __restrict__ WT, const double2* __restrict__ rho2p, int nG, int nGp, int mode,
+    double2* __restrict__ draw, double2* __restrict__ part) {
+  constexpr int n2 = NO * NO;
   const int lane = threadIdx.x & 63;
   const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
   const int x = blockIdx.x * 64 + lane;
-  if (j >= nO) return;
+  if (j >= NO) return;
   const bool in = x < nG;
   const int xc = in ? x : nG - 1;
-  double2 phi[kMaxOrbs];
-  for (int k = 0; k < nO; ++k) phi[k] = in ? orbs[(size_t)k * nG + xc] : cmk(0, 0);
+  double2 phi[NO];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) phi[k] = in ? orbs[(size_t)k * nG + xc] : cmk(0, 0);
   const double wx = in ? w[xc] : 0.0;
   double2 hphi = cmk(0, 0), corr = cmk(0, 0);
   if (in) {
     hphi = kinetic_at(orbs + (size_t)j * nG, xc, kp, 1.0 / wx);
     const double v = vext[xc] + efield * xg[xc];
-    hphi.x = fma(v, phi[j].x, hphi.x);
-    hphi.y = fma(v, phi[j].y, hphi.y);
+    double2 pj = orbs[(size_t)j * nG + xc];
+    hphi.x = fma(v, pj.x, hphi.x);
+    hphi.y = fma(v, pj.y, hphi.y);
   }
   if (mode != kModeLin) {
-    const int n2 = nO * nO;
+    const double2* rj = rho2p + n2 * j;
     int p = 0;
-    for (int l = 0; l < nO; ++l) {
-      for (int k = l; k < nO; ++k, ++p) {
+#pragma unroll
+    for (int l = 0; l < NO; ++l) {
+#pragma unroll
+      for (int k = l; k < NO; ++k, ++p) {
         const double2 W = in ? cmk(WT[(size_t)(2 * p) * nGp + x], WT[(size_t)(2 * p + 1) * nGp + x])
                              : cmk(0, 0);
         // A = sum_k1 phi_k1 rho2'(k1,k,j,l); B = sum_k1 phi_k1 rho2'(k1,l,j,k)
-        const double2* ra = rho2p + nO * k + n2 * j + n2 * nO * l;
+        const double2* ra = rj + NO * k + n2 * NO * l;
         double2 A = cmk(0, 0);
-        for (int k1 = 0; k1 < nO; ++k1) cfma(A, phi[k1], ra[k1]);
+#pragma unroll
+        for (int k1 = 0; k1 < NO; ++k1) cfma(A, phi[k1], ra[k1]);
         cfma(corr, W, A);
         if (k != l) {
-          const double2* rb = rho2p + nO * l + n2 * j + n2 * nO * k;
+          const double2* rb = rj + NO * l + n2 * NO * k;
           double2 B = cmk(0, 0);
-          for (int k1 = 0; k1 < nO; ++k1) cfma(B, phi[k1], rb[k1]);
+#pragma unroll
+          for (int k1 = 0; k1 < NO; ++k1) cfma(B, phi[k1], rb[k1]);
           cfma(corr, cconj(W), B);
         }
       }
@@ -372,18 +257,20 @@ __global__ __launch_bounds__(256) void k_orbital(
   else d = hphi;
   if (in) draw[(size_t)j * nG + x] = d;
   // partials: part[block][0][i + j nO] = h1, part[block][1][i + j nO] = inp
-  double2* pb = part + (size_t)blockIdx.x * 2 * nO * nO;
-  for (int i = 0; i < nO; ++i) {
+  double2* pb = part + (size_t)blockIdx.x * 2 * n2;
+  const double2 hw = cscale(hphi, wx), dw = cscale(d, wx);
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
     double2 h = cmk(0, 0), q = cmk(0, 0);
-    cfmac(h, phi[i], cscale(hphi, wx));
-    cfmac(q, phi[i], cscale(d, wx));
+    cfmac(h, phi[i], hw);
+    cfmac(q, phi[i], dw);
     h.x = wave_sum(h.x);
     h.y = wave_sum(h.y);
     q.x = wave_sum(q.x);
     q.y = wave_sum(q.y);
     if (lane == 0) {
-      pb[i + j * nO] = h;
-      pb[nO * nO + i + j * nO] = q;
+      pb[i + j * NO] = h;
+      pb[n2 + i + j * NO] = q;
     }
   }
 }
@@ -412,133 +299,27 @@ __global__ __launch_bounds__(256) void k_h2_gram(const double* __restrict__ ST,
   Gpart[((size_t)blockIdx.z * Pc + c1) * Pc + c2] = acc;
 }
 
-// ------------------------------------------------------------------ finalize: h1, inp, h2, CI part
-// One workgroup of 1024 threads.  Sums the partials, assembles
-//   h2(i1,a,j1,b) = 1/2 sum_x src_(i1,j1) W_(a,b)  (FillH2CorrelationBlock :841-872),
-// applies dC = H1(h1) C + H2(h2) C with the reference's connectivity lists
-// (S_Coeffs_Generic.f90:103-319, h2 folding :478-542), projects dC -= <C|dC> C
-// (src/Coeffs/S_Coeffs.f90:105-111), scales by -i, and optionally forms the energy
-// E = sum h1(k,l) rho1(l,k) + sum h2(k1,k2,l1,l2) rho2(l1,l2,k1,k2).
-__global__ __launch_bounds__(1024) void k_finalize(
-    const double2* __restrict__ C, int nC, int nO, int Pc, const double2* __restrict__ part,
-    int nPart, const double* __restrict__ Gpart, int nG2, const int* __restrict__ s_ptr,
-    const CiEntryDev* __restrict__ sing, const int* __restrict__ d_ptr,
-    const CiEntryDev* __restrict__ dbl, const double2* __restrict__ rho1,
-    const double2* __restrict__ rho2, double2* __restrict__ h1, double2* __restrict__ h2,
-    double2* __restrict__ inp, double* __restrict__ G, double2* __restrict__ dC_out, int ciMode,
-    double* __restrict__ energy) {
-  __shared__ double red[2][32];
-  __shared__ double2 s_ov;
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int n2 = nO * nO, n4 = n2 * n2;
-  for (int e = t; e < 2 * n2; e += nt) {
-    double2 acc = cmk(0, 0);
-    for (int b = 0; b < nPart; ++b) acc = cadd(acc, part[(size_t)b * 2 * n2 + e]);
-    if (e < n2) h1[e] = acc;
-    else inp[e - n2] = acc;
-  }
-  for (int e = t; e < Pc * Pc; e += nt) {
-    double acc = 0.0;
-    for (int s = 0; s < nG2; ++s) acc += Gpart[(size_t)s * Pc * Pc + e];
-    G[e] = acc;
-  }
-  __threadfence_block();
-  __syncthreads();
-  for (int e = t; e < n4; e += nt) {
-    const int i1 = e % nO, a = (e / nO) % nO, j1 = (e / n2) % nO, b = e / (n2 * nO);
-    const int p1 = i1 <= j1 ? pair_index(nO, i1, j1) : pair_index(nO, j1, i1);
-    const int p2 = a <= b ? pair_index(nO, a, b) : pair_index(nO, b, a);
-    const double s1 = i1 <= j1 ? 1.0 : -1.0, s2 = a <= b ? 1.0 : -1.0;
-    const double rr = G[(2 * p1) * Pc + 2 * p2], ii = G[(2 * p1 + 1) * Pc + 2 * p2 + 1];
-    const double ri = G[(2 * p1) * Pc + 2 * p2 + 1], ir = G[(2 * p1 + 1) * Pc + 2 * p2];
-    h2[e] = cmk(0.5 * (rr - s1 * s2 * ii), 0.5 * (s2 * ri + s1 * ir));
-  }
-  __threadfence_block();
-  __syncthreads();
-  // CI H-apply, one configuration per thread
-  double2 dc = cmk(0, 0);
-  if (t < nC && ciMode) {
-    for (int q = s_ptr[t]; q < s_ptr[t + 1]; ++q) {
-      const CiEntryDev en = sing[q];
-      const int i = en.code / nO, j = en.code % nO;
-      cfma(dc, cscale(h1[i + j * nO], en.f), C[en.jc]);
-    }
-    for (int q = d_ptr[t]; q < d_ptr[t + 1]; ++q) {
-      const CiEntryDev en = dbl[q];
-      int code = en.code;
-      const int j2 = code % nO;
-      code /= nO;
-      const int i2 = code % nO;
-      code /= nO;
-      const int j1 = code % nO;
-      const int i1 = code / nO;
-      if (i1 == i2 || j1 == j2) continue;
-      double2 hl = csub(h2[idx4(i1, i2, j1, j2, nO)], h2[idx4(i1, i2, j2, j1, nO)]);
-      hl = csub(hl, h2[idx4(i2, i1, j1, j2, nO)]);
-      hl = cadd(hl, h2[idx4(i2, i1, j2, j1, nO)]);
-      cfma(dc, cscale(hl, en.f), C[en.jc]);
-    }
-  }
-  // <C|dC>
-  double2 ov = cmk(0, 0);
-  if (t < nC) cfmac(ov, C[t], dc);
-  ov.x = wave_sum(ov.x);
-  ov.y = wave_sum(ov.y);
-  const int wv = t >> 6, ln = t & 63;
-  if (ln == 0) {
-    red[0][wv] = ov.x;
-    red[1][wv] = ov.y;
-  }
-  __syncthreads();
-  if (t == 0) {
-    double2 s = cmk(0, 0);
-    for (int k = 0; k < nt / 64; ++k) s = cadd(s, cmk(red[0][k], red[1][k]));
-    s_ov = s;
-  }
-  __syncthreads();
-  if (t < nC) {
-    double2 r = ciMode ? csub(dc, cmul(s_ov, C[t])) : cmk(0, 0);
-    dC_out[t] = mul_minus_i(r);
-  }
-  if (energy) {
-    __syncthreads();
-    double e = 0.0;
-    for (int q = t; q < n2; q += nt) {
-      const int k = q % nO, l = q / nO;
-      const double2 a = h1[k + l * nO], b = rho1[l + k * nO];
-      e += a.x * b.x - a.y * b.y;
-    }
-    for (int q = t; q < n4; q += nt) {
-      const int k1 = q % nO, k2 = (q / nO) % nO, l1 = (q / n2) % nO, l2 = q / (n2 * nO);
-      const double2 a = h2[q], b = rho2[idx4(l1, l2, k1, k2, nO)];
-      e += a.x * b.x - a.y * b.y;
-    }
-    e = wave_sum(e);
-    if (ln == 0) red[0][wv] = e;
-    __syncthreads();
-    if (t == 0) {
-      double s = 0.0;
-      for (int k = 0; k < nt / 64; ++k) s += red[0][k];
-      *energy = s;
-    }
-  }
-}
-
 // ------------------------------------------------------------------ projection + scaling
 // dphi_j = -i (draw_j - sum_i phi_i inp(i,j))  (Grid_ProjectOnSubspace, src/Grid/S_Grid.f90:170-203)
+template <int NO>
 __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orbs,
                                                  const double2* __restrict__ draw,
-                                                 const double2* __restrict__ inp, int nG, int nO,
+                                                 const double2* __restrict__ inp, int nG,
                                                  int project, double2* __restrict__ dorbs) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nG) return;
-  double2 phi[kMaxOrbs];
-  if (project)
-    for (int i = 0; i < nO; ++i) phi[i] = orbs[(size_t)i * nG + x];
-  for (int j = 0; j < nO; ++j) {
+  double2 phi[NO];
+  if (project) {
+#pragma unroll
+    for (int i = 0; i < NO; ++i) phi[i] = orbs[(size_t)i * nG + x];
+  }
+#pragma unroll
+  for (int j = 0; j < NO; ++j) {
     double2 d = draw[(size_t)j * nG + x];
-    if (project)
-      for (int i = 0; i < nO; ++i) d = csub(d, cmul(phi[i], inp[i + j * nO]));
+    if (project) {
+#pragma unroll
+      for (int i = 0; i < NO; ++i) d = csub(d, cmul(phi[i], inp[i + j * NO]));
+    }
     dorbs[(size_t)j * nG + x] = mul_minus_i(d);
   }
 }
@@ -560,16 +341,30 @@ void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a
   hipLaunchKernelGGL(k_fill_softyukawa, grid, dim3(256), 0, st, x, nG, nGp, s, a1, a2, damp, V);
 }
 
-void launch_ci_density(const double2* C, int nO, double reg, const DevCiTables& t, double2* rho1,
-                       double2* rho2, double2* rho2p, double2* reginv, hipStream_t st) {
-  hipLaunchKernelGGL(k_ci_density, dim3(1), dim3(256), 0, st, C, nO, reg, t.r1_ptr, t.r1, t.r2_ptr,
-                     t.r2, rho1, rho2, rho2p, reginv);
-}
+#define CODY_NO_SWITCH(NOV, CALL) \
+  switch (NOV) {                    \
+    case 1: CALL(1); break;         \
+    case 2: CALL(2); break;         \
+    case 3: CALL(3); break;         \
+    case 4: CALL(4); break;         \
+    case 5: CALL(5); break;         \
+    case 6: CALL(6); break;         \
+    case 7: CALL(7); break;         \
+    case 8: CALL(8); break;         \
+    case 9: CALL(9); break;         \
+    case 10: CALL(10); break;       \
+    case 11: CALL(11); break;       \
+    case 12: CALL(12); break;       \
+    default: break;                 \
+  }
 
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
                      double* ST, hipStream_t st) {
-  hipLaunchKernelGGL(k_pair_src, dim3((nGp + 255) / 256), dim3(256), 0, st, orbs, w, nG, nGp, nO,
-                     Pc, ST);
+#define CALL_PS(NO)                                                                            \
+  hipLaunchKernelGGL(k_pair_src<NO>, dim3((nGp + 255) / 256), dim3(256), 0, st, orbs, w, nG, nGp, \
+                     Pc, ST)
+  CODY_NO_SWITCH(nO, CALL_PS)
+#undef CALL_PS
 }
 
 template <int NT>
@@ -607,8 +402,11 @@ void launch_orbital(const double2* orbs, const double* w, const double* x, const
                     hipStream_t st) {
   KinParams kp{kl.kref, kl.nE, kl.nLoc, kl.xminExcl, nG, kl.invh, kl.fac};
   dim3 grid(nGp / 64, (nO + 3) / 4);
-  hipLaunchKernelGGL(k_orbital, grid, dim3(256), 0, st, orbs, w, x, vext, kp, efield, WT, rho2p, nG,
-                     nGp, nO, mode, draw, part);
+#define CALL_OR(NO)                                                                          \
+  hipLaunchKernelGGL(k_orbital<NO>, grid, dim3(256), 0, st, orbs, w, x, vext, kp, efield, WT, \
+                     rho2p, nG, nGp, mode, draw, part)
+  CODY_NO_SWITCH(nO, CALL_OR)
+#undef CALL_OR
 }
 
 void launch_h2_gram(const double* ST, const double* WT, int nGp, int Pc, int nxs, double* Gpart,
@@ -617,19 +415,13 @@ void launch_h2_gram(const double* ST, const double* WT, int nGp, int Pc, int nxs
   hipLaunchKernelGGL(k_h2_gram, grid, dim3(256), 0, st, ST, WT, nGp, Pc, nGp / nxs, Gpart);
 }
 
-void launch_finalize(const double2* C, int nC, int nO, int Pc, const double2* part, int nPart,
-                     const double* Gpart, int nG2, const DevCiTables& t, const double2* rho1,
-                     const double2* rho2, double2* h1, double2* h2, double2* inp, double* G,
-                     double2* dC, int ciMode, double* energy, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, C, nC, nO, Pc, part, nPart, Gpart, nG2,
-                     t.s_ptr, t.singles, t.d_ptr, t.doubles, rho1, rho2, h1, h2, inp, G, dC, ciMode,
-                     energy);
-}
-
 void launch_project(const double2* orbs, const double2* draw, const double2* inp, int nG, int nO,
                     int project, double2* dorbs, hipStream_t st) {
-  hipLaunchKernelGGL(k_project, dim3((nG + 255) / 256), dim3(256), 0, st, orbs, draw, inp, nG, nO,
-                     project, dorbs);
+#define CALL_PR(NO)                                                                                 \
+  hipLaunchKernelGGL(k_project<NO>, dim3((nG + 255) / 256), dim3(256), 0, st, orbs, draw, inp, nG, \
+                     project, dorbs)
+  CODY_NO_SWITCH(nO, CALL_PR)
+#undef CALL_PR
 }
 
 }  // namespace cody

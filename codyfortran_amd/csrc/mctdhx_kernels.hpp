@@ -37,8 +37,16 @@ struct KinLaunch {
 void launch_zero(double2* p, int n, hipStream_t st);
 void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a1, double a2,
                             double damp, double* V, hipStream_t st);
-void launch_ci_density(const double2* C, int nO, double reg, const DevCiTables& t, double2* rho1,
-                       double2* rho2, double2* rho2p, double2* reginv, hipStream_t st);
+void launch_ci_density(const double2* C, int nC, int nO, double reg, const DevCiTables& t,
+                       double2* rho1, double2* rho2, double2* rho2p, double2* reginv, hipStream_t st);
+void launch_reduce_partials(const double2* part, int nPart, int n2, const double* Gpart, int nxs,
+                            int nG2e, double2* h1, double2* inp, double* G, hipStream_t st);
+void launch_ci_finalize(const double2* C, int nC, int nO, int Pc, const double* G, const double2* h1,
+                        const DevCiTables& t, const double2* rho1, const double2* rho2, double2* h2,
+                        double2* dC, double* energy, hipStream_t st);
+size_t ci_density_lds_bytes(int nC, int nO);
+size_t ci_finalize_lds_bytes(int nC, int nO);
+int ci_kernels_allow_lds(size_t bytes);
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
                      double* ST, hipStream_t st);
 void launch_meanfield_f64(const double* V, const double* ST, double* Wpart, int nGp, int Pc,
@@ -50,10 +58,6 @@ void launch_orbital(const double2* orbs, const double* w, const double* x, const
                     hipStream_t st);
 void launch_h2_gram(const double* ST, const double* WT, int nGp, int Pc, int nxs, double* Gpart,
                     hipStream_t st);
-void launch_finalize(const double2* C, int nC, int nO, int Pc, const double2* part, int nPart,
-                     const double* Gpart, int nG2, const DevCiTables& t, const double2* rho1,
-                     const double2* rho2, double2* h1, double2* h2, double2* inp, double* G,
-                     double2* dC, int ciMode, double* energy, hipStream_t st);
 void launch_slice_rows(const double* X, int len, int nlines, int ns, int8_t* out, int* expo,
                        hipStream_t st);
 void launch_meanfield_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB,

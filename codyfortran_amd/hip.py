@@ -71,6 +71,7 @@ def load():
         "cody_rk4": (i, [vp, d, d, i]),
         "cody_strang": (i, [vp, d, d, i, d, i, i, ip]),
         "cody_profile_rhs": (i, [vp, i, i, dp, C.c_char_p, i, ip]),
+        "cody_profile_contraction": (i, [vp, i, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -249,6 +250,12 @@ class Context:
         return bool(ok.value)
 
     # ---------------------------------------------------------------- measurement
+    def profile_contraction(self, iters=50):
+        """Average launch duration (ms) of the contraction kernel, back-to-back launches."""
+        ms = C.c_double()
+        _check(self.lib.cody_profile_contraction(self.h, iters, C.byref(ms)))
+        return ms.value
+
     def profile_rhs(self, iters=20):
         n = 32
         ms = (C.c_double * n)()

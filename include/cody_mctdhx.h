@@ -149,6 +149,11 @@ int cody_strang(cody_ctx* ctx, double t0, double t1, int krylovDim, double tol, 
 int cody_profile_rhs(cody_ctx* ctx, int iters, int nMax, double* avg_ms, char* names, int namesLen,
                      int* nKernels);
 
+/* Average duration of one launch of the mean-field contraction kernel of the current precision
+ * (the dominant kernel), from `iters` back-to-back launches on the context's stream bracketed by
+ * two HIP events; the pair sources are formed once from the context's state beforehand. */
+int cody_profile_contraction(cody_ctx* ctx, int iters, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,3 +11,18 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
     config.addinivalue_line("markers", "slow: long-running CPU check")
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _gpu_runtime_first(request):
+    """On a GPU run, let torch initialise the HIP runtime before the library's first call."""
+    markexpr = request.config.getoption("-m") or ""
+    if "gpu" in markexpr and "not gpu" not in markexpr:
+        try:
+            import torch
+
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except Exception:
+            pass
+    yield

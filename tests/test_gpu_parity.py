@@ -61,7 +61,11 @@ def test_densities_and_regularised_inverse(pair):
     r1, r2, ri = ctx.densities(C)
     assert rel(r1, s.rdm1(C)) < 1e-14
     assert rel(r2, s.rdm2(C)) < 1e-14
-    assert rel(ri, s.reg_inverse(s.rdm1(C))) < 1e-11
+    # R = rho1 (rho1^2 + eps)^-1 is as well conditioned as rho1^2 + eps: both the eigen route of the
+    # reference and the Cholesky route of the kernel carry errors ~ unit roundoff x condition number
+    occ = np.linalg.eigvalsh(s.rdm1(C))
+    kappa = (occ.max() ** 2 + w.reg) / (occ.min() ** 2 + w.reg)
+    assert rel(ri, s.reg_inverse(s.rdm1(C))) < 1e-14 * max(kappa, 100.0)
 
 
 def test_full_rhs(pair):
@@ -136,5 +140,19 @@ def test_propagation_matches_oracle(pair):
         t += dt
     sg = ctx.download_state()
     assert state_rel(sg, so, w.nC, wts, w.M) < 1e-10
-    # norm conservation of the unitary split step
+    # the CI norm after the split steps (RK4 on the nonlinear part is not exactly norm preserving
+    # at this step size) is the oracle's
+    assert abs(np.linalg.norm(sg[: w.nC]) - np.linalg.norm(so[: w.nC])) < 1e-12
+
+
+def test_split_step_conserves_norm_at_small_step(pair):
+    name, w, s, ctx, state, wts = pair
+    if w.nG > 1000:
+        pytest.skip("small grids only")
+    ctx.upload_state(state)
+    t, dt = 0.0, 1e-5
+    for _ in range(5):
+        assert ctx.strang(t, t + dt, krylov_dim=24, tol=1e-12, max_restarts=8, metric=True)
+        t += dt
+    sg = ctx.download_state()
     assert abs(np.linalg.norm(sg[: w.nC]) - 1.0) < 1e-10
