@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcody_hip.so")
 
-SOURCES = ["host_setup.cpp", "mctdhx_kernels.hip", "ci_kernels.hip", "meanfield_lowprec.hip",
+SOURCES = ["host_setup.cpp", "mctdhx_kernels.hip", "ci_kernels.hip", "meanfield_gemm.hip",
+           "meanfield_lowprec.hip",
            "cody_api.hip", "cody_propagate.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

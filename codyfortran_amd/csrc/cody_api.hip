@@ -64,7 +64,7 @@ int prepare_precision(cody_ctx* c) {
           return r;
       }
       if (c->int_set && !c->vprec_ready)
-        launch_slice_rows(c->d_V, c->nGp, c->nGp, kInt8Slices, c->d_Vi8, c->d_eV, c->stream);
+        launch_slice_rows(c->d_V, c->nGp, c->nGp, kInt8Slices, 1, c->d_Vi8, c->d_eV, c->stream);
       break;
     case CODY_PREC_SPLIT_FP32:
       if (!c->d_Vhi) {
@@ -82,6 +82,8 @@ int prepare_precision(cody_ctx* c) {
       if (c->int_set && !c->vprec_ready) launch_split_bf16x3(c->d_V, nv, c->d_Vbf, c->stream);
       break;
     default:
+      if (!c->d_Vt && (r = ctx_alloc((void**)&c->d_Vt, nv * sizeof(double)))) return r;
+      if (c->int_set && !c->vprec_ready) launch_tile_f64(c->d_V, c->nGp, c->d_Vt, c->stream);
       break;
   }
   hipError_t e = hipGetLastError();
@@ -91,32 +93,35 @@ int prepare_precision(cody_ctx* c) {
   return CODY_OK;
 }
 
+// Row-major S^T is needed only by the reduced-precision contractions (FP64 reads the tiled copy).
+double* plain_st(cody_ctx* c) { return c->precision == CODY_PREC_FP64 ? nullptr : c->d_ST; }
+
 // Enqueue W = V S for the context's precision (ST already formed); returns the kernel name.
 const char* enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark) {
   const size_t ns = (size_t)c->Pc * c->nGp;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
-      launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
+      launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, 0, c->d_Si8, c->d_eS, st);
       mark("k_slice_rows");
-      launch_meanfield_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
-      mark("k_meanfield_i8");
-      return "k_meanfield_i8";
+      launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
+      mark("k_mf_i8");
+      return "k_mf_i8";
     case CODY_PREC_SPLIT_FP32:
       launch_split_f32(c->d_ST, ns, c->d_Shi, c->d_Slo, st);
       mark("k_split_f32");
-      launch_meanfield_f32x3(c->d_Vhi, c->d_Vlo, c->d_Shi, c->d_Slo, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
-      mark("k_meanfield_f32x3");
-      return "k_meanfield_f32x3";
+      launch_mf_f32x3(c->d_Vhi, c->d_Vlo, c->d_Shi, c->d_Slo, c->d_WT, c->nGp, c->Pc, st);
+      mark("k_mf_f32x3");
+      return "k_mf_f32x3";
     case CODY_PREC_BF16X3:
       launch_split_bf16x3(c->d_ST, ns, c->d_Sbf, st);
       mark("k_split_bf16x3");
-      launch_meanfield_bf16x3(c->d_Vbf, c->d_Sbf, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
-      mark("k_meanfield_bf16x3");
-      return "k_meanfield_bf16x3";
+      launch_mf_bf16x3(c->d_Vbf, c->d_Sbf, c->d_WT, c->nGp, c->Pc, st);
+      mark("k_mf_bf16x3");
+      return "k_mf_bf16x3";
     default:
-      launch_meanfield_f64(c->d_V, c->d_ST, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
-      mark("k_meanfield_f64");
-      return "k_meanfield_f64";
+      launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, st);
+      mark("k_mf_f64_w + k_wsum");
+      return "k_mf_f64_w";
   }
 }
 
@@ -157,7 +162,7 @@ int cody_create(cody_ctx** out, int device, int nG, int nBodies, int nOrbs, doub
   auto* c = new cody_ctx();
   c->device = device;
   c->nG = nG;
-  c->nGp = 64 * ((nG + 63) / 64);
+  c->nGp = 256 * ((nG + 255) / 256);
   c->N = nBodies;
   c->nO = nOrbs;
   c->reg = reg;
@@ -182,16 +187,17 @@ int cody_create(cody_ctx** out, int device, int nG, int nBodies, int nOrbs, doub
       c->nsplit = s;
       break;
     }
+  // split-K factor of the Gram matrix G = S^T W (nGp / (64 nxs) must be an integer)
   const int k64 = c->nGp / 64;
   c->nxs = 1;
-  for (int s : {8, 7, 6, 5, 4, 3, 2, 1})
+  for (int s : {32, 16, 8, 4, 2, 1})
     if (k64 % s == 0) {
       c->nxs = s;
       break;
     }
-  c->nPart = c->nGp / 64;
+  c->nPart = c->nGp / 64;  // one partial per 64-point workgroup of k_orbital
   {
-    const size_t lds = std::max(ci_density_lds_bytes(c->nC, nOrbs), ci_finalize_lds_bytes(c->nC, nOrbs));
+    const size_t lds = ci_density_lds_bytes(c->nC, nOrbs);
     if (lds > 160 * 1024) {
       delete c;
       return fail(CODY_ERR_UNSUPPORTED, "cody_create: CI workspace exceeds the 160 KiB of LDS");
@@ -241,6 +247,7 @@ int cody_set_grid(cody_ctx* c, const double* x, const double* w) {
   HIPCHK(hipMemcpy(c->d_x, x, sizeof(double) * c->nG, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_w, w, sizeof(double) * c->nG, hipMemcpyHostToDevice));
   c->grid_set = true;
+  c->drop_graphs();
   c->int_set = false;  // the interaction matrix depends on the grid
   return CODY_OK;
 }
@@ -258,6 +265,7 @@ int cody_set_kinetic_fedvr(cody_ctx* c, double xmin, double xmax, int nE, int nL
   HIPCHK(hipMemcpy(c->d_kref, kref.data(), sizeof(double) * nLoc * nLoc, hipMemcpyHostToDevice));
   c->kin = KinLaunch{c->d_kref, nE, nLoc, xminExcl != 0 ? 1 : 0, nE / (xmax - xmin), -0.5 / mass};
   c->kin_set = true;
+  c->drop_graphs();
   return CODY_OK;
 }
 
@@ -266,6 +274,7 @@ int cody_set_potential(cody_ctx* c, const double* vext) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpy(c->d_vext, vext, sizeof(double) * c->nG, hipMemcpyHostToDevice));
   c->pot_set = true;
+  c->drop_graphs();
   return CODY_OK;
 }
 
@@ -293,6 +302,7 @@ int cody_set_interaction_softyukawa(cody_ctx* c, double s, double a1, double a2,
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   c->int_set = true;
+  c->drop_graphs();
   c->vprec_ready = false;
   return prepare_precision(c);
 }
@@ -303,6 +313,12 @@ int cody_set_laser(cody_ctx* c, int nK, const double* coef, double T) {
   c->nLaser = nK;
   c->laserT = T;
   for (int k = 0; k < nK; ++k) c->laserC[k] = coef[k];
+  if (nK == 0) {  // the field stays zero: no per-call update of the device scalar
+    HIPCHK(hipSetDevice(c->device));
+    const double zero = 0.0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->d_efield, &zero, sizeof(double), hipMemcpyHostToDevice));
+  }
   return CODY_OK;
 }
 
@@ -313,6 +329,7 @@ int cody_set_precision(cody_ctx* c, int mode) {
   HIPCHK(hipSetDevice(c->device));
   c->precision = mode;
   c->vprec_ready = false;
+  c->drop_graphs();
   return prepare_precision(c);
 }
 
@@ -379,44 +396,109 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
     }
   };
   mark(nullptr);
-  const double efield = laser_field(c, t);
+  // E(t) is read from device memory, so that a captured right-hand side stays valid for any t
+  if (c->nLaser > 0 && !c->capturing) launch_set_scalar(c->d_efield, laser_field(c, t), st);
+  const int n2 = c->nO * c->nO, nPart1 = (c->nG + 255) / 256;
   if (mode == kModeLin) {
     // TimeDerivativeOrbsLin: dphi = -i (T + V) phi, CI frozen (S_Method_Mb_OrbBased_Mctdhx.f90:152-177)
-    launch_orbital(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield, c->d_WT, c->d_rho2p, c->nG,
-                   c->nGp, c->nO, kModeLin, c->d_draw, c->d_part, st);
-    mark("k_orbital");
-    launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 0, dorbs, st);
+    launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_draw, c->d_part1,
+                   st);
+    mark("k_onebody");
+    launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 0, dorbs, nullptr, nullptr, 0, nullptr, st);
     mark("k_project");
     launch_zero(dC, c->nC, st);
     mark("k_zero");
     return CODY_OK;
   }
-  launch_ci_density(C, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p, c->d_reginv, st);
-  mark("k_ci_density");
-  launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, st);
+  // side stream, part 1 (independent of the mean fields, overlaps the contraction): CI densities,
+  // R and rho2'; h phi and h1
+  hipEventRecord(c->ev_fork, st);
+  hipStreamWaitEvent(c->stream_ci, c->ev_fork, 0);
+  launch_ci_density(C, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_rho2p,
+                    c->d_reginv, c->stream_ci);
+  launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_hphi, c->d_part1,
+                 c->stream_ci);
+  launch_sum_parts(c->d_part1, nPart1, n2, n2, c->d_h1, c->stream_ci);
+  hipEventRecord(c->ev_a, c->stream_ci);
+  // main stream: pair sources and the mean fields W = V S
+  launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st);
   mark("k_pair_src");
   enqueue_contraction(c, st, mark);
-  launch_wsum(c->d_Wpart, c->nsplit, c->nGp, c->Pc, c->d_WT, st);
-  mark("k_wsum");
-  launch_orbital(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield, c->d_WT, c->d_rho2p, c->nG, c->nGp,
-                 c->nO, mode, c->d_draw, c->d_part, st);
+  hipEventRecord(c->ev_w, st);
+  // side stream, part 2: Gram matrix G = S^T W, CI H-apply (and the energy on request)
+  hipStreamWaitEvent(c->stream_ci, c->ev_w, 0);
+  launch_gram_f64(c->d_STt, c->d_WT, c->d_Gpart, c->nGp, c->Pc, c->nxs, c->stream_ci);
+  launch_gram_reduce(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->stream_ci);
+  launch_ci_apply(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_dCraw, c->stream_ci);
+  if (energy) launch_ci_energy(c->nO, c->Pc, c->d_G, c->d_h1, c->d_rho1, c->d_rho2, energy, c->stream_ci);
+  hipEventRecord(c->ev_ci, c->stream_ci);
+  // main stream: orbital correlation, projector overlaps, then both projections
+  hipStreamWaitEvent(st, c->ev_a, 0);
+  launch_orbital(orbs, c->d_w, c->d_hphi, c->d_WT, c->d_rho2p, c->nG, c->nGp, c->nO, mode, c->d_draw,
+                 c->d_part, st);
   mark("k_orbital");
-  launch_h2_gram(c->d_ST, c->d_WT, c->nGp, c->Pc, c->nxs, c->d_Gpart, st);
-  mark("k_h2_gram");
-  launch_reduce_partials(c->d_part, c->nPart, c->nO * c->nO, c->d_Gpart, c->nxs, c->Pc * c->Pc, c->d_h1,
-                         c->d_inp, c->d_G, st);
-  mark("k_reduce_partials");
-  launch_ci_finalize(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_rho1, c->d_rho2, c->d_h2, dC,
-                     energy, st);
-  mark("k_ci_finalize");
-  launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 1, dorbs, st);
+  launch_sum_parts(c->d_part, c->nPart, n2, n2, c->d_inp, st);
+  mark("k_sum_parts");
+  hipStreamWaitEvent(st, c->ev_ci, 0);
+  launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 1, dorbs, C, c->d_dCraw, c->nC, dC, st);
   mark("k_project");
   return CODY_OK;
+}
+
+// Right-hand side through a captured graph of enqueue_rhs (captured on the context's own stream
+// on first use for these buffers and mode, replayed on `st`); plain launches when graphs are off
+// or capture fails.  The graph holds no memory-set nodes.
+int run_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode, hipStream_t st) {
+  if (c->graphs_on) {
+    hipGraphExec_t ex = nullptr;
+    for (const auto& g : c->graphs)
+      if (g.state == state && g.dstate == dstate && g.mode == mode && g.energy == nullptr) ex = g.exec;
+    if (!ex) {
+      hipStream_t cs = c->stream;
+      if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        c->capturing = true;
+        const int r = enqueue_rhs(c, state, dstate, t, mode, cs, nullptr, nullptr);
+        c->capturing = false;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);
+        if (r == CODY_OK && e == hipSuccess && g &&
+            hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) {
+          if (c->graphs.size() >= 16) {
+            hipGraphExecDestroy(c->graphs.front().exec);
+            c->graphs.erase(c->graphs.begin());
+          }
+          c->graphs.push_back(RhsGraph{state, dstate, mode, nullptr, ex});
+        } else {
+          ex = nullptr;
+          c->graphs_on = false;  // capture unsupported here: plain launches from now on
+        }
+        if (g) hipGraphDestroy(g);
+        (void)hipGetLastError();
+      } else {
+        c->graphs_on = false;
+      }
+    }
+    if (ex) {
+      if (c->nLaser > 0) launch_set_scalar(c->d_efield, laser_field(c, t), st);
+      if (hipGraphLaunch(ex, st) != hipSuccess) return set_error(CODY_ERR_HIP, "hipGraphLaunch failed");
+      return CODY_OK;
+    }
+  }
+  return enqueue_rhs(c, state, dstate, t, mode, st, nullptr, nullptr);
 }
 
 }  // namespace cody
 
 extern "C" {
+
+int cody_set_graphs(cody_ctx* c, int enable) {
+  if (!c) return fail(CODY_ERR_ARG, "cody_set_graphs: null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->drop_graphs();
+  c->graphs_on = enable != 0;
+  return CODY_OK;
+}
 
 static int rhs_dev(cody_ctx* c, const void* d_state, void* d_dstate, double t, void* stream,
                    int mode) {
@@ -425,8 +507,7 @@ static int rhs_dev(cody_ctx* c, const void* d_state, void* d_dstate, double t, v
   CHK(check_ready(c));
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-  CHK(enqueue_rhs(c, static_cast<const double2*>(d_state), static_cast<double2*>(d_dstate), t, mode,
-                  st, nullptr, nullptr));
+  CHK(run_rhs(c, static_cast<const double2*>(d_state), static_cast<double2*>(d_dstate), t, mode, st));
   HIPCHK(hipGetLastError());
   return CODY_OK;
 }
@@ -471,9 +552,8 @@ int cody_meanfield_w(cody_ctx* c, const double* orbs, double* W) {
   HIPCHK(hipSetDevice(c->device));
   double2* d_orbs = c->d_tmp + c->nC;
   HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
-  launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, c->stream);
+  launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, c->stream);
   enqueue_contraction(c, c->stream, [](const char*) {});
-  launch_wsum(c->d_Wpart, c->nsplit, c->nGp, c->Pc, c->d_WT, c->stream);
   HIPCHK(hipGetLastError());
   std::vector<double> wt((size_t)c->Pc * c->nGp);
   HIPCHK(hipMemcpyAsync(wt.data(), c->d_WT, sizeof(double) * wt.size(), hipMemcpyDeviceToHost, c->stream));
@@ -492,25 +572,14 @@ int cody_apply_single_body(cody_ctx* c, const double* orbs, double t, double* do
   HIPCHK(hipSetDevice(c->device));
   double2* d_orbs = c->d_tmp + c->nC;
   HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
-  launch_orbital(d_orbs, c->d_w, c->d_x, c->d_vext, c->kin, laser_field(c, t), c->d_WT, c->d_rho2p,
-                 c->nG, c->nGp, c->nO, kModeLin, c->d_draw, c->d_part, c->stream);
+  if (c->nLaser > 0) launch_set_scalar(c->d_efield, laser_field(c, t), c->stream);
+  launch_onebody(d_orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_draw, c->d_part1,
+                 c->stream);
+  const int n2 = c->nO * c->nO;
+  launch_sum_parts(c->d_part1, (c->nG + 255) / 256, n2, n2, c->d_h1, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(dorbs, c->d_draw, sizeof(double2) * c->nG * c->nO, hipMemcpyDeviceToHost, c->stream));
-  if (h1) {
-    std::vector<double2> part((size_t)c->nPart * 2 * c->nO * c->nO);
-    HIPCHK(hipMemcpyAsync(part.data(), c->d_part, sizeof(double2) * part.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    const int n2 = c->nO * c->nO;
-    for (int e = 0; e < n2; ++e) {
-      double re = 0, im = 0;
-      for (int b = 0; b < c->nPart; ++b) {
-        re += part[(size_t)b * 2 * n2 + e].x;
-        im += part[(size_t)b * 2 * n2 + e].y;
-      }
-      h1[2 * e] = re;
-      h1[2 * e + 1] = im;
-    }
-  }
+  if (h1) HIPCHK(hipMemcpyAsync(h1, c->d_h1, sizeof(double2) * n2, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return CODY_OK;
 }
@@ -519,8 +588,8 @@ int cody_densities(cody_ctx* c, const double* coeffs, double* rdm1, double* rdm2
   if (!c || !coeffs) return fail(CODY_ERR_ARG, "cody_densities: null argument");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpyAsync(c->d_tmp, coeffs, sizeof(double2) * c->nC, hipMemcpyHostToDevice, c->stream));
-  launch_ci_density(c->d_tmp, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_rho2p,
-                    c->d_reginv, c->stream);
+  launch_ci_density(c->d_tmp, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can,
+                    c->d_rho2p, c->d_reginv, c->stream);
   HIPCHK(hipGetLastError());
   const size_t n2 = (size_t)c->nO * c->nO, n4 = n2 * n2;
   if (rdm1) HIPCHK(hipMemcpyAsync(rdm1, c->d_rho1, sizeof(double2) * n2, hipMemcpyDeviceToHost, c->stream));
@@ -568,7 +637,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   CHK(check_ready(c));
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, c->d_ST, st);
+  launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st);
   auto none = [](const char*) {};
   enqueue_contraction(c, st, none);  // warm-up
   hipEvent_t e0, e1;
@@ -578,16 +647,16 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   for (int i = 0; i < iters; ++i) {
     switch (c->precision) {
       case CODY_PREC_INT8_SLICED:
-        launch_meanfield_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+        launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
         break;
       case CODY_PREC_SPLIT_FP32:
-        launch_meanfield_f32x3(c->d_Vhi, c->d_Vlo, c->d_Shi, c->d_Slo, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+        launch_mf_f32x3(c->d_Vhi, c->d_Vlo, c->d_Shi, c->d_Slo, c->d_WT, c->nGp, c->Pc, st);
         break;
       case CODY_PREC_BF16X3:
-        launch_meanfield_bf16x3(c->d_Vbf, c->d_Sbf, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+        launch_mf_bf16x3(c->d_Vbf, c->d_Sbf, c->d_WT, c->nGp, c->Pc, st);
         break;
-      default:
-        launch_meanfield_f64(c->d_V, c->d_ST, c->d_Wpart, c->nGp, c->Pc, c->nsplit, st);
+      default:  // the contraction kernel alone (its partial sum k_wsum is not timed)
+        launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, nullptr, c->nGp, c->Pc, st);
     }
   }
   HIPCHK(hipEventRecord(e1, st));

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,15 @@ struct RhsTrace {
   std::vector<std::string> names;
 };
 
+// One captured right-hand side (both streams, all launches) for fixed buffers and mode.
+struct RhsGraph {
+  const void* state;
+  const void* dstate;
+  int mode;
+  const double* energy;
+  hipGraphExec_t exec;
+};
+
 // Work buffers of the device-side propagators (allocated on first use).
 struct PropBuffers {
   int kdim = 0;
@@ -27,7 +37,18 @@ struct PropBuffers {
   double2 *phi = nullptr, *nrm2 = nullptr, *trial = nullptr, *work = nullptr;
   double *alpha = nullptr, *beta = nullptr, *flags = nullptr, *h_flags = nullptr;
   double2 *rk_inc = nullptr, *rk_tmp = nullptr, *rk_d = nullptr;
+  // captured right-hand sides, replayed by the device-pointer entry points and the propagators;
+  // dropped whenever a setter changes what the launches bake in (kernels, sizes, parameters)
+  std::vector<RhsGraph> graphs;
+  bool graphs_on = true;
+  bool capturing = false;  // enqueue_rhs is being captured (E(t) is then set per replay)
+  void drop_graphs() {
+    for (auto& g : graphs) hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+
   void release() {
+    drop_graphs();
     void* ptrs[] = {basis, w, deriv, partial, dots, phi, nrm2, trial, work,
                     alpha, beta, flags, rk_inc, rk_tmp, rk_d};
     for (void* p : ptrs)
@@ -40,6 +61,10 @@ struct PropBuffers {
 struct cody_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // side stream (default priority): CI densities, one-body part and h1 while the contraction runs,
+  // then the Gram matrix and the CI H-apply while the orbital correlation runs
+  hipStream_t stream_ci = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_a = nullptr, ev_w = nullptr, ev_ci = nullptr;
   // sizes
   int nG = 0, nGp = 0, N = 0, nO = 0, nC = 0, nS = 0, P = 0, Pc = 0;
   int nsplit = 1, nxs = 1, nPart = 1;
@@ -58,14 +83,19 @@ struct cody_ctx {
   void* ci_blob = nullptr;
   // device buffers
   double *d_x = nullptr, *d_w = nullptr, *d_vext = nullptr, *d_kref = nullptr;
-  double *d_V = nullptr, *d_ST = nullptr, *d_Wpart = nullptr, *d_WT = nullptr;
+  double *d_V = nullptr, *d_ST = nullptr, *d_WT = nullptr;
+  double *d_STt = nullptr, *d_Wpart = nullptr;  // fragment-tiled S^T, k-split partials of W^T
   double *d_Gpart = nullptr, *d_G = nullptr, *d_energy = nullptr;
+  double* d_efield = nullptr;  // E(t) of the dipole coupling, read by the one-body kernel
   double2 *d_state = nullptr, *d_tmp = nullptr, *d_tmp2 = nullptr;
   double2 *d_rho1 = nullptr, *d_rho2 = nullptr, *d_rho2p = nullptr, *d_reginv = nullptr;
-  double2 *d_h1 = nullptr, *d_h2 = nullptr, *d_inp = nullptr;
-  double2 *d_draw = nullptr, *d_part = nullptr;
+  double2* d_r2can = nullptr;  // canonical rho2Lin (i1 < i2, j1 < j2)
+  double2 *d_h1 = nullptr, *d_inp = nullptr, *d_hphi = nullptr, *d_dCraw = nullptr;
+  double2 *d_draw = nullptr, *d_part = nullptr, *d_part1 = nullptr;
   PropBuffers prop;
-  // reduced-precision operands of the mean-field contraction (allocated by cody_set_precision)
+  // contraction operands derived from V (allocated by cody_set_precision): FP64 fragment-tiled
+  // copy, int8 slices, FP32 and BF16 splits
+  double* d_Vt = nullptr;
   int8_t *d_Vi8 = nullptr, *d_Si8 = nullptr;
   int *d_eV = nullptr, *d_eS = nullptr;
   float *d_Vhi = nullptr, *d_Vlo = nullptr, *d_Shi = nullptr, *d_Slo = nullptr;
@@ -74,8 +104,13 @@ struct cody_ctx {
 
   int allocate() {
     using cody::ctx_alloc;
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
-      return cody::set_error(2, "hipStreamCreate failed");
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&stream_ci, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_a, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_w, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_ci, hipEventDisableTiming) != hipSuccess)
+      return cody::set_error(2, "hipStreamCreate / hipEventCreate failed");
     const size_t n2 = (size_t)nO * nO, n4 = n2 * n2;
     int r = 0;
 #define ALLOC(ptr, bytes)                                   \
@@ -86,23 +121,32 @@ struct cody_ctx {
     ALLOC(d_kref, sizeof(double) * 32 * 32);
     ALLOC(d_V, sizeof(double) * nGp * nGp);
     ALLOC(d_ST, sizeof(double) * Pc * nGp);
-    ALLOC(d_Wpart, sizeof(double) * nsplit * nGp * Pc);
     ALLOC(d_WT, sizeof(double) * Pc * nGp);
+    ALLOC(d_STt, sizeof(double) * Pc * nGp);
+    ALLOC(d_Wpart, sizeof(double) * cody::kWSplit * Pc * nGp);
     ALLOC(d_Gpart, sizeof(double) * nxs * Pc * Pc);
     ALLOC(d_G, sizeof(double) * Pc * Pc);
     ALLOC(d_energy, sizeof(double) * 8);
+    ALLOC(d_efield, sizeof(double));
+    if (hipMemset(d_efield, 0, sizeof(double)) != hipSuccess) return cody::set_error(2, "hipMemset failed");
     ALLOC(d_state, sizeof(double2) * nS);
     ALLOC(d_tmp, sizeof(double2) * nS);
     ALLOC(d_tmp2, sizeof(double2) * nS);
     ALLOC(d_rho1, sizeof(double2) * n2);
     ALLOC(d_rho2, sizeof(double2) * n4);
+    // entries of rho2 with a repeated index pair are zero and never written by the kernels
+    if (hipMemset(d_rho2, 0, sizeof(double2) * n4) != hipSuccess)
+      return cody::set_error(2, "hipMemset of rho2 failed");
+    ALLOC(d_r2can, sizeof(double2) * std::max<size_t>(1, (size_t)(nO * (nO - 1) / 2) * (nO * (nO - 1) / 2)));
     ALLOC(d_rho2p, sizeof(double2) * n4);
     ALLOC(d_reginv, sizeof(double2) * n2);
     ALLOC(d_h1, sizeof(double2) * n2);
-    ALLOC(d_h2, sizeof(double2) * n4);
+    ALLOC(d_hphi, sizeof(double2) * nG * nO);
+    ALLOC(d_dCraw, sizeof(double2) * nC);
     ALLOC(d_inp, sizeof(double2) * n2);
     ALLOC(d_draw, sizeof(double2) * nG * nO);
-    ALLOC(d_part, sizeof(double2) * nPart * 2 * n2);
+    ALLOC(d_part, sizeof(double2) * nPart * n2);
+    ALLOC(d_part1, sizeof(double2) * ((nG + 255) / 256) * n2);
     // CI tables in one blob: ptr arrays then entry arrays
     const size_t ns = ci.s_ptr.size(), nd = ci.d_ptr.size(), n1 = ci.r1_ptr.size(), nr2 = ci.r2_ptr.size();
     const size_t es = ci.singles.size(), ed = ci.doubles.size(), e1 = ci.r1.size(), e2 = ci.r2.size();
@@ -144,17 +188,34 @@ struct cody_ctx {
     return 0;
   }
 
+  // captured right-hand sides, replayed by the device-pointer entry points and the propagators;
+  // dropped whenever a setter changes what the launches bake in (kernels, sizes, parameters)
+  std::vector<RhsGraph> graphs;
+  bool graphs_on = true;
+  bool capturing = false;  // enqueue_rhs is being captured (E(t) is then set per replay)
+  void drop_graphs() {
+    for (auto& g : graphs) hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+
   void release() {
-    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_Wpart, d_WT,
-                    d_Gpart, d_G,  d_energy, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
-                    d_rho2p, d_reginv, d_h1, d_h2,   d_inp,   d_draw,  d_part,  ci_blob};
+    drop_graphs();
+    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart,
+                    d_Gpart, d_G,  d_energy, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
+                    d_rho2p, d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_inp, d_draw, d_part, d_part1, ci_blob};
     for (void* p : ptrs)
       if (p) hipFree(p);
     prop.release();
-    void* lp[] = {d_Vi8, d_Si8, d_eV, d_eS, d_Vhi, d_Vlo, d_Shi, d_Slo, d_Vbf, d_Sbf};
+    void* lp[] = {d_Vt, d_Vi8, d_Si8, d_eV, d_eS, d_Vhi, d_Vlo, d_Shi, d_Slo, d_Vbf, d_Sbf};
     for (void* p : lp)
       if (p) hipFree(p);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_ci) hipEventDestroy(ev_ci);
+    if (ev_a) hipEventDestroy(ev_a);
+    if (ev_w) hipEventDestroy(ev_w);
+    if (stream_ci) hipStreamDestroy(stream_ci);
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
+    stream_ci = nullptr;
   }
 };

@@ -16,6 +16,7 @@
 namespace cody {
 int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode,
                 hipStream_t st, double* energy, RhsTrace* tr);
+int run_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode, hipStream_t st);
 int check_ready(const cody_ctx* c);
 }  // namespace cody
 
@@ -415,13 +416,13 @@ int rk4_impl(cody_ctx* c, double t0, double t1, int mode) {
   hipStream_t st = c->stream;
   const double dt = t1 - t0, dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
   int r;
-  if ((r = enqueue_rhs(c, c->d_state, b->rk_d, t0, mode, st, nullptr, nullptr))) return r;
+  if ((r = run_rhs(c, c->d_state, b->rk_d, t0, mode, st))) return r;
   hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt6, dt2, 1, b->rk_inc, b->rk_tmp, n);
-  if ((r = enqueue_rhs(c, b->rk_tmp, b->rk_d, t0 + dt2, mode, st, nullptr, nullptr))) return r;
+  if ((r = run_rhs(c, b->rk_tmp, b->rk_d, t0 + dt2, mode, st))) return r;
   hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt3, dt2, 0, b->rk_inc, b->rk_tmp, n);
-  if ((r = enqueue_rhs(c, b->rk_tmp, b->rk_d, t0 + dt2, mode, st, nullptr, nullptr))) return r;
+  if ((r = run_rhs(c, b->rk_tmp, b->rk_d, t0 + dt2, mode, st))) return r;
   hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt3, dt, 0, b->rk_inc, b->rk_tmp, n);
-  if ((r = enqueue_rhs(c, b->rk_tmp, b->rk_d, t0 + dt, mode, st, nullptr, nullptr))) return r;
+  if ((r = run_rhs(c, b->rk_tmp, b->rk_d, t0 + dt, mode, st))) return r;
   hipLaunchKernelGGL(k_rk_final, g, dim3(256), 0, st, c->d_state, b->rk_inc, b->rk_d, dt6, n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));

@@ -2,11 +2,10 @@
 //
 // Data layout in HBM (one context, one fermionic body type, nO orbitals, nG grid points):
 //   state      complex128 [C(nC) ; phi(nG, nO) column-major]   (the reference's Method_state)
-//   V          FP64 nGp x nGp row-major, V[i][j] = K(|x_i - x_j|), zero padded to nGp = 64*ceil(nG/64)
+//   V          FP64 nGp x nGp row-major, V[i][j] = K(|x_i - x_j|), zero padded to nGp = 256*ceil(nG/256)
 //   ST         FP64 Pc x nGp: row 2p / 2p+1 = Re / Im of the pair source conj(phi_l) phi_k w,
 //              p enumerating l <= k; Pc = 16*ceil(2P/16), unused rows zero
-//   Wpart      FP64 nsplit x nGp x Pc: split-K partial products V . S
-//   WT         FP64 Pc x nGp: mean-field potentials W_p(x) (Re/Im rows), summed over splits
+//   WT         FP64 Pc x nGp: mean-field potentials W_p(x) (Re/Im rows), from meanfield_gemm.hip
 // Complex scalars are double2 (x = re, y = im).
 #include <hip/hip_runtime.h>
 
@@ -92,68 +91,31 @@ __global__ __launch_bounds__(256) void k_pair_src(const double2* __restrict__ or
   for (; c < Pc; ++c) ST[(size_t)c * nGp + x] = 0.0;
 }
 
-// ------------------------------------------------------------------ mean-field contraction (FP64 MFMA)
-// Wpart[s][r][c] = sum_{k in split s} V[r][k] S[k][c], S[k][c] = ST[c][k].
-// v_mfma_f64_16x16x4f64: lane l holds A(row l%16, k l/16), B(k l/16, col l%16) and
-// D(row 4i + l/16, col l%16), i = 0..3.  The contraction index is permuted inside each
-// 16-wide k block (MFMA step s of lane group q uses k = kb + 4q + s) so that every lane
-// fetches its four A and four B values with one 32-byte load each.
-template <int NT>
-__global__ __launch_bounds__(256) void k_meanfield_f64(const double* __restrict__ V,
-                                                       const double* __restrict__ ST,
-                                                       double* __restrict__ Wpart, int nGp, int Pc,
-                                                       int kchunk) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = lane >> 4, rr = lane & 15;
-  const int row = blockIdx.x * 64 + wave * 16 + rr;
-  const int split = blockIdx.y;
-  const int k0 = split * kchunk;
-  typedef double double4_ __attribute__((ext_vector_type(4)));
-  double4_ acc[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = (double4_){0.0, 0.0, 0.0, 0.0};
-  const double* aptr = V + (size_t)row * nGp + k0 + 4 * q;
-  const double* bptr[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) bptr[t] = ST + (size_t)(t * 16 + rr) * nGp + k0 + 4 * q;
-  for (int kb = 0; kb < kchunk; kb += 16) {
-    const double4_ a = *reinterpret_cast<const double4_*>(aptr + kb);
-    double4_ b[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const double4_*>(bptr[t] + kb);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-#pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[t][s], acc[t], 0, 0, 0);
+// The same S^T fragment-tiled for the FP64 kernels, one output element per thread (coalesced):
+// STt[(t nK + K) 256 + 128 h + 2 l + e] = S^T[c = 16 t + l % 16][x = 16 K + 4 (l / 16) + 2 h + e].
+__global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ orbs,
+                                                    const double* __restrict__ w, int nG, int nGp, int nO,
+                                                    int Pc, double* __restrict__ STt) {
+  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nK = nGp >> 4;
+  if (o >= (size_t)Pc * nGp) return;
+  const int e = o & 1, h = (o >> 7) & 1, l = (o >> 1) & 63;
+  const size_t blk = o >> 8;
+  const int t = (int)(blk / nK), K = (int)(blk % nK);
+  const int c = 16 * t + (l & 15), x = 16 * K + 4 * (l >> 4) + 2 * h + e;
+  double v = 0.0;
+  const int p = c >> 1;
+  if (x < nG && p < nO * (nO + 1) / 2) {
+    int a = 0, r = p;
+    while (r >= nO - a) {
+      r -= nO - a;
+      ++a;
     }
+    const double2 s = cscale(cmul(cconj(orbs[(size_t)a * nG + x]), orbs[(size_t)(a + r) * nG + x]), w[x]);
+    v = (c & 1) ? s.y : s.x;
   }
-  // FP64 16x16 result layout: accumulator element i of lane l is row 4i + l/16, column l%16
-  double* out = Wpart + (size_t)split * nGp * Pc;
-  const int orow = blockIdx.x * 64 + wave * 16 + q;
-#pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) out[(size_t)(orow + 4 * i) * Pc + t * 16 + rr] = acc[t][i];
+  STt[o] = v;
 }
-
-// Sum the split-K partials in fixed order and transpose to WT[c][x].
-__global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, int nsplit, int nGp,
-                                              int Pc, double* __restrict__ WT) {
-  __shared__ double tile[32][kMaxPc + 1];
-  const int r0 = blockIdx.x * 32;
-  for (int e = threadIdx.x; e < 32 * Pc; e += blockDim.x) {
-    const int r = e / Pc, c = e % Pc;
-    double s = 0.0;
-    for (int sp = 0; sp < nsplit; ++sp) s += Wpart[((size_t)sp * nGp + r0 + r) * Pc + c];
-    tile[r][c] = s;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < 32 * Pc; e += blockDim.x) {
-    const int c = e / 32, r = e % 32;
-    WT[(size_t)c * nGp + r0 + r] = tile[r][c];
-  }
-}
-
 // ------------------------------------------------------------------ one-body operator at a point
 // FEDVR weak-form kinetic energy -(1/2m)(1/w_x) sum_e sum_b K(a,b)/h f_b
 // (src/Utils/M_Utils_DerivativeFedvr.f90:121-152): the point with global index g (numbered with
@@ -193,119 +155,160 @@ __device__ __forceinline__ double2 kinetic_at(const double2* __restrict__ f, int
   return cscale(acc, kp.invh * rw * kp.fac);
 }
 
-// ------------------------------------------------------------------ orbital right-hand side
-// One workgroup = 64 grid points (one per lane) x 4 orbitals j (one per wave).
-//   hphi_j   = (T + V_ext + E(t) x) phi_j                        (ApplySingleBodyOp :976-1034)
-//   corr_j   = sum_{k1,(l2,k2)} W_{l2k2} phi_k1 rho2'(k1,k2,j,l2) (ApplyCorrelationOp :762-838)
-//   draw_j   = [hphi_j] + [corr_j]  (mode)
-// and per-workgroup partial sums of h1(i,j) = <phi_i|hphi_j>_w and inp(i,j) = <phi_i|draw_j>_w.
-// The rho2' coefficients depend only on the wave's orbital j and the loop indices, so their
-// loads are uniform across the wave.
+// ------------------------------------------------------------------ one-body part
+// One workgroup = 256 grid points x one orbital j (blockIdx.y):
+//   hphi_j = (T + V_ext + E(t) x) phi_j                   (ApplySingleBodyOp :976-1034)
+// and the workgroup's partial sums of h1(i,j) = <phi_i|hphi_j>_w, part1[block][i + j NO].
 template <int NO>
-__global__ __launch_bounds__(256) void k_orbital(
-    const double2* __restrict__ orbs, const double* __restrict__ w,
-    const double* __restrict__ xg, const double* __restrict__ vext, KinParams kp, double efield,
-    const double* __restrict__ WT, const double2* __restrict__ rho2p, int nG, int nGp, int mode,
-    double2* __restrict__ draw, double2* __restrict__ part) {
-  constexpr int n2 = NO * NO;
-  const int lane = threadIdx.x & 63;
-  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
+__global__ __launch_bounds__(256) void k_onebody(const double2* __restrict__ orbs, const double* __restrict__ w,
+                                                 const double* __restrict__ xg, const double* __restrict__ vext,
+                                                 KinParams kp, const double* __restrict__ efield, int nG,
+                                                 double2* __restrict__ hphi, double2* __restrict__ part1) {
+  __shared__ double2 sRed[4][NO];
+  const int j = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const bool in = x < nG;
+  const int xc = in ? x : nG - 1;
+  double2 h = cmk(0, 0);
+  const double wx = in ? w[xc] : 0.0;
+  if (in) {
+    h = kinetic_at(orbs + (size_t)j * nG, xc, kp, 1.0 / wx);
+    const double v = vext[xc] + *efield * xg[xc];
+    const double2 pj = orbs[(size_t)j * nG + xc];
+    h.x = fma(v, pj.x, h.x);
+    h.y = fma(v, pj.y, h.y);
+    hphi[(size_t)j * nG + x] = h;
+  }
+  const double2 hw = cscale(h, wx);
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
+    double2 acc = cmk(0, 0);
+    if (in) cfmac(acc, orbs[(size_t)i * nG + xc], hw);
+    acc.x = wave_sum(acc.x);
+    acc.y = wave_sum(acc.y);
+    if (lane == 0) sRed[wave][i] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < NO) {
+    const int i = threadIdx.x;
+    const double2 acc = cadd(cadd(sRed[0][i], sRed[1][i]), cadd(sRed[2][i], sRed[3][i]));
+    part1[(size_t)blockIdx.x * NO * NO + i + j * NO] = acc;
+  }
+}
+
+// ------------------------------------------------------------------ orbital right-hand side
+// One workgroup = 64 grid points (one per lane) x one orbital j (blockIdx.y); its 4 waves split
+// the unordered pairs (l <= k, pair p taken by wave p % 4):
+//   corr_j   = sum_{k1,(l2,k2)} W_{l2k2} phi_k1 rho2'(k1,k2,j,l2) (ApplyCorrelationOp :762-838)
+//   draw_j   = hphi_j + corr_j (full) or corr_j (nonlinear part)
+// and per-workgroup partial sums of inp(i,j) = <phi_i|draw_j>_w, part[block][i + j NO] (the NO
+// sums are split over the waves).  The slice rho2'(:,:,j,:) (NO^3 values) is staged in LDS and
+// read as broadcasts; each wave loads all of its W values before using them, and the four
+// correlation shares are added in fixed order.
+template <int NO>
+__global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orbs, const double* __restrict__ w,
+                                                 const double2* __restrict__ hphi, const double* __restrict__ WT,
+                                                 const double2* __restrict__ rho2p, int nG, int nGp, int mode,
+                                                 double2* __restrict__ draw, double2* __restrict__ part) {
+  constexpr int n2 = NO * NO, n3 = n2 * NO, NP = NO * (NO + 1) / 2, PW = (NP + 3) / 4;
+  __shared__ double2 sR[n3];
+  __shared__ double2 sCorr[4][64];
+  __shared__ double2 sRed[NO];
+  const int j = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x = blockIdx.x * 64 + lane;
-  if (j >= NO) return;
+  // sR[k1 + NO (k + NO l)] = rho2'(k1, k, j, l)
+  for (int e = threadIdx.x; e < n3; e += 256) {
+    const int k1k = e % n2, l = e / n2;
+    sR[e] = rho2p[k1k + n2 * j + n2 * NO * l];
+  }
   const bool in = x < nG;
   const int xc = in ? x : nG - 1;
   double2 phi[NO];
 #pragma unroll
   for (int k = 0; k < NO; ++k) phi[k] = in ? orbs[(size_t)k * nG + xc] : cmk(0, 0);
   const double wx = in ? w[xc] : 0.0;
-  double2 hphi = cmk(0, 0), corr = cmk(0, 0);
-  if (in) {
-    hphi = kinetic_at(orbs + (size_t)j * nG, xc, kp, 1.0 / wx);
-    const double v = vext[xc] + efield * xg[xc];
-    double2 pj = orbs[(size_t)j * nG + xc];
-    hphi.x = fma(v, pj.x, hphi.x);
-    hphi.y = fma(v, pj.y, hphi.y);
+  double2 Wv[PW];
+#pragma unroll
+  for (int m = 0; m < PW; ++m) {
+    const int p = wave + 4 * m;
+    Wv[m] = (in && p < NP) ? cmk(WT[(size_t)(2 * p) * nGp + x], WT[(size_t)(2 * p + 1) * nGp + x]) : cmk(0, 0);
   }
-  if (mode != kModeLin) {
-    const double2* rj = rho2p + n2 * j;
-    int p = 0;
+  const double2 hj = (in && mode == kModeFull) ? hphi[(size_t)j * nG + x] : cmk(0, 0);
+  __syncthreads();  // sR staged
+  double2 corr = cmk(0, 0);
 #pragma unroll
-    for (int l = 0; l < NO; ++l) {
+  for (int m = 0; m < PW; ++m) {
+    const int p = wave + 4 * m;  // uniform over the wave
+    if (p >= NP) break;
+    int l = 0, r = p;
+    while (r >= NO - l) {
+      r -= NO - l;
+      ++l;
+    }
+    const int k = l + r;
+    // A = sum_k1 phi_k1 rho2'(k1,k,j,l); B = sum_k1 phi_k1 rho2'(k1,l,j,k)
+    const double2* ra = sR + NO * (k + NO * l);
+    double2 A = cmk(0, 0);
 #pragma unroll
-      for (int k = l; k < NO; ++k, ++p) {
-        const double2 W = in ? cmk(WT[(size_t)(2 * p) * nGp + x], WT[(size_t)(2 * p + 1) * nGp + x])
-                             : cmk(0, 0);
-        // A = sum_k1 phi_k1 rho2'(k1,k,j,l); B = sum_k1 phi_k1 rho2'(k1,l,j,k)
-        const double2* ra = rj + NO * k + n2 * NO * l;
-        double2 A = cmk(0, 0);
+    for (int k1 = 0; k1 < NO; ++k1) cfma(A, phi[k1], ra[k1]);
+    cfma(corr, Wv[m], A);
+    if (k != l) {
+      const double2* rb = sR + NO * (l + NO * k);
+      double2 B = cmk(0, 0);
 #pragma unroll
-        for (int k1 = 0; k1 < NO; ++k1) cfma(A, phi[k1], ra[k1]);
-        cfma(corr, W, A);
-        if (k != l) {
-          const double2* rb = rj + NO * l + n2 * NO * k;
-          double2 B = cmk(0, 0);
-#pragma unroll
-          for (int k1 = 0; k1 < NO; ++k1) cfma(B, phi[k1], rb[k1]);
-          cfma(corr, cconj(W), B);
-        }
-      }
+      for (int k1 = 0; k1 < NO; ++k1) cfma(B, phi[k1], rb[k1]);
+      cfma(corr, cconj(Wv[m]), B);
     }
   }
-  double2 d = cmk(0, 0);
-  if (mode == kModeFull) d = cadd(hphi, corr);
-  else if (mode == kModeNonLin) d = corr;
-  else d = hphi;
-  if (in) draw[(size_t)j * nG + x] = d;
-  // partials: part[block][0][i + j nO] = h1, part[block][1][i + j nO] = inp
-  double2* pb = part + (size_t)blockIdx.x * 2 * n2;
-  const double2 hw = cscale(hphi, wx), dw = cscale(d, wx);
+  sCorr[wave][lane] = corr;
+  __syncthreads();
+  corr = cadd(cadd(sCorr[0][lane], sCorr[1][lane]), cadd(sCorr[2][lane], sCorr[3][lane]));
+  const double2 d = cadd(hj, corr);
+  if (in && wave == 0) draw[(size_t)j * nG + x] = d;
+  // inp(i, j) partial sums, sum i formed by wave i % 4
+  const double2 dw = cscale(d, wx);
 #pragma unroll
   for (int i = 0; i < NO; ++i) {
-    double2 h = cmk(0, 0), q = cmk(0, 0);
-    cfmac(h, phi[i], hw);
-    cfmac(q, phi[i], dw);
-    h.x = wave_sum(h.x);
-    h.y = wave_sum(h.y);
-    q.x = wave_sum(q.x);
-    q.y = wave_sum(q.y);
-    if (lane == 0) {
-      pb[i + j * NO] = h;
-      pb[n2 + i + j * NO] = q;
-    }
+    if ((i & 3) != wave) continue;
+    double2 acc = cmk(0, 0);
+    cfmac(acc, phi[i], dw);
+    acc.x = wave_sum(acc.x);
+    acc.y = wave_sum(acc.y);
+    if (lane == 0) sRed[i] = acc;
   }
-}
-
-// ------------------------------------------------------------------ h2 Gram matrix
-// Gpart[s][c1][c2] = sum_{x in chunk s} ST[c1][x] WT[c2][x]  (16 x 16 tile per workgroup).
-__global__ __launch_bounds__(256) void k_h2_gram(const double* __restrict__ ST,
-                                                 const double* __restrict__ WT, int nGp, int Pc,
-                                                 int xchunk, double* __restrict__ Gpart) {
-  __shared__ double sa[16][65], sb[16][65];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int c1 = blockIdx.x * 16 + ty, c2 = blockIdx.y * 16 + tx;
-  const int x0 = blockIdx.z * xchunk;
-  double acc = 0.0;
-  for (int xb = 0; xb < xchunk; xb += 64) {
-    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-      const int r = e / 64, cx = e % 64;
-      sa[r][cx] = ST[(size_t)(blockIdx.x * 16 + r) * nGp + x0 + xb + cx];
-      sb[r][cx] = WT[(size_t)(blockIdx.y * 16 + r) * nGp + x0 + xb + cx];
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int cx = 0; cx < 64; ++cx) acc = fma(sa[ty][cx], sb[tx][cx], acc);
-    __syncthreads();
-  }
-  Gpart[((size_t)blockIdx.z * Pc + c1) * Pc + c2] = acc;
+  __syncthreads();
+  if (threadIdx.x < NO) part[(size_t)blockIdx.x * n2 + threadIdx.x + j * NO] = sRed[threadIdx.x];
 }
 
 // ------------------------------------------------------------------ projection + scaling
 // dphi_j = -i (draw_j - sum_i phi_i inp(i,j))  (Grid_ProjectOnSubspace, src/Grid/S_Grid.f90:170-203)
+// With dC non-null the last workgroup applies the CI projection and scaling
+// dC = -i (dCraw - <C|dCraw> C)  (src/Coeffs/S_Coeffs.f90:105-111).
 template <int NO>
 __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orbs,
                                                  const double2* __restrict__ draw,
                                                  const double2* __restrict__ inp, int nG,
-                                                 int project, double2* __restrict__ dorbs) {
+                                                 int project, double2* __restrict__ dorbs,
+                                                 const double2* __restrict__ C,
+                                                 const double2* __restrict__ dCraw, int nC,
+                                                 double2* __restrict__ dC) {
+  if (dC && blockIdx.x == gridDim.x - 1) {
+    __shared__ double red[2][4];
+    double2 ov = cmk(0, 0);
+    for (int i = threadIdx.x; i < nC; i += blockDim.x) cfmac(ov, C[i], dCraw[i]);
+    ov.x = wave_sum(ov.x);
+    ov.y = wave_sum(ov.y);
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x >> 6] = ov.x;
+      red[1][threadIdx.x >> 6] = ov.y;
+    }
+    __syncthreads();
+    ov = cmk((red[0][0] + red[0][1]) + (red[0][2] + red[0][3]), (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+    for (int i = threadIdx.x; i < nC; i += blockDim.x) dC[i] = mul_minus_i(csub(dCraw[i], cmul(ov, C[i])));
+    return;
+  }
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nG) return;
   double2 phi[NO];
@@ -322,6 +325,13 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
     }
     dorbs[(size_t)j * nG + x] = mul_minus_i(d);
   }
+}
+
+// ------------------------------------------------------------------ scalar store (graph-capturable)
+__global__ void k_set_scalar(double* __restrict__ p, double v) { *p = v; }
+
+void launch_set_scalar(double* p, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, st, p, v);
 }
 
 // ------------------------------------------------------------------ zero fill (graph-capturable)
@@ -358,8 +368,13 @@ void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a
     default: break;                 \
   }
 
+// STt (fragment-tiled, always) and, when ST is non-null, S^T row-major.
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
-                     double* ST, hipStream_t st) {
+                     double* ST, double* STt, hipStream_t st) {
+  const size_t n = (size_t)Pc * nGp;
+  hipLaunchKernelGGL(k_pair_src_t, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, orbs, w, nG, nGp, nO,
+                     Pc, STt);
+  if (!ST) return;
 #define CALL_PS(NO)                                                                            \
   hipLaunchKernelGGL(k_pair_src<NO>, dim3((nGp + 255) / 256), dim3(256), 0, st, orbs, w, nG, nGp, \
                      Pc, ST)
@@ -367,59 +382,34 @@ void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int 
 #undef CALL_PS
 }
 
-template <int NT>
-static void launch_mf(const double* V, const double* ST, double* Wpart, int nGp, int Pc, int nsplit,
-                      hipStream_t st) {
-  dim3 grid(nGp / 64, nsplit);
-  hipLaunchKernelGGL(k_meanfield_f64<NT>, grid, dim3(256), 0, st, V, ST, Wpart, nGp, Pc,
-                     nGp / nsplit);
-}
-
-void launch_meanfield_f64(const double* V, const double* ST, double* Wpart, int nGp, int Pc,
-                          int nsplit, hipStream_t st) {
-  switch (Pc / 16) {
-    case 1: launch_mf<1>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 2: launch_mf<2>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 3: launch_mf<3>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 4: launch_mf<4>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 5: launch_mf<5>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 6: launch_mf<6>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 7: launch_mf<7>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 8: launch_mf<8>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 9: launch_mf<9>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    case 10: launch_mf<10>(V, ST, Wpart, nGp, Pc, nsplit, st); break;
-    default: break;
-  }
-}
-
-void launch_wsum(const double* Wpart, int nsplit, int nGp, int Pc, double* WT, hipStream_t st) {
-  hipLaunchKernelGGL(k_wsum, dim3(nGp / 32), dim3(256), 0, st, Wpart, nsplit, nGp, Pc, WT);
-}
-
-void launch_orbital(const double2* orbs, const double* w, const double* x, const double* vext,
-                    const KinLaunch& kl, double efield, const double* WT, const double2* rho2p,
-                    int nG, int nGp, int nO, int mode, double2* draw, double2* part,
+void launch_onebody(const double2* orbs, const double* w, const double* x, const double* vext,
+                    const KinLaunch& kl, const double* efield, int nG, int nO, double2* hphi, double2* part1,
                     hipStream_t st) {
   KinParams kp{kl.kref, kl.nE, kl.nLoc, kl.xminExcl, nG, kl.invh, kl.fac};
-  dim3 grid(nGp / 64, (nO + 3) / 4);
-#define CALL_OR(NO)                                                                          \
-  hipLaunchKernelGGL(k_orbital<NO>, grid, dim3(256), 0, st, orbs, w, x, vext, kp, efield, WT, \
-                     rho2p, nG, nGp, mode, draw, part)
+  dim3 grid((nG + 255) / 256, nO);
+#define CALL_OB(NO) \
+  hipLaunchKernelGGL(k_onebody<NO>, grid, dim3(256), 0, st, orbs, w, x, vext, kp, efield, nG, hphi, part1)
+  CODY_NO_SWITCH(nO, CALL_OB)
+#undef CALL_OB
+}
+
+void launch_orbital(const double2* orbs, const double* w, const double2* hphi, const double* WT,
+                    const double2* rho2p, int nG, int nGp, int nO, int mode, double2* draw, double2* part,
+                    hipStream_t st) {
+  dim3 grid(nGp / 64, nO);
+#define CALL_OR(NO) \
+  hipLaunchKernelGGL(k_orbital<NO>, grid, dim3(256), 0, st, orbs, w, hphi, WT, rho2p, nG, nGp, mode, draw, part)
   CODY_NO_SWITCH(nO, CALL_OR)
 #undef CALL_OR
 }
 
-void launch_h2_gram(const double* ST, const double* WT, int nGp, int Pc, int nxs, double* Gpart,
+void launch_project(const double2* orbs, const double2* draw, const double2* inp, int nG, int nO, int project,
+                    double2* dorbs, const double2* C, const double2* dCraw, int nC, double2* dC,
                     hipStream_t st) {
-  dim3 grid(Pc / 16, Pc / 16, nxs);
-  hipLaunchKernelGGL(k_h2_gram, grid, dim3(256), 0, st, ST, WT, nGp, Pc, nGp / nxs, Gpart);
-}
-
-void launch_project(const double2* orbs, const double2* draw, const double2* inp, int nG, int nO,
-                    int project, double2* dorbs, hipStream_t st) {
-#define CALL_PR(NO)                                                                                 \
-  hipLaunchKernelGGL(k_project<NO>, dim3((nG + 255) / 256), dim3(256), 0, st, orbs, draw, inp, nG, \
-                     project, dorbs)
+  const int nb = (nG + 255) / 256 + (dC ? 1 : 0);
+#define CALL_PR(NO)                                                                                         \
+  hipLaunchKernelGGL(k_project<NO>, dim3(nb), dim3(256), 0, st, orbs, draw, inp, nG, project, dorbs, C, dCraw, \
+                     nC, dC)
   CODY_NO_SWITCH(nO, CALL_PR)
 #undef CALL_PR
 }

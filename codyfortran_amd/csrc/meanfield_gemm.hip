@@ -1,0 +1,542 @@
+// Mean-field contraction W^T = (V S)^T on gfx950 MFMA, all precisions (A4, the dominant kernel).
+//
+// Work decomposition: one workgroup (4 waves) owns 16 rows x all P_c columns of W; its four waves
+// take one quarter of the contraction index each, so a 4096-point grid runs 256 workgroups /
+// 1024 waves with every V element read exactly once.  Each wave keeps two or three k blocks of
+// operands in registers (later blocks are in flight while the MFMAs of the current one issue).
+// The four quarter sums are added in LDS in fixed order (bit-reproducible) and written
+// transposed, W^T[c][x], which is the layout the per-grid-point kernels read coalesced.
+//
+// Operand layouts (k contiguous in both): V[row][k] (V is symmetric, row-major), S^T[col][k].
+// Inside each k-block the contraction index is permuted so that every lane fetches the values
+// of its MFMA steps with one wide load: lane group q = l/16 takes k = kb + w*q + s at step s,
+// w = values per lane per MFMA (4 for FP64, 64/4 = 16 for int8, ...).
+//
+// Result layouts of the 16x16 MFMAs: FP64 (v_mfma_f64_16x16x4f64): element i of lane l is row
+// 4i + l/16; 32-bit results (f32/bf16/i8): row 4(l/16) + i; column l%16 in both.
+#include <hip/hip_runtime.h>
+
+#include "mctdhx_kernels.hpp"
+
+namespace cody {
+
+namespace {
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef int i4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+// Sum the WAVES waves' 16 x (16 NT) FP64 tiles in LDS (pairwise halving, fixed order, so the
+// result is bit-reproducible) and store W^T[c][row0 + r].
+// vals[t][i] holds row rowOf(i) of column t*16 + l%16 of this wave's share of the contraction.
+template <int NT, bool F64LAYOUT, int WAVES = 4>
+__device__ __forceinline__ void reduce_store(double (&vals)[NT][4], double* __restrict__ WT, int ldc,
+                                             int row0, double* red /* [WAVES/2][16][NT*16+1] */) {
+  constexpr int LD = NT * 16 + 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  auto rowOf = [&](int i) { return F64LAYOUT ? 4 * i + q : 4 * q + i; };
+#pragma unroll
+  for (int half = WAVES / 2; half >= 1; half >>= 1) {
+    if (wave >= half && wave < 2 * half) {
+      double* slot = red + (wave - half) * 16 * LD;
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) slot[rowOf(i) * LD + t * 16 + rr] = vals[t][i];
+    }
+    __syncthreads();
+    if (wave < half) {
+      const double* slot = red + wave * 16 * LD;
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) vals[t][i] += slot[rowOf(i) * LD + t * 16 + rr];
+    }
+    __syncthreads();
+  }
+  if (wave == 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) red[rowOf(i) * LD + t * 16 + rr] = vals[t][i];
+  }
+  __syncthreads();
+  // coalesced transposed store: 16 consecutive rows of one column per 16 threads
+  for (int e = threadIdx.x; e < 16 * NT * 16; e += blockDim.x) {
+    const int c = e >> 4, r = e & 15;
+    WT[(size_t)c * ldc + row0 + r] = red[r * LD + c];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------- FP64
+// Gram split-K partials Gpart[y][c2][c1] = sum_{x in range y} S^T[c1][x] W^T[c2][x] (G = S^T W,
+// used by h2): A = S^T fragment-tiled (as k_pair_src_t writes it), B = W^T row-major; one
+// workgroup per 16 rows of S^T and k range blockIdx.y, one quarter of it per wave.
+template <int NT>
+__global__ __launch_bounds__(256) void k_gram_f64(const double* __restrict__ At, int nK,
+                                                  const double* __restrict__ B, int ldb,
+                                                  double* __restrict__ Cout, int ldc, int kq,
+                                                  size_t cSplitStride) {
+  __shared__ double red[2 * 16 * (NT * 16 + 1)];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  const int row0 = blockIdx.x * 16;
+  const int kw = (blockIdx.y * 4 + wave) * kq;
+  const double* ap = At + ((size_t)blockIdx.x * nK + (kw >> 4)) * 256 + 2 * lane;
+  const double* bp = B + (size_t)rr * ldb + kw + 4 * q;
+  const size_t bstep = (size_t)16 * ldb;
+  d4_t acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
+  // NBUF register stages rotate through the k blocks: the load issued at step j fills the stage
+  // consumed NBUF - 1 steps later, so each MFMA batch only waits for loads that are that old.
+  // Loads past the last block are clamped onto it (redundant, never consumed), so every load is
+  // unconditional and the compiler's wait counts stay exact.
+  constexpr int NBUF = NT <= 6 ? 3 : 2;
+  const int nb = kq >> 4;
+  d4_t a[NBUF];
+  d4_t b[NBUF][NT];
+  auto load = [&](int u, int blk) {
+    const int kb = min(blk, nb - 1) << 4;
+    const d2_t lo = *reinterpret_cast<const d2_t*>(ap + 16 * kb);
+    const d2_t hi = *reinterpret_cast<const d2_t*>(ap + 16 * kb + 128);
+    a[u] = (d4_t){lo[0], lo[1], hi[0], hi[1]};
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b[u][t] = *reinterpret_cast<const d4_t*>(bp + t * bstep + kb);
+  };
+  auto mma = [&](int u) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][s], b[u][t][s], acc[t], 0, 0, 0);
+  };
+#pragma unroll
+  for (int j = 0; j < NBUF - 1; ++j) load(j, j);
+  int i = 0;
+  for (; i + NBUF <= nb; i += NBUF) {
+#pragma unroll
+    for (int j = 0; j < NBUF; ++j) {
+      load((j + NBUF - 1) % NBUF, i + j + NBUF - 1);
+      mma(j);
+    }
+  }
+  // the remaining nb - i < NBUF blocks are already in stages 0 .. nb - i - 1
+#pragma unroll
+  for (int j = 0; j < NBUF - 1; ++j)
+    if (j < nb - i) mma(j);
+  double vals[NT][4];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) vals[t][i] = acc[t][i];
+  reduce_store<NT, true>(vals, Cout + blockIdx.y * cSplitStride, ldc, row0, red);
+}
+
+// W^T = (V S)^T with both operands fragment-tiled: the 16 x 16 block (R, K) of V is 2 KB
+// contiguous at Vt + (R nK + K) 256 (k_tile_f64), as two 1 KB halves [h][lane][2] holding
+// V[16R + l%16][16K + 4(l/16) + 2h + e]; S^T is tiled the same way with R the column tile
+// (k_pair_src).
+//
+// Decomposition: a workgroup owns 64 rows (4 row tiles) and one of kWSplit contiguous k ranges;
+// its waves are (row tile r, k sub-range h).  Per 16-wide k block the S^T fragments of both k
+// sub-ranges (KH x NT x 2 KB) are fetched once by the whole workgroup, 1 KB per wave and chunk,
+// and shared through LDS, so S^T crosses L2 once per 64 rows (it is the S^T traffic, not V, that
+// bounds a 16-row decomposition).  Each wave streams its V fragments straight into registers.
+//
+// Pipeline (D register stages, two LDS buffers, one barrier per block): at step j a wave issues
+// the global loads of block j + D - 1 (V fragment and its S^T chunks), writes the S^T chunks of
+// block j + 1 into LDS buffer (j + 1) % 2, reads block j's fragments from buffer j % 2 and issues
+// its MFMAs.  Every global load is issued D - 1 steps ahead of its use: loads complete in issue
+// order, so operands fetched at different distances would make every wait cover the farthest.
+//
+// The k sub-ranges are summed in LDS and each workgroup stores its k-split partial; k_wsum adds
+// the kWSplit partials in fixed order (bit-reproducible).  The partials are combined by a second
+// kernel rather than by the last workgroup of a row group: the device-scope fences that a
+// last-arriver scheme needs write back and invalidate whole L2 caches on this GPU, which cost
+// more than the extra launch.
+template <int NT>
+__global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
+    const double* __restrict__ Vt, const double* __restrict__ STt, int nGp, double* __restrict__ Wpart) {
+  constexpr int KH = NT <= 5 ? 2 : 1, W = 4 * KH, D = 4, LD = NT * 16 + 1;
+  constexpr int NCH = KH * NT * 2, CPW = (NCH + W - 1) / W;  // 1 KB S^T chunks per block, per wave
+  constexpr int BUF = W * CPW * 128;  // doubles per LDS buffer (chunks NCH.. are never read)
+  constexpr int LDS = 2 * BUF > 4 * 16 * LD ? 2 * BUF : 4 * 16 * LD;
+  __shared__ double lds[LDS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  const int r = wave & 3, h = wave >> 2;
+  const int nK = nGp >> 4, nbw = nK / (kWSplit * KH);
+  const int Kb = blockIdx.y * (nK / kWSplit);
+  const double* ab = Vt + ((size_t)(blockIdx.x * 4 + r) * nK + Kb + h * nbw) * 256 + 2 * lane;
+  // this wave's S^T chunks: chunk c = wave + W m = (hh NT + t) 2 + half of every block (chunks
+  // past NCH are clamped onto the last one for the load and not written)
+  auto chunk_src = [&](int m) {
+    const int c = min(wave + W * m, NCH - 1);
+    const int half = c & 1, ht = c >> 1, hh = ht / NT, t = ht % NT;
+    return STt + ((size_t)t * nK + Kb + hh * nbw) * 256 + 128 * half + 2 * lane;
+  };
+  d4_t acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
+  d4_t a[D];
+  d2_t stg[D][CPW];
+  auto gload = [&](int u, int blk) {
+    const size_t o = (size_t)min(blk, nbw - 1) * 256;
+    const d2_t lo = *reinterpret_cast<const d2_t*>(ab + o);
+    const d2_t hi = *reinterpret_cast<const d2_t*>(ab + o + 128);
+    a[u] = (d4_t){lo[0], lo[1], hi[0], hi[1]};
+#pragma unroll
+    for (int m = 0; m < CPW; ++m) stg[u][m] = *reinterpret_cast<const d2_t*>(chunk_src(m) + o);
+  };
+  auto lwrite = [&](int u, int buf) {
+#pragma unroll
+    for (int m = 0; m < CPW; ++m) *reinterpret_cast<d2_t*>(lds + buf * BUF + (wave + W * m) * 128 + 2 * lane) = stg[u][m];
+  };
+  auto step = [&](int u, int buf) {
+    d4_t bf[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double* p = lds + buf * BUF + (h * NT + t) * 256 + 2 * lane;
+      const d2_t lo = *reinterpret_cast<const d2_t*>(p);
+      const d2_t hi = *reinterpret_cast<const d2_t*>(p + 128);
+      bf[t] = (d4_t){lo[0], lo[1], hi[0], hi[1]};
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][s], bf[t][s], acc[t], 0, 0, 0);
+  };
+  // stage invariant at the top of an iteration (i a multiple of D, D even): register stages
+  // 0..D-2 hold blocks i..i+D-2 (stage 0 already written to LDS buffer 0).  Loads past the last
+  // block are clamped onto it and never consumed.
+  static_assert(D == 4, "the steps below are written out for four stages");
+  // (loops holding a barrier are not unrolled by the compiler, and the stage index must be a
+  // compile-time constant for the stages to live in registers)
+#define CODY_W_STEP(J)                                 \
+  {                                                    \
+    gload(((J) + D - 1) % D, i + (J) + D - 1);         \
+    lwrite(((J) + 1) % D, ((J) + 1) & 1);              \
+    step((J), (J) & 1);                                \
+    __syncthreads();                                   \
+  }
+  gload(0, 0);
+  gload(1, 1);
+  gload(2, 2);
+  lwrite(0, 0);
+  __syncthreads();
+  int i = 0;
+  for (; i + D <= nbw; i += D) {
+    CODY_W_STEP(0)
+    CODY_W_STEP(1)
+    CODY_W_STEP(2)
+    CODY_W_STEP(3)
+  }
+  // remaining nbw - i < D blocks (the conditions are uniform over the workgroup)
+  if (i < nbw) CODY_W_STEP(0)
+  if (i + 1 < nbw) CODY_W_STEP(1)
+  if (i + 2 < nbw) CODY_W_STEP(2)
+#undef CODY_W_STEP
+  // sum the k sub-ranges of each row tile into red[r] (element u of lane l: row 4u + l/16)
+  double* slot = lds + r * 16 * LD;
+  if (h == KH - 1) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) slot[(4 * u + q) * LD + t * 16 + rr] = acc[t][u];
+  }
+  __syncthreads();
+  if (KH == 2 && h == 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) slot[(4 * u + q) * LD + t * 16 + rr] += acc[t][u];
+  }
+  __syncthreads();
+  // store this k split's 64 x (16 NT) partial transposed: 64 consecutive rows per column
+  const int row0 = blockIdx.x * 64;
+  const size_t plane = (size_t)NT * 16 * nGp;
+  double* Wp = Wpart + blockIdx.y * plane;
+  for (int e = threadIdx.x; e < 64 * NT * 16; e += blockDim.x) {
+    const int c = e >> 6, rl = e & 63;
+    Wp[(size_t)c * nGp + row0 + rl] = lds[(rl >> 4) * 16 * LD + (rl & 15) * LD + c];
+  }
+}
+
+// W^T = sum of the kWSplit partials, in fixed order (two doubles per thread).
+__global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, size_t plane,
+                                              double* __restrict__ WT) {
+  const size_t i = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= plane) return;
+  d2_t v = *reinterpret_cast<const d2_t*>(Wpart + i);
+#pragma unroll
+  for (int ks = 1; ks < kWSplit; ++ks) v += *reinterpret_cast<const d2_t*>(Wpart + ks * plane + i);
+  *reinterpret_cast<d2_t*>(WT + i) = v;
+}
+
+// ---------------------------------------------------------------------------------------- int8 slices
+// NS slices per operand; all slice products with p + q <= NS - 1 (0-based) that share the scale
+// 2^{-7(p+q+2)} accumulate into one int32 tile (exact: |sum| < NS 2^24 < 2^31 for K <= 4096).
+template <int NT, int NS>
+__global__ __launch_bounds__(256) void k_mf_i8(const int8_t* __restrict__ As, const int* __restrict__ eA,
+                                               const int8_t* __restrict__ Bs, const int* __restrict__ fB,
+                                               double* __restrict__ WT, int nGp, int Pc) {
+  __shared__ double red[2 * 16 * (NT * 16 + 1)];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  const int row0 = blockIdx.x * 16;
+  const int kq = nGp >> 2;
+  const int k0 = wave * kq + 16 * q;
+  const size_t planeB = (size_t)Pc * nGp, bstep = (size_t)16 * nGp;
+  // A (the V slices) is fragment-tiled (k_slice_rows with tiled output): the NS slices of the
+  // 16 x 64 block (R, K) are NS contiguous 1 KB fragments [lane][16 bytes]; this wave's blocks
+  // are K = wave nb ... wave nb + nb - 1.  B addresses are a uniform base plus one lane offset.
+  const int nK64 = nGp >> 6;
+  const int8_t* ap = As + ((size_t)blockIdx.x * nK64 + wave * (kq >> 6)) * NS * 1024 + 16 * lane;
+  const int8_t* bp = Bs + (size_t)rr * nGp + k0;
+  const unsigned offB = (unsigned)(rr * nGp + k0);
+  i4_t acc[NT][NS];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int d = 0; d < NS; ++d) acc[t][d] = (i4_t){0, 0, 0, 0};
+  const int nb = kq >> 6;
+  if constexpr (NT <= 6) {
+    // one register set per column tile: as soon as tile t's MFMAs of block i have issued, tile t
+    // of block i+1 is loaded into the same registers, so each load has NT - 1 tiles of MFMA work
+    // to land; A of block i+1 is loaded at the start of block i.  Loads past the last block are
+    // clamped onto it (unconsumed), so every load is unconditional.
+    i4_t a[NS], b[NT][NS];
+    auto loadA = [&](i4_t (&dst)[NS], int blk) {
+      const int kb = min(blk, nb - 1) << 6;
+#pragma unroll
+      for (int p = 0; p < NS; ++p) dst[p] = *reinterpret_cast<const i4_t*>(ap + (size_t)(kb >> 6) * NS * 1024 + p * 1024);
+    };
+    auto loadB = [&](int t, int blk) {
+      const int kb = min(blk, nb - 1) << 6;
+#pragma unroll
+      for (int p = 0; p < NS; ++p) b[t][p] = *reinterpret_cast<const i4_t*>(Bs + (t * bstep + p * planeB) + (offB + kb));
+    };
+    loadA(a, 0);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) loadB(t, 0);
+    for (int i = 0; i < nb; ++i) {
+      i4_t an[NS];
+      loadA(an, i + 1);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int d = 0; d < NS; ++d)
+#pragma unroll
+          for (int p = 0; p <= d; ++p)
+            acc[t][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], b[t][d - p], acc[t][d], 0, 0, 0);
+        loadB(t, i + 1);
+      }
+#pragma unroll
+      for (int p = 0; p < NS; ++p) a[p] = an[p];
+    }
+  } else {
+    // wide column sets: A one block ahead, B one column tile ahead
+    i4_t a[NS];
+#pragma unroll
+    for (int p = 0; p < NS; ++p) a[p] = *reinterpret_cast<const i4_t*>(ap + p * 1024);
+    auto block = [&](int kb) {
+      i4_t b[NS];
+#pragma unroll
+      for (int p = 0; p < NS; ++p) b[p] = *reinterpret_cast<const i4_t*>(bp + p * planeB + kb);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        i4_t bn[NS];
+        const int tn = min(t + 1, NT - 1);
+#pragma unroll
+        for (int p = 0; p < NS; ++p) bn[p] = *reinterpret_cast<const i4_t*>(bp + tn * bstep + p * planeB + kb);
+#pragma unroll
+        for (int d = 0; d < NS; ++d)
+#pragma unroll
+          for (int p = 0; p <= d; ++p)
+            acc[t][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], b[d - p], acc[t][d], 0, 0, 0);
+#pragma unroll
+        for (int p = 0; p < NS; ++p) b[p] = bn[p];
+      }
+    };
+    for (int i = 0; i + 1 < nb; ++i) {
+      i4_t an[NS];
+#pragma unroll
+      for (int p = 0; p < NS; ++p) an[p] = *reinterpret_cast<const i4_t*>(ap + (size_t)(i + 1) * NS * 1024 + p * 1024);
+      block(i << 6);
+#pragma unroll
+      for (int p = 0; p < NS; ++p) a[p] = an[p];
+    }
+    block((nb - 1) << 6);
+  }
+  double vals[NT][4];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int fc = fB[t * 16 + rr];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int d = NS - 1; d >= 0; --d) s += ldexp(static_cast<double>(acc[t][d][i]), -7 * (d + 2));
+      vals[t][i] = ldexp(s, eA[row0 + 4 * q + i] + fc);
+    }
+  }
+  reduce_store<NT, false>(vals, WT, nGp, row0, red);
+}
+
+// ---------------------------------------------------------------------------------------- split FP32
+template <int NT>
+__global__ __launch_bounds__(256) void k_mf_f32x3(const float* __restrict__ Ah, const float* __restrict__ Al,
+                                                  const float* __restrict__ Bh, const float* __restrict__ Bl,
+                                                  double* __restrict__ WT, int nGp) {
+  __shared__ double red[2 * 16 * (NT * 16 + 1)];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  const int row0 = blockIdx.x * 16;
+  const int kq = nGp >> 2;
+  const int k0 = wave * kq + 4 * q;
+  const size_t aoff = (size_t)(row0 + rr) * nGp + k0, bstep = (size_t)16 * nGp;
+  const size_t boff = (size_t)rr * nGp + k0;
+  double vals[NT][4];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) vals[t][i] = 0.0;
+  for (int kc = 0; kc < kq; kc += 32) {
+    f4_t acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = (f4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 32; kb += 16) {
+      const f4_t ah = *reinterpret_cast<const f4_t*>(Ah + aoff + kc + kb);
+      const f4_t al = *reinterpret_cast<const f4_t*>(Al + aoff + kc + kb);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const f4_t bh = *reinterpret_cast<const f4_t*>(Bh + boff + t * bstep + kc + kb);
+        const f4_t bl = *reinterpret_cast<const f4_t*>(Bl + boff + t * bstep + kc + kb);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[s], bh[s], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[s], bl[s], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[s], bh[s], acc[t], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) vals[t][i] += static_cast<double>(acc[t][i]);
+  }
+  reduce_store<NT, false>(vals, WT, nGp, row0, red);
+}
+
+// ---------------------------------------------------------------------------------------- 3xBF16
+template <int NT>
+__global__ __launch_bounds__(256) void k_mf_bf16x3(const __bf16* __restrict__ A, const __bf16* __restrict__ B,
+                                                   double* __restrict__ WT, int nGp, int Pc) {
+  __shared__ double red[2 * 16 * (NT * 16 + 1)];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  const int row0 = blockIdx.x * 16;
+  const int kq = nGp >> 2;
+  const int k0 = wave * kq + 8 * q;
+  const size_t planeA = (size_t)nGp * nGp, planeB = (size_t)Pc * nGp, bstep = (size_t)16 * nGp;
+  const __bf16* ap = A + (size_t)(row0 + rr) * nGp + k0;
+  const __bf16* bp = B + (size_t)rr * nGp + k0;
+  double vals[NT][4];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) vals[t][i] = 0.0;
+  for (int kc = 0; kc < kq; kc += 64) {
+    f4_t acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = (f4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 64; kb += 32) {
+      bf8_t a[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) a[p] = *reinterpret_cast<const bf8_t*>(ap + p * planeA + kc + kb);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        bf8_t b[3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) b[p] = *reinterpret_cast<const bf8_t*>(bp + t * bstep + p * planeB + kc + kb);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) vals[t][i] += static_cast<double>(acc[t][i]);
+  }
+  reduce_store<NT, false>(vals, WT, nGp, row0, red);
+}
+
+// ---------------------------------------------------------------------------------------- launchers
+#define CODY_NT_SWITCH2(PC, CALL) \
+  switch ((PC) / 16) {            \
+    case 1: CALL(1); break;       \
+    case 2: CALL(2); break;       \
+    case 3: CALL(3); break;       \
+    case 4: CALL(4); break;       \
+    case 5: CALL(5); break;       \
+    case 6: CALL(6); break;       \
+    case 7: CALL(7); break;       \
+    case 8: CALL(8); break;       \
+    case 9: CALL(9); break;       \
+    case 10: CALL(10); break;     \
+    default: break;               \
+  }
+
+// nGp must be a multiple of 256; Wpart holds kWSplit x Pc x nGp doubles.
+// With WT null only the partials are formed (k_wsum is not launched).
+void launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
+                     hipStream_t st) {
+#define CALL_W(NT)                                                                                        \
+  hipLaunchKernelGGL(k_mf_f64_w<NT>, dim3(nGp / 64, kWSplit), dim3((NT) <= 5 ? 512 : 256), 0, st, Vt, STt, \
+                     nGp, Wpart)
+  CODY_NT_SWITCH2(Pc, CALL_W)
+#undef CALL_W
+  if (!WT) return;
+  const size_t plane = (size_t)Pc * nGp;
+  hipLaunchKernelGGL(k_wsum, dim3((unsigned)((plane / 2 + 255) / 256)), dim3(256), 0, st, Wpart, plane, WT);
+}
+
+// Gpart[s][c2][c1] = sum_{x in split s} ST[c1][x] WT[c2][x]; nsplit * 64 must divide nGp.
+void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp, int Pc, int nsplit,
+                     hipStream_t st) {
+  const int kq = nGp / (4 * nsplit);
+#define CALL_G(NT)                                                                                  \
+  hipLaunchKernelGGL(k_gram_f64<NT>, dim3(Pc / 16, nsplit), dim3(256), 0, st, STt, nGp / 16, WT, nGp, Gpart, \
+                     Pc, kq, (size_t)Pc * Pc)
+  CODY_NT_SWITCH2(Pc, CALL_G)
+#undef CALL_G
+}
+
+void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,
+                  int Pc, hipStream_t st) {
+#define CALL_B(NT)                                                                                     \
+  hipLaunchKernelGGL((k_mf_i8<NT, kInt8Slices>), dim3(nGp / 16), dim3(256), 0, st, As, eA, Bs, fB, WT, nGp, \
+                     Pc)
+  CODY_NT_SWITCH2(Pc, CALL_B)
+#undef CALL_B
+}
+
+void launch_mf_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl, double* WT, int nGp,
+                     int Pc, hipStream_t st) {
+#define CALL_C(NT) \
+  hipLaunchKernelGGL(k_mf_f32x3<NT>, dim3(nGp / 16), dim3(256), 0, st, Ah, Al, Bh, Bl, WT, nGp)
+  CODY_NT_SWITCH2(Pc, CALL_C)
+#undef CALL_C
+}
+
+void launch_mf_bf16x3(const __bf16* A, const __bf16* B, double* WT, int nGp, int Pc, hipStream_t st) {
+#define CALL_D(NT) hipLaunchKernelGGL(k_mf_bf16x3<NT>, dim3(nGp / 16), dim3(256), 0, st, A, B, WT, nGp, Pc)
+  CODY_NT_SWITCH2(Pc, CALL_D)
+#undef CALL_D
+}
+
+}  // namespace cody

@@ -1,4 +1,5 @@
-// Reduced-precision forms of the mean-field contraction W = V S (gfx950 MFMA).
+// Operand preparation of the reduced-precision forms of the mean-field contraction W = V S
+// (the contraction kernels themselves are in meanfield_gemm.hip).
 //
 //  * int8-sliced (exact products): every row of V and every column of S is scaled by a power of
 //    two into [-1/2, 1/2] and split into NS signed 7-bit slices, x = 2^e sum_p X_p 2^{-7p}
@@ -10,7 +11,6 @@
 //    v_mfma_f32_16x16x4f32, FP32 accumulation over 32-wide k chunks flushed into FP64.
 //  * 3xBF16: x = b1 + b2 + b3 (successive BF16 roundings); the six products with p + q <= 4 on
 //    v_mfma_f32_16x16x32_bf16, FP32 accumulation over 64-wide k chunks flushed into FP64.
-// Output layout as the FP64 kernel: Wpart[split][row][col], FP64.
 #include <hip/hip_runtime.h>
 
 #include "mctdhx_kernels.hpp"
@@ -23,12 +23,14 @@ typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 // ------------------------------------------------------------------ slicing
 // One workgroup per row (V) or per column of S (ST row): exponent e = ceil(log2 max|x|) + 1,
-// slices written to out[p][line][k], p = 0..ns-1 (int8).
-__global__ __launch_bounds__(256) void k_slice_rows(const double* __restrict__ X, int len, int nlines,
-                                                    int ns, int8_t* __restrict__ out,
-                                                    int* __restrict__ expo) {
-  __shared__ double red[4];
-  const int line = blockIdx.x;
+// slices written (int8) either line-major, out[p][line][k], or fragment-tiled for the MFMA A
+// operand: slice p of the 16-line x 64-k block (R, K) is 1 KB contiguous at
+// ((R nK64 + K) ns + p) 1024, lane l = 16 (k%64 / 16) + line%16 holding bytes k%16.
+__global__ __launch_bounds__(1024) void k_slice_rows(const double* __restrict__ X, int len, int nlines,
+                                                     int ns, int tiled, int8_t* __restrict__ out,
+                                                     int* __restrict__ expo) {
+  __shared__ double red[16];
+  const int line = blockIdx.x, nw = blockDim.x >> 6;
   const double* row = X + (size_t)line * len;
   double m = 0.0;
   for (int k = threadIdx.x; k < len; k += blockDim.x) m = fmax(m, fabs(row[k]));
@@ -36,7 +38,8 @@ __global__ __launch_bounds__(256) void k_slice_rows(const double* __restrict__ X
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  m = 0.0;
+  for (int w = 0; w < nw; ++w) m = fmax(m, red[w]);
   int e = 0;
   if (m > 0.0) {
     int ex;
@@ -45,70 +48,45 @@ __global__ __launch_bounds__(256) void k_slice_rows(const double* __restrict__ X
   }
   if (threadIdx.x == 0) expo[line] = e;
   const size_t plane = (size_t)nlines * len;
-  for (int k = threadIdx.x; k < len; k += blockDim.x) {
-    double r = ldexp(row[k], -e);
+  const int nK64 = len >> 6;
+  // four consecutive k per thread: one 32-bit store per slice (len is a multiple of 64)
+  for (int k = 4 * threadIdx.x; k < len; k += 4 * blockDim.x) {
+    double r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = ldexp(row[k + u], -e);
+    size_t o;
+    if (tiled) {
+      const int lane = 16 * ((k & 63) >> 4) + (line & 15);
+      o = ((size_t)(line >> 4) * nK64 + (k >> 6)) * ns * 1024 + 16 * lane + (k & 15);
+    } else {
+      o = (size_t)line * len + k;
+    }
     for (int p = 1; p <= ns; ++p) {
-      const double q = rint(ldexp(r, 7 * p));
-      out[(size_t)(p - 1) * plane + (size_t)line * len + k] = static_cast<int8_t>(q);
-      r -= ldexp(q, -7 * p);
+      unsigned word = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double q = rint(ldexp(r[u], 7 * p));
+        r[u] -= ldexp(q, -7 * p);
+        word |= (unsigned)(uint8_t)static_cast<int8_t>(q) << (8 * u);
+      }
+      *reinterpret_cast<unsigned*>(out + o + (tiled ? (size_t)(p - 1) * 1024 : (size_t)(p - 1) * plane)) = word;
     }
   }
 }
 
-// ------------------------------------------------------------------ int8-sliced contraction
-// Lane l of a wave holds row (l%16) of a 16-row tile and the 16 consecutive k of group l/16 of
-// each 64-wide k block, for A = V slices and B = S slices alike, so each i8 MFMA contracts 64 k.
-template <int NT, int NS>
-__global__ __launch_bounds__(256) void k_meanfield_i8(const int8_t* __restrict__ As,
-                                                      const int* __restrict__ eA,
-                                                      const int8_t* __restrict__ Bs,
-                                                      const int* __restrict__ fB,
-                                                      double* __restrict__ Wpart, int nGp, int Pc,
-                                                      int kchunk) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = lane >> 4, rr = lane & 15;
-  const int row = blockIdx.x * 64 + wave * 16 + rr;
-  const int split = blockIdx.y;
-  const int k0 = split * kchunk;
-  const size_t planeA = (size_t)nGp * nGp, planeB = (size_t)Pc * nGp;
-  i4_t acc[NT][NS];
-#pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int d = 0; d < NS; ++d) acc[t][d] = (i4_t){0, 0, 0, 0};
-  const int8_t* abase = As + (size_t)row * nGp + k0 + 16 * q;
-  for (int kb = 0; kb < kchunk; kb += 64) {
-    i4_t a[NS];
-#pragma unroll
-    for (int p = 0; p < NS; ++p) a[p] = *reinterpret_cast<const i4_t*>(abase + p * planeA + kb);
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int8_t* bbase = Bs + (size_t)(t * 16 + rr) * nGp + k0 + 16 * q + kb;
-      i4_t b[NS];
-#pragma unroll
-      for (int p = 0; p < NS; ++p) b[p] = *reinterpret_cast<const i4_t*>(bbase + p * planeB);
-      // diagonal d = p + q - 2 (0-based slice indices p, q with p + q <= NS - 1)
-#pragma unroll
-      for (int d = 0; d < NS; ++d)
-#pragma unroll
-        for (int p = 0; p <= d; ++p)
-          acc[t][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[p], b[d - p], acc[t][d], 0, 0, 0);
-    }
-  }
-  double* out = Wpart + (size_t)split * nGp * Pc;
-  const int orow = blockIdx.x * 64 + wave * 16 + 4 * q;
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int col = t * 16 + rr;
-    const int fc = fB[col];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      double s = 0.0;
-#pragma unroll
-      for (int d = NS - 1; d >= 0; --d) s += ldexp(static_cast<double>(acc[t][d][i]), -7 * (d + 2));
-      out[(size_t)(orow + i) * Pc + col] = ldexp(s, eA[orow + i] + fc);
-    }
-  }
+// ------------------------------------------------------------------ FP64 tiling
+// Vt[(R nK + K) 256 + 128 h + 2 l + e] = V[16 R + l % 16][16 K + 4 (l / 16) + 2 h + e]: every 16 x 16
+// block of V becomes 2 KB of contiguous MFMA fragments (two 1 KB loads per wave).
+__global__ __launch_bounds__(256) void k_tile_f64(const double* __restrict__ V, int nGp,
+                                                  double* __restrict__ Vt) {
+  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= (size_t)nGp * nGp) return;
+  const int nK = nGp >> 4;
+  const int e = o & 1, l = (o >> 1) & 63, h = (o >> 7) & 1;
+  const size_t blk = o >> 8;
+  const int R = (int)(blk / nK), K = (int)(blk % nK);
+  const int row = 16 * R + (l & 15), k = 16 * K + 4 * (l >> 4) + 2 * h + e;
+  Vt[o] = V[(size_t)row * nGp + k];
 }
 
 // ------------------------------------------------------------------ split-FP32 contraction
@@ -120,58 +98,6 @@ __global__ __launch_bounds__(256) void k_split_f32(const double* __restrict__ X,
   const float h = static_cast<float>(x);
   hi[i] = h;
   lo[i] = static_cast<float>(x - static_cast<double>(h));
-}
-
-template <int NT>
-__global__ __launch_bounds__(256) void k_meanfield_f32x3(const float* __restrict__ Ah,
-                                                         const float* __restrict__ Al,
-                                                         const float* __restrict__ Bh,
-                                                         const float* __restrict__ Bl,
-                                                         double* __restrict__ Wpart, int nGp, int Pc,
-                                                         int kchunk) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = lane >> 4, rr = lane & 15;
-  const int row = blockIdx.x * 64 + wave * 16 + rr;
-  const int split = blockIdx.y;
-  const int k0 = split * kchunk;
-  double acc64[NT][4];
-#pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc64[t][i] = 0.0;
-  const size_t aoff = (size_t)row * nGp + k0 + 4 * q;
-  for (int kc = 0; kc < kchunk; kc += 32) {
-    f4_t acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = (f4_t){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kb = 0; kb < 32; kb += 16) {
-      const f4_t ah = *reinterpret_cast<const f4_t*>(Ah + aoff + kc + kb);
-      const f4_t al = *reinterpret_cast<const f4_t*>(Al + aoff + kc + kb);
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const size_t boff = (size_t)(t * 16 + rr) * nGp + k0 + 4 * q + kc + kb;
-        const f4_t bh = *reinterpret_cast<const f4_t*>(Bh + boff);
-        const f4_t bl = *reinterpret_cast<const f4_t*>(Bl + boff);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[s], bh[s], acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[s], bl[s], acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[s], bh[s], acc[t], 0, 0, 0);
-        }
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc64[t][i] += static_cast<double>(acc[t][i]);
-  }
-  double* out = Wpart + (size_t)split * nGp * Pc;
-  const int orow = blockIdx.x * 64 + wave * 16 + 4 * q;
-#pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) out[(size_t)(orow + i) * Pc + t * 16 + rr] = acc64[t][i];
 }
 
 // ------------------------------------------------------------------ 3xBF16 contraction
@@ -191,63 +117,15 @@ __global__ __launch_bounds__(256) void k_split_bf16x3(const double* __restrict__
   b3[i] = v3;
 }
 
-template <int NT>
-__global__ __launch_bounds__(256) void k_meanfield_bf16x3(const __bf16* __restrict__ A, const __bf16* __restrict__ B,
-                                                          double* __restrict__ Wpart, int nGp, int Pc,
-                                                          int kchunk) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = lane >> 4, rr = lane & 15;
-  const int row = blockIdx.x * 64 + wave * 16 + rr;
-  const int split = blockIdx.y;
-  const int k0 = split * kchunk;
-  const size_t planeA = (size_t)nGp * nGp, planeB = (size_t)Pc * nGp;
-  double acc64[NT][4];
-#pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc64[t][i] = 0.0;
-  const __bf16* abase = A + (size_t)row * nGp + k0 + 8 * q;
-  for (int kc = 0; kc < kchunk; kc += 64) {
-    f4_t acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = (f4_t){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kb = 0; kb < 64; kb += 32) {
-      bf8_t a[3];
-#pragma unroll
-      for (int p = 0; p < 3; ++p) a[p] = *reinterpret_cast<const bf8_t*>(abase + p * planeA + kc + kb);
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const __bf16* bbase = B + (size_t)(t * 16 + rr) * nGp + k0 + 8 * q + kc + kb;
-        bf8_t b[3];
-#pragma unroll
-        for (int p = 0; p < 3; ++p) b[p] = *reinterpret_cast<const bf8_t*>(bbase + p * planeB);
-        // small terms first
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc[t], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc64[t][i] += static_cast<double>(acc[t][i]);
-  }
-  double* out = Wpart + (size_t)split * nGp * Pc;
-  const int orow = blockIdx.x * 64 + wave * 16 + 4 * q;
-#pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) out[(size_t)(orow + i) * Pc + t * 16 + rr] = acc64[t][i];
+// ------------------------------------------------------------------ launchers
+void launch_slice_rows(const double* X, int len, int nlines, int ns, int tiled, int8_t* out, int* expo,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_slice_rows, dim3(nlines), dim3(1024), 0, st, X, len, nlines, ns, tiled, out, expo);
 }
 
-// ------------------------------------------------------------------ launchers
-void launch_slice_rows(const double* X, int len, int nlines, int ns, int8_t* out, int* expo,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_slice_rows, dim3(nlines), dim3(256), 0, st, X, len, nlines, ns, out, expo);
+void launch_tile_f64(const double* V, int nGp, double* Vt, hipStream_t st) {
+  const size_t n = (size_t)nGp * nGp;
+  hipLaunchKernelGGL(k_tile_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, V, nGp, Vt);
 }
 
 void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream_t st) {
@@ -257,53 +135,6 @@ void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream
 void launch_split_bf16x3(const double* X, size_t n, __bf16* out, hipStream_t st) {
   hipLaunchKernelGGL(k_split_bf16x3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, X, n, out,
                      out + n, out + 2 * n);
-}
-
-#define CODY_NT_SWITCH(PC, CALL) \
-  switch ((PC) / 16) {           \
-    case 1: CALL(1); break;      \
-    case 2: CALL(2); break;      \
-    case 3: CALL(3); break;      \
-    case 4: CALL(4); break;      \
-    case 5: CALL(5); break;      \
-    case 6: CALL(6); break;      \
-    case 7: CALL(7); break;      \
-    case 8: CALL(8); break;      \
-    case 9: CALL(9); break;      \
-    case 10: CALL(10); break;    \
-    default: break;              \
-  }
-
-void launch_meanfield_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB,
-                         double* Wpart, int nGp, int Pc, int nsplit, hipStream_t st) {
-  dim3 grid(nGp / 64, nsplit);
-  const int kchunk = nGp / nsplit;
-#define CALL_I8(NT)                                                                                   \
-  hipLaunchKernelGGL((k_meanfield_i8<NT, kInt8Slices>), grid, dim3(256), 0, st, As, eA, Bs, fB, Wpart, \
-                     nGp, Pc, kchunk)
-  CODY_NT_SWITCH(Pc, CALL_I8)
-#undef CALL_I8
-}
-
-void launch_meanfield_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl,
-                            double* Wpart, int nGp, int Pc, int nsplit, hipStream_t st) {
-  dim3 grid(nGp / 64, nsplit);
-  const int kchunk = nGp / nsplit;
-#define CALL_F32(NT)                                                                              \
-  hipLaunchKernelGGL(k_meanfield_f32x3<NT>, grid, dim3(256), 0, st, Ah, Al, Bh, Bl, Wpart, nGp, Pc, \
-                     kchunk)
-  CODY_NT_SWITCH(Pc, CALL_F32)
-#undef CALL_F32
-}
-
-void launch_meanfield_bf16x3(const __bf16* A, const __bf16* B, double* Wpart, int nGp, int Pc,
-                             int nsplit, hipStream_t st) {
-  dim3 grid(nGp / 64, nsplit);
-  const int kchunk = nGp / nsplit;
-#define CALL_BF(NT) \
-  hipLaunchKernelGGL(k_meanfield_bf16x3<NT>, grid, dim3(256), 0, st, A, B, Wpart, nGp, Pc, kchunk)
-  CODY_NT_SWITCH(Pc, CALL_BF)
-#undef CALL_BF
 }
 
 }  // namespace cody

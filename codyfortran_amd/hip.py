@@ -56,6 +56,7 @@ def load():
         "cody_set_interaction_softyukawa": (i, [vp, d, d, d, d]),
         "cody_set_laser": (i, [vp, i, dp, d]),
         "cody_set_precision": (i, [vp, i]),
+        "cody_set_graphs": (i, [vp, i]),
         "cody_upload_state": (i, [vp, vp]),
         "cody_download_state": (i, [vp, vp]),
         "cody_state_device_ptr": (i, [vp, C.POINTER(vp)]),
@@ -175,6 +176,10 @@ class Context:
 
     def set_precision(self, mode):
         _check(self.lib.cody_set_precision(self.h, int(mode)))
+
+    def set_graphs(self, enable: bool):
+        """Replay captured right-hand sides (True, the default) or launch every kernel."""
+        _check(self.lib.cody_set_graphs(self.h, int(bool(enable))))
 
     # ---------------------------------------------------------------- state
     def upload_state(self, state):

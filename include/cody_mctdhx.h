@@ -89,6 +89,11 @@ int cody_set_interaction_softyukawa(cody_ctx* ctx, double strength, double soft1
 int cody_set_laser(cody_ctx* ctx, int nK, const double* c, double T);
 /* Precision of the mean-field contraction (CODY_PREC_*). */
 int cody_set_precision(cody_ctx* ctx, int mode);
+/* Replay of captured right-hand sides (no counterpart in the reference, whose right-hand side is
+ * a chain of host calls): with enable != 0 (the default) the device-pointer entry points and the
+ * RK4 stages capture the launches of one right-hand side per (state, dState, mode) into a HIP
+ * graph on first use and replay it; 0 launches every kernel individually. */
+int cody_set_graphs(cody_ctx* ctx, int enable);
 
 /* ---------------------------------------------------------------- state transfer */
 /* The context's device-resident state (Method_state, src/Method/M_Method.f90:47). */

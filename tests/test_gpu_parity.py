@@ -156,3 +156,53 @@ def test_split_step_conserves_norm_at_small_step(pair):
         t += dt
     sg = ctx.download_state()
     assert abs(np.linalg.norm(sg[: w.nC]) - 1.0) < 1e-10
+
+
+def test_graph_replays_match_plain_launches(pair):
+    """Captured right-hand sides replayed with other work and synchronisations in between: every
+    replay equals the plain-launch result bit for bit; with a dipole field E(t) the replays follow
+    t (E(t) is set per replay, outside the graph) and match the oracle."""
+    import torch
+
+    name, w, s, ctx, state, wts = pair
+    dev = torch.device("cuda:0")
+    st = torch.from_numpy(state.view(np.float64).copy()).to(dev)
+    out = torch.zeros_like(st)
+    stream = torch.cuda.current_stream().cuda_stream
+    ctx.set_graphs(False)
+    ref = {}
+    for mode in ("full", "nonlin"):
+        ctx.time_derivative_dev(st.data_ptr(), out.data_ptr(), 0.0, stream, mode=mode)
+        torch.cuda.synchronize()
+        ref[mode] = out.cpu().numpy().copy()
+    ctx.set_graphs(True)
+    for rep in range(4):
+        for mode in ("full", "nonlin"):
+            out.zero_()
+            ctx.time_derivative_dev(st.data_ptr(), out.data_ptr(), 0.0, stream, mode=mode)
+            torch.cuda.synchronize()
+            assert np.array_equal(out.cpu().numpy(), ref[mode]), (rep, mode)
+        # eager work of the same context and of torch between replays
+        ctx.time_derivative(state)
+        (st * 2.0).sum().item()
+    # time-dependent field: replays at several t against plain launches and the oracle
+    coef, T = [0.05, -0.02], 40.0
+    ctx.set_laser(coef, T)
+    s.set_laser(coef, T)
+    try:
+        for t in (3.0, 11.5, 3.0):
+            ctx.set_graphs(False)
+            ctx.time_derivative_dev(st.data_ptr(), out.data_ptr(), t, stream)
+            torch.cuda.synchronize()
+            plain = out.cpu().numpy().copy()
+            ctx.set_graphs(True)
+            for rep in range(2):
+                out.zero_()
+                ctx.time_derivative_dev(st.data_ptr(), out.data_ptr(), t, stream)
+                torch.cuda.synchronize()
+                assert np.array_equal(out.cpu().numpy(), plain), (t, rep)
+            got = plain.view(np.complex128)
+            assert state_rel(got, s.time_derivative(state, t), w.nC, wts, w.M) < 1e-12, t
+    finally:
+        ctx.set_laser([], 1.0)
+        s.set_laser([], 1.0)
