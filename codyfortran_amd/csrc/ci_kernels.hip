@@ -1,7 +1,7 @@
 // CI-side kernels of the MCTDHX right-hand side (the CI space is small, n_C <= 1024):
 //   k_ci_rdm      : rho1, rho2 from C, one wave per density element
-//   k_ci_reginv   : R = rho1 (rho1^2 + eps)^{-1}, rho2' = rho2 x_3 R (one workgroup, LDS)
-//   k_gram_reduce : G = S^T W from the split-K partials of the Gram kernel
+//   k_ci_reginv   : R = rho1 (rho1^2 + eps)^{-1} (one wave)
+//   k_gram_reduce : G = S^T W from the split-K partials of the Gram kernel; h1 from its partials
 //   k_ci_apply    : CI H-apply with h2 formed from G, one wave per configuration
 //   k_ci_energy   : E = Tr[rho1 h1] + Tr[rho2 h2]
 //   k_sum_parts   : h1 / inp from per-workgroup partials
@@ -64,7 +64,7 @@ __device__ __forceinline__ void strict_pair_decode(int nO, int p, int& a, int& b
 // of one element are issued together.  Each canonical rho2 value is also written to its four
 // antisymmetric positions of the full rho2; entries with i1 = i2 or j1 = j2 are never written
 // (they are zero from allocation).
-__global__ __launch_bounds__(256) void k_ci_rdm(const double2* __restrict__ C, int nO,
+__global__ __launch_bounds__(256) void k_ci_rdm(const double2* __restrict__ C, int nC, int nO,
                                                 const int* __restrict__ r1_ptr,
                                                 const CiEntryDev* __restrict__ r1,
                                                 const int* __restrict__ r2_ptr,
@@ -73,6 +73,9 @@ __global__ __launch_bounds__(256) void k_ci_rdm(const double2* __restrict__ C, i
                                                 double2* __restrict__ rho2) {
   const int n2 = nO * nO, np = nO * (nO - 1) / 2, nCan = np * np;
   const int item = blockIdx.x * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63;
+  extern __shared__ double2 sC[];  // C staged once per workgroup
+  for (int i = threadIdx.x; i < nC; i += blockDim.x) sC[i] = C[i];
+  __syncthreads();
   if (item >= n2 + nCan) return;
   int code, i1 = 0, i2 = 0, j1 = 0, j2 = 0;
   const int* ptr;
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(256) void k_ci_rdm(const double2* __restrict__ C, i
   double2 acc = cmk(0, 0);
   for (int q = ptr[code] + ln; q < ptr[code + 1]; q += 64) {
     const CiEntryDev en = tab[q];
-    cfma(acc, cscale(cconj(C[en.jc]), en.f), C[en.ic]);
+    cfma(acc, cscale(cconj(sC[en.jc]), en.f), sC[en.ic]);
   }
   acc.x = wave_sum(acc.x);
   acc.y = wave_sum(acc.y);
@@ -109,91 +112,64 @@ __global__ __launch_bounds__(256) void k_ci_rdm(const double2* __restrict__ C, i
   }
 }
 
-// One workgroup: R = rho1 (rho1^2 + eps)^{-1} and rho2' = rho2 x_3 R from rho1 and the canonical
-// rho2 staged in LDS.
-// Dynamic LDS: r1[n2] | r2can[nCan] | A0[n2] | M[2 n2] | X[n2] | R[n2] | f[nO]
-__global__ __launch_bounds__(256) void k_ci_reginv(const double2* __restrict__ rho1g,
-                                                   const double2* __restrict__ r2g, int nO, double reg,
-                                                   double2* __restrict__ rho2p, double2* __restrict__ reginv) {
-  extern __shared__ double2 lds[];
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int n2 = nO * nO, n4 = n2 * n2, np = nO * (nO - 1) / 2, nCan = np * np;
-  double2* s1 = lds;
-  double2* s2 = s1 + n2;
-  double2* sA0 = s2 + nCan;
-  double2* sM = sA0 + n2;  // augmented [A | I], nO x 2nO, column-major
-  double2* sX = sM + 2 * n2;
-  double2* sR = sX + n2;
-  double2* sF = sR + n2;
-  for (int i = t; i < n2; i += nt) s1[i] = rho1g[i];
-#pragma unroll 4
-  for (int i = t; i < nCan; i += nt) s2[i] = r2g[i];
+// R = rho1 (rho1^2 + eps)^{-1} (regInvMat, S_Method_Mb_OrbBased.f90:737-744) by Gauss-Jordan on
+// rho1^2 + eps I (Hermitian positive definite: no pivoting) and one refinement step, in one wave
+// (the barriers of a one-wave workgroup cost nothing).  rho2' = rho2 x_3 R is formed slice by
+// slice inside the orbital kernel.
+template <int NO>
+__global__ __launch_bounds__(64) void k_ci_reginv(const double2* __restrict__ rho1g, double reg,
+                                                  double2* __restrict__ reginv) {
+  constexpr int n2 = NO * NO;
+  __shared__ double2 s1[n2], sA0[n2], sM[2 * n2], sX[n2], sR[n2], sF[NO];
+  const int t = threadIdx.x;
+  for (int i = t; i < n2; i += 64) s1[i] = rho1g[i];
   __syncthreads();
-  // A = rho1^2 + eps I ; augmented [A | I]
-  for (int e = t; e < n2; e += nt) {
-    const int i = e % nO, j = e / nO;
+  for (int e = t; e < n2; e += 64) {
+    const int i = e % NO, j = e / NO;
     double2 acc = cmk(i == j ? reg : 0.0, 0.0);
-    for (int k = 0; k < nO; ++k) cfma(acc, s1[i + k * nO], s1[k + j * nO]);
+#pragma unroll
+    for (int k = 0; k < NO; ++k) cfma(acc, s1[i + k * NO], s1[k + j * NO]);
     sA0[e] = acc;
     sM[e] = acc;
     sM[n2 + e] = cmk(i == j ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
-  // Gauss-Jordan inverse (A is Hermitian positive definite: no pivoting needed)
-  for (int k = 0; k < nO; ++k) {
-    const double2 piv = sM[k + k * nO];
+  for (int k = 0; k < NO; ++k) {
+    const double2 piv = sM[k + k * NO];
     __syncthreads();
-    for (int c = t; c < 2 * nO; c += nt) sM[k + c * nO] = cdiv(sM[k + c * nO], piv);
-    for (int r = t; r < nO; r += nt) sF[r] = sM[r + k * nO];
+    if (t < 2 * NO) sM[k + t * NO] = cdiv(sM[k + t * NO], piv);
+    if (t < NO) sF[t] = sM[t + k * NO];
     __syncthreads();
-    for (int e = t; e < 2 * n2; e += nt) {
-      const int r = e % nO, c = e / nO;
-      if (r != k) sM[r + c * nO] = csub(sM[r + c * nO], cmul(sF[r], sM[k + c * nO]));
+    for (int e = t; e < 2 * n2; e += 64) {
+      const int r = e % NO, c = e / NO;
+      if (r != k) sM[r + c * NO] = csub(sM[r + c * NO], cmul(sF[r], sM[k + c * NO]));
     }
     __syncthreads();
   }
   // X = A^-1 rho1, then one refinement X += A^-1 (rho1 - A X)
   const double2* sInv = sM + n2;
-  for (int e = t; e < n2; e += nt) {
-    const int i = e % nO, j = e / nO;
+  for (int e = t; e < n2; e += 64) {
+    const int i = e % NO, j = e / NO;
     double2 acc = cmk(0, 0);
-    for (int k = 0; k < nO; ++k) cfma(acc, sInv[i + k * nO], s1[k + j * nO]);
+#pragma unroll
+    for (int k = 0; k < NO; ++k) cfma(acc, sInv[i + k * NO], s1[k + j * NO]);
     sX[e] = acc;
   }
   __syncthreads();
-  for (int e = t; e < n2; e += nt) {
-    const int i = e % nO, j = e / nO;
+  for (int e = t; e < n2; e += 64) {
+    const int i = e % NO, j = e / NO;
     double2 r = s1[e];
-    for (int k = 0; k < nO; ++k) r = csub(r, cmul(sA0[i + k * nO], sX[k + j * nO]));
+#pragma unroll
+    for (int k = 0; k < NO; ++k) r = csub(r, cmul(sA0[i + k * NO], sX[k + j * NO]));
     sR[e] = r;  // residual
   }
   __syncthreads();
-  for (int e = t; e < n2; e += nt) {
-    const int i = e % nO, j = e / nO;
+  for (int e = t; e < n2; e += 64) {
+    const int i = e % NO, j = e / NO;
     double2 acc = sX[e];
-    for (int k = 0; k < nO; ++k) cfma(acc, sInv[i + k * nO], sR[k + j * nO]);
+#pragma unroll
+    for (int k = 0; k < NO; ++k) cfma(acc, sInv[i + k * NO], sR[k + j * NO]);
     reginv[e] = acc;
-    sA0[e] = acc;  // R
-  }
-  __syncthreads();
-  // rho2'(i1,i2,j1,l2) = sum_k rho2(i1,i2,k,l2) R(k,j1)   (S_Method_Mb_OrbBased.f90:746-753)
-  for (int e = t; e < n4; e += nt) {
-    int a = e % nO, b = (e / nO) % nO;
-    const int j1 = (e / n2) % nO, l2 = e / (n2 * nO);
-    double2 acc = cmk(0, 0);
-    if (a != b) {
-      double sa = 1.0;
-      if (a > b) { int tmp = a; a = b; b = tmp; sa = -1.0; }
-      const int pa = strict_pair(nO, a, b);
-      for (int k = 0; k < nO; ++k) {
-        if (k == l2) continue;
-        int c = k, d = l2;
-        double sg = sa;
-        if (c > d) { int tmp = c; c = d; d = tmp; sg = -sg; }
-        cfma(acc, cscale(s2[pa + np * strict_pair(nO, c, d)], sg), sA0[k + j1 * nO]);
-      }
-    }
-    rho2p[e] = acc;
   }
 }
 
@@ -209,12 +185,22 @@ __global__ __launch_bounds__(256) void k_sum_parts(const double2* __restrict__ p
   out[e] = acc;
 }
 
-// G[c1][c2] = sum_s Gpart[s][c2][c1] = sum_x S_c1(x) W_c2(x), fixed order.
+// G[c1][c2] = sum_s Gpart[s][c2][c1] = sum_x S_c1(x) W_c2(x), fixed order; threads past Pc^2 form
+// h1 = sum_b part1[b][e] from the one-body kernel's per-workgroup partials.
 __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ Gpart, int nxs, int Pc,
-                                                     double* __restrict__ G) {
+                                                     double* __restrict__ G, const double2* __restrict__ part1,
+                                                     int nPart1, int n2, double2* __restrict__ h1) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = Pc * Pc;
-  if (g >= n) return;
+  if (g >= n) {
+    const int e = g - n;
+    if (e >= n2) return;
+    double2 acc = cmk(0, 0);
+#pragma unroll 8
+    for (int b = 0; b < nPart1; ++b) acc = cadd(acc, part1[(size_t)b * n2 + e]);
+    h1[e] = acc;
+    return;
+  }
   double acc = 0.0;
 #pragma unroll 8
   for (int s = 0; s < nxs; ++s) acc += Gpart[(size_t)s * n + g];
@@ -308,34 +294,49 @@ __global__ __launch_bounds__(256) void k_ci_energy(int nO, int Pc, const double*
 }
 
 // ------------------------------------------------------------------------------------ launchers
+// Dynamic LDS of the density kernels: C (k_ci_rdm); the others use static LDS.
 size_t ci_density_lds_bytes(int nC, int nO) {
-  (void)nC;
-  const int n2 = nO * nO, np = nO * (nO - 1) / 2;
-  return sizeof(double2) * (size_t)(np * np + 6 * n2 + nO);
+  (void)nO;
+  return sizeof(double2) * (size_t)nC;
 }
+
 int ci_kernels_allow_lds(size_t bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ci_reginv),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess
-             ? 0
-             : 1;
+  (void)bytes;  // static LDS only
+  return 0;
 }
 
 void launch_ci_density(const double2* C, int nC, int nO, double reg, const DevCiTables& t,
-                       double2* rho1, double2* rho2, double2* r2can, double2* rho2p, double2* reginv,
-                       hipStream_t st) {
+                       double2* rho1, double2* rho2, double2* r2can, double2* reginv, hipStream_t st) {
   const int n2 = nO * nO, np = nO * (nO - 1) / 2, items = n2 + np * np;
-  hipLaunchKernelGGL(k_ci_rdm, dim3((items + 3) / 4), dim3(256), 0, st, C, nO, t.r1_ptr, t.r1, t.r2_ptr, t.r2,
-                     rho1, r2can, rho2);
-  hipLaunchKernelGGL(k_ci_reginv, dim3(1), dim3(256), ci_density_lds_bytes(nC, nO), st, rho1, r2can, nO, reg,
-                     rho2p, reginv);
+  hipLaunchKernelGGL(k_ci_rdm, dim3((items + 3) / 4), dim3(256), sizeof(double2) * nC, st, C, nC, nO, t.r1_ptr,
+                     t.r1, t.r2_ptr, t.r2, rho1, r2can, rho2);
+#define CALL_RI(NO) hipLaunchKernelGGL(k_ci_reginv<NO>, dim3(1), dim3(64), 0, st, rho1, reg, reginv)
+  switch (nO) {
+    case 1: CALL_RI(1); break;
+    case 2: CALL_RI(2); break;
+    case 3: CALL_RI(3); break;
+    case 4: CALL_RI(4); break;
+    case 5: CALL_RI(5); break;
+    case 6: CALL_RI(6); break;
+    case 7: CALL_RI(7); break;
+    case 8: CALL_RI(8); break;
+    case 9: CALL_RI(9); break;
+    case 10: CALL_RI(10); break;
+    case 11: CALL_RI(11); break;
+    case 12: CALL_RI(12); break;
+    default: break;
+  }
+#undef CALL_RI
 }
 
 void launch_sum_parts(const double2* part, int nPart, int stride, int n, double2* out, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_parts, dim3((n + 255) / 256), dim3(256), 0, st, part, nPart, stride, n, out);
 }
 
-void launch_gram_reduce(const double* Gpart, int nxs, int Pc, double* G, hipStream_t st) {
-  hipLaunchKernelGGL(k_gram_reduce, dim3((Pc * Pc + 255) / 256), dim3(256), 0, st, Gpart, nxs, Pc, G);
+void launch_gram_reduce(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
+                        int n2, double2* h1, hipStream_t st) {
+  hipLaunchKernelGGL(k_gram_reduce, dim3((Pc * Pc + n2 + 255) / 256), dim3(256), 0, st, Gpart, nxs, Pc, G, part1,
+                     nPart1, n2, h1);
 }
 
 void launch_ci_apply(const double2* C, int nC, int nO, int Pc, const double* G, const double2* h1,

@@ -46,6 +46,10 @@ int ctx_alloc(void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail(CODY_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   e = hipMemset(*p, 0, bytes);
+  // hipMemset may return before the fill has run, and the context's streams are non-blocking
+  // (not ordered after it): wait here, or a kernel launched on them right after the allocation
+  // (the V tiling, the propagator's first step) could have its output zeroed by the late fill
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) return fail(CODY_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
   return CODY_OK;
 }
@@ -119,7 +123,7 @@ const char* enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function
       mark("k_mf_bf16x3");
       return "k_mf_bf16x3";
     default:
-      launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, st);
+      launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st);
       mark("k_mf_f64_w + k_wsum");
       return "k_mf_f64_w";
   }
@@ -404,43 +408,52 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
     launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_draw, c->d_part1,
                    st);
     mark("k_onebody");
-    launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 0, dorbs, nullptr, nullptr, 0, nullptr, st);
+    launch_project(orbs, c->d_draw, nullptr, 0, c->nG, c->nO, 0, dorbs, nullptr, nullptr, 0, nullptr, st);
     mark("k_project");
     launch_zero(dC, c->nC, st);
     mark("k_zero");
     return CODY_OK;
   }
-  // side stream, part 1 (independent of the mean fields, overlaps the contraction): CI densities,
-  // R and rho2'; h phi and h1
-  hipEventRecord(c->ev_fork, st);
-  hipStreamWaitEvent(c->stream_ci, c->ev_fork, 0);
-  launch_ci_density(C, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_rho2p,
-                    c->d_reginv, c->stream_ci);
-  launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_hphi, c->d_part1,
-                 c->stream_ci);
-  launch_sum_parts(c->d_part1, nPart1, n2, n2, c->d_h1, c->stream_ci);
-  hipEventRecord(c->ev_a, c->stream_ci);
-  // main stream: pair sources and the mean fields W = V S
+  // Two schedules of the same launches.  Split (default): the CI densities, R and the one-body
+  // part run on the side stream while the contraction runs; after the contraction the Gram matrix,
+  // h1 and the CI H-apply run on the side stream while the orbital correlation runs; the
+  // projections join both.  One queue (c->one_queue, used when many instances share a graph):
+  // every launch in order on `st`.
+  const bool split = !c->one_queue;
+  hipStream_t sa = split ? c->stream_ci : st;
+  // pair sources first, then the fork, then the mean fields W = V S.  (The pair-source kernel is
+  // the only root of a captured graph and the contraction is created before the side-stream work,
+  // so that a graph executor walking nodes in creation order starts the contraction first.)
   launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st);
   mark("k_pair_src");
+  if (split) hipEventRecord(c->ev_fork, st);
   enqueue_contraction(c, st, mark);
-  hipEventRecord(c->ev_w, st);
-  // side stream, part 2: Gram matrix G = S^T W, CI H-apply (and the energy on request)
-  hipStreamWaitEvent(c->stream_ci, c->ev_w, 0);
-  launch_gram_f64(c->d_STt, c->d_WT, c->d_Gpart, c->nGp, c->Pc, c->nxs, c->stream_ci);
-  launch_gram_reduce(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->stream_ci);
-  launch_ci_apply(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_dCraw, c->stream_ci);
-  if (energy) launch_ci_energy(c->nO, c->Pc, c->d_G, c->d_h1, c->d_rho1, c->d_rho2, energy, c->stream_ci);
-  hipEventRecord(c->ev_ci, c->stream_ci);
-  // main stream: orbital correlation, projector overlaps, then both projections
-  hipStreamWaitEvent(st, c->ev_a, 0);
-  launch_orbital(orbs, c->d_w, c->d_hphi, c->d_WT, c->d_rho2p, c->nG, c->nGp, c->nO, mode, c->d_draw,
-                 c->d_part, st);
+  if (split) {
+    hipEventRecord(c->ev_w, st);
+    hipStreamWaitEvent(sa, c->ev_fork, 0);
+  }
+  // independent of the mean fields: CI densities and R; h phi and the partial sums of h1
+  launch_ci_density(C, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_reginv, sa);
+  launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_hphi, c->d_part1, sa);
+  if (split) {
+    hipEventRecord(c->ev_a, sa);
+    hipStreamWaitEvent(sa, c->ev_w, 0);
+  }
+  // CI part: Gram matrix G = S^T W and h1, CI H-apply (and the energy on request)
+  launch_gram_f64(c->d_STt, c->d_WT, c->d_Gpart, c->nGp, c->Pc, c->nxs, sa);
+  launch_gram_reduce(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->d_part1, nPart1, n2, c->d_h1, sa);
+  launch_ci_apply(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_dCraw, sa);
+  if (energy) launch_ci_energy(c->nO, c->Pc, c->d_G, c->d_h1, c->d_rho1, c->d_rho2, energy, sa);
+  if (split) {
+    hipEventRecord(c->ev_ci, sa);
+    hipStreamWaitEvent(st, c->ev_a, 0);
+  }
+  // orbital correlation with its projector-overlap partials, then both projections
+  launch_orbital(orbs, c->d_w, c->d_hphi, c->d_WT, c->d_r2can, c->d_reginv, c->nG, c->nGp, c->nO, mode,
+                 c->d_draw, c->d_part, st);
   mark("k_orbital");
-  launch_sum_parts(c->d_part, c->nPart, n2, n2, c->d_inp, st);
-  mark("k_sum_parts");
-  hipStreamWaitEvent(st, c->ev_ci, 0);
-  launch_project(orbs, c->d_draw, c->d_inp, c->nG, c->nO, 1, dorbs, C, c->d_dCraw, c->nC, dC, st);
+  if (split) hipStreamWaitEvent(st, c->ev_ci, 0);
+  launch_project(orbs, c->d_draw, c->d_part, c->nPart, c->nG, c->nO, 1, dorbs, C, c->d_dCraw, c->nC, dC, st);
   mark("k_project");
   return CODY_OK;
 }
@@ -490,6 +503,48 @@ int run_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mo
 }  // namespace cody
 
 extern "C" {
+
+int cody_autotune(cody_ctx* c, int* wsplit) {
+  if (!c) return fail(CODY_ERR_ARG, "cody_autotune: null context");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  HIPCHK(hipStreamSynchronize(st));
+  if (wsplit && *wsplit > 0) {  // a given split instead of the timed choice
+    if (*wsplit != 1 && *wsplit != 2 && *wsplit != 4 && *wsplit != 8)
+      return fail(CODY_ERR_ARG, "cody_autotune: the k split must be 1, 2, 4 or 8");
+    c->wsplit = *wsplit;
+    c->drop_graphs();
+    return CODY_OK;
+  }
+  if (c->precision == CODY_PREC_FP64) {
+    launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int pick = c->wsplit;
+    for (int ks : {1, 2, 4, 8}) {
+      launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st);  // warm-up
+      HIPCHK(hipEventRecord(e0, st));
+      for (int i = 0; i < 8; ++i) launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st);
+      HIPCHK(hipEventRecord(e1, st));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        pick = ks;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->wsplit = pick;
+    c->drop_graphs();
+  }
+  if (wsplit) *wsplit = c->wsplit;
+  return CODY_OK;
+}
 
 int cody_set_graphs(cody_ctx* c, int enable) {
   if (!c) return fail(CODY_ERR_ARG, "cody_set_graphs: null context");
@@ -588,8 +643,8 @@ int cody_densities(cody_ctx* c, const double* coeffs, double* rdm1, double* rdm2
   if (!c || !coeffs) return fail(CODY_ERR_ARG, "cody_densities: null argument");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpyAsync(c->d_tmp, coeffs, sizeof(double2) * c->nC, hipMemcpyHostToDevice, c->stream));
-  launch_ci_density(c->d_tmp, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can,
-                    c->d_rho2p, c->d_reginv, c->stream);
+  launch_ci_density(c->d_tmp, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_reginv,
+                    c->stream);
   HIPCHK(hipGetLastError());
   const size_t n2 = (size_t)c->nO * c->nO, n4 = n2 * n2;
   if (rdm1) HIPCHK(hipMemcpyAsync(rdm1, c->d_rho1, sizeof(double2) * n2, hipMemcpyDeviceToHost, c->stream));
@@ -632,6 +687,100 @@ int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* nam
   return CODY_OK;
 }
 
+// ---------------------------------------------------------------- batches of instances
+int cody_batch_create(cody_batch** out, int n, cody_ctx* const* ctxs) {
+  if (!out || n < 1 || !ctxs) return fail(CODY_ERR_ARG, "cody_batch_create: bad arguments");
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return fail(CODY_ERR_ARG, "cody_batch_create: null context");
+    if (ctxs[i]->device != ctxs[0]->device) return fail(CODY_ERR_ARG, "cody_batch_create: contexts on different devices");
+    for (int k = 0; k < i; ++k)
+      if (ctxs[k] == ctxs[i]) return fail(CODY_ERR_ARG, "cody_batch_create: a context appears twice");
+  }
+  auto* b = new cody_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  if (hipSetDevice(ctxs[0]->device) != hipSuccess || hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+    delete b;
+    return fail(CODY_ERR_HIP, "cody_batch_create: stream / event creation failed");
+  }
+  for (int k = 0; k < cody_batch::kChains; ++k)
+    if (hipStreamCreateWithFlags(&b->chain[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) != hipSuccess) {
+      b->release();
+      delete b;
+      return fail(CODY_ERR_HIP, "cody_batch_create: stream / event creation failed");
+    }
+  *out = b;
+  return CODY_OK;
+}
+
+int cody_batch_destroy(cody_batch* b) {
+  if (!b) return CODY_OK;
+  hipSetDevice(b->ctx[0]->device);
+  b->release();
+  delete b;
+  return CODY_OK;
+}
+
+int cody_batch_time_derivative_dev(cody_batch* b, void* const* d_states, void* const* d_dstates, double t,
+                                   void* stream) {
+  if (!b || !d_states || !d_dstates) return fail(CODY_ERR_ARG, "cody_batch_time_derivative_dev: null argument");
+  const int n = static_cast<int>(b->ctx.size());
+  for (int i = 0; i < n; ++i) {
+    if (!d_states[i] || !d_dstates[i] || d_states[i] == d_dstates[i])
+      return fail(CODY_ERR_ARG, "cody_batch_time_derivative_dev: bad state pointers");
+    CHK(check_ready(b->ctx[i]));
+  }
+  HIPCHK(hipSetDevice(b->ctx[0]->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b->cs;
+  std::vector<const void*> key;
+  for (int i = 0; i < n; ++i) {
+    key.push_back(d_states[i]);
+    key.push_back(d_dstates[i]);
+  }
+  bool stale = !b->exec || key != b->key;
+  for (int i = 0; i < n && !stale; ++i) stale = b->ctx[i]->generation != b->gen[i];
+  if (stale) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    // one graph: the instances' right-hand sides on kChains parallel chains, each instance with
+    // all of its launches on its chain's queue
+    HIPCHK(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
+    hipEventRecord(b->fork, b->cs);
+    for (int k = 0; k < cody_batch::kChains; ++k) hipStreamWaitEvent(b->chain[k], b->fork, 0);
+    int r = CODY_OK;
+    for (int i = 0; i < n && r == CODY_OK; ++i) {
+      cody_ctx* c = b->ctx[i];
+      const bool oq = c->one_queue;
+      c->one_queue = true;
+      c->capturing = true;
+      r = enqueue_rhs(c, static_cast<const double2*>(d_states[i]), static_cast<double2*>(d_dstates[i]), t,
+                      kModeFull, b->chain[i % cody_batch::kChains], nullptr, nullptr);
+      c->capturing = false;
+      c->one_queue = oq;
+    }
+    for (int k = 0; k < cody_batch::kChains; ++k) {
+      hipEventRecord(b->join[k], b->chain[k]);
+      hipStreamWaitEvent(b->cs, b->join[k], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cs, &g);
+    if (r != CODY_OK || e != hipSuccess || !g || hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      b->exec = nullptr;
+      return r != CODY_OK ? r : fail(CODY_ERR_HIP, "cody_batch_time_derivative_dev: graph capture failed");
+    }
+    hipGraphDestroy(g);
+    b->key = key;
+    b->gen.resize(n);
+    for (int i = 0; i < n; ++i) b->gen[i] = b->ctx[i]->generation;
+  }
+  for (int i = 0; i < n; ++i)
+    if (b->ctx[i]->nLaser > 0) launch_set_scalar(b->ctx[i]->d_efield, laser_field(b->ctx[i], t), st);
+  HIPCHK(hipGraphLaunch(b->exec, st));
+  return CODY_OK;
+}
+
 int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   if (!c || iters < 1 || !avg_ms) return fail(CODY_ERR_ARG, "cody_profile_contraction: bad arguments");
   CHK(check_ready(c));
@@ -656,7 +805,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
         launch_mf_bf16x3(c->d_Vbf, c->d_Sbf, c->d_WT, c->nGp, c->Pc, st);
         break;
       default:  // the contraction kernel alone (its partial sum k_wsum is not timed)
-        launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, nullptr, c->nGp, c->Pc, st);
+        launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, nullptr, c->nGp, c->Pc, c->wsplit, st);
     }
   }
   HIPCHK(hipEventRecord(e1, st));

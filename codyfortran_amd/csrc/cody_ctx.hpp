@@ -42,9 +42,11 @@ struct PropBuffers {
   std::vector<RhsGraph> graphs;
   bool graphs_on = true;
   bool capturing = false;  // enqueue_rhs is being captured (E(t) is then set per replay)
+  unsigned generation = 0;  // bumped whenever captured launches become stale (batches check it)
   void drop_graphs() {
     for (auto& g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
+    ++generation;
   }
 
   void release() {
@@ -61,13 +63,15 @@ struct PropBuffers {
 struct cody_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  // side stream (default priority): CI densities, one-body part and h1 while the contraction runs,
-  // then the Gram matrix and the CI H-apply while the orbital correlation runs
+  // side stream (default priority): CI densities, R and the one-body part while the contraction
+  // runs, then the CI part while the orbital correlation runs
   hipStream_t stream_ci = nullptr;
   hipEvent_t ev_fork = nullptr, ev_a = nullptr, ev_w = nullptr, ev_ci = nullptr;
+  bool one_queue = false;  // every launch of a right-hand side on the caller's stream
   // sizes
   int nG = 0, nGp = 0, N = 0, nO = 0, nC = 0, nS = 0, P = 0, Pc = 0;
   int nsplit = 1, nxs = 1, nPart = 1;
+  int wsplit = 4;  // k split of the FP64 contraction (cody_autotune may change it)
   double reg = 0.0;
   int precision = 0;
   // setup flags
@@ -88,9 +92,9 @@ struct cody_ctx {
   double *d_Gpart = nullptr, *d_G = nullptr, *d_energy = nullptr;
   double* d_efield = nullptr;  // E(t) of the dipole coupling, read by the one-body kernel
   double2 *d_state = nullptr, *d_tmp = nullptr, *d_tmp2 = nullptr;
-  double2 *d_rho1 = nullptr, *d_rho2 = nullptr, *d_rho2p = nullptr, *d_reginv = nullptr;
+  double2 *d_rho1 = nullptr, *d_rho2 = nullptr, *d_reginv = nullptr;
   double2* d_r2can = nullptr;  // canonical rho2Lin (i1 < i2, j1 < j2)
-  double2 *d_h1 = nullptr, *d_inp = nullptr, *d_hphi = nullptr, *d_dCraw = nullptr;
+  double2 *d_h1 = nullptr, *d_hphi = nullptr, *d_dCraw = nullptr;
   double2 *d_draw = nullptr, *d_part = nullptr, *d_part1 = nullptr;
   PropBuffers prop;
   // contraction operands derived from V (allocated by cody_set_precision): FP64 fragment-tiled
@@ -123,27 +127,23 @@ struct cody_ctx {
     ALLOC(d_ST, sizeof(double) * Pc * nGp);
     ALLOC(d_WT, sizeof(double) * Pc * nGp);
     ALLOC(d_STt, sizeof(double) * Pc * nGp);
-    ALLOC(d_Wpart, sizeof(double) * cody::kWSplit * Pc * nGp);
+    ALLOC(d_Wpart, sizeof(double) * cody::kMaxWSplit * Pc * nGp);
     ALLOC(d_Gpart, sizeof(double) * nxs * Pc * Pc);
     ALLOC(d_G, sizeof(double) * Pc * Pc);
     ALLOC(d_energy, sizeof(double) * 8);
-    ALLOC(d_efield, sizeof(double));
-    if (hipMemset(d_efield, 0, sizeof(double)) != hipSuccess) return cody::set_error(2, "hipMemset failed");
+    ALLOC(d_efield, sizeof(double));  // (every allocation starts zero-filled: E = 0)
     ALLOC(d_state, sizeof(double2) * nS);
     ALLOC(d_tmp, sizeof(double2) * nS);
     ALLOC(d_tmp2, sizeof(double2) * nS);
     ALLOC(d_rho1, sizeof(double2) * n2);
+    // entries of rho2 with a repeated index pair stay zero (from the allocation's fill): the
+    // kernels never write them
     ALLOC(d_rho2, sizeof(double2) * n4);
-    // entries of rho2 with a repeated index pair are zero and never written by the kernels
-    if (hipMemset(d_rho2, 0, sizeof(double2) * n4) != hipSuccess)
-      return cody::set_error(2, "hipMemset of rho2 failed");
     ALLOC(d_r2can, sizeof(double2) * std::max<size_t>(1, (size_t)(nO * (nO - 1) / 2) * (nO * (nO - 1) / 2)));
-    ALLOC(d_rho2p, sizeof(double2) * n4);
     ALLOC(d_reginv, sizeof(double2) * n2);
     ALLOC(d_h1, sizeof(double2) * n2);
     ALLOC(d_hphi, sizeof(double2) * nG * nO);
     ALLOC(d_dCraw, sizeof(double2) * nC);
-    ALLOC(d_inp, sizeof(double2) * n2);
     ALLOC(d_draw, sizeof(double2) * nG * nO);
     ALLOC(d_part, sizeof(double2) * nPart * n2);
     ALLOC(d_part1, sizeof(double2) * ((nG + 255) / 256) * n2);
@@ -193,16 +193,18 @@ struct cody_ctx {
   std::vector<RhsGraph> graphs;
   bool graphs_on = true;
   bool capturing = false;  // enqueue_rhs is being captured (E(t) is then set per replay)
+  unsigned generation = 0;  // bumped whenever captured launches become stale (batches check it)
   void drop_graphs() {
     for (auto& g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
+    ++generation;
   }
 
   void release() {
     drop_graphs();
     void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart,
                     d_Gpart, d_G,  d_energy, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
-                    d_rho2p, d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_inp, d_draw, d_part, d_part1, ci_blob};
+                    d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, ci_blob};
     for (void* p : ptrs)
       if (p) hipFree(p);
     prop.release();
@@ -210,12 +212,39 @@ struct cody_ctx {
     for (void* p : lp)
       if (p) hipFree(p);
     if (ev_fork) hipEventDestroy(ev_fork);
-    if (ev_ci) hipEventDestroy(ev_ci);
     if (ev_a) hipEventDestroy(ev_a);
     if (ev_w) hipEventDestroy(ev_w);
+    if (ev_ci) hipEventDestroy(ev_ci);
     if (stream_ci) hipStreamDestroy(stream_ci);
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
     stream_ci = nullptr;
+  }
+};
+
+// Several contexts whose right-hand sides are replayed as one captured graph.
+struct cody_batch {
+  static constexpr int kChains = 4;  // parallel chains (the device exposes four hardware queues)
+  std::vector<cody_ctx*> ctx;
+  std::vector<unsigned> gen;
+  std::vector<const void*> key;
+  hipStream_t cs = nullptr;
+  hipStream_t chain[kChains] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kChains] = {};
+  hipGraphExec_t exec = nullptr;
+  void release() {
+    if (exec) hipGraphExecDestroy(exec);
+    exec = nullptr;
+    for (int k = 0; k < kChains; ++k) {
+      if (join[k]) hipEventDestroy(join[k]);
+      if (chain[k]) hipStreamDestroy(chain[k]);
+      join[k] = nullptr;
+      chain[k] = nullptr;
+    }
+    if (fork) hipEventDestroy(fork);
+    if (cs) hipStreamDestroy(cs);
+    fork = nullptr;
+    cs = nullptr;
   }
 };

@@ -209,19 +209,47 @@ __global__ __launch_bounds__(256) void k_onebody(const double2* __restrict__ orb
 template <int NO>
 __global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orbs, const double* __restrict__ w,
                                                  const double2* __restrict__ hphi, const double* __restrict__ WT,
-                                                 const double2* __restrict__ rho2p, int nG, int nGp, int mode,
-                                                 double2* __restrict__ draw, double2* __restrict__ part) {
+                                                 const double2* __restrict__ r2can, const double2* __restrict__ reginv,
+                                                 int nG, int nGp, int mode, double2* __restrict__ draw,
+                                                 double2* __restrict__ part) {
   constexpr int n2 = NO * NO, n3 = n2 * NO, NP = NO * (NO + 1) / 2, PW = (NP + 3) / 4;
+  constexpr int np = NO * (NO - 1) / 2, nCan = np * np, kNone = -(n2 + 1);
+  __shared__ double2 s2[nCan > 0 ? nCan : 1];
   __shared__ double2 sR[n3];
+  __shared__ double2 sRj[NO];
+  __shared__ int sPair[n2];  // strict-pair index of {a, b}; -1 - index when a > b; kNone when a = b
   __shared__ double2 sCorr[4][64];
   __shared__ double2 sRed[NO];
   const int j = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x = blockIdx.x * 64 + lane;
-  // sR[k1 + NO (k + NO l)] = rho2'(k1, k, j, l)
+  // canonical rho2 (i1 < i2, j1 < j2), column j of R and the pair table
+  for (int e = threadIdx.x; e < nCan; e += 256) s2[e] = r2can[e];
+  for (int e = threadIdx.x; e < n2; e += 256) {
+    const int a = e % NO, b = e / NO;
+    sPair[e] = a == b ? kNone
+                      : (a < b ? a * NO - (a * (a + 1)) / 2 + (b - a - 1) : -1 - (b * NO - (b * (b + 1)) / 2 + (a - b - 1)));
+  }
+  if (threadIdx.x < NO) sRj[threadIdx.x] = reginv[threadIdx.x + j * NO];
+  __syncthreads();
+  // sR[k1 + NO (k + NO l)] = rho2'(k1, k, j, l) = sum_m rho2(k1, k, m, l) R(m, j)
+  // (S_Method_Mb_OrbBased.f90:746-753)
   for (int e = threadIdx.x; e < n3; e += 256) {
-    const int k1k = e % n2, l = e / n2;
-    sR[e] = rho2p[k1k + n2 * j + n2 * NO * l];
+    const int ab = e % n2, l = e / n2;
+    const int pab = sPair[ab];
+    double2 acc = cmk(0, 0);
+    if (pab != kNone) {
+      const int pa = pab >= 0 ? pab : -1 - pab;
+      const double sa = pab >= 0 ? 1.0 : -1.0;
+#pragma unroll
+      for (int m = 0; m < NO; ++m) {
+        const int pcd = sPair[m + NO * l];
+        if (pcd == kNone) continue;
+        const int pc = pcd >= 0 ? pcd : -1 - pcd;
+        cfma(acc, cscale(s2[pa + np * pc], pcd >= 0 ? sa : -sa), sRj[m]);
+      }
+    }
+    sR[e] = acc;
   }
   const bool in = x < nG;
   const int xc = in ? x : nG - 1;
@@ -283,17 +311,19 @@ __global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orb
 }
 
 // ------------------------------------------------------------------ projection + scaling
-// dphi_j = -i (draw_j - sum_i phi_i inp(i,j))  (Grid_ProjectOnSubspace, src/Grid/S_Grid.f90:170-203)
-// With dC non-null the last workgroup applies the CI projection and scaling
-// dC = -i (dCraw - <C|dCraw> C)  (src/Coeffs/S_Coeffs.f90:105-111).
+// dphi_j = -i (draw_j - sum_i phi_i inp(i,j))  (Grid_ProjectOnSubspace, src/Grid/S_Grid.f90:170-203),
+// inp = sum_b part[b] over the orbital kernel's per-workgroup partials (summed by every workgroup
+// into LDS, in fixed order).  With dC non-null the last workgroup applies the CI projection and
+// scaling dC = -i (dCraw - <C|dCraw> C)  (src/Coeffs/S_Coeffs.f90:105-111).
 template <int NO>
 __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orbs,
                                                  const double2* __restrict__ draw,
-                                                 const double2* __restrict__ inp, int nG,
+                                                 const double2* __restrict__ part, int nPart, int nG,
                                                  int project, double2* __restrict__ dorbs,
                                                  const double2* __restrict__ C,
                                                  const double2* __restrict__ dCraw, int nC,
                                                  double2* __restrict__ dC) {
+  constexpr int n2 = NO * NO, G = n2 >= 256 ? 1 : 256 / n2;
   if (dC && blockIdx.x == gridDim.x - 1) {
     __shared__ double red[2][4];
     double2 ov = cmk(0, 0);
@@ -309,6 +339,25 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
     for (int i = threadIdx.x; i < nC; i += blockDim.x) dC[i] = mul_minus_i(csub(dCraw[i], cmul(ov, C[i])));
     return;
   }
+  __shared__ double2 sG[G][n2];
+  __shared__ double2 sInp[n2];
+  if (project) {
+    // G groups of threads sum interleaved partial blocks, then the groups are added in order
+    for (int e = threadIdx.x; e < G * n2; e += blockDim.x) {
+      const int g = e / n2, i = e % n2;
+      double2 acc = cmk(0, 0);
+#pragma unroll 8
+      for (int b = g; b < nPart; b += G) acc = cadd(acc, part[(size_t)b * n2 + i]);
+      sG[g][i] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      double2 acc = sG[0][i];
+      for (int g = 1; g < G; ++g) acc = cadd(acc, sG[g][i]);
+      sInp[i] = acc;
+    }
+    __syncthreads();
+  }
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nG) return;
   double2 phi[NO];
@@ -321,7 +370,7 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
     double2 d = draw[(size_t)j * nG + x];
     if (project) {
 #pragma unroll
-      for (int i = 0; i < NO; ++i) d = csub(d, cmul(phi[i], inp[i + j * NO]));
+      for (int i = 0; i < NO; ++i) d = csub(d, cmul(phi[i], sInp[i + j * NO]));
     }
     dorbs[(size_t)j * nG + x] = mul_minus_i(d);
   }
@@ -394,22 +443,23 @@ void launch_onebody(const double2* orbs, const double* w, const double* x, const
 }
 
 void launch_orbital(const double2* orbs, const double* w, const double2* hphi, const double* WT,
-                    const double2* rho2p, int nG, int nGp, int nO, int mode, double2* draw, double2* part,
-                    hipStream_t st) {
+                    const double2* r2can, const double2* reginv, int nG, int nGp, int nO, int mode,
+                    double2* draw, double2* part, hipStream_t st) {
   dim3 grid(nGp / 64, nO);
-#define CALL_OR(NO) \
-  hipLaunchKernelGGL(k_orbital<NO>, grid, dim3(256), 0, st, orbs, w, hphi, WT, rho2p, nG, nGp, mode, draw, part)
+#define CALL_OR(NO)                                                                                  \
+  hipLaunchKernelGGL(k_orbital<NO>, grid, dim3(256), 0, st, orbs, w, hphi, WT, r2can, reginv, nG, nGp, mode, \
+                     draw, part)
   CODY_NO_SWITCH(nO, CALL_OR)
 #undef CALL_OR
 }
 
-void launch_project(const double2* orbs, const double2* draw, const double2* inp, int nG, int nO, int project,
-                    double2* dorbs, const double2* C, const double2* dCraw, int nC, double2* dC,
+void launch_project(const double2* orbs, const double2* draw, const double2* part, int nPart, int nG, int nO,
+                    int project, double2* dorbs, const double2* C, const double2* dCraw, int nC, double2* dC,
                     hipStream_t st) {
   const int nb = (nG + 255) / 256 + (dC ? 1 : 0);
-#define CALL_PR(NO)                                                                                         \
-  hipLaunchKernelGGL(k_project<NO>, dim3(nb), dim3(256), 0, st, orbs, draw, inp, nG, project, dorbs, C, dCraw, \
-                     nC, dC)
+#define CALL_PR(NO)                                                                                           \
+  hipLaunchKernelGGL(k_project<NO>, dim3(nb), dim3(256), 0, st, orbs, draw, part, nPart, nG, project, dorbs, C, \
+                     dCraw, nC, dC)
   CODY_NO_SWITCH(nO, CALL_PR)
 #undef CALL_PR
 }

@@ -38,10 +38,10 @@ void launch_zero(double2* p, int n, hipStream_t st);
 void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a1, double a2,
                             double damp, double* V, hipStream_t st);
 void launch_ci_density(const double2* C, int nC, int nO, double reg, const DevCiTables& t,
-                       double2* rho1, double2* rho2, double2* r2can, double2* rho2p, double2* reginv,
-                       hipStream_t st);
+                       double2* rho1, double2* rho2, double2* r2can, double2* reginv, hipStream_t st);
 void launch_sum_parts(const double2* part, int nPart, int stride, int n, double2* out, hipStream_t st);
-void launch_gram_reduce(const double* Gpart, int nxs, int Pc, double* G, hipStream_t st);
+void launch_gram_reduce(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
+                        int n2, double2* h1, hipStream_t st);
 void launch_ci_apply(const double2* C, int nC, int nO, int Pc, const double* G, const double2* h1,
                      const DevCiTables& t, double2* dCraw, hipStream_t st);
 void launch_ci_energy(int nO, int Pc, const double* G, const double2* h1, const double2* rho1,
@@ -55,15 +55,15 @@ void launch_onebody(const double2* orbs, const double* w, const double* x, const
                     hipStream_t st);
 void launch_set_scalar(double* p, double v, hipStream_t st);
 void launch_orbital(const double2* orbs, const double* w, const double2* hphi, const double* WT,
-                    const double2* rho2p, int nG, int nGp, int nO, int mode, double2* draw, double2* part,
-                    hipStream_t st);
+                    const double2* r2can, const double2* reginv, int nG, int nGp, int nO, int mode,
+                    double2* draw, double2* part, hipStream_t st);
 void launch_slice_rows(const double* X, int len, int nlines, int ns, int tiled, int8_t* out, int* expo,
                        hipStream_t st);
 void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream_t st);
 void launch_split_bf16x3(const double* X, size_t n, __bf16* out, hipStream_t st);
-constexpr int kWSplit = 4;  // k splits of the FP64 contraction (partials buffer depth)
+constexpr int kMaxWSplit = 8;  // largest k split of the FP64 contraction (partials buffer depth)
 void launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
-                     hipStream_t st);
+                     int nsplit, hipStream_t st);
 void launch_tile_f64(const double* V, int nGp, double* Vt, hipStream_t st);
 void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp, int Pc, int nsplit,
                      hipStream_t st);
@@ -72,8 +72,8 @@ void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* 
 void launch_mf_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl, double* WT, int nGp,
                      int Pc, hipStream_t st);
 void launch_mf_bf16x3(const __bf16* A, const __bf16* B, double* WT, int nGp, int Pc, hipStream_t st);
-void launch_project(const double2* orbs, const double2* draw, const double2* inp, int nG, int nO, int project,
-                    double2* dorbs, const double2* C, const double2* dCraw, int nC, double2* dC,
+void launch_project(const double2* orbs, const double2* draw, const double2* part, int nPart, int nG, int nO,
+                    int project, double2* dorbs, const double2* C, const double2* dCraw, int nC, double2* dC,
                     hipStream_t st);
 
 }  // namespace cody

@@ -139,7 +139,8 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double* __restrict__ At,
 // V[16R + l%16][16K + 4(l/16) + 2h + e]; S^T is tiled the same way with R the column tile
 // (k_pair_src).
 //
-// Decomposition: a workgroup owns 64 rows (4 row tiles) and one of kWSplit contiguous k ranges;
+// Decomposition: a workgroup owns 64 rows (4 row tiles) and one of gridDim.y contiguous k ranges
+// (the k split, 1..kMaxWSplit, chosen per shape by cody_autotune);
 // its waves are (row tile r, k sub-range h).  Per 16-wide k block the S^T fragments of both k
 // sub-ranges (KH x NT x 2 KB) are fetched once by the whole workgroup, 1 KB per wave and chunk,
 // and shared through LDS, so S^T crosses L2 once per 64 rows (it is the S^T traffic, not V, that
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double* __restrict__ At,
 // order, so operands fetched at different distances would make every wait cover the farthest.
 //
 // The k sub-ranges are summed in LDS and each workgroup stores its k-split partial; k_wsum adds
-// the kWSplit partials in fixed order (bit-reproducible).  The partials are combined by a second
+// the partials in fixed order (bit-reproducible).  With a single k range the kernel writes W^T.  The partials are combined by a second
 // kernel rather than by the last workgroup of a row group: the device-scope fences that a
 // last-arriver scheme needs write back and invalidate whole L2 caches on this GPU, which cost
 // more than the extra launch.
@@ -166,8 +167,8 @@ __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
   __shared__ double lds[LDS];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
   const int r = wave & 3, h = wave >> 2;
-  const int nK = nGp >> 4, nbw = nK / (kWSplit * KH);
-  const int Kb = blockIdx.y * (nK / kWSplit);
+  const int nK = nGp >> 4, nbw = nK / (gridDim.y * KH);
+  const int Kb = blockIdx.y * (nK / gridDim.y);
   const double* ab = Vt + ((size_t)(blockIdx.x * 4 + r) * nK + Kb + h * nbw) * 256 + 2 * lane;
   // this wave's S^T chunks: chunk c = wave + W m = (hh NT + t) 2 + half of every block (chunks
   // past NCH are clamped onto the last one for the load and not written)
@@ -256,21 +257,20 @@ __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
   // store this k split's 64 x (16 NT) partial transposed: 64 consecutive rows per column
   const int row0 = blockIdx.x * 64;
   const size_t plane = (size_t)NT * 16 * nGp;
-  double* Wp = Wpart + blockIdx.y * plane;
+  double* Wp = Wpart + blockIdx.y * plane;  // (the plane of a single k range is W^T itself)
   for (int e = threadIdx.x; e < 64 * NT * 16; e += blockDim.x) {
     const int c = e >> 6, rl = e & 63;
     Wp[(size_t)c * nGp + row0 + rl] = lds[(rl >> 4) * 16 * LD + (rl & 15) * LD + c];
   }
 }
 
-// W^T = sum of the kWSplit partials, in fixed order (two doubles per thread).
-__global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, size_t plane,
+// W^T = sum of the nsplit partials, in fixed order (two doubles per thread).
+__global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, int nsplit, size_t plane,
                                               double* __restrict__ WT) {
   const size_t i = 2 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= plane) return;
   d2_t v = *reinterpret_cast<const d2_t*>(Wpart + i);
-#pragma unroll
-  for (int ks = 1; ks < kWSplit; ++ks) v += *reinterpret_cast<const d2_t*>(Wpart + ks * plane + i);
+  for (int ks = 1; ks < nsplit; ++ks) v += *reinterpret_cast<const d2_t*>(Wpart + ks * plane + i);
   *reinterpret_cast<d2_t*>(WT + i) = v;
 }
 
@@ -491,18 +491,21 @@ __global__ __launch_bounds__(256) void k_mf_bf16x3(const __bf16* __restrict__ A,
     default: break;               \
   }
 
-// nGp must be a multiple of 256; Wpart holds kWSplit x Pc x nGp doubles.
-// With WT null only the partials are formed (k_wsum is not launched).
+// nGp must be a multiple of 256 and nsplit one of 1, 2, 4, 8; Wpart holds nsplit x Pc x nGp doubles
+// (unused for nsplit = 1, when the kernel writes WT).  With WT null only the partials are formed
+// (k_wsum is not launched).
 void launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
-                     hipStream_t st) {
-#define CALL_W(NT)                                                                                        \
-  hipLaunchKernelGGL(k_mf_f64_w<NT>, dim3(nGp / 64, kWSplit), dim3((NT) <= 5 ? 512 : 256), 0, st, Vt, STt, \
-                     nGp, Wpart)
+                     int nsplit, hipStream_t st) {
+  double* out = (nsplit == 1 && WT) ? WT : Wpart;
+#define CALL_W(NT)                                                                                    \
+  hipLaunchKernelGGL(k_mf_f64_w<NT>, dim3(nGp / 64, nsplit), dim3((NT) <= 5 ? 512 : 256), 0, st, Vt, STt, \
+                     nGp, out)
   CODY_NT_SWITCH2(Pc, CALL_W)
 #undef CALL_W
-  if (!WT) return;
+  if (!WT || nsplit == 1) return;
   const size_t plane = (size_t)Pc * nGp;
-  hipLaunchKernelGGL(k_wsum, dim3((unsigned)((plane / 2 + 255) / 256)), dim3(256), 0, st, Wpart, plane, WT);
+  hipLaunchKernelGGL(k_wsum, dim3((unsigned)((plane / 2 + 255) / 256)), dim3(256), 0, st, Wpart, nsplit, plane,
+                     WT);
 }
 
 // Gpart[s][c2][c1] = sum_{x in split s} ST[c1][x] WT[c2][x]; nsplit * 64 must divide nGp.

@@ -57,6 +57,10 @@ def load():
         "cody_set_laser": (i, [vp, i, dp, d]),
         "cody_set_precision": (i, [vp, i]),
         "cody_set_graphs": (i, [vp, i]),
+        "cody_autotune": (i, [vp, C.POINTER(C.c_int)]),
+        "cody_batch_create": (i, [C.POINTER(vp), i, C.POINTER(vp)]),
+        "cody_batch_destroy": (i, [vp]),
+        "cody_batch_time_derivative_dev": (i, [vp, C.POINTER(vp), C.POINTER(vp), d, vp]),
         "cody_upload_state": (i, [vp, vp]),
         "cody_download_state": (i, [vp, vp]),
         "cody_state_device_ptr": (i, [vp, C.POINTER(vp)]),
@@ -177,6 +181,13 @@ class Context:
     def set_precision(self, mode):
         _check(self.lib.cody_set_precision(self.h, int(mode)))
 
+    def autotune(self, split: int = 0) -> int:
+        """Time the FP64 contraction for each k split and keep the fastest (or set `split`);
+        returns the split in use."""
+        k = C.c_int(int(split))
+        _check(self.lib.cody_autotune(self.h, C.byref(k)))
+        return k.value
+
     def set_graphs(self, enable: bool):
         """Replay captured right-hand sides (True, the default) or launch every kernel."""
         _check(self.lib.cody_set_graphs(self.h, int(bool(enable))))
@@ -269,3 +280,33 @@ class Context:
         _check(self.lib.cody_profile_rhs(self.h, iters, n, ms, names, 1024, C.byref(nk)))
         labels = names.value.decode().split(";")
         return {labels[k] if k < len(labels) else f"k{k}": ms[k] for k in range(nk.value)}
+
+
+class Batch:
+    """Independent contexts whose right-hand sides are replayed as one captured graph."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        self.lib = load()
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        _check(self.lib.cody_batch_create(C.byref(h), len(self.contexts), arr))
+        self.h = h
+
+    def time_derivative_dev(self, d_states, d_dstates, t=0.0, stream: int = 0):
+        n = len(self.contexts)
+        assert len(d_states) == n and len(d_dstates) == n
+        s = (C.c_void_p * n)(*d_states)
+        d = (C.c_void_p * n)(*d_dstates)
+        _check(self.lib.cody_batch_time_derivative_dev(self.h, s, d, float(t), C.c_void_p(stream or None)))
+
+    def close(self):
+        if self.h:
+            self.lib.cody_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

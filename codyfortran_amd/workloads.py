@@ -59,6 +59,25 @@ WORKLOADS = {
 }
 
 
+def batched_workloads(n: int = 64, seed: int = 3, sizes=(512, 1024, 2048, 4096), orbitals=(4, 6, 8, 12)):
+    """Seeded independent instances for batched evaluation: N_g drawn uniformly from `sizes`
+    (realised as the nearest FEDVR size with 6 GLL points per element and both ends kept:
+    511, 1026, 2046, 4096), M uniformly from `orbitals`, N = M/2 spin-polarised fermions, and one
+    soft-Coulomb softening a ~ U(0.5, 2.0) for both the external potential and the interaction.
+    The elements have the length of the 4096-point grid (400/819), centred on x = 0."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        ng = int(rng.choice(sizes))
+        M = int(rng.choice(orbitals))
+        a = float(rng.uniform(0.5, 2.0))
+        nE = int(round((ng - 1) / 5))
+        half = 0.5 * nE * 400.0 / 819.0
+        out.append(Workload(f"batch{i}_g{nE * 5 + 1}_m{M}", -half, half, nE, 6, M // 2, M,
+                            pot_soft1=a, int_soft1=a))
+    return out
+
+
 def orthonormalize(orbs: np.ndarray, w: np.ndarray) -> np.ndarray:
     """Modified Gram-Schmidt, two sweeps, in the metric <f|g> = sum conj(f) g w."""
     o = np.array(orbs, dtype=np.complex128, order="F", copy=True)

@@ -38,6 +38,7 @@ extern "C" {
                                    products with int32 accumulation, FP64 recombination */
 
 typedef struct cody_ctx cody_ctx;
+typedef struct cody_batch cody_batch;
 
 const char* cody_last_error(void);
 
@@ -94,6 +95,11 @@ int cody_set_precision(cody_ctx* ctx, int mode);
  * RK4 stages capture the launches of one right-hand side per (state, dState, mode) into a HIP
  * graph on first use and replay it; 0 launches every kernel individually. */
 int cody_set_graphs(cody_ctx* ctx, int enable);
+/* Project autotuner (no counterpart in the reference, which has no tiled kernels): times the FP64
+ * contraction of this context's shape for each k split (1, 2, 4, 8 contiguous k ranges per row
+ * group) and keeps the fastest; *wsplit (optional) receives the choice.  The timing uses the
+ * context's own state.  With *wsplit > 0 on entry that split is set without timing. */
+int cody_autotune(cody_ctx* ctx, int* wsplit);
 
 /* ---------------------------------------------------------------- state transfer */
 /* The context's device-resident state (Method_state, src/Method/M_Method.f90:47). */
@@ -146,6 +152,19 @@ int cody_rk4(cody_ctx* ctx, double t0, double t1, int full);
  * (S_Propagator_SplitStep_Order2.f90:52-77). */
 int cody_strang(cody_ctx* ctx, double t0, double t1, int krylovDim, double tol, int maxRestarts,
                 int useMetric, int* converged);
+
+/* ---------------------------------------------------------------- batches of instances */
+/* Independent instances (each its own context: grid, orbitals, interaction) evaluated together.
+ * No counterpart in the reference, which runs one Method per process; this is the GPU form of
+ * calling Method_TimeDerivative (S_Method_Mb_OrbBased_Mctdhx.f90:101-149) for every instance:
+ * all right-hand sides are captured into one HIP graph (four parallel chains) on first use for
+ * a set of state pointers and replayed afterwards.  The contexts must outlive the batch. */
+int cody_batch_create(cody_batch** batch, int n, cody_ctx* const* ctxs);
+int cody_batch_destroy(cody_batch* batch);
+/* dStates[i] = TimeDerivative(states[i], t) of instance i, device pointers, enqueued on `stream`
+ * (0: the batch's own stream). */
+int cody_batch_time_derivative_dev(cody_batch* batch, void* const* d_states, void* const* d_dstates,
+                                   double t, void* stream);
 
 /* ---------------------------------------------------------------- measurement */
 /* Runs `iters` right-hand sides of the context's state and returns the average duration of each
