@@ -124,6 +124,45 @@ module M_CodyHip
       integer(c_int) :: r
     end function
 
+    function cody_set_graphs(ctx, enable) bind(C, name="cody_set_graphs") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+      integer(c_int) :: r
+    end function
+
+    function cody_autotune(ctx, wsplit) bind(C, name="cody_autotune") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: wsplit
+      integer(c_int) :: r
+    end function
+
+    !> Independent instances evaluated as one captured graph: ctxs(1:n) are context handles,
+    !> dStates/states device pointers, one per instance.
+    function cody_batch_create(batch, n, ctxs) bind(C, name="cody_batch_create") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr) :: batch
+      integer(c_int), value :: n
+      type(c_ptr) :: ctxs(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_batch_destroy(batch) bind(C, name="cody_batch_destroy") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: batch
+      integer(c_int) :: r
+    end function
+
+    function cody_batch_time_derivative_dev(batch, states, dstates, t, stream) &
+        bind(C, name="cody_batch_time_derivative_dev") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: batch, stream
+      type(c_ptr) :: states(*), dstates(*)
+      real(c_double), value :: t
+      integer(c_int) :: r
+    end function
+
     function cody_strang(ctx, t0, t1, krylovDim, tol, maxRestarts, useMetric, converged) &
         bind(C, name="cody_strang") result(r)
       import :: c_ptr, c_int, c_double
