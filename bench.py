@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "MCTDHF RHS evals/s (M=8, 4096 pts): split-FP32 MFMA vs FP64, rel err ≤1e-9"
 PEAK = {"fp64": 78.6, "split_fp32": 157.3, "bf16x3": 2500.0, "int8_sliced": 5000.0}  # dense MFMA, MI355X
 PREC_CODE = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "int8_sliced": 3}
-KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8"}
+KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_w"}
 TOL = 1e-9
 
 

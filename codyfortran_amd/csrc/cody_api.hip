@@ -97,14 +97,24 @@ int prepare_precision(cody_ctx* c) {
   return CODY_OK;
 }
 
-// Row-major S^T is needed only by the reduced-precision contractions (FP64 reads the tiled copy).
-double* plain_st(cody_ctx* c) { return c->precision == CODY_PREC_FP64 ? nullptr : c->d_ST; }
+// The int8 form with at most 80 pair columns slices the tiled S^T (per-line maxima from the
+// pair-source kernel); the other reduced-precision forms read row-major S^T.
+bool int8_tiled(const cody_ctx* c) { return c->precision == CODY_PREC_INT8_SLICED && c->Pc <= 80; }
+double* plain_st(cody_ctx* c) { return (c->precision == CODY_PREC_FP64 || int8_tiled(c)) ? nullptr : c->d_ST; }
+unsigned long long* st_max(cody_ctx* c) { return int8_tiled(c) ? c->d_smax : nullptr; }
 
 // Enqueue W = V S for the context's precision (ST already formed); returns the kernel name.
 const char* enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark) {
   const size_t ns = (size_t)c->Pc * c->nGp;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
+      if (int8_tiled(c)) {  // 64-row kernel: S slices fragment-tiled, four k ranges
+        launch_slice_s_tiled(c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
+        mark("k_slice_s_tiled");
+        launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->d_WT, c->nGp, c->Pc, 4, c->d_smax, st);
+        mark("k_mf_i8_w + k_wsum");
+        return "k_mf_i8_w";
+      }
       launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, 0, c->d_Si8, c->d_eS, st);
       mark("k_slice_rows");
       launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
@@ -424,7 +434,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   // pair sources first, then the fork, then the mean fields W = V S.  (The pair-source kernel is
   // the only root of a captured graph and the contraction is created before the side-stream work,
   // so that a graph executor walking nodes in creation order starts the contraction first.)
-  launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st);
+  launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
   mark("k_pair_src");
   if (split) hipEventRecord(c->ev_fork, st);
   enqueue_contraction(c, st, mark);
@@ -518,7 +528,7 @@ int cody_autotune(cody_ctx* c, int* wsplit) {
     return CODY_OK;
   }
   if (c->precision == CODY_PREC_FP64) {
-    launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st);
+    launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -607,7 +617,7 @@ int cody_meanfield_w(cody_ctx* c, const double* orbs, double* W) {
   HIPCHK(hipSetDevice(c->device));
   double2* d_orbs = c->d_tmp + c->nC;
   HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
-  launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, c->stream);
+  launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->stream);
   enqueue_contraction(c, c->stream, [](const char*) {});
   HIPCHK(hipGetLastError());
   std::vector<double> wt((size_t)c->Pc * c->nGp);
@@ -786,7 +796,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   CHK(check_ready(c));
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st);
+  launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
   auto none = [](const char*) {};
   enqueue_contraction(c, st, none);  // warm-up
   hipEvent_t e0, e1;
@@ -795,8 +805,11 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   HIPCHK(hipEventRecord(e0, st));
   for (int i = 0; i < iters; ++i) {
     switch (c->precision) {
-      case CODY_PREC_INT8_SLICED:
-        launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
+      case CODY_PREC_INT8_SLICED:  // the contraction kernel alone
+        if (int8_tiled(c))
+          launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, nullptr, c->nGp, c->Pc, 4, nullptr, st);
+        else
+          launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
         break;
       case CODY_PREC_SPLIT_FP32:
         launch_mf_f32x3(c->d_Vhi, c->d_Vlo, c->d_Shi, c->d_Slo, c->d_WT, c->nGp, c->Pc, st);

@@ -89,6 +89,7 @@ struct cody_ctx {
   double *d_x = nullptr, *d_w = nullptr, *d_vext = nullptr, *d_kref = nullptr;
   double *d_V = nullptr, *d_ST = nullptr, *d_WT = nullptr;
   double *d_STt = nullptr, *d_Wpart = nullptr;  // fragment-tiled S^T, k-split partials of W^T
+  unsigned long long* d_smax = nullptr;          // per-line max |S^T| (int8 form), cleared per use
   double *d_Gpart = nullptr, *d_G = nullptr, *d_energy = nullptr;
   double* d_efield = nullptr;  // E(t) of the dipole coupling, read by the one-body kernel
   double2 *d_state = nullptr, *d_tmp = nullptr, *d_tmp2 = nullptr;
@@ -127,6 +128,7 @@ struct cody_ctx {
     ALLOC(d_ST, sizeof(double) * Pc * nGp);
     ALLOC(d_WT, sizeof(double) * Pc * nGp);
     ALLOC(d_STt, sizeof(double) * Pc * nGp);
+    ALLOC(d_smax, sizeof(unsigned long long) * Pc);
     ALLOC(d_Wpart, sizeof(double) * cody::kMaxWSplit * Pc * nGp);
     ALLOC(d_Gpart, sizeof(double) * nxs * Pc * Pc);
     ALLOC(d_G, sizeof(double) * Pc * Pc);
@@ -202,7 +204,7 @@ struct cody_ctx {
 
   void release() {
     drop_graphs();
-    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart,
+    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_smax,
                     d_Gpart, d_G,  d_energy, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
                     d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, ci_blob};
     for (void* p : ptrs)

@@ -93,12 +93,16 @@ __global__ __launch_bounds__(256) void k_pair_src(const double2* __restrict__ or
 
 // The same S^T fragment-tiled for the FP64 kernels, one output element per thread (coalesced):
 // STt[(t nK + K) 256 + 128 h + 2 l + e] = S^T[c = 16 t + l % 16][x = 16 K + 4 (l / 16) + 2 h + e].
+// A workgroup covers one 16 x 16 fragment; with smax non-null it also raises smax[c] to the
+// largest |S^T[c][x]| of its fragment (bit patterns of non-negative doubles order as integers),
+// which the int8 slicing needs as the per-line scale.
 __global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ orbs,
                                                     const double* __restrict__ w, int nG, int nGp, int nO,
-                                                    int Pc, double* __restrict__ STt) {
-  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                    int Pc, double* __restrict__ STt,
+                                                    unsigned long long* __restrict__ smax) {
+  __shared__ unsigned long long sm[16];
+  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // grid = Pc nGp / 256 exactly
   const int nK = nGp >> 4;
-  if (o >= (size_t)Pc * nGp) return;
   const int e = o & 1, h = (o >> 7) & 1, l = (o >> 1) & 63;
   const size_t blk = o >> 8;
   const int t = (int)(blk / nK), K = (int)(blk % nK);
@@ -115,7 +119,15 @@ __global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ 
     v = (c & 1) ? s.y : s.x;
   }
   STt[o] = v;
+  if (smax) {
+    if (threadIdx.x < 16) sm[threadIdx.x] = 0ull;
+    __syncthreads();
+    atomicMax(&sm[l & 15], (unsigned long long)__double_as_longlong(fabs(v)));
+    __syncthreads();
+    if (threadIdx.x < 16 && sm[threadIdx.x] != 0ull) atomicMax(&smax[16 * t + threadIdx.x], sm[threadIdx.x]);
+  }
 }
+
 // ------------------------------------------------------------------ one-body operator at a point
 // FEDVR weak-form kinetic energy -(1/2m)(1/w_x) sum_e sum_b K(a,b)/h f_b
 // (src/Utils/M_Utils_DerivativeFedvr.f90:121-152): the point with global index g (numbered with
@@ -417,12 +429,13 @@ void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a
     default: break;                 \
   }
 
-// STt (fragment-tiled, always) and, when ST is non-null, S^T row-major.
+// STt (fragment-tiled, always; with the per-line maxima when smax is non-null) and, when ST is
+// non-null, S^T row-major.
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
-                     double* ST, double* STt, hipStream_t st) {
+                     double* ST, double* STt, unsigned long long* smax, hipStream_t st) {
   const size_t n = (size_t)Pc * nGp;
-  hipLaunchKernelGGL(k_pair_src_t, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, orbs, w, nG, nGp, nO,
-                     Pc, STt);
+  hipLaunchKernelGGL(k_pair_src_t, dim3((unsigned)(n / 256)), dim3(256), 0, st, orbs, w, nG, nGp, nO, Pc, STt,
+                     smax);
   if (!ST) return;
 #define CALL_PS(NO)                                                                            \
   hipLaunchKernelGGL(k_pair_src<NO>, dim3((nGp + 255) / 256), dim3(256), 0, st, orbs, w, nG, nGp, \

@@ -49,7 +49,7 @@ void launch_ci_energy(int nO, int Pc, const double* G, const double2* h1, const 
 size_t ci_density_lds_bytes(int nC, int nO);
 int ci_kernels_allow_lds(size_t bytes);
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
-                     double* ST, double* STt, hipStream_t st);
+                     double* ST, double* STt, unsigned long long* smax, hipStream_t st);
 void launch_onebody(const double2* orbs, const double* w, const double* x, const double* vext,
                     const KinLaunch& kl, const double* efield, int nG, int nO, double2* hphi, double2* part1,
                     hipStream_t st);
@@ -67,6 +67,10 @@ void launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double*
 void launch_tile_f64(const double* V, int nGp, double* Vt, hipStream_t st);
 void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp, int Pc, int nsplit,
                      hipStream_t st);
+void launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
+                    double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st);
+void launch_slice_s_tiled(const double* STt, const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out,
+                          int* expo, hipStream_t st);
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,
                   int Pc, hipStream_t st);
 void launch_mf_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl, double* WT, int nGp,

@@ -274,6 +274,124 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, 
   *reinterpret_cast<d2_t*>(WT + i) = v;
 }
 
+// ---------------------------------------------------------------------------------------- int8, 64-row
+// The int8-sliced contraction with the decomposition of k_mf_f64_w: a workgroup owns 64 rows
+// (one 16-row tile per wave) and one of gridDim.y contiguous k ranges.  Both operands are
+// fragment-tiled (k_slice_rows with tiled output): slice p of the 16-line x 64-k block (R, K) is
+// 1 KiB at ((R nK64 + K) NS + p) 1024, lane l holding bytes k = 64K + 16(l/16) .. +15 of line
+// 16R + l%16.  Per 64-wide k block the S slices (NT x NS chunks of 1 KiB) are fetched once per
+// workgroup into LDS; each wave streams its own V slices.  Three register stages: every global
+// load is issued two blocks ahead of its MFMAs (loads complete in issue order).  The steps are
+// written out six at a time so that the register stage (mod 3) and the LDS buffer (mod 2) are
+// compile-time constants.  Products with p + q <= NS - 1 that share the scale 2^{-7(p+q+2)}
+// accumulate exactly in one int32 tile; the diagonals are recombined in FP64 with the row and
+// column exponents, and the workgroup stores its FP64 partial (k_wsum adds the k ranges).
+template <int NT>
+__global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, const int* __restrict__ eA,
+                                                 const int8_t* __restrict__ Bs, const int* __restrict__ fB,
+                                                 int nGp, double* __restrict__ Wpart,
+                                                 unsigned long long* __restrict__ smax) {
+  constexpr int NS = kInt8Slices, W = 4, D = 3, LD = NT * 16 + 1;
+  // the per-line maxima of S^T were consumed by the slicing (earlier on this stream): clear them
+  // for the next right-hand side
+  if (smax && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < NT * 16) smax[threadIdx.x] = 0ull;
+  constexpr int NCH = NT * NS, CPW = (NCH + W - 1) / W;  // 1 KiB S chunks per block, per wave
+  constexpr int BUF = W * CPW * 1024;                    // bytes per LDS buffer
+  constexpr int LDSB = 2 * BUF > 4 * 16 * LD * 8 ? 2 * BUF : 4 * 16 * LD * 8;
+  __shared__ __align__(16) int8_t lds[LDSB];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  const int nK64 = nGp >> 6, nbw = nK64 / gridDim.y;
+  const int Kb = blockIdx.y * nbw;
+  const int8_t* ab = As + ((size_t)(blockIdx.x * 4 + wave) * nK64 + Kb) * NS * 1024 + 16 * lane;
+  auto chunk_src = [&](int m) {  // chunk c = t NS + p of this wave's share
+    const int c = min(wave + W * m, NCH - 1);
+    const int t = c / NS, p = c % NS;
+    return Bs + (((size_t)t * nK64 + Kb) * NS + p) * 1024 + 16 * lane;
+  };
+  i4_t acc[NT][NS];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int d = 0; d < NS; ++d) acc[t][d] = (i4_t){0, 0, 0, 0};
+  i4_t a[D][NS];
+  i4_t stg[D][CPW];
+  auto gload = [&](int u, int blk) {
+    const size_t o = (size_t)min(blk, nbw - 1) * NS * 1024;
+#pragma unroll
+    for (int p = 0; p < NS; ++p) a[u][p] = *reinterpret_cast<const i4_t*>(ab + o + p * 1024);
+#pragma unroll
+    for (int m = 0; m < CPW; ++m) stg[u][m] = *reinterpret_cast<const i4_t*>(chunk_src(m) + o);
+  };
+  auto lwrite = [&](int u, int buf) {
+#pragma unroll
+    for (int m = 0; m < CPW; ++m) *reinterpret_cast<i4_t*>(lds + buf * BUF + (wave + W * m) * 1024 + 16 * lane) = stg[u][m];
+  };
+  auto step = [&](int u, int buf) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      i4_t b[NS];
+#pragma unroll
+      for (int p = 0; p < NS; ++p) b[p] = *reinterpret_cast<const i4_t*>(lds + buf * BUF + (t * NS + p) * 1024 + 16 * lane);
+#pragma unroll
+      for (int d = 0; d < NS; ++d)
+#pragma unroll
+        for (int p = 0; p <= d; ++p)
+          acc[t][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][p], b[d - p], acc[t][d], 0, 0, 0);
+    }
+  };
+  // stage invariant at the top of an iteration (i a multiple of 6): register stages 0, 1 hold
+  // blocks i, i+1 and block i is in LDS buffer 0.  Loads past the last block are clamped onto it.
+#define CODY_I8_STEP(J)                          \
+  {                                              \
+    gload(((J) + 2) % D, i + (J) + 2);           \
+    lwrite(((J) + 1) % D, ((J) + 1) & 1);        \
+    step((J) % D, (J) & 1);                      \
+    __syncthreads();                             \
+  }
+  gload(0, 0);
+  gload(1, 1);
+  lwrite(0, 0);
+  __syncthreads();
+  int i = 0;
+  for (; i + 6 <= nbw; i += 6) {
+    CODY_I8_STEP(0)
+    CODY_I8_STEP(1)
+    CODY_I8_STEP(2)
+    CODY_I8_STEP(3)
+    CODY_I8_STEP(4)
+    CODY_I8_STEP(5)
+  }
+  // remaining nbw - i < 6 blocks (the conditions are uniform over the workgroup)
+  if (i < nbw) CODY_I8_STEP(0)
+  if (i + 1 < nbw) CODY_I8_STEP(1)
+  if (i + 2 < nbw) CODY_I8_STEP(2)
+  if (i + 3 < nbw) CODY_I8_STEP(3)
+  if (i + 4 < nbw) CODY_I8_STEP(4)
+#undef CODY_I8_STEP
+  // FP64 recombination; 32-bit MFMA results: element u of lane l is row 4(l/16) + u
+  double* red = reinterpret_cast<double*>(lds);
+  double* slot = red + wave * 16 * LD;
+  const int rowb = blockIdx.x * 64 + wave * 16;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int fc = fB[t * 16 + rr];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double v = 0.0;
+#pragma unroll
+      for (int d = NS - 1; d >= 0; --d) v += ldexp(static_cast<double>(acc[t][d][u]), -7 * (d + 2));
+      slot[(4 * q + u) * LD + t * 16 + rr] = ldexp(v, eA[rowb + 4 * q + u] + fc);
+    }
+  }
+  __syncthreads();
+  const int row0 = blockIdx.x * 64;
+  double* Wp = Wpart + blockIdx.y * (size_t)NT * 16 * nGp;
+  for (int e = threadIdx.x; e < 64 * NT * 16; e += blockDim.x) {
+    const int c = e >> 6, rl = e & 63;
+    Wp[(size_t)c * nGp + row0 + rl] = red[(rl >> 4) * 16 * LD + (rl & 15) * LD + c];
+  }
+}
+
 // ---------------------------------------------------------------------------------------- int8 slices
 // NS slices per operand; all slice products with p + q <= NS - 1 (0-based) that share the scale
 // 2^{-7(p+q+2)} accumulate into one int32 tile (exact: |sum| < NS 2^24 < 2^31 for K <= 4096).
@@ -517,6 +635,28 @@ void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp
                      Pc, kq, (size_t)Pc * Pc)
   CODY_NT_SWITCH2(Pc, CALL_G)
 #undef CALL_G
+}
+
+// 64-row int8 contraction (Pc <= 80): Bs fragment-tiled; nsplit in {1, 2, 4, 8} dividing nGp / 64
+// (WT is written directly for nsplit = 1).  With WT null only the partials are formed.
+void launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
+                    double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st) {
+  double* out = (nsplit == 1 && WT) ? WT : Wpart;
+#define CALL_I8(NT) \
+  hipLaunchKernelGGL(k_mf_i8_w<NT>, dim3(nGp / 64, nsplit), dim3(256), 0, st, As, eA, Bs, fB, nGp, out, smax)
+  switch (Pc / 16) {
+    case 1: CALL_I8(1); break;
+    case 2: CALL_I8(2); break;
+    case 3: CALL_I8(3); break;
+    case 4: CALL_I8(4); break;
+    case 5: CALL_I8(5); break;
+    default: break;
+  }
+#undef CALL_I8
+  if (!WT || nsplit == 1) return;
+  const size_t plane = (size_t)Pc * nGp;
+  hipLaunchKernelGGL(k_wsum, dim3((unsigned)((plane / 2 + 255) / 256)), dim3(256), 0, st, Wpart, nsplit, plane,
+                     WT);
 }
 
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,
