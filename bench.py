@@ -28,7 +28,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "MCTDHF RHS evals/s (M=8, 4096 pts): split-FP32 MFMA vs FP64, rel err ≤1e-9"
-PEAK = {"fp64": 78.6, "split_fp32": 157.3, "bf16x3": 2500.0, "int8_sliced": 5000.0}  # dense MFMA, MI355X
+# The bound of each contraction form: FP64 and split-FP32 are MFMA-bound (useful FLOP against the
+# dense FP64 / FP32 MFMA peak); the 3xBF16 and int8-sliced forms do their products faster than
+# HBM delivers the 6 bytes per V element (int8: 21 products of 2 N_g^2 2P ops take ~10 us at
+# 5.0 POP/s, the 101 MB of V slices ~12.6 us at 8 TB/s), so they are HBM-bound: algorithmic bytes
+# = V and S operands read once plus W^T written once, against 8 TB/s.
+BOUND = {"fp64": "mfma", "split_fp32": "mfma", "bf16x3": "hbm", "int8_sliced": "hbm"}
+PEAK = {"fp64": 78.6, "split_fp32": 157.3, "bf16x3": 8000.0, "int8_sliced": 8000.0}  # TFLOP/s or GB/s
 PREC_CODE = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "int8_sliced": 3}
 KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_w"}
 TOL = 1e-9
@@ -50,7 +56,45 @@ def parse():
     p.add_argument("--batch", type=int, default=64,
                    help="independent instances of the batched measurement (0: skip it)")
     p.add_argument("--batch-steps", type=int, default=20)
+    p.add_argument("--pmc-probe", action="store_true",
+                   help="short run for hardware-counter passes: a 256 MiB device copy (the unit "
+                        "of the byte counters) and a few launches of the FP64 and int8 "
+                        "contraction kernels; prints no JSON line")
     return p.parse_args()
+
+
+def pmc_traffic(kname: str, workload: str):
+    """HBM bytes per launch of `kname` from the committed hardware-counter summary (rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE passes over `bench.py --pmc-probe`, scaled by a 256 MiB copy);
+    None when no summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    for k, v in d["kernels"].items():
+        if k.split("<")[0] == kname:
+            return v["hbm_bytes"]
+    return None
+
+
+def pmc_probe(torch, wlm, wl):
+    """Launches counted by `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` runs: a plain copy of
+    known size, then the contraction kernels of the FP64 and int8-sliced forms."""
+    dev = torch.device("cuda:0")
+    a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)  # 256 MiB
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ctx, x, w = wlm.make_context(wl)
+    ctx.upload_state(wlm.seeded_state(x, w, wl.N, wl.M, seed=1234))
+    for prec in (PREC_CODE["fp64"], PREC_CODE["int8_sliced"]):
+        ctx.set_precision(prec)
+        ctx.profile_contraction(3)
+    torch.cuda.synchronize()
+    ctx.close()
+    print("pmc probe: 3 copies of 256 MiB, 4 launches of each contraction kernel", flush=True)
 
 
 def batched_measurement(args, torch, wlm, check: bool):
@@ -151,6 +195,9 @@ def main():
 
 
     wl = wlm.WORKLOADS[args.workload]
+    if args.pmc_probe:
+        pmc_probe(torch, wlm, wl)
+        return
     ctx, x, w = wlm.make_context(wl)
     state = wlm.seeded_state(x, w, wl.N, wl.M, seed=1234)
     dev = torch.device("cuda:0")
@@ -187,10 +234,21 @@ def main():
         kernels = ctx.profile_rhs(args.profile_iters)
         kname = KERNEL[prec]
         kms = ctx.profile_contraction(args.profile_iters)
-        ach = flops / (kms * 1e-3) / 1e12 if kms else None
-        roof = {"bound": "mfma", "kernel": kname, "achieved": ach, "peak": PEAK[prec], "unit": "TFLOP/s",
-                "frac": ach / PEAK[prec] if ach else None, "traffic": None, "flop_per_launch": flops,
-                "avg_launch_ms": kms}
+        traffic = pmc_traffic(kname, wl.name)
+        if BOUND[prec] == "mfma":
+            work, unit = flops, "TFLOP/s"
+            ach = flops / (kms * 1e-3) / 1e12 if kms else None
+        else:
+            nGp = 256 * ((wl.nG + 255) // 256)
+            Pc = 16 * ((2 * wl.pairs + 15) // 16)
+            work = 6.0 * nGp * nGp + 6.0 * Pc * nGp + 8.0 * Pc * nGp  # V, S operands (6 B/elem), W^T
+            unit = "GB/s"
+            ach = work / (kms * 1e-3) / 1e9 if kms else None
+        roof = {"bound": BOUND[prec], "kernel": kname, "achieved": ach, "peak": PEAK[prec], "unit": unit,
+                "frac": ach / PEAK[prec] if ach else None, "traffic": traffic,
+                ("flop_per_launch" if BOUND[prec] == "mfma" else "algorithmic_bytes_per_launch"): work,
+                "useful_flop_per_launch": flops, "avg_launch_ms": kms,
+                "traffic_source": f"profiles/pmc_traffic_{wl.name}.json (HBM bytes per launch)" if traffic else None}
         return {"precision": prec, "ms_per_step": ms, "value": 1000.0 / ms, "wall_s_timed": wall,
                 "roofline": roof, "kernels_ms": kernels, "dstate": out, "W": w_field, "energy": energy}
 
