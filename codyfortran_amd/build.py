@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcody_hip.so")
 
-SOURCES = ["host_setup.cpp", "mctdhx_kernels.hip", "ci_kernels.hip", "meanfield_gemm.hip",
+SOURCES = ["host_setup.cpp", "state_files.cpp", "mctdhx_kernels.hip", "ci_kernels.hip", "meanfield_gemm.hip",
            "meanfield_lowprec.hip",
            "cody_api.hip", "cody_propagate.hip"]
 
@@ -45,33 +45,41 @@ FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_SOURCES = ["M_Utils_Types.f90", "M_CodyHip.f90", "M_Method.f90", "M_Propagator.f90",
                    "M_Grid.f90", "M_Method_Mb_OrbBased_MctdhxHip.f90",
                    "S_Method_Mb_OrbBased_MctdhxHip.f90"]
-FORTRAN_DRIVER = os.path.join(ROOT, "tests", "fortran", "T_MctdhxHip_Rhs.f90")
+# test programs: (source, executable) -- each links the library and the objects above
+FORTRAN_DRIVERS = [
+    (os.path.join(ROOT, "tests", "fortran", "T_MctdhxHip_Rhs.f90"), "t_mctdhx_hip"),
+    (os.path.join(ROOT, "tests", "fortran", "T_StateFiles.f90"), "t_state_files"),
+]
 BIN = os.path.join(HERE, "bin")
 FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
 
 
-def build_fortran(verbose: bool = False) -> str:
-    """Fortran host API (mirror modules + GPU leaf) and the driver test program."""
+def _fc(path: str, mod: str) -> str:
+    out = os.path.join(mod, os.path.splitext(os.path.basename(path))[0] + ".o")
+    cmd = [FC, "-O2", "-fPIC", "-module-dir", mod, "-I", mod, "-c", path, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed for {path}:\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def build_fortran(verbose: bool = False) -> list[str]:
+    """Fortran host API (mirror modules + GPU leaf) and the test programs."""
     mod = os.path.join(OBJ, "fortran")
     os.makedirs(mod, exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
-    objs = []
-    for src in FORTRAN_SOURCES + [FORTRAN_DRIVER]:
-        path = src if os.path.isabs(src) else os.path.join(FORTRAN_DIR, src)
-        out = os.path.join(mod, os.path.splitext(os.path.basename(path))[0] + ".o")
-        cmd = [FC, "-O2", "-fPIC", "-module-dir", mod, "-I", mod, "-c", path, "-o", out]
+    objs = [_fc(os.path.join(FORTRAN_DIR, src), mod) for src in FORTRAN_SOURCES]
+    exes = []
+    for src, name in FORTRAN_DRIVERS:
+        exe = os.path.join(BIN, name)
+        cmd = [FC, *objs, _fc(src, mod), "-o", exe, "-L", HERE, "-lcody_hip", "-Wl,-rpath,$ORIGIN/.."]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError(f"amdflang failed for {path}:\n{r.stdout}\n{r.stderr}")
-        objs.append(out)
-    exe = os.path.join(BIN, "t_mctdhx_hip")
-    cmd = [FC, *objs, "-o", exe, "-L", HERE, "-lcody_hip", "-Wl,-rpath,$ORIGIN/.."]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"Fortran link failed:\n{r.stdout}\n{r.stderr}")
-    if verbose:
-        print(f"built {exe}")
-    return exe
+            raise RuntimeError(f"Fortran link failed for {name}:\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"built {exe}")
+        exes.append(exe)
+    return exes
 
 
 def build_library(verbose: bool = False) -> str:

@@ -172,6 +172,38 @@ module M_CodyHip
       integer(c_int) :: converged
       integer(c_int) :: r
     end function
+
+    !> Restart files coeffs.in / orbBB_II.in / rdm2BtXX_YY.in in directory dir (a
+    !> c_null_char-terminated string; "" is the working directory), the files the reference's
+    !> Coeffs_SaveCoeffs, Orbs_SaveOrbs and Coeffs_SaveTwoRdm write.
+    function cody_save_state_files(dir, nC, nG, nBodyTypes, nOrbs, state) &
+        bind(C, name="cody_save_state_files") result(r)
+      import :: c_char, c_int, c_double_complex
+      character(kind=c_char) :: dir(*)
+      integer(c_int), value :: nC, nG, nBodyTypes
+      integer(c_int) :: nOrbs(*)
+      complex(c_double_complex) :: state(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_load_state_files(dir, nC, nG, nBodyTypes, nOrbs, state) &
+        bind(C, name="cody_load_state_files") result(r)
+      import :: c_char, c_int, c_double_complex
+      character(kind=c_char) :: dir(*)
+      integer(c_int), value :: nC, nG, nBodyTypes
+      integer(c_int) :: nOrbs(*)
+      complex(c_double_complex) :: state(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_save_two_rdm(dir, nBodyTypes, nOrbs, rdm2) bind(C, name="cody_save_two_rdm") result(r)
+      import :: c_char, c_int, c_double_complex
+      character(kind=c_char) :: dir(*)
+      integer(c_int), value :: nBodyTypes
+      integer(c_int) :: nOrbs(*)
+      complex(c_double_complex) :: rdm2(*)
+      integer(c_int) :: r
+    end function
   end interface
 
 contains

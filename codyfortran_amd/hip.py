@@ -77,6 +77,9 @@ def load():
         "cody_strang": (i, [vp, d, d, i, d, i, i, ip]),
         "cody_profile_rhs": (i, [vp, i, i, dp, C.c_char_p, i, ip]),
         "cody_profile_contraction": (i, [vp, i, dp]),
+        "cody_save_state_files": (i, [C.c_char_p, i, i, i, ip, vp]),
+        "cody_load_state_files": (i, [C.c_char_p, i, i, i, ip, vp]),
+        "cody_save_two_rdm": (i, [C.c_char_p, i, ip, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -105,6 +108,38 @@ def _c128(a):
 
 
 # ----------------------------------------------------------------------------- grid helpers
+def _orb_counts(n_orbs):
+    counts = np.ascontiguousarray(np.atleast_1d(np.asarray(n_orbs, dtype=np.int32)))
+    return counts, counts.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def save_state_files(directory, state, nC, nG, n_orbs):
+    """Writes coeffs.in and orbBB_II.in (raw complex128) of the packed state [C ; orbs] into
+    `directory`; n_orbs is the orbital count of each body type (an int for one body type)."""
+    counts, cp = _orb_counts(n_orbs)
+    st = np.ascontiguousarray(state, dtype=np.complex128)
+    if st.size != nC + nG * int(counts.sum()):
+        raise CodyError("save_state_files: state size does not match nC + nG * sum(n_orbs)")
+    _check(load().cody_save_state_files(os.fsencode(directory), nC, nG, len(counts), cp, _vp(st)))
+
+
+def load_state_files(directory, nC, nG, n_orbs):
+    """Reads coeffs.in and orbBB_II.in from `directory` into a packed state [C ; orbs]."""
+    counts, cp = _orb_counts(n_orbs)
+    st = np.zeros(nC + nG * int(counts.sum()), dtype=np.complex128)
+    _check(load().cody_load_state_files(os.fsencode(directory), nC, nG, len(counts), cp, _vp(st)))
+    return st
+
+
+def save_two_rdm(directory, rdm2, n_orbs):
+    """Writes rdm2BtXX_YY.in for every pair of body types from the full rho2 (nO^4, column-major)."""
+    counts, cp = _orb_counts(n_orbs)
+    r = np.require(np.asarray(rdm2, dtype=np.complex128), requirements=["F", "A"])
+    if r.size != int(counts.sum()) ** 4:
+        raise CodyError("save_two_rdm: rdm2 must hold nO^4 values")
+    _check(load().cody_save_two_rdm(os.fsencode(directory), len(counts), cp, _vp(r)))
+
+
 def fedvr_grid(xmin, xmax, nE, nLoc, xmin_excl=False, xmax_excl=False):
     lib = load()
     n = lib.cody_fedvr_npoints(nE, nLoc, int(xmin_excl), int(xmax_excl))
@@ -202,6 +237,16 @@ class Context:
         s = np.zeros(self.n_state, dtype=np.complex128)
         _check(self.lib.cody_download_state(self.h, _vp(s)))
         return s
+
+    def save_state(self, directory):
+        """Writes the device-resident state as coeffs.in and orb01_II.in in `directory`."""
+        save_state_files(directory, self.download_state(), self.nC, self.nG, self.nO)
+
+    def load_state(self, directory):
+        """Reads coeffs.in and orb01_II.in from `directory` into the device-resident state."""
+        st = load_state_files(directory, self.nC, self.nG, self.nO)
+        self.upload_state(st)
+        return st
 
     def state_device_ptr(self) -> int:
         p = C.c_void_p()

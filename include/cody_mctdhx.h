@@ -29,6 +29,7 @@ extern "C" {
 #define CODY_ERR_HIP 2         /* HIP runtime failure */
 #define CODY_ERR_STATE 3       /* call out of order (e.g. RHS before grid/potentials are set) */
 #define CODY_ERR_UNSUPPORTED 4 /* configuration outside what the kernels implement */
+#define CODY_ERR_IO 5          /* a restart file cannot be opened, or holds fewer bytes than needed */
 
 /* precision of the mean-field contraction W = V . S (the dominant kernel) */
 #define CODY_PREC_FP64 0       /* FP64 MFMA, FP64 accumulation */
@@ -106,6 +107,29 @@ int cody_autotune(cody_ctx* ctx, int* wsplit);
 int cody_upload_state(cody_ctx* ctx, const double* state);
 int cody_download_state(cody_ctx* ctx, double* state);
 int cody_state_device_ptr(cody_ctx* ctx, void** d_state);
+
+/* ---------------------------------------------------------------- restart files (host, no context) */
+/* CodyFortran's restart files in directory `dir` (NULL or "": the current directory), each a
+ * headerless stream of native-endian complex128 values, as M_Utils_DataStorage writes them
+ * (src/Utils/M_Utils_DataStorage.f90:20-105):
+ *   coeffs.in       the nC CI coefficients: Coeffs_SaveCoeffs (src/Coeffs/S_Coeffs.f90:167-174)
+ *                   and CoeffsInit load (src/CoeffsInit/Load/S_CoeffsInit_Load.f90:70-79);
+ *   orbBB_II.in     orbital II (1-based) of body type BB, nG values: Orbs_SaveOrbs
+ *                   (src/Orbs/S_Orbs.f90:251-275) and OrbsInit load
+ *                   (src/OrbsInit/Load/S_OrbsInit_Load.f90:33-50).
+ * `state` is the packed host state [C(nC) ; orbs(nG, sum nOrbs)], orbitals ordered by body type.
+ * Saving replaces existing files; loading fails with CODY_ERR_IO when a file is missing or
+ * shorter than needed.  A device-resident state goes through cody_download_state first (or
+ * cody_upload_state after loading). */
+int cody_save_state_files(const char* dir, int nC, int nG, int nBodyTypes, const int* nOrbs,
+                          const double* state);
+int cody_load_state_files(const char* dir, int nC, int nG, int nBodyTypes, const int* nOrbs,
+                          double* state);
+/* rdm2BtXX_YY.in: the (XX, YY) body-type block rho2(i1, i2, j1, j2) of the two-body density
+ * (i1, j1 of body type XX; i2, j2 of YY), column-major, for every pair of body types:
+ * Coeffs_SaveTwoRdm (src/Coeffs/S_Coeffs.f90:185-214).  rdm2 is the full nO^4 array over the
+ * combined orbital index, as cody_densities returns it. */
+int cody_save_two_rdm(const char* dir, int nBodyTypes, const int* nOrbs, const double* rdm2);
 
 /* ---------------------------------------------------------------- Method */
 /* Full MCTDHX right-hand side  dState = -i [ (1-P)(h phi + corr) ; (H - <H>) C ]:

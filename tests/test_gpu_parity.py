@@ -206,3 +206,21 @@ def test_graph_replays_match_plain_launches(pair):
     finally:
         ctx.set_laser([], 1.0)
         s.set_laser([], 1.0)
+
+
+def test_restart_files_round_trip(pair, tmp_path):
+    """The device-resident state saved as coeffs.in / orb01_II.in and loaded back is bitwise
+    the same; rdm2Bt01_01.in written from the GPU densities holds the oracle's rho2."""
+    from codyfortran_amd import hip
+
+    name, w, s, ctx, state, wts = pair
+    ctx.upload_state(state)
+    ctx.save_state(str(tmp_path))
+    ctx.upload_state(np.zeros_like(state))
+    ctx.load_state(str(tmp_path))
+    np.testing.assert_array_equal(ctx.download_state(), state)
+    _, rdm2, _ = ctx.densities(state[:w.nC])
+    hip.save_two_rdm(str(tmp_path), rdm2, w.M)
+    got = np.fromfile(tmp_path / "rdm2Bt01_01.in", dtype="<c16")
+    ref = s.rdm2(state[:w.nC]).ravel(order="F")
+    assert np.max(np.abs(got - ref)) <= 1e-13
