@@ -56,11 +56,88 @@ def parse():
     p.add_argument("--batch", type=int, default=64,
                    help="independent instances of the batched measurement (0: skip it)")
     p.add_argument("--batch-steps", type=int, default=20)
+    p.add_argument("--prop-steps", type=int, default=1000,
+                   help="steps of each propagation measurement (0: skip them)")
     p.add_argument("--pmc-probe", action="store_true",
                    help="short run for hardware-counter passes: a 256 MiB device copy (the unit "
                         "of the byte counters) and a few launches of the FP64 and int8 "
                         "contraction kernels; prints no JSON line")
     return p.parse_args()
+
+
+def _propagate(ctx, wl, w, state, steps, dt, step_fn, chunk=50):
+    """Runs `steps` steps of `step_fn(t0, t1) -> converged` from `state`, timing only the
+    steps, in chunks of `chunk` with a check of the state between chunks; stops at the first
+    chunk whose state is non-finite, whose CI norm left [0.5, 2] times its start value, or whose
+    energy moved by more than its start value."""
+    import numpy as np
+    from codyfortran_amd.workloads import split_state
+
+    ctx.upload_state(state)
+    e0 = ctx.energy(state)
+    n0 = float(np.linalg.norm(state[:wl.nC]))
+    t, done, wall, ok = 0.0, 0, 0.0, True
+    out = {"dt": dt, "energy_start": e0}
+    while done < steps:
+        k = min(chunk, steps - done)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            ok &= bool(step_fn(t, t + dt))
+            t += dt
+        wall += time.perf_counter() - t0
+        done += k
+        s1 = ctx.download_state()
+        nrm = float(np.linalg.norm(s1[:wl.nC]))
+        e1 = ctx.energy(s1) if np.all(np.isfinite(s1)) else float("nan")
+        if not (np.isfinite(e1) and 0.5 < nrm / n0 < 2.0 and abs(e1 - e0) <= abs(e0)):
+            out.update({"diverged": True, "diverged_within_steps": [done - k + 1, done],
+                        "steps_run": done})
+            return out
+    _, orbs = split_state(s1, wl.nC, wl.nG, wl.M)
+    ov = orbs.conj().T @ (orbs * w[:, None])
+    out.update({"diverged": False, "steps_run": done, "ms_per_step": 1000.0 * wall / done,
+                "all_converged": bool(ok), "ci_norm_drift": abs(nrm - n0),
+                "orbital_orthonormality_error": float(np.max(np.abs(ov - np.eye(wl.M)))),
+                "energy_end": e1, "energy_drift_rel": abs(e1 - e0) / abs(e0)})
+    return out
+
+
+def propagation_measurement(args, wlm, wl, state, prec_code):
+    """The workload's propagation, device-resident between upload and download:
+    - the reference's Strang split step A(dt/2) B(dt) A(dt/2) (SIL with Krylov dimension
+      wl.krylov_dim in the FEDVR metric on the one-body part, RK4 on the rest) at the
+      workload's dt, from the seeded state and from a smooth seeded state;
+    - RK4 on the full right-hand side from the smooth state at dt = 0.002, below the explicit
+      stability limit 2.83/lambda_max of the 4096-point FEDVR one-body operator
+      (lambda_max ~ 8e2), which the Strang split does not need.
+    The seeded state has i.i.d. grid values (energy 666): the RK4 part of the split is far
+    outside its stability region at dt = 0.02 and the state diverges in the first step, on the
+    GPU and in the oracle alike.  The split step's energy change does not shrink with dt: the
+    reference's split counts the one-body CI term in both parts.  Full-RHS RK4 conserves the
+    energy to the level set by the rho^-1 regularisation."""
+    ctx, x, w = wlm.make_context(wl)
+    ctx.set_precision(prec_code)
+    smooth = wlm.smooth_state(x, w, wl.N, wl.M, seed=1234)
+    kd = wl.krylov_dim
+
+    def strang(t0, t1):
+        return ctx.strang(t0, t1, krylov_dim=kd, tol=1e-10, max_restarts=8, metric=True)
+
+    def rk4(t0, t1):
+        ctx.rk4(t0, t1, full=True)
+        return True
+
+    ctx.upload_state(smooth)
+    strang(0.0, wl.dt)  # warm-up: buffers and graphs
+    rk4(0.0, 0.002)
+    scheme = "Strang: SIL(dt/2) RK4(dt, nonlinear part) SIL(dt/2)"
+    out = {"steps": args.prop_steps, "krylov_dim": kd,
+           "strang_seeded_state": dict(scheme=scheme, **_propagate(ctx, wl, w, state, args.prop_steps, wl.dt, strang)),
+           "strang_smooth_state": dict(scheme=scheme, **_propagate(ctx, wl, w, smooth, args.prop_steps, wl.dt, strang)),
+           "rk4_full_rhs_smooth_state": dict(scheme="RK4 on the full RHS",
+                                             **_propagate(ctx, wl, w, smooth, args.prop_steps, 0.002, rk4))}
+    ctx.close()
+    return out
 
 
 def pmc_traffic(kname: str, workload: str):
@@ -297,6 +374,8 @@ def main():
                       "meets_1e-9": ok[f], "roofline": r["roofline"],
                       "kernels_ms": r["kernels_ms"]} for f, r in results.items()},
     }
+    if args.prop_steps > 0:
+        result["propagation"] = propagation_measurement(args, wlm, wl, state, PREC_CODE[best])
     if args.batch > 0:
         result["batched"] = batched_measurement(args, torch, wlm, check=not args.no_cpu_baseline)
     print(json.dumps(result), flush=True)

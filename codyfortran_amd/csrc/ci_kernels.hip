@@ -293,6 +293,27 @@ __global__ __launch_bounds__(256) void k_ci_energy(int nO, int Pc, const double*
   }
 }
 
+// <x> = sum_ij rho1(i,j) <phi_j|x phi_i>_w (PrinterObservableLinear_DumpDipole,
+// src/Utils/M_Utils_PrinterObservableLinear.f90:57-68, occupation 1 for unrestricted orbitals)
+// from the per-block partials of k_xmat (part1[b][i + j nO] = block b's share of <phi_i|x phi_j>),
+// summed in fixed order; one workgroup of 256 threads writes the real part.
+__global__ __launch_bounds__(256) void k_dipole(int nO, const double2* __restrict__ part1, int nPart,
+                                                const double2* __restrict__ rho1, double* __restrict__ dipole) {
+  __shared__ double red[4];
+  const int t = threadIdx.x, n2 = nO * nO;
+  double2 e = cmk(0, 0);
+  for (int q = t; q < n2; q += blockDim.x) {
+    const int i = q % nO, j = q / nO;
+    double2 xji = cmk(0, 0);
+    for (int b = 0; b < nPart; ++b) xji = cadd(xji, part1[(size_t)b * n2 + j + i * nO]);
+    cfma(e, rho1[i + j * nO], xji);
+  }
+  const double v = wave_sum(e.x);
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  if (t == 0) *dipole = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // ------------------------------------------------------------------------------------ launchers
 // Dynamic LDS of the density kernels: C (k_ci_rdm); the others use static LDS.
 size_t ci_density_lds_bytes(int nC, int nO) {
@@ -343,6 +364,10 @@ void launch_ci_apply(const double2* C, int nC, int nO, int Pc, const double* G, 
                      const DevCiTables& t, double2* dCraw, hipStream_t st) {
   hipLaunchKernelGGL(k_ci_apply, dim3((nC + 3) / 4), dim3(256), 0, st, C, nC, nO, Pc, G, h1, t.s_ptr, t.singles,
                      t.d_ptr, t.doubles, dCraw);
+}
+
+void launch_dipole(int nO, const double2* part1, int nPart, const double2* rho1, double* dipole, hipStream_t st) {
+  hipLaunchKernelGGL(k_dipole, dim3(1), dim3(256), 0, st, nO, part1, nPart, rho1, dipole);
 }
 
 void launch_ci_energy(int nO, int Pc, const double* G, const double2* h1, const double2* rho1,

@@ -611,6 +611,35 @@ int cody_energy(cody_ctx* c, const double* state, double t, double* energy) {
   return CODY_OK;
 }
 
+// <x> of a device state into one double on `st`: rho1 from the CI part, the position matrix of
+// the orbitals, then their contraction (PrinterObservableLinear_DumpDipole).
+static int enqueue_dipole(cody_ctx* c, const double2* state, double* d_out, hipStream_t st) {
+  launch_ci_density(state, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_reginv, st);
+  launch_xmat(state + c->nC, c->d_w, c->d_x, c->nG, c->nO, c->d_part1, st);
+  launch_dipole(c->nO, c->d_part1, (c->nG + 255) / 256, c->d_rho1, d_out, st);
+  HIPCHK(hipGetLastError());
+  return CODY_OK;
+}
+
+int cody_dipole(cody_ctx* c, const double* state, double* dipole) {
+  if (!c || !state || !dipole) return fail(CODY_ERR_ARG, "cody_dipole: null argument");
+  if (!c->grid_set) return fail(CODY_ERR_STATE, "cody_dipole: grid not set");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->d_tmp, state, sizeof(double2) * c->nS, hipMemcpyHostToDevice, c->stream));
+  CHK(enqueue_dipole(c, c->d_tmp, c->d_energy + 1, c->stream));
+  HIPCHK(hipMemcpyAsync(dipole, c->d_energy + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CODY_OK;
+}
+
+int cody_dipole_dev(cody_ctx* c, const void* d_state, void* d_dipole, void* stream) {
+  if (!c || !d_state || !d_dipole) return fail(CODY_ERR_ARG, "cody_dipole_dev: null argument");
+  if (!c->grid_set) return fail(CODY_ERR_STATE, "cody_dipole_dev: grid not set");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  return enqueue_dipole(c, static_cast<const double2*>(d_state), static_cast<double*>(d_dipole), st);
+}
+
 int cody_meanfield_w(cody_ctx* c, const double* orbs, double* W) {
   if (!c || !orbs || !W) return fail(CODY_ERR_ARG, "cody_meanfield_w: null argument");
   CHK(check_ready(c));

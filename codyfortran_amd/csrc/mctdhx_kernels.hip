@@ -209,6 +209,36 @@ __global__ __launch_bounds__(256) void k_onebody(const double2* __restrict__ orb
   }
 }
 
+// Partial sums of the position matrix x(i,j) = <phi_i|x phi_j>_w over 256-point blocks,
+// part1[block][i + j NO]: the orbital matrix of PrinterObservableLinear_DumpDipole
+// (src/Utils/M_Utils_PrinterObservableLinear.f90:57-68).  Same reduction shape as k_onebody.
+template <int NO>
+__global__ __launch_bounds__(256) void k_xmat(const double2* __restrict__ orbs, const double* __restrict__ w,
+                                              const double* __restrict__ xg, int nG,
+                                              double2* __restrict__ part1) {
+  __shared__ double2 sRed[4][NO];
+  const int j = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const bool in = x < nG;
+  const int xc = in ? x : nG - 1;
+  const double2 xw = in ? cscale(orbs[(size_t)j * nG + xc], xg[xc] * w[xc]) : cmk(0, 0);
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
+    double2 acc = cmk(0, 0);
+    if (in) cfmac(acc, orbs[(size_t)i * nG + xc], xw);
+    acc.x = wave_sum(acc.x);
+    acc.y = wave_sum(acc.y);
+    if (lane == 0) sRed[wave][i] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < NO) {
+    const int i = threadIdx.x;
+    const double2 acc = cadd(cadd(sRed[0][i], sRed[1][i]), cadd(sRed[2][i], sRed[3][i]));
+    part1[(size_t)blockIdx.x * NO * NO + i + j * NO] = acc;
+  }
+}
+
 // ------------------------------------------------------------------ orbital right-hand side
 // One workgroup = 64 grid points (one per lane) x one orbital j (blockIdx.y); its 4 waves split
 // the unordered pairs (l <= k, pair p taken by wave p % 4):
@@ -453,6 +483,14 @@ void launch_onebody(const double2* orbs, const double* w, const double* x, const
   hipLaunchKernelGGL(k_onebody<NO>, grid, dim3(256), 0, st, orbs, w, x, vext, kp, efield, nG, hphi, part1)
   CODY_NO_SWITCH(nO, CALL_OB)
 #undef CALL_OB
+}
+
+void launch_xmat(const double2* orbs, const double* w, const double* x, int nG, int nO, double2* part1,
+                 hipStream_t st) {
+  dim3 grid((nG + 255) / 256, nO);
+#define CALL_XM(NO) hipLaunchKernelGGL(k_xmat<NO>, grid, dim3(256), 0, st, orbs, w, x, nG, part1)
+  CODY_NO_SWITCH(nO, CALL_XM)
+#undef CALL_XM
 }
 
 void launch_orbital(const double2* orbs, const double* w, const double2* hphi, const double* WT,

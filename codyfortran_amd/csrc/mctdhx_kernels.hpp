@@ -40,6 +40,10 @@ void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a
 void launch_ci_density(const double2* C, int nC, int nO, double reg, const DevCiTables& t,
                        double2* rho1, double2* rho2, double2* r2can, double2* reginv, hipStream_t st);
 void launch_sum_parts(const double2* part, int nPart, int stride, int n, double2* out, hipStream_t st);
+// position matrix partials (k_xmat) and <x> from them and rho1 (k_dipole)
+void launch_xmat(const double2* orbs, const double* w, const double* x, int nG, int nO, double2* part1,
+                 hipStream_t st);
+void launch_dipole(int nO, const double2* part1, int nPart, const double2* rho1, double* dipole, hipStream_t st);
 void launch_gram_reduce(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
                         int n2, double2* h1, hipStream_t st);
 void launch_ci_apply(const double2* C, int nC, int nO, int Pc, const double* G, const double2* h1,

@@ -69,6 +69,8 @@ def load():
         "cody_time_derivative_orbs_lin_dev": (i, [vp, vp, vp, d, vp]),
         "cody_time_derivative_nonlin_dev": (i, [vp, vp, vp, d, vp]),
         "cody_energy": (i, [vp, vp, d, dp]),
+        "cody_dipole": (i, [vp, vp, dp]),
+        "cody_dipole_dev": (i, [vp, vp, vp, vp]),
         "cody_meanfield_w": (i, [vp, vp, vp]),
         "cody_apply_single_body": (i, [vp, vp, d, vp, vp]),
         "cody_densities": (i, [vp, vp, vp, vp, vp]),
@@ -271,6 +273,17 @@ class Context:
         e = C.c_double()
         _check(self.lib.cody_energy(self.h, _vp(s), float(t), C.byref(e)))
         return e.value
+
+    def dipole(self, state):
+        """<x> of a host state."""
+        s = _c128(state)
+        d = C.c_double()
+        _check(self.lib.cody_dipole(self.h, _vp(s), C.byref(d)))
+        return d.value
+
+    def dipole_dev(self, d_state: int, d_out: int, stream: int = 0):
+        """<x> of a device state into the double at d_out, enqueued on `stream`."""
+        _check(self.lib.cody_dipole_dev(self.h, C.c_void_p(d_state), C.c_void_p(d_out), C.c_void_p(stream)))
 
     # ---------------------------------------------------------------- pieces
     def meanfield_w(self, orbs):

@@ -101,6 +101,21 @@ def seeded_state(x: np.ndarray, w: np.ndarray, N: int, M: int, seed: int = 1234)
     return np.concatenate([C, orbs.T.ravel()]).astype(np.complex128)
 
 
+def smooth_state(x: np.ndarray, w: np.ndarray, N: int, M: int, seed: int = 1234):
+    """A low-energy seeded state on the same grid: orbital m starts as x^m exp(-x^2/8) mixed by
+    complex Gaussian coefficients (seeded), then Gram-Schmidt orthonormalised in the weights;
+    CI coefficients as in `seeded_state`.  Unlike i.i.d. point values, these orbitals have no
+    grid-scale components, so explicit integrators stay stable at ordinary step sizes."""
+    rng = np.random.default_rng(seed)
+    env = np.exp(-x ** 2 / 8.0)
+    basis = np.stack([(x / 2.0) ** m * env for m in range(M)], axis=1)
+    mix = rng.standard_normal((M, M)) + 1j * rng.standard_normal((M, M))
+    orbs = orthonormalize(basis @ mix, w)
+    C = rng.standard_normal(comb(M, N)) + 1j * rng.standard_normal(comb(M, N))
+    C = C / np.linalg.norm(C)
+    return np.concatenate([C, orbs.T.ravel()]).astype(np.complex128)
+
+
 def split_state(state: np.ndarray, nC: int, nG: int, M: int):
     return state[:nC], state[nC:].reshape(M, nG).T
 

@@ -148,6 +148,11 @@ int cody_time_derivative_nonlin_dev(cody_ctx* ctx, const void* d_state, void* d_
 /* Energy E = Tr[rho1 h1] + Tr[rho2 h2] of a host state: replaces Method_GetEnergy
  * (src/Method/Mb/OrbBased/S_Method_Mb_OrbBased.f90:132-161). */
 int cody_energy(cody_ctx* ctx, const double* state, double t, double* energy);
+/* Dipole <x> = Re sum_ij rho1(i,j) <phi_j|x phi_i>_w of a host state: replaces the value of
+ * PrinterObservableLinear_DumpDipole (src/Utils/M_Utils_PrinterObservableLinear.f90:25-70). */
+int cody_dipole(cody_ctx* ctx, const double* state, double* dipole);
+/* Same for a device state; the value (one double) is written to d_dipole, enqueued on `stream`. */
+int cody_dipole_dev(cody_ctx* ctx, const void* d_state, void* d_dipole, void* stream);
 
 /* ---------------------------------------------------------------- operator pieces (host in/out) */
 /* Mean-field potentials of the unordered pairs l <= k (pair p = l*nO - l(l-1)/2 + k-l):

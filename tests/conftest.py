@@ -15,9 +15,9 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session", autouse=True)
 def _gpu_runtime_first(request):
-    """On a GPU run, let torch initialise the HIP runtime before the library's first call."""
-    markexpr = request.config.getoption("-m") or ""
-    if "gpu" in markexpr and "not gpu" not in markexpr:
+    """When GPU tests are collected, let torch initialise the HIP runtime before the library's
+    first call (torch cannot initialise after another user of the runtime has)."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
         try:
             import torch
 

@@ -96,6 +96,27 @@ def test_energy(pair):
     assert abs(eg - eo) <= 1e-12 * max(1.0, abs(eo))
 
 
+def test_dipole(pair):
+    """<x> of the host and the device entry points against the oracle (a shifted copy of the
+    state, so that <x> is not zero by symmetry)."""
+    import torch
+
+    name, w, s, ctx, state, wts = pair
+    _, orbs = wl.split_state(state, w.nC, w.nG, w.M)
+    x = ctx.x
+    shifted = wl.orthonormalize(orbs * np.exp(0.2 * np.tanh(x / 4.0))[:, None], wts)
+    st = np.concatenate([state[:w.nC], shifted.T.ravel()])
+    ref = s.dipole(st)
+    assert abs(ref) > 1e-3
+    assert abs(ctx.dipole(st) - ref) <= 1e-12 * max(1.0, abs(ref))
+    dev = torch.device("cuda:0")
+    dst = torch.from_numpy(st.view(np.float64).copy()).to(dev)
+    out = torch.zeros(1, dtype=torch.float64, device=dev)
+    ctx.dipole_dev(dst.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert abs(float(out.item()) - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
 def test_device_pointer_entry_points(pair):
     import torch
 
