@@ -392,6 +392,8 @@ int check_ready(const cody_ctx* c) {
   return CODY_OK;
 }
 
+const double* efield_src(const cody_ctx* c) { return c->efield_src ? c->efield_src : c->d_efield; }
+
 // Enqueue one right-hand side on `st`.  mode: kModeFull / kModeNonLin / kModeLin.
 // When `ev` is non-null, an event is recorded after each launch (profiling).
 int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode,
@@ -415,7 +417,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   const int n2 = c->nO * c->nO, nPart1 = (c->nG + 255) / 256;
   if (mode == kModeLin) {
     // TimeDerivativeOrbsLin: dphi = -i (T + V) phi, CI frozen (S_Method_Mb_OrbBased_Mctdhx.f90:152-177)
-    launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_draw, c->d_part1,
+    launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield_src(c), c->nG, c->nO, c->d_draw, c->d_part1,
                    st);
     mark("k_onebody");
     launch_project(orbs, c->d_draw, nullptr, 0, c->nG, c->nO, 0, dorbs, nullptr, nullptr, 0, nullptr, st);
@@ -444,7 +446,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   }
   // independent of the mean fields: CI densities and R; h phi and the partial sums of h1
   launch_ci_density(C, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_reginv, sa);
-  launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_hphi, c->d_part1, sa);
+  launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield_src(c), c->nG, c->nO, c->d_hphi, c->d_part1, sa);
   if (split) {
     hipEventRecord(c->ev_a, sa);
     hipStreamWaitEvent(sa, c->ev_w, 0);
@@ -667,7 +669,7 @@ int cody_apply_single_body(cody_ctx* c, const double* orbs, double t, double* do
   double2* d_orbs = c->d_tmp + c->nC;
   HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
   if (c->nLaser > 0) launch_set_scalar(c->d_efield, laser_field(c, t), c->stream);
-  launch_onebody(d_orbs, c->d_w, c->d_x, c->d_vext, c->kin, c->d_efield, c->nG, c->nO, c->d_draw, c->d_part1,
+  launch_onebody(d_orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield_src(c), c->nG, c->nO, c->d_draw, c->d_part1,
                  c->stream);
   const int n2 = c->nO * c->nO;
   launch_sum_parts(c->d_part1, (c->nG + 255) / 256, n2, n2, c->d_h1, c->stream);

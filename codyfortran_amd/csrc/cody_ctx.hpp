@@ -30,29 +30,36 @@ struct RhsGraph {
   hipGraphExec_t exec;
 };
 
+// A captured propagation launch sequence: one SIL sub-step (kind 0) from `in` into `out`, or one
+// Strang step of the context's state (kind 1), for the parameters it bakes in.
+struct PropGraph {
+  int kind;
+  const void* in;
+  const void* out;
+  int maxDim, useMetric;
+  double tol, dt;
+  hipGraphExec_t exec;
+};
+
 // Work buffers of the device-side propagators (allocated on first use).
 struct PropBuffers {
   int kdim = 0;
-  double2 *basis = nullptr, *w = nullptr, *deriv = nullptr, *partial = nullptr, *dots = nullptr;
-  double2 *phi = nullptr, *nrm2 = nullptr, *trial = nullptr, *work = nullptr;
-  double *alpha = nullptr, *beta = nullptr, *flags = nullptr, *h_flags = nullptr;
+  double2 *basis = nullptr, *w = nullptr, *partial = nullptr, *phi = nullptr;
+  double2 *trial = nullptr, *work = nullptr, *backup = nullptr;
+  double *alpha = nullptr, *beta = nullptr, *nrm2 = nullptr, *par = nullptr, *flags = nullptr;
+  double* h_flags = nullptr;  // pinned host copy of flags
+  int* ctl = nullptr;         // Lanczos loop control: done, Krylov dimension, zero input
   double2 *rk_inc = nullptr, *rk_tmp = nullptr, *rk_d = nullptr;
-  // captured right-hand sides, replayed by the device-pointer entry points and the propagators;
-  // dropped whenever a setter changes what the launches bake in (kernels, sizes, parameters)
-  std::vector<RhsGraph> graphs;
-  bool graphs_on = true;
-  bool capturing = false;  // enqueue_rhs is being captured (E(t) is then set per replay)
-  unsigned generation = 0;  // bumped whenever captured launches become stale (batches check it)
+  std::vector<PropGraph> graphs;
   void drop_graphs() {
     for (auto& g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
-    ++generation;
   }
 
   void release() {
     drop_graphs();
-    void* ptrs[] = {basis, w, deriv, partial, dots, phi, nrm2, trial, work,
-                    alpha, beta, flags, rk_inc, rk_tmp, rk_d};
+    void* ptrs[] = {basis, w, partial, phi, trial, work, backup, alpha, beta, nrm2, par, flags, ctl,
+                    rk_inc, rk_tmp, rk_d};
     for (void* p : ptrs)
       if (p) hipFree(p);
     if (h_flags) hipHostFree(h_flags);
@@ -196,9 +203,12 @@ struct cody_ctx {
   bool graphs_on = true;
   bool capturing = false;  // enqueue_rhs is being captured (E(t) is then set per replay)
   unsigned generation = 0;  // bumped whenever captured launches become stale (batches check it)
+  // E(t) as the one-body kernels read it; a captured Strang step points it at per-stage slots
+  const double* efield_src = nullptr;
   void drop_graphs() {
     for (auto& g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
+    prop.drop_graphs();
     ++generation;
   }
 

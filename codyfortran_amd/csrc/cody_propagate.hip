@@ -1,16 +1,32 @@
 // Device-resident propagation of the context's state: short iterative Lanczos (SIL) on the
-// linear split part, classical RK4, and the Strang split step.  All vector algebra, Gram-Schmidt
-// re-orthogonalisation and the tridiagonal exponential run in FP64 on the GPU; the host only
-// reads back the convergence flags that drive the Krylov-dimension and step-halving control of
-// the reference (src/IntegratorList/Sil/S_IntegratorList_Sil.f90:114-312).
+// linear split part, classical RK4, and the Strang split step.
+//
+// The Lanczos loop of one SIL sub-step (src/IntegratorList/Sil/S_IntegratorList_Sil.f90:207-268)
+// runs without the host: every iteration is four launches whose first act is to test a device
+// flag `done`, which the iteration that meets the tolerance (or breaks down, or reaches the
+// maximal dimension) sets together with the Krylov dimension; the launches after it return at
+// once.  A sub-step is therefore a fixed launch sequence, captured once as a HIP graph and
+// replayed, and the host reads the convergence flags once per sub-step to apply the reference's
+// step halving (:155-170).  A Strang step (S_Propagator_SplitStep_Order2.f90:52-77) is one graph:
+// SIL(dt/2), RK4 on the nonlinear part, SIL(dt/2), each SIL as a single sub-step, after a backup
+// of the state.  If either SIL misses the tolerance in that single sub-step, the state is restored
+// and the step is redone with host-driven halving, so the result is the reference's in every case.
+//
+// All vector algebra, Gram-Schmidt re-orthogonalisation and the tridiagonal exponential run in
+// FP64.  Dot products are per-block partial sums that every block of the next launch adds in the
+// same fixed order, so all blocks use bitwise identical coefficients.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <functional>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
 #include "../../include/cody_mctdhx.h"
 #include "cody_ctx.hpp"
+#include "fedvr_device.hpp"
 #include "mctdhx_kernels.hpp"
 
 namespace cody {
@@ -18,145 +34,210 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
                 hipStream_t st, double* energy, RhsTrace* tr);
 int run_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode, hipStream_t st);
 int check_ready(const cody_ctx* c);
+double laser_field(const cody_ctx* c, double t);
 }  // namespace cody
 
 using namespace cody;
 
 namespace {
 
-constexpr int kDotBlocks = 120;
-constexpr int kMaxKrylov = 32;
+constexpr int kLzBlocks = 128;     // blocks of every Lanczos vector launch (and partial-sum rows)
+constexpr int kMaxKrylov = 32;     // partial-sum row stride and maximal Krylov dimension
+constexpr double kZero = 1.0e-14;  // zeroThreshold of the reference
+
+// parameter slots (device doubles) written by the host before a replay
+enum {
+  kParSubDt = 0, kParSubE = 1,               // one SIL sub-step: dt, E(midpoint)
+  kParSil1Dt = 2, kParSil1E = 3,             // Strang: first SIL
+  kParSil2Dt = 4, kParSil2E = 5,             //         second SIL
+  kParRk0 = 6,                               //         E at t0, t0 + dt/2, t0 + dt (RK4 stages)
+  kParCount = 16
+};
 
 __device__ __forceinline__ double metric_at(int i, int nC, int nG, const double* __restrict__ w) {
   return (w == nullptr || i < nC) ? 1.0 : w[(i - nC) % nG];
 }
 
-// partial[b][k] = sum_{i in block b} conj(V_k[i]) m_i y[i], k = 0..nv-1 (V_k = V + k*ld)
-__global__ __launch_bounds__(256) void k_multidot(const double2* __restrict__ V, size_t ld, int nv,
-                                                  const double2* __restrict__ y, int n, int nC, int nG,
-                                                  const double* __restrict__ w,
-                                                  double2* __restrict__ partial) {
-  __shared__ double sr[4][kMaxKrylov], si[4][kMaxKrylov];
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// sOut[k] = sum_b partial[b][k] for k < nv, fixed order (8 groups of 32 threads, then the groups).
+__device__ void lz_reduce(const double2* __restrict__ partial, int nv, double2* sOut,
+                          double2 (*sTmp)[kMaxKrylov]) {
+  const int t = threadIdx.x, k = t & 31, q = t >> 5;
+  if (k < nv) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int b = q; b < kLzBlocks; b += 8) {
+      const double2 p = partial[(size_t)b * kMaxKrylov + k];
+      acc.x += p.x;
+      acc.y += p.y;
+    }
+    sTmp[q][k] = acc;
+  }
+  __syncthreads();
+  if (t < nv) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int g = 0; g < 8; ++g) {
+      acc.x += sTmp[g][t].x;
+      acc.y += sTmp[g][t].y;
+    }
+    sOut[t] = acc;
+  }
+  __syncthreads();
+}
+
+// partial[block][k] = sum over this block's entries of m_i conj(V_k[i]) y[i], k < nv.
+__device__ void lz_dots(const double2* __restrict__ V, size_t ld, int nv, const double2* __restrict__ y,
+                        int n, int nC, int nG, const double* __restrict__ metric, double2* __restrict__ partial,
+                        double (*sr)[kMaxKrylov], double (*si)[kMaxKrylov]) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int k = 0; k < nv; ++k) {
     double ar = 0.0, ai = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      const double m = metric_at(i, nC, nG, w);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
+      const double m = metric_at(i, nC, nG, metric);
       const double2 a = V[k * ld + i], b = y[i];
       ar = fma(m, a.x * b.x + a.y * b.y, ar);
       ai = fma(m, a.x * b.y - a.y * b.x, ai);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      ar += __shfl_xor(ar, off, 64);
-      ai += __shfl_xor(ai, off, 64);
-    }
+    ar = wave_sum64(ar);
+    ai = wave_sum64(ai);
     if (lane == 0) {
       sr[wv][k] = ar;
       si[wv][k] = ai;
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < nv; k += blockDim.x) {
-    double r = 0.0, im = 0.0;
-    for (int q = 0; q < 4; ++q) {
-      r += sr[q][k];
-      im += si[q][k];
+  for (int k = threadIdx.x; k < nv; k += 256)
+    partial[(size_t)blockIdx.x * kMaxKrylov + k] =
+        make_double2((sr[0][k] + sr[1][k]) + (sr[2][k] + sr[3][k]), (si[0][k] + si[1][k]) + (si[2][k] + si[3][k]));
+}
+
+__global__ void k_set_par(double* __restrict__ par, int off, int n, double v0, double v1, double v2, double v3,
+                          double v4, double v5, double v6) {
+  const double v[7] = {v0, v1, v2, v3, v4, v5, v6};
+  for (int k = 0; k < n; ++k) par[off + k] = v[k];
+}
+
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ a, double2* __restrict__ b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// Start of a sub-step: clear the loop control and form the partial sums of <in|in>_m.
+__global__ __launch_bounds__(256) void k_lz_norm(const double2* __restrict__ in, int n, int nC, int nG,
+                                                 const double* __restrict__ metric, double2* __restrict__ partial,
+                                                 int* __restrict__ ctl) {
+  __shared__ double sr[4][kMaxKrylov], si[4][kMaxKrylov];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl[0] = 0;
+    ctl[1] = 0;
+    ctl[2] = 0;
+  }
+  lz_dots(in, 0, 1, in, n, nC, nG, metric, partial, sr, si);
+}
+
+// v_0 = in / |in|_m; a zero input ends the loop at once (the sub-step returns its input).
+__global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ partial, const double2* __restrict__ in,
+                                                  double2* __restrict__ v0, int n, double* __restrict__ nrm2,
+                                                  int* __restrict__ ctl, double* __restrict__ flags) {
+  __shared__ double2 sC[kMaxKrylov], sTmp[8][kMaxKrylov];
+  lz_reduce(partial, 1, sC, sTmp);
+  const double s2 = sC[0].x, nr = sqrt(s2);
+  if (nr <= kZero) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl[0] = 1;
+      ctl[1] = 0;
+      ctl[2] = 1;
+      flags[0] = 0.0;
+      flags[1] = 1.0;
     }
-    partial[(size_t)blockIdx.x * kMaxKrylov + k] = make_double2(r, im);
+    return;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nrm2 = s2;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256)
+    v0[i] = make_double2(in[i].x / nr, in[i].y / nr);
 }
 
-// out[k] = sum_b partial[b][k]
-__global__ void k_multidot_final(const double2* __restrict__ partial, int nb, int nv,
-                                 double2* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k >= nv) return;
-  double r = 0.0, im = 0.0;
-  for (int b = 0; b < nb; ++b) {
-    r += partial[(size_t)b * kMaxKrylov + k].x;
-    im += partial[(size_t)b * kMaxKrylov + k].y;
+// First step of Lanczos iteration j on the linear split part (:228-236 with TimeDerivativeOrbsLin,
+// S_Method_Mb_OrbBased_Mctdhx.f90:152-177, whose derivative -i H v the SIL multiplies by i):
+//   w = H v_j - beta_{j-1} v_{j-1},  H v = (T + V_ext + E x) v on each orbital block, 0 on C,
+// and the partial sums of c_k = <v_k|w>_m for k <= j.  c_j is alpha_j + i (rounding), so the
+// reference's w -= alpha_j v_j followed by its projections on v_0..v_j (:237-246) is the single
+// subtraction of sum_k c_k v_k that k_lz_reorth performs.
+__global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ basis, size_t ld, int j,
+                                                  const double* __restrict__ beta, const int* __restrict__ ctl,
+                                                  const double* __restrict__ w, const double* __restrict__ xg,
+                                                  const double* __restrict__ vext, KinParams kp,
+                                                  const double* __restrict__ efield, int nC, int nG, int n,
+                                                  const double* __restrict__ metric, double2* __restrict__ wout,
+                                                  double2* __restrict__ pout) {
+  __shared__ double sr[4][kMaxKrylov], si[4][kMaxKrylov];
+  if (ctl[0]) return;
+  const double E = *efield;
+  const double bprev = j > 0 ? beta[j - 1] : 0.0;
+  const double2* v = basis + (size_t)j * ld;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
+    const double2 vi = v[i];
+    double2 h = cmk(0, 0);
+    if (i >= nC) {
+      const int o = (i - nC) / nG, g = (i - nC) - o * nG;
+      h = kinetic_at(v + nC + (size_t)o * nG, g, kp, 1.0 / w[g]);
+      const double pot = vext[g] + E * xg[g];
+      h.x = fma(pot, vi.x, h.x);
+      h.y = fma(pot, vi.y, h.y);
+    }
+    if (j > 0) {
+      const double2 p = v[i - ld];
+      h.x -= bprev * p.x;
+      h.y -= bprev * p.y;
+    }
+    wout[i] = h;
   }
-  out[k] = make_double2(r, im);
+  lz_dots(basis, ld, j + 1, wout, n, nC, nG, metric, pout, sr, si);
 }
 
-// y -= sum_k coef[k] V_k   (coef on the device)
-__global__ __launch_bounds__(256) void k_sub_combination(const double2* __restrict__ V, size_t ld,
-                                                         int nv, const double2* __restrict__ coef,
-                                                         double2* __restrict__ y, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double2 acc = y[i];
-  for (int k = 0; k < nv; ++k) {
-    const double2 a = V[k * ld + i], cf = coef[k];
-    acc.x -= cf.x * a.x - cf.y * a.y;
-    acc.y -= cf.x * a.y + cf.y * a.x;
+// w -= sum_{k<nv} c_k v_k with c_k the sums of pin; then the partial sums of the next pass
+// (<v_k|w>, k < nv) or, for the last pass, of <w|w>.  The first pass stores alpha_{nv-1} = Re c_{nv-1}.
+__global__ __launch_bounds__(256) void k_lz_reorth(const int* __restrict__ ctl, const double2* __restrict__ pin,
+                                                   int nv, const double2* __restrict__ basis, size_t ld,
+                                                   double2* __restrict__ w, int n, int nC, int nG,
+                                                   const double* __restrict__ metric, int last,
+                                                   double2* __restrict__ pout, double* __restrict__ alpha) {
+  __shared__ double2 sC[kMaxKrylov], sTmp[8][kMaxKrylov];
+  __shared__ double sr[4][kMaxKrylov], si[4][kMaxKrylov];
+  if (ctl[0]) return;
+  lz_reduce(pin, nv, sC, sTmp);
+  if (alpha && blockIdx.x == 0 && threadIdx.x == 0) alpha[nv - 1] = sC[nv - 1].x;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
+    double2 r = w[i];
+    for (int k = 0; k < nv; ++k) {
+      const double2 v = basis[k * ld + i], cf = sC[k];
+      r.x -= cf.x * v.x - cf.y * v.y;
+      r.y -= cf.x * v.y + cf.y * v.x;
+    }
+    w[i] = r;
   }
-  y[i] = acc;
+  if (last)
+    lz_dots(w, 0, 1, w, n, nC, nG, metric, pout, sr, si);
+  else
+    lz_dots(basis, ld, nv, w, n, nC, nG, metric, pout, sr, si);
 }
 
-// w = i*deriv - beta_prev * vprev  (beta_prev from the device, 0 for j = 0)
-__global__ __launch_bounds__(256) void k_lanczos_w(const double2* __restrict__ deriv,
-                                                   const double2* __restrict__ vprev,
-                                                   const double* __restrict__ beta, int j,
-                                                   double2* __restrict__ w, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double2 d = deriv[i];
-  double2 r = make_double2(-d.y, d.x);
-  if (j > 0) {
-    const double b = beta[j - 1];
-    const double2 v = vprev[i];
-    r.x -= b * v.x;
-    r.y -= b * v.y;
-  }
-  w[i] = r;
-}
-
-// y = s * x with s = scale / sqrt(Re(*nrm2))   (normalisation by a device-side norm)
-__global__ __launch_bounds__(256) void k_scale_by_norm(const double2* __restrict__ x,
-                                                       const double2* __restrict__ nrm2, double scale,
-                                                       int invert, double2* __restrict__ y, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double nr = sqrt(nrm2->x);
-  const double s = invert ? scale / nr : scale * nr;
-  y[i] = make_double2(s * x[i].x, s * x[i].y);
-}
-
-// Lanczos scalars of iteration j (one thread): alpha_j from <v_j|w>, beta_j from <w|w>.
-__global__ void k_store_alpha(const double2* __restrict__ dotv, double* __restrict__ alpha, int j) {
-  alpha[j] = dotv[0].x;
-}
-__global__ __launch_bounds__(256) void k_axpy_real(const double* __restrict__ alpha, int j,
-                                                   const double2* __restrict__ v,
-                                                   double2* __restrict__ w, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double a = alpha[j];
-  w[i].x -= a * v[i].x;
-  w[i].y -= a * v[i].y;
-}
-__global__ void k_store_beta(const double2* __restrict__ dotw, double* __restrict__ beta, int j) {
-  beta[j] = sqrt(fmax(dotw[0].x, 0.0));
-}
-
-// phi = exp(-i dt T) e1 for the (m x m) Lanczos tridiagonal T(alpha, beta) by scaling and
-// squaring of a degree-20 Taylor polynomial (one workgroup, FP64), and the residual estimate
-// err = beta_{m-1} |phi_{m-1}| (:277-310).  flags[0] = err, flags[1] = beta_{m-1}.
-__global__ __launch_bounds__(256) void k_tridiag_expm(const double* __restrict__ alpha,
-                                                      const double* __restrict__ beta, int m,
-                                                      double dt, double2* __restrict__ phi,
-                                                      double* __restrict__ flags) {
-  __shared__ double2 E[kMaxKrylov * kMaxKrylov], Tm[kMaxKrylov * kMaxKrylov], Pw[kMaxKrylov * kMaxKrylov];
-  __shared__ int s_sq;
+// phi = exp(-i dt T) e1 for the m x m Lanczos tridiagonal T(sA, sB) (LDS inputs) by scaling and
+// squaring of a degree-12 Taylor polynomial; one workgroup, FP64.  The reference diagonalises T
+// (EvaluateExpInSubspace, :277-310); both give exp(-i dt T) e1 to rounding.  Returns |phi_{m-1}|.
+__device__ double tridiag_expm(const double* sA, const double* sB, int m, double dt, double2* __restrict__ phi,
+                               double2* E, double2* Tm, double2* Pw, int* sSq) {
   const int t = threadIdx.x, nt = blockDim.x, mm = m * m;
   if (t == 0) {
     double nrm = 0.0;
     for (int i = 0; i < m; ++i) {
-      double r = fabs(alpha[i]);
-      if (i > 0) r += fabs(beta[i - 1]);
-      if (i < m - 1) r += fabs(beta[i]);
+      double r = fabs(sA[i]);
+      if (i > 0) r += fabs(sB[i - 1]);
+      if (i < m - 1) r += fabs(sB[i]);
       nrm = fmax(nrm, r);
     }
     nrm *= fabs(dt);
@@ -165,24 +246,26 @@ __global__ __launch_bounds__(256) void k_tridiag_expm(const double* __restrict__
       nrm *= 0.5;
       ++s;
     }
-    s_sq = s;
+    *sSq = s;
   }
   __syncthreads();
-  const double sc = ldexp(1.0, -s_sq);
-  // A = -i dt T / 2^s ; E = I ; P = I
+  const int nsq = *sSq;
+  const double sc = ldexp(1.0, -nsq);
   for (int e = t; e < mm; e += nt) {
     const int i = e % m, j = e / m;
     double tv = 0.0;
-    if (i == j) tv = alpha[i];
-    else if (i == j + 1) tv = beta[j];
-    else if (j == i + 1) tv = beta[i];
+    if (i == j)
+      tv = sA[i];
+    else if (i == j + 1)
+      tv = sB[j];
+    else if (j == i + 1)
+      tv = sB[i];
     Tm[e] = make_double2(0.0, -dt * sc * tv);
     E[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     Pw[e] = E[e];
   }
   __syncthreads();
-  for (int k = 1; k <= 20; ++k) {
-    // Pw = Pw * A / k
+  for (int k = 1; k <= 12; ++k) {  // |A| <= 1/4: the remainder is below 4e-18
     double2 r[4];
     int cnt = 0;
     for (int e = t; e < mm; e += nt, ++cnt) {
@@ -204,7 +287,7 @@ __global__ __launch_bounds__(256) void k_tridiag_expm(const double* __restrict__
     }
     __syncthreads();
   }
-  for (int sq = 0; sq < s_sq; ++sq) {
+  for (int sq = 0; sq < nsq; ++sq) {
     double2 r[4];
     int cnt = 0;
     for (int e = t; e < mm; e += nt, ++cnt) {
@@ -223,28 +306,74 @@ __global__ __launch_bounds__(256) void k_tridiag_expm(const double* __restrict__
     __syncthreads();
   }
   for (int i = t; i < m; i += nt) phi[i] = E[i];  // first column
-  if (t == 0) {
-    const double2 last = E[m - 1];
-    flags[0] = beta[m - 1] * sqrt(last.x * last.x + last.y * last.y);
-    flags[1] = beta[m - 1];
+  const double2 last = E[m - 1];
+  return sqrt(last.x * last.x + last.y * last.y);
+}
+
+// beta_j = |w|_m; v_{j+1} = w / beta_j; block 0 evaluates exp(-i dt T_{j+1}) e1 into phi and the
+// error estimate beta_j |phi_j| (0 on breakdown), and ends the loop when it is within tol, on
+// breakdown, or at the maximal dimension (:238-254).  flags = (error, converged).
+__global__ __launch_bounds__(256) void k_lz_beta_expm(int* __restrict__ ctl, const double2* __restrict__ pin,
+                                                      const double* __restrict__ alpha, double* __restrict__ beta,
+                                                      int j, int maxDim, const double2* __restrict__ w,
+                                                      double2* __restrict__ vnext, int n,
+                                                      const double* __restrict__ dtp, double tol,
+                                                      double2* __restrict__ phi, double* __restrict__ flags) {
+  __shared__ double2 E[kMaxKrylov * kMaxKrylov], Tm[kMaxKrylov * kMaxKrylov], Pw[kMaxKrylov * kMaxKrylov];
+  __shared__ double2 sC[kMaxKrylov], sTmp[8][kMaxKrylov];
+  __shared__ double sA[kMaxKrylov], sB[kMaxKrylov];
+  __shared__ int sDone, sSq;
+  // one read of the flag per block: block 0 of this launch may set it while others start
+  if (threadIdx.x == 0) sDone = ctl[0];
+  __syncthreads();
+  if (sDone) return;
+  lz_reduce(pin, 1, sC, sTmp);
+  const double bj = sqrt(fmax(sC[0].x, 0.0));
+  const bool breakdown = bj <= kZero;
+  if (vnext && !breakdown)
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256)
+      vnext[i] = make_double2(w[i].x / bj, w[i].y / bj);
+  if (blockIdx.x != 0) return;
+  for (int k = threadIdx.x; k <= j; k += 256) {
+    sA[k] = alpha[k];
+    sB[k] = k < j ? beta[k] : bj;
+  }
+  __syncthreads();
+  const double pl = tridiag_expm(sA, sB, j + 1, *dtp, phi, E, Tm, Pw, &sSq);
+  if (threadIdx.x == 0) {
+    beta[j] = bj;
+    const double err = breakdown ? 0.0 : bj * pl;
+    const bool conv = err <= tol;
+    flags[0] = err;
+    flags[1] = conv ? 1.0 : 0.0;
+    if (conv || breakdown || j == maxDim - 1) {
+      ctl[1] = j + 1;
+      ctl[0] = 1;
+    }
   }
 }
 
-// out = nrm * sum_j phi_j V_j
-__global__ __launch_bounds__(256) void k_combine(const double2* __restrict__ V, size_t ld, int m,
-                                                 const double2* __restrict__ phi,
-                                                 const double2* __restrict__ nrm2,
-                                                 double2* __restrict__ out, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double nr = sqrt(nrm2->x);
-  double2 acc = make_double2(0.0, 0.0);
-  for (int j = 0; j < m; ++j) {
-    const double2 a = V[j * ld + i], p = phi[j];
-    acc.x += a.x * p.x - a.y * p.y;
-    acc.y += a.x * p.y + a.y * p.x;
+// out = |in|_m sum_{k < mDim} phi_k v_k, or out = in for a zero input (:258).  out may be in.
+__global__ __launch_bounds__(256) void k_lz_combine(const int* __restrict__ ctl, const double2* __restrict__ basis,
+                                                    size_t ld, const double2* __restrict__ phi,
+                                                    const double* __restrict__ nrm2, const double2* in,
+                                                    double2* out, int n) {
+  const int m = ctl[1];
+  const bool zero = ctl[2] != 0;
+  const double nr = zero ? 0.0 : sqrt(*nrm2);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    if (zero) {
+      out[i] = in[i];
+      continue;
+    }
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = 0; k < m; ++k) {
+      const double2 a = basis[k * ld + i], p = phi[k];
+      acc.x += a.x * p.x - a.y * p.y;
+      acc.y += a.x * p.y + a.y * p.x;
+    }
+    out[i] = make_double2(nr * acc.x, nr * acc.y);
   }
-  out[i] = make_double2(nr * acc.x, nr * acc.y);
 }
 
 // RK4 stage updates (S_IntegratorList_Rk_O4Expl.f90:107-137)
@@ -287,113 +416,169 @@ PropBuffers* buffers(cody_ctx* c, int kdim) {
   b.kdim = kdim;
   const size_t n = c->nS;
   if (ctx_alloc((void**)&b.basis, sizeof(double2) * n * kdim) || ctx_alloc((void**)&b.w, sizeof(double2) * n) ||
-      ctx_alloc((void**)&b.deriv, sizeof(double2) * n) ||
-      ctx_alloc((void**)&b.partial, sizeof(double2) * kDotBlocks * kMaxKrylov) ||
-      ctx_alloc((void**)&b.dots, sizeof(double2) * kMaxKrylov) ||
-      ctx_alloc((void**)&b.phi, sizeof(double2) * kMaxKrylov) || ctx_alloc((void**)&b.nrm2, sizeof(double2) * 2) ||
+      ctx_alloc((void**)&b.partial, sizeof(double2) * 2 * kLzBlocks * kMaxKrylov) ||
+      ctx_alloc((void**)&b.phi, sizeof(double2) * kMaxKrylov) ||
       ctx_alloc((void**)&b.trial, sizeof(double2) * n) || ctx_alloc((void**)&b.work, sizeof(double2) * n) ||
+      ctx_alloc((void**)&b.backup, sizeof(double2) * n) ||
       ctx_alloc((void**)&b.alpha, sizeof(double) * kMaxKrylov) ||
-      ctx_alloc((void**)&b.beta, sizeof(double) * kMaxKrylov) || ctx_alloc((void**)&b.flags, sizeof(double) * 4) ||
+      ctx_alloc((void**)&b.beta, sizeof(double) * kMaxKrylov) || ctx_alloc((void**)&b.nrm2, sizeof(double)) ||
+      ctx_alloc((void**)&b.par, sizeof(double) * kParCount) || ctx_alloc((void**)&b.flags, sizeof(double) * 8) ||
+      ctx_alloc((void**)&b.ctl, sizeof(int) * 4) ||
       ctx_alloc((void**)&b.rk_inc, sizeof(double2) * n) || ctx_alloc((void**)&b.rk_tmp, sizeof(double2) * n) ||
       ctx_alloc((void**)&b.rk_d, sizeof(double2) * n))
     return nullptr;
-  if (hipHostMalloc((void**)&b.h_flags, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess) return nullptr;
+  if (hipHostMalloc((void**)&b.h_flags, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess) return nullptr;
   return &b;
 }
 
 inline dim3 blocks_for(int n) { return dim3((n + 255) / 256); }
 
-void multidot(cody_ctx* c, PropBuffers* b, const double2* V, int nv, const double2* y,
-              const double* metric, double2* out) {
-  const int n = c->nS;
-  hipLaunchKernelGGL(k_multidot, dim3(kDotBlocks), dim3(256), 0, c->stream, V, (size_t)n, nv, y, n,
-                     c->nC, c->nG, metric, b->partial);
-  hipLaunchKernelGGL(k_multidot_final, dim3(1), dim3(kMaxKrylov), 0, c->stream, b->partial, kDotBlocks,
-                     nv, out);
+void set_par(PropBuffers* b, int off, std::initializer_list<double> v, hipStream_t st) {
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  int k = 0;
+  for (double x : v)
+    if (k < 7) a[k++] = x;
+  hipLaunchKernelGGL(k_set_par, dim3(1), dim3(1), 0, st, b->par, off, k, a[0], a[1], a[2], a[3], a[4], a[5], a[6]);
 }
 
-// One Lanczos sub-step from b->work over dt (PerformLanczosStep, :207-268).
-// Returns 0 on success and sets *conv / *err.
-int lanczos_step(cody_ctx* c, PropBuffers* b, int maxDim, double tol, const double* metric,
-                 double tbase, double dt, int* conv, double* err) {
+// One SIL sub-step from `in` into `out` (PerformLanczosStep, :207-268): dt from *dtp, E of the
+// linear part from *efp, flags[0..1] = (error, converged).  Launches only.
+void enqueue_sil_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, int useMetric, const double2* in,
+                         double2* out, const double* dtp, const double* efp, double* flags, hipStream_t st) {
+  const int n = c->nS;
+  const size_t ld = (size_t)n;
+  const double* metric = useMetric ? c->d_w : nullptr;
+  double2* p0 = b->partial;
+  double2* p1 = b->partial + kLzBlocks * kMaxKrylov;
+  const dim3 g(kLzBlocks), t(256);
+  hipLaunchKernelGGL(k_lz_norm, g, t, 0, st, in, n, c->nC, c->nG, metric, p0, b->ctl);
+  hipLaunchKernelGGL(k_lz_start, g, t, 0, st, p0, in, b->basis, n, b->nrm2, b->ctl, flags);
+  const KinParams kp{c->kin.kref, c->kin.nE, c->kin.nLoc, c->kin.xminExcl, c->nG, c->kin.invh, c->kin.fac};
+  for (int j = 0; j < maxDim; ++j) {
+    hipLaunchKernelGGL(k_lz_apply, g, t, 0, st, b->basis, ld, j, b->beta, b->ctl, c->d_w, c->d_x, c->d_vext, kp,
+                       efp, c->nC, c->nG, n, metric, b->w, p0);
+    hipLaunchKernelGGL(k_lz_reorth, g, t, 0, st, b->ctl, p0, j + 1, b->basis, ld, b->w, n, c->nC, c->nG, metric,
+                       0, p1, b->alpha);
+    hipLaunchKernelGGL(k_lz_reorth, g, t, 0, st, b->ctl, p1, j + 1, b->basis, ld, b->w, n, c->nC, c->nG, metric,
+                       1, p0, nullptr);
+    double2* vnext = j + 1 < maxDim ? b->basis + (size_t)(j + 1) * ld : nullptr;
+    hipLaunchKernelGGL(k_lz_beta_expm, g, t, 0, st, b->ctl, p0, b->alpha, b->beta, j, maxDim, b->w, vnext, n, dtp,
+                       tol, b->phi, flags);
+  }
+  hipLaunchKernelGGL(k_lz_combine, blocks_for(n), t, 0, st, b->ctl, b->basis, ld, b->phi, b->nrm2, in, out, n);
+}
+
+// RK4 stages on `mode` (S_IntegratorList_Rk_O4Expl.f90:90-141).  With replay_rhs each right-hand
+// side is the cached graph of run_rhs (which sets E(t) itself); otherwise the launches are
+// enqueued (or captured) on st and read E from efp[0..2] = E(t0), E(t0 + dt/2), E(t0 + dt).
+int enqueue_rk4(cody_ctx* c, PropBuffers* b, double t0, double t1, int mode, hipStream_t st, bool replay_rhs,
+                const double* efp) {
   const int n = c->nS;
   const dim3 g = blocks_for(n);
+  const double dt = t1 - t0, dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
+  auto rhs = [&](const double2* s, double t, int slot) {
+    if (replay_rhs) return run_rhs(c, s, b->rk_d, t, mode, st);
+    c->efield_src = efp ? efp + slot : nullptr;
+    const int r = enqueue_rhs(c, s, b->rk_d, t, mode, st, nullptr, nullptr);
+    c->efield_src = nullptr;
+    return r;
+  };
+  int r;
+  if ((r = rhs(c->d_state, t0, 0))) return r;
+  hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt6, dt2, 1, b->rk_inc, b->rk_tmp, n);
+  if ((r = rhs(b->rk_tmp, t0 + dt2, 1))) return r;
+  hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt3, dt2, 0, b->rk_inc, b->rk_tmp, n);
+  if ((r = rhs(b->rk_tmp, t0 + dt2, 1))) return r;
+  hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt3, dt, 0, b->rk_inc, b->rk_tmp, n);
+  if ((r = rhs(b->rk_tmp, t0 + dt, 2))) return r;
+  hipLaunchKernelGGL(k_rk_final, g, dim3(256), 0, st, c->d_state, b->rk_inc, b->rk_d, dt6, n);
+  return CODY_OK;
+}
+
+// Replays the cached graph of `key` (capturing `body` on first use) or, with graphs off or when
+// capture fails, launches `body` directly on the context's stream.
+int run_prop_graph(cody_ctx* c, PropBuffers* b, const PropGraph& key, const std::function<int(hipStream_t)>& body) {
   hipStream_t st = c->stream;
-  // norm of the input
-  multidot(c, b, b->work, 1, b->work, metric, b->nrm2);
-  HIPCHK(hipMemcpyAsync(b->h_flags, b->nrm2, sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  const double zt = 1.0e-14;
-  if (std::sqrt(b->h_flags[0]) <= zt) {
-    HIPCHK(hipMemcpyAsync(b->trial, b->work, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
-    *conv = 1;
-    *err = 0.0;
-    return 0;
-  }
-  hipLaunchKernelGGL(k_scale_by_norm, g, dim3(256), 0, st, b->work, b->nrm2, 1.0, 1, b->basis, n);
-  const double tev = tbase + 0.5 * dt;
-  int mDim = maxDim;
-  for (int j = 0; j < maxDim; ++j) {
-    double2* v = b->basis + (size_t)j * n;
-    int r = enqueue_rhs(c, v, b->deriv, tev, kModeLin, st, nullptr, nullptr);
-    if (r) return r;
-    hipLaunchKernelGGL(k_lanczos_w, g, dim3(256), 0, st, b->deriv,
-                       j > 0 ? b->basis + (size_t)(j - 1) * n : v, b->beta, j, b->w, n);
-    multidot(c, b, v, 1, b->w, metric, b->dots);
-    hipLaunchKernelGGL(k_store_alpha, dim3(1), dim3(1), 0, st, b->dots, b->alpha, j);
-    hipLaunchKernelGGL(k_axpy_real, g, dim3(256), 0, st, b->alpha, j, v, b->w, n);
-    // full re-orthogonalisation against v_0..v_j, done twice (classical Gram-Schmidt)
-    for (int pass = 0; pass < 2; ++pass) {
-      multidot(c, b, b->basis, j + 1, b->w, metric, b->dots);
-      hipLaunchKernelGGL(k_sub_combination, g, dim3(256), 0, st, b->basis, (size_t)n, j + 1, b->dots, b->w, n);
+  if (c->graphs_on) {
+    hipGraphExec_t ex = nullptr;
+    for (const auto& g : b->graphs)
+      if (g.kind == key.kind && g.in == key.in && g.out == key.out && g.maxDim == key.maxDim &&
+          g.useMetric == key.useMetric && g.tol == key.tol && g.dt == key.dt)
+        ex = g.exec;
+    if (!ex) {
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        c->capturing = true;
+        const int r = body(st);
+        c->capturing = false;
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &gr);
+        if (r == CODY_OK && e == hipSuccess && gr && hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) == hipSuccess) {
+          if (b->graphs.size() >= 8) {
+            hipGraphExecDestroy(b->graphs.front().exec);
+            b->graphs.erase(b->graphs.begin());
+          }
+          PropGraph pg = key;
+          pg.exec = ex;
+          b->graphs.push_back(pg);
+        } else {
+          ex = nullptr;
+          c->graphs_on = false;  // capture unsupported here: plain launches from now on
+        }
+        if (gr) hipGraphDestroy(gr);
+        (void)hipGetLastError();
+      } else {
+        c->graphs_on = false;
+      }
     }
-    multidot(c, b, b->w, 1, b->w, metric, b->dots);
-    hipLaunchKernelGGL(k_store_beta, dim3(1), dim3(1), 0, st, b->dots, b->beta, j);
-    hipLaunchKernelGGL(k_tridiag_expm, dim3(1), dim3(256), 0, st, b->alpha, b->beta, j + 1, dt, b->phi,
-                       b->flags);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(b->h_flags, b->flags, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    double e = b->h_flags[0];
-    const bool breakdown = b->h_flags[1] <= zt;
-    if (breakdown) e = 0.0;
-    *err = e;
-    *conv = e <= tol;
-    if (*conv || breakdown || j == maxDim - 1) {
-      mDim = j + 1;
-      break;
+    if (ex) {
+      HIPCHK(hipGraphLaunch(ex, st));
+      return CODY_OK;
     }
-    // v_{j+1} = w / beta_j
-    hipLaunchKernelGGL(k_scale_by_norm, g, dim3(256), 0, st, b->w, b->dots, 1.0, 1,
-                       b->basis + (size_t)(j + 1) * n, n);
   }
-  hipLaunchKernelGGL(k_combine, g, dim3(256), 0, st, b->basis, (size_t)n, mDim, b->phi, b->nrm2, b->trial, n);
+  return body(st);
+}
+
+// One SIL sub-step of b->work over [tbase, tbase + dt] into b->trial; one host read of the flags.
+int lanczos_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, int useMetric, double tbase, double dt,
+                    int* conv, double* err) {
+  hipStream_t st = c->stream;
+  set_par(b, kParSubDt, {dt, laser_field(c, tbase + 0.5 * dt)}, st);
+  const PropGraph key{0, b->work, b->trial, maxDim, useMetric, tol, 0.0, nullptr};
+  int r = run_prop_graph(c, b, key, [&](hipStream_t s) {
+    enqueue_sil_substep(c, b, maxDim, tol, useMetric, b->work, b->trial, b->par + kParSubDt, b->par + kParSubE,
+                        b->flags, s);
+    return CODY_OK;
+  });
+  if (r) return r;
   HIPCHK(hipGetLastError());
-  return 0;
+  HIPCHK(hipMemcpyAsync(b->h_flags, b->flags, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *err = b->h_flags[0];
+  *conv = b->h_flags[1] != 0.0;
+  return CODY_OK;
 }
 
 int sil_impl(cody_ctx* c, double t0, double t1, int krylovDim, double tol, int maxRestarts,
              int useMetric, int* converged) {
-  const double zt = 1.0e-14;
   if (converged) *converged = 1;
-  if (std::fabs(t1 - t0) <= zt) return CODY_OK;
+  if (std::fabs(t1 - t0) <= kZero) return CODY_OK;
   const int maxDim = std::max(1, std::min(krylovDim, c->nS));
   PropBuffers* b = buffers(c, maxDim);
   if (!b) return perr(CODY_ERR_HIP, "SIL buffer allocation failed");
-  const double* metric = useMetric ? c->d_w : nullptr;
   hipStream_t st = c->stream;
   HIPCHK(hipMemcpyAsync(b->work, c->d_state, sizeof(double2) * c->nS, hipMemcpyDeviceToDevice, st));
   double tc = t0, rem = t1 - t0;
   int all = 1;
-  while (std::fabs(rem) > zt) {
+  // step halving on a missed tolerance (S_IntegratorList_Sil.f90:155-175)
+  while (std::fabs(rem) > kZero) {
     double dts = rem;
     int restarts = 0, conv = 0;
     double err = 0.0;
     for (;;) {
-      int r = lanczos_step(c, b, maxDim, tol, metric, tc, dts, &conv, &err);
+      int r = lanczos_substep(c, b, maxDim, tol, useMetric, tc, dts, &conv, &err);
       if (r) return r;
       if (conv) break;
-      if (std::fabs(dts) <= zt || restarts >= maxRestarts) break;
+      if (std::fabs(dts) <= kZero || restarts >= maxRestarts) break;
       dts *= 0.5;
       ++restarts;
     }
@@ -411,21 +596,54 @@ int sil_impl(cody_ctx* c, double t0, double t1, int krylovDim, double tol, int m
 int rk4_impl(cody_ctx* c, double t0, double t1, int mode) {
   PropBuffers* b = buffers(c, 1);
   if (!b) return perr(CODY_ERR_HIP, "RK4 buffer allocation failed");
-  const int n = c->nS;
-  const dim3 g = blocks_for(n);
-  hipStream_t st = c->stream;
-  const double dt = t1 - t0, dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
-  int r;
-  if ((r = run_rhs(c, c->d_state, b->rk_d, t0, mode, st))) return r;
-  hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt6, dt2, 1, b->rk_inc, b->rk_tmp, n);
-  if ((r = run_rhs(c, b->rk_tmp, b->rk_d, t0 + dt2, mode, st))) return r;
-  hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt3, dt2, 0, b->rk_inc, b->rk_tmp, n);
-  if ((r = run_rhs(c, b->rk_tmp, b->rk_d, t0 + dt2, mode, st))) return r;
-  hipLaunchKernelGGL(k_rk_stage, g, dim3(256), 0, st, c->d_state, b->rk_d, dt3, dt, 0, b->rk_inc, b->rk_tmp, n);
-  if ((r = run_rhs(c, b->rk_tmp, b->rk_d, t0 + dt, mode, st))) return r;
-  hipLaunchKernelGGL(k_rk_final, g, dim3(256), 0, st, c->d_state, b->rk_inc, b->rk_d, dt6, n);
+  int r = enqueue_rk4(c, b, t0, t1, mode, c->stream, true, nullptr);
+  if (r) return r;
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CODY_OK;
+}
+
+// A(dt/2) B(dt) A(dt/2): S_Propagator_SplitStep_Order2.f90:61-75.  Fast path: one graph with each
+// SIL as a single sub-step; if either misses the tolerance, restore and use host-driven halving.
+int strang_impl(cody_ctx* c, double t0, double t1, int krylovDim, double tol, int maxRestarts, int useMetric,
+                int* converged) {
+  const double dt = t1 - t0, t2 = t0 + 0.5 * dt;
+  const int maxDim = std::max(1, std::min(krylovDim, c->nS));
+  PropBuffers* b = buffers(c, maxDim);
+  if (!b) return perr(CODY_ERR_HIP, "propagator buffer allocation failed");
+  hipStream_t st = c->stream;
+  const int n = c->nS;
+  if (std::fabs(0.5 * dt) > kZero) {
+    set_par(b, kParSil1Dt,
+            {0.5 * dt, laser_field(c, t0 + 0.25 * dt), 0.5 * dt, laser_field(c, t2 + 0.25 * dt), laser_field(c, t0),
+             laser_field(c, t0 + 0.5 * dt), laser_field(c, t0 + dt)},
+            st);
+    const PropGraph key{1, c->d_state, c->d_state, maxDim, useMetric, tol, dt, nullptr};
+    int r = run_prop_graph(c, b, key, [&](hipStream_t s) {
+      hipLaunchKernelGGL(k_copy, blocks_for(n), dim3(256), 0, s, c->d_state, b->backup, n);
+      enqueue_sil_substep(c, b, maxDim, tol, useMetric, c->d_state, c->d_state, b->par + kParSil1Dt,
+                          b->par + kParSil1E, b->flags, s);
+      int rr = enqueue_rk4(c, b, t0, t1, kModeNonLin, s, false, b->par + kParRk0);
+      if (rr) return rr;
+      enqueue_sil_substep(c, b, maxDim, tol, useMetric, c->d_state, c->d_state, b->par + kParSil2Dt,
+                          b->par + kParSil2E, b->flags + 2, s);
+      return CODY_OK;
+    });
+    if (r) return r;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->h_flags, b->flags, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (b->h_flags[1] != 0.0 && b->h_flags[3] != 0.0) {
+      if (converged) *converged = 1;
+      return CODY_OK;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_state, b->backup, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+  }
+  int ok1 = 1, ok2 = 1, r;
+  if ((r = sil_impl(c, t0, t0 + 0.5 * dt, krylovDim, tol, maxRestarts, useMetric, &ok1))) return r;
+  if ((r = rk4_impl(c, t0, t0 + dt, kModeNonLin))) return r;
+  if ((r = sil_impl(c, t2, t2 + 0.5 * dt, krylovDim, tol, maxRestarts, useMetric, &ok2))) return r;
+  if (converged) *converged = ok1 && ok2;
   return CODY_OK;
 }
 
@@ -458,14 +676,7 @@ int cody_strang(cody_ctx* c, double t0, double t1, int krylovDim, double tol, in
   int r = check_ready(c);
   if (r) return r;
   HIPCHK(hipSetDevice(c->device));
-  // A(1/2) B(1) A(1/2): S_Propagator_SplitStep_Order2.f90:61-75
-  const double dt = t1 - t0, t2 = t0 + 0.5 * dt;
-  int ok1 = 1, ok2 = 1;
-  if ((r = sil_impl(c, t0, t0 + 0.5 * dt, krylovDim, tol, maxRestarts, useMetric, &ok1))) return r;
-  if ((r = rk4_impl(c, t0, t0 + dt, kModeNonLin))) return r;
-  if ((r = sil_impl(c, t2, t2 + 0.5 * dt, krylovDim, tol, maxRestarts, useMetric, &ok2))) return r;
-  if (converged) *converged = ok1 && ok2;
-  return CODY_OK;
+  return strang_impl(c, t0, t1, krylovDim, tol, maxRestarts, useMetric, converged);
 }
 
 }  // extern "C"

@@ -9,36 +9,10 @@
 // Complex scalars are double2 (x = re, y = im).
 #include <hip/hip_runtime.h>
 
+#include "fedvr_device.hpp"
 #include "mctdhx_kernels.hpp"
 
 namespace cody {
-
-// ------------------------------------------------------------------ complex helpers
-__device__ __forceinline__ double2 cmk(double r, double i) { return make_double2(r, i); }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return cmk(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return cmk(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cconj(double2 a) { return cmk(a.x, -a.y); }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return cmk(a.x * s, a.y * s); }
-// acc += a * b
-__device__ __forceinline__ void cfma(double2& acc, double2 a, double2 b) {
-  acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
-  acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
-}
-// acc += conj(a) * b
-__device__ __forceinline__ void cfmac(double2& acc, double2 a, double2 b) {
-  acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
-  acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
-}
-__device__ __forceinline__ double2 mul_minus_i(double2 a) { return cmk(a.y, -a.x); }
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 __device__ __forceinline__ int pair_index(int nO, int l, int k) {  // l <= k
   return l * nO - (l * (l - 1)) / 2 + (k - l);
@@ -126,45 +100,6 @@ __global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ 
     __syncthreads();
     if (threadIdx.x < 16 && sm[threadIdx.x] != 0ull) atomicMax(&smax[16 * t + threadIdx.x], sm[threadIdx.x]);
   }
-}
-
-// ------------------------------------------------------------------ one-body operator at a point
-// FEDVR weak-form kinetic energy -(1/2m)(1/w_x) sum_e sum_b K(a,b)/h f_b
-// (src/Utils/M_Utils_DerivativeFedvr.f90:121-152): the point with global index g (numbered with
-// the excluded xmin node counted) is node a = g mod (nLoc-1) of element g div (nLoc-1) and, for
-// a = 0, also node nLoc-1 of the previous element.  Excluded end nodes contribute f = 0.
-struct KinParams {
-  const double* kref;
-  int nE, nLoc, xminExcl, nG;
-  double invh, fac;  // 1/h, -1/(2m)
-};
-
-__device__ __forceinline__ double2 kinetic_at(const double2* __restrict__ f, int x,
-                                              const KinParams& kp, double rw) {
-  const int gi = x + kp.xminExcl;
-  const int per = kp.nLoc - 1;
-  double2 acc = cmk(0, 0);
-  int e = gi / per, a = gi % per;
-  // element e (node a), if it exists
-  for (int pass = 0; pass < 2; ++pass) {
-    int ee = e, aa = a;
-    if (pass == 1) {
-      if (a != 0 || e == 0) break;
-      ee = e - 1;
-      aa = per;
-    }
-    if (ee >= kp.nE) continue;
-    const int base = ee * per - kp.xminExcl;
-    for (int b = 0; b < kp.nLoc; ++b) {
-      const int j = base + b;
-      if (j < 0 || j >= kp.nG) continue;
-      const double kab = kp.kref[aa + b * kp.nLoc];
-      const double2 fj = f[j];
-      acc.x = fma(kab, fj.x, acc.x);
-      acc.y = fma(kab, fj.y, acc.y);
-    }
-  }
-  return cscale(acc, kp.invh * rw * kp.fac);
 }
 
 // ------------------------------------------------------------------ one-body part

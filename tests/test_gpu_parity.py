@@ -179,6 +179,52 @@ def test_split_step_conserves_norm_at_small_step(pair):
     assert abs(np.linalg.norm(sg[: w.nC]) - 1.0) < 1e-10
 
 
+def test_strang_graph_with_field_and_step_halving(pair):
+    """The captured Strang step under a dipole field E(t) (E enters the SIL part and, through h1,
+    the CI part of the RK4 stages) against the oracle; with Krylov dimension 4 the single-sub-step
+    graph misses the tolerance and the step is redone with host-driven halving, which must give
+    the oracle's result and convergence flag; replays equal plain launches bit for bit."""
+    name, w, s, ctx, state, wts = pair
+    if w.nG > 1000:
+        pytest.skip("small grids only")
+    metric = s.metric_weights()
+    coef, T = [0.3, -0.2, 0.1], 0.12
+    ctx.set_laser(coef, T)
+    s.set_laser(coef, T)
+    try:
+        # (the seeded states are noise on the grid: dt stays below the explicit RK4 limit)
+        for kd, dt in ((16, 0.01), (4, 0.005)):
+            if kd == 4:  # one sub-step of dt/2 does not converge: the fallback path runs
+                _, ok1 = s.sil(state, 0.0, 0.5 * dt, krylov_dim=kd, tol=1e-12, max_restarts=0, metric=metric)
+                assert not ok1
+            res = {}
+            for graphs in (False, True, True):
+                ctx.set_graphs(graphs)
+                ctx.upload_state(state)
+                t, oks = 0.0, []
+                for _ in range(3):
+                    oks.append(ctx.strang(t, t + dt, krylov_dim=kd, tol=1e-12, max_restarts=8, metric=True))
+                    t += dt
+                res.setdefault(graphs, []).append((ctx.download_state(), oks))
+            plain, oks_plain = res[False][0]
+            assert np.all(np.isfinite(plain))
+            for sg, oks in res[True]:
+                assert np.array_equal(sg, plain), (kd, dt)
+                assert oks == oks_plain
+            so, t = state.copy(), 0.0
+            oko = []
+            for _ in range(3):
+                so, ok = s.strang(so, t, t + dt, krylov_dim=kd, tol=1e-12, max_restarts=8, metric=metric)
+                oko.append(bool(ok))
+                t += dt
+            assert oks_plain == oko, (kd, oks_plain, oko)
+            assert state_rel(plain, so, w.nC, wts, w.M) < 1e-10, kd
+    finally:
+        ctx.set_graphs(True)
+        ctx.set_laser([], 1.0)
+        s.set_laser([], 1.0)
+
+
 def test_graph_replays_match_plain_launches(pair):
     """Captured right-hand sides replayed with other work and synchronisations in between: every
     replay equals the plain-launch result bit for bit; with a dipole field E(t) the replays follow
