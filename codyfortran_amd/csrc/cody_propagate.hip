@@ -2,7 +2,7 @@
 // linear split part, classical RK4, and the Strang split step.
 //
 // The Lanczos loop of one SIL sub-step (src/IntegratorList/Sil/S_IntegratorList_Sil.f90:207-268)
-// runs without the host: every iteration is four launches whose first act is to test a device
+// runs without the host: every iteration is three launches whose first act is to test a device
 // flag `done`, which the iteration that meets the tolerance (or breaks down, or reaches the
 // maximal dimension) sets together with the Krylov dimension; the launches after it return at
 // once.  A sub-step is therefore a fixed launch sequence, captured once as a HIP graph and
@@ -42,8 +42,10 @@ using namespace cody;
 namespace {
 
 constexpr int kLzBlocks = 128;     // blocks of every Lanczos vector launch (and partial-sum rows)
-constexpr int kMaxKrylov = 32;     // partial-sum row stride and maximal Krylov dimension
+constexpr int kMaxKrylov = 32;     // maximal Krylov dimension
+constexpr int kPS = 64;            // partial-sum row stride (up to kMaxKrylov dots and one norm)
 constexpr double kZero = 1.0e-14;  // zeroThreshold of the reference
+constexpr int kChebMax = 200;      // Chebyshev terms of the subspace exponential (else Taylor)
 
 // parameter slots (device doubles) written by the host before a replay
 enum {
@@ -58,20 +60,14 @@ __device__ __forceinline__ double metric_at(int i, int nC, int nG, const double*
   return (w == nullptr || i < nC) ? 1.0 : w[(i - nC) % nG];
 }
 
-__device__ __forceinline__ double wave_sum64(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// sOut[k] = sum_b partial[b][k] for k < nv, fixed order (8 groups of 32 threads, then the groups).
-__device__ void lz_reduce(const double2* __restrict__ partial, int nv, double2* sOut,
-                          double2 (*sTmp)[kMaxKrylov]) {
-  const int t = threadIdx.x, k = t & 31, q = t >> 5;
+// sOut[k] = sum_b partial[b][k] for k < nv <= kPS, fixed order (4 waves over the block rows, then
+// the waves in order).
+__device__ void lz_reduce(const double2* __restrict__ partial, int nv, double2* sOut, double2 (*sTmp)[kPS]) {
+  const int t = threadIdx.x, k = t & 63, q = t >> 6;
   if (k < nv) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int b = q; b < kLzBlocks; b += 8) {
-      const double2 p = partial[(size_t)b * kMaxKrylov + k];
+    for (int b = q; b < kLzBlocks; b += 4) {
+      const double2 p = partial[(size_t)b * kPS + k];
       acc.x += p.x;
       acc.y += p.y;
     }
@@ -79,8 +75,8 @@ __device__ void lz_reduce(const double2* __restrict__ partial, int nv, double2* 
   }
   __syncthreads();
   if (t < nv) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int g = 0; g < 8; ++g) {
+    double2 acc = sTmp[0][t];
+    for (int g = 1; g < 4; ++g) {
       acc.x += sTmp[g][t].x;
       acc.y += sTmp[g][t].y;
     }
@@ -89,30 +85,67 @@ __device__ void lz_reduce(const double2* __restrict__ partial, int nv, double2* 
   __syncthreads();
 }
 
-// partial[block][k] = sum over this block's entries of m_i conj(V_k[i]) y[i], k < nv.
-__device__ void lz_dots(const double2* __restrict__ V, size_t ld, int nv, const double2* __restrict__ y,
-                        int n, int nC, int nG, const double* __restrict__ metric, double2* __restrict__ partial,
-                        double (*sr)[kMaxKrylov], double (*si)[kMaxKrylov]) {
+// partial[block][k] = sum over this block's entries of m_i conj(V_k[i]) y[i] for k < nv, and with
+// withNorm also partial[block][nv] = sum m_i |y_i|^2.  The vectors are taken eight at a time so
+// that their loads are in flight together.
+__device__ void lz_dots(const double2* __restrict__ V, size_t ld, int nv, int withNorm, const double2* y, int n,
+                        int nC, int nG, const double* __restrict__ metric, double2* __restrict__ partial,
+                        double (*sr)[kPS], double (*si)[kPS]) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int k = 0; k < nv; ++k) {
-    double ar = 0.0, ai = 0.0;
+  const int nt = nv + (withNorm ? 1 : 0);
+  for (int k0 = 0; k0 < nt; k0 += 8) {
+    double ar[8], ai[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ar[u] = ai[u] = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
       const double m = metric_at(i, nC, nG, metric);
-      const double2 a = V[k * ld + i], b = y[i];
-      ar = fma(m, a.x * b.x + a.y * b.y, ar);
-      ai = fma(m, a.x * b.y - a.y * b.x, ai);
+      const double2 b = y[i];
+      double2 a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u;
+        a[u] = k < nv ? V[k * ld + i] : (k == nv ? b : make_double2(0.0, 0.0));
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        ar[u] = fma(m, a[u].x * b.x + a[u].y * b.y, ar[u]);
+        ai[u] = fma(m, a[u].x * b.y - a[u].y * b.x, ai[u]);
+      }
     }
-    ar = wave_sum64(ar);
-    ai = wave_sum64(ai);
-    if (lane == 0) {
-      sr[wv][k] = ar;
-      si[wv][k] = ai;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (k0 + u < nt) {  // uniform
+        const double r = wave_sum(ar[u]), im = wave_sum(ai[u]);
+        if (lane == 0) {
+          sr[wv][k0 + u] = r;
+          si[wv][k0 + u] = im;
+        }
+      }
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < nv; k += 256)
-    partial[(size_t)blockIdx.x * kMaxKrylov + k] =
+  for (int k = threadIdx.x; k < nt; k += 256)
+    partial[(size_t)blockIdx.x * kPS + k] =
         make_double2((sr[0][k] + sr[1][k]) + (sr[2][k] + sr[3][k]), (si[0][k] + si[1][k]) + (si[2][k] + si[3][k]));
+}
+
+// r -= sum_{k<nv} c_k v_k[i], eight vectors' loads at a time.
+__device__ __forceinline__ double2 lz_subtract(double2 r, const double2* __restrict__ basis, size_t ld, int nv,
+                                               const double2* c, int i) {
+  for (int k0 = 0; k0 < nv; k0 += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = k0 + u < nv ? basis[(k0 + u) * ld + i] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (k0 + u < nv) {
+        const double2 cf = c[k0 + u];
+        r.x -= cf.x * v[u].x - cf.y * v[u].y;
+        r.y -= cf.x * v[u].y + cf.y * v[u].x;
+      }
+    }
+  }
+  return r;
 }
 
 __global__ void k_set_par(double* __restrict__ par, int off, int n, double v0, double v1, double v2, double v3,
@@ -130,20 +163,20 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ a, dou
 __global__ __launch_bounds__(256) void k_lz_norm(const double2* __restrict__ in, int n, int nC, int nG,
                                                  const double* __restrict__ metric, double2* __restrict__ partial,
                                                  int* __restrict__ ctl) {
-  __shared__ double sr[4][kMaxKrylov], si[4][kMaxKrylov];
+  __shared__ double sr[4][kPS], si[4][kPS];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl[0] = 0;
     ctl[1] = 0;
     ctl[2] = 0;
   }
-  lz_dots(in, 0, 1, in, n, nC, nG, metric, partial, sr, si);
+  lz_dots(in, 0, 0, 1, in, n, nC, nG, metric, partial, sr, si);
 }
 
 // v_0 = in / |in|_m; a zero input ends the loop at once (the sub-step returns its input).
 __global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ partial, const double2* __restrict__ in,
                                                   double2* __restrict__ v0, int n, double* __restrict__ nrm2,
                                                   int* __restrict__ ctl, double* __restrict__ flags) {
-  __shared__ double2 sC[kMaxKrylov], sTmp[8][kMaxKrylov];
+  __shared__ double2 sC[kPS], sTmp[4][kPS];
   lz_reduce(partial, 1, sC, sTmp);
   const double s2 = sC[0].x, nr = sqrt(s2);
   if (nr <= kZero) {
@@ -166,7 +199,7 @@ __global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ pa
 //   w = H v_j - beta_{j-1} v_{j-1},  H v = (T + V_ext + E x) v on each orbital block, 0 on C,
 // and the partial sums of c_k = <v_k|w>_m for k <= j.  c_j is alpha_j + i (rounding), so the
 // reference's w -= alpha_j v_j followed by its projections on v_0..v_j (:237-246) is the single
-// subtraction of sum_k c_k v_k that k_lz_reorth performs.
+// subtraction of sum_k c_k v_k that k_lz_pass1 performs.
 __global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ basis, size_t ld, int j,
                                                   const double* __restrict__ beta, const int* __restrict__ ctl,
                                                   const double* __restrict__ w, const double* __restrict__ xg,
@@ -174,7 +207,7 @@ __global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ ba
                                                   const double* __restrict__ efield, int nC, int nG, int n,
                                                   const double* __restrict__ metric, double2* __restrict__ wout,
                                                   double2* __restrict__ pout) {
-  __shared__ double sr[4][kMaxKrylov], si[4][kMaxKrylov];
+  __shared__ double sr[4][kPS], si[4][kPS];
   if (ctl[0]) return;
   const double E = *efield;
   const double bprev = j > 0 ? beta[j - 1] : 0.0;
@@ -196,41 +229,68 @@ __global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ ba
     }
     wout[i] = h;
   }
-  lz_dots(basis, ld, j + 1, wout, n, nC, nG, metric, pout, sr, si);
+  lz_dots(basis, ld, j + 1, 0, wout, n, nC, nG, metric, pout, sr, si);
 }
 
-// w -= sum_{k<nv} c_k v_k with c_k the sums of pin; then the partial sums of the next pass
-// (<v_k|w>, k < nv) or, for the last pass, of <w|w>.  The first pass stores alpha_{nv-1} = Re c_{nv-1}.
-__global__ __launch_bounds__(256) void k_lz_reorth(const int* __restrict__ ctl, const double2* __restrict__ pin,
-                                                   int nv, const double2* __restrict__ basis, size_t ld,
-                                                   double2* __restrict__ w, int n, int nC, int nG,
-                                                   const double* __restrict__ metric, int last,
-                                                   double2* __restrict__ pout, double* __restrict__ alpha) {
-  __shared__ double2 sC[kMaxKrylov], sTmp[8][kMaxKrylov];
-  __shared__ double sr[4][kMaxKrylov], si[4][kMaxKrylov];
+// First re-orthogonalisation pass: w -= sum_{k<=j} c_k v_k (c from pin), alpha_j = Re c_j; then
+// the partial sums of the second pass, <v_k|w> for k <= j, and of <w|w>.
+__global__ __launch_bounds__(256) void k_lz_pass1(const int* __restrict__ ctl, const double2* __restrict__ pin,
+                                                  int j, const double2* __restrict__ basis, size_t ld,
+                                                  double2* __restrict__ w, int n, int nC, int nG,
+                                                  const double* __restrict__ metric, double* __restrict__ alpha,
+                                                  double2* __restrict__ pout) {
+  __shared__ double2 sC[kPS], sTmp[4][kPS];
+  __shared__ double sr[4][kPS], si[4][kPS];
   if (ctl[0]) return;
+  const int nv = j + 1;
   lz_reduce(pin, nv, sC, sTmp);
-  if (alpha && blockIdx.x == 0 && threadIdx.x == 0) alpha[nv - 1] = sC[nv - 1].x;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
-    double2 r = w[i];
-    for (int k = 0; k < nv; ++k) {
-      const double2 v = basis[k * ld + i], cf = sC[k];
-      r.x -= cf.x * v.x - cf.y * v.y;
-      r.y -= cf.x * v.y + cf.y * v.x;
-    }
-    w[i] = r;
-  }
-  if (last)
-    lz_dots(w, 0, 1, w, n, nC, nG, metric, pout, sr, si);
-  else
-    lz_dots(basis, ld, nv, w, n, nC, nG, metric, pout, sr, si);
+  if (blockIdx.x == 0 && threadIdx.x == 0) alpha[j] = sC[j].x;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) w[i] = lz_subtract(w[i], basis, ld, nv, sC, i);
+  lz_dots(basis, ld, nv, 1, w, n, nC, nG, metric, pout, sr, si);
 }
 
-// phi = exp(-i dt T) e1 for the m x m Lanczos tridiagonal T(sA, sB) (LDS inputs) by scaling and
-// squaring of a degree-12 Taylor polynomial; one workgroup, FP64.  The reference diagonalises T
-// (EvaluateExpInSubspace, :277-310); both give exp(-i dt T) e1 to rounding.  Returns |phi_{m-1}|.
-__device__ double tridiag_expm(const double* sA, const double* sB, int m, double dt, double2* __restrict__ phi,
-                               double2* E, double2* Tm, double2* Pw, int* sSq) {
+// J_k(a), k = 0..K (0 < a), by Miller's backward recurrence J_{k-1} = (2k/a) J_k - J_{k+1} from
+// far above K, normalised with J_0 + 2 sum_k J_2k = 1 (Abramowitz & Stegun 9.1.27, 9.1.46);
+// rescaled against overflow.  One thread.
+__device__ void bessel_j_miller(double a, int K, double* J) {
+  int N = K + 2 * (int)sqrt(40.0 * (K + 1)) + 10;
+  N += N & 1;
+  const double ta = 2.0 / a;
+  double jn1 = 0.0, jn = 1.0, sum = 0.0;
+  for (int k = N; k >= 1; --k) {
+    if (k <= K) J[k] = jn;
+    if ((k & 1) == 0) sum += 2.0 * jn;
+    const double jm = (k * ta) * jn - jn1;
+    jn1 = jn;
+    jn = jm;
+    if (fabs(jn) > 1e250) {
+      jn *= 1e-250;
+      jn1 *= 1e-250;
+      sum *= 1e-250;
+      for (int q = k; q <= K; ++q) J[q] *= 1e-250;
+    }
+  }
+  J[0] = jn;
+  sum += jn;
+  const double is = 1.0 / sum;
+  for (int q = 0; q <= K; ++q) J[q] *= is;
+}
+
+// Number of Chebyshev terms for exp(-i a X), |X| <= 1: the first k > a with (a/2)^k / k! < 1e-18
+// (which bounds |J_k(a)| and the rest of the series); -1 when more than kChebMax would be needed.
+__device__ int cheb_terms(double a) {
+  double t = 1.0;
+  for (int k = 1; k < kChebMax; ++k) {
+    t *= 0.5 * a / k;
+    if (k > a && t < 1e-18) return k;
+  }
+  return -1;
+}
+
+// phi = exp(-i dt T) e1 for the m x m Lanczos tridiagonal T(sA, sB) by scaling and squaring of a
+// degree-12 Taylor polynomial; one workgroup.  Used when the Chebyshev series would be too long.
+__device__ void tridiag_expm_taylor(const double* sA, const double* sB, int m, double dt, double2* sPhi,
+                                    double2* E, double2* Tm, double2* Pw, int* sSq) {
   const int t = threadIdx.x, nt = blockDim.x, mm = m * m;
   if (t == 0) {
     double nrm = 0.0;
@@ -305,44 +365,136 @@ __device__ double tridiag_expm(const double* sA, const double* sB, int m, double
     for (int e = t; e < mm; e += nt, ++cnt) E[e] = r[cnt];
     __syncthreads();
   }
-  for (int i = t; i < m; i += nt) phi[i] = E[i];  // first column
-  const double2 last = E[m - 1];
-  return sqrt(last.x * last.x + last.y * last.y);
+  for (int i = t; i < m; i += nt) sPhi[i] = E[i];  // first column
+  __syncthreads();
 }
 
-// beta_j = |w|_m; v_{j+1} = w / beta_j; block 0 evaluates exp(-i dt T_{j+1}) e1 into phi and the
-// error estimate beta_j |phi_j| (0 on breakdown), and ends the loop when it is within tol, on
-// breakdown, or at the maximal dimension (:238-254).  flags = (error, converged).
-__global__ __launch_bounds__(256) void k_lz_beta_expm(int* __restrict__ ctl, const double2* __restrict__ pin,
-                                                      const double* __restrict__ alpha, double* __restrict__ beta,
-                                                      int j, int maxDim, const double2* __restrict__ w,
-                                                      double2* __restrict__ vnext, int n,
-                                                      const double* __restrict__ dtp, double tol,
-                                                      double2* __restrict__ phi, double* __restrict__ flags) {
+// phi = exp(-i dt T) e1 into sPhi (LDS) for the Lanczos tridiagonal T: alpha sA[0..m-1], beta
+// sB[0..m-2].  The reference diagonalises T (EvaluateExpInSubspace, :277-310); here the
+// Chebyshev expansion exp(-i dt T) = e^{-i dt c} sum_k (2 - delta_k0) (-i)^k J_k(dt r) T_k(X),
+// X = (T - c)/r with c, r the centre and half-width of the Gershgorin interval (Tal-Ezer and
+// Kosloff, J. Chem. Phys. 81 (1984) 3967), runs on wave 0 with vector component l in lane l and
+// the tridiagonal product by lane shuffles.  Whole workgroup, one call per block.
+__device__ void subspace_exp(const double* sA, const double* sB, int m, double dt, double2* sPhi, double* sJ,
+                             double* sPar, double2* E, double2* Tm, double2* Pw, int* sSq) {
+  const int t = threadIdx.x, l = t & 63;
+  if (t < 64) {
+    const double al = l < m ? sA[l] : 0.0;
+    const double bu = l < m - 1 ? sB[l] : 0.0;           // coupling to l + 1
+    const double bd = (l > 0 && l < m) ? sB[l - 1] : 0.0;  // coupling to l - 1
+    double lo = l < m ? al - fabs(bu) - fabs(bd) : 1e300;
+    double hi = l < m ? al + fabs(bu) + fabs(bd) : -1e300;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = fmin(lo, __shfl_xor(lo, off, 64));
+      hi = fmax(hi, __shfl_xor(hi, off, 64));
+    }
+    if (l == 0) {
+      const double c = 0.5 * (lo + hi), r = 0.5 * (hi - lo), a = fabs(dt) * r;
+      sPar[0] = c;
+      sPar[1] = r;
+      sPar[2] = a;
+      sPar[3] = (r > 0.0 && dt != 0.0) ? (double)cheb_terms(a) : 0.0;
+    }
+  }
+  __syncthreads();
+  const int K = (int)sPar[3];
+  if (K < 0) {
+    tridiag_expm_taylor(sA, sB, m, dt, sPhi, E, Tm, Pw, sSq);
+    return;
+  }
+  const double c = sPar[0], r = sPar[1], a = sPar[2];
+  if (t == 0 && K > 0) bessel_j_miller(a, K, sJ);
+  __syncthreads();
+  if (t < 64) {
+    double sre = 0.0, sim = 0.0;
+    if (K == 0) {
+      sre = l == 0 ? 1.0 : 0.0;  // T = c I (or dt = 0)
+    } else {
+      const double al = l < m ? sA[l] - c : 0.0;
+      const double bu = l < m - 1 ? sB[l] : 0.0;
+      const double bd = (l > 0 && l < m) ? sB[l - 1] : 0.0;
+      const double ir = 1.0 / r;
+      // the expansion is in exp(-i |dt| r X); dt < 0 conjugates it
+      const double sg = dt < 0.0 ? -1.0 : 1.0;
+      double u0 = l == 0 ? 1.0 : 0.0;
+      auto applyX = [&](double u) {
+        const double um = __shfl_up(u, 1, 64), up = __shfl_down(u, 1, 64);
+        return (al * u + (l > 0 ? bd * um : 0.0) + bu * up) * ir;
+      };
+      double u1 = applyX(u0);
+      sre = sJ[0] * u0;
+      sim = -2.0 * sJ[1] * u1 * sg;
+      for (int k = 2; k <= K; ++k) {
+        const double u2 = 2.0 * applyX(u1) - u0;
+        const double cf = 2.0 * sJ[k] * u2;
+        switch (k & 3) {
+          case 0: sre += cf; break;
+          case 1: sim -= cf * sg; break;
+          case 2: sre -= cf; break;
+          default: sim += cf * sg; break;
+        }
+        u0 = u1;
+        u1 = u2;
+      }
+    }
+    // times e^{-i dt c}
+    double sn, cs;
+    sincos(dt * c, &sn, &cs);
+    if (l < m) sPhi[l] = make_double2(cs * sre + sn * sim, cs * sim - sn * sre);
+  }
+  __syncthreads();
+}
+
+// Second pass and the end of iteration j: c2 and |w|^2 from pin; w2 = w - sum c2_k v_k;
+// beta_j = |w2|_m = sqrt(|w|^2 - sum |c2_k|^2) (the v_k are orthonormal in the metric and the
+// second pass removes rounding-size components only); v_{j+1} = w2 / beta_j.  Block 0 evaluates
+// phi = exp(-i dt T_{j+1}) e1, the error estimate beta_j |phi_j| (0 on breakdown) and ends the
+// loop when it is within tol, on breakdown, or at the maximal dimension (:238-254).
+// flags = (error, converged).
+__global__ __launch_bounds__(256) void k_lz_pass2(int* __restrict__ ctl, const double2* __restrict__ pin,
+                                                  const double* __restrict__ alpha, double* __restrict__ beta,
+                                                  int j, int maxDim, const double2* __restrict__ basis, size_t ld,
+                                                  const double2* __restrict__ w, double2* __restrict__ vnext, int n,
+                                                  const double* __restrict__ dtp, double tol,
+                                                  double2* __restrict__ phi, double* __restrict__ flags) {
   __shared__ double2 E[kMaxKrylov * kMaxKrylov], Tm[kMaxKrylov * kMaxKrylov], Pw[kMaxKrylov * kMaxKrylov];
-  __shared__ double2 sC[kMaxKrylov], sTmp[8][kMaxKrylov];
-  __shared__ double sA[kMaxKrylov], sB[kMaxKrylov];
+  __shared__ double2 sC[kPS], sTmp[4][kPS], sPhi[kMaxKrylov];
+  __shared__ double sA[kMaxKrylov], sB[kMaxKrylov], sJ[kChebMax + 1], sPar[4];
+  __shared__ double sBeta;
   __shared__ int sDone, sSq;
   // one read of the flag per block: block 0 of this launch may set it while others start
   if (threadIdx.x == 0) sDone = ctl[0];
   __syncthreads();
   if (sDone) return;
-  lz_reduce(pin, 1, sC, sTmp);
-  const double bj = sqrt(fmax(sC[0].x, 0.0));
+  const int nv = j + 1;
+  lz_reduce(pin, nv + 1, sC, sTmp);
+  if (threadIdx.x == 0) {
+    double b2 = sC[nv].x;
+    for (int k = 0; k < nv; ++k) b2 -= sC[k].x * sC[k].x + sC[k].y * sC[k].y;
+    sBeta = sqrt(fmax(b2, 0.0));
+  }
+  __syncthreads();
+  const double bj = sBeta;
   const bool breakdown = bj <= kZero;
-  if (vnext && !breakdown)
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256)
-      vnext[i] = make_double2(w[i].x / bj, w[i].y / bj);
+  if (vnext && !breakdown) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
+      const double2 r = lz_subtract(w[i], basis, ld, nv, sC, i);
+      vnext[i] = make_double2(r.x / bj, r.y / bj);
+    }
+  }
   if (blockIdx.x != 0) return;
   for (int k = threadIdx.x; k <= j; k += 256) {
     sA[k] = alpha[k];
     sB[k] = k < j ? beta[k] : bj;
   }
   __syncthreads();
-  const double pl = tridiag_expm(sA, sB, j + 1, *dtp, phi, E, Tm, Pw, &sSq);
+  subspace_exp(sA, sB, nv, *dtp, sPhi, sJ, sPar, E, Tm, Pw, &sSq);
+  for (int k = threadIdx.x; k < nv; k += 256) phi[k] = sPhi[k];
   if (threadIdx.x == 0) {
     beta[j] = bj;
-    const double err = breakdown ? 0.0 : bj * pl;
+    const double2 last = sPhi[j];
+    const double err = breakdown ? 0.0 : bj * sqrt(last.x * last.x + last.y * last.y);
     const bool conv = err <= tol;
     flags[0] = err;
     flags[1] = conv ? 1.0 : 0.0;
@@ -416,7 +568,7 @@ PropBuffers* buffers(cody_ctx* c, int kdim) {
   b.kdim = kdim;
   const size_t n = c->nS;
   if (ctx_alloc((void**)&b.basis, sizeof(double2) * n * kdim) || ctx_alloc((void**)&b.w, sizeof(double2) * n) ||
-      ctx_alloc((void**)&b.partial, sizeof(double2) * 2 * kLzBlocks * kMaxKrylov) ||
+      ctx_alloc((void**)&b.partial, sizeof(double2) * 2 * kLzBlocks * kPS) ||
       ctx_alloc((void**)&b.phi, sizeof(double2) * kMaxKrylov) ||
       ctx_alloc((void**)&b.trial, sizeof(double2) * n) || ctx_alloc((void**)&b.work, sizeof(double2) * n) ||
       ctx_alloc((void**)&b.backup, sizeof(double2) * n) ||
@@ -449,7 +601,7 @@ void enqueue_sil_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, in
   const size_t ld = (size_t)n;
   const double* metric = useMetric ? c->d_w : nullptr;
   double2* p0 = b->partial;
-  double2* p1 = b->partial + kLzBlocks * kMaxKrylov;
+  double2* p1 = b->partial + kLzBlocks * kPS;
   const dim3 g(kLzBlocks), t(256);
   hipLaunchKernelGGL(k_lz_norm, g, t, 0, st, in, n, c->nC, c->nG, metric, p0, b->ctl);
   hipLaunchKernelGGL(k_lz_start, g, t, 0, st, p0, in, b->basis, n, b->nrm2, b->ctl, flags);
@@ -457,13 +609,11 @@ void enqueue_sil_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, in
   for (int j = 0; j < maxDim; ++j) {
     hipLaunchKernelGGL(k_lz_apply, g, t, 0, st, b->basis, ld, j, b->beta, b->ctl, c->d_w, c->d_x, c->d_vext, kp,
                        efp, c->nC, c->nG, n, metric, b->w, p0);
-    hipLaunchKernelGGL(k_lz_reorth, g, t, 0, st, b->ctl, p0, j + 1, b->basis, ld, b->w, n, c->nC, c->nG, metric,
-                       0, p1, b->alpha);
-    hipLaunchKernelGGL(k_lz_reorth, g, t, 0, st, b->ctl, p1, j + 1, b->basis, ld, b->w, n, c->nC, c->nG, metric,
-                       1, p0, nullptr);
-    double2* vnext = j + 1 < maxDim ? b->basis + (size_t)(j + 1) * ld : nullptr;
-    hipLaunchKernelGGL(k_lz_beta_expm, g, t, 0, st, b->ctl, p0, b->alpha, b->beta, j, maxDim, b->w, vnext, n, dtp,
-                       tol, b->phi, flags);
+    hipLaunchKernelGGL(k_lz_pass1, g, t, 0, st, b->ctl, p0, j, b->basis, ld, b->w, n, c->nC, c->nG, metric,
+                       b->alpha, p1);
+    double2* vnext = j + 1 < maxDim ? b->basis + (size_t)(j + 1) * ld : nullptr;  // slot j+1 (disjoint from 0..j)
+    hipLaunchKernelGGL(k_lz_pass2, g, t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, b->basis, ld, b->w, vnext,
+                       n, dtp, tol, b->phi, flags);
   }
   hipLaunchKernelGGL(k_lz_combine, blocks_for(n), t, 0, st, b->ctl, b->basis, ld, b->phi, b->nrm2, in, out, n);
 }
