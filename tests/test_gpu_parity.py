@@ -166,6 +166,30 @@ def test_propagation_matches_oracle(pair):
     assert abs(np.linalg.norm(sg[: w.nC]) - np.linalg.norm(so[: w.nC])) < 1e-12
 
 
+def test_full_size_strang_matches_oracle(pair):
+    """At the 4096-point size: two Strang steps (dt 0.02, Krylov dimension 16, tol 1e-10) of a smooth
+    seeded state, and one of the i.i.d. seeded state, whose RK4 part leaves its stability region
+    (the CI norm grows by ~1e10 in that step on both sides) -- the GPU follows the oracle."""
+    name, w, s, ctx, state, wts = pair
+    if w.nG < 4000:
+        pytest.skip("the 4096-point case only")
+    metric = s.metric_weights()
+    smooth = wl.smooth_state(ctx.x, wts, w.N, w.M, seed=5)
+    for st, nsteps in ((smooth, 2), (state, 1)):
+        ctx.upload_state(st)
+        so, t = st.copy(), 0.0
+        for _ in range(nsteps):
+            ok = ctx.strang(t, t + w.dt, krylov_dim=w.krylov_dim, tol=1e-10, max_restarts=8, metric=True)
+            so, oko = s.strang(so, t, t + w.dt, krylov_dim=w.krylov_dim, tol=1e-10, max_restarts=8, metric=metric)
+            assert ok == bool(oko)
+            t += w.dt
+        sg = ctx.download_state()
+        assert np.all(np.isfinite(sg))
+        err = state_rel(sg, so, w.nC, wts, w.M)
+        print(f"  {nsteps} Strang step(s): rel err {err:.2e}, CI norm {np.linalg.norm(sg[:w.nC]):.3e}")
+        assert err < 1e-10
+
+
 def test_split_step_conserves_norm_at_small_step(pair):
     name, w, s, ctx, state, wts = pair
     if w.nG > 1000:
