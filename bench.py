@@ -156,8 +156,9 @@ def pmc_traffic(kname: str, workload: str):
 
 
 def pmc_probe(torch, wlm, wl):
-    """Launches counted by `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` runs: a plain copy of
-    known size, then the contraction kernels of the FP64 and int8-sliced forms."""
+    """Launches counted by `rocprofv3 --pmc` runs: a plain copy of known size (the unit of the
+    byte counters), then, for the FP64 and the int8-sliced form, four launches of the contraction
+    kernel alone and three full right-hand sides (every kernel of the path)."""
     dev = torch.device("cuda:0")
     a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)  # 256 MiB
     b = torch.empty_like(a)
@@ -165,13 +166,17 @@ def pmc_probe(torch, wlm, wl):
         b.copy_(a)
     torch.cuda.synchronize()
     ctx, x, w = wlm.make_context(wl)
-    ctx.upload_state(wlm.seeded_state(x, w, wl.N, wl.M, seed=1234))
+    st = wlm.seeded_state(x, w, wl.N, wl.M, seed=1234)
+    ctx.upload_state(st)
     for prec in (PREC_CODE["fp64"], PREC_CODE["int8_sliced"]):
         ctx.set_precision(prec)
         ctx.profile_contraction(3)
+        for _ in range(3):
+            ctx.time_derivative(st)
     torch.cuda.synchronize()
     ctx.close()
-    print("pmc probe: 3 copies of 256 MiB, 4 launches of each contraction kernel", flush=True)
+    print("pmc probe: 3 copies of 256 MiB; per form 4 launches of the contraction kernel and 3 full "
+          "right-hand sides", flush=True)
 
 
 def batched_measurement(args, torch, wlm, check: bool):
