@@ -30,12 +30,14 @@ sys.path.insert(0, ROOT)
 METRIC = "MCTDHF RHS evals/s (M=8, 4096 pts): split-FP32 MFMA vs FP64, rel err ≤1e-9"
 # The bound of each contraction form: FP64 and split-FP32 are MFMA-bound (useful FLOP against the
 # dense FP64 / FP32 MFMA peak); the 3xBF16 and int8-sliced forms do their products faster than
-# HBM delivers the 6 bytes per V element (int8: 21 products of 2 N_g^2 2P ops take ~10 us at
-# 5.0 POP/s, the 101 MB of V slices ~12.6 us at 8 TB/s), so they are HBM-bound: algorithmic bytes
+# HBM delivers their operand bytes (int8: 15 digit products of 2 N_g^2 Pc ops take ~8 us at
+# 5.0 POP/s, the 84 MB of V digits ~10.5 us at 8 TB/s), so they are HBM-bound: algorithmic bytes
 # = V and S operands read once plus W^T written once, against 8 TB/s.
 BOUND = {"fp64": "mfma", "split_fp32": "mfma", "bf16x3": "hbm", "int8_sliced": "hbm"}
 PEAK = {"fp64": 78.6, "split_fp32": 157.3, "bf16x3": 8000.0, "int8_sliced": 8000.0}  # TFLOP/s or GB/s
 PREC_CODE = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "int8_sliced": 3}
+# bytes per operand element of the HBM-bound forms: three BF16 terms; five int8 digits
+OPERAND_BYTES = {"bf16x3": 6.0, "int8_sliced": 5.0}
 KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_w"}
 TOL = 1e-9
 
@@ -323,7 +325,8 @@ def main():
         else:
             nGp = 256 * ((wl.nG + 255) // 256)
             Pc = 16 * ((2 * wl.pairs + 15) // 16)
-            work = 6.0 * nGp * nGp + 6.0 * Pc * nGp + 8.0 * Pc * nGp  # V, S operands (6 B/elem), W^T
+            eb = OPERAND_BYTES[prec]
+            work = eb * nGp * nGp + eb * Pc * nGp + 8.0 * Pc * nGp  # V and S operands once, W^T once
             unit = "GB/s"
             ach = work / (kms * 1e-3) / 1e9 if kms else None
         roof = {"bound": BOUND[prec], "kernel": kname, "achieved": ach, "peak": PEAK[prec], "unit": unit,

@@ -9,7 +9,22 @@ namespace cody {
 
 constexpr int kMaxOrbs = 12;   // orbitals per body type handled by the kernels
 constexpr int kMaxPc = 160;    // 16*ceil(M(M+1)/16) for M <= 12
-constexpr int kInt8Slices = 6; // 7-bit slices per operand of the int8-sliced contraction
+constexpr int kInt8Slices = 5; // balanced base-254 digits per operand of the int8-sliced contraction
+// Digit base of the int8 slices.  Values scaled into [-1/2, 1/2] have balanced base-254 digits
+// in [-127, 127] (254/2 = 127 fits int8 without a carry case); each digit carries log2 254 = 7.99 bits.
+constexpr double kSliceBase = 254.0;
+// kSliceScale[k] = 254^-k (correctly rounded at compile time)
+struct SliceScales {
+  double v[16];
+  constexpr SliceScales() : v() {
+    double s = 1.0;
+    for (int k = 0; k < 16; ++k) {
+      v[k] = s;
+      s /= kSliceBase;
+    }
+  }
+};
+constexpr SliceScales kSliceScale{};
 enum { kModeFull = 0, kModeNonLin = 1, kModeLin = 2 };
 
 struct CiEntryDev {

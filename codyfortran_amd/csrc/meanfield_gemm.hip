@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, 
 // workgroup into LDS; each wave streams its own V slices.  Three register stages: every global
 // load is issued two blocks ahead of its MFMAs (loads complete in issue order).  The steps are
 // written out six at a time so that the register stage (mod 3) and the LDS buffer (mod 2) are
-// compile-time constants.  Products with p + q <= NS - 1 that share the scale 2^{-7(p+q+2)}
+// compile-time constants.  Products with p + q <= NS - 1 that share the scale 254^{-(p+q+2)}
 // accumulate exactly in one int32 tile; the diagonals are recombined in FP64 with the row and
 // column exponents, and the workgroup stores its FP64 partial (k_wsum adds the k ranges).
 template <int NT>
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, 
     for (int u = 0; u < 4; ++u) {
       double v = 0.0;
 #pragma unroll
-      for (int d = NS - 1; d >= 0; --d) v += ldexp(static_cast<double>(acc[t][d][u]), -7 * (d + 2));
+      for (int d = NS - 1; d >= 0; --d) v += static_cast<double>(acc[t][d][u]) * kSliceScale.v[d + 2];
       slot[(4 * q + u) * LD + t * 16 + rr] = ldexp(v, eA[rowb + 4 * q + u] + fc);
     }
   }
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, 
 
 // ---------------------------------------------------------------------------------------- int8 slices
 // NS slices per operand; all slice products with p + q <= NS - 1 (0-based) that share the scale
-// 2^{-7(p+q+2)} accumulate into one int32 tile (exact: |sum| < NS 2^24 < 2^31 for K <= 4096).
+// 254^{-(p+q+2)} accumulate into one int32 tile (exact: |sum| <= NS 127^2 K < 2^31 for K <= 4096).
 template <int NT, int NS>
 __global__ __launch_bounds__(256) void k_mf_i8(const int8_t* __restrict__ As, const int* __restrict__ eA,
                                                const int8_t* __restrict__ Bs, const int* __restrict__ fB,
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(256) void k_mf_i8(const int8_t* __restrict__ As, co
     for (int i = 0; i < 4; ++i) {
       double s = 0.0;
 #pragma unroll
-      for (int d = NS - 1; d >= 0; --d) s += ldexp(static_cast<double>(acc[t][d][i]), -7 * (d + 2));
+      for (int d = NS - 1; d >= 0; --d) s += static_cast<double>(acc[t][d][i]) * kSliceScale.v[d + 2];
       vals[t][i] = ldexp(s, eA[row0 + 4 * q + i] + fc);
     }
   }

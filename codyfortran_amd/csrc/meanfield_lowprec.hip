@@ -2,11 +2,12 @@
 // (the contraction kernels themselves are in meanfield_gemm.hip).
 //
 //  * int8-sliced (exact products): every row of V and every column of S is scaled by a power of
-//    two into [-1/2, 1/2] and split into NS signed 7-bit slices, x = 2^e sum_p X_p 2^{-7p}
-//    (round-to-nearest residuals, |X_p| <= 64).  Slice products run on v_mfma_i32_16x16x64_i8 with
-//    exact int32 accumulation over the whole contraction (|sum| < 2^27); all products with
-//    p + q <= NS + 1 that share the scale 2^{-7(p+q)} are summed in one int32 accumulator, and
-//    the diagonals are recombined in FP64.  The only error is the slice truncation.
+//    two into [-1/2, 1/2] and split into NS balanced base-254 digits, x = 2^e sum_p X_p 254^{-p}
+//    (t = 254 r, X_p = rint(t), r = t - X_p: |r| <= 1/2 and |X_p| <= 127 at every step).  Digit
+//    products run on v_mfma_i32_16x16x64_i8 with exact int32 accumulation over the contraction
+//    (|sum| <= NS 127^2 N_g < 2^31); all products with p + q <= NS + 1 that share the scale
+//    254^{-(p+q)} are summed in one int32 accumulator, and the diagonals are recombined in FP64.
+//    The only error is the digit truncation (relative ~NS 254^{-NS}).
 //  * split-FP32: x = hi + lo with hi = fl32(x), lo = fl32(x - hi); hi*hi + hi*lo + lo*hi on
 //    v_mfma_f32_16x16x4f32, FP32 accumulation over 32-wide k chunks flushed into FP64.
 //  * 3xBF16: x = b1 + b2 + b3 (successive BF16 roundings); the six products with p + q <= 4 on
@@ -65,8 +66,9 @@ __global__ __launch_bounds__(1024) void k_slice_rows(const double* __restrict__ 
       unsigned word = 0;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const double q = rint(ldexp(r[u], 7 * p));
-        r[u] -= ldexp(q, -7 * p);
+        const double t = r[u] * kSliceBase;
+        const double q = rint(t);
+        r[u] = t - q;
         word |= (unsigned)(uint8_t)static_cast<int8_t>(q) << (8 * u);
       }
       *reinterpret_cast<unsigned*>(out + o + (tiled ? (size_t)(p - 1) * 1024 : (size_t)(p - 1) * plane)) = word;
@@ -107,8 +109,9 @@ __global__ __launch_bounds__(256) void k_slice_s_tiled(const double* __restrict_
     unsigned word = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const double q = rint(ldexp(r[u], 7 * p));
-      r[u] -= ldexp(q, -7 * p);
+      const double t = r[u] * kSliceBase;
+      const double q = rint(t);
+      r[u] = t - q;
       word |= (unsigned)(uint8_t)static_cast<int8_t>(q) << (8 * u);
     }
     *reinterpret_cast<unsigned*>(out + o + (size_t)(p - 1) * 1024) = word;
