@@ -35,8 +35,9 @@ extern "C" {
 #define CODY_PREC_FP64 0       /* FP64 MFMA, FP64 accumulation */
 #define CODY_PREC_SPLIT_FP32 1 /* V and S split into FP32 hi+lo, 3 FP32-MFMA products, FP64 chunk sums */
 #define CODY_PREC_BF16X3 2     /* V and S split into 3 BF16 terms, 6 BF16-MFMA products, FP32 accumulate */
-#define CODY_PREC_INT8_SLICED 3 /* V and S scaled and split into six 7-bit slices; exact int8-MFMA
-                                   products with int32 accumulation, FP64 recombination */
+#define CODY_PREC_INT8_SLICED 3 /* V and S scaled and split into five balanced base-254 digits;
+                                   exact int8-MFMA products with int32 accumulation, FP64
+                                   recombination */
 
 typedef struct cody_ctx cody_ctx;
 typedef struct cody_batch cody_batch;
