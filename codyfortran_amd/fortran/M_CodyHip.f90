@@ -124,6 +124,14 @@ module M_CodyHip
       integer(c_int) :: r
     end function
 
+    function cody_dipole(ctx, state, d) bind(C, name="cody_dipole") result(r)
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: state(*)
+      real(c_double) :: d
+      integer(c_int) :: r
+    end function
+
     function cody_set_graphs(ctx, enable) bind(C, name="cody_set_graphs") result(r)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

@@ -42,6 +42,11 @@ module M_Method_Mb_OrbBased_MctdhxHip
     !> Releases the device context.
     module subroutine Method_Mb_OrbBased_MctdhxHip_Finalize
     end subroutine
+    !> Dipole <x> = Re sum_ij rho1(i,j) <phi_j|x phi_i> of Method_state on the GPU: the value of
+    !> PrinterObservableLinear_DumpDipole (src/Utils/M_Utils_PrinterObservableLinear.f90:25-70).
+    module function Method_Mb_OrbBased_MctdhxHip_Dipole() result(res)
+      real(R64) :: res
+    end function
   end interface
 
 end module

@@ -82,6 +82,14 @@ contains
     res = e
   end function
 
+  module function Method_Mb_OrbBased_MctdhxHip_Dipole() result(res)
+    use M_Method
+    real(R64) :: res
+    real(c_double) :: d
+    call CodyHip_Check(cody_dipole(MctdhxHip_ctx, Method_state, d), "dipole")
+    res = d
+  end function
+
   subroutine PropagatorSetup
   end subroutine
 

@@ -64,6 +64,7 @@ program T_MctdhxHip_Rhs
   end do
   e1 = Method_GetEnergy(t)
   write (*, '(A,ES24.16)') "energy5 ", e1
+  write (*, '(A,ES24.16)') "dipole5 ", Method_Mb_OrbBased_MctdhxHip_Dipole()
   write (*, '(A,ES24.16)') "gridsum ", sum(Grid_Linear_weights)
   open (newunit=io, file=trim(outDir)//"/state5.bin", access="stream", form="unformatted", status="replace")
   write (io) Method_state

@@ -45,3 +45,4 @@ def test_fortran_driver_matches_oracle():
         t += 0.01
     assert state_rel(state5, so, w.nC, wts, w.M) < 1e-10
     assert abs(float(vals["energy5"]) - s.energy(so, t)) < 1e-10 * abs(e0)
+    assert abs(float(vals["dipole5"]) - s.dipole(so)) < 1e-10
