@@ -412,6 +412,8 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
     }
   };
   mark(nullptr);
+  // the nonlinear split part of the context's split: the reference's or full minus linear
+  if (mode == kModeNonLin && c->split_consistent) mode = kModeNonLinConsistent;
   // E(t) is read from device memory, so that a captured right-hand side stays valid for any t
   if (c->nLaser > 0 && !c->capturing) launch_set_scalar(c->d_efield, laser_field(c, t), st);
   const int n2 = c->nO * c->nO, nPart1 = (c->nG + 255) / 256;
@@ -420,7 +422,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
     launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield_src(c), c->nG, c->nO, c->d_draw, c->d_part1,
                    st);
     mark("k_onebody");
-    launch_project(orbs, c->d_draw, nullptr, 0, c->nG, c->nO, 0, dorbs, nullptr, nullptr, 0, nullptr, st);
+    launch_project(orbs, c->d_draw, nullptr, 0, c->nG, c->nO, 0, dorbs, nullptr, nullptr, 0, nullptr, nullptr, st);
     mark("k_project");
     launch_zero(dC, c->nC, st);
     mark("k_zero");
@@ -465,7 +467,8 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
                  c->d_draw, c->d_part, st);
   mark("k_orbital");
   if (split) hipStreamWaitEvent(st, c->ev_ci, 0);
-  launch_project(orbs, c->d_draw, c->d_part, c->nPart, c->nG, c->nO, 1, dorbs, C, c->d_dCraw, c->nC, dC, st);
+  launch_project(orbs, c->d_draw, c->d_part, c->nPart, c->nG, c->nO, 1, dorbs, C, c->d_dCraw, c->nC, dC,
+                 mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st);
   mark("k_project");
   return CODY_OK;
 }
@@ -564,6 +567,16 @@ int cody_set_graphs(cody_ctx* c, int enable) {
   HIPCHK(hipStreamSynchronize(c->stream));
   c->drop_graphs();
   c->graphs_on = enable != 0;
+  return CODY_OK;
+}
+
+int cody_set_split(cody_ctx* c, int mode) {
+  if (!c || (mode != CODY_SPLIT_REFERENCE && mode != CODY_SPLIT_CONSISTENT))
+    return fail(CODY_ERR_ARG, "cody_set_split: unknown split");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->split_consistent = mode == CODY_SPLIT_CONSISTENT;
+  c->drop_graphs();  // captured nonlinear parts and Strang steps bake the split in
   return CODY_OK;
 }
 

@@ -74,6 +74,7 @@ struct cody_ctx {
   // runs, then the CI part while the orbital correlation runs
   hipStream_t stream_ci = nullptr;
   hipEvent_t ev_fork = nullptr, ev_a = nullptr, ev_w = nullptr, ev_ci = nullptr;
+  bool split_consistent = false;  // nonlinear split part = full minus linear (cody_set_split)
   bool one_queue = false;  // every launch of a right-hand side on the caller's stream
   // sizes
   int nG = 0, nGp = 0, N = 0, nO = 0, nC = 0, nS = 0, P = 0, Pc = 0;

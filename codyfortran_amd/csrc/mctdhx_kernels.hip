@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orb
     const int p = wave + 4 * m;
     Wv[m] = (in && p < NP) ? cmk(WT[(size_t)(2 * p) * nGp + x], WT[(size_t)(2 * p + 1) * nGp + x]) : cmk(0, 0);
   }
-  const double2 hj = (in && mode == kModeFull) ? hphi[(size_t)j * nG + x] : cmk(0, 0);
+  const double2 hj = (in && (mode == kModeFull || mode == kModeNonLinConsistent)) ? hphi[(size_t)j * nG + x] : cmk(0, 0);
   __syncthreads();  // sR staged
   double2 corr = cmk(0, 0);
 #pragma unroll
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
                                                  int project, double2* __restrict__ dorbs,
                                                  const double2* __restrict__ C,
                                                  const double2* __restrict__ dCraw, int nC,
-                                                 double2* __restrict__ dC) {
+                                                 double2* __restrict__ dC, const double2* __restrict__ hsub) {
   constexpr int n2 = NO * NO, G = n2 >= 256 ? 1 : 256 / n2;
   if (dC && blockIdx.x == gridDim.x - 1) {
     __shared__ double red[2][4];
@@ -349,6 +349,7 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
 #pragma unroll
       for (int i = 0; i < NO; ++i) d = csub(d, cmul(phi[i], sInp[i + j * NO]));
     }
+    if (hsub) d = csub(d, hsub[(size_t)j * nG + x]);
     dorbs[(size_t)j * nG + x] = mul_minus_i(d);
   }
 }
@@ -441,11 +442,11 @@ void launch_orbital(const double2* orbs, const double* w, const double2* hphi, c
 
 void launch_project(const double2* orbs, const double2* draw, const double2* part, int nPart, int nG, int nO,
                     int project, double2* dorbs, const double2* C, const double2* dCraw, int nC, double2* dC,
-                    hipStream_t st) {
+                    const double2* hsub, hipStream_t st) {
   const int nb = (nG + 255) / 256 + (dC ? 1 : 0);
 #define CALL_PR(NO)                                                                                           \
   hipLaunchKernelGGL(k_project<NO>, dim3(nb), dim3(256), 0, st, orbs, draw, part, nPart, nG, project, dorbs, C, \
-                     dCraw, nC, dC)
+                     dCraw, nC, dC, hsub)
   CODY_NO_SWITCH(nO, CALL_PR)
 #undef CALL_PR
 }

@@ -25,7 +25,9 @@ struct SliceScales {
   }
 };
 constexpr SliceScales kSliceScale{};
-enum { kModeFull = 0, kModeNonLin = 1, kModeLin = 2 };
+// right-hand-side modes: the full one, the reference's nonlinear split part, the linear part, and
+// the consistent nonlinear part (full minus linear)
+enum { kModeFull = 0, kModeNonLin = 1, kModeLin = 2, kModeNonLinConsistent = 3 };
 
 struct CiEntryDev {
   int32_t ic, jc, code;
@@ -95,8 +97,9 @@ void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* 
 void launch_mf_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl, double* WT, int nGp,
                      int Pc, hipStream_t st);
 void launch_mf_bf16x3(const __bf16* A, const __bf16* B, double* WT, int nGp, int Pc, hipStream_t st);
+// dorbs = -i [(1 - P) draw - hsub] (hsub optional), and the CI projection of dCraw into dC
 void launch_project(const double2* orbs, const double2* draw, const double2* part, int nPart, int nG, int nO,
                     int project, double2* dorbs, const double2* C, const double2* dCraw, int nC, double2* dC,
-                    hipStream_t st);
+                    const double2* hsub, hipStream_t st);
 
 }  // namespace cody

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libcody_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "cody_mctdhx.h")
 
 PREC_FP64, PREC_SPLIT_FP32, PREC_BF16X3, PREC_INT8_SLICED = 0, 1, 2, 3
+SPLIT_REFERENCE, SPLIT_CONSISTENT = 0, 1
 
 _lib = None
 
@@ -57,6 +58,7 @@ def load():
         "cody_set_laser": (i, [vp, i, dp, d]),
         "cody_set_precision": (i, [vp, i]),
         "cody_set_graphs": (i, [vp, i]),
+        "cody_set_split": (i, [vp, i]),
         "cody_autotune": (i, [vp, C.POINTER(C.c_int)]),
         "cody_batch_create": (i, [C.POINTER(vp), i, C.POINTER(vp)]),
         "cody_batch_destroy": (i, [vp]),
@@ -224,6 +226,10 @@ class Context:
         k = C.c_int(int(split))
         _check(self.lib.cody_autotune(self.h, C.byref(k)))
         return k.value
+
+    def set_split(self, consistent: bool):
+        """Nonlinear split part: the reference's (False, the default) or full minus linear (True)."""
+        _check(self.lib.cody_set_split(self.h, SPLIT_CONSISTENT if consistent else SPLIT_REFERENCE))
 
     def set_graphs(self, enable: bool):
         """Replay captured right-hand sides (True, the default) or launch every kernel."""

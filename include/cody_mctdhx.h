@@ -168,6 +168,17 @@ int cody_apply_single_body(cody_ctx* ctx, const double* orbs, double t, double* 
 int cody_densities(cody_ctx* ctx, const double* coeffs, double* rdm1, double* rdm2, double* reginv);
 
 /* ---------------------------------------------------------------- propagation (device state) */
+/* The nonlinear part B of the split step, used by cody_time_derivative_nonlin_dev, cody_rk4(full=0)
+ * and cody_strang:
+ *   CODY_SPLIT_REFERENCE (default): TimeDerivativeCoeffsPlusOrbsNonLin
+ *     (S_Method_Mb_OrbBased_Mctdhx.f90:180-226), which evolves C with the full h1 but leaves out
+ *     the -P h phi term of the orbitals, so A + B differs from the full right-hand side;
+ *   CODY_SPLIT_CONSISTENT: the full right-hand side minus the linear part, so that A + B is the
+ *     full right-hand side and the Strang step is second order.  No counterpart in the reference;
+ *     the option for runs that need the step-size convergence of the energy. */
+#define CODY_SPLIT_REFERENCE 0
+#define CODY_SPLIT_CONSISTENT 1
+int cody_set_split(cody_ctx* ctx, int mode);
 /* Short iterative Lanczos on the linear split part with full re-orthogonalisation and step
  * halving (S_IntegratorList_Sil.f90:114-312), metric = unit CI weights + grid weights per
  * orbital block when useMetric != 0.  Propagates the context's state from t0 to t1.

@@ -18,7 +18,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-RHS_FULL, RHS_LIN, RHS_NONLIN, RHS_IMAG = 0, 1, 2, 3
+RHS_FULL, RHS_LIN, RHS_NONLIN, RHS_IMAG, RHS_NONLIN_CONSISTENT = 0, 1, 2, 3, 4
 
 
 def build(native: bool = False, build_dir: str | None = None) -> str:
@@ -79,6 +79,7 @@ def load(path: str | None = None):
         "orc_rk1": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_double]),
         "orc_sil": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, vp]),
         "orc_strang": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, vp]),
+        "orc_strang_split": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, vp]),
         "orc_orthonormalize": (C.c_int, [vp, vp]),
         "orc_normalize": (C.c_int, [vp, vp]),
         "orc_symeig": (C.c_int, [C.c_int, dp, dp, dp]),
@@ -345,11 +346,13 @@ class System:
                               None if m is None else _vp(m))
         return s, bool(ok)
 
-    def strang(self, state, t0, t1, krylov_dim=32, tol=1e-10, max_restarts=8, metric=None):
+    def strang(self, state, t0, t1, krylov_dim=32, tol=1e-10, max_restarts=8, metric=None, consistent=False):
+        """Strang step; consistent=True integrates B = full - linear part instead of the
+        reference's nonlinear part."""
         s = _c128(state).copy()
         m = None if metric is None else _f64(metric)
-        ok = self.lib.orc_strang(self.h, _vp(s), t0, t1, krylov_dim, tol, max_restarts,
-                                 None if m is None else _vp(m))
+        ok = self.lib.orc_strang_split(self.h, RHS_NONLIN_CONSISTENT if consistent else RHS_NONLIN, _vp(s), t0, t1,
+                                       krylov_dim, tol, max_restarts, None if m is None else _vp(m))
         return s, bool(ok)
 
     def orthonormalize(self, orbs):

@@ -1177,6 +1177,16 @@ static int eval_rhs(const orc_sys* s, int rhs, const cplx* state, double t, cplx
       return orc_time_derivative_orbs_lin(s, state, t, d);
     case ORC_RHS_NONLIN:
       return orc_time_derivative_nonlin(s, state, t, d);
+    case ORC_RHS_NONLIN_CONSISTENT: {
+      size_t nS = (size_t)s->nC + (size_t)s->nG * s->nO;
+      cplx* lin = (cplx*)malloc(sizeof(cplx) * nS);
+      if (!lin) return -1;
+      int r = orc_time_derivative(s, state, t, d);
+      r |= orc_time_derivative_orbs_lin(s, state, t, lin);
+      for (size_t i = 0; i < nS; ++i) d[i] -= lin[i];
+      free(lin);
+      return r;
+    }
     case ORC_RHS_IMAG: {
       /* src/Utils/M_Utils_Actions.f90:66-78 */
       int r = orc_time_derivative(s, state, t, d);
@@ -1350,13 +1360,18 @@ int orc_sil(const orc_sys* s, int rhs, cplx* state, double t0, double t1, int kr
 
 /* Strang A(1/2) B(1) A(1/2) with A = SIL on the linear part and B = RK4 on the nonlinear part:
  * src/Propagator/SplitStep/Order2/S_Propagator_SplitStep_Order2.f90:52-77 */
-int orc_strang(const orc_sys* s, cplx* state, double t0, double t1, int krylovDim, double tol,
-               int maxRestarts, const double* metric) {
+int orc_strang_split(const orc_sys* s, int rhsB, cplx* state, double t0, double t1, int krylovDim, double tol,
+                     int maxRestarts, const double* metric) {
   double dt = t1 - t0, t2 = t0 + 0.5 * dt;
   int ok = orc_sil(s, ORC_RHS_LIN, state, t0, t0 + 0.5 * dt, krylovDim, tol, maxRestarts, metric);
-  orc_rk4(s, ORC_RHS_NONLIN, state, t0, t0 + dt);
+  orc_rk4(s, rhsB, state, t0, t0 + dt);
   ok &= orc_sil(s, ORC_RHS_LIN, state, t2, t2 + 0.5 * dt, krylovDim, tol, maxRestarts, metric);
   return ok;
+}
+
+int orc_strang(const orc_sys* s, cplx* state, double t0, double t1, int krylovDim, double tol,
+               int maxRestarts, const double* metric) {
+  return orc_strang_split(s, ORC_RHS_NONLIN, state, t0, t1, krylovDim, tol, maxRestarts, metric);
 }
 
 /* MGS twice per body type: src/Grid/S_Grid.f90:120-151 */

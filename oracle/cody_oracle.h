@@ -80,7 +80,10 @@ double orc_energy(const orc_sys* s, const double complex* state, double t);
 double orc_dipole(const orc_sys* s, const double complex* state);
 
 /* ---- integrators / propagators ---- */
-enum { ORC_RHS_FULL = 0, ORC_RHS_LIN = 1, ORC_RHS_NONLIN = 2, ORC_RHS_IMAG = 3 };
+/* ORC_RHS_NONLIN is the reference's nonlinear split part (TimeDerivativeCoeffsPlusOrbsNonLin);
+ * ORC_RHS_NONLIN_CONSISTENT is the full right-hand side minus the linear part, the nonlinear part
+ * for which A + B is the full vector field (not a reference routine). */
+enum { ORC_RHS_FULL = 0, ORC_RHS_LIN = 1, ORC_RHS_NONLIN = 2, ORC_RHS_IMAG = 3, ORC_RHS_NONLIN_CONSISTENT = 4 };
 int orc_rk4(const orc_sys* s, int rhs, double complex* state, double t0, double t1);
 int orc_rk1(const orc_sys* s, int rhs, double complex* state, double t0, double t1);
 /* returns 1 if every sub-step converged, 0 otherwise; metric may be NULL */
@@ -88,6 +91,9 @@ int orc_sil(const orc_sys* s, int rhs, double complex* state, double t0, double 
             double tol, int maxRestarts, const double* metric);
 int orc_strang(const orc_sys* s, double complex* state, double t0, double t1, int krylovDim,
                double tol, int maxRestarts, const double* metric);
+/* Strang step with B integrated on `rhsB` (ORC_RHS_NONLIN: the reference's step) */
+int orc_strang_split(const orc_sys* s, int rhsB, double complex* state, double t0, double t1, int krylovDim,
+                     double tol, int maxRestarts, const double* metric);
 int orc_orthonormalize(const orc_sys* s, double complex* orbs);
 int orc_normalize(const orc_sys* s, double complex* coeffs);
 

@@ -190,6 +190,42 @@ def test_full_size_strang_matches_oracle(pair):
         assert err < 1e-10
 
 
+def test_consistent_split(pair):
+    """cody_set_split(CONSISTENT): the nonlinear part is the full right-hand side minus the linear
+    part, and Strang steps with it follow the oracle's consistent split; back to the reference
+    split afterwards (the captured graphs follow the switch)."""
+    import torch
+
+    name, w, s, ctx, state, wts = pair
+    if w.nG > 1000:
+        pytest.skip("small grids only")
+    metric = s.metric_weights()
+    smooth = wl.smooth_state(ctx.x, wts, w.N, w.M, seed=3)
+    dev = torch.device("cuda:0")
+    st = torch.from_numpy(smooth.view(np.float64).copy()).to(dev)
+    out = torch.zeros_like(st)
+    stream = torch.cuda.current_stream().cuda_stream
+    ctx.set_split(True)
+    try:
+        ctx.time_derivative_dev(st.data_ptr(), out.data_ptr(), 0.2, stream, mode="nonlin")
+        torch.cuda.synchronize()
+        ref = s.time_derivative(smooth, 0.2) - s.time_derivative_orbs_lin(smooth, 0.2)
+        assert state_rel(out.cpu().numpy().view(np.complex128), ref, w.nC, wts, w.M) < 1e-12
+        ctx.upload_state(smooth)
+        so, t, dt = smooth.copy(), 0.0, 0.01
+        for _ in range(3):
+            assert ctx.strang(t, t + dt, krylov_dim=16, tol=1e-12, max_restarts=8, metric=True)
+            so, _ = s.strang(so, t, t + dt, krylov_dim=16, tol=1e-12, max_restarts=8, metric=metric, consistent=True)
+            t += dt
+        assert state_rel(ctx.download_state(), so, w.nC, wts, w.M) < 1e-10
+    finally:
+        ctx.set_split(False)
+    ctx.time_derivative_dev(st.data_ptr(), out.data_ptr(), 0.2, stream, mode="nonlin")
+    torch.cuda.synchronize()
+    ref = s.time_derivative_nonlin(smooth, 0.2)
+    assert state_rel(out.cpu().numpy().view(np.complex128), ref, w.nC, wts, w.M) < 1e-12
+
+
 def test_split_step_conserves_norm_at_small_step(pair):
     name, w, s, ctx, state, wts = pair
     if w.nG > 1000:
