@@ -151,6 +151,28 @@ def propagation_measurement(args, wlm, wl, state, prec_code):
     return out
 
 
+def propagated_energy_by_form(args, wlm, wl, forms):
+    """The propagated quantity of the precision comparison: the energy after args.prop_steps
+    Strang steps (consistent split, dt = wl.dt, Krylov dimension wl.krylov_dim) from the smooth
+    seeded state, for each contraction form, relative to the FP64 form's."""
+    ctx, x, w = wlm.make_context(wl)
+    smooth = wlm.smooth_state(x, w, wl.N, wl.M, seed=1234)
+    ctx.set_split(True)
+    energy = {}
+    for f in forms:
+        ctx.set_precision(PREC_CODE[f])
+        ctx.upload_state(smooth)
+        t = 0.0
+        for _ in range(args.prop_steps):
+            ctx.strang(t, t + wl.dt, krylov_dim=wl.krylov_dim, tol=1e-10, max_restarts=8, metric=True)
+            t += wl.dt
+        energy[f] = ctx.energy(ctx.download_state())
+    ctx.close()
+    ref = energy["fp64"]
+    return {"steps": args.prop_steps, "t_end": args.prop_steps * wl.dt, "energy_fp64": ref,
+            "energy_rel_err_vs_fp64": {f: abs(e - ref) / abs(ref) for f, e in energy.items() if f != "fp64"}}
+
+
 def pmc_traffic(kname: str, workload: str):
     """HBM bytes per launch of `kname` from the committed hardware-counter summary (rocprofv3
     --pmc FETCH_SIZE / WRITE_SIZE passes over `bench.py --pmc-probe`, scaled by a 256 MiB copy);
@@ -393,6 +415,8 @@ def main():
     }
     if args.prop_steps > 0:
         result["propagation"] = propagation_measurement(args, wlm, wl, state, PREC_CODE[best])
+        if "fp64" in forms and len(forms) > 1:
+            result["propagation"]["by_contraction_form"] = propagated_energy_by_form(args, wlm, wl, forms)
     if args.batch > 0:
         result["batched"] = batched_measurement(args, torch, wlm, check=not args.no_cpu_baseline)
     print(json.dumps(result), flush=True)

@@ -132,6 +132,13 @@ module M_CodyHip
       integer(c_int) :: r
     end function
 
+    function cody_set_split(ctx, mode) bind(C, name="cody_set_split") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+      integer(c_int) :: r
+    end function
+
     function cody_set_graphs(ctx, enable) bind(C, name="cody_set_graphs") result(r)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

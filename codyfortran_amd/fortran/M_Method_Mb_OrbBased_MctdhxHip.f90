@@ -22,6 +22,9 @@ module M_Method_Mb_OrbBased_MctdhxHip
     integer(I32) :: krylovDim = 32, maxRestarts = 8
     real(R64) :: tolerance = 1.0e-10_R64
     logical :: metricQ = .true.
+    !> .true.: the split step's nonlinear part is the full right-hand side minus the linear part
+    !> (second order in dt); .false. (default): the reference's TimeDerivativeCoeffsPlusOrbsNonLin
+    logical :: consistentSplitQ = .false.
     integer(I32) :: device = 0
   end type
 

@@ -57,6 +57,7 @@ contains
     call CodyHip_Check(cody_set_interaction_softyukawa(MctdhxHip_ctx, in%strength, in%intSoftening1, &
                                                        in%intSoftening2, in%intDampening), &
                        "set_interaction_softyukawa")
+    if (in%consistentSplitQ) call CodyHip_Check(cody_set_split(MctdhxHip_ctx, 1_c_int), "set_split")
     if (allocated(Method_state)) deallocate (Method_state)
     allocate (Method_state(nS))
     Method_state(:) = (0.0_R64, 0.0_R64)
