@@ -288,6 +288,16 @@ def cpu_baseline(wl, state, budget_s: float):
         if el >= budget_s or n >= 200:
             break
     threads = lib.orc_num_threads()
+    # the same evaluation on one thread (a bounded sample: a few evaluations)
+    lib.orc_set_num_threads(1)
+    n1, t1 = 0, time.perf_counter()
+    while True:
+        s.time_derivative(state)
+        n1 += 1
+        el1 = time.perf_counter() - t1
+        if el1 >= budget_s / 3 or n1 >= 50:
+            break
+    lib.orc_set_num_threads(threads)
     return {
         "value": n / el,
         "unit": "RHS evals/s",
@@ -295,7 +305,53 @@ def cpu_baseline(wl, state, budget_s: float):
         "kind": "port",
         "sample": f"{n} full RHS evaluations of the same {wl.nG}-point, M={wl.M}, N={wl.N} state "
                   f"({el:.1f} s, OpenMP threads={threads}, no BLAS; built -O3 -march={'native' if lib_path else 'x86-64-v3'})",
+        "single_thread": {"value": n1 / el1, "unit": "RHS evals/s", "cores": 1,
+                          "sample": f"{n1} evaluations of the same state on one OpenMP thread ({el1:.1f} s)"},
     }, d, s, wts
+
+
+def small_workload_measurement(args, torch, wlm):
+    """The 401-point, M=4, N=2 system: FP64 GPU and CPU-oracle RHS evals/s and the RHS error."""
+    wl = wlm.WORKLOADS["mctdhf_1d_401"]
+    ctx, x, w = wlm.make_context(wl)
+    state = wlm.seeded_state(x, w, wl.N, wl.M, seed=1234)
+    dev = torch.device("cuda:0")
+    d_state = torch.from_numpy(state.view(np.float64).copy()).to(dev)
+    d_out = torch.zeros_like(d_state)
+    stream = torch.cuda.Stream(device=dev)
+    sptr = stream.cuda_stream
+    ctx.set_precision(0)
+    for _ in range(args.warmup):
+        ctx.time_derivative_dev(d_state.data_ptr(), d_out.data_ptr(), 0.0, sptr)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    steps = max(args.steps, 1)
+    start.record(stream)
+    for _ in range(steps):
+        ctx.time_derivative_dev(d_state.data_ptr(), d_out.data_ptr(), 0.0, sptr)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    out = {"workload": wl.name, "nG": wl.nG, "N": wl.N, "M": wl.M, "precision": "fp64",
+           "gpu_evals_per_s": 1000.0 / ms, "gpu_ms_per_eval": ms}
+    if not args.no_cpu_baseline:
+        from tests.common import oracle_system, state_rel
+
+        s, _, wts = oracle_system(wl)
+        d = s.time_derivative(state)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            s.time_derivative(state)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= 2.0 or n >= 2000:
+                break
+        g = d_out.cpu().numpy().view(np.complex128)
+        out["cpu_evals_per_s"] = n / el
+        out["cpu_sample"] = f"{n} evaluations on the oracle's OpenMP threads ({el:.1f} s)"
+        out["rhs_rel_err_vs_fp64_oracle"] = state_rel(g, d, wl.nC, wts, wl.M)
+    ctx.close()
+    return out
 
 
 def main():
@@ -419,6 +475,8 @@ def main():
             result["propagation"]["by_contraction_form"] = propagated_energy_by_form(args, wlm, wl, forms)
     if args.batch > 0:
         result["batched"] = batched_measurement(args, torch, wlm, check=not args.no_cpu_baseline)
+    if wl.name != "mctdhf_1d_401":
+        result["small_workload"] = small_workload_measurement(args, torch, wlm)
     print(json.dumps(result), flush=True)
     ctx.close()
 

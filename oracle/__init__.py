@@ -80,10 +80,17 @@ def load(path: str | None = None):
         "orc_sil": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, vp]),
         "orc_strang": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, vp]),
         "orc_strang_split": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, vp]),
+        "orc_rhs_vjp": (C.c_int, [vp, C.c_int, vp, C.c_double, vp, vp, dp]),
+        "orc_dipole_grad": (C.c_int, [vp, vp, vp]),
+        "orc_cheb_degree": (C.c_int, [C.c_double, C.c_double, C.c_double]),
+        "orc_strang_cheb": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+        "orc_dipole_gradient": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_double, C.c_int, C.c_double,
+                                           C.c_double, C.c_int, dp, dp]),
         "orc_orthonormalize": (C.c_int, [vp, vp]),
         "orc_normalize": (C.c_int, [vp, vp]),
         "orc_symeig": (C.c_int, [C.c_int, dp, dp, dp]),
         "orc_num_threads": (C.c_int, []),
+        "orc_set_num_threads": (None, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -209,9 +216,12 @@ class System:
         d = self.x[:, None] - self.x[None, :]
         self.set_interaction_matrix(softyukawa(d, strength, a1, a2, damp))
 
+    n_laser = 0
+
     def set_laser(self, c, T):
         c = _f64(c)
         self.lib.orc_set_laser(self.h, len(c), _dp(c), float(T))
+        self.n_laser = len(c)
 
     # state layout
     @property
@@ -354,6 +364,45 @@ class System:
         ok = self.lib.orc_strang_split(self.h, RHS_NONLIN_CONSISTENT if consistent else RHS_NONLIN, _vp(s), t0, t1,
                                        krylov_dim, tol, max_restarts, None if m is None else _vp(m))
         return s, bool(ok)
+
+    # adjoint (one body type); gradients in the metric of metric_weights()
+    def rhs_vjp(self, state, lam, t=0.0, rhs=RHS_FULL):
+        """(J^T lam, <lam, df/dE>) of the right-hand side `rhs` at (state, t)."""
+        s = _c128(state)
+        l = _c128(lam)
+        g = np.zeros_like(s)
+        eb = C.c_double(0.0)
+        if self.lib.orc_rhs_vjp(self.h, rhs, _vp(s), float(t), _vp(l), _vp(g), C.byref(eb)):
+            raise ValueError("oracle: the adjoint covers one body type and the FULL/LIN/NONLIN parts")
+        return g, eb.value
+
+    def dipole_grad(self, state):
+        s = _c128(state)
+        g = np.zeros_like(s)
+        if self.lib.orc_dipole_grad(self.h, _vp(s), _vp(g)):
+            raise ValueError("oracle: the adjoint covers one body type")
+        return g
+
+    def cheb_degree(self, tau, lo, hi):
+        return int(self.lib.orc_cheb_degree(float(tau), float(lo), float(hi)))
+
+    def strang_cheb(self, state, t0, t1, lo, hi, K, consistent=False):
+        s = _c128(state).copy()
+        if self.lib.orc_strang_cheb(self.h, RHS_NONLIN_CONSISTENT if consistent else RHS_NONLIN, _vp(s),
+                                    float(t0), float(t1), float(lo), float(hi), int(K)):
+            raise ValueError("oracle: invalid Chebyshev Strang step")
+        return s
+
+    def dipole_gradient(self, state, t0, dt, n_steps, lo, hi, K, consistent=False):
+        """(<x>(T), dJ/dc, final state) after n_steps Chebyshev-Strang steps."""
+        s = _c128(state).copy()
+        J = C.c_double(0.0)
+        g = np.zeros(max(self.n_laser, 1))
+        if self.lib.orc_dipole_gradient(self.h, RHS_NONLIN_CONSISTENT if consistent else RHS_NONLIN, _vp(s),
+                                        float(t0), float(dt), int(n_steps), float(lo), float(hi), int(K),
+                                        C.byref(J), _dp(g)):
+            raise ValueError("oracle: invalid gradient request")
+        return J.value, g[: self.n_laser].copy(), s
 
     def orthonormalize(self, orbs):
         o = _c128(orbs).copy(order="F")

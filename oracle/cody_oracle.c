@@ -73,6 +73,14 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ======================================================================== */
 /* Grid helpers                                                              */
 /* ======================================================================== */
@@ -1402,5 +1410,609 @@ int orc_normalize(const orc_sys* s, cplx* C) {
   for (int i = 0; i < s->nC; ++i) n2 += creal(conj(C[i]) * C[i]);
   double nrm = sqrt(n2);
   for (int i = 0; i < s->nC; ++i) C[i] /= nrm;
+  return 0;
+}
+
+/* ======================================================================== */
+/* Adjoint: vector-Jacobian products and the gradient of <x>(T)               */
+/* ======================================================================== */
+/* The reference has no adjoint.  What follows is the transpose, for one body type, of the maps
+ * above: the right-hand sides of orc_time_derivative / _nonlin / _orbs_lin (and their
+ * difference), the dipole, a Strang step whose linear half-steps are a fixed-degree Chebyshev
+ * expansion of exp(-i tau h(E)), and its checkpoint-free discrete adjoint.  Gradients are taken
+ * with respect to the real inner product
+ *   <a, b> = Re( sum_C conj(a) b + sum_orbitals sum_x w(x) conj(a(x)) b(x) ),
+ * in which h = T + V_ext + E x is self-adjoint. */
+
+/* (T + V_ext + E x) f for one orbital (one body type) */
+static void onebody(const orc_sys* s, const cplx* f, double E, cplx* out) {
+  kinetic(s, f, 0, out);
+  for (int g = 0; g < s->nG; ++g) out[g] += ((s->vext ? s->vext[g] : 0.0) + E * s->x[g]) * f[g];
+}
+
+/* out[jC] += f conj(hLin[code]) D[iC] over the singles and doubles tables: the conjugate
+ * transpose of the H-apply of apply_h1_fill_rdm1 / apply_h2_fill_rdm2 (one body type) */
+static void ci_apply_t(const orc_sys* s, const cplx* D, const cplx* h1Lin, const cplx* h2Lin, cplx* out) {
+  const cfg_t* c = &s->cfg[0];
+  for (int iC = 0; iC < s->nC; ++iC) {
+    for (int k = 0; k < c->sN[iC]; ++k) {
+      int q = k + iC * c->sMax;
+      out[c->sExc[q]] += c->sFac[q] * conj(h1Lin[c->sCode[q]]) * D[iC];
+    }
+    if (!h2Lin) continue;
+    for (int k = 0; k < c->dN[iC]; ++k) {
+      int q = k + iC * c->dMax;
+      out[c->dExc[q]] += c->dFac[q] * conj(h2Lin[c->dCode[q]]) * D[iC];
+    }
+  }
+}
+
+/* transition densities lin[code] = sum f conj(bra[jC]) ket[iC] (the fill of apply_h*_fill_rdm*
+ * with two vectors; bra = ket gives rdm1Lin / rdm2Lin) */
+static void ci_transition(const orc_sys* s, const cplx* bra, const cplx* ket, cplx* lin1, cplx* lin2) {
+  const cfg_t* c = &s->cfg[0];
+  int nO2 = s->nO * s->nO;
+  memset(lin1, 0, sizeof(cplx) * nO2);
+  memset(lin2, 0, sizeof(cplx) * nO2 * nO2);
+  for (int iC = 0; iC < s->nC; ++iC) {
+    for (int k = 0; k < c->sN[iC]; ++k) {
+      int q = k + iC * c->sMax;
+      lin1[c->sCode[q]] += c->sFac[q] * conj(bra[c->sExc[q]]) * ket[iC];
+    }
+    for (int k = 0; k < c->dN[iC]; ++k) {
+      int q = k + iC * c->dMax;
+      lin2[c->dCode[q]] += c->dFac[q] * conj(bra[c->dExc[q]]) * ket[iC];
+    }
+  }
+}
+
+/* gC += d/dC of Re sum conj(rho1bar) rho1 + Re sum conj(rho2bar) rho2 (either may be NULL) */
+static void rdm_grad(const orc_sys* s, const cplx* C, const cplx* rho1bar, const cplx* rho2bar, cplx* gC) {
+  int nO = s->nO;
+  size_t n4 = (size_t)nO * nO * nO * nO;
+  cplx* l1 = (cplx*)calloc(nO * nO, sizeof(cplx));
+  cplx* l2 = (cplx*)calloc(n4, sizeof(cplx));
+  if (rho1bar) h1_pack(s, rho1bar, l1); /* the adjoint of the unpack permutation */
+  if (rho2bar) h2_pack(s, rho2bar, l2); /* the antisymmetrising pack is the unpack's transpose */
+  apply_h1_fill_rdm1(s, C, NULL, gC, l1);
+  apply_h2_fill_rdm2(s, C, NULL, gC, l2);
+  ci_apply_t(s, C, l1, l2, gC);
+  free(l1);
+  free(l2);
+}
+
+/* X = A^{-1} B for a small complex n x n A (column-major), Gauss-Jordan with partial pivoting */
+static int csolve(int n, const cplx* A, const cplx* B, cplx* X) {
+  cplx* M = (cplx*)malloc(sizeof(cplx) * n * n);
+  memcpy(M, A, sizeof(cplx) * n * n);
+  memcpy(X, B, sizeof(cplx) * n * n);
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < n; ++i)
+      if (cabs(M[i + k * n]) > cabs(M[piv + k * n])) piv = i;
+    if (cabs(M[piv + k * n]) == 0.0) {
+      free(M);
+      return -1;
+    }
+    if (piv != k)
+      for (int j = 0; j < n; ++j) {
+        cplx tm = M[k + j * n];
+        M[k + j * n] = M[piv + j * n];
+        M[piv + j * n] = tm;
+        tm = X[k + j * n];
+        X[k + j * n] = X[piv + j * n];
+        X[piv + j * n] = tm;
+      }
+    cplx inv = 1.0 / M[k + k * n];
+    for (int j = 0; j < n; ++j) {
+      M[k + j * n] *= inv;
+      X[k + j * n] *= inv;
+    }
+    for (int i = 0; i < n; ++i) {
+      if (i == k) continue;
+      cplx f = M[i + k * n];
+      if (f == 0.0) continue;
+      for (int j = 0; j < n; ++j) {
+        M[i + j * n] -= f * M[k + j * n];
+        X[i + j * n] -= f * X[k + j * n];
+      }
+    }
+  }
+  free(M);
+  return 0;
+}
+
+/* W_ab(x) = V[conj(phi_b) phi_a w](x) for ordered (a, b) from the unordered pairs of orc_meanfield_w */
+static cplx w_ab(const cplx* W, int nO, int nG, int a, int b, int g) {
+  return a >= b ? W[(size_t)pair_index(nO, b, a) * nG + g] : conj(W[(size_t)pair_index(nO, a, b) * nG + g]);
+}
+
+/* grad = J^T lam for the right-hand side `rhs` (ORC_RHS_FULL, _LIN, _NONLIN, _NONLIN_CONSISTENT)
+ * at (state, t), and *ebar = <lam, df/dE>. */
+int orc_rhs_vjp(const orc_sys* s, int rhs, const cplx* state, double t, const cplx* lam, cplx* grad,
+                double* ebar) {
+  if (s->nBT != 1) return -1;
+  if (rhs != ORC_RHS_FULL && rhs != ORC_RHS_LIN && rhs != ORC_RHS_NONLIN && rhs != ORC_RHS_NONLIN_CONSISTENT)
+    return -1;
+  const int nC = s->nC, nO = s->nO, nG = s->nG, P = nO * (nO + 1) / 2;
+  const size_t n2 = (size_t)nO * nO, n4 = n2 * n2, nOG = (size_t)nO * nG;
+  const cplx* C = state;
+  const cplx* orbs = state + nC;
+  const double E = laser_field(s, t);
+  cplx* gC = grad;
+  cplx* gO = grad + nC;
+  memset(grad, 0, sizeof(cplx) * (nC + nOG));
+  cplx* LO = (cplx*)malloc(sizeof(cplx) * nOG);
+  cplx* hphi = (cplx*)malloc(sizeof(cplx) * nOG);
+  cplx* hbar = (cplx*)calloc(nOG, sizeof(cplx));
+  cplx* tmp = (cplx*)malloc(sizeof(cplx) * nG);
+  for (size_t i = 0; i < nOG; ++i) LO[i] = I * lam[nC + i]; /* the adjoint of the factor -i */
+  for (int j = 0; j < nO; ++j) onebody(s, orbs + (size_t)j * nG, E, hphi + (size_t)j * nG);
+  if (rhs == ORC_RHS_LIN) {
+    memcpy(hbar, LO, sizeof(cplx) * nOG);
+  } else {
+    const double ah = rhs == ORC_RHS_NONLIN ? 0.0 : 1.0, bh = rhs == ORC_RHS_NONLIN_CONSISTENT ? -1.0 : 0.0;
+    cplx* rdm1 = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* rdm2 = (cplx*)malloc(sizeof(cplx) * n4);
+    cplx* R = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* rho2p = (cplx*)calloc(n4, sizeof(cplx));
+    cplx* h1 = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* h2 = (cplx*)malloc(sizeof(cplx) * n4);
+    cplx* W = (cplx*)malloc(sizeof(cplx) * (size_t)nG * P);
+    cplx* draw = (cplx*)calloc(nOG, sizeof(cplx));
+    cplx* inp = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* h1Lin = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* h2Lin = (cplx*)malloc(sizeof(cplx) * n4);
+    cplx* dCraw = (cplx*)calloc(nC, sizeof(cplx));
+    cplx* D = (cplx*)malloc(sizeof(cplx) * nC);
+    cplx* l1 = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* l2 = (cplx*)malloc(sizeof(cplx) * n4);
+    cplx* h1b = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* h2b = (cplx*)malloc(sizeof(cplx) * n4);
+    cplx* B = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* dbar = (cplx*)malloc(sizeof(cplx) * nOG);
+    cplx* Wbar = (cplx*)calloc(n2 * nG, sizeof(cplx));
+    cplx* r2pb = (cplx*)calloc(n4, sizeof(cplx));
+    cplx* r2b = (cplx*)calloc(n4, sizeof(cplx));
+    cplx* Rb = (cplx*)calloc(n2, sizeof(cplx));
+    cplx* A = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* Q = (cplx*)malloc(sizeof(cplx) * n2);
+    cplx* r1b = (cplx*)malloc(sizeof(cplx) * n2);
+    /* forward quantities */
+    orc_rdm1(s, C, rdm1);
+    orc_rdm2(s, C, rdm2);
+    orc_reg_inverse(s, rdm1, R);
+    for (int l2 = 0; l2 < nO; ++l2)
+      for (int j1 = 0; j1 < nO; ++j1)
+        for (int k1 = 0; k1 < nO; ++k1)
+          for (int i2 = 0; i2 < nO; ++i2)
+            for (int i1 = 0; i1 < nO; ++i1)
+              rho2p[IDX4(i1, i2, j1, l2, nO)] += rdm2[IDX4(i1, i2, k1, l2, nO)] * R[k1 + j1 * nO];
+    for (int j = 0; j < nO; ++j)
+      for (int i = 0; i < nO; ++i) h1[i + j * nO] = inner(s, orbs + (size_t)i * nG, hphi + (size_t)j * nG);
+    orc_meanfield_w(s, orbs, W);
+    for (int a = 0; a < nO; ++a)
+      for (int b = 0; b < nO; ++b)
+        for (int j1 = 0; j1 < nO; ++j1)
+          for (int i1 = 0; i1 < nO; ++i1) {
+            cplx acc = 0.0;
+            for (int g = 0; g < nG; ++g)
+              acc += conj(orbs[(size_t)i1 * nG + g]) * w_ab(W, nO, nG, a, b, g) * orbs[(size_t)j1 * nG + g] * s->w[g];
+            h2[IDX4(i1, b, j1, a, nO)] = 0.5 * acc;
+          }
+    for (int g = 0; g < nG; ++g)
+      for (int a = 0; a < nO; ++a)
+        for (int b = 0; b < nO; ++b) {
+          cplx wv = w_ab(W, nO, nG, a, b, g);
+          for (int j = 0; j < nO; ++j) {
+            cplx acc = 0.0;
+            for (int k1 = 0; k1 < nO; ++k1) acc += orbs[(size_t)k1 * nG + g] * rho2p[IDX4(k1, a, j, b, nO)];
+            draw[(size_t)j * nG + g] += wv * acc;
+          }
+        }
+    for (size_t i = 0; i < nOG; ++i) draw[i] += ah * hphi[i];
+    for (int j = 0; j < nO; ++j)
+      for (int i = 0; i < nO; ++i) inp[i + j * nO] = inner(s, orbs + (size_t)i * nG, draw + (size_t)j * nG);
+    h1_pack(s, h1, h1Lin);
+    h2_pack(s, h2, h2Lin);
+    apply_h1_fill_rdm1(s, C, NULL, dCraw, h1Lin);
+    apply_h2_fill_rdm2(s, C, NULL, dCraw, h2Lin);
+    cplx ov = 0.0, alpha = 0.0;
+    for (int i = 0; i < nC; ++i) ov += conj(C[i]) * dCraw[i];
+    /* CI projection and H-apply (dC = -i [(1 - C C^H) H(h1, h2) C]) */
+    for (int i = 0; i < nC; ++i) alpha += conj(C[i]) * (I * lam[i]);
+    for (int i = 0; i < nC; ++i) {
+      cplx LC = I * lam[i];
+      D[i] = LC - alpha * C[i];
+      gC[i] = -conj(alpha) * dCraw[i] - conj(ov) * LC;
+    }
+    ci_apply_t(s, D, h1Lin, h2Lin, gC);
+    ci_transition(s, C, D, l1, l2);
+    rdm1_unpack(s, l1, h1b);
+    rdm2_unpack(s, l2, h2b);
+    /* orbital projection dphi = -i [(1 - P) draw + bh hphi] */
+    for (int j = 0; j < nO; ++j)
+      for (int i = 0; i < nO; ++i) B[i + j * nO] = inner(s, orbs + (size_t)i * nG, LO + (size_t)j * nG);
+    for (int j = 0; j < nO; ++j)
+      for (int g = 0; g < nG; ++g) {
+        cplx acc = LO[(size_t)j * nG + g];
+        for (int i = 0; i < nO; ++i) acc -= orbs[(size_t)i * nG + g] * B[i + j * nO];
+        dbar[(size_t)j * nG + g] = acc;
+      }
+    for (int i = 0; i < nO; ++i)
+      for (int g = 0; g < nG; ++g) {
+        cplx acc = 0.0;
+        for (int j = 0; j < nO; ++j)
+          acc += conj(inp[i + j * nO]) * LO[(size_t)j * nG + g] + conj(B[i + j * nO]) * draw[(size_t)j * nG + g];
+        gO[(size_t)i * nG + g] -= acc;
+      }
+    /* h1(i,j) = <phi_i|hphi_j> and the draw's h phi term */
+    for (int j = 0; j < nO; ++j)
+      for (int g = 0; g < nG; ++g) {
+        cplx acc = ah * dbar[(size_t)j * nG + g] + bh * LO[(size_t)j * nG + g];
+        for (int i = 0; i < nO; ++i) acc += h1b[i + j * nO] * orbs[(size_t)i * nG + g];
+        hbar[(size_t)j * nG + g] = acc;
+      }
+    for (int i = 0; i < nO; ++i)
+      for (int g = 0; g < nG; ++g) {
+        cplx acc = 0.0;
+        for (int j = 0; j < nO; ++j) acc += conj(h1b[i + j * nO]) * hphi[(size_t)j * nG + g];
+        gO[(size_t)i * nG + g] += acc;
+      }
+    /* correlation sum_{a,b,k1} W_ab phi_k1 rho2'(k1,a,j1,b) and h2(i1,b,j1,a) = 1/2 <phi_i1|W_ab phi_j1> */
+    for (int g = 0; g < nG; ++g) {
+      for (int a = 0; a < nO; ++a)
+        for (int b = 0; b < nO; ++b) {
+          cplx wv = w_ab(W, nO, nG, a, b, g), acc = 0.0;
+          for (int k1 = 0; k1 < nO; ++k1) {
+            cplx pk = orbs[(size_t)k1 * nG + g];
+            for (int j1 = 0; j1 < nO; ++j1) {
+              cplx dj = dbar[(size_t)j1 * nG + g], pj = orbs[(size_t)j1 * nG + g];
+              cplx r = conj(rho2p[IDX4(k1, a, j1, b, nO)]);
+              acc += r * conj(pk) * dj + 0.5 * h2b[IDX4(k1, b, j1, a, nO)] * pk * conj(pj);
+              gO[(size_t)k1 * nG + g] += conj(wv) * r * dj + 0.5 * conj(h2b[IDX4(k1, b, j1, a, nO)]) * wv * pj;
+              gO[(size_t)j1 * nG + g] += 0.5 * h2b[IDX4(k1, b, j1, a, nO)] * conj(wv) * pk;
+              r2pb[IDX4(k1, a, j1, b, nO)] += s->w[g] * conj(wv * pk) * dj;
+            }
+          }
+          Wbar[(a + b * nO) * (size_t)nG + g] = acc;
+        }
+    }
+    /* rho2' = rho2 x_3 R */
+    for (int l2 = 0; l2 < nO; ++l2)
+      for (int j1 = 0; j1 < nO; ++j1)
+        for (int k = 0; k < nO; ++k)
+          for (int i2 = 0; i2 < nO; ++i2)
+            for (int i1 = 0; i1 < nO; ++i1) {
+              cplx pb = r2pb[IDX4(i1, i2, j1, l2, nO)];
+              r2b[IDX4(i1, i2, k, l2, nO)] += pb * conj(R[k + j1 * nO]);
+              Rb[k + j1 * nO] += pb * conj(rdm2[IDX4(i1, i2, k, l2, nO)]);
+            }
+    /* R = (rho1^2 + eps)^{-1} rho1: rho1bar = Q - Q R^H rho1 - rho1 Q R^H, Q = (rho1^2 + eps)^{-1} Rbar */
+    for (int j = 0; j < nO; ++j)
+      for (int i = 0; i < nO; ++i) {
+        cplx acc = i == j ? s->reg : 0.0;
+        for (int k = 0; k < nO; ++k) acc += rdm1[i + k * nO] * rdm1[k + j * nO];
+        A[i + j * nO] = acc;
+      }
+    csolve(nO, A, Rb, Q);
+    for (int j = 0; j < nO; ++j)
+      for (int i = 0; i < nO; ++i) {
+        cplx acc = Q[i + j * nO];
+        for (int k = 0; k < nO; ++k)
+          for (int m = 0; m < nO; ++m) {
+            acc -= Q[i + k * nO] * conj(R[m + k * nO]) * rdm1[m + j * nO];
+            acc -= rdm1[i + k * nO] * Q[k + m * nO] * conj(R[j + m * nO]);
+          }
+        r1b[i + j * nO] = acc;
+      }
+    rdm_grad(s, C, r1b, r2b, gC);
+    /* W_kl = V [conj(phi_l) phi_k w] for the stored pairs (l <= k): sbar = V (w Wbar) */
+    for (int l = 0; l < nO; ++l)
+      for (int k = l; k < nO; ++k) {
+        for (int g = 0; g < nG; ++g) {
+          cplx v = Wbar[(k + l * nO) * (size_t)nG + g];
+          if (k != l) v += conj(Wbar[(l + k * nO) * (size_t)nG + g]);
+          tmp[g] = s->w[g] * v;
+        }
+        for (int g = 0; g < nG; ++g) {
+          cplx sb = 0.0;
+          const double* row = s->vint + (size_t)g * nG;
+          for (int i = 0; i < nG; ++i) sb += row[i] * tmp[i];
+          gO[(size_t)k * nG + g] += sb * orbs[(size_t)l * nG + g];
+          gO[(size_t)l * nG + g] += conj(sb) * orbs[(size_t)k * nG + g];
+        }
+      }
+    free(rdm1), free(rdm2), free(R), free(rho2p), free(h1), free(h2), free(W), free(draw), free(inp);
+    free(h1Lin), free(h2Lin), free(dCraw), free(D), free(l1), free(l2), free(h1b), free(h2b), free(B);
+    free(dbar), free(Wbar), free(r2pb), free(r2b), free(Rb), free(A), free(Q), free(r1b);
+  }
+  /* hphi = (T + V + E x) phi: self-adjoint in the weighted product */
+  double eb = 0.0;
+  for (int j = 0; j < nO; ++j) {
+    onebody(s, hbar + (size_t)j * nG, E, tmp);
+    for (int g = 0; g < nG; ++g) {
+      gO[(size_t)j * nG + g] += tmp[g];
+      eb += s->w[g] * s->x[g] * creal(conj(hbar[(size_t)j * nG + g]) * orbs[(size_t)j * nG + g]);
+    }
+  }
+  *ebar = eb;
+  free(LO), free(hphi), free(hbar), free(tmp);
+  return 0;
+}
+
+/* gradient of <x> = sum_kl rdm1(l,k) <phi_k|x|phi_l> (orc_dipole) */
+int orc_dipole_grad(const orc_sys* s, const cplx* state, cplx* grad) {
+  if (s->nBT != 1) return -1;
+  const int nC = s->nC, nO = s->nO, nG = s->nG;
+  const cplx* orbs = state + nC;
+  cplx* rdm1 = (cplx*)malloc(sizeof(cplx) * nO * nO);
+  cplx* X = (cplx*)malloc(sizeof(cplx) * nO * nO);
+  orc_rdm1(s, state, rdm1);
+  for (int l = 0; l < nO; ++l)
+    for (int k = 0; k < nO; ++k) {
+      cplx acc = 0.0;
+      for (int g = 0; g < nG; ++g)
+        acc += conj(orbs[(size_t)k * nG + g]) * s->x[g] * orbs[(size_t)l * nG + g] * s->w[g];
+      X[k + l * nO] = acc;
+    }
+  memset(grad, 0, sizeof(cplx) * (nC + (size_t)nO * nG));
+  for (int k = 0; k < nO; ++k)
+    for (int g = 0; g < nG; ++g) {
+      cplx acc = 0.0;
+      for (int l = 0; l < nO; ++l) acc += rdm1[l + k * nO] * orbs[(size_t)l * nG + g];
+      grad[nC + (size_t)k * nG + g] = 2.0 * s->x[g] * acc;
+    }
+  rdm_grad(s, state, X, NULL, grad); /* rho1bar(l,k) = conj(X(k,l)) = X(l,k) */
+  free(rdm1);
+  free(X);
+  return 0;
+}
+
+/* Bessel J_0..J_K(z), z > 0: Miller's backward recurrence normalised by J_0 + 2 sum J_2k = 1 */
+static void bessel_j(int K, double z, double* J) {
+  int M = K > (int)z ? K : (int)z;
+  M = 2 * ((M + 20 + (int)sqrt(40.0 * (M + 1))) / 2);
+  double jp = 0.0, jc = 1e-300, norm = 0.0;
+  double* v = (double*)malloc(sizeof(double) * (M + 1));
+  v[M] = jc;
+  for (int k = M; k >= 1; --k) {
+    double jm = (2.0 * k / z) * jc - jp;
+    jp = jc;
+    jc = jm;
+    v[k - 1] = jc;
+    if (fabs(jc) > 1e250) { /* rescale */
+      for (int q = k - 1; q <= M; ++q) v[q] *= 1e-250;
+      jp *= 1e-250;
+      jc *= 1e-250;
+    }
+  }
+  norm = v[0];
+  for (int k = 2; k <= M; k += 2) norm += 2.0 * v[k];
+  for (int k = 0; k <= K; ++k) J[k] = k <= M ? v[k] / norm : 0.0;
+  free(v);
+}
+
+/* smallest degree K with |J_k(tau r)| < 1e-17 for k > K, r = (hi - lo)/2 */
+int orc_cheb_degree(double tau, double lo, double hi) {
+  double z = fabs(tau) * 0.5 * (hi - lo);
+  int Kmax = (int)(z + 60 + 10 * cbrt(z + 1));
+  double* J = (double*)malloc(sizeof(double) * (Kmax + 1));
+  bessel_j(Kmax, z, J);
+  int K = Kmax;
+  while (K > 1 && fabs(J[K]) < 1e-17 && fabs(J[K - 1]) < 1e-17) --K;
+  free(J);
+  return K;
+}
+
+/* a_k of exp(-i tau (c + r y)) = sum_k a_k T_k(y): e^{-i tau c} (2 - delta_k0) (-i)^k J_k(tau r) */
+static void cheb_coeffs(double tau, double lo, double hi, int K, cplx* a) {
+  double c = 0.5 * (hi + lo), r = 0.5 * (hi - lo);
+  double* J = (double*)malloc(sizeof(double) * (K + 1));
+  if (tau >= 0) {
+    bessel_j(K, tau * r, J);
+  } else { /* J_k(-z) = (-1)^k J_k(z) */
+    bessel_j(K, -tau * r, J);
+    for (int k = 1; k <= K; k += 2) J[k] = -J[k];
+  }
+  cplx ph = cexp(-I * tau * c), mi = 1.0;
+  for (int k = 0; k <= K; ++k) {
+    a[k] = (k == 0 ? 1.0 : 2.0) * mi * J[k] * ph;
+    mi *= -I;
+  }
+  free(J);
+}
+
+/* orbs <- exp(-i tau h(E)) orbs by the degree-K expansion on [lo, hi]; with U non-null the terms
+ * u_k = T_k(h~) phi_j (h~ = (h - c)/r) are kept at U[(k nO + j) nG] */
+static void cheb_apply(const orc_sys* s, cplx* orbs, double tau, double E, double lo, double hi, int K, cplx* U) {
+  const int nO = s->nO, nG = s->nG;
+  const double c = 0.5 * (hi + lo), r = 0.5 * (hi - lo);
+  cplx* a = (cplx*)malloc(sizeof(cplx) * (K + 1));
+  cheb_coeffs(tau, lo, hi, K, a);
+  cplx* um = (cplx*)malloc(sizeof(cplx) * nG);
+  cplx* uc = (cplx*)malloc(sizeof(cplx) * nG);
+  cplx* un = (cplx*)malloc(sizeof(cplx) * nG);
+  cplx* out = (cplx*)malloc(sizeof(cplx) * nG);
+  for (int j = 0; j < nO; ++j) {
+    cplx* f = orbs + (size_t)j * nG;
+    memcpy(uc, f, sizeof(cplx) * nG);
+    for (int g = 0; g < nG; ++g) out[g] = a[0] * uc[g];
+    if (U) memcpy(U + (size_t)j * nG, uc, sizeof(cplx) * nG);
+    for (int k = 1; k <= K; ++k) {
+      onebody(s, uc, E, un);
+      for (int g = 0; g < nG; ++g) {
+        cplx ht = (un[g] - c * uc[g]) / r;
+        un[g] = k == 1 ? ht : 2.0 * ht - um[g];
+        out[g] += a[k] * un[g];
+      }
+      if (U) memcpy(U + ((size_t)k * nO + j) * nG, un, sizeof(cplx) * nG);
+      cplx* tp = um;
+      um = uc;
+      uc = un;
+      un = tp;
+    }
+    memcpy(f, out, sizeof(cplx) * nG);
+  }
+  free(a), free(um), free(uc), free(un), free(out);
+}
+
+/* lam <- A^T lam for the map above (orbital part; C passes through); returns <lam, dA/dE phi> */
+static double cheb_adjoint(const orc_sys* s, cplx* lamOrbs, double tau, double E, double lo, double hi, int K,
+                           const cplx* U) {
+  const int nO = s->nO, nG = s->nG;
+  const double c = 0.5 * (hi + lo), r = 0.5 * (hi - lo);
+  cplx* a = (cplx*)malloc(sizeof(cplx) * (K + 1));
+  cheb_coeffs(tau, lo, hi, K, a);
+  cplx* ub = (cplx*)malloc(sizeof(cplx) * (K + 1) * (size_t)nG);
+  cplx* hv = (cplx*)malloc(sizeof(cplx) * nG);
+  double eb = 0.0;
+  for (int j = 0; j < nO; ++j) {
+    cplx* lj = lamOrbs + (size_t)j * nG;
+    for (int k = 0; k <= K; ++k)
+      for (int g = 0; g < nG; ++g) ub[(size_t)k * nG + g] = conj(a[k]) * lj[g];
+    for (int k = K; k >= 1; --k) {
+      cplx* bk = ub + (size_t)k * nG;
+      const cplx* uprev = U + ((size_t)(k - 1) * nO + j) * nG;
+      double fac = k == 1 ? 1.0 : 2.0;
+      onebody(s, bk, E, hv);
+      for (int g = 0; g < nG; ++g) {
+        ub[(size_t)(k - 1) * nG + g] += fac * (hv[g] - c * bk[g]) / r;
+        if (k >= 2) ub[(size_t)(k - 2) * nG + g] -= bk[g];
+        eb += fac * s->w[g] * s->x[g] / r * creal(conj(bk[g]) * uprev[g]);
+      }
+    }
+    memcpy(lj, ub, sizeof(cplx) * nG);
+  }
+  free(a), free(ub), free(hv);
+  return eb;
+}
+
+static double laser_basis(const orc_sys* s, int k, double t) {
+  if (s->nLaser <= 0 || t < 0.0 || t > s->laserT) return 0.0;
+  double env = sin(M_PI * t / s->laserT);
+  return sin((k + 1) * M_PI * t / s->laserT) * env * env;
+}
+
+typedef struct {
+  cplx *U1, *U2, *y1, *y2, *y3, *y4; /* Chebyshev terms of both half-steps, RK4 stage inputs */
+} strang_tape;
+
+/* classical RK4 with the arithmetic of orc_rk4, recording the stage inputs */
+static void rk4_taped(const orc_sys* s, int rhs, cplx* state, double t0, double dt, strang_tape* tp) {
+  size_t n = state_size(s);
+  cplx* d = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* inc = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* tmp = (cplx*)malloc(sizeof(cplx) * n);
+  double dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
+  if (tp) memcpy(tp->y1, state, sizeof(cplx) * n);
+  eval_rhs(s, rhs, state, t0, d);
+  for (size_t i = 0; i < n; ++i) {
+    inc[i] = d[i] * dt6;
+    tmp[i] = state[i] + d[i] * dt2;
+  }
+  if (tp) memcpy(tp->y2, tmp, sizeof(cplx) * n);
+  eval_rhs(s, rhs, tmp, t0 + dt2, d);
+  for (size_t i = 0; i < n; ++i) {
+    inc[i] += d[i] * dt3;
+    tmp[i] = state[i] + d[i] * dt2;
+  }
+  if (tp) memcpy(tp->y3, tmp, sizeof(cplx) * n);
+  eval_rhs(s, rhs, tmp, t0 + dt2, d);
+  for (size_t i = 0; i < n; ++i) {
+    inc[i] += d[i] * dt3;
+    tmp[i] = state[i] + d[i] * dt;
+  }
+  if (tp) memcpy(tp->y4, tmp, sizeof(cplx) * n);
+  eval_rhs(s, rhs, tmp, t0 + dt, d);
+  for (size_t i = 0; i < n; ++i) state[i] += inc[i] + d[i] * dt6;
+  free(d), free(inc), free(tmp);
+}
+
+/* A(dt/2) B(dt) A(dt/2): A = degree-K Chebyshev exp(-i dt/2 h(E)) with E at the half-step's
+ * midpoint (the time at which the Lanczos step evaluates the linear part), B = RK4 on rhsB */
+static void strang_cheb_step(const orc_sys* s, int rhsB, cplx* state, double t0, double dt, double lo, double hi,
+                             int K, strang_tape* tp) {
+  cheb_apply(s, state + s->nC, 0.5 * dt, laser_field(s, t0 + 0.25 * dt), lo, hi, K, tp ? tp->U1 : NULL);
+  rk4_taped(s, rhsB, state, t0, dt, tp);
+  cheb_apply(s, state + s->nC, 0.5 * dt, laser_field(s, t0 + 0.75 * dt), lo, hi, K, tp ? tp->U2 : NULL);
+}
+
+int orc_strang_cheb(const orc_sys* s, int rhsB, cplx* state, double t0, double t1, double lo, double hi, int K) {
+  if (s->nBT != 1 || K < 1) return -1;
+  strang_cheb_step(s, rhsB, state, t0, t1 - t0, lo, hi, K, NULL);
+  return 0;
+}
+
+/* J = <x>(t0 + nSteps dt) after nSteps Strang-Chebyshev steps from `state` (overwritten by the
+ * final state) and gradc[k] = dJ/dc_k for the laser coefficients of orc_set_laser, by the
+ * discrete adjoint over all stored steps. */
+int orc_dipole_gradient(const orc_sys* s, int rhsB, cplx* state, double t0, double dt, int nSteps, double lo,
+                        double hi, int K, double* J, double* gradc) {
+  if (s->nBT != 1 || K < 1 || nSteps < 1) return -1;
+  const size_t n = state_size(s), nOG = (size_t)s->nO * s->nG;
+  cplx* traj = (cplx*)malloc(sizeof(cplx) * n * (nSteps + 1));
+  strang_tape tp;
+  tp.U1 = (cplx*)malloc(sizeof(cplx) * nOG * (K + 1));
+  tp.U2 = (cplx*)malloc(sizeof(cplx) * nOG * (K + 1));
+  tp.y1 = (cplx*)malloc(sizeof(cplx) * n);
+  tp.y2 = (cplx*)malloc(sizeof(cplx) * n);
+  tp.y3 = (cplx*)malloc(sizeof(cplx) * n);
+  tp.y4 = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* lam = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* yb = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* d1 = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* d2 = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* d3 = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* d4 = (cplx*)malloc(sizeof(cplx) * n);
+  cplx* g = (cplx*)malloc(sizeof(cplx) * n);
+  memcpy(traj, state, sizeof(cplx) * n);
+  for (int m = 0; m < nSteps; ++m) {
+    memcpy(traj + n * (m + 1), traj + n * m, sizeof(cplx) * n);
+    strang_cheb_step(s, rhsB, traj + n * (m + 1), t0 + m * dt, dt, lo, hi, K, NULL);
+  }
+  memcpy(state, traj + n * nSteps, sizeof(cplx) * n);
+  *J = orc_dipole(s, state);
+  for (int k = 0; k < s->nLaser; ++k) gradc[k] = 0.0;
+  orc_dipole_grad(s, state, lam);
+  for (int m = nSteps - 1; m >= 0; --m) {
+    const double tm = t0 + m * dt, dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
+    memcpy(yb, traj + n * m, sizeof(cplx) * n);
+    strang_cheb_step(s, rhsB, yb, tm, dt, lo, hi, K, &tp); /* recompute with the tape */
+    double e, ts[6] = {tm + 0.75 * dt, tm + dt, tm + dt2, tm + dt2, tm, tm + 0.25 * dt}, eb[6];
+    eb[0] = cheb_adjoint(s, lam + s->nC, 0.5 * dt, laser_field(s, ts[0]), lo, hi, K, tp.U2);
+    for (size_t i = 0; i < n; ++i) {
+      d4[i] = dt6 * lam[i];
+      d3[i] = dt3 * lam[i];
+      d2[i] = dt3 * lam[i];
+      d1[i] = dt6 * lam[i];
+    }
+    orc_rhs_vjp(s, rhsB, tp.y4, tm + dt, d4, g, &e);
+    eb[1] = e;
+    for (size_t i = 0; i < n; ++i) {
+      lam[i] += g[i];
+      d3[i] += dt * g[i];
+    }
+    orc_rhs_vjp(s, rhsB, tp.y3, tm + dt2, d3, g, &e);
+    eb[2] = e;
+    for (size_t i = 0; i < n; ++i) {
+      lam[i] += g[i];
+      d2[i] += dt2 * g[i];
+    }
+    orc_rhs_vjp(s, rhsB, tp.y2, tm + dt2, d2, g, &e);
+    eb[3] = e;
+    for (size_t i = 0; i < n; ++i) {
+      lam[i] += g[i];
+      d1[i] += dt2 * g[i];
+    }
+    orc_rhs_vjp(s, rhsB, tp.y1, tm, d1, g, &e);
+    eb[4] = e;
+    for (size_t i = 0; i < n; ++i) lam[i] += g[i];
+    eb[5] = cheb_adjoint(s, lam + s->nC, 0.5 * dt, laser_field(s, ts[5]), lo, hi, K, tp.U1);
+    for (int q = 0; q < 6; ++q)
+      for (int k = 0; k < s->nLaser; ++k) gradc[k] += eb[q] * laser_basis(s, k, ts[q]);
+  }
+  free(traj), free(tp.U1), free(tp.U2), free(tp.y1), free(tp.y2), free(tp.y3), free(tp.y4);
+  free(lam), free(yb), free(d1), free(d2), free(d3), free(d4), free(g);
   return 0;
 }

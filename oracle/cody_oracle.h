@@ -94,6 +94,19 @@ int orc_strang(const orc_sys* s, double complex* state, double t0, double t1, in
 /* Strang step with B integrated on `rhsB` (ORC_RHS_NONLIN: the reference's step) */
 int orc_strang_split(const orc_sys* s, int rhsB, double complex* state, double t0, double t1, int krylovDim,
                      double tol, int maxRestarts, const double* metric);
+/* Adjoint (one body type).  Gradients are with respect to Re(sum_C conj(a) b + sum_orbs sum_x w
+ * conj(a) b).  orc_rhs_vjp: grad = J^T lam of the right-hand side rhs (FULL, LIN, NONLIN,
+ * NONLIN_CONSISTENT) at (state, t), *ebar = <lam, df/dE>.  orc_strang_cheb: one Strang step with
+ * the linear half-steps as a degree-K Chebyshev expansion of exp(-i dt/2 h) on [lo, hi].
+ * orc_dipole_gradient: <x> after nSteps such steps and its gradient in the laser coefficients. */
+int orc_rhs_vjp(const orc_sys* s, int rhs, const double complex* state, double t, const double complex* lam,
+                double complex* grad, double* ebar);
+int orc_dipole_grad(const orc_sys* s, const double complex* state, double complex* grad);
+int orc_cheb_degree(double tau, double lo, double hi);
+int orc_strang_cheb(const orc_sys* s, int rhsB, double complex* state, double t0, double t1, double lo, double hi,
+                    int K);
+int orc_dipole_gradient(const orc_sys* s, int rhsB, double complex* state, double t0, double dt, int nSteps,
+                        double lo, double hi, int K, double* J, double* gradc);
 int orc_orthonormalize(const orc_sys* s, double complex* orbs);
 int orc_normalize(const orc_sys* s, double complex* coeffs);
 
@@ -101,6 +114,8 @@ int orc_normalize(const orc_sys* s, double complex* coeffs);
 int orc_symeig(int n, const double* a, double* evals, double* evecs);
 
 int orc_num_threads(void);
+/* OpenMP threads of later calls (the single-thread CPU baseline sets 1) */
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }
