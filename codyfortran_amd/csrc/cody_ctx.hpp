@@ -67,6 +67,47 @@ struct PropBuffers {
   }
 };
 
+// Work buffers of the adjoint (vector-Jacobian products, Chebyshev half-steps, checkpointed
+// gradient), allocated on first use.
+struct AdjBuffers {
+  bool ready = false;
+  double2 *f = nullptr, *D = nullptr, *tmpC = nullptr, *Bpart = nullptr, *B = nullptr, *inp = nullptr;
+  double2 *h1b = nullptr, *h2b = nullptr, *rho2p = nullptr, *r2pb = nullptr, *r2b = nullptr;
+  double2 *lin1 = nullptr, *lin2 = nullptr, *dbar = nullptr, *hbar = nullptr, *X = nullptr;
+  double *STadj = nullptr, *WTadj = nullptr, *epart = nullptr, *eacc = nullptr, *J = nullptr;
+  // Chebyshev half-steps exp(-i dt/2 h(E)) on [lo, hi] of degree K (cody_set_chebyshev)
+  int K = 0;
+  double lo = 0.0, hi = 0.0, dt = 0.0;
+  std::vector<double2> a;  // expansion coefficients a_0..a_K
+  double2 *U1 = nullptr, *U2 = nullptr, *cacc = nullptr;
+  double2* cb[3] = {};
+  // Strang-step tape, adjoint state and checkpoints
+  double2* y[4] = {};
+  double2* dl[4] = {};
+  double2 *rk_d = nullptr, *rk_inc = nullptr, *lam = nullptr, *g = nullptr, *ys = nullptr;
+  double2 *ck = nullptr, *seg = nullptr;
+  size_t nck = 0, nseg = 0;
+  double* ebar = nullptr;
+  size_t nebar = 0;
+  void free_cheb() {
+    void* p[] = {U1, U2, cacc, cb[0], cb[1], cb[2]};
+    for (void* q : p)
+      if (q) hipFree(q);
+    U1 = U2 = cacc = nullptr;
+    cb[0] = cb[1] = cb[2] = nullptr;
+    K = 0;
+  }
+  void release() {
+    free_cheb();
+    void* p[] = {f,     D,     tmpC,  Bpart, B,     inp,   h1b,   h2b,   rho2p, r2pb,  r2b,   lin1,  lin2, dbar,
+                 hbar,  X,     STadj, WTadj, epart, eacc,  J,     y[0],  y[1],  y[2],  y[3],  dl[0], dl[1], dl[2],
+                 dl[3], rk_d,  rk_inc, lam,  g,     ys,    ck,    seg,   ebar};
+    for (void* q : p)
+      if (q) hipFree(q);
+    *this = AdjBuffers();
+  }
+};
+
 struct cody_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -106,6 +147,7 @@ struct cody_ctx {
   double2 *d_h1 = nullptr, *d_hphi = nullptr, *d_dCraw = nullptr;
   double2 *d_draw = nullptr, *d_part = nullptr, *d_part1 = nullptr;
   PropBuffers prop;
+  AdjBuffers adj;
   // contraction operands derived from V (allocated by cody_set_precision): FP64 fragment-tiled
   // copy, int8 slices, FP32 and BF16 splits
   double* d_Vt = nullptr;
@@ -221,6 +263,7 @@ struct cody_ctx {
     for (void* p : ptrs)
       if (p) hipFree(p);
     prop.release();
+    adj.release();
     void* lp[] = {d_Vt, d_Vi8, d_Si8, d_eV, d_eS, d_Vhi, d_Vlo, d_Shi, d_Slo, d_Vbf, d_Sbf};
     for (void* p : lp)
       if (p) hipFree(p);

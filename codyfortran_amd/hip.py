@@ -18,6 +18,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cody_mctdhx.h")
 
 PREC_FP64, PREC_SPLIT_FP32, PREC_BF16X3, PREC_INT8_SLICED = 0, 1, 2, 3
 SPLIT_REFERENCE, SPLIT_CONSISTENT = 0, 1
+PART_FULL, PART_LIN, PART_NONLIN = 0, 1, 2
 
 _lib = None
 
@@ -84,6 +85,11 @@ def load():
         "cody_save_state_files": (i, [C.c_char_p, i, i, i, ip, vp]),
         "cody_load_state_files": (i, [C.c_char_p, i, i, i, ip, vp]),
         "cody_save_two_rdm": (i, [C.c_char_p, i, ip, vp]),
+        "cody_time_derivative_vjp_dev": (i, [vp, i, vp, vp, vp, d, vp, vp]),
+        "cody_dipole_grad_dev": (i, [vp, vp, vp, vp]),
+        "cody_set_chebyshev": (i, [vp, d, d, dp, dp, ip]),
+        "cody_strang_cheb": (i, [vp, d, d]),
+        "cody_dipole_gradient": (i, [vp, d, d, i, i, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -213,9 +219,12 @@ class Context:
     def set_interaction_softyukawa(self, strength=1.0, soft1=1.0, soft2=0.0, damp=0.0):
         _check(self.lib.cody_set_interaction_softyukawa(self.h, strength, soft1, soft2, damp))
 
+    n_laser = 0
+
     def set_laser(self, c, T):
         c = np.ascontiguousarray(c, dtype=np.float64)
         _check(self.lib.cody_set_laser(self.h, len(c), _dp(c), float(T)))
+        self.n_laser = len(c)
 
     def set_precision(self, mode):
         _check(self.lib.cody_set_precision(self.h, int(mode)))
@@ -328,6 +337,34 @@ class Context:
         ok = C.c_int()
         _check(self.lib.cody_strang(self.h, t0, t1, krylov_dim, tol, max_restarts, int(metric), C.byref(ok)))
         return bool(ok.value)
+
+    # ---------------------------------------------------------------- adjoint
+    def vjp_dev(self, part: int, d_state: int, d_lam: int, d_grad: int, t: float, d_ebar: int, stream: int = 0):
+        """d_grad = J^T d_lam of the right-hand side `part` (PART_FULL / PART_LIN / PART_NONLIN) and
+        the double at d_ebar = <d_lam, df/dE>, in the metric of the SIL (unit on C, w on orbitals)."""
+        _check(self.lib.cody_time_derivative_vjp_dev(self.h, int(part), C.c_void_p(d_state), C.c_void_p(d_lam),
+                                                     C.c_void_p(d_grad), float(t), C.c_void_p(d_ebar),
+                                                     C.c_void_p(stream)))
+
+    def dipole_grad_dev(self, d_state: int, d_grad: int, stream: int = 0):
+        _check(self.lib.cody_dipole_grad_dev(self.h, C.c_void_p(d_state), C.c_void_p(d_grad), C.c_void_p(stream)))
+
+    def set_chebyshev(self, dt: float, emax: float):
+        """Fixes the Chebyshev half-steps for steps of dt and fields |E| <= emax: (lo, hi, degree)."""
+        lo, hi, k = C.c_double(), C.c_double(), C.c_int()
+        _check(self.lib.cody_set_chebyshev(self.h, float(dt), float(emax), C.byref(lo), C.byref(hi), C.byref(k)))
+        return lo.value, hi.value, k.value
+
+    def strang_cheb(self, t0: float, t1: float):
+        _check(self.lib.cody_strang_cheb(self.h, float(t0), float(t1)))
+
+    def dipole_gradient(self, t0: float, dt: float, n_steps: int, checkpoint_every: int = 50):
+        """(<x>(t0 + n dt), dJ/dc) from the context's state, which is left at the final time."""
+        J = C.c_double()
+        g = np.zeros(max(self.n_laser, 1))
+        _check(self.lib.cody_dipole_gradient(self.h, float(t0), float(dt), int(n_steps), int(checkpoint_every),
+                                             C.byref(J), _dp(g)))
+        return J.value, g[: self.n_laser].copy()
 
     # ---------------------------------------------------------------- measurement
     def profile_contraction(self, iters=50):

@@ -194,6 +194,38 @@ int cody_rk4(cody_ctx* ctx, double t0, double t1, int full);
 int cody_strang(cody_ctx* ctx, double t0, double t1, int krylovDim, double tol, int maxRestarts,
                 int useMetric, int* converged);
 
+/* ---------------------------------------------------------------- adjoint (differentiation) */
+/* No counterpart in the reference, which has no adjoint: the backward pass of the right-hand
+ * side and of a Strang propagation, for gradients of the final dipole <x>(T) with respect to the
+ * laser coefficients of cody_set_laser.  Gradients are taken with respect to the real inner
+ * product <a, b> = Re(sum_C conj(a) b + sum_orbitals sum_x w(x) conj(a(x)) b(x)) (the SIL
+ * metric), in which h = T + V + E(t) x is self-adjoint.  They need the FP64 contraction. */
+#define CODY_PART_FULL 0   /* Method_TimeDerivative (S_Method_Mb_OrbBased_Mctdhx.f90:101-149) */
+#define CODY_PART_LIN 1    /* TimeDerivativeOrbsLin (:152-177) */
+#define CODY_PART_NONLIN 2 /* the nonlinear split part of cody_set_split */
+/* d_grad = J^T d_lam for the right-hand side `part` at (d_state, t) and *d_ebar = <d_lam, df/dE>
+ * (device pointers; d_ebar one double), enqueued on `stream` (NULL: the context's stream). */
+int cody_time_derivative_vjp_dev(cody_ctx* ctx, int part, const void* d_state, const void* d_lam, void* d_grad,
+                                 double t, void* d_ebar, void* stream);
+/* d_grad = gradient of <x> (cody_dipole_dev) at d_state. */
+int cody_dipole_grad_dev(cody_ctx* ctx, const void* d_state, void* d_grad, void* stream);
+/* Fixes the linear half-steps of cody_strang_cheb / cody_dipole_gradient: a Chebyshev expansion of
+ * exp(-i dt/2 h(E)) on Gershgorin bounds [lo, hi] of h(E) for |E| <= emax, of the degree that
+ * brings the dropped Bessel terms below 1e-17.  A fixed polynomial makes the step a smooth
+ * function of the field (a Krylov step's dimension can change with it). */
+int cody_set_chebyshev(cody_ctx* ctx, double dt, double emax, double* lo, double* hi, int* degree);
+/* One Strang step of the context's state, A(dt/2) B(dt) A(dt/2) as in cody_strang
+ * (S_Propagator_SplitStep_Order2.f90:52-77) with A the Chebyshev half-step (E at the half-step's
+ * midpoint) and B classical RK4 on the nonlinear part; t1 - t0 must be the dt of
+ * cody_set_chebyshev. */
+int cody_strang_cheb(cody_ctx* ctx, double t0, double t1);
+/* Propagates the context's state by nSteps Chebyshev-Strang steps of dt from t0 (the state is left
+ * at t0 + nSteps dt), *J = <x> at the end and gradc[k] = dJ/dc_k for the nLaser coefficients, by
+ * the discrete adjoint of the steps: the forward pass keeps a checkpoint every `checkpointEvery`
+ * steps, the backward pass recomputes each segment from its checkpoint. */
+int cody_dipole_gradient(cody_ctx* ctx, double t0, double dt, int nSteps, int checkpointEvery, double* J,
+                         double* gradc);
+
 /* ---------------------------------------------------------------- batches of instances */
 /* Independent instances (each its own context: grid, orbitals, interaction) evaluated together.
  * No counterpart in the reference, which runs one Method per process; this is the GPU form of
