@@ -60,6 +60,10 @@ def parse():
     p.add_argument("--batch-steps", type=int, default=20)
     p.add_argument("--prop-steps", type=int, default=1000,
                    help="steps of each propagation measurement (0: skip them)")
+    p.add_argument("--adjoint-steps", type=int, default=1000,
+                   help="steps of the laser-gradient measurement (0: skip it)")
+    p.add_argument("--adjoint-fd", type=int, default=4,
+                   help="laser coefficients whose gradient is checked by central differences")
     p.add_argument("--pmc-probe", action="store_true",
                    help="short run for hardware-counter passes: a 256 MiB device copy (the unit "
                         "of the byte counters) and a few launches of the FP64 and int8 "
@@ -149,6 +153,73 @@ def propagation_measurement(args, wlm, wl, state, prec_code):
                                             **_propagate(ctx, wl, w, smooth, args.prop_steps, 0.002, rk4))
     ctx.close()
     return out
+
+
+def adjoint_measurement(args, wlm, wl):
+    """Gradient of the final dipole <x>(T) in 32 laser coefficients: E(t) = sum_k c_k
+    sin(k pi t / T) sin^2(pi t / T), T = 20, c_k ~ N(0, 0.01^2) (seeded), args.adjoint_steps
+    Strang steps of dt = T / steps from the smooth seeded state, consistent split, FP64; the
+    linear half-steps are the fixed-degree Chebyshev expansion, checkpoints every 50 steps.
+    Reported: forward and gradient wall time, the gradient's agreement with central differences
+    (h = 1e-6) of the same forward map for the first args.adjoint_fd coefficients and along a
+    random unit direction, and with a central difference (h = 1e-4) along the same direction of
+    the Lanczos (SIL) Strang map."""
+    n = args.adjoint_steps
+    T = 20.0
+    dt = T / n
+    rng = np.random.default_rng(2024)
+    c = 0.01 * rng.standard_normal(32)
+    ctx, x, w = wlm.make_context(wl)
+    ctx.set_split(True)
+    state = wlm.smooth_state(x, w, wl.N, wl.M, seed=1234)
+    emax = float(np.sum(np.abs(c))) * 1.05
+    lo, hi, K = ctx.set_chebyshev(dt, emax)
+
+    def forward(cc, sil=False):
+        ctx.set_laser(cc, T)
+        ctx.upload_state(state)
+        for m in range(n):
+            if sil:
+                ctx.strang(m * dt, (m + 1) * dt, krylov_dim=wl.krylov_dim, tol=1e-10, max_restarts=8, metric=True)
+            else:
+                ctx.strang_cheb(m * dt, (m + 1) * dt)
+        return ctx.dipole(ctx.download_state())
+
+    ctx.set_laser(c, T)
+    ctx.upload_state(state)
+    ctx.dipole_gradient(0.0, dt, 2, 1)  # warm-up: buffers and graphs
+    t0 = time.perf_counter()
+    J_fwd = forward(c)
+    t_fwd = time.perf_counter() - t0
+    ctx.set_laser(c, T)
+    ctx.upload_state(state)
+    t0 = time.perf_counter()
+    J, g = ctx.dipole_gradient(0.0, dt, n, 50)
+    t_grad = time.perf_counter() - t0
+    h = 1e-6
+    fd = []
+    for k in range(min(args.adjoint_fd, len(c))):
+        e = np.zeros(len(c))
+        e[k] = h
+        fd.append((forward(c + e) - forward(c - e)) / (2 * h))
+    fd = np.array(fd)
+    u = rng.standard_normal(len(c))
+    u /= np.linalg.norm(u)
+    d_fd = (forward(c + h * u) - forward(c - h * u)) / (2 * h)
+    hs = 1e-4
+    d_sil = (forward(c + hs * u, sil=True) - forward(c - hs * u, sil=True)) / (2 * hs)
+    ctx.close()
+    gn = float(np.linalg.norm(g))
+    return {
+        "workload": wl.name, "steps": n, "dt": dt, "T": T, "coefficients": len(c), "checkpoint_every": 50,
+        "split": "consistent", "chebyshev": {"degree": K, "lo": lo, "hi": hi},
+        "J": J, "J_forward_only": J_fwd, "grad_norm": gn,
+        "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
+        "fd_components": len(fd),
+        "fd_max_abs_err_over_grad_norm": float(np.max(np.abs(g[: len(fd)] - fd)) / gn) if len(fd) else None,
+        "directional_fd": {"adjoint": float(g @ u), "fd": d_fd, "rel_err": abs(float(g @ u) - d_fd) / gn},
+        "directional_fd_sil_strang": {"fd": d_sil, "rel_err": abs(float(g @ u) - d_sil) / gn, "h": hs},
+    }
 
 
 def propagated_energy_by_form(args, wlm, wl, forms):
@@ -477,6 +548,8 @@ def main():
         result["batched"] = batched_measurement(args, torch, wlm, check=not args.no_cpu_baseline)
     if wl.name != "mctdhf_1d_401":
         result["small_workload"] = small_workload_measurement(args, torch, wlm)
+    if args.adjoint_steps > 0:
+        result["adjoint"] = adjoint_measurement(args, wlm, wl)
     print(json.dumps(result), flush=True)
     ctx.close()
 

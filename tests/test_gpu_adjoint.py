@@ -194,3 +194,26 @@ def test_gradient_matches_finite_differences_4096():
     err = np.linalg.norm(g - fd) / np.linalg.norm(fd)
     print(f"\n  J {J:.10e}; |grad| {np.linalg.norm(g):.4e}; adjoint vs FD rel err {err:.3e}")
     assert err < 1e-6
+
+
+def test_autograd_harness_gradcheck():
+    """torch.autograd.gradcheck of the right-hand-side wrapper (state and laser coefficients) on
+    a 41-point system, and of the propagated-dipole wrapper in the coefficients."""
+    import torch
+
+    from codyfortran_amd.autograd import DipoleAfterPropagation, RightHandSide
+
+    w = small("tiny41", -8.0, 8.0, 10, 5, 2, 4)
+    ctx, xg, wg = wl.make_context(w)
+    state = wl.smooth_state(xg, wg, w.N, w.M, seed=2)
+    dev = torch.device("cuda:0")
+    st = torch.from_numpy(state.view(np.float64).copy()).to(dev).requires_grad_(True)
+    c = torch.tensor([0.3, -0.2, 0.1], dtype=torch.float64, device=dev, requires_grad=True)
+    for part in (hip.PART_FULL, hip.PART_NONLIN, hip.PART_LIN):
+        assert torch.autograd.gradcheck(lambda s_, c_: RightHandSide.apply(s_, c_, ctx, 0.4, part, 1.5), (st, c),
+                                        eps=1e-6, atol=1e-7, rtol=1e-6, nondet_tol=1e-12)
+    ctx.set_chebyshev(0.05, 1.0)
+    cc = torch.tensor([0.3, -0.2, 0.1], dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda c_: DipoleAfterPropagation.apply(c_, ctx, state, 0.5, 0.0, 0.05, 10, 4),
+                                    (cc,), eps=1e-6, atol=1e-8, rtol=1e-6)
+    ctx.close()
