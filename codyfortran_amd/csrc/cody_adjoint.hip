@@ -311,64 +311,79 @@ __global__ __launch_bounds__(256) void k_adj_rho2p(const double2* __restrict__ r
   rho2p[e] = acc;
 }
 
-// Per grid point: Wbar of every stored pair (folded: Wbar_p = Wbar_kl + conj(Wbar_lk)), written
-// times w into the tiled source of the backward contraction, and the orbital gradient terms of
-// the correlation apply and of h2.  Points past nG write zero sources.
+// Wbar of the stored pairs (folded: Wbar_p = Wbar_kl + conj(Wbar_lk)), written times w into the
+// tiled source of the backward contraction, and the orbital gradient terms of the correlation
+// apply and of h2.  Workgroup = 64 grid points (one per lane) x 4 pairs (one per wave,
+// p = 4 blockIdx.y + wave); the four waves' orbital terms are added in LDS in fixed order and
+// written to gpart[blockIdx.y] (summed by k_adj_final).  Points past nG write zero sources.
 template <int NO>
 __global__ __launch_bounds__(256) void k_adj_local(const double2* __restrict__ orbs, const double* __restrict__ w,
                                                    const double* __restrict__ WT, int nG, int nGp,
                                                    const double2* __restrict__ rho2p, const double2* __restrict__ h2b,
-                                                   const double2* __restrict__ dbar, double2* __restrict__ gO,
+                                                   const double2* __restrict__ dbar, double2* __restrict__ gpart,
                                                    double* __restrict__ STadj) {
-  const int x = blockIdx.x * 256 + threadIdx.x;  // grid = nGp / 256
+  constexpr int NP = NO * (NO + 1) / 2;
+  __shared__ double2 sG[4][NO][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p = __builtin_amdgcn_readfirstlane(4 * (int)blockIdx.y + wave);
+  const int x = blockIdx.x * 64 + lane;
   const int nK = nGp >> 4;
-  if (x >= nG) {
-    for (int p = 0; p < NO * (NO + 1) / 2; ++p) {
-      STadj[tile_off(2 * p, x, nK)] = 0.0;
-      STadj[tile_off(2 * p + 1, x, nK)] = 0.0;
+  const bool in = x < nG;
+  const int xc = in ? x : nG - 1;
+  double2 g[NO];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) g[k] = cmk(0, 0);
+  if (p < NP) {
+    int l = 0, r = p;
+    while (r >= NO - l) {
+      r -= NO - l;
+      ++l;
     }
-    return;
-  }
-  double2 phi[NO], db[NO], g[NO];
+    const int k = l + r;
+    double2 phi[NO], db[NO];
 #pragma unroll
-  for (int k = 0; k < NO; ++k) {
-    phi[k] = orbs[(size_t)k * nG + x];
-    db[k] = dbar[(size_t)k * nG + x];
-    g[k] = cmk(0, 0);
-  }
-  const double wx = w[x];
-  int p = 0;
-  for (int l = 0; l < NO; ++l)
-    for (int k = l; k < NO; ++k, ++p) {
-      const double2 Wp = cmk(WT[(size_t)(2 * p) * nGp + x], WT[(size_t)(2 * p + 1) * nGp + x]);  // W_{k l}
-      double2 fold = cmk(0, 0);
-      for (int s = 0; s < (k == l ? 1 : 2); ++s) {
-        const int a = s ? l : k, b = s ? k : l;
-        const double2 Wab = s ? cconj(Wp) : Wp;
-        const double2 cW = cconj(Wab), hW = cscale(Wab, 0.5);
-        double2 X = cmk(0, 0);
+    for (int q = 0; q < NO; ++q) {
+      phi[q] = orbs[(size_t)q * nG + xc];
+      db[q] = dbar[(size_t)q * nG + xc];
+    }
+    const double2 Wp = cmk(WT[(size_t)(2 * p) * nGp + x], WT[(size_t)(2 * p + 1) * nGp + x]);  // W_{k l}
+    double2 fold = cmk(0, 0);
+    for (int s = 0; s < (k == l ? 1 : 2); ++s) {
+      const int a = s ? l : k, b = s ? k : l;
+      const double2 Wab = s ? cconj(Wp) : Wp;
+      const double2 cW = cconj(Wab), hW = cscale(Wab, 0.5);
+      double2 X = cmk(0, 0);
 #pragma unroll
-        for (int k1 = 0; k1 < NO; ++k1) {
-          double2 gk = cmk(0, 0);
+      for (int k1 = 0; k1 < NO; ++k1) {
+        double2 gk = cmk(0, 0);
 #pragma unroll
-          for (int j1 = 0; j1 < NO; ++j1) {
-            const double2 r = cconj(rho2p[idx4(k1, a, j1, b, NO)]);
-            const double2 hb = h2b[idx4(k1, b, j1, a, NO)];
-            cfma(X, cmul(r, cconj(phi[k1])), db[j1]);
-            cfma(X, cscale(cmul(hb, phi[k1]), 0.5), cconj(phi[j1]));
-            cfma(gk, cmul(cW, r), db[j1]);
-            cfma(gk, cmul(hW, cconj(hb)), phi[j1]);
-            cfma(g[j1], cmul(hb, cconj(hW)), phi[k1]);
-          }
-          g[k1] = cadd(g[k1], gk);
+        for (int j1 = 0; j1 < NO; ++j1) {
+          const double2 rr = cconj(rho2p[idx4(k1, a, j1, b, NO)]);
+          const double2 hb = h2b[idx4(k1, b, j1, a, NO)];
+          cfma(X, cmul(rr, cconj(phi[k1])), db[j1]);
+          cfma(X, cscale(cmul(hb, phi[k1]), 0.5), cconj(phi[j1]));
+          cfma(gk, cmul(cW, rr), db[j1]);
+          cfma(gk, cmul(hW, cconj(hb)), phi[j1]);
+          cfma(g[j1], cmul(hb, cconj(hW)), phi[k1]);
         }
-        fold = cadd(fold, s ? cconj(X) : X);
+        g[k1] = cadd(g[k1], gk);
       }
-      STadj[tile_off(2 * p, x, nK)] = wx * fold.x;
-      STadj[tile_off(2 * p + 1, x, nK)] = wx * fold.y;
+      fold = cadd(fold, s ? cconj(X) : X);
     }
+    const double wx = in ? w[xc] : 0.0;
+    STadj[tile_off(2 * p, x, nK)] = wx * fold.x;
+    STadj[tile_off(2 * p + 1, x, nK)] = wx * fold.y;
+  }
 #pragma unroll
-  for (int k = 0; k < NO; ++k) gO[(size_t)k * nG + x] = cadd(gO[(size_t)k * nG + x], g[k]);
+  for (int q = 0; q < NO; ++q) sG[wave][q][lane] = g[q];
+  __syncthreads();
+  for (int e = threadIdx.x; e < NO * 64; e += 256) {
+    const int q = e >> 6, ln = e & 63, xo = blockIdx.x * 64 + ln;
+    if (xo >= nG) continue;
+    gpart[((size_t)blockIdx.y * NO + q) * nG + xo] =
+        cadd(cadd(sG[0][q][ln], sG[1][q][ln]), cadd(sG[2][q][ln], sG[3][q][ln]));
+  }
 }
 
 // rho2'bar(k1,a,j1,b) = sum_x w conj(W_ab phi_k1) dbar_j1, one workgroup per (k1, a, b)
@@ -508,14 +523,16 @@ __global__ __launch_bounds__(256) void k_adj_rdm(const double2* __restrict__ rho
   }
 }
 
-// mean-field and one-body terms, one grid point per thread: g_k += sbar_p phi_l and
+// mean-field and one-body terms, one grid point per thread: g += the pair groups' terms of
+// k_adj_local (gpart, ngroups of them), g_k += sbar_p phi_l and
 // g_l += conj(sbar_p) phi_k for the stored pairs (WTadj = V (w Wbar), may be null), g_j += h hbar_j,
 // and per-workgroup partials of Ebar = sum_j <hbar_j, x phi_j>_w
 template <int NO>
 __global__ __launch_bounds__(256) void k_adj_final(const double2* __restrict__ orbs, const double* __restrict__ w,
                                                    const double* __restrict__ xg, const double* __restrict__ vext,
                                                    KinParams kp, double E, const double* __restrict__ WTadj, int nG,
-                                                   int nGp, const double2* __restrict__ hbar, double2* __restrict__ gO,
+                                                   int nGp, const double2* __restrict__ gpart, int ngroups,
+                                                   const double2* __restrict__ hbar, double2* __restrict__ gO,
                                                    double* __restrict__ epart) {
   __shared__ double red[4];
   const int x = blockIdx.x * 256 + threadIdx.x;
@@ -527,6 +544,9 @@ __global__ __launch_bounds__(256) void k_adj_final(const double2* __restrict__ o
       phi[k] = orbs[(size_t)k * nG + x];
       g[k] = gO[(size_t)k * nG + x];
     }
+    for (int q = 0; q < ngroups; ++q)
+#pragma unroll
+      for (int k = 0; k < NO; ++k) g[k] = cadd(g[k], gpart[((size_t)q * NO + k) * nG + x]);
     if (WTadj) {
       int p = 0;
 #pragma unroll
@@ -796,6 +816,7 @@ int alloc_vjp(cody_ctx* c) {
   AL(a.WTadj, sizeof(double) * ns);
   AL(a.epart, sizeof(double) * nb);
   AL(a.eacc, sizeof(double) * nOG);
+  AL(a.gpart, sizeof(double2) * nOG * ((c->P + 3) / 4));
   AL(a.J, sizeof(double) * 2);
   for (int q = 0; q < 4; ++q) {
     AL(a.y[q], sizeof(double2) * nS);
@@ -836,7 +857,7 @@ int enqueue_vjp(cody_ctx* c, int part, const double2* state, const double2* lam,
     launch_zero(gO, nO * nG, st);
 #define CALL_F0(NO)                                                                                              \
   hipLaunchKernelGGL(k_adj_final<NO>, dim3(nb), dim3(256), 0, st, orbs, c->d_w, c->d_x, c->d_vext, kp, E, nullptr, \
-                     nG, nGp, a.hbar, gO, a.epart)
+                     nG, nGp, nullptr, 0, a.hbar, gO, a.epart)
     ADJ_NO_SWITCH(nO, CALL_F0)
 #undef CALL_F0
     hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, st, a.epart, (int)nb, ebar);
@@ -864,8 +885,8 @@ int enqueue_vjp(cody_ctx* c, int part, const double2* state, const double2* lam,
                      a.h1b, nG, ah, bh, a.dbar, gO, a.hbar);                                                     \
   hipLaunchKernelGGL(k_adj_rho2p<NO>, dim3((NO * NO * NO * NO + 255) / 256), dim3(256), 0, st, c->d_rho2,        \
                      c->d_reginv, a.rho2p);                                                                      \
-  hipLaunchKernelGGL(k_adj_local<NO>, dim3(nGp / 256), dim3(256), 0, st, orbs, c->d_w, c->d_WT, nG, nGp, a.rho2p,  \
-                     a.h2b, a.dbar, gO, a.STadj);                                                                \
+  hipLaunchKernelGGL(k_adj_local<NO>, dim3(nGp / 64, (NO * (NO + 1) / 2 + 3) / 4), dim3(256), 0, st, orbs, c->d_w, \
+                     c->d_WT, nG, nGp, a.rho2p, a.h2b, a.dbar, a.gpart, a.STadj);                                \
   hipLaunchKernelGGL(k_adj_r2pb<NO>, dim3(NO * NO * NO), dim3(256), 0, st, orbs, c->d_w, c->d_WT, nG, nGp, a.dbar, \
                      a.r2pb);                                                                                    \
   hipLaunchKernelGGL(k_adj_rdm<NO>, dim3(1), dim3(256), 0, st, c->d_rho1, c->d_reginv, c->d_rho2, a.r2pb, c->reg,  \
@@ -878,7 +899,7 @@ int enqueue_vjp(cody_ctx* c, int part, const double2* state, const double2* lam,
   launch_mf_f64_w(c->d_Vt, a.STadj, c->d_Wpart, a.WTadj, nGp, c->Pc, c->wsplit, st);
 #define CALL_F(NO)                                                                                               \
   hipLaunchKernelGGL(k_adj_final<NO>, dim3(nb), dim3(256), 0, st, orbs, c->d_w, c->d_x, c->d_vext, kp, E, a.WTadj, \
-                     nG, nGp, a.hbar, gO, a.epart)
+                     nG, nGp, a.gpart, (NO * (NO + 1) / 2 + 3) / 4, a.hbar, gO, a.epart)
   ADJ_NO_SWITCH(nO, CALL_F)
 #undef CALL_F
   hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, st, a.epart, (int)nb, ebar);
@@ -961,47 +982,79 @@ int cheb_half_adj(cody_ctx* c, double2* lamOrbs, double E, const double2* U, dou
   return CODY_OK;
 }
 
-// One Strang step of y at tm: A(dt/2) with E(tm + dt/4), RK4 on the nonlinear part recording
-// its stage inputs y[0..3], A(dt/2) with E(tm + 3dt/4); Chebyshev terms in U1, U2
-int strang_cheb_step(cody_ctx* c, double2* y, double tm, hipStream_t st) {
+// What the adjoint of one Strang step reads: the Chebyshev terms of both half-steps and the
+// inputs of the four RK4 stages
+struct StepTape {
+  double2 *U1, *U2, *y[4];
+};
+
+StepTape default_tape(cody_ctx* c) {
+  AdjBuffers& a = c->adj;
+  return StepTape{a.U1, a.U2, {a.y[0], a.y[1], a.y[2], a.y[3]}};
+}
+
+size_t tape_elems(const cody_ctx* c) {
+  return 2 * (size_t)(c->adj.K + 1) * c->nO * c->nG + 4 * (size_t)c->nS;
+}
+
+StepTape tape_slot(cody_ctx* c, size_t i) {
+  double2* b = c->adj.tape + i * tape_elems(c);
+  const size_t nu = (size_t)(c->adj.K + 1) * c->nO * c->nG;
+  return StepTape{b, b + nu, {b + 2 * nu, b + 2 * nu + c->nS, b + 2 * nu + 2 * c->nS, b + 2 * nu + 3 * c->nS}};
+}
+
+// One Strang step of y at tm: A(dt/2) with E(tm + dt/4), RK4 on the nonlinear part, A(dt/2)
+// with E(tm + 3dt/4); the Chebyshev terms go to tp.U1, tp.U2 and the stage inputs to tp.y[0..3].
+// The stages are formed in the fixed buffers a.y[0..3] (the keys of the captured right-hand
+// sides) and copied to the tape when it is elsewhere.
+int strang_cheb_step(cody_ctx* c, double2* y, double tm, const StepTape& tp, hipStream_t st) {
   AdjBuffers& a = c->adj;
   const double dt = a.dt, dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
   const int n = c->nS;
   const unsigned nb = blocks(n);
-  CHK(cheb_half(c, y + c->nC, laser_field(c, tm + 0.25 * dt), a.U1, st));
+  auto keep = [&](int q) -> int {
+    if (tp.y[q] != a.y[q])
+      HIPCHK(hipMemcpyAsync(tp.y[q], a.y[q], sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+    return CODY_OK;
+  };
+  CHK(cheb_half(c, y + c->nC, laser_field(c, tm + 0.25 * dt), tp.U1, st));
   HIPCHK(hipMemcpyAsync(a.y[0], y, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+  CHK(keep(0));
   CHK(run_rhs(c, a.y[0], a.rk_d, tm, kModeNonLin, st));
   hipLaunchKernelGGL(k_rk_stage, dim3(nb), dim3(256), 0, st, a.y[0], a.rk_d, a.rk_inc, a.y[1], dt6, dt2, 1, n);
+  CHK(keep(1));
   CHK(run_rhs(c, a.y[1], a.rk_d, tm + dt2, kModeNonLin, st));
   hipLaunchKernelGGL(k_rk_stage, dim3(nb), dim3(256), 0, st, a.y[0], a.rk_d, a.rk_inc, a.y[2], dt3, dt2, 0, n);
+  CHK(keep(2));
   CHK(run_rhs(c, a.y[2], a.rk_d, tm + dt2, kModeNonLin, st));
   hipLaunchKernelGGL(k_rk_stage, dim3(nb), dim3(256), 0, st, a.y[0], a.rk_d, a.rk_inc, a.y[3], dt3, dt, 0, n);
+  CHK(keep(3));
   CHK(run_rhs(c, a.y[3], a.rk_d, tm + dt, kModeNonLin, st));
   hipLaunchKernelGGL(k_rk_last, dim3(nb), dim3(256), 0, st, y, a.y[0], a.rk_inc, a.rk_d, dt6, n);
-  CHK(cheb_half(c, y + c->nC, laser_field(c, tm + 0.75 * dt), a.U2, st));
+  CHK(cheb_half(c, y + c->nC, laser_field(c, tm + 0.75 * dt), tp.U2, st));
   return CODY_OK;
 }
 
-// a.lam <- (step at tm)^T a.lam for the step just re-run by strang_cheb_step; the six dE
-// cotangents go to eb[0..5] at tm + 3dt/4, tm + dt, tm + dt/2, tm + dt/2, tm, tm + dt/4
-int strang_cheb_adj(cody_ctx* c, double tm, double* eb, hipStream_t st) {
+// a.lam <- (step at tm)^T a.lam from the step's tape; the six dE cotangents go to eb[0..5] at
+// tm + 3dt/4, tm + dt, tm + dt/2, tm + dt/2, tm, tm + dt/4
+int strang_cheb_adj(cody_ctx* c, double tm, const StepTape& tp, double* eb, hipStream_t st) {
   AdjBuffers& a = c->adj;
   const double dt = a.dt, dt2 = dt / 2.0, dt3 = dt / 3.0, dt6 = dt / 6.0;
   const int n = c->nS;
   const unsigned nb = blocks(n);
-  CHK(cheb_half_adj(c, a.lam + c->nC, laser_field(c, tm + 0.75 * dt), a.U2, eb + 0, st));
+  CHK(cheb_half_adj(c, a.lam + c->nC, laser_field(c, tm + 0.75 * dt), tp.U2, eb + 0, st));
   // d1..d4 = dt6, dt3, dt3, dt6 times lam
   hipLaunchKernelGGL(k_rk_adj_init, dim3(nb), dim3(256), 0, st, a.lam, a.dl[0], a.dl[1], a.dl[2], a.dl[3], dt6, dt3,
                      n);
-  CHK(enqueue_vjp(c, CODY_PART_NONLIN, a.y[3], a.dl[3], a.g, tm + dt, eb + 1, st));
+  CHK(enqueue_vjp(c, CODY_PART_NONLIN, tp.y[3], a.dl[3], a.g, tm + dt, eb + 1, st));
   hipLaunchKernelGGL(k_rk_adj_acc, dim3(nb), dim3(256), 0, st, a.lam, a.dl[2], a.g, dt, n);
-  CHK(enqueue_vjp(c, CODY_PART_NONLIN, a.y[2], a.dl[2], a.g, tm + dt2, eb + 2, st));
+  CHK(enqueue_vjp(c, CODY_PART_NONLIN, tp.y[2], a.dl[2], a.g, tm + dt2, eb + 2, st));
   hipLaunchKernelGGL(k_rk_adj_acc, dim3(nb), dim3(256), 0, st, a.lam, a.dl[1], a.g, dt2, n);
-  CHK(enqueue_vjp(c, CODY_PART_NONLIN, a.y[1], a.dl[1], a.g, tm + dt2, eb + 3, st));
+  CHK(enqueue_vjp(c, CODY_PART_NONLIN, tp.y[1], a.dl[1], a.g, tm + dt2, eb + 3, st));
   hipLaunchKernelGGL(k_rk_adj_acc, dim3(nb), dim3(256), 0, st, a.lam, a.dl[0], a.g, dt2, n);
-  CHK(enqueue_vjp(c, CODY_PART_NONLIN, a.y[0], a.dl[0], a.g, tm, eb + 4, st));
+  CHK(enqueue_vjp(c, CODY_PART_NONLIN, tp.y[0], a.dl[0], a.g, tm, eb + 4, st));
   hipLaunchKernelGGL(k_rk_adj_acc, dim3(nb), dim3(256), 0, st, a.lam, nullptr, a.g, 0.0, n);
-  CHK(cheb_half_adj(c, a.lam + c->nC, laser_field(c, tm + 0.25 * dt), a.U1, eb + 5, st));
+  CHK(cheb_half_adj(c, a.lam + c->nC, laser_field(c, tm + 0.25 * dt), tp.U1, eb + 5, st));
   return CODY_OK;
 }
 
@@ -1120,7 +1173,7 @@ int cody_strang_cheb(cody_ctx* c, double t0, double t1) {
   CHK(check_adjoint(c));
   CHK(check_cheb_dt(c, t1 - t0));
   HIPCHK(hipSetDevice(c->device));
-  CHK(strang_cheb_step(c, c->d_state, t0, c->stream));
+  CHK(strang_cheb_step(c, c->d_state, t0, default_tape(c), c->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return CODY_OK;
@@ -1143,7 +1196,22 @@ int cody_dipole_gradient(cody_ctx* c, double t0, double dt, int nSteps, int ever
     CHK(ctx_alloc((void**)&a.ck, sizeof(double2) * nS * nCk));
     a.nck = nCk;
   }
-  if (a.nseg < (size_t)every) {
+  // one tape per step of a segment (about 37 MB per step at 4096 points, M = 8, degree 32);
+  // without room for them the backward pass re-runs each step into the single work tape
+  const size_t tape_bytes = sizeof(double2) * tape_elems(c) * every;
+  if (a.ntape < (size_t)every || a.tape_K != a.K) {
+    if (a.tape) hipFree(a.tape);
+    a.tape = nullptr;
+    a.ntape = 0;
+    if (hipMalloc((void**)&a.tape, tape_bytes) == hipSuccess) {
+      a.ntape = every;
+      a.tape_K = a.K;
+    } else {
+      (void)hipGetLastError();
+      a.tape = nullptr;
+    }
+  }
+  if (!a.tape && a.nseg < (size_t)every) {
     if (a.seg) hipFree(a.seg);
     a.seg = nullptr;
     a.nseg = 0;
@@ -1158,10 +1226,11 @@ int cody_dipole_gradient(cody_ctx* c, double t0, double dt, int nSteps, int ever
     a.nebar = 6 * (size_t)nSteps;
   }
   // forward pass with checkpoints
+  const StepTape work = default_tape(c);
   for (int m = 0; m < nSteps; ++m) {
     if (m % every == 0)
       HIPCHK(hipMemcpyAsync(a.ck + nS * (m / every), c->d_state, sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
-    CHK(strang_cheb_step(c, c->d_state, t0 + m * dt, st));
+    CHK(strang_cheb_step(c, c->d_state, t0 + m * dt, work, st));
   }
   // the objective and its gradient at the final state
   launch_ci_density(c->d_state, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_reginv, st);
@@ -1171,15 +1240,23 @@ int cody_dipole_gradient(cody_ctx* c, double t0, double dt, int nSteps, int ever
   // backward pass, segment by segment from the last checkpoint
   for (long s = (long)nCk - 1; s >= 0; --s) {
     const int m0 = (int)s * every, m1 = std::min(nSteps, m0 + every);
-    HIPCHK(hipMemcpyAsync(a.seg, a.ck + nS * s, sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
-    for (int i = 1; i < m1 - m0; ++i) {
-      HIPCHK(hipMemcpyAsync(a.seg + nS * i, a.seg + nS * (i - 1), sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
-      CHK(strang_cheb_step(c, a.seg + nS * i, t0 + (m0 + i - 1) * dt, st));
-    }
-    for (int m = m1 - 1; m >= m0; --m) {
-      HIPCHK(hipMemcpyAsync(a.ys, a.seg + nS * (m - m0), sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
-      CHK(strang_cheb_step(c, a.ys, t0 + m * dt, st));  // the tape of step m
-      CHK(strang_cheb_adj(c, t0 + m * dt, a.ebar + 6 * (size_t)m, st));
+    HIPCHK(hipMemcpyAsync(a.ys, a.ck + nS * s, sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
+    if (a.tape) {
+      // recompute the segment once, keeping every step's tape, then run its adjoint backwards
+      for (int m = m0; m < m1; ++m) CHK(strang_cheb_step(c, a.ys, t0 + m * dt, tape_slot(c, m - m0), st));
+      for (int m = m1 - 1; m >= m0; --m)
+        CHK(strang_cheb_adj(c, t0 + m * dt, tape_slot(c, m - m0), a.ebar + 6 * (size_t)m, st));
+    } else {
+      // the segment's states, then each step re-run into the work tape before its adjoint
+      for (int m = m0; m < m1; ++m) {
+        HIPCHK(hipMemcpyAsync(a.seg + nS * (m - m0), a.ys, sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
+        if (m + 1 < m1) CHK(strang_cheb_step(c, a.ys, t0 + m * dt, work, st));
+      }
+      for (int m = m1 - 1; m >= m0; --m) {
+        HIPCHK(hipMemcpyAsync(a.ys, a.seg + nS * (m - m0), sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
+        CHK(strang_cheb_step(c, a.ys, t0 + m * dt, work, st));
+        CHK(strang_cheb_adj(c, t0 + m * dt, work, a.ebar + 6 * (size_t)m, st));
+      }
     }
     HIPCHK(hipGetLastError());
   }

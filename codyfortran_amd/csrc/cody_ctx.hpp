@@ -74,6 +74,7 @@ struct AdjBuffers {
   double2 *f = nullptr, *D = nullptr, *tmpC = nullptr, *Bpart = nullptr, *B = nullptr, *inp = nullptr;
   double2 *h1b = nullptr, *h2b = nullptr, *rho2p = nullptr, *r2pb = nullptr, *r2b = nullptr;
   double2 *lin1 = nullptr, *lin2 = nullptr, *dbar = nullptr, *hbar = nullptr, *X = nullptr;
+  double2* gpart = nullptr;  // orbital terms of the pair groups of the correlation adjoint
   double *STadj = nullptr, *WTadj = nullptr, *epart = nullptr, *eacc = nullptr, *J = nullptr;
   // Chebyshev half-steps exp(-i dt/2 h(E)) on [lo, hi] of degree K (cody_set_chebyshev)
   int K = 0;
@@ -87,6 +88,9 @@ struct AdjBuffers {
   double2 *rk_d = nullptr, *rk_inc = nullptr, *lam = nullptr, *g = nullptr, *ys = nullptr;
   double2 *ck = nullptr, *seg = nullptr;
   size_t nck = 0, nseg = 0;
+  double2* tape = nullptr;  // per-step tapes of one checkpoint segment
+  size_t ntape = 0;
+  int tape_K = 0;
   double* ebar = nullptr;
   size_t nebar = 0;
   void free_cheb() {
@@ -100,8 +104,8 @@ struct AdjBuffers {
   void release() {
     free_cheb();
     void* p[] = {f,     D,     tmpC,  Bpart, B,     inp,   h1b,   h2b,   rho2p, r2pb,  r2b,   lin1,  lin2, dbar,
-                 hbar,  X,     STadj, WTadj, epart, eacc,  J,     y[0],  y[1],  y[2],  y[3],  dl[0], dl[1], dl[2],
-                 dl[3], rk_d,  rk_inc, lam,  g,     ys,    ck,    seg,   ebar};
+                 hbar,  X,     gpart, STadj, WTadj, epart, eacc,  J,     y[0],  y[1],  y[2],  y[3],  dl[0], dl[1], dl[2],
+                 dl[3], rk_d,  rk_inc, lam,  g,     ys,    ck,    seg,   ebar,  tape};
     for (void* q : p)
       if (q) hipFree(q);
     *this = AdjBuffers();
