@@ -64,6 +64,9 @@ def parse():
                    help="steps of the laser-gradient measurement (0: skip it)")
     p.add_argument("--adjoint-fd", type=int, default=4,
                    help="laser coefficients whose gradient is checked by central differences")
+    p.add_argument("--adjoint-only", action="store_true",
+                   help="run only the laser-gradient measurement (no finite-difference checks unless "
+                        "--adjoint-fd > 0): the command of the adjoint kernel profile")
     p.add_argument("--pmc-probe", action="store_true",
                    help="short run for hardware-counter passes: a 256 MiB device copy (the unit "
                         "of the byte counters) and a few launches of the FP64 and int8 "
@@ -155,7 +158,7 @@ def propagation_measurement(args, wlm, wl, state, prec_code):
     return out
 
 
-def adjoint_measurement(args, wlm, wl):
+def adjoint_measurement(args, wlm, wl, checks=True):
     """Gradient of the final dipole <x>(T) in 32 laser coefficients: E(t) = sum_k c_k
     sin(k pi t / T) sin^2(pi t / T), T = 20, c_k ~ N(0, 0.01^2) (seeded), args.adjoint_steps
     Strang steps of dt = T / steps from the smooth seeded state, consistent split, FP64; the
@@ -196,6 +199,12 @@ def adjoint_measurement(args, wlm, wl):
     t0 = time.perf_counter()
     J, g = ctx.dipole_gradient(0.0, dt, n, 50)
     t_grad = time.perf_counter() - t0
+    if not checks:
+        ctx.close()
+        return {"workload": wl.name, "steps": n, "dt": dt, "T": T, "coefficients": len(c), "checkpoint_every": 50,
+                "split": "consistent", "chebyshev": {"degree": K, "lo": lo, "hi": hi}, "J": J,
+                "grad_norm": float(np.linalg.norm(g)), "forward_s": t_fwd, "gradient_s": t_grad,
+                "gradient_over_forward": t_grad / t_fwd}
     h = 1e-6
     fd = []
     for k in range(min(args.adjoint_fd, len(c))):
@@ -210,6 +219,22 @@ def adjoint_measurement(args, wlm, wl):
     d_sil = (forward(c + hs * u, sil=True) - forward(c - hs * u, sil=True)) / (2 * hs)
     ctx.close()
     gn = float(np.linalg.norm(g))
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's stored-trajectory adjoint on the same map, a bounded sample of two steps
+        import oracle
+        from tests.common import oracle_system
+
+        so, _, _ = oracle_system(wl)
+        so.set_laser(c, T)
+        m = 2
+        t0 = time.perf_counter()
+        so.dipole_gradient(state, 0.0, dt, m, lo, hi, K, consistent=True)
+        el = time.perf_counter() - t0
+        threads = int(oracle._LIB.orc_num_threads()) if oracle._LIB is not None else None
+        cpu = {"value": el / m, "unit": "s per step of the gradient", "cores": threads, "kind": "port",
+               "sample": f"forward and adjoint of {m} Chebyshev-Strang steps of the same system on the oracle "
+                         f"(C + OpenMP, {threads} threads), {el:.1f} s"}
     return {
         "workload": wl.name, "steps": n, "dt": dt, "T": T, "coefficients": len(c), "checkpoint_every": 50,
         "split": "consistent", "chebyshev": {"degree": K, "lo": lo, "hi": hi},
@@ -219,6 +244,8 @@ def adjoint_measurement(args, wlm, wl):
         "fd_max_abs_err_over_grad_norm": float(np.max(np.abs(g[: len(fd)] - fd)) / gn) if len(fd) else None,
         "directional_fd": {"adjoint": float(g @ u), "fd": d_fd, "rel_err": abs(float(g @ u) - d_fd) / gn},
         "directional_fd_sil_strang": {"fd": d_sil, "rel_err": abs(float(g @ u) - d_sil) / gn, "h": hs},
+        "gpu_s_per_step": t_grad / n,
+        "cpu_baseline": cpu,
     }
 
 
@@ -439,6 +466,9 @@ def main():
     wl = wlm.WORKLOADS[args.workload]
     if args.pmc_probe:
         pmc_probe(torch, wlm, wl)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_measurement(args, wlm, wl, checks=args.adjoint_fd > 0)}), flush=True)
         return
     ctx, x, w = wlm.make_context(wl)
     state = wlm.seeded_state(x, w, wl.N, wl.M, seed=1234)

@@ -188,6 +188,35 @@ module M_CodyHip
       integer(c_int) :: r
     end function
 
+    !> Adjoint: Chebyshev half-steps for steps of dt and fields |E| <= emax, one Chebyshev-Strang
+    !> step of the device state, and <x> after nSteps such steps with its gradient in the laser
+    !> coefficients (checkpoints every `every` steps).
+    function cody_set_chebyshev(ctx, dt, emax, lo, hi, degree) bind(C, name="cody_set_chebyshev") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt, emax
+      real(c_double) :: lo, hi
+      integer(c_int) :: degree
+      integer(c_int) :: r
+    end function
+
+    function cody_strang_cheb(ctx, t0, t1) bind(C, name="cody_strang_cheb") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t0, t1
+      integer(c_int) :: r
+    end function
+
+    function cody_dipole_gradient(ctx, t0, dt, nSteps, every, J, gradc) bind(C, name="cody_dipole_gradient") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t0, dt
+      integer(c_int), value :: nSteps, every
+      real(c_double) :: J
+      real(c_double) :: gradc(*)
+      integer(c_int) :: r
+    end function
+
     !> Restart files coeffs.in / orbBB_II.in / rdm2BtXX_YY.in in directory dir (a
     !> c_null_char-terminated string; "" is the working directory), the files the reference's
     !> Coeffs_SaveCoeffs, Orbs_SaveOrbs and Coeffs_SaveTwoRdm write.

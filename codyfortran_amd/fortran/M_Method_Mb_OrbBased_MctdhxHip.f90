@@ -32,6 +32,7 @@ module M_Method_Mb_OrbBased_MctdhxHip
   type(c_ptr) :: MctdhxHip_ctx = c_null_ptr
   integer(I32) :: MctdhxHip_nCoeffs = 0
   integer(I32) :: MctdhxHip_nState = 0
+  real(R64) :: MctdhxHip_laserBound = 0.0_R64  !< sum |c_k| of the laser coefficients: a bound of |E(t)|
 
   interface
     !> Binds Method_Setup, Method_TimeDerivative and Method_GetEnergy to the GPU leaf.
@@ -48,6 +49,26 @@ module M_Method_Mb_OrbBased_MctdhxHip
     !> Dipole <x> = Re sum_ij rho1(i,j) <phi_j|x phi_i> of Method_state on the GPU: the value of
     !> PrinterObservableLinear_DumpDipole (src/Utils/M_Utils_PrinterObservableLinear.f90:25-70).
     module function Method_Mb_OrbBased_MctdhxHip_Dipole() result(res)
+      real(R64) :: res
+    end function
+    !> Length-gauge dipole coupling E(t) x with E(t) = sum_k c_k sin(k pi t / T) sin^2(pi t / T)
+    !> for 0 <= t <= T (zero outside): the 1D counterpart of the time-dependent field of
+    !> src/SysPotential/Ylm/CoulombLaser (SysPotential_timeIndependentQ = .false.).
+    module subroutine Method_Mb_OrbBased_MctdhxHip_SetLaser(coefficients, period)
+      real(R64), intent(in) :: coefficients(:)
+      real(R64), intent(in) :: period
+    end subroutine
+    !> <x> after nSteps Strang steps of dt from Method_state (left at t0 + nSteps dt), the linear
+    !> half-steps a fixed-degree Chebyshev expansion of exp(-i dt/2 h(E)), and its gradient in the
+    !> laser coefficients by the discrete adjoint with a checkpoint every checkpointEvery steps.
+    !> No counterpart in the reference, which has no adjoint.
+    module function Method_Mb_OrbBased_MctdhxHip_DipoleGradient(t0, dt, nSteps, checkpointEvery, gradient, &
+                                                                chebLo, chebHi, chebDegree) result(res)
+      real(R64), intent(in) :: t0, dt
+      integer(I32), intent(in) :: nSteps, checkpointEvery
+      real(R64), intent(out) :: gradient(:)
+      real(R64), intent(out), optional :: chebLo, chebHi
+      integer(I32), intent(out), optional :: chebDegree
       real(R64) :: res
     end function
   end interface

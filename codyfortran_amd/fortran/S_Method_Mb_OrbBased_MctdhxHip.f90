@@ -91,6 +91,40 @@ contains
     res = d
   end function
 
+  module subroutine Method_Mb_OrbBased_MctdhxHip_SetLaser(coefficients, period)
+    real(R64), intent(in) :: coefficients(:)
+    real(R64), intent(in) :: period
+    real(c_double) :: c(max(1, size(coefficients)))
+    c = 0.0_R64
+    c(1:size(coefficients)) = coefficients
+    call CodyHip_Check(cody_set_laser(MctdhxHip_ctx, int(size(coefficients), c_int), c, period), "set_laser")
+    MctdhxHip_laserBound = sum(abs(coefficients))
+  end subroutine
+
+  module function Method_Mb_OrbBased_MctdhxHip_DipoleGradient(t0, dt, nSteps, checkpointEvery, gradient, &
+                                                              chebLo, chebHi, chebDegree) result(res)
+    use M_Method
+    real(R64), intent(in) :: t0, dt
+    integer(I32), intent(in) :: nSteps, checkpointEvery
+    real(R64), intent(out) :: gradient(:)
+    real(R64), intent(out), optional :: chebLo, chebHi
+    integer(I32), intent(out), optional :: chebDegree
+    real(R64) :: res
+    real(c_double) :: lo, hi, J
+    real(c_double) :: g(max(1, size(gradient)))
+    integer(c_int) :: K
+    call CodyHip_Check(cody_set_chebyshev(MctdhxHip_ctx, dt, MctdhxHip_laserBound, lo, hi, K), "set_chebyshev")
+    call CodyHip_Check(cody_upload_state(MctdhxHip_ctx, Method_state), "upload_state")
+    call CodyHip_Check(cody_dipole_gradient(MctdhxHip_ctx, t0, dt, int(nSteps, c_int), int(checkpointEvery, c_int), &
+                                            J, g), "dipole_gradient")
+    call CodyHip_Check(cody_download_state(MctdhxHip_ctx, Method_state), "download_state")
+    gradient = g(1:size(gradient))
+    if (present(chebLo)) chebLo = lo
+    if (present(chebHi)) chebHi = hi
+    if (present(chebDegree)) chebDegree = K
+    res = J
+  end function
+
   subroutine PropagatorSetup
   end subroutine
 

@@ -1591,6 +1591,7 @@ int orc_rhs_vjp(const orc_sys* s, int rhs, const cplx* state, double t, const cp
     for (int j = 0; j < nO; ++j)
       for (int i = 0; i < nO; ++i) h1[i + j * nO] = inner(s, orbs + (size_t)i * nG, hphi + (size_t)j * nG);
     orc_meanfield_w(s, orbs, W);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int a = 0; a < nO; ++a)
       for (int b = 0; b < nO; ++b)
         for (int j1 = 0; j1 < nO; ++j1)
@@ -1600,6 +1601,7 @@ int orc_rhs_vjp(const orc_sys* s, int rhs, const cplx* state, double t, const cp
               acc += conj(orbs[(size_t)i1 * nG + g]) * w_ab(W, nO, nG, a, b, g) * orbs[(size_t)j1 * nG + g] * s->w[g];
             h2[IDX4(i1, b, j1, a, nO)] = 0.5 * acc;
           }
+#pragma omp parallel for schedule(static)
     for (int g = 0; g < nG; ++g)
       for (int a = 0; a < nO; ++a)
         for (int b = 0; b < nO; ++b) {
@@ -1660,6 +1662,7 @@ int orc_rhs_vjp(const orc_sys* s, int rhs, const cplx* state, double t, const cp
         gO[(size_t)i * nG + g] += acc;
       }
     /* correlation sum_{a,b,k1} W_ab phi_k1 rho2'(k1,a,j1,b) and h2(i1,b,j1,a) = 1/2 <phi_i1|W_ab phi_j1> */
+#pragma omp parallel for schedule(static)
     for (int g = 0; g < nG; ++g) {
       for (int a = 0; a < nO; ++a)
         for (int b = 0; b < nO; ++b) {
@@ -1672,12 +1675,22 @@ int orc_rhs_vjp(const orc_sys* s, int rhs, const cplx* state, double t, const cp
               acc += r * conj(pk) * dj + 0.5 * h2b[IDX4(k1, b, j1, a, nO)] * pk * conj(pj);
               gO[(size_t)k1 * nG + g] += conj(wv) * r * dj + 0.5 * conj(h2b[IDX4(k1, b, j1, a, nO)]) * wv * pj;
               gO[(size_t)j1 * nG + g] += 0.5 * h2b[IDX4(k1, b, j1, a, nO)] * conj(wv) * pk;
-              r2pb[IDX4(k1, a, j1, b, nO)] += s->w[g] * conj(wv * pk) * dj;
             }
           }
           Wbar[(a + b * nO) * (size_t)nG + g] = acc;
         }
     }
+    /* rho2'bar(k1,a,j1,b) = sum_x w conj(W_ab phi_k1) dbar_j1 */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int a = 0; a < nO; ++a)
+      for (int b = 0; b < nO; ++b)
+        for (int k1 = 0; k1 < nO; ++k1)
+          for (int j1 = 0; j1 < nO; ++j1) {
+            cplx acc = 0.0;
+            for (int g = 0; g < nG; ++g)
+              acc += s->w[g] * conj(w_ab(W, nO, nG, a, b, g) * orbs[(size_t)k1 * nG + g]) * dbar[(size_t)j1 * nG + g];
+            r2pb[IDX4(k1, a, j1, b, nO)] = acc;
+          }
     /* rho2' = rho2 x_3 R */
     for (int l2 = 0; l2 < nO; ++l2)
       for (int j1 = 0; j1 < nO; ++j1)
@@ -1715,6 +1728,7 @@ int orc_rhs_vjp(const orc_sys* s, int rhs, const cplx* state, double t, const cp
           if (k != l) v += conj(Wbar[(l + k * nO) * (size_t)nG + g]);
           tmp[g] = s->w[g] * v;
         }
+#pragma omp parallel for schedule(static)
         for (int g = 0; g < nG; ++g) {
           cplx sb = 0.0;
           const double* row = s->vint + (size_t)g * nG;

@@ -14,8 +14,9 @@ program T_MctdhxHip_Rhs
   type(T_MctdhxHip_Input) :: input
   character(len=1024) :: inFile, outDir
   complex(R64), allocatable :: dState(:)
-  real(R64) :: t, dt, e0, e1
+  real(R64) :: t, dt, e0, e1, gJ, grad(3), chebLo, chebHi
   integer :: io, iStep
+  integer(I32) :: chebK
 
   call get_command_argument(1, inFile)
   call get_command_argument(2, outDir)
@@ -69,6 +70,20 @@ program T_MctdhxHip_Rhs
   open (newunit=io, file=trim(outDir)//"/state5.bin", access="stream", form="unformatted", status="replace")
   write (io) Method_state
   close (io)
+
+  ! gradient of <x> after 10 Chebyshev-Strang steps in three laser coefficients, from the initial state
+  open (newunit=io, file=trim(inFile), access="stream", form="unformatted", status="old", action="read")
+  read (io) Method_state
+  close (io)
+  call Method_Mb_OrbBased_MctdhxHip_SetLaser([0.4_R64, -0.3_R64, 0.2_R64], 0.5_R64)
+  gJ = Method_Mb_OrbBased_MctdhxHip_DipoleGradient(0.0_R64, 0.05_R64, 10, 3, grad, chebLo, chebHi, chebK)
+  write (*, '(A,ES24.16)') "gradJ ", gJ
+  write (*, '(A,ES24.16)') "grad1 ", grad(1)
+  write (*, '(A,ES24.16)') "grad2 ", grad(2)
+  write (*, '(A,ES24.16)') "grad3 ", grad(3)
+  write (*, '(A,ES24.16)') "chebLo ", chebLo
+  write (*, '(A,ES24.16)') "chebHi ", chebHi
+  write (*, '(A,I0)') "chebK ", chebK
 
   call Method_Mb_OrbBased_MctdhxHip_Finalize
   write (*, '(A)') "done"

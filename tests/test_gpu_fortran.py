@@ -46,3 +46,11 @@ def test_fortran_driver_matches_oracle():
     assert state_rel(state5, so, w.nC, wts, w.M) < 1e-10
     assert abs(float(vals["energy5"]) - s.energy(so, t)) < 1e-10 * abs(e0)
     assert abs(float(vals["dipole5"]) - s.dipole(so)) < 1e-10
+    # the leaf's dipole gradient against the oracle's adjoint on the same Chebyshev-Strang map
+    c = np.array([0.4, -0.3, 0.2])
+    s.set_laser(c, 0.5)
+    J_o, g_o, _ = s.dipole_gradient(state, 0.0, 0.05, 10, float(vals["chebLo"]), float(vals["chebHi"]),
+                                    int(vals["chebK"]))
+    g_f = np.array([float(vals[f"grad{k}"]) for k in (1, 2, 3)])
+    assert abs(float(vals["gradJ"]) - J_o) < 1e-11 * max(1.0, abs(J_o))
+    assert np.linalg.norm(g_f - g_o) < 1e-9 * np.linalg.norm(g_o)
