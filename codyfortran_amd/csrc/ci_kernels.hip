@@ -265,6 +265,16 @@ __global__ __launch_bounds__(256) void k_ci_apply(const double2* __restrict__ C,
   if (ln == 0) dCraw[ic] = acc;
 }
 
+// the full h2(i1,a,j1,b) tensor from G (the h2_ output of ApplyCorrelationOp,
+// S_Method_Mb_OrbBased.f90:841-872), one element per thread
+__global__ __launch_bounds__(256) void k_h2_full(const double* __restrict__ G, int Pc, int nO,
+                                                 double2* __restrict__ h2) {
+  const int n2 = nO * nO, e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n2 * n2) return;
+  const int i1 = e % nO, a = (e / nO) % nO, j1 = (e / n2) % nO, b = e / (n2 * nO);
+  h2[e] = h2_from_G(G, Pc, nO, i1, a, j1, b);
+}
+
 // E = sum h1(k,l) rho1(l,k) + sum h2(k1,k2,l1,l2) rho2(l1,l2,k1,k2)  (Method_GetEnergy), one
 // workgroup of 256 threads.
 __global__ __launch_bounds__(256) void k_ci_energy(int nO, int Pc, const double* __restrict__ G,
@@ -348,6 +358,31 @@ void launch_ci_density(const double2* C, int nC, int nO, double reg, const DevCi
     default: break;
   }
 #undef CALL_RI
+}
+
+void launch_h2_full(const double* G, int Pc, int nO, double2* h2, hipStream_t st) {
+  const int n4 = nO * nO * nO * nO;
+  hipLaunchKernelGGL(k_h2_full, dim3((n4 + 255) / 256), dim3(256), 0, st, G, Pc, nO, h2);
+}
+
+void launch_reginv(const double2* rho1, int nO, double reg, double2* reginv, hipStream_t st) {
+#define CALL_RG(NO) hipLaunchKernelGGL(k_ci_reginv<NO>, dim3(1), dim3(64), 0, st, rho1, reg, reginv)
+  switch (nO) {
+    case 1: CALL_RG(1); break;
+    case 2: CALL_RG(2); break;
+    case 3: CALL_RG(3); break;
+    case 4: CALL_RG(4); break;
+    case 5: CALL_RG(5); break;
+    case 6: CALL_RG(6); break;
+    case 7: CALL_RG(7); break;
+    case 8: CALL_RG(8); break;
+    case 9: CALL_RG(9); break;
+    case 10: CALL_RG(10); break;
+    case 11: CALL_RG(11); break;
+    case 12: CALL_RG(12); break;
+    default: break;
+  }
+#undef CALL_RG
 }
 
 void launch_sum_parts(const double2* part, int nPart, int stride, int n, double2* out, hipStream_t st) {

@@ -708,6 +708,123 @@ int cody_densities(cody_ctx* c, const double* coeffs, double* rdm1, double* rdm2
   return CODY_OK;
 }
 
+int cody_interaction_potential(cody_ctx* c, int nSrc, const double* src, double* pot) {
+  if (!c || nSrc < 0 || (nSrc > 0 && (!src || !pot)))
+    return fail(CODY_ERR_ARG, "cody_interaction_potential: bad arguments");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const int chunk = c->Pc / 2;  // sources per contraction
+  if (!c->d_src) CHK(ctx_alloc((void**)&c->d_src, sizeof(double2) * c->nG * chunk));
+  // the FP64 operand of V (kept current by the FP64 form; tiled here for the others)
+  if (c->precision != CODY_PREC_FP64 || !c->d_Vt) {
+    if (!c->d_Vt) CHK(ctx_alloc((void**)&c->d_Vt, sizeof(double) * c->nGp * c->nGp));
+    launch_tile_f64(c->d_V, c->nGp, c->d_Vt, st);
+  }
+  std::vector<double> wt((size_t)c->Pc * c->nGp);
+  for (int s0 = 0; s0 < nSrc; s0 += chunk) {
+    const int ns = std::min(chunk, nSrc - s0);
+    HIPCHK(hipMemcpyAsync(c->d_src, src + 2 * (size_t)s0 * c->nG, sizeof(double2) * c->nG * ns,
+                          hipMemcpyHostToDevice, st));
+    launch_tile_src(c->d_src, ns, c->nG, c->nGp, c->Pc, c->d_STt, st);
+    launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(wt.data(), c->d_WT, sizeof(double) * wt.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int q = 0; q < ns; ++q)
+      for (int x = 0; x < c->nG; ++x) {
+        pot[2 * ((size_t)(s0 + q) * c->nG + x)] = wt[(size_t)(2 * q) * c->nGp + x];
+        pot[2 * ((size_t)(s0 + q) * c->nG + x) + 1] = wt[(size_t)(2 * q + 1) * c->nGp + x];
+      }
+  }
+  return CODY_OK;
+}
+
+int cody_apply_correlation(cody_ctx* c, const double* orbs, const double* rdm1, const double* rdm2, double* dorbs,
+                           double* h2) {
+  if (!c || !orbs || !rdm1 || !rdm2 || !dorbs) return fail(CODY_ERR_ARG, "cody_apply_correlation: null argument");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const int nO = c->nO, nG = c->nG, np = nO * (nO - 1) / 2;
+  const size_t n2 = (size_t)nO * nO, n4 = n2 * n2, nOG = (size_t)nO * nG;
+  if (h2 && !c->d_h2) CHK(ctx_alloc((void**)&c->d_h2, sizeof(double2) * n4));
+  // canonical rho2Lin (i1 < i2, j1 < j2), the operand of the orbital kernel
+  std::vector<double> can(2 * std::max(1, np * np));
+  auto strict = [&](int a, int b) { return a * nO - (a * (a + 1)) / 2 + (b - a - 1); };
+  for (int j2 = 0; j2 < nO; ++j2)
+    for (int j1 = 0; j1 < j2; ++j1)
+      for (int i2 = 0; i2 < nO; ++i2)
+        for (int i1 = 0; i1 < i2; ++i1) {
+          const size_t e = strict(i1, i2) + (size_t)np * strict(j1, j2);
+          const size_t f = i1 + nO * (i2 + nO * (j1 + (size_t)nO * j2));
+          can[2 * e] = rdm2[2 * f];
+          can[2 * e + 1] = rdm2[2 * f + 1];
+        }
+  double2* d_orbs = c->d_tmp + c->nC;
+  HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * nOG, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->d_rho1, rdm1, sizeof(double2) * n2, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->d_r2can, can.data(), sizeof(double) * can.size(), hipMemcpyHostToDevice, st));
+  launch_reginv(c->d_rho1, nO, c->reg, c->d_reginv, st);
+  launch_pair_src(d_orbs, c->d_w, nG, c->nGp, nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
+  enqueue_contraction(c, st, [](const char*) {});
+  launch_orbital(d_orbs, c->d_w, c->d_hphi, c->d_WT, c->d_r2can, c->d_reginv, nG, c->nGp, nO, kModeNonLin,
+                 c->d_draw, c->d_part, st);
+  if (h2) {
+    launch_gram_f64(c->d_STt, c->d_WT, c->d_Gpart, c->nGp, c->Pc, c->nxs, st);
+    launch_gram_reduce(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->d_part1, 0, 0, nullptr, st);
+    launch_h2_full(c->d_G, c->Pc, nO, c->d_h2, st);
+  }
+  HIPCHK(hipGetLastError());
+  std::vector<double> corr(2 * nOG);
+  HIPCHK(hipMemcpyAsync(corr.data(), c->d_draw, sizeof(double2) * nOG, hipMemcpyDeviceToHost, st));
+  if (h2) HIPCHK(hipMemcpyAsync(h2, c->d_h2, sizeof(double2) * n4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (size_t i = 0; i < 2 * nOG; ++i) dorbs[i] += corr[i];  // accumulate semantics
+  return CODY_OK;
+}
+
+int cody_project_orbs(cody_ctx* c, const double* orbs, double* dorbs) {
+  if (!c || !orbs || !dorbs) return fail(CODY_ERR_ARG, "cody_project_orbs: null argument");
+  if (!c->grid_set) return fail(CODY_ERR_STATE, "cody_project_orbs: grid not set");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const size_t nOG = (size_t)c->nO * c->nG;
+  double2* d_orbs = c->d_tmp + c->nC;
+  HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * nOG, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->d_draw, dorbs, sizeof(double2) * nOG, hipMemcpyHostToDevice, st));
+  launch_overlap_parts(d_orbs, c->d_w, c->d_draw, c->nG, c->nO, c->d_part1, st);
+  launch_project(d_orbs, c->d_draw, c->d_part1, (c->nG + 255) / 256, c->nG, c->nO, 1, c->d_hphi, nullptr, nullptr, 0,
+                 nullptr, nullptr, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dorbs, c->d_hphi, sizeof(double2) * nOG, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // the projection kernel scales by -i (the right-hand side's factor): undo it exactly
+  for (size_t i = 0; i < nOG; ++i) {
+    const double re = dorbs[2 * i], im = dorbs[2 * i + 1];
+    dorbs[2 * i] = -im;
+    dorbs[2 * i + 1] = re;
+  }
+  return CODY_OK;
+}
+
+int cody_apply_kinetic(cody_ctx* c, const double* orbs, double* dorbs) {
+  if (!c || !orbs || !dorbs) return fail(CODY_ERR_ARG, "cody_apply_kinetic: null argument");
+  if (!c->grid_set || !c->kin_set) return fail(CODY_ERR_STATE, "cody_apply_kinetic: grid or kinetic operator not set");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const size_t nOG = (size_t)c->nO * c->nG;
+  double2* d_orbs = c->d_tmp + c->nC;
+  HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * nOG, hipMemcpyHostToDevice, st));
+  // the one-body kernel with a zero potential and a zero field
+  launch_onebody(d_orbs, c->d_w, c->d_x, c->d_zeros, c->kin, c->d_zeros + c->nG, c->nG, c->nO, c->d_draw,
+                 c->d_part1, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dorbs, c->d_draw, sizeof(double2) * nOG, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CODY_OK;
+}
+
 int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* names, int namesLen,
                      int* nKernels) {
   if (!c || iters < 1 || !avg_ms || nMax < 1) return fail(CODY_ERR_ARG, "cody_profile_rhs: bad arguments");

@@ -150,6 +150,9 @@ struct cody_ctx {
   double2* d_r2can = nullptr;  // canonical rho2Lin (i1 < i2, j1 < j2)
   double2 *d_h1 = nullptr, *d_hphi = nullptr, *d_dCraw = nullptr;
   double2 *d_draw = nullptr, *d_part = nullptr, *d_part1 = nullptr;
+  double* d_zeros = nullptr;  // nG + 1 zeros: a zero potential and a zero field (the kinetic piece)
+  double2* d_src = nullptr;   // interaction sources of cody_interaction_potential (nG x P, on use)
+  double2* d_h2 = nullptr;    // full h2 tensor of cody_apply_correlation (nO^4, on use)
   PropBuffers prop;
   AdjBuffers adj;
   // contraction operands derived from V (allocated by cody_set_precision): FP64 fragment-tiled
@@ -203,6 +206,7 @@ struct cody_ctx {
     ALLOC(d_draw, sizeof(double2) * nG * nO);
     ALLOC(d_part, sizeof(double2) * nPart * n2);
     ALLOC(d_part1, sizeof(double2) * ((nG + 255) / 256) * n2);
+    ALLOC(d_zeros, sizeof(double) * (nG + 1));
     // CI tables in one blob: ptr arrays then entry arrays
     const size_t ns = ci.s_ptr.size(), nd = ci.d_ptr.size(), n1 = ci.r1_ptr.size(), nr2 = ci.r2_ptr.size();
     const size_t es = ci.singles.size(), ed = ci.doubles.size(), e1 = ci.r1.size(), e2 = ci.r2.size();
@@ -263,7 +267,7 @@ struct cody_ctx {
     drop_graphs();
     void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_smax,
                     d_Gpart, d_G,  d_energy, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
-                    d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, ci_blob};
+                    d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, d_zeros, d_src, d_h2, ci_blob};
     for (void* p : ptrs)
       if (p) hipFree(p);
     prop.release();

@@ -102,6 +102,54 @@ __global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ 
   }
 }
 
+// Arbitrary complex sources (ns columns of nG points, column-major) in the fragment-tiled S^T
+// layout of k_pair_src_t: rows 2s / 2s+1 hold Re / Im of source s; other rows and points past nG
+// are zero.  One output element per thread (grid = Pc nGp / 256).
+__global__ __launch_bounds__(256) void k_tile_src(const double2* __restrict__ src, int ns, int nG, int nGp,
+                                                  double* __restrict__ STt) {
+  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nK = nGp >> 4;
+  const int e = o & 1, h = (o >> 7) & 1, l = (o >> 1) & 63;
+  const size_t blk = o >> 8;
+  const int t = (int)(blk / nK), K = (int)(blk % nK);
+  const int c = 16 * t + (l & 15), x = 16 * K + 4 * (l >> 4) + 2 * h + e;
+  double v = 0.0;
+  if (x < nG && (c >> 1) < ns) {
+    const double2 z = src[(size_t)(c >> 1) * nG + x];
+    v = (c & 1) ? z.y : z.x;
+  }
+  STt[o] = v;
+}
+
+// Partial sums of the overlaps <phi_i|f_j>_w over 256-point blocks, part[block][i + j NO]
+// (the overlaps of Grid_ProjectOnSubspace, src/Grid/S_Grid.f90:170-203).
+template <int NO>
+__global__ __launch_bounds__(256) void k_overlap(const double2* __restrict__ orbs, const double* __restrict__ w,
+                                                 const double2* __restrict__ f, int nG,
+                                                 double2* __restrict__ part) {
+  __shared__ double2 sRed[4][NO];
+  const int j = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const bool in = x < nG;
+  const int xc = in ? x : nG - 1;
+  const double2 fw = in ? cscale(f[(size_t)j * nG + xc], w[xc]) : cmk(0, 0);
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
+    double2 acc = cmk(0, 0);
+    if (in) cfmac(acc, orbs[(size_t)i * nG + xc], fw);
+    acc.x = wave_sum(acc.x);
+    acc.y = wave_sum(acc.y);
+    if (lane == 0) sRed[wave][i] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < NO) {
+    const int i = threadIdx.x;
+    part[(size_t)blockIdx.x * NO * NO + i + j * NO] =
+        cadd(cadd(sRed[0][i], sRed[1][i]), cadd(sRed[2][i], sRed[3][i]));
+  }
+}
+
 // ------------------------------------------------------------------ one-body part
 // One workgroup = 256 grid points x one orbital j (blockIdx.y):
 //   hphi_j = (T + V_ext + E(t) x) phi_j                   (ApplySingleBodyOp :976-1034)
@@ -408,6 +456,18 @@ void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int 
                      Pc, ST)
   CODY_NO_SWITCH(nO, CALL_PS)
 #undef CALL_PS
+}
+
+void launch_tile_src(const double2* src, int ns, int nG, int nGp, int Pc, double* STt, hipStream_t st) {
+  hipLaunchKernelGGL(k_tile_src, dim3((unsigned)((size_t)Pc * nGp / 256)), dim3(256), 0, st, src, ns, nG, nGp, STt);
+}
+
+void launch_overlap_parts(const double2* orbs, const double* w, const double2* f, int nG, int nO, double2* part,
+                          hipStream_t st) {
+  dim3 grid((nG + 255) / 256, nO);
+#define CALL_OV(NO) hipLaunchKernelGGL(k_overlap<NO>, grid, dim3(256), 0, st, orbs, w, f, nG, part)
+  CODY_NO_SWITCH(nO, CALL_OV)
+#undef CALL_OV
 }
 
 void launch_onebody(const double2* orbs, const double* w, const double* x, const double* vext,

@@ -57,6 +57,13 @@ void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a
 void launch_ci_density(const double2* C, int nC, int nO, double reg, const DevCiTables& t,
                        double2* rho1, double2* rho2, double2* r2can, double2* reginv, hipStream_t st);
 void launch_sum_parts(const double2* part, int nPart, int stride, int n, double2* out, hipStream_t st);
+// operator pieces: sources in the tiled S^T layout, overlap partials <phi_i|f_j>, the full h2
+// tensor from G, R from a given rho1
+void launch_tile_src(const double2* src, int ns, int nG, int nGp, int Pc, double* STt, hipStream_t st);
+void launch_overlap_parts(const double2* orbs, const double* w, const double2* f, int nG, int nO, double2* part,
+                          hipStream_t st);
+void launch_h2_full(const double* G, int Pc, int nO, double2* h2, hipStream_t st);
+void launch_reginv(const double2* rho1, int nO, double reg, double2* reginv, hipStream_t st);
 // position matrix partials (k_xmat) and <x> from them and rho1 (k_dipole)
 void launch_xmat(const double2* orbs, const double* w, const double* x, int nG, int nO, double2* part1,
                  hipStream_t st);

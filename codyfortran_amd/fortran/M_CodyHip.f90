@@ -188,6 +188,43 @@ module M_CodyHip
       integer(c_int) :: r
     end function
 
+    !> Operator pieces (the reference's procedure-pointer semantics) and the CI densities.
+    function cody_densities(ctx, coeffs, rdm1, rdm2, reginv) bind(C, name="cody_densities") result(r)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: coeffs(*), rdm1(*), rdm2(*), reginv(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_interaction_potential(ctx, nSrc, src, pot) bind(C, name="cody_interaction_potential") result(r)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nSrc
+      complex(c_double_complex) :: src(*), pot(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_apply_correlation(ctx, orbs, rdm1, rdm2, dorbs, h2) bind(C, name="cody_apply_correlation") result(r)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: orbs(*), rdm1(*), rdm2(*), dorbs(*), h2(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_project_orbs(ctx, orbs, dorbs) bind(C, name="cody_project_orbs") result(r)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: orbs(*), dorbs(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_apply_kinetic(ctx, orbs, dorbs) bind(C, name="cody_apply_kinetic") result(r)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: orbs(*), dorbs(*)
+      integer(c_int) :: r
+    end function
+
     !> Adjoint: Chebyshev half-steps for steps of dt and fields |E| <= emax, one Chebyshev-Strang
     !> step of the device state, and <x> after nSteps such steps with its gradient in the laser
     !> coefficients (checkpoints every `every` steps).

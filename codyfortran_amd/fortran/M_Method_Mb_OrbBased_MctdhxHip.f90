@@ -51,6 +51,42 @@ module M_Method_Mb_OrbBased_MctdhxHip
     module function Method_Mb_OrbBased_MctdhxHip_Dipole() result(res)
       real(R64) :: res
     end function
+    !> Pieces with the abstract interfaces of the reference's procedure pointers, so that
+    !> SysInteraction_FillInteractionPotential (src/SysInteraction/M_SysInteraction.f90:94-110),
+    !> SysKinetic_MultiplyWithKineticOp (src/SysKinetic/M_SysKinetic.f90:141-160),
+    !> Orbs_ProjectOnSubspace (src/Orbs/M_Orbs.f90:132-153) and
+    !> Method_Mb_OrbBased_ApplyCorrelationOp (src/Method/Mb/OrbBased/M_Method_Mb_OrbBased.f90:211-231)
+    !> can point at the GPU (one body type: the optional body-type arguments must be 1 or absent).
+    module subroutine MctdhxHip_FillInteractionPotential(interactionPotential, src, time, bt1_, bt2_)
+      complex(R64), intent(out), allocatable :: interactionPotential(:)
+      complex(R64), intent(in), contiguous :: src(:)
+      real(R64), intent(in) :: time
+      integer(I32), intent(in), optional :: bt1_
+      integer(I32), intent(in), optional :: bt2_
+    end subroutine
+    module subroutine MctdhxHip_MultiplyWithKineticOp(dOrb, orb, time, bt_)
+      complex(R64), intent(out), contiguous :: dOrb(:)
+      complex(R64), intent(in), contiguous :: orb(:)
+      real(R64), intent(in) :: time
+      integer(I32), intent(in), optional :: bt_
+    end subroutine
+    module subroutine MctdhxHip_ProjectOnSubspace(dOrbs, orbs)
+      complex(R64), intent(inout), contiguous :: dOrbs(:, :)
+      complex(R64), intent(in), contiguous :: orbs(:, :)
+    end subroutine
+    module subroutine MctdhxHip_ApplyCorrelationOp(dOrbs, orbs, rdm1, rdm2, time, h2_)
+      complex(R64), intent(inout), contiguous, target :: dOrbs(:, :)
+      complex(R64), intent(in), contiguous, target :: orbs(:, :)
+      complex(R64), intent(in), contiguous :: rdm1(:, :)
+      complex(R64), intent(in), contiguous :: rdm2(:, :, :, :)
+      real(R64), intent(in) :: time
+      complex(R64), intent(out), allocatable, optional :: h2_(:, :, :, :)
+    end subroutine
+    !> rho1, rho2 of the CI part of Method_state (the densities ApplyCorrelationOp takes).
+    module subroutine MctdhxHip_Densities(rdm1, rdm2)
+      complex(R64), intent(out) :: rdm1(:, :)
+      complex(R64), intent(out) :: rdm2(:, :, :, :)
+    end subroutine
     !> Length-gauge dipole coupling E(t) x with E(t) = sum_k c_k sin(k pi t / T) sin^2(pi t / T)
     !> for 0 <= t <= T (zero outside): the 1D counterpart of the time-dependent field of
     !> src/SysPotential/Ylm/CoulombLaser (SysPotential_timeIndependentQ = .false.).

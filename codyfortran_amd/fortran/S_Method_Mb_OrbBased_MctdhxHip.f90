@@ -91,6 +91,94 @@ contains
     res = d
   end function
 
+  subroutine CheckBodyType(bt, where)
+    integer(I32), intent(in), optional :: bt
+    character(len=*), intent(in) :: where
+    if (present(bt)) then
+      if (bt /= 1) then
+        write (*, '(A)') "MctdhxHip "//where//": the GPU leaf holds one body type"
+        error stop "MI355X MCTDHX: unsupported body type"
+      end if
+    end if
+  end subroutine
+
+  module subroutine MctdhxHip_FillInteractionPotential(interactionPotential, src, time, bt1_, bt2_)
+    complex(R64), intent(out), allocatable :: interactionPotential(:)
+    complex(R64), intent(in), contiguous :: src(:)
+    real(R64), intent(in) :: time
+    integer(I32), intent(in), optional :: bt1_
+    integer(I32), intent(in), optional :: bt2_
+    complex(c_double_complex), allocatable :: s(:), p(:)
+    call CheckBodyType(bt1_, "FillInteractionPotential")
+    call CheckBodyType(bt2_, "FillInteractionPotential")
+    if (size(src) /= Grid_nPoints_of_ctx()) error stop "MctdhxHip FillInteractionPotential: source size"
+    allocate (s(size(src)), p(size(src)))
+    s = src
+    call CodyHip_Check(cody_interaction_potential(MctdhxHip_ctx, 1_c_int, s, p), "interaction_potential")
+    allocate (interactionPotential(size(src)))
+    interactionPotential = p
+  end subroutine
+
+  module subroutine MctdhxHip_MultiplyWithKineticOp(dOrb, orb, time, bt_)
+    complex(R64), intent(out), contiguous :: dOrb(:)
+    complex(R64), intent(in), contiguous :: orb(:)
+    real(R64), intent(in) :: time
+    integer(I32), intent(in), optional :: bt_
+    complex(c_double_complex), allocatable :: o(:, :), d(:, :)
+    integer :: nG
+    call CheckBodyType(bt_, "MultiplyWithKineticOp")
+    nG = Grid_nPoints_of_ctx()
+    if (size(orb) /= nG .or. size(dOrb) /= nG) error stop "MctdhxHip MultiplyWithKineticOp: orbital size"
+    ! the library applies T to the context's nO columns: the orbital in column 1, zeros elsewhere
+    allocate (o(nG, MctdhxHip_input%nOrbs), d(nG, MctdhxHip_input%nOrbs))
+    o = (0.0_R64, 0.0_R64)
+    o(:, 1) = orb
+    call CodyHip_Check(cody_apply_kinetic(MctdhxHip_ctx, o, d), "apply_kinetic")
+    dOrb = d(:, 1)
+  end subroutine
+
+  module subroutine MctdhxHip_ProjectOnSubspace(dOrbs, orbs)
+    complex(R64), intent(inout), contiguous :: dOrbs(:, :)
+    complex(R64), intent(in), contiguous :: orbs(:, :)
+    if (size(orbs, 2) /= MctdhxHip_input%nOrbs .or. any(shape(dOrbs) /= shape(orbs))) &
+      error stop "MctdhxHip ProjectOnSubspace: shapes must be nG x nOrbs"
+    call CodyHip_Check(cody_project_orbs(MctdhxHip_ctx, orbs, dOrbs), "project_orbs")
+  end subroutine
+
+  module subroutine MctdhxHip_ApplyCorrelationOp(dOrbs, orbs, rdm1, rdm2, time, h2_)
+    complex(R64), intent(inout), contiguous, target :: dOrbs(:, :)
+    complex(R64), intent(in), contiguous, target :: orbs(:, :)
+    complex(R64), intent(in), contiguous :: rdm1(:, :)
+    complex(R64), intent(in), contiguous :: rdm2(:, :, :, :)
+    real(R64), intent(in) :: time
+    complex(R64), intent(out), allocatable, optional :: h2_(:, :, :, :)
+    complex(c_double_complex), allocatable :: h2(:, :, :, :)
+    integer :: n
+    n = MctdhxHip_input%nOrbs
+    if (size(orbs, 2) /= n .or. any(shape(dOrbs) /= shape(orbs))) &
+      error stop "MctdhxHip ApplyCorrelationOp: shapes must be nG x nOrbs"
+    allocate (h2(n, n, n, n))
+    call CodyHip_Check(cody_apply_correlation(MctdhxHip_ctx, orbs, rdm1, rdm2, dOrbs, h2), "apply_correlation")
+    if (present(h2_)) then
+      allocate (h2_(n, n, n, n))
+      h2_ = h2
+    end if
+  end subroutine
+
+  module subroutine MctdhxHip_Densities(rdm1, rdm2)
+    use M_Method
+    complex(R64), intent(out) :: rdm1(:, :)
+    complex(R64), intent(out) :: rdm2(:, :, :, :)
+    complex(c_double_complex), allocatable :: ri(:, :)
+    allocate (ri(size(rdm1, 1), size(rdm1, 2)))
+    call CodyHip_Check(cody_densities(MctdhxHip_ctx, Method_state, rdm1, rdm2, ri), "densities")
+  end subroutine
+
+  integer function Grid_nPoints_of_ctx()
+    use M_Grid
+    Grid_nPoints_of_ctx = Grid_nPoints
+  end function
+
   module subroutine Method_Mb_OrbBased_MctdhxHip_SetLaser(coefficients, period)
     real(R64), intent(in) :: coefficients(:)
     real(R64), intent(in) :: period

@@ -85,6 +85,10 @@ def load():
         "cody_save_state_files": (i, [C.c_char_p, i, i, i, ip, vp]),
         "cody_load_state_files": (i, [C.c_char_p, i, i, i, ip, vp]),
         "cody_save_two_rdm": (i, [C.c_char_p, i, ip, vp]),
+        "cody_interaction_potential": (i, [vp, i, vp, vp]),
+        "cody_apply_correlation": (i, [vp, vp, vp, vp, vp, vp]),
+        "cody_project_orbs": (i, [vp, vp, vp]),
+        "cody_apply_kinetic": (i, [vp, vp, vp]),
         "cody_time_derivative_vjp_dev": (i, [vp, i, vp, vp, vp, d, vp, vp]),
         "cody_dipole_grad_dev": (i, [vp, vp, vp, vp]),
         "cody_set_chebyshev": (i, [vp, d, d, dp, dp, ip]),
@@ -314,6 +318,36 @@ class Context:
         h1 = np.zeros((self.nO, self.nO), dtype=np.complex128, order="F")
         _check(self.lib.cody_apply_single_body(self.h, _vp(o), float(t), _vp(d), _vp(h1)))
         return d, h1
+
+    def interaction_potential(self, src):
+        """pot(:, s) = sum_j K(x_i - x_j) src(x_j, s) for sources holding the weights."""
+        sr = np.asfortranarray(np.asarray(src, dtype=np.complex128).reshape(self.nG, -1))
+        pot = np.zeros_like(sr, order="F")
+        _check(self.lib.cody_interaction_potential(self.h, sr.shape[1], _vp(sr), _vp(pot)))
+        return pot
+
+    def apply_correlation(self, orbs, rdm1, rdm2, dorbs=None):
+        """(dorbs + correlation term, h2), the accumulate semantics of ApplyCorrelationOp."""
+        o = _c128(orbs)
+        r1 = _c128(rdm1)
+        r2 = _c128(rdm2)
+        d = np.zeros((self.nG, self.nO), dtype=np.complex128, order="F") if dorbs is None else _c128(dorbs).copy(order="F")
+        n = self.nO
+        h2 = np.zeros((n, n, n, n), dtype=np.complex128, order="F")
+        _check(self.lib.cody_apply_correlation(self.h, _vp(o), _vp(r1), _vp(r2), _vp(d), _vp(h2)))
+        return d, h2
+
+    def project_orbs(self, orbs, dorbs):
+        o = _c128(orbs)
+        d = _c128(dorbs).copy(order="F")
+        _check(self.lib.cody_project_orbs(self.h, _vp(o), _vp(d)))
+        return d
+
+    def apply_kinetic(self, orbs):
+        o = _c128(orbs)
+        d = np.zeros((self.nG, self.nO), dtype=np.complex128, order="F")
+        _check(self.lib.cody_apply_kinetic(self.h, _vp(o), _vp(d)))
+        return d
 
     def densities(self, coeffs):
         c = _c128(coeffs)

@@ -167,6 +167,26 @@ int cody_apply_single_body(cody_ctx* ctx, const double* orbs, double t, double* 
  * rho1 (rho1^2 + eps)^-1 (S_Method_Mb_OrbBased.f90:733-744). */
 int cody_densities(cody_ctx* ctx, const double* coeffs, double* rdm1, double* rdm2, double* reginv);
 
+/* Operator pieces with the reference's procedure-pointer semantics (host arrays, column-major
+ * complex128, nG x nO orbitals):
+ *   cody_interaction_potential: pot(:, s) = sum_j K(x_i - x_j) src(x_j, s) for nSrc sources that
+ *     already hold the weights (src = conj(phi_a) phi_b w): SysInteraction_FillInteractionPotential
+ *     (src/SysInteraction/M_SysInteraction.f90:94-110, source convention
+ *     src/SysInteraction/Linear/S_SysInteraction_Linear.f90:57-72), on the FP64 MFMA contraction;
+ *   cody_apply_correlation: dorbs += the correlation term, h2 (nO^4, may be NULL) =
+ *     1/2 <phi_i1|W_(b a) phi_j1> at (i1, a, j1, b): Method_Mb_OrbBased_ApplyCorrelationOp
+ *     (src/Method/Mb/OrbBased/M_Method_Mb_OrbBased.f90:211-231, S_Method_Mb_OrbBased.f90:647-872),
+ *     accumulate semantics;
+ *   cody_project_orbs: dorbs <- dorbs - sum_i phi_i <phi_i|dorbs>: Orbs_ProjectOnSubspace
+ *     (src/Orbs/M_Orbs.f90:132-153);
+ *   cody_apply_kinetic: dorbs = T orbs: SysKinetic_MultiplyWithKineticOp
+ *     (src/SysKinetic/M_SysKinetic.f90:141-160), overwrite semantics. */
+int cody_interaction_potential(cody_ctx* ctx, int nSrc, const double* src, double* pot);
+int cody_apply_correlation(cody_ctx* ctx, const double* orbs, const double* rdm1, const double* rdm2, double* dorbs,
+                           double* h2);
+int cody_project_orbs(cody_ctx* ctx, const double* orbs, double* dorbs);
+int cody_apply_kinetic(cody_ctx* ctx, const double* orbs, double* dorbs);
+
 /* ---------------------------------------------------------------- propagation (device state) */
 /* The nonlinear part B of the split step, used by cody_time_derivative_nonlin_dev, cody_rk4(full=0)
  * and cody_strang:

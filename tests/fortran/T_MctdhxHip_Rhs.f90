@@ -49,6 +49,7 @@ program T_MctdhxHip_Rhs
   close (io)
 
   allocate (dState(size(Method_state)))
+  call WritePieces
   call Method_TimeDerivative(dState, Method_state, 0.0_R64)
   open (newunit=io, file=trim(outDir)//"/dstate.bin", access="stream", form="unformatted", status="replace")
   write (io) dState
@@ -87,4 +88,31 @@ program T_MctdhxHip_Rhs
 
   call Method_Mb_OrbBased_MctdhxHip_Finalize
   write (*, '(A)') "done"
+
+contains
+
+  !> The operator pieces on the initial state, through the leaf's procedures with the reference's
+  !> abstract interfaces: pieces.bin = [T phi_1 ; V src ; (1 - P)(x phi) ; corr(0) ; h2].
+  subroutine WritePieces
+    integer :: nC, nG, nO, j, u
+    complex(R64), allocatable :: orbs(:, :), d(:, :), kin(:), pot(:), rdm1(:, :), rdm2(:, :, :, :), h2(:, :, :, :)
+    nC = MctdhxHip_nCoeffs
+    nG = Grid_nPoints
+    nO = input%nOrbs
+    orbs = reshape(Method_state(nC + 1:), [nG, nO])
+    allocate (kin(nG), d(nG, nO), rdm1(nO, nO), rdm2(nO, nO, nO, nO))
+    call MctdhxHip_MultiplyWithKineticOp(kin, orbs(:, 1), 0.0_R64)
+    call MctdhxHip_FillInteractionPotential(pot, conjg(orbs(:, 1)) * orbs(:, 2) * Grid_Linear_weights, 0.0_R64)
+    do j = 1, nO
+      d(:, j) = Grid_Linear_xCoord * orbs(:, j)
+    end do
+    call MctdhxHip_ProjectOnSubspace(d, orbs)
+    open (newunit=u, file=trim(outDir)//"/pieces.bin", access="stream", form="unformatted", status="replace")
+    write (u) kin, pot, d
+    call MctdhxHip_Densities(rdm1, rdm2)
+    d = (0.0_R64, 0.0_R64)
+    call MctdhxHip_ApplyCorrelationOp(d, orbs, rdm1, rdm2, 0.0_R64, h2)
+    write (u) d, h2
+    close (u)
+  end subroutine
 end program

@@ -32,9 +32,32 @@ def test_fortran_driver_matches_oracle():
         print(r.stdout, r.stderr)
         assert r.returncode == 0
         dstate = np.fromfile(os.path.join(d, "dstate.bin"), dtype=np.complex128)
+        pieces = np.fromfile(os.path.join(d, "pieces.bin"), dtype=np.complex128)
         state5 = np.fromfile(os.path.join(d, "state5.bin"), dtype=np.complex128)
     vals = dict(line.split() for line in r.stdout.splitlines() if len(line.split()) == 2)
     assert state_rel(dstate, s.time_derivative(state), w.nC, wts, w.M) < 1e-12
+    # the operator pieces through the leaf's reference-interface procedures
+    import oracle
+
+    nG, M = w.nG, w.M
+    C, orbs = wl.split_state(state, w.nC, nG, M)
+    o = 0
+    kin, o = pieces[o:o + nG], o + nG
+    pot, o = pieces[o:o + nG], o + nG
+    proj, o = pieces[o:o + nG * M].reshape(M, nG).T, o + nG * M
+    corr, o = pieces[o:o + nG * M].reshape(M, nG).T, o + nG * M
+    h2f = pieces[o:o + M ** 4].reshape((M,) * 4, order="F")
+    vext = -w.pot_charge / (np.sqrt(x ** 2 + w.pot_soft1 ** 2) + w.pot_soft2)
+    tk = s.apply_single_body(orbs, t=-1.0)[0][:, 0] - vext * orbs[:, 0]
+    assert np.linalg.norm(kin - tk) < 1e-12 * np.linalg.norm(tk)
+    V = oracle.softyukawa(x[:, None] - x[None, :], w.int_strength, w.int_soft1, w.int_soft2, 0.0)
+    pr = V @ (np.conj(orbs[:, 0]) * orbs[:, 1] * wts)
+    assert np.linalg.norm(pot - pr) < 1e-13 * np.linalg.norm(pr)
+    pj = s.project_orbs(orbs, x[:, None] * orbs)
+    assert np.linalg.norm(proj - pj) < 1e-13 * np.linalg.norm(pj)
+    co, h2o = s.apply_correlation(orbs, s.rdm1(C), s.rdm2(C))
+    assert np.linalg.norm(corr - co) < 1e-12 * np.linalg.norm(co)
+    assert np.linalg.norm(h2f - h2o) < 1e-12 * np.linalg.norm(h2o)
     e0 = s.energy(state)
     assert abs(float(vals["energy0"]) - e0) < 1e-12 * abs(e0)
     so = state.copy()

@@ -351,3 +351,30 @@ def test_restart_files_round_trip(pair, tmp_path):
     got = np.fromfile(tmp_path / "rdm2Bt01_01.in", dtype="<c16")
     ref = s.rdm2(state[:w.nC]).ravel(order="F")
     assert np.max(np.abs(got - ref)) <= 1e-13
+
+
+def test_operator_pieces(pair):
+    """The piece entry points with the reference's procedure-pointer semantics: the interaction
+    potential of weighted sources (FillInteractionPotential) against the dense kernel matrix, the
+    correlation apply (accumulating, with h2), the orbital projector and the kinetic operator
+    against the oracle."""
+    import oracle
+
+    name, w, s, ctx, state, wts = pair
+    C, orbs = wl.split_state(state, w.nC, w.nG, w.M)
+    x = ctx.x
+    rng = np.random.default_rng(17)
+    ns = 3 if name != "m12" else w.M * (w.M + 1) // 2 + 5  # more sources than one contraction holds
+    src = (rng.standard_normal((w.nG, ns)) + 1j * rng.standard_normal((w.nG, ns))) * wts[:, None]
+    V = oracle.softyukawa(x[:, None] - x[None, :], w.int_strength, w.int_soft1, w.int_soft2, 0.0)
+    assert rel(ctx.interaction_potential(src), V @ src) < 1e-13
+    r1, r2 = s.rdm1(C), s.rdm2(C)
+    d0 = np.asfortranarray(rng.standard_normal((w.nG, w.M)) + 1j * rng.standard_normal((w.nG, w.M)))
+    dg, h2g = ctx.apply_correlation(orbs, r1, r2, d0)
+    do, h2o = s.apply_correlation(orbs, r1, r2)
+    assert rel(dg - d0, do) < 1e-12
+    assert rel(h2g, h2o) < 1e-12
+    assert rel(ctx.project_orbs(orbs, d0), s.project_orbs(orbs, d0)) < 1e-13
+    vext = -w.pot_charge / (np.sqrt(x ** 2 + w.pot_soft1 ** 2) + w.pot_soft2)
+    tk = s.apply_single_body(orbs, t=-1.0)[0] - vext[:, None] * orbs
+    assert rel(ctx.apply_kinetic(orbs), tk) < 1e-12
