@@ -377,14 +377,19 @@ def cpu_baseline(wl, state, budget_s: float):
     from tests.common import oracle_system
 
     s, x, wts = oracle_system(wl)
-    d = s.time_derivative(state)  # warm-up
-    n, t0 = 0, time.perf_counter()
+    d = s.time_derivative(state)  # warm-up (three evaluations)
+    s.time_derivative(state)
+    s.time_derivative(state)
+    times, t0 = [], time.perf_counter()
     while True:
+        t1 = time.perf_counter()
         s.time_derivative(state)
-        n += 1
+        times.append(time.perf_counter() - t1)
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 200:
+        if (el >= budget_s and len(times) >= 20) or len(times) >= 200:
             break
+    n = len(times)
+    med = float(np.median(times))
     threads = lib.orc_num_threads()
     # the same evaluation on one thread (a bounded sample: a few evaluations)
     lib.orc_set_num_threads(1)
@@ -397,12 +402,14 @@ def cpu_baseline(wl, state, budget_s: float):
             break
     lib.orc_set_num_threads(threads)
     return {
-        "value": n / el,
+        "value": 1.0 / med,
         "unit": "RHS evals/s",
         "cores": int(threads),
         "kind": "port",
-        "sample": f"{n} full RHS evaluations of the same {wl.nG}-point, M={wl.M}, N={wl.N} state "
-                  f"({el:.1f} s, OpenMP threads={threads}, no BLAS; built -O3 -march={'native' if lib_path else 'x86-64-v3'})",
+        "sample": f"median of {n} timed full RHS evaluations (after 3 warm-up) of the same {wl.nG}-point, "
+                  f"M={wl.M}, N={wl.N} state ({el:.1f} s, OpenMP threads={threads}, no BLAS; built -O3 "
+                  f"-march={'native' if lib_path else 'x86-64-v3'})",
+        "mean_value": n / el,
         "single_thread": {"value": n1 / el1, "unit": "RHS evals/s", "cores": 1,
                           "sample": f"{n1} evaluations of the same state on one OpenMP thread ({el1:.1f} s)"},
     }, d, s, wts
