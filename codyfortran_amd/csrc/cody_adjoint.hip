@@ -1144,6 +1144,7 @@ int cody_set_chebyshev(cody_ctx* c, double dt, double emax, double* lo, double* 
   a.lo = l;
   a.hi = h;
   a.dt = dt;
+  a.emax = emax;
   // a_k = (2 - delta_k0) (-i)^k J_k(tau r) e^{-i tau c}
   const double cc = 0.5 * (h + l), ph = -tau * cc;
   const double2 eph = make_double2(std::cos(ph), std::sin(ph));
@@ -1162,9 +1163,15 @@ int cody_set_chebyshev(cody_ctx* c, double dt, double emax, double* lo, double* 
 }
 
 static int check_cheb_dt(const cody_ctx* c, double dt) {
-  if (c->adj.K < 1) return aerr(CODY_ERR_STATE, "cody_set_chebyshev has not been called");
+  if (c->adj.K < 1)
+    return aerr(CODY_ERR_STATE, "cody_set_chebyshev has not been called since the one-body operator was set");
   if (std::fabs(dt - c->adj.dt) > 1e-12 * std::fabs(c->adj.dt))
     return aerr(CODY_ERR_ARG, "the step differs from the dt of cody_set_chebyshev");
+  // |E(t)| <= sum |c_k|: the expansion's interval must hold the spectrum for every field value
+  double bound = 0.0;
+  for (int k = 0; k < c->nLaser; ++k) bound += std::fabs(c->laserC[k]);
+  if (bound > c->adj.emax * (1.0 + 1e-12))
+    return aerr(CODY_ERR_ARG, "the laser field can exceed the emax of cody_set_chebyshev");
   return CODY_OK;
 }
 

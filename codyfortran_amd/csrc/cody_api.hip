@@ -261,6 +261,7 @@ int cody_set_grid(cody_ctx* c, const double* x, const double* w) {
   HIPCHK(hipMemcpy(c->d_x, x, sizeof(double) * c->nG, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_w, w, sizeof(double) * c->nG, hipMemcpyHostToDevice));
   c->grid_set = true;
+  c->adj.K = 0;  // the Chebyshev bounds of the one-body operator are stale
   c->drop_graphs();
   c->int_set = false;  // the interaction matrix depends on the grid
   return CODY_OK;
@@ -279,6 +280,7 @@ int cody_set_kinetic_fedvr(cody_ctx* c, double xmin, double xmax, int nE, int nL
   HIPCHK(hipMemcpy(c->d_kref, kref.data(), sizeof(double) * nLoc * nLoc, hipMemcpyHostToDevice));
   c->kin = KinLaunch{c->d_kref, nE, nLoc, xminExcl != 0 ? 1 : 0, nE / (xmax - xmin), -0.5 / mass};
   c->kin_set = true;
+  c->adj.K = 0;  // the Chebyshev bounds of the one-body operator are stale
   c->drop_graphs();
   return CODY_OK;
 }
@@ -288,6 +290,7 @@ int cody_set_potential(cody_ctx* c, const double* vext) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpy(c->d_vext, vext, sizeof(double) * c->nG, hipMemcpyHostToDevice));
   c->pot_set = true;
+  c->adj.K = 0;  // the Chebyshev bounds of the one-body operator are stale
   c->drop_graphs();
   return CODY_OK;
 }

@@ -77,8 +77,8 @@ struct AdjBuffers {
   double2* gpart = nullptr;  // orbital terms of the pair groups of the correlation adjoint
   double *STadj = nullptr, *WTadj = nullptr, *epart = nullptr, *eacc = nullptr, *J = nullptr;
   // Chebyshev half-steps exp(-i dt/2 h(E)) on [lo, hi] of degree K (cody_set_chebyshev)
-  int K = 0;
-  double lo = 0.0, hi = 0.0, dt = 0.0;
+  int K = 0;  // 0: not set up (or invalidated by a change of the grid, kinetic or potential)
+  double lo = 0.0, hi = 0.0, dt = 0.0, emax = 0.0;
   std::vector<double2> a;  // expansion coefficients a_0..a_K
   double2 *U1 = nullptr, *U2 = nullptr, *cacc = nullptr;
   double2* cb[3] = {};

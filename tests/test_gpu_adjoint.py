@@ -217,3 +217,26 @@ def test_autograd_harness_gradcheck():
     assert torch.autograd.gradcheck(lambda c_: DipoleAfterPropagation.apply(c_, ctx, state, 0.5, 0.0, 0.05, 10, 4),
                                     (cc,), eps=1e-6, atol=1e-8, rtol=1e-6)
     ctx.close()
+
+
+def test_chebyshev_setup_is_guarded():
+    """The Chebyshev half-steps refuse a step other than the one set up, a laser that can exceed
+    the field bound of the set-up, and a one-body operator changed after it."""
+    w = wl.WORKLOADS["mctdhf_1d_401"]
+    ctx, xg, wg = wl.make_context(w)
+    ctx.upload_state(wl.smooth_state(xg, wg, w.N, w.M, seed=1))
+    ctx.set_laser(np.array([0.2, 0.1]), 1.0)
+    with pytest.raises(hip.CodyError, match="cody_set_chebyshev"):
+        ctx.strang_cheb(0.0, 0.05)
+    ctx.set_chebyshev(0.05, 0.3)
+    ctx.strang_cheb(0.0, 0.05)
+    with pytest.raises(hip.CodyError, match="dt of cody_set_chebyshev"):
+        ctx.strang_cheb(0.0, 0.04)
+    ctx.set_laser(np.array([0.3, 0.1]), 1.0)
+    with pytest.raises(hip.CodyError, match="emax"):
+        ctx.strang_cheb(0.05, 0.1)
+    ctx.set_laser(np.array([0.2, 0.1]), 1.0)
+    ctx.set_potential_softyukawa([0.0], [3.0], [w.pot_soft1], [w.pot_soft2], [0.0])
+    with pytest.raises(hip.CodyError, match="cody_set_chebyshev"):
+        ctx.strang_cheb(0.05, 0.1)
+    ctx.close()
