@@ -22,6 +22,10 @@ T_LASER = 2.0
 CASES = {
     "n401": wl.WORKLOADS["mctdhf_1d_401"],
     "m8": small("m8", -20.0, 20.0, 40, 6, 4, 8),
+    "odd223": small("odd223", -15.0, 15.0, 37, 7, 3, 6),  # 223 points: not a multiple of 64
+    "m12": small("m12", -10.0, 10.0, 32, 5, 6, 12),  # 924 configurations, 78 pairs
+    "n1": small("n1", -6.0, 6.0, 9, 5, 1, 3),  # one particle: no doubles, rho2 = 0
+    "full_occupation": small("full_occupation", -6.0, 6.0, 9, 5, 3, 3),  # one configuration
     "n4096": wl.WORKLOADS["mctdhf_1d_4096"],
 }
 
