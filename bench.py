@@ -331,31 +331,51 @@ def batched_measurement(args, torch, wlm, check: bool):
     b = hip.Batch([c for c, _ in ctxs])
     stream = torch.cuda.Stream(device=dev)
     ps, po = [t.data_ptr() for t in d_s], [t.data_ptr() for t in d_o]
-    for _ in range(3):
-        b.time_derivative_dev(ps, po, 0.0, stream.cuda_stream)
-    torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.batch_steps):
-        b.time_derivative_dev(ps, po, 0.0, stream.cuda_stream)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / args.batch_steps
+
+    def timed():
+        for _ in range(3):
+            b.time_derivative_dev(ps, po, 0.0, stream.cuda_stream)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(args.batch_steps):
+            b.time_derivative_dev(ps, po, 0.0, stream.cuda_stream)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / args.batch_steps
+
+    refs = None
+    if check:
+        from tests.common import oracle_system
+
+        refs = []
+        for w, st in zip(ws, states):
+            s, _, _ = oracle_system(w)
+            refs.append(s.time_derivative(st))
+            del s  # frees the oracle's dense interaction matrix
+
+    def worst_err():
+        from tests.common import state_rel
+
+        return max(state_rel(o.cpu().numpy().view(np.complex128), r, w.nC, wts, w.M)
+                   for (ctx, wts), w, o, r in zip(ctxs, ws, d_o, refs))
+
+    ms = timed()
     flops = sum(2.0 * w.nG ** 2 * 2 * w.pairs for w in ws)
     out = {"instances": len(ws), "value": len(ws) * 1000.0 / ms, "unit": "instance RHS evals/s",
            "ms_per_batched_rhs": ms, "contraction_flop_per_batched_rhs": flops,
            "contraction_tflops_aggregate": flops / (ms * 1e-3) / 1e12, "dtype": "f64",
            "k_split_by_shape": splits, "schedule": "one HIP graph, 4 parallel chains"}
     if check:
-        from tests.common import oracle_system, state_rel
-
-        worst = 0.0
-        for (ctx, wts), w, st, o in zip(ctxs, ws, states, d_o):
-            s, _, _ = oracle_system(w)
-            err = state_rel(o.cpu().numpy().view(np.complex128), s.time_derivative(st), w.nC, wts, w.M)
-            worst = max(worst, err)
-            del s  # frees the oracle's dense interaction matrix
-        out["max_rhs_rel_err_vs_fp64_oracle"] = worst
+        out["max_rhs_rel_err_vs_fp64_oracle"] = worst_err()
+    # the same batch with the int8-sliced contraction in every instance
+    for ctx, _ in ctxs:
+        ctx.set_precision(hip.PREC_INT8_SLICED)
+    ms8 = timed()
+    out["int8_sliced"] = {"value": len(ws) * 1000.0 / ms8, "ms_per_batched_rhs": ms8,
+                          "dtype": "f64 (int8_sliced contraction)"}
+    if check:
+        out["int8_sliced"]["max_rhs_rel_err_vs_fp64_oracle"] = worst_err()
     b.close()
     for c, _ in ctxs:
         c.close()
