@@ -1205,13 +1205,14 @@ int cody_dipole_gradient(cody_ctx* c, double t0, double dt, int nSteps, int ever
   }
   // one tape per step of a segment (about 37 MB per step at 4096 points, M = 8, degree 32);
   // without room for them the backward pass re-runs each step into the single work tape
-  const size_t tape_bytes = sizeof(double2) * tape_elems(c) * every;
-  if (a.ntape < (size_t)every || a.tape_K != a.K) {
+  const int seg = std::min(every, nSteps);
+  const size_t tape_bytes = sizeof(double2) * tape_elems(c) * seg;
+  if (a.ntape < (size_t)seg || a.tape_K != a.K) {
     if (a.tape) hipFree(a.tape);
     a.tape = nullptr;
     a.ntape = 0;
     if (hipMalloc((void**)&a.tape, tape_bytes) == hipSuccess) {
-      a.ntape = every;
+      a.ntape = seg;
       a.tape_K = a.K;
     } else {
       (void)hipGetLastError();
@@ -1232,12 +1233,14 @@ int cody_dipole_gradient(cody_ctx* c, double t0, double dt, int nSteps, int ever
     CHK(ctx_alloc((void**)&a.ebar, sizeof(double) * 6 * nSteps));
     a.nebar = 6 * (size_t)nSteps;
   }
-  // forward pass with checkpoints
+  // forward pass with checkpoints; with one segment over the whole run and room for its tape,
+  // the forward pass records every step's tape itself and nothing is recomputed
   const StepTape work = default_tape(c);
+  const bool taped_forward = a.tape && every >= nSteps;
   for (int m = 0; m < nSteps; ++m) {
     if (m % every == 0)
       HIPCHK(hipMemcpyAsync(a.ck + nS * (m / every), c->d_state, sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
-    CHK(strang_cheb_step(c, c->d_state, t0 + m * dt, work, st));
+    CHK(strang_cheb_step(c, c->d_state, t0 + m * dt, taped_forward ? tape_slot(c, m) : work, st));
   }
   // the objective and its gradient at the final state
   launch_ci_density(c->d_state, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_reginv, st);
@@ -1249,8 +1252,10 @@ int cody_dipole_gradient(cody_ctx* c, double t0, double dt, int nSteps, int ever
     const int m0 = (int)s * every, m1 = std::min(nSteps, m0 + every);
     HIPCHK(hipMemcpyAsync(a.ys, a.ck + nS * s, sizeof(double2) * nS, hipMemcpyDeviceToDevice, st));
     if (a.tape) {
-      // recompute the segment once, keeping every step's tape, then run its adjoint backwards
-      for (int m = m0; m < m1; ++m) CHK(strang_cheb_step(c, a.ys, t0 + m * dt, tape_slot(c, m - m0), st));
+      // recompute the segment once (unless the forward pass kept its tape), keeping every
+      // step's tape, then run its adjoint backwards
+      if (!taped_forward)
+        for (int m = m0; m < m1; ++m) CHK(strang_cheb_step(c, a.ys, t0 + m * dt, tape_slot(c, m - m0), st));
       for (int m = m1 - 1; m >= m0; --m)
         CHK(strang_cheb_adj(c, t0 + m * dt, tape_slot(c, m - m0), a.ebar + 6 * (size_t)m, st));
     } else {

@@ -894,7 +894,7 @@ int orc_symeig(int n, const double* a, double* ev, double* V) {
  * its real 2nO x 2nO embedding [[Re, -Im], [Im, Re]]; f(embedding) = embedding of f(rho1). */
 int orc_reg_inverse(const orc_sys* s, const cplx* rdm1, cplx* R) {
   int n = s->nO, m = 2 * n;
-  double* E = (double*)malloc(sizeof(double) * m * m);
+  double* E = (double*)calloc((size_t)m * m, sizeof(double));
   double* V = (double*)malloc(sizeof(double) * m * m);
   double* ev = (double*)malloc(sizeof(double) * m);
   for (int j = 0; j < n; ++j)

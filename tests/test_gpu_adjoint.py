@@ -140,10 +140,13 @@ def test_gershgorin_bounds_contain_spectrum():
         assert lo <= ev[0] and ev[-1] <= hi
 
 
+@pytest.mark.parametrize("every", [3, 10, 25])
 @pytest.mark.parametrize("consistent", [False, True])
-def test_gradient_matches_oracle_adjoint(consistent):
-    """The checkpointed GPU gradient (checkpoints every 3 of 10 steps: a partial last segment)
-    against the oracle's stored-trajectory adjoint on the same Chebyshev-Strang map."""
+def test_gradient_matches_oracle_adjoint(consistent, every):
+    """The GPU gradient against the oracle's stored-trajectory adjoint on the same Chebyshev-Strang
+    map: checkpoints every 3 of 10 steps (a partial last segment), and checkpoint intervals of 10
+    and 25 >= the step count, where the forward pass keeps the whole tape and nothing is
+    recomputed."""
     w = wl.WORKLOADS["mctdhf_1d_401"]
     s, x, wts = oracle_system(w)
     ctx, xg, wg = wl.make_context(w)
@@ -156,7 +159,7 @@ def test_gradient_matches_oracle_adjoint(consistent):
     dt, n = 0.05, 10
     lo, hi, K = ctx.set_chebyshev(dt, float(np.sum(np.abs(c))))
     ctx.upload_state(state)
-    J, g = ctx.dipole_gradient(0.0, dt, n, checkpoint_every=3)
+    J, g = ctx.dipole_gradient(0.0, dt, n, checkpoint_every=every)
     J_o, g_o, _ = s.dipole_gradient(state, 0.0, dt, n, lo, hi, K, consistent=consistent)
     ctx.close()
     print(f"\n  J {J:.12e} vs {J_o:.12e}; grad {g} vs {g_o}")
