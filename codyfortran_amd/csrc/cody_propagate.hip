@@ -61,15 +61,24 @@ __device__ __forceinline__ double metric_at(int i, int nC, int nG, const double*
 }
 
 // sOut[k] = sum_b partial[b][k] for k < nv <= kPS, fixed order (4 waves over the block rows, then
-// the waves in order).
+// the waves in order).  Each wave's rows are loaded 16 at a time before they are added, so a
+// reduction costs two memory round trips rather than one per row (all 32 at once measured no
+// faster).
 __device__ void lz_reduce(const double2* __restrict__ partial, int nv, double2* sOut, double2 (*sTmp)[kPS]) {
+  static_assert(kLzBlocks % 64 == 0, "rows are loaded 16 per wave at a time");
   const int t = threadIdx.x, k = t & 63, q = t >> 6;
   if (k < nv) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int b = q; b < kLzBlocks; b += 4) {
-      const double2 p = partial[(size_t)b * kPS + k];
-      acc.x += p.x;
-      acc.y += p.y;
+#pragma unroll
+    for (int r0 = 0; r0 < kLzBlocks / 4; r0 += 16) {
+      double2 p[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) p[u] = partial[(size_t)(q + 4 * (r0 + u)) * kPS + k];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        acc.x += p[u].x;
+        acc.y += p[u].y;
+      }
     }
     sTmp[q][k] = acc;
   }
@@ -518,11 +527,20 @@ __global__ __launch_bounds__(256) void k_lz_combine(const int* __restrict__ ctl,
       out[i] = in[i];
       continue;
     }
+    // eight basis vectors' loads in flight at a time, accumulated in order
     double2 acc = make_double2(0.0, 0.0);
-    for (int k = 0; k < m; ++k) {
-      const double2 a = basis[k * ld + i], p = phi[k];
-      acc.x += a.x * p.x - a.y * p.y;
-      acc.y += a.x * p.y + a.y * p.x;
+    for (int k0 = 0; k0 < m; k0 += 8) {
+      double2 a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = k0 + u < m ? basis[(k0 + u) * ld + i] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (k0 + u < m) {
+          const double2 p = phi[k0 + u];
+          acc.x += a[u].x * p.x - a[u].y * p.y;
+          acc.y += a[u].x * p.y + a[u].y * p.x;
+        }
+      }
     }
     out[i] = make_double2(nr * acc.x, nr * acc.y);
   }
