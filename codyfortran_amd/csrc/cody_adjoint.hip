@@ -649,6 +649,61 @@ __global__ __launch_bounds__(256) void k_cheb_term(const double2* __restrict__ u
     acc[off] = s;
 }
 
+// Two terms per launch: u_k on the workgroup's 256-point tile and a halo of one element
+// (nLoc - 1 points) on either side, kept in LDS, then u_{k+1} = 2 ((h - c)/r) u_k - u_{k-1} on the
+// tile from it; s += a_k u_k + a_{k+1} u_{k+1} (s = a_0 u_0 first when k = 1), written to `out`
+// (the pair ends the series) or to acc.  Every value is formed by the operations, in the order,
+// of two k_cheb_term launches.
+constexpr int kChebHaloMax = 32;
+__global__ __launch_bounds__(256) void k_cheb_term2(const double2* __restrict__ uprev, const double2* __restrict__ ucur,
+                                                    double2* __restrict__ uk_out, double2* __restrict__ uk1_out,
+                                                    double2* __restrict__ acc, double2* __restrict__ out, KinParams kp,
+                                                    const double* __restrict__ w, const double* __restrict__ xg,
+                                                    const double* __restrict__ vext, double E, double cc, double rinv,
+                                                    double2 a0, double2 ak, double2 ak1, int first, int halo, int nG) {
+  __shared__ double2 sU[256 + 2 * kChebHaloMax];
+  const int j = blockIdx.y, x0 = blockIdx.x * 256, xlo = x0 - halo;
+  const double2* uc = ucur + (size_t)j * nG;
+  for (int t = threadIdx.x; t < 256 + 2 * halo; t += 256) {
+    const int x = xlo + t;
+    double2 un = cmk(0, 0);
+    if (x >= 0 && x < nG) {
+      const double2 u = uc[x];
+      double2 hu = kinetic_at(uc, x, kp, 1.0 / w[x]);
+      const double v = vext[x] + E * xg[x] - cc;
+      hu.x = fma(v, u.x, hu.x);
+      hu.y = fma(v, u.y, hu.y);
+      if (first) {
+        un = cscale(hu, rinv);
+      } else {
+        const double2 up = uprev[(size_t)j * nG + x];
+        un = cmk(2.0 * rinv * hu.x - up.x, 2.0 * rinv * hu.y - up.y);
+      }
+    }
+    sU[t] = un;
+  }
+  __syncthreads();
+  const int x = x0 + threadIdx.x;
+  if (x >= nG) return;
+  const size_t off = (size_t)j * nG + x;
+  const double2 uk = sU[threadIdx.x + halo];
+  uk_out[off] = uk;
+  double2 s = first ? cmul(a0, uc[x]) : acc[off];
+  cfma(s, ak, uk);
+  double2 hu = kinetic_at_lds(sU, xlo, x, kp, 1.0 / w[x]);
+  const double v = vext[x] + E * xg[x] - cc;
+  hu.x = fma(v, uk.x, hu.x);
+  hu.y = fma(v, uk.y, hu.y);
+  const double2 up = uc[x];
+  const double2 un = cmk(2.0 * rinv * hu.x - up.x, 2.0 * rinv * hu.y - up.y);
+  uk1_out[off] = un;
+  cfma(s, ak1, un);
+  if (out)
+    out[off] = s;
+  else
+    acc[off] = s;
+}
+
 // One Clenshaw step of the adjoint: b_k = conj(a_k) lam + f ((h - c)/r) b_{k+1} - b_{k+2}
 // (terms absent when the pointers are null) and eacc += f <b_{k+1}, (x/r) u_k> per point
 __global__ __launch_bounds__(256) void k_cheb_adj_term(const double2* __restrict__ lam,
@@ -676,6 +731,65 @@ __global__ __launch_bounds__(256) void k_cheb_adj_term(const double2* __restrict
   }
   if (bnext2) b = csub(b, bnext2[off]);
   bout[off] = b;
+}
+
+// Two Clenshaw steps per launch, k and k - 1 (k >= 2, so both have b_{k+1} and f = 2): b_k on the
+// tile and a one-element halo into LDS, then b_{k-1} on the tile from it; the eacc updates of both
+// steps in order.  Reads b_{k+1}, b_{k+2} and writes b_k, b_{k-1}: four distinct buffers, so no
+// workgroup overwrites what another one's halo still reads.  Same operations, in the same order,
+// as two k_cheb_adj_term launches.
+__global__ __launch_bounds__(256) void k_cheb_adj_term2(const double2* __restrict__ lam,
+                                                        const double2* __restrict__ bnext,
+                                                        const double2* __restrict__ bnext2, double2* __restrict__ bk_out,
+                                                        double2* __restrict__ bk1_out, const double2* __restrict__ uk,
+                                                        const double2* __restrict__ uk1, double* __restrict__ eacc,
+                                                        KinParams kp, const double* __restrict__ w,
+                                                        const double* __restrict__ xg, const double* __restrict__ vext,
+                                                        double E, double cc, double rinv, double2 cak, double2 cak1,
+                                                        int halo, int nG) {
+  __shared__ double2 sB[256 + 2 * kChebHaloMax];
+  const int j = blockIdx.y, x0 = blockIdx.x * 256, xlo = x0 - halo;
+  const double2* bn_o = bnext + (size_t)j * nG;
+  const double f = 2.0;
+  for (int t = threadIdx.x; t < 256 + 2 * halo; t += 256) {
+    const int x = xlo + t;
+    double2 b = cmk(0, 0);
+    if (x >= 0 && x < nG) {
+      const size_t off = (size_t)j * nG + x;
+      b = cmul(cak, lam[off]);
+      const double2 bn = bn_o[x];
+      double2 hb = kinetic_at(bn_o, x, kp, 1.0 / w[x]);
+      const double v = vext[x] + E * xg[x] - cc;
+      hb.x = fma(v, bn.x, hb.x);
+      hb.y = fma(v, bn.y, hb.y);
+      b.x = fma(f * rinv, hb.x, b.x);
+      b.y = fma(f * rinv, hb.y, b.y);
+      if (bnext2) b = csub(b, bnext2[off]);
+    }
+    sB[t] = b;
+  }
+  __syncthreads();
+  const int x = x0 + threadIdx.x;
+  if (x >= nG) return;
+  const size_t off = (size_t)j * nG + x;
+  const double2 bn = bn_o[x];
+  {
+    const double2 u = uk[off];
+    eacc[off] += f * rinv * w[x] * xg[x] * (bn.x * u.x + bn.y * u.y);
+  }
+  const double2 bk = sB[threadIdx.x + halo];
+  bk_out[off] = bk;
+  double2 b = cmul(cak1, lam[off]);
+  double2 hb = kinetic_at_lds(sB, xlo, x, kp, 1.0 / w[x]);
+  const double v = vext[x] + E * xg[x] - cc;
+  hb.x = fma(v, bk.x, hb.x);
+  hb.y = fma(v, bk.y, hb.y);
+  b.x = fma(f * rinv, hb.x, b.x);
+  b.y = fma(f * rinv, hb.y, b.y);
+  const double2 u = uk1[off];
+  eacc[off] += f * rinv * w[x] * xg[x] * (bk.x * u.x + bk.y * u.y);
+  b = csub(b, bn);
+  bk1_out[off] = b;
 }
 
 __global__ void k_zero_d(double* __restrict__ p, int n) {
@@ -954,7 +1068,15 @@ int cheb_half(cody_ctx* c, double2* orbs, double E, double2* U, hipStream_t st) 
   const KinParams kp = kin_params(c);
   HIPCHK(hipMemcpyAsync(U, orbs, sizeof(double2) * nOG, hipMemcpyDeviceToDevice, st));
   const dim3 grid(blocks(c->nG), c->nO);
-  for (int k = 1; k <= a.K; ++k)
+  const int halo = kp.nLoc - 1;
+  const bool pairs = halo >= 1 && halo <= kChebHaloMax;
+  int k = 1;
+  for (; pairs && k + 1 <= a.K; k += 2)
+    hipLaunchKernelGGL(k_cheb_term2, grid, dim3(256), 0, st, k >= 2 ? U + (size_t)(k - 2) * nOG : nullptr,
+                       U + (size_t)(k - 1) * nOG, U + (size_t)k * nOG, U + (size_t)(k + 1) * nOG, a.cacc,
+                       k + 1 == a.K ? orbs : nullptr, kp, c->d_w, c->d_x, c->d_vext, E, cc, rinv, a.a[0], a.a[k],
+                       a.a[k + 1], k == 1 ? 1 : 0, halo, c->nG);
+  for (; k <= a.K; ++k)
     hipLaunchKernelGGL(k_cheb_term, grid, dim3(256), 0, st, k >= 2 ? U + (size_t)(k - 2) * nOG : nullptr,
                        U + (size_t)(k - 1) * nOG, U + (size_t)k * nOG, a.cacc, k == a.K ? orbs : nullptr, kp, c->d_w,
                        c->d_x, c->d_vext, E, cc, rinv, a.a[0], a.a[k], k == 1 ? 1 : 0, c->nG);
@@ -971,11 +1093,20 @@ int cheb_half_adj(cody_ctx* c, double2* lamOrbs, double E, const double2* U, dou
   hipLaunchKernelGGL(k_zero_d, dim3(blocks(nOG)), dim3(256), 0, st, a.eacc, (int)nOG);
   const int K = a.K;
   auto conj_a = [&](int k) { return make_double2(a.a[k].x, -a.a[k].y); };
-  hipLaunchKernelGGL(k_cheb_adj_term, grid, dim3(256), 0, st, lamOrbs, nullptr, nullptr, a.cb[K % 3], nullptr,
+  // b_k lives in cb[k % 4] (b_0 in lamOrbs); steps K - 1 .. 2 run two per launch
+  hipLaunchKernelGGL(k_cheb_adj_term, grid, dim3(256), 0, st, lamOrbs, nullptr, nullptr, a.cb[K % 4], nullptr,
                      a.eacc, kp, c->d_w, c->d_x, c->d_vext, E, cc, rinv, conj_a(K), 0.0, c->nG);
-  for (int k = K - 1; k >= 0; --k)
-    hipLaunchKernelGGL(k_cheb_adj_term, grid, dim3(256), 0, st, lamOrbs, a.cb[(k + 1) % 3],
-                       k + 2 <= K ? a.cb[(k + 2) % 3] : nullptr, k == 0 ? lamOrbs : a.cb[k % 3],
+  const int halo = kp.nLoc - 1;
+  const bool pairs = halo >= 1 && halo <= kChebHaloMax;
+  int k = K - 1;
+  for (; pairs && k - 1 >= 1; k -= 2)
+    hipLaunchKernelGGL(k_cheb_adj_term2, grid, dim3(256), 0, st, lamOrbs, a.cb[(k + 1) % 4],
+                       k + 2 <= K ? a.cb[(k + 2) % 4] : nullptr, a.cb[k % 4], a.cb[(k - 1) % 4], U + (size_t)k * nOG,
+                       U + (size_t)(k - 1) * nOG, a.eacc, kp, c->d_w, c->d_x, c->d_vext, E, cc, rinv, conj_a(k),
+                       conj_a(k - 1), halo, c->nG);
+  for (; k >= 0; --k)
+    hipLaunchKernelGGL(k_cheb_adj_term, grid, dim3(256), 0, st, lamOrbs, a.cb[(k + 1) % 4],
+                       k + 2 <= K ? a.cb[(k + 2) % 4] : nullptr, k == 0 ? lamOrbs : a.cb[k % 4],
                        U + (size_t)k * nOG, a.eacc, kp, c->d_w, c->d_x, c->d_vext, E, cc, rinv, conj_a(k),
                        k >= 1 ? 2.0 : 1.0, c->nG);
   hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, st, a.eacc, (int)nOG, ebar);
@@ -1137,7 +1268,8 @@ int cody_set_chebyshev(cody_ctx* c, double dt, double emax, double* lo, double* 
     if ((r = ctx_alloc((void**)&a.U1, sizeof(double2) * nOG * (K + 1))) ||
         (r = ctx_alloc((void**)&a.U2, sizeof(double2) * nOG * (K + 1))) ||
         (r = ctx_alloc((void**)&a.cacc, sizeof(double2) * nOG)) || (r = ctx_alloc((void**)&a.cb[0], sizeof(double2) * nOG)) ||
-        (r = ctx_alloc((void**)&a.cb[1], sizeof(double2) * nOG)) || (r = ctx_alloc((void**)&a.cb[2], sizeof(double2) * nOG)))
+        (r = ctx_alloc((void**)&a.cb[1], sizeof(double2) * nOG)) || (r = ctx_alloc((void**)&a.cb[2], sizeof(double2) * nOG)) ||
+        (r = ctx_alloc((void**)&a.cb[3], sizeof(double2) * nOG)))
       return r;
   }
   a.K = K;

@@ -81,7 +81,7 @@ struct AdjBuffers {
   double lo = 0.0, hi = 0.0, dt = 0.0, emax = 0.0;
   std::vector<double2> a;  // expansion coefficients a_0..a_K
   double2 *U1 = nullptr, *U2 = nullptr, *cacc = nullptr;
-  double2* cb[3] = {};
+  double2* cb[4] = {};  // Clenshaw ring of the Chebyshev adjoint
   // Strang-step tape, adjoint state and checkpoints
   double2* y[4] = {};
   double2* dl[4] = {};
@@ -94,11 +94,11 @@ struct AdjBuffers {
   double* ebar = nullptr;
   size_t nebar = 0;
   void free_cheb() {
-    void* p[] = {U1, U2, cacc, cb[0], cb[1], cb[2]};
+    void* p[] = {U1, U2, cacc, cb[0], cb[1], cb[2], cb[3]};
     for (void* q : p)
       if (q) hipFree(q);
     U1 = U2 = cacc = nullptr;
-    cb[0] = cb[1] = cb[2] = nullptr;
+    cb[0] = cb[1] = cb[2] = cb[3] = nullptr;
     K = 0;
   }
   void release() {

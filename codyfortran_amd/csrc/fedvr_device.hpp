@@ -72,4 +72,34 @@ __device__ __forceinline__ double2 kinetic_at(const double2* __restrict__ f, int
   return cscale(acc, kp.invh * rw * kp.fac);
 }
 
+// kinetic_at with f staged in LDS: sF[j - xlo] holds f at grid index j, for every j that the
+// stencil of x reaches ([x - (nLoc - 1), x + nLoc - 1] inside [0, nG)).  Same operations in the
+// same order as kinetic_at.
+__device__ __forceinline__ double2 kinetic_at_lds(const double2* sF, int xlo, int x, const KinParams& kp,
+                                                  double rw) {
+  const int gi = x + kp.xminExcl;
+  const int per = kp.nLoc - 1;
+  double2 acc = cmk(0, 0);
+  int e = gi / per, a = gi % per;
+  for (int pass = 0; pass < 2; ++pass) {
+    int ee = e, aa = a;
+    if (pass == 1) {
+      if (a != 0 || e == 0) break;
+      ee = e - 1;
+      aa = per;
+    }
+    if (ee >= kp.nE) continue;
+    const int base = ee * per - kp.xminExcl;
+    for (int b = 0; b < kp.nLoc; ++b) {
+      const int j = base + b;
+      if (j < 0 || j >= kp.nG) continue;
+      const double kab = kp.kref[aa + b * kp.nLoc];
+      const double2 fj = sF[j - xlo];
+      acc.x = fma(kab, fj.x, acc.x);
+      acc.y = fma(kab, fj.y, acc.y);
+    }
+  }
+  return cscale(acc, kp.invh * rw * kp.fac);
+}
+
 }  // namespace cody
