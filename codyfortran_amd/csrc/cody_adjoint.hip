@@ -423,7 +423,8 @@ __global__ __launch_bounds__(256) void k_adj_rdm(const double2* __restrict__ rho
                                                  double reg, double2* __restrict__ r2b, double2* __restrict__ lin1,
                                                  double2* __restrict__ lin2) {
   constexpr int n2 = NO * NO, n3 = n2 * NO, n4 = n2 * n2;
-  __shared__ double2 s1[n2], sR[n2], sM[2 * n2], sT[n2], sQ[n2], sF[NO];
+  __shared__ double2 s1[n2], sR[n2], sM[2 * n2], sT[n2], sQ[n2], sF[NO], sRb[256];
+  static_assert(n2 <= 256, "one Rbar output per thread group member");
   const int t = threadIdx.x;
   for (int e = t; e < n2; e += 256) {
     s1[e] = rho1[e];
@@ -437,19 +438,39 @@ __global__ __launch_bounds__(256) void k_adj_rdm(const double2* __restrict__ rho
     for (int j1 = 0; j1 < NO; ++j1) cfma(acc, r2pb[i12 + n2 * (j1 + NO * l2)], cconj(sR[k + j1 * NO]));
     r2b[e] = acc;
   }
+  // Rbar(k, j1) = sum_{l2, i12} conj(rho2(i12, k, l2)) r2pb(i12, j1, l2): output o = k + j1 NO
+  // by G thread groups, group g taking l2 = g, g + G, ...; the group sums are added in order
+  {
+    constexpr int G = n2 <= 64 ? 4 : (n2 <= 128 ? 2 : 1), S = 256 / G;
+    const int o = t % S, g = t / S;
+    if (o < n2) {
+      const int k = o % NO, j1 = o / NO;
+      double2 acc = cmk(0, 0);
+      for (int l2 = g; l2 < NO; l2 += G) {
+        const double2* ra = rho2 + n2 * (k + NO * l2);
+        const double2* rb = r2pb + n2 * (j1 + NO * l2);
+#pragma unroll 8
+        for (int i12 = 0; i12 < n2; ++i12) cfmac(acc, ra[i12], rb[i12]);
+      }
+      sRb[g * S + o] = acc;
+    }
+  }
   for (int e = t; e < n2; e += 256) {
-    const int k = e % NO, j1 = e / NO;
-    double2 acc = cmk(0, 0);
-    for (int l2 = 0; l2 < NO; ++l2)
-      for (int i12 = 0; i12 < n2; ++i12)
-        cfmac(acc, rho2[i12 + n2 * (k + NO * l2)], r2pb[i12 + n2 * (j1 + NO * l2)]);
     // [A | Rbar], A = rho1^2 + eps
     const int i = e % NO, j = e / NO;
     double2 a = cmk(i == j ? reg : 0.0, 0.0);
 #pragma unroll
     for (int q = 0; q < NO; ++q) cfma(a, s1[i + q * NO], s1[q + j * NO]);
     sM[e] = a;
-    sM[n2 + e] = acc;  // Rbar(k, j1) at k + j1 NO
+  }
+  __syncthreads();
+  {
+    constexpr int G = n2 <= 64 ? 4 : (n2 <= 128 ? 2 : 1), S = 256 / G;
+    for (int e = t; e < n2; e += 256) {
+      double2 acc = sRb[e];
+      for (int g = 1; g < G; ++g) acc = cadd(acc, sRb[g * S + e]);
+      sM[n2 + e] = acc;  // Rbar(k, j1) at k + j1 NO
+    }
   }
   __syncthreads();
   for (int k = 0; k < NO; ++k) {
@@ -578,7 +599,8 @@ __global__ __launch_bounds__(256) void k_adj_final(const double2* __restrict__ o
 __global__ __launch_bounds__(256) void k_adj_sum(const double* __restrict__ v, int n, double* __restrict__ out) {
   __shared__ double red[4];
   double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) acc += v[i];
+#pragma unroll 16
+  for (int i = threadIdx.x; i < n; i += 256) acc += v[i];  // 16 loads in flight, added in order
   const double s = block_sum(acc, red);
   if (threadIdx.x == 0) *out = s;
 }
