@@ -396,7 +396,8 @@ __global__ __launch_bounds__(256) void k_adj_r2pb(const double2* __restrict__ or
   double2 acc[NO];
 #pragma unroll
   for (int j = 0; j < NO; ++j) acc[j] = cmk(0, 0);
-  for (int x = threadIdx.x; x < nG; x += 256) {
+#pragma unroll 4
+  for (int x = threadIdx.x; x < nG; x += 256) {  // four points' loads in flight, added in order
     const double2 v = cscale(cconj(cmul(w_ab<NO>(WT, nGp, a, b, x), orbs[(size_t)k1 * nG + x])), w[x]);
 #pragma unroll
     for (int j = 0; j < NO; ++j) cfma(acc[j], v, dbar[(size_t)j * nG + x]);
