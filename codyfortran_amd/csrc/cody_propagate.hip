@@ -209,14 +209,25 @@ __global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ pa
 // and the partial sums of c_k = <v_k|w>_m for k <= j.  c_j is alpha_j + i (rounding), so the
 // reference's w -= alpha_j v_j followed by its projections on v_0..v_j (:237-246) is the single
 // subtraction of sum_k c_k v_k that k_lz_pass1 performs.
+// The launch has one workgroup more than the vector work: workgroup kLzBlocks ends iteration
+// j - 1 (lz_eval on T_j) while the others form w, so the subspace exponential is off the
+// critical path.  When it ends the loop, the w of this launch is never used.
+__device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m, int maxDim, const double* dtp,
+                        double tol, double2* phi, double* flags);
 __global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ basis, size_t ld, int j,
-                                                  const double* __restrict__ beta, const int* __restrict__ ctl,
+                                                  const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                  int* ctl, int maxDim, const double* __restrict__ dtp, double tol,
+                                                  double2* __restrict__ phi, double* __restrict__ flags,
                                                   const double* __restrict__ w, const double* __restrict__ xg,
                                                   const double* __restrict__ vext, KinParams kp,
                                                   const double* __restrict__ efield, int nC, int nG, int n,
                                                   const double* __restrict__ metric, double2* __restrict__ wout,
                                                   double2* __restrict__ pout) {
   __shared__ double sr[4][kPS], si[4][kPS];
+  if (blockIdx.x == kLzBlocks) {
+    if (j > 0 && !ctl[0]) lz_eval(ctl, alpha, beta, j, maxDim, dtp, tol, phi, flags);
+    return;
+  }
   if (ctl[0]) return;
   const double E = *efield;
   const double bprev = j > 0 ? beta[j - 1] : 0.0;
@@ -455,27 +466,19 @@ __device__ void subspace_exp(const double* sA, const double* sB, int m, double d
   __syncthreads();
 }
 
-// Second pass and the end of iteration j: c2 and |w|^2 from pin; w2 = w - sum c2_k v_k;
+// Second pass of iteration j: c2 and |w|^2 from pin; w2 = w - sum c2_k v_k;
 // beta_j = |w2|_m = sqrt(|w|^2 - sum |c2_k|^2) (the v_k are orthonormal in the metric and the
-// second pass removes rounding-size components only); v_{j+1} = w2 / beta_j.  Block 0 evaluates
-// phi = exp(-i dt T_{j+1}) e1, the error estimate beta_j |phi_j| (0 on breakdown) and ends the
-// loop when it is within tol, on breakdown, or at the maximal dimension (:238-254).
-// flags = (error, converged).
-__global__ __launch_bounds__(256) void k_lz_pass2(int* __restrict__ ctl, const double2* __restrict__ pin,
-                                                  const double* __restrict__ alpha, double* __restrict__ beta,
-                                                  int j, int maxDim, const double2* __restrict__ basis, size_t ld,
-                                                  const double2* __restrict__ w, double2* __restrict__ vnext, int n,
-                                                  const double* __restrict__ dtp, double tol,
-                                                  double2* __restrict__ phi, double* __restrict__ flags) {
-  __shared__ double2 E[kMaxKrylov * kMaxKrylov], Tm[kMaxKrylov * kMaxKrylov], Pw[kMaxKrylov * kMaxKrylov];
-  __shared__ double2 sC[kPS], sTmp[4][kPS], sPhi[kMaxKrylov];
-  __shared__ double sA[kMaxKrylov], sB[kMaxKrylov], sJ[kChebMax + 1], sPar[4];
+// second pass removes rounding-size components only); v_{j+1} = w2 / beta_j (not on breakdown).
+// Block 0 records beta_j.  The end of the iteration (:238-254: phi = exp(-i dt T_{j+1}) e1, the
+// error estimate beta_j |phi_j|, the end of the loop) is lz_eval in the next k_lz_apply launch,
+// or in k_lz_finish after the last iteration.
+__global__ __launch_bounds__(256) void k_lz_pass2(const int* __restrict__ ctl, const double2* __restrict__ pin,
+                                                  double* __restrict__ beta, int j,
+                                                  const double2* __restrict__ basis, size_t ld,
+                                                  const double2* __restrict__ w, double2* __restrict__ vnext, int n) {
+  __shared__ double2 sC[kPS], sTmp[4][kPS];
   __shared__ double sBeta;
-  __shared__ int sDone, sSq;
-  // one read of the flag per block: block 0 of this launch may set it while others start
-  if (threadIdx.x == 0) sDone = ctl[0];
-  __syncthreads();
-  if (sDone) return;
+  if (ctl[0]) return;
   const int nv = j + 1;
   lz_reduce(pin, nv + 1, sC, sTmp);
   if (threadIdx.x == 0) {
@@ -492,26 +495,48 @@ __global__ __launch_bounds__(256) void k_lz_pass2(int* __restrict__ ctl, const d
       vnext[i] = make_double2(r.x / bj, r.y / bj);
     }
   }
-  if (blockIdx.x != 0) return;
-  for (int k = threadIdx.x; k <= j; k += 256) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) beta[j] = bj;
+}
+
+// The end of a Lanczos iteration with Krylov dimension m (one workgroup): phi = exp(-i dt T_m) e1
+// for the tridiagonal alpha[0..m-1], beta[0..m-2], the error estimate beta[m-1] |phi[m-1]| (0 on
+// breakdown, beta[m-1] <= zeroThreshold), flags = (error, converged), and the end of the loop
+// (ctl) when it is within tol, on breakdown, or at m = maxDim (:238-254).
+__device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m, int maxDim, const double* dtp,
+                        double tol, double2* phi, double* flags) {
+  __shared__ double2 E[kMaxKrylov * kMaxKrylov], Tm[kMaxKrylov * kMaxKrylov], Pw[kMaxKrylov * kMaxKrylov];
+  __shared__ double2 sPhi[kMaxKrylov];
+  __shared__ double sA[kMaxKrylov], sB[kMaxKrylov], sJ[kChebMax + 1], sPar[4];
+  __shared__ int sSq;
+  for (int k = threadIdx.x; k < m; k += blockDim.x) {
     sA[k] = alpha[k];
-    sB[k] = k < j ? beta[k] : bj;
+    sB[k] = beta[k];
   }
   __syncthreads();
-  subspace_exp(sA, sB, nv, *dtp, sPhi, sJ, sPar, E, Tm, Pw, &sSq);
-  for (int k = threadIdx.x; k < nv; k += 256) phi[k] = sPhi[k];
+  const double bj = sB[m - 1];
+  const bool breakdown = bj <= kZero;
+  subspace_exp(sA, sB, m, *dtp, sPhi, sJ, sPar, E, Tm, Pw, &sSq);
+  for (int k = threadIdx.x; k < m; k += blockDim.x) phi[k] = sPhi[k];
   if (threadIdx.x == 0) {
-    beta[j] = bj;
-    const double2 last = sPhi[j];
+    const double2 last = sPhi[m - 1];
     const double err = breakdown ? 0.0 : bj * sqrt(last.x * last.x + last.y * last.y);
     const bool conv = err <= tol;
     flags[0] = err;
     flags[1] = conv ? 1.0 : 0.0;
-    if (conv || breakdown || j == maxDim - 1) {
-      ctl[1] = j + 1;
+    if (conv || breakdown || m == maxDim) {
+      ctl[1] = m;
       ctl[0] = 1;
     }
   }
+}
+
+// After the last iteration: ends the loop at the maximal dimension unless it has ended.
+__global__ __launch_bounds__(256) void k_lz_finish(int* ctl, const double* __restrict__ alpha,
+                                                   const double* __restrict__ beta, int maxDim,
+                                                   const double* __restrict__ dtp, double tol,
+                                                   double2* __restrict__ phi, double* __restrict__ flags) {
+  if (ctl[0]) return;
+  lz_eval(ctl, alpha, beta, maxDim, maxDim, dtp, tol, phi, flags);
 }
 
 // out = |in|_m sum_{k < mDim} phi_k v_k, or out = in for a zero input (:258).  out may be in.
@@ -624,15 +649,16 @@ void enqueue_sil_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, in
   hipLaunchKernelGGL(k_lz_norm, g, t, 0, st, in, n, c->nC, c->nG, metric, p0, b->ctl);
   hipLaunchKernelGGL(k_lz_start, g, t, 0, st, p0, in, b->basis, n, b->nrm2, b->ctl, flags);
   const KinParams kp{c->kin.kref, c->kin.nE, c->kin.nLoc, c->kin.xminExcl, c->nG, c->kin.invh, c->kin.fac};
+  const dim3 ga(kLzBlocks + 1);  // the vector work and the end of the previous iteration
   for (int j = 0; j < maxDim; ++j) {
-    hipLaunchKernelGGL(k_lz_apply, g, t, 0, st, b->basis, ld, j, b->beta, b->ctl, c->d_w, c->d_x, c->d_vext, kp,
-                       efp, c->nC, c->nG, n, metric, b->w, p0);
+    hipLaunchKernelGGL(k_lz_apply, ga, t, 0, st, b->basis, ld, j, b->alpha, b->beta, b->ctl, maxDim, dtp, tol, b->phi,
+                       flags, c->d_w, c->d_x, c->d_vext, kp, efp, c->nC, c->nG, n, metric, b->w, p0);
     hipLaunchKernelGGL(k_lz_pass1, g, t, 0, st, b->ctl, p0, j, b->basis, ld, b->w, n, c->nC, c->nG, metric,
                        b->alpha, p1);
     double2* vnext = j + 1 < maxDim ? b->basis + (size_t)(j + 1) * ld : nullptr;  // slot j+1 (disjoint from 0..j)
-    hipLaunchKernelGGL(k_lz_pass2, g, t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, b->basis, ld, b->w, vnext,
-                       n, dtp, tol, b->phi, flags);
+    hipLaunchKernelGGL(k_lz_pass2, g, t, 0, st, b->ctl, p1, b->beta, j, b->basis, ld, b->w, vnext, n);
   }
+  hipLaunchKernelGGL(k_lz_finish, dim3(1), t, 0, st, b->ctl, b->alpha, b->beta, maxDim, dtp, tol, b->phi, flags);
   hipLaunchKernelGGL(k_lz_combine, blocks_for(n), t, 0, st, b->ctl, b->basis, ld, b->phi, b->nrm2, in, out, n);
 }
 
