@@ -44,7 +44,7 @@ struct PropGraph {
 // Work buffers of the device-side propagators (allocated on first use).
 struct PropBuffers {
   int kdim = 0;
-  double2 *basis = nullptr, *w = nullptr, *partial = nullptr, *phi = nullptr;
+  double2 *basis = nullptr, *w = nullptr, *w2 = nullptr, *partial = nullptr, *phi = nullptr;
   double2 *trial = nullptr, *work = nullptr, *backup = nullptr;
   double *alpha = nullptr, *beta = nullptr, *nrm2 = nullptr, *par = nullptr, *flags = nullptr;
   double* h_flags = nullptr;  // pinned host copy of flags
@@ -58,7 +58,7 @@ struct PropBuffers {
 
   void release() {
     drop_graphs();
-    void* ptrs[] = {basis, w, partial, phi, trial, work, backup, alpha, beta, nrm2, par, flags, ctl,
+    void* ptrs[] = {basis, w, w2, partial, phi, trial, work, backup, alpha, beta, nrm2, par, flags, ctl,
                     rk_inc, rk_tmp, rk_d};
     for (void* p : ptrs)
       if (p) hipFree(p);

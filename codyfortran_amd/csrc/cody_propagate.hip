@@ -208,26 +208,16 @@ __global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ pa
 //   w = H v_j - beta_{j-1} v_{j-1},  H v = (T + V_ext + E x) v on each orbital block, 0 on C,
 // and the partial sums of c_k = <v_k|w>_m for k <= j.  c_j is alpha_j + i (rounding), so the
 // reference's w -= alpha_j v_j followed by its projections on v_0..v_j (:237-246) is the single
-// subtraction of sum_k c_k v_k that k_lz_pass1 performs.
-// The launch has one workgroup more than the vector work: workgroup kLzBlocks ends iteration
-// j - 1 (lz_eval on T_j) while the others form w, so the subspace exponential is off the
-// critical path.  When it ends the loop, the w of this launch is never used.
-__device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m, int maxDim, const double* dtp,
-                        double tol, double2* phi, double* flags);
+// subtraction of sum_k c_k v_k that k_lz_pass1 performs.  Used for iteration 0; the later
+// iterations form w inside k_lz_next.
 __global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ basis, size_t ld, int j,
-                                                  const double* __restrict__ alpha, const double* __restrict__ beta,
-                                                  int* ctl, int maxDim, const double* __restrict__ dtp, double tol,
-                                                  double2* __restrict__ phi, double* __restrict__ flags,
+                                                  const double* __restrict__ beta, const int* __restrict__ ctl,
                                                   const double* __restrict__ w, const double* __restrict__ xg,
                                                   const double* __restrict__ vext, KinParams kp,
                                                   const double* __restrict__ efield, int nC, int nG, int n,
                                                   const double* __restrict__ metric, double2* __restrict__ wout,
                                                   double2* __restrict__ pout) {
   __shared__ double sr[4][kPS], si[4][kPS];
-  if (blockIdx.x == kLzBlocks) {
-    if (j > 0 && !ctl[0]) lz_eval(ctl, alpha, beta, j, maxDim, dtp, tol, phi, flags);
-    return;
-  }
   if (ctl[0]) return;
   const double E = *efield;
   const double bprev = j > 0 ? beta[j - 1] : 0.0;
@@ -466,60 +456,28 @@ __device__ void subspace_exp(const double* sA, const double* sB, int m, double d
   __syncthreads();
 }
 
-// Second pass of iteration j: c2 and |w|^2 from pin; w2 = w - sum c2_k v_k;
-// beta_j = |w2|_m = sqrt(|w|^2 - sum |c2_k|^2) (the v_k are orthonormal in the metric and the
-// second pass removes rounding-size components only); v_{j+1} = w2 / beta_j (not on breakdown).
-// Block 0 records beta_j.  The end of the iteration (:238-254: phi = exp(-i dt T_{j+1}) e1, the
-// error estimate beta_j |phi_j|, the end of the loop) is lz_eval in the next k_lz_apply launch,
-// or in k_lz_finish after the last iteration.
-__global__ __launch_bounds__(256) void k_lz_pass2(const int* __restrict__ ctl, const double2* __restrict__ pin,
-                                                  double* __restrict__ beta, int j,
-                                                  const double2* __restrict__ basis, size_t ld,
-                                                  const double2* __restrict__ w, double2* __restrict__ vnext, int n) {
-  __shared__ double2 sC[kPS], sTmp[4][kPS];
-  __shared__ double sBeta;
-  if (ctl[0]) return;
-  const int nv = j + 1;
-  lz_reduce(pin, nv + 1, sC, sTmp);
-  if (threadIdx.x == 0) {
-    double b2 = sC[nv].x;
-    for (int k = 0; k < nv; ++k) b2 -= sC[k].x * sC[k].x + sC[k].y * sC[k].y;
-    sBeta = sqrt(fmax(b2, 0.0));
-  }
-  __syncthreads();
-  const double bj = sBeta;
-  const bool breakdown = bj <= kZero;
-  if (vnext && !breakdown) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
-      const double2 r = lz_subtract(w[i], basis, ld, nv, sC, i);
-      vnext[i] = make_double2(r.x / bj, r.y / bj);
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) beta[j] = bj;
-}
-
 // The end of a Lanczos iteration with Krylov dimension m (one workgroup): phi = exp(-i dt T_m) e1
-// for the tridiagonal alpha[0..m-1], beta[0..m-2], the error estimate beta[m-1] |phi[m-1]| (0 on
-// breakdown, beta[m-1] <= zeroThreshold), flags = (error, converged), and the end of the loop
-// (ctl) when it is within tol, on breakdown, or at m = maxDim (:238-254).
-__device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m, int maxDim, const double* dtp,
-                        double tol, double2* phi, double* flags) {
-  __shared__ double2 E[kMaxKrylov * kMaxKrylov], Tm[kMaxKrylov * kMaxKrylov], Pw[kMaxKrylov * kMaxKrylov];
+// for the tridiagonal alpha[0..m-1], beta[0..m-2], the error estimate bres |phi[m-1]| with the
+// residual coupling bres = beta_{m-1} (0 on breakdown, bres <= zeroThreshold), flags = (error,
+// converged), and the end of the loop (ctl) when it is within tol, on breakdown, or at m = maxDim
+// (:238-254).  ws: 3 kMaxKrylov^2 values of LDS for the Taylor fallback of the exponential.
+__device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m, double bres, int maxDim,
+                        const double* dtp, double tol, double2* phi, double* flags, double2* ws) {
   __shared__ double2 sPhi[kMaxKrylov];
   __shared__ double sA[kMaxKrylov], sB[kMaxKrylov], sJ[kChebMax + 1], sPar[4];
   __shared__ int sSq;
   for (int k = threadIdx.x; k < m; k += blockDim.x) {
     sA[k] = alpha[k];
-    sB[k] = beta[k];
+    sB[k] = k < m - 1 ? beta[k] : bres;
   }
   __syncthreads();
-  const double bj = sB[m - 1];
-  const bool breakdown = bj <= kZero;
-  subspace_exp(sA, sB, m, *dtp, sPhi, sJ, sPar, E, Tm, Pw, &sSq);
+  const bool breakdown = bres <= kZero;
+  constexpr int K2 = kMaxKrylov * kMaxKrylov;
+  subspace_exp(sA, sB, m, *dtp, sPhi, sJ, sPar, ws, ws + K2, ws + 2 * K2, &sSq);
   for (int k = threadIdx.x; k < m; k += blockDim.x) phi[k] = sPhi[k];
   if (threadIdx.x == 0) {
     const double2 last = sPhi[m - 1];
-    const double err = breakdown ? 0.0 : bj * sqrt(last.x * last.x + last.y * last.y);
+    const double err = breakdown ? 0.0 : bres * sqrt(last.x * last.x + last.y * last.y);
     const bool conv = err <= tol;
     flags[0] = err;
     flags[1] = conv ? 1.0 : 0.0;
@@ -530,13 +488,103 @@ __device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m
   }
 }
 
-// After the last iteration: ends the loop at the maximal dimension unless it has ended.
-__global__ __launch_bounds__(256) void k_lz_finish(int* ctl, const double* __restrict__ alpha,
-                                                   const double* __restrict__ beta, int maxDim,
-                                                   const double* __restrict__ dtp, double tol,
-                                                   double2* __restrict__ phi, double* __restrict__ flags) {
+// beta_j = |w2|_m = sqrt(|w|^2 - sum |c2_k|^2) from the second pass's sums (the v_k are
+// orthonormal in the metric and the second pass removes rounding-size components only); every
+// workgroup forms it from the same sums in the same order.
+__device__ double lz_beta(const double2* __restrict__ pin, int nv, double2* sC, double2 (*sTmp)[kPS], double* sBeta) {
+  lz_reduce(pin, nv + 1, sC, sTmp);
+  if (threadIdx.x == 0) {
+    double b2 = sC[nv].x;
+    for (int k = 0; k < nv; ++k) b2 -= sC[k].x * sC[k].x + sC[k].y * sC[k].y;
+    *sBeta = sqrt(fmax(b2, 0.0));
+  }
+  __syncthreads();
+  return *sBeta;
+}
+
+// The end of iteration j and the first step of iteration j + 1 in one launch (j + 1 < maxDim):
+//   v_{j+1} = (w - sum c2_k v_k) / beta_j (second pass, c2 from pin),
+//   w' = H v_{j+1} - beta_j v_j (as k_lz_apply) and the partial sums of <v_k|w'>_m, k <= j + 1.
+// v_{j+1} is formed on each 256-entry tile and on `halo` = nLoc - 1 entries either side (the
+// reach of the FEDVR stencil) in LDS, so H needs no other workgroup's values; w' goes to the
+// other w buffer, so no workgroup overwrites the w that another one's halo reads.  Workgroup
+// kLzBlocks ends iteration j (lz_eval on T_{j+1}) while the others work; when it ends the loop
+// the launch's vector results are never used.  Block 0 records beta_j.
+__global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __restrict__ pin,
+                                                 const double* __restrict__ alpha, double* __restrict__ beta, int j,
+                                                 int maxDim, const double* __restrict__ dtp, double tol,
+                                                 double2* __restrict__ phi, double* __restrict__ flags,
+                                                 double2* __restrict__ basis, size_t ld,
+                                                 const double2* __restrict__ wcur, double2* __restrict__ wnext,
+                                                 const double* __restrict__ wg, const double* __restrict__ xg,
+                                                 const double* __restrict__ vext, KinParams kp,
+                                                 const double* __restrict__ efield, int nC, int nG, int n,
+                                                 const double* __restrict__ metric, double2* __restrict__ pout,
+                                                 int halo) {
+  __shared__ double2 sWork[3 * kMaxKrylov * kMaxKrylov];  // exponential workspace, or the v tile
+  __shared__ double2 sC[kPS], sTmp[4][kPS];
+  __shared__ double sr[4][kPS], si[4][kPS];
+  __shared__ double sBeta;
   if (ctl[0]) return;
-  lz_eval(ctl, alpha, beta, maxDim, maxDim, dtp, tol, phi, flags);
+  const int nv = j + 1;
+  const double bj = lz_beta(pin, nv, sC, sTmp, &sBeta);
+  if (blockIdx.x == kLzBlocks) {
+    lz_eval(ctl, alpha, beta, nv, bj, maxDim, dtp, tol, phi, flags, sWork);
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) beta[j] = bj;
+  if (bj <= kZero) return;  // breakdown: workgroup kLzBlocks ends the loop
+  const double E = *efield;
+  double2* vn = basis + (size_t)nv * ld;
+  const double2* vj = basis + (size_t)j * ld;
+  double2* sV = sWork;
+  for (int tile = blockIdx.x; tile * 256 < n; tile += kLzBlocks) {
+    const int i0 = tile * 256, lo = i0 - halo;
+    __syncthreads();  // the previous tile's reads of sV are done
+    for (int t = threadIdx.x; t < 256 + 2 * halo; t += 256) {
+      const int i = lo + t;
+      double2 v = make_double2(0.0, 0.0);
+      if (i >= 0 && i < n) {
+        const double2 r = lz_subtract(wcur[i], basis, ld, nv, sC, i);
+        v = make_double2(r.x / bj, r.y / bj);
+      }
+      sV[t] = v;
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.x;
+    if (i < n) {
+      const double2 vi = sV[threadIdx.x + halo];
+      vn[i] = vi;
+      double2 h = cmk(0, 0);
+      if (i >= nC) {
+        const int o = (i - nC) / nG, g = (i - nC) - o * nG;
+        h = kinetic_at_lds(sV, lo - (nC + o * nG), g, kp, 1.0 / wg[g]);
+        const double pot = vext[g] + E * xg[g];
+        h.x = fma(pot, vi.x, h.x);
+        h.y = fma(pot, vi.y, h.y);
+      }
+      const double2 p = vj[i];
+      h.x -= bj * p.x;
+      h.y -= bj * p.y;
+      wnext[i] = h;
+    }
+  }
+  lz_dots(basis, ld, nv + 1, 0, wnext, n, nC, nG, metric, pout, sr, si);
+}
+
+// The end of the last iteration j = maxDim - 1 (one workgroup): beta_j and lz_eval on T_maxDim,
+// which ends the loop.
+__global__ __launch_bounds__(256) void k_lz_last(int* ctl, const double2* __restrict__ pin,
+                                                 const double* __restrict__ alpha, double* __restrict__ beta, int j,
+                                                 int maxDim, const double* __restrict__ dtp, double tol,
+                                                 double2* __restrict__ phi, double* __restrict__ flags) {
+  __shared__ double2 sWork[3 * kMaxKrylov * kMaxKrylov];
+  __shared__ double2 sC[kPS], sTmp[4][kPS];
+  __shared__ double sBeta;
+  if (ctl[0]) return;
+  const double bj = lz_beta(pin, j + 1, sC, sTmp, &sBeta);
+  if (threadIdx.x == 0) beta[j] = bj;
+  lz_eval(ctl, alpha, beta, j + 1, bj, maxDim, dtp, tol, phi, flags, sWork);
 }
 
 // out = |in|_m sum_{k < mDim} phi_k v_k, or out = in for a zero input (:258).  out may be in.
@@ -611,6 +659,7 @@ PropBuffers* buffers(cody_ctx* c, int kdim) {
   b.kdim = kdim;
   const size_t n = c->nS;
   if (ctx_alloc((void**)&b.basis, sizeof(double2) * n * kdim) || ctx_alloc((void**)&b.w, sizeof(double2) * n) ||
+      ctx_alloc((void**)&b.w2, sizeof(double2) * n) ||
       ctx_alloc((void**)&b.partial, sizeof(double2) * 2 * kLzBlocks * kPS) ||
       ctx_alloc((void**)&b.phi, sizeof(double2) * kMaxKrylov) ||
       ctx_alloc((void**)&b.trial, sizeof(double2) * n) || ctx_alloc((void**)&b.work, sizeof(double2) * n) ||
@@ -649,16 +698,23 @@ void enqueue_sil_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, in
   hipLaunchKernelGGL(k_lz_norm, g, t, 0, st, in, n, c->nC, c->nG, metric, p0, b->ctl);
   hipLaunchKernelGGL(k_lz_start, g, t, 0, st, p0, in, b->basis, n, b->nrm2, b->ctl, flags);
   const KinParams kp{c->kin.kref, c->kin.nE, c->kin.nLoc, c->kin.xminExcl, c->nG, c->kin.invh, c->kin.fac};
-  const dim3 ga(kLzBlocks + 1);  // the vector work and the end of the previous iteration
+  // iteration j: k_lz_pass1 on w[j % 2], then k_lz_next (the end of j and w[(j + 1) % 2] of
+  // j + 1, one more workgroup for the end of j) or, after the last iteration, k_lz_last
+  double2* wb[2] = {b->w, b->w2};
+  const int halo = kp.nLoc - 1;
+  hipLaunchKernelGGL(k_lz_apply, g, t, 0, st, b->basis, ld, 0, b->beta, b->ctl, c->d_w, c->d_x, c->d_vext, kp, efp,
+                     c->nC, c->nG, n, metric, wb[0], p0);
   for (int j = 0; j < maxDim; ++j) {
-    hipLaunchKernelGGL(k_lz_apply, ga, t, 0, st, b->basis, ld, j, b->alpha, b->beta, b->ctl, maxDim, dtp, tol, b->phi,
-                       flags, c->d_w, c->d_x, c->d_vext, kp, efp, c->nC, c->nG, n, metric, b->w, p0);
-    hipLaunchKernelGGL(k_lz_pass1, g, t, 0, st, b->ctl, p0, j, b->basis, ld, b->w, n, c->nC, c->nG, metric,
+    hipLaunchKernelGGL(k_lz_pass1, g, t, 0, st, b->ctl, p0, j, b->basis, ld, wb[j % 2], n, c->nC, c->nG, metric,
                        b->alpha, p1);
-    double2* vnext = j + 1 < maxDim ? b->basis + (size_t)(j + 1) * ld : nullptr;  // slot j+1 (disjoint from 0..j)
-    hipLaunchKernelGGL(k_lz_pass2, g, t, 0, st, b->ctl, p1, b->beta, j, b->basis, ld, b->w, vnext, n);
+    if (j + 1 < maxDim)
+      hipLaunchKernelGGL(k_lz_next, dim3(kLzBlocks + 1), t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, dtp, tol,
+                         b->phi, flags, b->basis, ld, wb[j % 2], wb[(j + 1) % 2], c->d_w, c->d_x, c->d_vext, kp, efp,
+                         c->nC, c->nG, n, metric, p0, halo);
+    else
+      hipLaunchKernelGGL(k_lz_last, dim3(1), t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, dtp, tol, b->phi,
+                         flags);
   }
-  hipLaunchKernelGGL(k_lz_finish, dim3(1), t, 0, st, b->ctl, b->alpha, b->beta, maxDim, dtp, tol, b->phi, flags);
   hipLaunchKernelGGL(k_lz_combine, blocks_for(n), t, 0, st, b->ctl, b->basis, ld, b->phi, b->nrm2, in, out, n);
 }
 
