@@ -335,6 +335,30 @@ def test_graph_replays_match_plain_launches(pair):
         s.set_laser([], 1.0)
 
 
+def test_sil_breakdown_and_zero_input(pair):
+    """The Lanczos loop's two early ends. A state with zero orbitals lies in the null space of the
+    linear split part (it acts on the orbitals only), so beta_0 = 0 (breakdown at the first
+    iteration) and the sub-step returns its input, as the oracle's does. A zero state ends the
+    loop before the first iteration and is returned as it is."""
+    name, w, s, ctx, state, wts = pair
+    if w.nG > 1000:
+        pytest.skip("small grids only")
+    metric = s.metric_weights()
+    c_only = state.copy()
+    c_only[w.nC:] = 0.0
+    ctx.upload_state(c_only)
+    ok = ctx.sil(0.0, 0.01, krylov_dim=16, tol=1e-12, max_restarts=8, metric=True)
+    so, oko = s.sil(c_only, 0.0, 0.01, krylov_dim=16, tol=1e-12, max_restarts=8, metric=metric)
+    got = ctx.download_state()
+    assert ok and oko
+    assert np.max(np.abs(got - c_only)) <= 1e-15
+    assert np.max(np.abs(got - so)) <= 1e-15
+    zero = np.zeros_like(state)
+    ctx.upload_state(zero)
+    assert ctx.sil(0.0, 0.01, krylov_dim=16, tol=1e-12, max_restarts=8, metric=True)
+    assert np.array_equal(ctx.download_state(), zero)
+
+
 def test_restart_files_round_trip(pair, tmp_path):
     """The device-resident state saved as coeffs.in / orb01_II.in and loaded back is bitwise
     the same; rdm2Bt01_01.in written from the GPU densities holds the oracle's rho2."""
