@@ -2,10 +2,11 @@
 // linear split part, classical RK4, and the Strang split step.
 //
 // The Lanczos loop of one SIL sub-step (src/IntegratorList/Sil/S_IntegratorList_Sil.f90:207-268)
-// runs without the host: every iteration is three launches whose first act is to test a device
-// flag `done`, which the iteration that meets the tolerance (or breaks down, or reaches the
-// maximal dimension) sets together with the Krylov dimension; the launches after it return at
-// once.  A sub-step is therefore a fixed launch sequence, captured once as a HIP graph and
+// runs without the host: every iteration is two launches (k_lz_pass1, then k_lz_next, which also
+// starts the next iteration) whose first act is to test a device flag `done`.  The end of an
+// iteration (lz_eval: the subspace exponential and the error estimate) sets it, together with the
+// Krylov dimension, when the tolerance is met, on breakdown or at the maximal dimension; the
+// launches after it return at once.  A sub-step is therefore a fixed launch sequence, captured once as a HIP graph and
 // replayed, and the host reads the convergence flags once per sub-step to apply the reference's
 // step halving (:155-170).  A Strang step (S_Propagator_SplitStep_Order2.f90:52-77) is one graph:
 // SIL(dt/2), RK4 on the nonlinear part, SIL(dt/2), each SIL as a single sub-step, after a backup
