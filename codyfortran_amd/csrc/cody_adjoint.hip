@@ -353,22 +353,33 @@ __global__ __launch_bounds__(256) void k_adj_local(const double2* __restrict__ o
       const int a = s ? l : k, b = s ? k : l;
       const double2 Wab = s ? cconj(Wp) : Wp;
       const double2 cW = cconj(Wab), hW = cscale(Wab, 0.5);
+      // with rr = conj(rho2'(k1,a,j1,b)) and hb = h2bar(k1,b,j1,a):
+      //   X     += sum_k1 conj(phi_k1) T1_k1 + 1/2 phi_k1 T2_k1,
+      //   g_k1  += cW T1_k1 + hW conj(T2_k1),   g_j1 += conj(hW) sum_k1 hb phi_k1,
+      // T1_k1 = sum_j1 rr dbar_j1, T2_k1 = sum_j1 hb conj(phi_j1)
       double2 X = cmk(0, 0);
+      double2 U[NO];
+#pragma unroll
+      for (int j1 = 0; j1 < NO; ++j1) U[j1] = cmk(0, 0);
 #pragma unroll
       for (int k1 = 0; k1 < NO; ++k1) {
-        double2 gk = cmk(0, 0);
+        double2 T1 = cmk(0, 0), T2 = cmk(0, 0);
 #pragma unroll
         for (int j1 = 0; j1 < NO; ++j1) {
           const double2 rr = cconj(rho2p[idx4(k1, a, j1, b, NO)]);
           const double2 hb = h2b[idx4(k1, b, j1, a, NO)];
-          cfma(X, cmul(rr, cconj(phi[k1])), db[j1]);
-          cfma(X, cscale(cmul(hb, phi[k1]), 0.5), cconj(phi[j1]));
-          cfma(gk, cmul(cW, rr), db[j1]);
-          cfma(gk, cmul(hW, cconj(hb)), phi[j1]);
-          cfma(g[j1], cmul(hb, cconj(hW)), phi[k1]);
+          cfma(T1, rr, db[j1]);
+          cfma(T2, hb, cconj(phi[j1]));
+          cfma(U[j1], hb, phi[k1]);
         }
-        g[k1] = cadd(g[k1], gk);
+        cfma(X, cconj(phi[k1]), T1);
+        cfma(X, cscale(phi[k1], 0.5), T2);
+        cfma(g[k1], cW, T1);
+        cfma(g[k1], hW, cconj(T2));
       }
+      const double2 chW = cconj(hW);
+#pragma unroll
+      for (int j1 = 0; j1 < NO; ++j1) cfma(g[j1], chW, U[j1]);
       fold = cadd(fold, s ? cconj(X) : X);
     }
     const double wx = in ? w[xc] : 0.0;
