@@ -560,6 +560,9 @@ __global__ __launch_bounds__(256) void k_adj_rdm(const double2* __restrict__ rho
 // k_adj_local (gpart, ngroups of them), g_k += sbar_p phi_l and
 // g_l += conj(sbar_p) phi_k for the stored pairs (WTadj = V (w Wbar), may be null), g_j += h hbar_j,
 // and per-workgroup partials of Ebar = sum_j <hbar_j, x phi_j>_w
+// One thread per grid point and orbital j = blockIdx.y: the pair terms that land on g_j are taken in
+// the stored pair order, so every g_j receives its terms in a fixed order; the Ebar partials are
+// per (orbital, workgroup), epart[blockIdx.y gridDim.x + blockIdx.x].
 template <int NO>
 __global__ __launch_bounds__(256) void k_adj_final(const double2* __restrict__ orbs, const double* __restrict__ w,
                                                    const double* __restrict__ xg, const double* __restrict__ vext,
@@ -568,43 +571,38 @@ __global__ __launch_bounds__(256) void k_adj_final(const double2* __restrict__ o
                                                    const double2* __restrict__ hbar, double2* __restrict__ gO,
                                                    double* __restrict__ epart) {
   __shared__ double red[4];
-  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
   double e = 0.0;
   if (x < nG) {
-    double2 phi[NO], g[NO];
-#pragma unroll
-    for (int k = 0; k < NO; ++k) {
-      phi[k] = orbs[(size_t)k * nG + x];
-      g[k] = gO[(size_t)k * nG + x];
-    }
-    for (int q = 0; q < ngroups; ++q)
-#pragma unroll
-      for (int k = 0; k < NO; ++k) g[k] = cadd(g[k], gpart[((size_t)q * NO + k) * nG + x]);
+    double2 g = gO[(size_t)j * nG + x];
+#pragma unroll 4
+    for (int q = 0; q < ngroups; ++q) g = cadd(g, gpart[((size_t)q * NO + j) * nG + x]);
     if (WTadj) {
+      double2 phi[NO];
+#pragma unroll
+      for (int k = 0; k < NO; ++k) phi[k] = orbs[(size_t)k * nG + x];
       int p = 0;
 #pragma unroll
       for (int l = 0; l < NO; ++l)
 #pragma unroll
         for (int k = l; k < NO; ++k, ++p) {
+          if (k != j && l != j) continue;  // uniform over the workgroup
           const double2 sb = cmk(WTadj[(size_t)(2 * p) * nGp + x], WTadj[(size_t)(2 * p + 1) * nGp + x]);
-          cfma(g[k], sb, phi[l]);
-          cfma(g[l], cconj(sb), phi[k]);
+          if (k == j) cfma(g, sb, phi[l]);
+          if (l == j) cfma(g, cconj(sb), phi[k]);
         }
     }
     const double wx = w[x], v = vext[x] + E * xg[x];
-#pragma unroll
-    for (int j = 0; j < NO; ++j) {
-      const double2 hb = hbar[(size_t)j * nG + x];
-      double2 hh = kinetic_at(hbar + (size_t)j * nG, x, kp, 1.0 / wx);
-      hh.x = fma(v, hb.x, hh.x);
-      hh.y = fma(v, hb.y, hh.y);
-      gO[(size_t)j * nG + x] = cadd(g[j], hh);
-      e += hb.x * phi[j].x + hb.y * phi[j].y;
-    }
-    e *= wx * xg[x];
+    const double2 hb = hbar[(size_t)j * nG + x];
+    double2 hh = kinetic_at(hbar + (size_t)j * nG, x, kp, 1.0 / wx);
+    hh.x = fma(v, hb.x, hh.x);
+    hh.y = fma(v, hb.y, hh.y);
+    gO[(size_t)j * nG + x] = cadd(g, hh);
+    const double2 pj = orbs[(size_t)j * nG + x];
+    e = (hb.x * pj.x + hb.y * pj.y) * (wx * xg[x]);
   }
   const double s = block_sum(e, red);
-  if (threadIdx.x == 0) epart[blockIdx.x] = s;
+  if (threadIdx.x == 0) epart[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
 // out = sum of v[0..n) in a fixed order (one workgroup)
@@ -962,7 +960,7 @@ int alloc_vjp(cody_ctx* c) {
   AL(a.X, sizeof(double2) * n2);
   AL(a.STadj, sizeof(double) * ns);  // pad rows (c >= 2P) stay zero
   AL(a.WTadj, sizeof(double) * ns);
-  AL(a.epart, sizeof(double) * nb);
+  AL(a.epart, sizeof(double) * nb * kMaxOrbs);  // one row per orbital
   AL(a.eacc, sizeof(double) * nOG);
   AL(a.gpart, sizeof(double2) * nOG * ((c->P + 3) / 4));
   AL(a.J, sizeof(double) * 2);
@@ -1004,11 +1002,11 @@ int enqueue_vjp(cody_ctx* c, int part, const double2* state, const double2* lam,
     hipLaunchKernelGGL(k_adj_hbar_lin, dim3(blocks((size_t)nO * nG)), dim3(256), 0, st, lam + nC, nO * nG, a.hbar);
     launch_zero(gO, nO * nG, st);
 #define CALL_F0(NO)                                                                                              \
-  hipLaunchKernelGGL(k_adj_final<NO>, dim3(nb), dim3(256), 0, st, orbs, c->d_w, c->d_x, c->d_vext, kp, E, nullptr, \
-                     nG, nGp, nullptr, 0, a.hbar, gO, a.epart)
+  hipLaunchKernelGGL(k_adj_final<NO>, dim3(nb, NO), dim3(256), 0, st, orbs, c->d_w, c->d_x, c->d_vext, kp, E,    \
+                     nullptr, nG, nGp, nullptr, 0, a.hbar, gO, a.epart)
     ADJ_NO_SWITCH(nO, CALL_F0)
 #undef CALL_F0
-    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, st, a.epart, (int)nb, ebar);
+    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, st, a.epart, (int)(nb * nO), ebar);
     return CODY_OK;
   }
   int mode = part == CODY_PART_FULL ? kModeFull : kModeNonLin;
@@ -1046,11 +1044,11 @@ int enqueue_vjp(cody_ctx* c, int part, const double2* state, const double2* lam,
   // sbar = V (w Wbar) on the FP64 MFMA contraction of the forward pass
   launch_mf_f64_w(c->d_Vt, a.STadj, c->d_Wpart, a.WTadj, nGp, c->Pc, c->wsplit, st);
 #define CALL_F(NO)                                                                                               \
-  hipLaunchKernelGGL(k_adj_final<NO>, dim3(nb), dim3(256), 0, st, orbs, c->d_w, c->d_x, c->d_vext, kp, E, a.WTadj, \
-                     nG, nGp, a.gpart, (NO * (NO + 1) / 2 + 3) / 4, a.hbar, gO, a.epart)
+  hipLaunchKernelGGL(k_adj_final<NO>, dim3(nb, NO), dim3(256), 0, st, orbs, c->d_w, c->d_x, c->d_vext, kp, E,    \
+                     a.WTadj, nG, nGp, a.gpart, (NO * (NO + 1) / 2 + 3) / 4, a.hbar, gO, a.epart)
   ADJ_NO_SWITCH(nO, CALL_F)
 #undef CALL_F
-  hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, st, a.epart, (int)nb, ebar);
+  hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, st, a.epart, (int)(nb * nO), ebar);
   return CODY_OK;
 }
 
