@@ -1042,7 +1042,7 @@ int enqueue_vjp(cody_ctx* c, int part, const double2* state, const double2* lam,
   hipLaunchKernelGGL(k_adj_ci_apply_lin, dim3((nC + 3) / 4), dim3(256), 0, st, C, nC, nO, c->dci.s_ptr,
                      c->dci.singles, c->dci.d_ptr, c->dci.doubles, a.lin1, a.lin2, grad);
   // sbar = V (w Wbar) on the FP64 MFMA contraction of the forward pass
-  launch_mf_f64_w(c->d_Vt, a.STadj, c->d_Wpart, a.WTadj, nGp, c->Pc, c->wsplit, st);
+  if (int r = launch_mf_f64_w(c->d_Vt, a.STadj, c->d_Wpart, a.WTadj, nGp, c->Pc, c->wsplit, st)) return r;
 #define CALL_F(NO)                                                                                               \
   hipLaunchKernelGGL(k_adj_final<NO>, dim3(nb, NO), dim3(256), 0, st, orbs, c->d_w, c->d_x, c->d_vext, kp, E,    \
                      a.WTadj, nG, nGp, a.gpart, (NO * (NO + 1) / 2 + 3) / 4, a.hbar, gO, a.epart)

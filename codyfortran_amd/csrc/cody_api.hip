@@ -103,39 +103,42 @@ bool int8_tiled(const cody_ctx* c) { return c->precision == CODY_PREC_INT8_SLICE
 double* plain_st(cody_ctx* c) { return (c->precision == CODY_PREC_FP64 || int8_tiled(c)) ? nullptr : c->d_ST; }
 unsigned long long* st_max(cody_ctx* c) { return int8_tiled(c) ? c->d_smax : nullptr; }
 
-// Enqueue W = V S for the context's precision (ST already formed); returns the kernel name.
-const char* enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark) {
+// Enqueue W = V S for the context's precision (ST already formed).  Returns 0, or the launcher's
+// error (nothing of the contraction launched) when the shape is outside what the kernels take.
+int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark) {
   const size_t ns = (size_t)c->Pc * c->nGp;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
       if (int8_tiled(c)) {  // 64-row kernel: S slices fragment-tiled, four k ranges
+        if (int r = check_contraction_shape("k_mf_i8_w", c->nGp, c->Pc, kInt8Split, 5, c->nGp / 64)) return r;
         launch_slice_s_tiled(c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
         mark("k_slice_s_tiled");
-        launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->d_WT, c->nGp, c->Pc, 4, c->d_smax, st);
+        CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->d_WT, c->nGp, c->Pc, kInt8Split,
+                           c->d_smax, st));
         mark("k_mf_i8_w + k_wsum");
-        return "k_mf_i8_w";
+        return CODY_OK;
       }
       launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, 0, c->d_Si8, c->d_eS, st);
       mark("k_slice_rows");
       launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
       mark("k_mf_i8");
-      return "k_mf_i8";
+      return CODY_OK;
     case CODY_PREC_SPLIT_FP32:
       launch_split_f32(c->d_ST, ns, c->d_Shi, c->d_Slo, st);
       mark("k_split_f32");
       launch_mf_f32x3(c->d_Vhi, c->d_Vlo, c->d_Shi, c->d_Slo, c->d_WT, c->nGp, c->Pc, st);
       mark("k_mf_f32x3");
-      return "k_mf_f32x3";
+      return CODY_OK;
     case CODY_PREC_BF16X3:
       launch_split_bf16x3(c->d_ST, ns, c->d_Sbf, st);
       mark("k_split_bf16x3");
       launch_mf_bf16x3(c->d_Vbf, c->d_Sbf, c->d_WT, c->nGp, c->Pc, st);
       mark("k_mf_bf16x3");
-      return "k_mf_bf16x3";
+      return CODY_OK;
     default:
-      launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st);
+      CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st));
       mark("k_mf_f64_w + k_wsum");
-      return "k_mf_f64_w";
+      return CODY_OK;
   }
 }
 
@@ -444,7 +447,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
   mark("k_pair_src");
   if (split) hipEventRecord(c->ev_fork, st);
-  enqueue_contraction(c, st, mark);
+  CHK(enqueue_contraction(c, st, mark));
   if (split) {
     hipEventRecord(c->ev_w, st);
     hipStreamWaitEvent(sa, c->ev_fork, 0);
@@ -528,9 +531,9 @@ int cody_autotune(cody_ctx* c, int* wsplit) {
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
   HIPCHK(hipStreamSynchronize(st));
+  const int kBlocksF64 = c->nGp / 16 / (c->Pc / 16 <= 5 ? 2 : 1);  // k blocks the FP64 kernel splits
   if (wsplit && *wsplit > 0) {  // a given split instead of the timed choice
-    if (*wsplit != 1 && *wsplit != 2 && *wsplit != 4 && *wsplit != 8)
-      return fail(CODY_ERR_ARG, "cody_autotune: the k split must be 1, 2, 4 or 8");
+    if (check_contraction_shape("cody_autotune", c->nGp, c->Pc, *wsplit, 10, kBlocksF64)) return CODY_ERR_ARG;
     c->wsplit = *wsplit;
     c->drop_graphs();
     return CODY_OK;
@@ -543,9 +546,11 @@ int cody_autotune(cody_ctx* c, int* wsplit) {
     float best = 1e30f;
     int pick = c->wsplit;
     for (int ks : {1, 2, 4, 8}) {
-      launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st);  // warm-up
+      // (splits the shape does not take are skipped; the check launches nothing)
+      if (check_contraction_shape("k_mf_f64_w", c->nGp, c->Pc, ks, 10, kBlocksF64)) continue;
+      CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st));  // warm-up
       HIPCHK(hipEventRecord(e0, st));
-      for (int i = 0; i < 8; ++i) launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st);
+      for (int i = 0; i < 8; ++i) CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st));
       HIPCHK(hipEventRecord(e1, st));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -665,7 +670,7 @@ int cody_meanfield_w(cody_ctx* c, const double* orbs, double* W) {
   double2* d_orbs = c->d_tmp + c->nC;
   HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
   launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->stream);
-  enqueue_contraction(c, c->stream, [](const char*) {});
+  CHK(enqueue_contraction(c, c->stream, [](const char*) {}));
   HIPCHK(hipGetLastError());
   std::vector<double> wt((size_t)c->Pc * c->nGp);
   HIPCHK(hipMemcpyAsync(wt.data(), c->d_WT, sizeof(double) * wt.size(), hipMemcpyDeviceToHost, c->stream));
@@ -730,7 +735,7 @@ int cody_interaction_potential(cody_ctx* c, int nSrc, const double* src, double*
     HIPCHK(hipMemcpyAsync(c->d_src, src + 2 * (size_t)s0 * c->nG, sizeof(double2) * c->nG * ns,
                           hipMemcpyHostToDevice, st));
     launch_tile_src(c->d_src, ns, c->nG, c->nGp, c->Pc, c->d_STt, st);
-    launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st);
+    CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st));
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(wt.data(), c->d_WT, sizeof(double) * wt.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -770,7 +775,7 @@ int cody_apply_correlation(cody_ctx* c, const double* orbs, const double* rdm1, 
   HIPCHK(hipMemcpyAsync(c->d_r2can, can.data(), sizeof(double) * can.size(), hipMemcpyHostToDevice, st));
   launch_reginv(c->d_rho1, nO, c->reg, c->d_reginv, st);
   launch_pair_src(d_orbs, c->d_w, nG, c->nGp, nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
-  enqueue_contraction(c, st, [](const char*) {});
+  CHK(enqueue_contraction(c, st, [](const char*) {}));
   launch_orbital(d_orbs, c->d_w, c->d_hphi, c->d_WT, c->d_r2can, c->d_reginv, nG, c->nGp, nO, kModeNonLin,
                  c->d_draw, c->d_part, st);
   if (h2) {
@@ -978,7 +983,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   hipStream_t st = c->stream;
   launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
   auto none = [](const char*) {};
-  enqueue_contraction(c, st, none);  // warm-up
+  CHK(enqueue_contraction(c, st, none));  // warm-up
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -987,7 +992,8 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
     switch (c->precision) {
       case CODY_PREC_INT8_SLICED:  // the contraction kernel alone
         if (int8_tiled(c))
-          launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, nullptr, c->nGp, c->Pc, 4, nullptr, st);
+          CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, nullptr, c->nGp, c->Pc, kInt8Split,
+                             nullptr, st));
         else
           launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
         break;
@@ -998,7 +1004,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
         launch_mf_bf16x3(c->d_Vbf, c->d_Sbf, c->d_WT, c->nGp, c->Pc, st);
         break;
       default:  // the contraction kernel alone (its partial sum k_wsum is not timed)
-        launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, nullptr, c->nGp, c->Pc, c->wsplit, st);
+        CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, nullptr, c->nGp, c->Pc, c->wsplit, st));
     }
   }
   HIPCHK(hipEventRecord(e1, st));

@@ -483,8 +483,11 @@ __device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m
     flags[0] = err;
     flags[1] = conv ? 1.0 : 0.0;
     if (conv || breakdown || m == maxDim) {
-      ctl[1] = m;
-      ctl[0] = 1;
+      // The other workgroups of the same k_lz_next launch read ctl[0] at their entry: relaxed
+      // atomic accesses, so a reader sees either value.  Both are correct: when this workgroup
+      // ends the loop, that launch's vector results (v_{j+1}, w', their sums) are never used.
+      __hip_atomic_store(&ctl[1], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -526,7 +529,8 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
   __shared__ double2 sC[kPS], sTmp[4][kPS];
   __shared__ double sr[4][kPS], si[4][kPS];
   __shared__ double sBeta;
-  if (ctl[0]) return;
+  // (workgroup kLzBlocks of this launch may set ctl[0]: relaxed atomic load, see lz_eval)
+  if (__hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const int nv = j + 1;
   const double bj = lz_beta(pin, nv, sC, sTmp, &sBeta);
   if (blockIdx.x == kLzBlocks) {

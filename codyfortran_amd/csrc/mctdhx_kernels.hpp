@@ -7,6 +7,8 @@
 
 namespace cody {
 
+int set_error(int code, const char* msg);  // records the library's last error, returns code
+
 constexpr int kMaxOrbs = 12;   // orbitals per body type handled by the kernels
 constexpr int kMaxPc = 160;    // 16*ceil(M(M+1)/16) for M <= 12
 constexpr int kInt8Slices = 5; // balanced base-254 digits per operand of the int8-sliced contraction
@@ -90,13 +92,16 @@ void launch_slice_rows(const double* X, int len, int nlines, int ns, int tiled, 
 void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream_t st);
 void launch_split_bf16x3(const double* X, size_t n, __bf16* out, hipStream_t st);
 constexpr int kMaxWSplit = 8;  // largest k split of the FP64 contraction (partials buffer depth)
-void launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
-                     int nsplit, hipStream_t st);
+constexpr int kInt8Split = 4;  // k split of the 64-row int8 contraction
+// shape check of the k-split contraction launchers (0 or CODY_ERR_ARG, nothing launched)
+int check_contraction_shape(const char* who, int nGp, int Pc, int nsplit, int maxTiles, int kBlocks);
+int launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
+                    int nsplit, hipStream_t st);
 void launch_tile_f64(const double* V, int nGp, double* Vt, hipStream_t st);
 void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp, int Pc, int nsplit,
                      hipStream_t st);
-void launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
-                    double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st);
+int launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
+                   double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st);
 void launch_slice_s_tiled(const double* STt, const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out,
                           int* expo, hipStream_t st);
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,

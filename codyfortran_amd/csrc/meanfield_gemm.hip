@@ -16,6 +16,8 @@
 // 4i + l/16; 32-bit results (f32/bf16/i8): row 4(l/16) + i; column l%16 in both.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+
 #include "mctdhx_kernels.hpp"
 
 namespace cody {
@@ -609,21 +611,52 @@ __global__ __launch_bounds__(256) void k_mf_bf16x3(const __bf16* __restrict__ A,
     default: break;               \
   }
 
-// nGp must be a multiple of 256 and nsplit one of 1, 2, 4, 8; Wpart holds nsplit x Pc x nGp doubles
-// (unused for nsplit = 1, when the kernel writes WT).  With WT null only the partials are formed
-// (k_wsum is not launched).
-void launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
-                     int nsplit, hipStream_t st) {
+// Shape check shared by the k-split contraction launchers, before anything is launched: nGp a
+// positive multiple of 256, Pc a multiple of 16 with Pc / 16 in 1..maxTiles (the kernels'
+// instantiations), nsplit one of 1, 2, 4, 8 (the partials buffer holds kMaxWSplit = 8 planes) and a
+// divisor of the kernel's k blocks.  Returns 0 or CODY_ERR_ARG (1) with the reason in the library's
+// last error.
+int check_contraction_shape(const char* who, int nGp, int Pc, int nsplit, int maxTiles, int kBlocks) {
+  char msg[192];
+  if (nGp <= 0 || nGp % 256 != 0) {
+    std::snprintf(msg, sizeof msg, "%s: padded grid %d is not a positive multiple of 256", who, nGp);
+    return set_error(1, msg);
+  }
+  if (Pc <= 0 || Pc % 16 != 0 || Pc / 16 > maxTiles) {
+    std::snprintf(msg, sizeof msg, "%s: %d pair columns are outside 16..%d in steps of 16", who, Pc, 16 * maxTiles);
+    return set_error(1, msg);
+  }
+  if (nsplit != 1 && nsplit != 2 && nsplit != 4 && nsplit != 8) {
+    std::snprintf(msg, sizeof msg, "%s: k split %d is not one of 1, 2, 4, 8", who, nsplit);
+    return set_error(1, msg);
+  }
+  static_assert(kMaxWSplit == 8, "the split check above allows at most kMaxWSplit planes");
+  if (kBlocks % nsplit != 0) {
+    std::snprintf(msg, sizeof msg, "%s: k split %d does not divide the %d k blocks", who, nsplit, kBlocks);
+    return set_error(1, msg);
+  }
+  return 0;
+}
+
+// Wpart holds nsplit x Pc x nGp doubles (unused for nsplit = 1, when the kernel writes WT).  With
+// WT null only the partials are formed (k_wsum is not launched).  Returns 0, or CODY_ERR_ARG
+// without launching anything when the shape is outside check_contraction_shape.
+int launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
+                    int nsplit, hipStream_t st) {
+  // per row group the kernel splits the nGp / 16 k blocks into nsplit ranges of KH sub-ranges
+  const int kh = Pc / 16 <= 5 ? 2 : 1;
+  if (int r = check_contraction_shape("k_mf_f64_w", nGp, Pc, nsplit, 10, nGp / 16 / kh)) return r;
   double* out = (nsplit == 1 && WT) ? WT : Wpart;
 #define CALL_W(NT)                                                                                    \
   hipLaunchKernelGGL(k_mf_f64_w<NT>, dim3(nGp / 64, nsplit), dim3((NT) <= 5 ? 512 : 256), 0, st, Vt, STt, \
                      nGp, out)
   CODY_NT_SWITCH2(Pc, CALL_W)
 #undef CALL_W
-  if (!WT || nsplit == 1) return;
+  if (!WT || nsplit == 1) return 0;
   const size_t plane = (size_t)Pc * nGp;
   hipLaunchKernelGGL(k_wsum, dim3((unsigned)((plane / 2 + 255) / 256)), dim3(256), 0, st, Wpart, nsplit, plane,
                      WT);
+  return 0;
 }
 
 // Gpart[s][c2][c1] = sum_{x in split s} ST[c1][x] WT[c2][x]; nsplit * 64 must divide nGp.
@@ -638,9 +671,11 @@ void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp
 }
 
 // 64-row int8 contraction (Pc <= 80): Bs fragment-tiled; nsplit in {1, 2, 4, 8} dividing nGp / 64
-// (WT is written directly for nsplit = 1).  With WT null only the partials are formed.
-void launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
-                    double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st) {
+// (WT is written directly for nsplit = 1).  With WT null only the partials are formed.  Returns 0,
+// or CODY_ERR_ARG without launching anything when the shape is outside check_contraction_shape.
+int launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
+                   double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st) {
+  if (int r = check_contraction_shape("k_mf_i8_w", nGp, Pc, nsplit, 5, nGp / 64)) return r;
   double* out = (nsplit == 1 && WT) ? WT : Wpart;
 #define CALL_I8(NT) \
   hipLaunchKernelGGL(k_mf_i8_w<NT>, dim3(nGp / 64, nsplit), dim3(256), 0, st, As, eA, Bs, fB, nGp, out, smax)
@@ -650,13 +685,14 @@ void launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int
     case 3: CALL_I8(3); break;
     case 4: CALL_I8(4); break;
     case 5: CALL_I8(5); break;
-    default: break;
+    default: break;  // (excluded by the shape check)
   }
 #undef CALL_I8
-  if (!WT || nsplit == 1) return;
+  if (!WT || nsplit == 1) return 0;
   const size_t plane = (size_t)Pc * nGp;
   hipLaunchKernelGGL(k_wsum, dim3((unsigned)((plane / 2 + 255) / 256)), dim3(256), 0, st, Wpart, nsplit, plane,
                      WT);
+  return 0;
 }
 
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,

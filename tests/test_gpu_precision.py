@@ -1,8 +1,17 @@
-"""Reduced-precision forms of the mean-field contraction against the FP64 oracle.
+"""Reduced-precision forms of the mean-field contraction against the FP64 oracle (4096-point
+grid, M = 8, N = 4, seeded state).
 
-Bars: the int8-sliced form keeps W within 1e-10 and the right-hand side within 1e-9 relative
-(normwise, per quantity).  Split-FP32 and 3xBF16 are measured and their errors recorded;
-their bounds below are the accuracy those schemes can reach with FP32 accumulation.
+Every form is measured per quantity: the mean fields W, the full right-hand side and the energy,
+each relative to the FP64 oracle.
+
+- int8-sliced (five balanced base-254 digits, exact int8-MFMA products, FP64 recombination)
+  meets the 1e-9 bar on all three quantities; W within 1e-10.
+- Split-FP32 and 3xBF16 accumulate their products in FP32 inside the MFMA, which rounds every
+  partial sum at 2^-24 ~ 6e-8; their W errors sit there (measured 6.6e-8 and 5.7e-8).  They are
+  kept as named regression bounds at those errors, and their 1e-9 test is skipped with that
+  reason: a split-FP32 form that met 1e-9 would need exact products and chunk sums (pieces of
+  at most 10 bits, six products per element) and could not beat the FP64 MFMA form, whose peak is
+  half the FP32 one.
 """
 import numpy as np
 import pytest
@@ -13,10 +22,16 @@ from tests.common import oracle_system, rel, state_rel
 
 pytestmark = pytest.mark.gpu
 
+BAR = 1e-9  # every quantity of the forms the benchmark may report
+# (W, RHS, energy) bounds per form
+INT8_BOUNDS = (1e-10, BAR, BAR)
+# regression bounds of the FP32-accumulating forms, at their measured W errors (6.6e-8, 5.7e-8)
+SPLIT_FP32_REGRESSION = (1e-7, BAR, BAR)
+BF16X3_REGRESSION = (1e-7, BAR, BAR)
 MODES = {
-    "int8_sliced": (hip.PREC_INT8_SLICED, 1e-10, 1e-9),
-    "split_fp32": (hip.PREC_SPLIT_FP32, 1e-6, 1e-5),
-    "bf16x3": (hip.PREC_BF16X3, 1e-5, 1e-4),
+    "int8_sliced": (hip.PREC_INT8_SLICED, INT8_BOUNDS),
+    "split_fp32": (hip.PREC_SPLIT_FP32, SPLIT_FP32_REGRESSION),
+    "bf16x3": (hip.PREC_BF16X3, BF16X3_REGRESSION),
 }
 
 
@@ -29,20 +44,39 @@ def full_size():
     C, orbs = wl.split_state(state, w.nC, w.nG, w.M)
     ref_w = s.meanfield_w(orbs)
     ref_d = s.time_derivative(state)
-    yield w, ctx, state, orbs, ref_w, ref_d, wts
+    ref_e = s.energy(state)
+    yield w, ctx, state, orbs, ref_w, ref_d, ref_e, wts
     ctx.close()
 
 
-@pytest.mark.parametrize("mode", list(MODES))
-def test_contraction_and_rhs_error(full_size, mode):
-    w, ctx, state, orbs, ref_w, ref_d, wts = full_size
-    code, wtol, rtol = MODES[mode]
+def _errors(full_size, code):
+    w, ctx, state, orbs, ref_w, ref_d, ref_e, wts = full_size
     ctx.set_precision(code)
     try:
-        ew = rel(ctx.meanfield_w(orbs), ref_w)
+        Wg = ctx.meanfield_w(orbs)
+        ew = rel(Wg, ref_w)
+        pair_err = max(rel(Wg[:, p], ref_w[:, p]) for p in range(ref_w.shape[1]))
         ed = state_rel(ctx.time_derivative(state), ref_d, w.nC, wts, w.M)
-        pair_err = max(rel(ctx.meanfield_w(orbs)[:, p], ref_w[:, p]) for p in range(ref_w.shape[1]))
-        print(f"\n{mode}: W rel err {ew:.3e} (worst pair {pair_err:.3e}), RHS rel err {ed:.3e}")
-        assert ew < wtol and ed < rtol
+        ee = abs(ctx.energy(state) - ref_e) / abs(ref_e)
     finally:
         ctx.set_precision(hip.PREC_FP64)
+    return ew, pair_err, ed, ee
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_contraction_rhs_and_energy_error(full_size, mode):
+    code, (wb, rb, eb) = MODES[mode]
+    ew, pair_err, ed, ee = _errors(full_size, code)
+    print(f"\n{mode}: W rel err {ew:.3e} (worst pair {pair_err:.3e}), RHS rel err {ed:.3e}, energy rel err {ee:.3e}")
+    assert ew < wb, ew
+    assert ed < rb, ed
+    assert ee < eb, ee
+
+
+@pytest.mark.skip(reason="split-FP32 and 3xBF16 accumulate in FP32 inside the MFMA (rounding 2^-24 ~ 6e-8 per "
+                         "partial sum), so W stays near 6e-8; the int8-sliced form is the one that meets 1e-9")
+@pytest.mark.parametrize("mode", ["split_fp32", "bf16x3"])
+def test_fp32_accumulating_forms_meet_the_bar(full_size, mode):
+    code, _ = MODES[mode]
+    ew, _, ed, ee = _errors(full_size, code)
+    assert max(ew, ed, ee) < BAR

@@ -1,0 +1,154 @@
+"""Direct pins of the oracle's two integrators on the linear split part, where the exact flow is
+known in closed form.
+
+The linear part (TimeDerivativeOrbsLin, S_Method_Mb_OrbBased_Mctdhx.f90:152-177) is
+d phi_k/dt = -i h phi_k on every orbital with h = T + V + E(t) x, and dC/dt = 0.  Its dense
+matrix h is assembled column by column from the oracle's one-body apply, so every check below
+compares an integrator with an independent library routine on the same operator:
+
+- orc_sil (S_IntegratorList_Sil.f90:114-312) against scipy.linalg.expm(-i h dt), with the FEDVR
+  weights as the metric (h is self-adjoint there) and, on a uniform finite-difference grid where
+  h is symmetric, without a metric;
+- orc_rk4 (S_IntegratorList_Rk_O4Expl.f90:90-141): on a time-independent linear problem one
+  classical RK4 step is exactly the degree-4 Taylor polynomial of exp(-i h dt) (any wrong stage
+  weight or stage coefficient changes a power of dt); under a time-dependent field its global
+  error falls 16-fold per halving of dt against an independent high-order solution (a wrong
+  stage time drops the order).
+"""
+import numpy as np
+import pytest
+import scipy.integrate
+import scipy.linalg
+
+import oracle
+from codyfortran_amd import workloads as wl
+from tests.common import oracle_system, small, state_rel
+
+
+def dense_h(s, t=0.0):
+    """h (nG x nG) of the oracle's one-body apply at time t, column by column."""
+    nG, nO = s.nG, s.nO
+    H = np.zeros((nG, nG), dtype=np.complex128)
+    for j0 in range(0, nG, nO):
+        cols = np.zeros((nG, nO), dtype=np.complex128, order="F")
+        js = list(range(j0, min(j0 + nO, nG)))
+        for k, j in enumerate(js):
+            cols[j, k] = 1.0
+        d, _ = s.apply_single_body(cols, t)
+        H[:, js] = d[:, : len(js)]
+    return H
+
+
+def exact_linear(state, nC, nG, M, U):
+    """The linear part's exact flow: C unchanged, every orbital multiplied by U."""
+    C, orbs = state[:nC], state[nC:].reshape(M, nG).T
+    return np.concatenate([C, (U @ orbs).T.ravel()])
+
+
+@pytest.fixture(scope="module")
+def fedvr401():
+    w = wl.WORKLOADS["mctdhf_1d_401"]
+    s, x, wts = oracle_system(w)
+    st = wl.seeded_state(x, wts, w.N, w.M, seed=11)
+    H = dense_h(s)
+    return w, s, wts, st, H
+
+
+def test_dense_h_is_self_adjoint_in_the_weights(fedvr401):
+    w, s, wts, st, H = fedvr401
+    WH = wts[:, None] * H
+    assert np.max(np.abs(WH - WH.conj().T)) < 1e-12 * np.max(np.abs(WH))
+
+
+@pytest.mark.parametrize("krylov_dim,tol,bar", [(32, 1e-13, 1e-12), (12, 1e-10, 1e-10)])
+def test_sil_matches_expm_with_metric(fedvr401, krylov_dim, tol, bar):
+    """FEDVR 401-point grid (100 elements x 5 points), weighted metric; dt = 0.05 (the configs[0]
+    step). The second case is configs[0]'s own Krylov dimension and tolerance."""
+    w, s, wts, st, H = fedvr401
+    dt = 0.05
+    U = scipy.linalg.expm(-1j * dt * H)
+    ref = exact_linear(st, w.nC, w.nG, w.M, U)
+    got, ok = s.sil(st, 0.0, dt, krylov_dim=krylov_dim, tol=tol, max_restarts=8, metric=s.metric_weights())
+    assert ok
+    err = state_rel(got, ref, w.nC, wts, w.M)
+    assert err < bar, err
+
+
+def test_sil_matches_expm_without_metric():
+    """Uniform finite-difference grid with equal weights: h is symmetric, the Euclidean Lanczos
+    process is exact, and orc_sil without a metric must reproduce expm."""
+    nG, dx = 201, 0.1
+    x = (np.arange(nG) - nG // 2) * dx
+    wts = np.full(nG, dx)
+    s = oracle.System(x, wts, [2], [4], reg=1e-8)
+    s.set_kinetic_fd(dx)
+    s.set_potential(-2.0 / np.sqrt(x ** 2 + 1.0))
+    s.set_interaction_softyukawa(1.0, 1.0, 0.0, 0.0)
+    H = dense_h(s)
+    assert np.max(np.abs(H - H.conj().T)) < 1e-13 * np.max(np.abs(H))
+    st = wl.seeded_state(x, wts, 2, 4, seed=5)
+    nC = s.nC
+    dt = 0.02
+    ref = exact_linear(st, nC, nG, 4, scipy.linalg.expm(-1j * dt * H))
+    got, ok = s.sil(st, 0.0, dt, krylov_dim=32, tol=1e-13, max_restarts=8, metric=None)
+    assert ok
+    assert state_rel(got, ref, nC, wts, 4) < 1e-12
+
+
+def test_rk4_step_is_the_degree_four_taylor_polynomial(fedvr401):
+    w, s, wts, st, H = fedvr401
+    lam = np.max(np.abs(np.linalg.eigvals(H)))
+    dt = 1.0 / lam  # |dt h| = 1: every power of dt h contributes
+    A = -1j * dt * H
+    P = np.eye(w.nG, dtype=np.complex128)
+    term = np.eye(w.nG, dtype=np.complex128)
+    for k in range(1, 5):
+        term = term @ A / k
+        P = P + term
+    ref = exact_linear(st, w.nC, w.nG, w.M, P)
+    got = s.rk4(st, 0.0, dt, rhs=oracle.RHS_LIN)
+    assert state_rel(got, ref, w.nC, wts, w.M) < 1e-13
+    # the bar separates the tableau from its plausible slips: a doubled A^4 coefficient (1/12
+    # instead of 1/24) is far outside it
+    P_bad = P + term
+    assert state_rel(exact_linear(st, w.nC, w.nG, w.M, P_bad), ref, w.nC, wts, w.M) > 1e-3
+
+
+def test_rk4_global_error_is_fourth_order_under_a_field():
+    """Time-dependent linear problem: h(t) = T + V + E(t) x with a two-term pulse on a 41-point
+    FEDVR grid; RK4 with n, 2n, 4n steps over [0, 0.4] against DOP853 at rtol 1e-13."""
+    w = small("tiny41", -8.0, 8.0, 10, 5, 2, 4)
+    s, x, wts = oracle_system(w)
+    T = 0.4
+    s.set_laser([3.0, -2.0], T)
+    st = wl.seeded_state(x, wts, w.N, w.M, seed=2)
+    H0 = dense_h(s, -1.0)  # outside [0, T] the field is zero
+    X = np.diag(x).astype(np.complex128)
+
+    def field(t):
+        env = np.sin(np.pi * t / T) ** 2
+        return (3.0 * np.sin(np.pi * t / T) - 2.0 * np.sin(2 * np.pi * t / T)) * env
+
+    # the dense operator at a time inside the pulse equals H0 + E(t) x
+    t_probe = 0.13
+    assert np.max(np.abs(dense_h(s, t_probe) - (H0 + field(t_probe) * X))) < 1e-12 * np.max(np.abs(H0))
+    orbs0 = st[w.nC:].reshape(w.M, w.nG).T
+
+    def f(t, y):
+        Y = y.reshape(w.nG, w.M)
+        return (-1j * ((H0 + field(t) * X) @ Y)).ravel()
+
+    sol = scipy.integrate.solve_ivp(f, (0.0, T), orbs0.astype(np.complex128).ravel(), method="DOP853",
+                                    rtol=1e-13, atol=1e-15)
+    ref = np.concatenate([st[: w.nC], sol.y[:, -1].reshape(w.nG, w.M).T.ravel()])
+    lam = np.max(np.abs(np.linalg.eigvals(H0)))
+    n0 = int(np.ceil(T * lam / 1.0))  # |dt h| <= 1 at the coarsest step
+    errs = []
+    for n in (n0, 2 * n0, 4 * n0):
+        y, dt = st.copy(), T / n
+        for k in range(n):
+            y = s.rk4(y, k * dt, (k + 1) * dt, rhs=oracle.RHS_LIN)
+        errs.append(state_rel(y, ref, w.nC, wts, w.M))
+    assert errs[-1] > 1e-11, errs  # still well above the reference solution's own error
+    for a, b in zip(errs, errs[1:]):
+        assert 14.0 < a / b < 18.5, errs
