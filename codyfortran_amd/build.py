@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libcody_hip.so")
 
 SOURCES = ["host_setup.cpp", "state_files.cpp", "mctdhx_kernels.hip", "ci_kernels.hip", "meanfield_gemm.hip",
            "meanfield_lowprec.hip",
-           "cody_api.hip", "cody_propagate.hip", "cody_adjoint.hip"]
+           "cody_api.hip", "cody_propagate.hip", "cody_adjoint.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -335,6 +335,57 @@ __global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orb
   if (threadIdx.x < NO) part[(size_t)blockIdx.x * n2 + threadIdx.x + j * NO] = sRed[threadIdx.x];
 }
 
+// ------------------------------------------------------------------ overlap sums
+// inp(i, j) = sum_b part[b][i + j NO] over nPart per-workgroup partials, into sInp (LDS): G groups
+// of threads sum interleaved partial blocks, then the groups are added in order, so every
+// workgroup (and every kernel that uses this) forms bit-identical sums.
+template <int NO>
+constexpr int kOverlapGroups = NO * NO >= 256 ? 1 : 256 / (NO * NO);
+
+template <int NO>
+__device__ __forceinline__ void sum_overlap_parts(const double2* __restrict__ part, int nPart,
+                                                  double2 (*sG)[NO * NO], double2* sInp) {
+  constexpr int n2 = NO * NO, G = kOverlapGroups<NO>;
+  for (int e = threadIdx.x; e < G * n2; e += blockDim.x) {
+    const int g = e / n2, i = e % n2;
+    double2 acc = cmk(0, 0);
+#pragma unroll 8
+    for (int b = g; b < nPart; b += G) acc = cadd(acc, part[(size_t)b * n2 + i]);
+    sG[g][i] = acc;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+    double2 acc = sG[0][i];
+    for (int g = 1; g < G; ++g) acc = cadd(acc, sG[g][i]);
+    sInp[i] = acc;
+  }
+  __syncthreads();
+}
+
+// In-place projection f_j <- f_j - sum_i phi_i inp(i, j), inp from the k_overlap partials
+// (Grid_ProjectOnSubspace, src/Grid/S_Grid.f90:170-203, without the right-hand side's -i).
+template <int NO>
+__global__ __launch_bounds__(256) void k_project_inplace(const double2* __restrict__ orbs,
+                                                         const double2* __restrict__ part, int nPart, int nG,
+                                                         double2* __restrict__ f) {
+  constexpr int n2 = NO * NO, G = kOverlapGroups<NO>;
+  __shared__ double2 sG[G][n2];
+  __shared__ double2 sInp[n2];
+  sum_overlap_parts<NO>(part, nPart, sG, sInp);
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nG) return;
+  double2 phi[NO];
+#pragma unroll
+  for (int i = 0; i < NO; ++i) phi[i] = orbs[(size_t)i * nG + x];
+#pragma unroll
+  for (int j = 0; j < NO; ++j) {
+    double2 d = f[(size_t)j * nG + x];
+#pragma unroll
+    for (int i = 0; i < NO; ++i) d = csub(d, cmul(phi[i], sInp[i + j * NO]));
+    f[(size_t)j * nG + x] = d;
+  }
+}
+
 // ------------------------------------------------------------------ projection + scaling
 // dphi_j = -i (draw_j - sum_i phi_i inp(i,j))  (Grid_ProjectOnSubspace, src/Grid/S_Grid.f90:170-203),
 // inp = sum_b part[b] over the orbital kernel's per-workgroup partials (summed by every workgroup
@@ -348,7 +399,7 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
                                                  const double2* __restrict__ C,
                                                  const double2* __restrict__ dCraw, int nC,
                                                  double2* __restrict__ dC, const double2* __restrict__ hsub) {
-  constexpr int n2 = NO * NO, G = n2 >= 256 ? 1 : 256 / n2;
+  constexpr int n2 = NO * NO, G = kOverlapGroups<NO>;
   if (dC && blockIdx.x == gridDim.x - 1) {
     __shared__ double red[2][4];
     double2 ov = cmk(0, 0);
@@ -366,23 +417,7 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
   }
   __shared__ double2 sG[G][n2];
   __shared__ double2 sInp[n2];
-  if (project) {
-    // G groups of threads sum interleaved partial blocks, then the groups are added in order
-    for (int e = threadIdx.x; e < G * n2; e += blockDim.x) {
-      const int g = e / n2, i = e % n2;
-      double2 acc = cmk(0, 0);
-#pragma unroll 8
-      for (int b = g; b < nPart; b += G) acc = cadd(acc, part[(size_t)b * n2 + i]);
-      sG[g][i] = acc;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-      double2 acc = sG[0][i];
-      for (int g = 1; g < G; ++g) acc = cadd(acc, sG[g][i]);
-      sInp[i] = acc;
-    }
-    __syncthreads();
-  }
+  if (project) sum_overlap_parts<NO>(part, nPart, sG, sInp);
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nG) return;
   double2 phi[NO];
@@ -468,6 +503,13 @@ void launch_overlap_parts(const double2* orbs, const double* w, const double2* f
 #define CALL_OV(NO) hipLaunchKernelGGL(k_overlap<NO>, grid, dim3(256), 0, st, orbs, w, f, nG, part)
   CODY_NO_SWITCH(nO, CALL_OV)
 #undef CALL_OV
+}
+
+void launch_project_inplace(const double2* orbs, const double2* part, int nG, int nO, double2* f, hipStream_t st) {
+  const int nb = (nG + 255) / 256;
+#define CALL_PI(NO) hipLaunchKernelGGL(k_project_inplace<NO>, dim3(nb), dim3(256), 0, st, orbs, part, nb, nG, f)
+  CODY_NO_SWITCH(nO, CALL_PI)
+#undef CALL_PI
 }
 
 void launch_onebody(const double2* orbs, const double* w, const double* x, const double* vext,

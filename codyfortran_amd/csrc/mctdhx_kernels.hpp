@@ -65,6 +65,8 @@ void launch_tile_src(const double2* src, int ns, int nG, int nGp, int Pc, double
 void launch_overlap_parts(const double2* orbs, const double* w, const double2* f, int nG, int nO, double2* part,
                           hipStream_t st);
 void launch_h2_full(const double* G, int Pc, int nO, double2* h2, hipStream_t st);
+// f <- f - sum_i phi_i <phi_i|f>_w in place, from the ceil(nG / 256) x nO^2 partials of launch_overlap_parts
+void launch_project_inplace(const double2* orbs, const double2* part, int nG, int nO, double2* f, hipStream_t st);
 void launch_reginv(const double2* rho1, int nO, double reg, double2* reginv, hipStream_t st);
 // position matrix partials (k_xmat) and <x> from them and rho1 (k_dipole)
 void launch_xmat(const double2* orbs, const double* w, const double* x, int nG, int nO, double2* part1,

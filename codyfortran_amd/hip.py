@@ -94,6 +94,7 @@ def load():
         "cody_set_chebyshev": (i, [vp, d, d, dp, dp, ip]),
         "cody_strang_cheb": (i, [vp, d, d]),
         "cody_dipole_gradient": (i, [vp, d, d, i, i, dp, dp]),
+        "cody_project_orbs_dev_noalloc": (i, [i, i, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -169,6 +170,13 @@ def fedvr_kref(nLoc):
     K = np.zeros((nLoc, nLoc), order="F")
     _check(load().cody_fedvr_kref(nLoc, _dp(K)))
     return K
+
+
+def project_orbs_dev_noalloc(nG: int, nO: int, d_w: int, d_orbs: int, d_dorbs: int, stream: int = 0):
+    """In-place d_dorbs <- d_dorbs - sum_i phi_i <phi_i|d_dorbs>_w on device pointers, with no
+    allocation (static scratch of the library; nG <= 65536, nO <= 12)."""
+    _check(load().cody_project_orbs_dev_noalloc(int(nG), int(nO), C.c_void_p(d_w), C.c_void_p(d_orbs),
+                                                C.c_void_p(d_dorbs), C.c_void_p(stream or None)))
 
 
 class Context:

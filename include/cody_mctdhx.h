@@ -187,6 +187,18 @@ int cody_apply_correlation(cody_ctx* ctx, const double* orbs, const double* rdm1
 int cody_project_orbs(cody_ctx* ctx, const double* orbs, double* dorbs);
 int cody_apply_kinetic(cody_ctx* ctx, const double* orbs, double* dorbs);
 
+/* No counterpart in the reference: the orbital projector of cody_project_orbs for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight loops, harnesses that forbid
+ * hipMalloc).  Context-free, device pointers only, enqueued on `stream` (NULL: the null stream):
+ *   d_dorbs <- d_dorbs - sum_i phi_i <phi_i|d_dorbs>_w  (in place; Orbs_ProjectOnSubspace,
+ *   src/Orbs/M_Orbs.f90:132-153), d_orbs nG x nO complex128 column-major, d_w the nG weights.
+ * It allocates nothing: its scratch (the overlap partial sums) is a static device array of
+ * 576 KiB per GPU, so nG <= CODY_NOALLOC_MAX_POINTS and nO <= 12, else CODY_ERR_ARG.  All calls
+ * on one GPU share that array: calls that overlap in time on one GPU race. */
+#define CODY_NOALLOC_MAX_POINTS 65536
+int cody_project_orbs_dev_noalloc(int nG, int nO, const double* d_w, const void* d_orbs, void* d_dorbs,
+                                  void* stream);
+
 /* ---------------------------------------------------------------- propagation (device state) */
 /* The nonlinear part B of the split step, used by cody_time_derivative_nonlin_dev, cody_rk4(full=0)
  * and cody_strang:
