@@ -102,6 +102,8 @@ int prepare_precision(cody_ctx* c) {
 bool int8_tiled(const cody_ctx* c) { return c->precision == CODY_PREC_INT8_SLICED && c->Pc <= 80; }
 double* plain_st(cody_ctx* c) { return (c->precision == CODY_PREC_FP64 || int8_tiled(c)) ? nullptr : c->d_ST; }
 unsigned long long* st_max(cody_ctx* c) { return int8_tiled(c) ? c->d_smax : nullptr; }
+// activity flags of the pair sources for the contraction (null: every k block is contracted)
+const int* act_flags(const cody_ctx* c) { return c->skip_zero ? c->d_act : nullptr; }
 
 // Enqueue W = V S for the context's precision (ST already formed).  Returns 0, or the launcher's
 // error (nothing of the contraction launched) when the shape is outside what the kernels take.
@@ -114,7 +116,7 @@ int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(co
         launch_slice_s_tiled(c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
         mark("k_slice_s_tiled");
         CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->d_WT, c->nGp, c->Pc, kInt8Split,
-                           c->d_smax, st));
+                           c->d_smax, st, act_flags(c)));
         mark("k_mf_i8_w + k_wsum");
         return CODY_OK;
       }
@@ -136,7 +138,7 @@ int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(co
       mark("k_mf_bf16x3");
       return CODY_OK;
     default:
-      CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st));
+      CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st, act_flags(c)));
       mark("k_mf_f64_w + k_wsum");
       return CODY_OK;
   }
@@ -444,7 +446,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   // pair sources first, then the fork, then the mean fields W = V S.  (The pair-source kernel is
   // the only root of a captured graph and the contraction is created before the side-stream work,
   // so that a graph executor walking nodes in creation order starts the contraction first.)
-  launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
+  launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
   mark("k_pair_src");
   if (split) hipEventRecord(c->ev_fork, st);
   CHK(enqueue_contraction(c, st, mark));
@@ -539,7 +541,7 @@ int cody_autotune(cody_ctx* c, int* wsplit) {
     return CODY_OK;
   }
   if (c->precision == CODY_PREC_FP64) {
-    launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
+    launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -548,9 +550,10 @@ int cody_autotune(cody_ctx* c, int* wsplit) {
     for (int ks : {1, 2, 4, 8}) {
       // (splits the shape does not take are skipped; the check launches nothing)
       if (check_contraction_shape("k_mf_f64_w", c->nGp, c->Pc, ks, 10, kBlocksF64)) continue;
-      CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st));  // warm-up
+      CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st, act_flags(c)));  // warm-up
       HIPCHK(hipEventRecord(e0, st));
-      for (int i = 0; i < 8; ++i) CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st));
+      for (int i = 0; i < 8; ++i)
+        CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, ks, st, act_flags(c)));
       HIPCHK(hipEventRecord(e1, st));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -575,6 +578,15 @@ int cody_set_graphs(cody_ctx* c, int enable) {
   HIPCHK(hipStreamSynchronize(c->stream));
   c->drop_graphs();
   c->graphs_on = enable != 0;
+  return CODY_OK;
+}
+
+int cody_set_source_skip(cody_ctx* c, int enable) {
+  if (!c) return fail(CODY_ERR_ARG, "cody_set_source_skip: null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->skip_zero = enable != 0;
+  c->drop_graphs();  // captured contractions bake the flag pointer in
   return CODY_OK;
 }
 
@@ -669,7 +681,7 @@ int cody_meanfield_w(cody_ctx* c, const double* orbs, double* W) {
   HIPCHK(hipSetDevice(c->device));
   double2* d_orbs = c->d_tmp + c->nC;
   HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
-  launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->stream);
+  launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, c->stream);
   CHK(enqueue_contraction(c, c->stream, [](const char*) {}));
   HIPCHK(hipGetLastError());
   std::vector<double> wt((size_t)c->Pc * c->nGp);
@@ -774,7 +786,7 @@ int cody_apply_correlation(cody_ctx* c, const double* orbs, const double* rdm1, 
   HIPCHK(hipMemcpyAsync(c->d_rho1, rdm1, sizeof(double2) * n2, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->d_r2can, can.data(), sizeof(double) * can.size(), hipMemcpyHostToDevice, st));
   launch_reginv(c->d_rho1, nO, c->reg, c->d_reginv, st);
-  launch_pair_src(d_orbs, c->d_w, nG, c->nGp, nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
+  launch_pair_src(d_orbs, c->d_w, nG, c->nGp, nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
   CHK(enqueue_contraction(c, st, [](const char*) {}));
   launch_orbital(d_orbs, c->d_w, c->d_hphi, c->d_WT, c->d_r2can, c->d_reginv, nG, c->nGp, nO, kModeNonLin,
                  c->d_draw, c->d_part, st);
@@ -981,7 +993,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   CHK(check_ready(c));
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), st);
+  launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
   auto none = [](const char*) {};
   CHK(enqueue_contraction(c, st, none));  // warm-up
   hipEvent_t e0, e1;
@@ -993,7 +1005,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
       case CODY_PREC_INT8_SLICED:  // the contraction kernel alone
         if (int8_tiled(c))
           CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, nullptr, c->nGp, c->Pc, kInt8Split,
-                             nullptr, st));
+                             nullptr, st, act_flags(c)));
         else
           launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
         break;
@@ -1004,7 +1016,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
         launch_mf_bf16x3(c->d_Vbf, c->d_Sbf, c->d_WT, c->nGp, c->Pc, st);
         break;
       default:  // the contraction kernel alone (its partial sum k_wsum is not timed)
-        CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, nullptr, c->nGp, c->Pc, c->wsplit, st));
+        CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, nullptr, c->nGp, c->Pc, c->wsplit, st, act_flags(c)));
     }
   }
   HIPCHK(hipEventRecord(e1, st));

@@ -69,11 +69,15 @@ __global__ __launch_bounds__(256) void k_pair_src(const double2* __restrict__ or
 // STt[(t nK + K) 256 + 128 h + 2 l + e] = S^T[c = 16 t + l % 16][x = 16 K + 4 (l / 16) + 2 h + e].
 // A workgroup covers one 16 x 16 fragment; with smax non-null it also raises smax[c] to the
 // largest |S^T[c][x]| of its fragment (bit patterns of non-negative doubles order as integers),
-// which the int8 slicing needs as the per-line scale.
+// which the int8 slicing needs as the per-line scale.  With act non-null it writes act[fragment]
+// = 1 if any of its values is non-zero, else 0 (a plain store every call: nothing to clear); the
+// contraction kernels skip the k blocks whose fragments are all zero, the sources the reference
+// leaves out of its convolution sums (src/Utils/M_Utils_ConvolutionIntegral.f90:84-93).
 __global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ orbs,
                                                     const double* __restrict__ w, int nG, int nGp, int nO,
                                                     int Pc, double* __restrict__ STt,
-                                                    unsigned long long* __restrict__ smax) {
+                                                    unsigned long long* __restrict__ smax,
+                                                    int* __restrict__ act) {
   __shared__ unsigned long long sm[16];
   const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // grid = Pc nGp / 256 exactly
   const int nK = nGp >> 4;
@@ -99,6 +103,10 @@ __global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ 
     atomicMax(&sm[l & 15], (unsigned long long)__double_as_longlong(fabs(v)));
     __syncthreads();
     if (threadIdx.x < 16 && sm[threadIdx.x] != 0ull) atomicMax(&smax[16 * t + threadIdx.x], sm[threadIdx.x]);
+  }
+  if (act) {
+    const int any = __syncthreads_or(v != 0.0);
+    if (threadIdx.x == 0) act[blk] = any ? 1 : 0;
   }
 }
 
@@ -481,10 +489,10 @@ void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a
 // STt (fragment-tiled, always; with the per-line maxima when smax is non-null) and, when ST is
 // non-null, S^T row-major.
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
-                     double* ST, double* STt, unsigned long long* smax, hipStream_t st) {
+                     double* ST, double* STt, unsigned long long* smax, int* act, hipStream_t st) {
   const size_t n = (size_t)Pc * nGp;
   hipLaunchKernelGGL(k_pair_src_t, dim3((unsigned)(n / 256)), dim3(256), 0, st, orbs, w, nG, nGp, nO, Pc, STt,
-                     smax);
+                     smax, act);
   if (!ST) return;
 #define CALL_PS(NO)                                                                            \
   hipLaunchKernelGGL(k_pair_src<NO>, dim3((nGp + 255) / 256), dim3(256), 0, st, orbs, w, nG, nGp, \

@@ -80,8 +80,10 @@ void launch_ci_energy(int nO, int Pc, const double* G, const double2* h1, const 
                       const double2* rho2, double* energy, hipStream_t st);
 size_t ci_density_lds_bytes(int nC, int nO);
 int ci_kernels_allow_lds(size_t bytes);
+// pair sources S^T (fragment-tiled STt, optionally row-major ST), per-line maxima (smax, optional)
+// and per-fragment activity flags (act, optional: Pc/16 x nGp/16 ints, 1 = a non-zero value)
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
-                     double* ST, double* STt, unsigned long long* smax, hipStream_t st);
+                     double* ST, double* STt, unsigned long long* smax, int* act, hipStream_t st);
 void launch_onebody(const double2* orbs, const double* w, const double* x, const double* vext,
                     const KinLaunch& kl, const double* efield, int nG, int nO, double2* hphi, double2* part1,
                     hipStream_t st);
@@ -95,15 +97,19 @@ void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream
 void launch_split_bf16x3(const double* X, size_t n, __bf16* out, hipStream_t st);
 constexpr int kMaxWSplit = 8;  // largest k split of the FP64 contraction (partials buffer depth)
 constexpr int kInt8Split = 4;  // k split of the 64-row int8 contraction
+constexpr int kSkipMaxBlocks16 = 1024;  // longest grid (in 16-point blocks) whose zero k blocks are skipped
 // shape check of the k-split contraction launchers (0 or CODY_ERR_ARG, nothing launched)
 int check_contraction_shape(const char* who, int nGp, int Pc, int nsplit, int maxTiles, int kBlocks);
+// act (optional): the pair-source activity flags of launch_pair_src; the k blocks whose sources
+// are all zero are skipped (grids up to kSkipMaxBlocks16 16-point blocks; dense beyond)
 int launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
-                    int nsplit, hipStream_t st);
+                    int nsplit, hipStream_t st, const int* act = nullptr);
 void launch_tile_f64(const double* V, int nGp, double* Vt, hipStream_t st);
 void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp, int Pc, int nsplit,
                      hipStream_t st);
 int launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
-                   double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st);
+                   double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st,
+                   const int* act = nullptr);
 void launch_slice_s_tiled(const double* STt, const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out,
                           int* expo, hipStream_t st);
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,

@@ -71,6 +71,54 @@ __device__ __forceinline__ void reduce_store(double (&vals)[NT][4], double* __re
   }
 }
 
+// Active k blocks of the pair sources.  Block b (of nBlk blocks, each g 16-point fragments wide)
+// is active when any of its nT x g fragment flags act[t nK16 + g b + q] (k_pair_src_t) is set,
+// i.e. when some pair source is non-zero there; an inactive block's products are all zero, as in
+// the reference's convolution sums, which leave zero sources out
+// (src/Utils/M_Utils_ConvolutionIntegral.f90:84-93).  The active blocks go to sList in increasing
+// order, padded with one inactive block up to a multiple of `mult` (its sources are zero, so the
+// padding adds nothing; with no inactive block nBlk is already a multiple of mult), so that every
+// k range of a workgroup holds the same number of blocks.  Returns the padded count.  Called by
+// every thread of the workgroup; sWave holds one int per wave plus one.
+__device__ int active_blocks(const int* __restrict__ act, int nT, int nK16, int g, int nBlk, int mult,
+                             int* __restrict__ sList, int* __restrict__ sWave) {
+  const int nth = blockDim.x, nw = nth >> 6, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int per = (nBlk + nth - 1) / nth, b0 = min(tid * per, nBlk), b1 = min(b0 + per, nBlk);
+  unsigned mine = 0;  // bit i: block b0 + i is active (per <= 32)
+  int zero = nBlk;    // first inactive block of this thread's range
+  for (int b = b0; b < b1; ++b) {
+    int any = 0;
+    for (int t = 0; t < nT; ++t)
+      for (int q = 0; q < g; ++q) any |= act[(size_t)t * nK16 + g * b + q];
+    if (any) mine |= 1u << (b - b0);
+    else if (zero == nBlk) zero = b;
+  }
+  // exclusive prefix sum of the per-thread counts: within the wave by shuffles, then over waves
+  const int cnt = __popc(mine);
+  int incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (tid == 0) sWave[nw] = nBlk;
+  if (lane == 63) sWave[wave] = incl;
+  __syncthreads();
+  if (zero < nBlk) atomicMin(&sWave[nw], zero);
+  int off = incl - cnt;
+  for (int v = 0; v < wave; ++v) off += sWave[v];
+  int total = 0;
+  for (int v = 0; v < nw; ++v) total += sWave[v];
+  for (int i = 0; mine; ++i, mine >>= 1)
+    if (mine & 1u) sList[off++] = b0 + i;
+  __syncthreads();
+  const int padded = (total + mult - 1) / mult * mult;
+  const int z = sWave[nw];
+  for (int k = total + tid; k < padded; k += nth) sList[k] = z;
+  __syncthreads();
+  return padded;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------- FP64
@@ -161,23 +209,32 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double* __restrict__ At,
 // more than the extra launch.
 template <int NT>
 __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
-    const double* __restrict__ Vt, const double* __restrict__ STt, int nGp, double* __restrict__ Wpart) {
+    const double* __restrict__ Vt, const double* __restrict__ STt, int nGp, double* __restrict__ Wpart,
+    const int* __restrict__ act) {
   constexpr int KH = NT <= 5 ? 2 : 1, W = 4 * KH, D = 4, LD = NT * 16 + 1;
   constexpr int NCH = KH * NT * 2, CPW = (NCH + W - 1) / W;  // 1 KB S^T chunks per block, per wave
   constexpr int BUF = W * CPW * 128;  // doubles per LDS buffer (chunks NCH.. are never read)
   constexpr int LDS = 2 * BUF > 4 * 16 * LD ? 2 * BUF : 4 * 16 * LD;
   __shared__ double lds[LDS];
+  __shared__ int sList[kSkipMaxBlocks16];
+  __shared__ int sWave[4 * KH + 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
   const int r = wave & 3, h = wave >> 2;
-  const int nK = nGp >> 4, nbw = nK / (gridDim.y * KH);
-  const int Kb = blockIdx.y * (nK / gridDim.y);
-  const double* ab = Vt + ((size_t)(blockIdx.x * 4 + r) * nK + Kb + h * nbw) * 256 + 2 * lane;
+  const int nK = nGp >> 4;
+  // k blocks of each (k range, sub-range): consecutive positions of the active-block list, or of
+  // all blocks when act is null
+  const int nbw = (act ? active_blocks(act, NT, nK, 1, nK, gridDim.y * KH, sList, sWave) : nK) / (gridDim.y * KH);
+  auto kblock = [&](int hh, int blk) {
+    const int j = (blockIdx.y * KH + hh) * nbw + min(blk, nbw - 1);
+    return act ? sList[j] : j;
+  };
+  const double* ab = Vt + (size_t)(blockIdx.x * 4 + r) * nK * 256 + 2 * lane;
   // this wave's S^T chunks: chunk c = wave + W m = (hh NT + t) 2 + half of every block (chunks
   // past NCH are clamped onto the last one for the load and not written)
-  auto chunk_src = [&](int m) {
+  auto chunk_src = [&](int m, int blk) {
     const int c = min(wave + W * m, NCH - 1);
     const int half = c & 1, ht = c >> 1, hh = ht / NT, t = ht % NT;
-    return STt + ((size_t)t * nK + Kb + hh * nbw) * 256 + 128 * half + 2 * lane;
+    return STt + ((size_t)t * nK + kblock(hh, blk)) * 256 + 128 * half + 2 * lane;
   };
   d4_t acc[NT];
 #pragma unroll
@@ -185,12 +242,12 @@ __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
   d4_t a[D];
   d2_t stg[D][CPW];
   auto gload = [&](int u, int blk) {
-    const size_t o = (size_t)min(blk, nbw - 1) * 256;
+    const size_t o = (size_t)kblock(h, blk) * 256;
     const d2_t lo = *reinterpret_cast<const d2_t*>(ab + o);
     const d2_t hi = *reinterpret_cast<const d2_t*>(ab + o + 128);
     a[u] = (d4_t){lo[0], lo[1], hi[0], hi[1]};
 #pragma unroll
-    for (int m = 0; m < CPW; ++m) stg[u][m] = *reinterpret_cast<const d2_t*>(chunk_src(m) + o);
+    for (int m = 0; m < CPW; ++m) stg[u][m] = *reinterpret_cast<const d2_t*>(chunk_src(m, blk));
   };
   auto lwrite = [&](int u, int buf) {
 #pragma unroll
@@ -223,22 +280,24 @@ __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
     step((J), (J) & 1);                                \
     __syncthreads();                                   \
   }
-  gload(0, 0);
-  gload(1, 1);
-  gload(2, 2);
-  lwrite(0, 0);
-  __syncthreads();
-  int i = 0;
-  for (; i + D <= nbw; i += D) {
-    CODY_W_STEP(0)
-    CODY_W_STEP(1)
-    CODY_W_STEP(2)
-    CODY_W_STEP(3)
+  if (nbw > 0) {  // (no active block: the partial stays zero)
+    gload(0, 0);
+    gload(1, 1);
+    gload(2, 2);
+    lwrite(0, 0);
+    __syncthreads();
+    int i = 0;
+    for (; i + D <= nbw; i += D) {
+      CODY_W_STEP(0)
+      CODY_W_STEP(1)
+      CODY_W_STEP(2)
+      CODY_W_STEP(3)
+    }
+    // remaining nbw - i < D blocks (the conditions are uniform over the workgroup)
+    if (i < nbw) CODY_W_STEP(0)
+    if (i + 1 < nbw) CODY_W_STEP(1)
+    if (i + 2 < nbw) CODY_W_STEP(2)
   }
-  // remaining nbw - i < D blocks (the conditions are uniform over the workgroup)
-  if (i < nbw) CODY_W_STEP(0)
-  if (i + 1 < nbw) CODY_W_STEP(1)
-  if (i + 2 < nbw) CODY_W_STEP(2)
 #undef CODY_W_STEP
   // sum the k sub-ranges of each row tile into red[r] (element u of lane l: row 4u + l/16)
   double* slot = lds + r * 16 * LD;
@@ -292,7 +351,8 @@ template <int NT>
 __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, const int* __restrict__ eA,
                                                  const int8_t* __restrict__ Bs, const int* __restrict__ fB,
                                                  int nGp, double* __restrict__ Wpart,
-                                                 unsigned long long* __restrict__ smax) {
+                                                 unsigned long long* __restrict__ smax,
+                                                 const int* __restrict__ act) {
   constexpr int NS = kInt8Slices, W = 4, D = 3, LD = NT * 16 + 1;
   // the per-line maxima of S^T were consumed by the slicing (earlier on this stream): clear them
   // for the next right-hand side
@@ -301,14 +361,22 @@ __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, 
   constexpr int BUF = W * CPW * 1024;                    // bytes per LDS buffer
   constexpr int LDSB = 2 * BUF > 4 * 16 * LD * 8 ? 2 * BUF : 4 * 16 * LD * 8;
   __shared__ __align__(16) int8_t lds[LDSB];
+  __shared__ int sList[kSkipMaxBlocks16 / 4];
+  __shared__ int sWave[W + 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
-  const int nK64 = nGp >> 6, nbw = nK64 / gridDim.y;
-  const int Kb = blockIdx.y * nbw;
-  const int8_t* ab = As + ((size_t)(blockIdx.x * 4 + wave) * nK64 + Kb) * NS * 1024 + 16 * lane;
-  auto chunk_src = [&](int m) {  // chunk c = t NS + p of this wave's share
+  const int nK64 = nGp >> 6;
+  // k blocks of this k range: consecutive positions of the active-block list, or of all blocks
+  // when act is null
+  const int nbw = (act ? active_blocks(act, NT, nGp >> 4, 4, nK64, gridDim.y, sList, sWave) : nK64) / gridDim.y;
+  auto kblock = [&](int blk) {
+    const int j = blockIdx.y * nbw + min(blk, nbw - 1);
+    return act ? sList[j] : j;
+  };
+  const int8_t* ab = As + (size_t)(blockIdx.x * 4 + wave) * nK64 * NS * 1024 + 16 * lane;
+  auto chunk_src = [&](int m, int K) {  // chunk c = t NS + p of this wave's share, k block K
     const int c = min(wave + W * m, NCH - 1);
     const int t = c / NS, p = c % NS;
-    return Bs + (((size_t)t * nK64 + Kb) * NS + p) * 1024 + 16 * lane;
+    return Bs + (((size_t)t * nK64 + K) * NS + p) * 1024 + 16 * lane;
   };
   i4_t acc[NT][NS];
 #pragma unroll
@@ -318,11 +386,12 @@ __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, 
   i4_t a[D][NS];
   i4_t stg[D][CPW];
   auto gload = [&](int u, int blk) {
-    const size_t o = (size_t)min(blk, nbw - 1) * NS * 1024;
+    const int K = kblock(blk);
+    const size_t o = (size_t)K * NS * 1024;
 #pragma unroll
     for (int p = 0; p < NS; ++p) a[u][p] = *reinterpret_cast<const i4_t*>(ab + o + p * 1024);
 #pragma unroll
-    for (int m = 0; m < CPW; ++m) stg[u][m] = *reinterpret_cast<const i4_t*>(chunk_src(m) + o);
+    for (int m = 0; m < CPW; ++m) stg[u][m] = *reinterpret_cast<const i4_t*>(chunk_src(m, K));
   };
   auto lwrite = [&](int u, int buf) {
 #pragma unroll
@@ -350,25 +419,27 @@ __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, 
     step((J) % D, (J) & 1);                      \
     __syncthreads();                             \
   }
-  gload(0, 0);
-  gload(1, 1);
-  lwrite(0, 0);
-  __syncthreads();
-  int i = 0;
-  for (; i + 6 <= nbw; i += 6) {
-    CODY_I8_STEP(0)
-    CODY_I8_STEP(1)
-    CODY_I8_STEP(2)
-    CODY_I8_STEP(3)
-    CODY_I8_STEP(4)
-    CODY_I8_STEP(5)
+  if (nbw > 0) {  // (no active block: the partial stays zero)
+    gload(0, 0);
+    gload(1, 1);
+    lwrite(0, 0);
+    __syncthreads();
+    int i = 0;
+    for (; i + 6 <= nbw; i += 6) {
+      CODY_I8_STEP(0)
+      CODY_I8_STEP(1)
+      CODY_I8_STEP(2)
+      CODY_I8_STEP(3)
+      CODY_I8_STEP(4)
+      CODY_I8_STEP(5)
+    }
+    // remaining nbw - i < 6 blocks (the conditions are uniform over the workgroup)
+    if (i < nbw) CODY_I8_STEP(0)
+    if (i + 1 < nbw) CODY_I8_STEP(1)
+    if (i + 2 < nbw) CODY_I8_STEP(2)
+    if (i + 3 < nbw) CODY_I8_STEP(3)
+    if (i + 4 < nbw) CODY_I8_STEP(4)
   }
-  // remaining nbw - i < 6 blocks (the conditions are uniform over the workgroup)
-  if (i < nbw) CODY_I8_STEP(0)
-  if (i + 1 < nbw) CODY_I8_STEP(1)
-  if (i + 2 < nbw) CODY_I8_STEP(2)
-  if (i + 3 < nbw) CODY_I8_STEP(3)
-  if (i + 4 < nbw) CODY_I8_STEP(4)
 #undef CODY_I8_STEP
   // FP64 recombination; 32-bit MFMA results: element u of lane l is row 4(l/16) + u
   double* red = reinterpret_cast<double*>(lds);
@@ -642,14 +713,15 @@ int check_contraction_shape(const char* who, int nGp, int Pc, int nsplit, int ma
 // WT null only the partials are formed (k_wsum is not launched).  Returns 0, or CODY_ERR_ARG
 // without launching anything when the shape is outside check_contraction_shape.
 int launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* WT, int nGp, int Pc,
-                    int nsplit, hipStream_t st) {
+                    int nsplit, hipStream_t st, const int* act) {
   // per row group the kernel splits the nGp / 16 k blocks into nsplit ranges of KH sub-ranges
   const int kh = Pc / 16 <= 5 ? 2 : 1;
   if (int r = check_contraction_shape("k_mf_f64_w", nGp, Pc, nsplit, 10, nGp / 16 / kh)) return r;
+  if (nGp / 16 > kSkipMaxBlocks16) act = nullptr;  // (the block list lives in LDS)
   double* out = (nsplit == 1 && WT) ? WT : Wpart;
 #define CALL_W(NT)                                                                                    \
   hipLaunchKernelGGL(k_mf_f64_w<NT>, dim3(nGp / 64, nsplit), dim3((NT) <= 5 ? 512 : 256), 0, st, Vt, STt, \
-                     nGp, out)
+                     nGp, out, act)
   CODY_NT_SWITCH2(Pc, CALL_W)
 #undef CALL_W
   if (!WT || nsplit == 1) return 0;
@@ -674,11 +746,13 @@ void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp
 // (WT is written directly for nsplit = 1).  With WT null only the partials are formed.  Returns 0,
 // or CODY_ERR_ARG without launching anything when the shape is outside check_contraction_shape.
 int launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
-                   double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st) {
+                   double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st,
+                   const int* act) {
   if (int r = check_contraction_shape("k_mf_i8_w", nGp, Pc, nsplit, 5, nGp / 64)) return r;
+  if (nGp / 16 > kSkipMaxBlocks16) act = nullptr;  // (the block list lives in LDS)
   double* out = (nsplit == 1 && WT) ? WT : Wpart;
 #define CALL_I8(NT) \
-  hipLaunchKernelGGL(k_mf_i8_w<NT>, dim3(nGp / 64, nsplit), dim3(256), 0, st, As, eA, Bs, fB, nGp, out, smax)
+  hipLaunchKernelGGL(k_mf_i8_w<NT>, dim3(nGp / 64, nsplit), dim3(256), 0, st, As, eA, Bs, fB, nGp, out, smax, act)
   switch (Pc / 16) {
     case 1: CALL_I8(1); break;
     case 2: CALL_I8(2); break;

@@ -60,6 +60,7 @@ def load():
         "cody_set_precision": (i, [vp, i]),
         "cody_set_graphs": (i, [vp, i]),
         "cody_set_split": (i, [vp, i]),
+        "cody_set_source_skip": (i, [vp, i]),
         "cody_autotune": (i, [vp, C.POINTER(C.c_int)]),
         "cody_batch_create": (i, [C.POINTER(vp), i, C.POINTER(vp)]),
         "cody_batch_destroy": (i, [vp]),
@@ -251,6 +252,10 @@ class Context:
     def set_split(self, consistent: bool):
         """Nonlinear split part: the reference's (False, the default) or full minus linear (True)."""
         _check(self.lib.cody_set_split(self.h, SPLIT_CONSISTENT if consistent else SPLIT_REFERENCE))
+
+    def set_source_skip(self, enable: bool):
+        """Skip (True, the default) or contract the k blocks whose pair sources are all zero."""
+        _check(self.lib.cody_set_source_skip(self.h, int(bool(enable))))
 
     def set_graphs(self, enable: bool):
         """Replay captured right-hand sides (True, the default) or launch every kernel."""

@@ -97,6 +97,14 @@ int cody_set_precision(cody_ctx* ctx, int mode);
  * RK4 stages capture the launches of one right-hand side per (state, dState, mode) into a HIP
  * graph on first use and replay it; 0 launches every kernel individually. */
 int cody_set_graphs(cody_ctx* ctx, int enable);
+/* Zero sources of the mean-field contraction: with enable != 0 (the default) the contraction skips
+ * every 64-point (int8 form) or 16-point (FP64 form) block of the contraction index where all pair
+ * sources conj(phi_l) phi_k w are exactly zero, as the reference leaves zero sources out of its
+ * convolution sums (ConvolutionIntegral_Apply, src/Utils/M_Utils_ConvolutionIntegral.f90:84-93).
+ * The skipped products are exact zeros: W changes only by the summation order (~1e-16).  0
+ * contracts every block (the dense work, for measurement).  Grids above 16384 points are always
+ * contracted densely. */
+int cody_set_source_skip(cody_ctx* ctx, int enable);
 /* Project autotuner (no counterpart in the reference, which has no tiled kernels): times the FP64
  * contraction of this context's shape for each k split (1, 2, 4, 8 contiguous k ranges per row
  * group) and keeps the fastest; *wsplit (optional) receives the choice.  The timing uses the
