@@ -105,19 +105,25 @@ unsigned long long* st_max(cody_ctx* c) { return int8_tiled(c) ? c->d_smax : nul
 // activity flags of the pair sources for the contraction (null: every k block is contracted)
 const int* act_flags(const cody_ctx* c) { return c->skip_zero ? c->d_act : nullptr; }
 
-// Enqueue W = V S for the context's precision (ST already formed).  Returns 0, or the launcher's
-// error (nothing of the contraction launched) when the shape is outside what the kernels take.
-int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark) {
+// Enqueue W = V S for the context's precision (ST already formed).  With partialSplit null, W^T
+// is complete at the end; otherwise a k-split contraction leaves its partial planes in d_Wpart
+// and *partialSplit receives their count (0 when W^T itself was written), for a later sum.
+// Returns 0, or the launcher's error (nothing of the contraction launched) when the shape is
+// outside what the kernels take.
+int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark,
+                        int* partialSplit = nullptr) {
   const size_t ns = (size_t)c->Pc * c->nGp;
+  if (partialSplit) *partialSplit = 0;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
       if (int8_tiled(c)) {  // 64-row kernel: S slices fragment-tiled, four k ranges
         if (int r = check_contraction_shape("k_mf_i8_w", c->nGp, c->Pc, kInt8Split, 5, c->nGp / 64)) return r;
         launch_slice_s_tiled(c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
         mark("k_slice_s_tiled");
-        CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, c->d_WT, c->nGp, c->Pc, kInt8Split,
-                           c->d_smax, st, act_flags(c)));
-        mark("k_mf_i8_w + k_wsum");
+        CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, partialSplit ? nullptr : c->d_WT,
+                           c->nGp, c->Pc, kInt8Split, c->d_smax, st, act_flags(c)));
+        if (partialSplit) *partialSplit = kInt8Split;
+        mark(partialSplit ? "k_mf_i8_w" : "k_mf_i8_w + k_wsum");
         return CODY_OK;
       }
       launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, 0, c->d_Si8, c->d_eS, st);
@@ -137,10 +143,14 @@ int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(co
       launch_mf_bf16x3(c->d_Vbf, c->d_Sbf, c->d_WT, c->nGp, c->Pc, st);
       mark("k_mf_bf16x3");
       return CODY_OK;
-    default:
-      CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st, act_flags(c)));
-      mark("k_mf_f64_w + k_wsum");
+    default: {
+      const bool keep = partialSplit && c->wsplit > 1;  // (a single k range writes W^T itself)
+      CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, keep ? nullptr : c->d_WT, c->nGp, c->Pc, c->wsplit, st,
+                          act_flags(c)));
+      if (keep) *partialSplit = c->wsplit;
+      mark(keep ? "k_mf_f64_w" : "k_mf_f64_w + k_wsum");
       return CODY_OK;
+    }
   }
 }
 
@@ -436,48 +446,34 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
     mark("k_zero");
     return CODY_OK;
   }
-  // Two schedules of the same launches.  Split (default): the CI densities, R and the one-body
-  // part run on the side stream while the contraction runs; after the contraction the Gram matrix,
-  // h1 and the CI H-apply run on the side stream while the orbital correlation runs; the
-  // projections join both.  One queue (c->one_queue, used when many instances share a graph):
-  // every launch in order on `st`.
-  const bool split = !c->one_queue;
-  hipStream_t sa = split ? c->stream_ci : st;
-  // pair sources first, then the fork, then the mean fields W = V S.  (The pair-source kernel is
-  // the only root of a captured graph and the contraction is created before the side-stream work,
-  // so that a graph executor walking nodes in creation order starts the contraction first.)
-  launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
-  mark("k_pair_src");
-  if (split) hipEventRecord(c->ev_fork, st);
-  CHK(enqueue_contraction(c, st, mark));
-  if (split) {
-    hipEventRecord(c->ev_w, st);
-    hipStreamWaitEvent(sa, c->ev_fork, 0);
-  }
-  // independent of the mean fields: CI densities and R; h phi and the partial sums of h1
-  launch_ci_density(C, c->nC, c->nO, c->reg, c->dci, c->d_rho1, c->d_rho2, c->d_r2can, c->d_reginv, sa);
-  launch_onebody(orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield_src(c), c->nG, c->nO, c->d_hphi, c->d_part1, sa);
-  if (split) {
-    hipEventRecord(c->ev_a, sa);
-    hipStreamWaitEvent(sa, c->ev_w, 0);
-  }
-  // CI part: Gram matrix G = S^T W and h1, CI H-apply (and the energy on request)
-  launch_gram_f64(c->d_STt, c->d_WT, c->d_Gpart, c->nGp, c->Pc, c->nxs, sa);
-  launch_gram_reduce(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->d_part1, nPart1, n2, c->d_h1, sa);
-  launch_ci_apply(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_dCraw, sa);
-  if (energy) launch_ci_energy(c->nO, c->Pc, c->d_G, c->d_h1, c->d_rho1, c->d_rho2, energy, sa);
-  if (split) {
-    hipEventRecord(c->ev_ci, sa);
-    hipStreamWaitEvent(st, c->ev_a, 0);
-  }
-  // orbital correlation with its projector-overlap partials, then both projections
-  launch_orbital(orbs, c->d_w, c->d_hphi, c->d_WT, c->d_r2can, c->d_reginv, c->nG, c->nGp, c->nO, mode,
-                 c->d_draw, c->d_part, st);
-  mark("k_orbital");
-  if (split) hipStreamWaitEvent(st, c->ev_ci, 0);
-  launch_project(orbs, c->d_draw, c->d_part, c->nPart, c->nG, c->nO, 1, dorbs, C, c->d_dCraw, c->nC, dC,
-                 mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st);
-  mark("k_project");
+  // One queue, few launches: independent pieces share a launch as workgroup roles.
+  //   front  CI densities | one-body part and h1 partials | pair sources, maxima, activity flags
+  //   (S^T row-major for the forms that read it, the int8 slices of S)
+  //   W = V S, leaving its k-split partials
+  //   wsum   R = rho1 (rho1^2 + eps)^-1 | k-split sum of W^T
+  //   mid    Gram partials of G = S^T W | orbital correlation and projector-overlap partials
+  //   post   G and h1 from their partials | orbital projection
+  //   post   also the energy E = <C|H C> in partial sums
+  //   final  CI H-apply and CI projection with E
+  launch_rhs_front(C, c->nC, c->nO, c->dci, c->d_rho1, c->d_r2can, c->d_rho2, orbs, c->d_w, c->d_x, c->d_vext, c->kin,
+                   efield_src(c), c->nG, c->d_hphi, c->d_part1, c->nGp, c->Pc, c->d_STt, st_max(c), c->d_act, st);
+  mark("k_rhs_front");
+  if (double* sr = plain_st(c)) launch_pair_src_rows(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, sr, st);
+  int ps = 0;
+  CHK(enqueue_contraction(c, st, mark, &ps));
+  launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st);
+  mark("k_rhs_wsum");
+  launch_rhs_mid(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, orbs, c->d_w, c->d_hphi, c->d_r2can,
+                 c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st);
+  mark("k_rhs_mid");
+  launch_rhs_post(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->d_part1, nPart1, c->d_h1, c->d_rho1, c->d_rho2, c->d_epart,
+                  orbs, c->d_draw, c->d_part, c->nPart, c->nG, c->nO, dorbs,
+                  mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st);
+  mark("k_rhs_post");
+  launch_ci_apply_project(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_epart,
+                          rhs_post_energy_parts(c->Pc, c->nO), c->d_dCraw, dC, st);
+  mark("k_ci_apply_project");
+  if (energy) launch_ci_energy(c->nO, c->Pc, c->d_G, c->d_h1, c->d_rho1, c->d_rho2, energy, st);
   return CODY_OK;
 }
 
@@ -485,7 +481,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
 // on first use for these buffers and mode, replayed on `st`); plain launches when graphs are off
 // or capture fails.  The graph holds no memory-set nodes.
 int run_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, int mode, hipStream_t st) {
-  if (c->graphs_on) {
+  if (c->graphs_on && c->rhs_graphs) {
     hipGraphExec_t ex = nullptr;
     for (const auto& g : c->graphs)
       if (g.state == state && g.dstate == dstate && g.mode == mode && g.energy == nullptr) ex = g.exec;
@@ -577,7 +573,7 @@ int cody_set_graphs(cody_ctx* c, int enable) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->drop_graphs();
-  c->graphs_on = enable != 0;
+  c->graphs_on = c->rhs_graphs = enable != 0;
   return CODY_OK;
 }
 
@@ -958,13 +954,10 @@ int cody_batch_time_derivative_dev(cody_batch* b, void* const* d_states, void* c
     for (int q = 0; q < n && r == CODY_OK; ++q) {
       const int i = order[q];
       cody_ctx* c = b->ctx[i];
-      const bool oq = c->one_queue;
-      c->one_queue = true;
       c->capturing = true;
       r = enqueue_rhs(c, static_cast<const double2*>(d_states[i]), static_cast<double2*>(d_dstates[i]), t,
                       kModeFull, b->chain[chainOf[i]], nullptr, nullptr);
       c->capturing = false;
-      c->one_queue = oq;
     }
     for (int k = 0; k < cody_batch::kChains; ++k) {
       hipEventRecord(b->join[k], b->chain[k]);

@@ -115,12 +115,7 @@ struct AdjBuffers {
 struct cody_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  // side stream (default priority): CI densities, R and the one-body part while the contraction
-  // runs, then the CI part while the orbital correlation runs
-  hipStream_t stream_ci = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_a = nullptr, ev_w = nullptr, ev_ci = nullptr;
   bool split_consistent = false;  // nonlinear split part = full minus linear (cody_set_split)
-  bool one_queue = false;  // every launch of a right-hand side on the caller's stream
   // sizes
   int nG = 0, nGp = 0, N = 0, nO = 0, nC = 0, nS = 0, P = 0, Pc = 0;
   int nsplit = 1, nxs = 1, nPart = 1;
@@ -146,6 +141,7 @@ struct cody_ctx {
   int* d_act = nullptr;        // per-fragment activity of S^T (k_pair_src_t), Pc/16 x nGp/16
   bool skip_zero = true;       // contraction skips the k blocks whose pair sources are all zero
   double *d_Gpart = nullptr, *d_G = nullptr, *d_energy = nullptr;
+  double* d_epart = nullptr;  // energy partials of the G reduction (the CI projection's overlap)
   double* d_efield = nullptr;  // E(t) of the dipole coupling, read by the one-body kernel
   double2 *d_state = nullptr, *d_tmp = nullptr, *d_tmp2 = nullptr;
   double2 *d_rho1 = nullptr, *d_rho2 = nullptr, *d_reginv = nullptr;
@@ -168,14 +164,8 @@ struct cody_ctx {
 
   int allocate() {
     using cody::ctx_alloc;
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&stream_ci, hipStreamNonBlocking) != hipSuccess ||
-
-        hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_a, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_w, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_ci, hipEventDisableTiming) != hipSuccess)
-      return cody::set_error(2, "hipStreamCreate / hipEventCreate failed");
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+      return cody::set_error(2, "hipStreamCreate failed");
     const size_t n2 = (size_t)nO * nO, n4 = n2 * n2;
     int r = 0;
 #define ALLOC(ptr, bytes)                                   \
@@ -193,6 +183,7 @@ struct cody_ctx {
     ALLOC(d_Wpart, sizeof(double) * cody::kMaxWSplit * Pc * nGp);
     ALLOC(d_Gpart, sizeof(double) * nxs * Pc * Pc);
     ALLOC(d_G, sizeof(double) * Pc * Pc);
+    ALLOC(d_epart, sizeof(double) * ((Pc * Pc + nO * nO + 255) / 256));
     ALLOC(d_energy, sizeof(double) * 8);
     ALLOC(d_efield, sizeof(double));  // (every allocation starts zero-filled: E = 0)
     ALLOC(d_state, sizeof(double2) * nS);
@@ -255,7 +246,8 @@ struct cody_ctx {
   // captured right-hand sides, replayed by the device-pointer entry points and the propagators;
   // dropped whenever a setter changes what the launches bake in (kernels, sizes, parameters)
   std::vector<RhsGraph> graphs;
-  bool graphs_on = true;
+  bool graphs_on = true;    // propagators replay captured steps (cody_set_graphs)
+  bool rhs_graphs = false;  // single right-hand sides replay captured graphs (cody_set_graphs)
   bool capturing = false;  // enqueue_rhs is being captured (E(t) is then set per replay)
   unsigned generation = 0;  // bumped whenever captured launches become stale (batches check it)
   // E(t) as the one-body kernels read it; a captured Strang step points it at per-stage slots
@@ -270,7 +262,7 @@ struct cody_ctx {
   void release() {
     drop_graphs();
     void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_smax, d_act,
-                    d_Gpart, d_G,  d_energy, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
+                    d_Gpart, d_G,  d_energy, d_epart, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
                     d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, d_zeros, d_src, d_h2, ci_blob};
     for (void* p : ptrs)
       if (p) hipFree(p);
@@ -279,15 +271,8 @@ struct cody_ctx {
     void* lp[] = {d_Vt, d_Vi8, d_Si8, d_eV, d_eS, d_Vhi, d_Vlo, d_Shi, d_Slo, d_Vbf, d_Sbf};
     for (void* p : lp)
       if (p) hipFree(p);
-    if (ev_fork) hipEventDestroy(ev_fork);
-    if (ev_a) hipEventDestroy(ev_a);
-    if (ev_w) hipEventDestroy(ev_w);
-    if (ev_ci) hipEventDestroy(ev_ci);
-
-    if (stream_ci) hipStreamDestroy(stream_ci);
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
-    stream_ci = nullptr;
   }
 };
 

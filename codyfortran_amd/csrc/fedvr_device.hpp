@@ -26,6 +26,26 @@ __device__ __forceinline__ void cfmac(double2& acc, double2 a, double2 b) {
   acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
 }
 __device__ __forceinline__ double2 mul_minus_i(double2 a) { return cmk(a.y, -a.x); }
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return cmk((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// ------------------------------------------------------------------ orbital index helpers
+// column-major index of a 4-index tensor of side n
+__device__ __forceinline__ int idx4(int a, int b, int c, int d, int n) { return a + n * (b + n * (c + n * d)); }
+// unordered pair l <= k among the nO (nO + 1) / 2 pairs
+__device__ __forceinline__ int pair_index(int nO, int l, int k) { return l * nO - (l * (l - 1)) / 2 + (k - l); }
+// strict pair a < b among the nO (nO - 1) / 2 strict pairs, and its inverse
+__device__ __forceinline__ int strict_pair(int nO, int a, int b) { return a * nO - (a * (a + 1)) / 2 + (b - a - 1); }
+__device__ __forceinline__ void strict_pair_decode(int nO, int p, int& a, int& b) {
+  a = 0;
+  while (p >= nO - 1 - a) {
+    p -= nO - 1 - a;
+    ++a;
+  }
+  b = a + 1 + p;
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -9,18 +9,12 @@
 // Complex scalars are double2 (x = re, y = im).
 #include <hip/hip_runtime.h>
 
+#include "ci_device.hpp"
 #include "fedvr_device.hpp"
+#include "gemm_device.hpp"
 #include "mctdhx_kernels.hpp"
 
 namespace cody {
-
-__device__ __forceinline__ int pair_index(int nO, int l, int k) {  // l <= k
-  return l * nO - (l * (l - 1)) / 2 + (k - l);
-}
-
-__device__ __forceinline__ int idx4(int a, int b, int c, int d, int n) {
-  return a + n * (b + n * (c + n * d));
-}
 
 // ------------------------------------------------------------------ interaction matrix
 // V[i][j] = s e^{-d r}/(sqrt(r^2 + a1^2) + a2), r = |x_i - x_j|
@@ -73,13 +67,13 @@ __global__ __launch_bounds__(256) void k_pair_src(const double2* __restrict__ or
 // = 1 if any of its values is non-zero, else 0 (a plain store every call: nothing to clear); the
 // contraction kernels skip the k blocks whose fragments are all zero, the sources the reference
 // leaves out of its convolution sums (src/Utils/M_Utils_ConvolutionIntegral.f90:84-93).
-__global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ orbs,
+__device__ __forceinline__ void pair_src_t_body(int bx, const double2* __restrict__ orbs,
                                                     const double* __restrict__ w, int nG, int nGp, int nO,
                                                     int Pc, double* __restrict__ STt,
                                                     unsigned long long* __restrict__ smax,
                                                     int* __restrict__ act) {
   __shared__ unsigned long long sm[16];
-  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // grid = Pc nGp / 256 exactly
+  const size_t o = (size_t)bx * blockDim.x + threadIdx.x;  // grid = Pc nGp / 256 exactly
   const int nK = nGp >> 4;
   const int e = o & 1, h = (o >> 7) & 1, l = (o >> 1) & 63;
   const size_t blk = o >> 8;
@@ -108,6 +102,14 @@ __global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ 
     const int any = __syncthreads_or(v != 0.0);
     if (threadIdx.x == 0) act[blk] = any ? 1 : 0;
   }
+}
+
+__global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ orbs,
+                                                    const double* __restrict__ w, int nG, int nGp, int nO,
+                                                    int Pc, double* __restrict__ STt,
+                                                    unsigned long long* __restrict__ smax,
+                                                    int* __restrict__ act) {
+  pair_src_t_body(blockIdx.x, orbs, w, nG, nGp, nO, Pc, STt, smax, act);
 }
 
 // Arbitrary complex sources (ns columns of nG points, column-major) in the fragment-tiled S^T
@@ -163,22 +165,34 @@ __global__ __launch_bounds__(256) void k_overlap(const double2* __restrict__ orb
 //   hphi_j = (T + V_ext + E(t) x) phi_j                   (ApplySingleBodyOp :976-1034)
 // and the workgroup's partial sums of h1(i,j) = <phi_i|hphi_j>_w, part1[block][i + j NO].
 template <int NO>
-__global__ __launch_bounds__(256) void k_onebody(const double2* __restrict__ orbs, const double* __restrict__ w,
-                                                 const double* __restrict__ xg, const double* __restrict__ vext,
-                                                 KinParams kp, const double* __restrict__ efield, int nG,
-                                                 double2* __restrict__ hphi, double2* __restrict__ part1) {
+__device__ __forceinline__ void onebody_body(int bx, int j, const double2* __restrict__ orbs,
+                                             const double* __restrict__ w, const double* __restrict__ xg,
+                                             const double* __restrict__ vext, KinParams kp,
+                                             const double* __restrict__ efield, int nG, double2* __restrict__ hphi,
+                                             double2* __restrict__ part1) {
   __shared__ double2 sRed[4][NO];
-  const int j = blockIdx.y;
+  __shared__ double2 sF[256 + 2 * kMaxStencilReach];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int x = bx * 256 + threadIdx.x;
   const bool in = x < nG;
   const int xc = in ? x : nG - 1;
-  double2 h = cmk(0, 0);
+  // orbital j on this tile and on the stencil's reach (nLoc - 1 points) either side, in LDS
+  const double2* fj = orbs + (size_t)j * nG;
+  const int halo = kp.nLoc - 1, lo = bx * 256 - halo;
+  for (int t = threadIdx.x; t < 256 + 2 * halo; t += 256) {
+    const int i = lo + t;
+    sF[t] = (i >= 0 && i < nG) ? fj[i] : cmk(0, 0);
+  }
+  double2 phi[NO];
+#pragma unroll
+  for (int i = 0; i < NO; ++i) phi[i] = in ? orbs[(size_t)i * nG + xc] : cmk(0, 0);
   const double wx = in ? w[xc] : 0.0;
+  const double v = in ? vext[xc] + *efield * xg[xc] : 0.0;
+  __syncthreads();
+  double2 h = cmk(0, 0);
   if (in) {
-    h = kinetic_at(orbs + (size_t)j * nG, xc, kp, 1.0 / wx);
-    const double v = vext[xc] + *efield * xg[xc];
-    const double2 pj = orbs[(size_t)j * nG + xc];
+    h = kinetic_at_lds(sF, lo, xc, kp, 1.0 / wx);
+    const double2 pj = sF[xc - lo];
     h.x = fma(v, pj.x, h.x);
     h.y = fma(v, pj.y, h.y);
     hphi[(size_t)j * nG + x] = h;
@@ -187,7 +201,7 @@ __global__ __launch_bounds__(256) void k_onebody(const double2* __restrict__ orb
 #pragma unroll
   for (int i = 0; i < NO; ++i) {
     double2 acc = cmk(0, 0);
-    if (in) cfmac(acc, orbs[(size_t)i * nG + xc], hw);
+    if (in) cfmac(acc, phi[i], hw);
     acc.x = wave_sum(acc.x);
     acc.y = wave_sum(acc.y);
     if (lane == 0) sRed[wave][i] = acc;
@@ -196,8 +210,16 @@ __global__ __launch_bounds__(256) void k_onebody(const double2* __restrict__ orb
   if (threadIdx.x < NO) {
     const int i = threadIdx.x;
     const double2 acc = cadd(cadd(sRed[0][i], sRed[1][i]), cadd(sRed[2][i], sRed[3][i]));
-    part1[(size_t)blockIdx.x * NO * NO + i + j * NO] = acc;
+    part1[(size_t)bx * NO * NO + i + j * NO] = acc;
   }
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_onebody(const double2* __restrict__ orbs, const double* __restrict__ w,
+                                                 const double* __restrict__ xg, const double* __restrict__ vext,
+                                                 KinParams kp, const double* __restrict__ efield, int nG,
+                                                 double2* __restrict__ hphi, double2* __restrict__ part1) {
+  onebody_body<NO>(blockIdx.x, blockIdx.y, orbs, w, xg, vext, kp, efield, nG, hphi, part1);
 }
 
 // Partial sums of the position matrix x(i,j) = <phi_i|x phi_j>_w over 256-point blocks,
@@ -240,7 +262,7 @@ __global__ __launch_bounds__(256) void k_xmat(const double2* __restrict__ orbs, 
 // read as broadcasts; each wave loads all of its W values before using them, and the four
 // correlation shares are added in fixed order.
 template <int NO>
-__global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orbs, const double* __restrict__ w,
+__device__ __forceinline__ void orbital_body(int bx, int j, const double2* __restrict__ orbs, const double* __restrict__ w,
                                                  const double2* __restrict__ hphi, const double* __restrict__ WT,
                                                  const double2* __restrict__ r2can, const double2* __restrict__ reginv,
                                                  int nG, int nGp, int mode, double2* __restrict__ draw,
@@ -253,9 +275,8 @@ __global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orb
   __shared__ int sPair[n2];  // strict-pair index of {a, b}; -1 - index when a > b; kNone when a = b
   __shared__ double2 sCorr[4][64];
   __shared__ double2 sRed[NO];
-  const int j = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x = blockIdx.x * 64 + lane;
+  const int x = bx * 64 + lane;
   // canonical rho2 (i1 < i2, j1 < j2), column j of R and the pair table
   for (int e = threadIdx.x; e < nCan; e += 256) s2[e] = r2can[e];
   for (int e = threadIdx.x; e < n2; e += 256) {
@@ -340,7 +361,16 @@ __global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orb
     if (lane == 0) sRed[i] = acc;
   }
   __syncthreads();
-  if (threadIdx.x < NO) part[(size_t)blockIdx.x * n2 + threadIdx.x + j * NO] = sRed[threadIdx.x];
+  if (threadIdx.x < NO) part[(size_t)bx * n2 + threadIdx.x + j * NO] = sRed[threadIdx.x];
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orbs, const double* __restrict__ w,
+                                                 const double2* __restrict__ hphi, const double* __restrict__ WT,
+                                                 const double2* __restrict__ r2can, const double2* __restrict__ reginv,
+                                                 int nG, int nGp, int mode, double2* __restrict__ draw,
+                                                 double2* __restrict__ part) {
+  orbital_body<NO>(blockIdx.x, blockIdx.y, orbs, w, hphi, WT, r2can, reginv, nG, nGp, mode, draw, part);
 }
 
 // ------------------------------------------------------------------ overlap sums
@@ -400,7 +430,7 @@ __global__ __launch_bounds__(256) void k_project_inplace(const double2* __restri
 // into LDS, in fixed order).  With dC non-null the last workgroup applies the CI projection and
 // scaling dC = -i (dCraw - <C|dCraw> C)  (src/Coeffs/S_Coeffs.f90:105-111).
 template <int NO>
-__global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orbs,
+__device__ __forceinline__ void project_body(int bx, bool ciBlock, const double2* __restrict__ orbs,
                                                  const double2* __restrict__ draw,
                                                  const double2* __restrict__ part, int nPart, int nG,
                                                  int project, double2* __restrict__ dorbs,
@@ -408,7 +438,7 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
                                                  const double2* __restrict__ dCraw, int nC,
                                                  double2* __restrict__ dC, const double2* __restrict__ hsub) {
   constexpr int n2 = NO * NO, G = kOverlapGroups<NO>;
-  if (dC && blockIdx.x == gridDim.x - 1) {
+  if (ciBlock) {
     __shared__ double red[2][4];
     double2 ov = cmk(0, 0);
     for (int i = threadIdx.x; i < nC; i += blockDim.x) cfmac(ov, C[i], dCraw[i]);
@@ -426,7 +456,7 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
   __shared__ double2 sG[G][n2];
   __shared__ double2 sInp[n2];
   if (project) sum_overlap_parts<NO>(part, nPart, sG, sInp);
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = bx * blockDim.x + threadIdx.x;
   if (x >= nG) return;
   double2 phi[NO];
   if (project) {
@@ -443,6 +473,18 @@ __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orb
     if (hsub) d = csub(d, hsub[(size_t)j * nG + x]);
     dorbs[(size_t)j * nG + x] = mul_minus_i(d);
   }
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orbs,
+                                                 const double2* __restrict__ draw,
+                                                 const double2* __restrict__ part, int nPart, int nG,
+                                                 int project, double2* __restrict__ dorbs,
+                                                 const double2* __restrict__ C,
+                                                 const double2* __restrict__ dCraw, int nC,
+                                                 double2* __restrict__ dC, const double2* __restrict__ hsub) {
+  project_body<NO>(blockIdx.x, dC && blockIdx.x == gridDim.x - 1, orbs, draw, part, nPart, nG, project, dorbs, C,
+                   dCraw, nC, dC, hsub);
 }
 
 // ------------------------------------------------------------------ scalar store (graph-capturable)
@@ -499,6 +541,15 @@ void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int 
                      Pc, ST)
   CODY_NO_SWITCH(nO, CALL_PS)
 #undef CALL_PS
+}
+
+void launch_pair_src_rows(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc, double* ST,
+                          hipStream_t st) {
+#define CALL_PR(NO)                                                                                            \
+  hipLaunchKernelGGL(k_pair_src<NO>, dim3((nGp + 255) / 256), dim3(256), 0, st, orbs, w, nG, nGp, \
+                     Pc, ST)
+  CODY_NO_SWITCH(nO, CALL_PR)
+#undef CALL_PR
 }
 
 void launch_tile_src(const double2* src, int ns, int nG, int nGp, int Pc, double* STt, hipStream_t st) {
@@ -560,5 +611,143 @@ void launch_project(const double2* orbs, const double2* draw, const double2* par
   CODY_NO_SWITCH(nO, CALL_PR)
 #undef CALL_PR
 }
+
+
+// ------------------------------------------------------------------ fused right-hand-side launches
+// The right-hand side runs on one queue as a handful of launches.  Pieces that do not depend on
+// each other share a launch as workgroup roles (each role a contiguous range of workgroup
+// indices), so a latency-bound piece costs no launch of its own and no cross-queue dependency.
+
+// Front: from the state alone, the CI densities (rho1, rho2, canonical rho2), the one-body part
+// (h phi, h1 partials) and the pair sources (S^T tiled, line maxima, activity flags).
+template <int NO>
+__global__ __launch_bounds__(256) void k_rhs_front(const double2* __restrict__ C, int nC, DevCiTables t,
+                                                   double2* __restrict__ rho1, double2* __restrict__ r2can,
+                                                   double2* __restrict__ rho2, int nRdm,
+                                                   const double2* __restrict__ orbs, const double* __restrict__ w,
+                                                   const double* __restrict__ xg, const double* __restrict__ vext,
+                                                   KinParams kp, const double* __restrict__ efield, int nG,
+                                                   double2* __restrict__ hphi, double2* __restrict__ part1, int nObx,
+                                                   int nGp, int Pc, double* __restrict__ STt,
+                                                   unsigned long long* __restrict__ smax, int* __restrict__ act) {
+  int b = blockIdx.x;
+  if (b < nRdm) {
+    ci_rdm_body(b, C, nC, NO, t.r1_ptr, t.r1, t.r2_ptr, t.r2, rho1, r2can, rho2);
+    return;
+  }
+  b -= nRdm;
+  if (b < nObx * NO) {
+    onebody_body<NO>(b % nObx, b / nObx, orbs, w, xg, vext, kp, efield, nG, hphi, part1);
+    return;
+  }
+  b -= nObx * NO;
+  pair_src_t_body(b, orbs, w, nG, nGp, NO, Pc, STt, smax, act);
+}
+
+// After the contraction: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0) and the k-split sum of W^T
+// (the others; none when the contraction wrote W^T itself).
+template <int NO>
+__global__ __launch_bounds__(256) void k_rhs_wsum(const double2* __restrict__ rho1, double reg,
+                                                  double2* __restrict__ reginv, const double* __restrict__ Wpart,
+                                                  int nsplit, size_t plane, double* __restrict__ WT) {
+  if (blockIdx.x == 0) {
+    // one wave: its barriers cost nothing once the other waves have left the workgroup
+    if (threadIdx.x >= 64) return;
+    reginv_body<NO>(rho1, reg, reginv, 64);
+    return;
+  }
+  wsum_body(blockIdx.x - 1, Wpart, nsplit, plane, WT);
+}
+
+// After W: the Gram matrix's split-K partials (G = S^T W) and the orbital correlation with its
+// projector-overlap partials, side by side (both only read W^T).
+template <int NO>
+__global__ __launch_bounds__(256) void k_rhs_mid(const double* __restrict__ STt, const double* __restrict__ WT,
+                                                 int nGp, int Pc, double* __restrict__ Gpart, int nxs,
+                                                 const double2* __restrict__ orbs, const double* __restrict__ w,
+                                                 const double2* __restrict__ hphi, const double2* __restrict__ r2can,
+                                                 const double2* __restrict__ reginv, int nG, int mode,
+                                                 double2* __restrict__ draw, double2* __restrict__ part) {
+  constexpr int NT = (NO * (NO + 1) + 15) / 16;  // = Pc / 16
+  const int nGram = NT * nxs;
+  int b = blockIdx.x;
+  if (b < nGram) {
+    gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc);
+    return;
+  }
+  b -= nGram;
+  const int nOrbX = nGp / 64;
+  orbital_body<NO>(b % nOrbX, b / nOrbX, orbs, w, hphi, WT, r2can, reginv, nG, nGp, mode, draw, part);
+}
+
+// Then: G and h1 from their partials with the energy partials (the first nRed workgroups), and the
+// orbital projection with the right-hand side's factor -i (the others).
+template <int NO>
+__global__ __launch_bounds__(256) void k_rhs_post(const double* __restrict__ Gpart, int nxs, int Pc,
+                                                  double* __restrict__ G, const double2* __restrict__ part1,
+                                                  int nPart1, double2* __restrict__ h1,
+                                                  const double2* __restrict__ rho1, const double2* __restrict__ rho2,
+                                                  double* __restrict__ epart, int nRed,
+                                                  const double2* __restrict__ orbs, const double2* __restrict__ draw,
+                                                  const double2* __restrict__ part, int nPart, int nG,
+                                                  double2* __restrict__ dorbs, const double2* __restrict__ hsub) {
+  int b = blockIdx.x;
+  if (b < nRed) {
+    gram_reduce_energy_body(b, Gpart, nxs, Pc, G, part1, nPart1, NO, h1, rho1, rho2, epart);
+    return;
+  }
+  project_body<NO>(b - nRed, false, orbs, draw, part, nPart, nG, 1, dorbs, nullptr, nullptr, 0, nullptr, hsub);
+}
+
+void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, double2* rho1, double2* r2can,
+                      double2* rho2, const double2* orbs, const double* w, const double* x, const double* vext,
+                      const KinLaunch& kl, const double* efield, int nG, double2* hphi, double2* part1, int nGp,
+                      int Pc, double* STt, unsigned long long* smax, int* act, hipStream_t st) {
+  const int n2 = nO * nO, np = nO * (nO - 1) / 2, nRdm = (n2 + np * np + 3) / 4;
+  const int nObx = (nG + 255) / 256, nPair = (int)((size_t)Pc * nGp / 256);
+  KinParams kp{kl.kref, kl.nE, kl.nLoc, kl.xminExcl, nG, kl.invh, kl.fac};
+  const unsigned grid = (unsigned)(nRdm + nObx * nO + nPair);
+#define CALL_FR(NO)                                                                                             \
+  hipLaunchKernelGGL(k_rhs_front<NO>, dim3(grid), dim3(256), sizeof(double2) * nC, st, C, nC, t, rho1, r2can, rho2, \
+                     nRdm, orbs, w, x, vext, kp, efield, nG, hphi, part1, nObx, nGp, Pc, STt, smax, act)
+  CODY_NO_SWITCH(nO, CALL_FR)
+#undef CALL_FR
+}
+
+void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, const double* Wpart, int nsplit,
+                     size_t plane, double* WT, hipStream_t st) {
+  const unsigned nSum = nsplit > 1 ? (unsigned)((plane / 2 + 255) / 256) : 0u;
+#define CALL_WS(NO) \
+  hipLaunchKernelGGL(k_rhs_wsum<NO>, dim3(1 + nSum), dim3(256), 0, st, rho1, reg, reginv, Wpart, nsplit, plane, WT)
+  CODY_NO_SWITCH(nO, CALL_WS)
+#undef CALL_WS
+}
+
+void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
+                    const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
+                    const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st) {
+  const unsigned grid = (unsigned)(Pc / 16 * nxs + nGp / 64 * nO);
+#define CALL_MI(NO)                                                                                             \
+  hipLaunchKernelGGL(k_rhs_mid<NO>, dim3(grid), dim3(256), 0, st, STt, WT, nGp, Pc, Gpart, nxs, orbs, w, hphi, r2can, \
+                     reginv, nG, mode, draw, part)
+  CODY_NO_SWITCH(nO, CALL_MI)
+#undef CALL_MI
+}
+
+int rhs_post_energy_parts(int Pc, int nO) { return (Pc * Pc + nO * nO + 255) / 256; }
+
+void launch_rhs_post(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
+                     double2* h1, const double2* rho1, const double2* rho2, double* epart, const double2* orbs,
+                     const double2* draw, const double2* part, int nPart, int nG, int nO, double2* dorbs,
+                     const double2* hsub, hipStream_t st) {
+  const int nRed = rhs_post_energy_parts(Pc, nO);
+  const unsigned grid = (unsigned)(nRed + (nG + 255) / 256);
+#define CALL_PO(NO)                                                                                                \
+  hipLaunchKernelGGL(k_rhs_post<NO>, dim3(grid), dim3(256), 0, st, Gpart, nxs, Pc, G, part1, nPart1, h1, rho1, rho2, \
+                     epart, nRed, orbs, draw, part, nPart, nG, dorbs, hsub)
+  CODY_NO_SWITCH(nO, CALL_PO)
+#undef CALL_PO
+}
+
 
 }  // namespace cody

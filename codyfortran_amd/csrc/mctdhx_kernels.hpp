@@ -10,6 +10,7 @@ namespace cody {
 int set_error(int code, const char* msg);  // records the library's last error, returns code
 
 constexpr int kMaxOrbs = 12;   // orbitals per body type handled by the kernels
+constexpr int kMaxStencilReach = 31;  // nLoc - 1 for the largest FEDVR element (nLoc <= 32)
 constexpr int kMaxPc = 160;    // 16*ceil(M(M+1)/16) for M <= 12
 constexpr int kInt8Slices = 5; // balanced base-254 digits per operand of the int8-sliced contraction
 // Digit base of the int8 slices.  Values scaled into [-1/2, 1/2] have balanced base-254 digits
@@ -78,12 +79,40 @@ void launch_ci_apply(const double2* C, int nC, int nO, int Pc, const double* G, 
                      const DevCiTables& t, double2* dCraw, hipStream_t st);
 void launch_ci_energy(int nO, int Pc, const double* G, const double2* h1, const double2* rho1,
                       const double2* rho2, double* energy, hipStream_t st);
+// CI H-apply and CI projection in one launch: dCraw = H C, dC = -i (dCraw - E C), E the sum of the
+// nE energy partials of launch_rhs_post (E = <C|H C>)
+void launch_ci_apply_project(const double2* C, int nC, int nO, int Pc, const double* G, const double2* h1,
+                             const DevCiTables& t, const double* epart, int nE, double2* dCraw, double2* dC,
+                             hipStream_t st);
+// Fused launches of the right-hand side (one queue):
+//   front: CI densities, one-body part (h phi, h1 partials), pair sources (+ maxima, flags);
+//   wsum:  R = rho1 (rho1^2 + eps)^-1 and the k-split sum of W^T (nsplit <= 1: none);
+//   mid:   Gram partials (nxs x-ranges) and the orbital correlation with its overlap partials;
+//   post:  G and h1 from their partials with the energy partials epart (one per reduction
+//          workgroup, rhs_post_energy_parts of them), and the orbital projection (times -i).
+void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, double2* rho1, double2* r2can,
+                      double2* rho2, const double2* orbs, const double* w, const double* x, const double* vext,
+                      const KinLaunch& kl, const double* efield, int nG, double2* hphi, double2* part1, int nGp,
+                      int Pc, double* STt, unsigned long long* smax, int* act, hipStream_t st);
+void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, const double* Wpart, int nsplit,
+                     size_t plane, double* WT, hipStream_t st);
+void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
+                    const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
+                    const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st);
+int rhs_post_energy_parts(int Pc, int nO);
+void launch_rhs_post(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
+                     double2* h1, const double2* rho1, const double2* rho2, double* epart, const double2* orbs,
+                     const double2* draw, const double2* part, int nPart, int nG, int nO, double2* dorbs,
+                     const double2* hsub, hipStream_t st);
 size_t ci_density_lds_bytes(int nC, int nO);
 int ci_kernels_allow_lds(size_t bytes);
 // pair sources S^T (fragment-tiled STt, optionally row-major ST), per-line maxima (smax, optional)
 // and per-fragment activity flags (act, optional: Pc/16 x nGp/16 ints, 1 = a non-zero value)
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
                      double* ST, double* STt, unsigned long long* smax, int* act, hipStream_t st);
+// row-major S^T alone (the operand of the split-FP32, BF16 and untiled int8 forms)
+void launch_pair_src_rows(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc, double* ST,
+                          hipStream_t st);
 void launch_onebody(const double2* orbs, const double* w, const double* x, const double* vext,
                     const KinLaunch& kl, const double* efield, int nG, int nO, double2* hphi, double2* part1,
                     hipStream_t st);

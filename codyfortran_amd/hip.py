@@ -258,7 +258,8 @@ class Context:
         _check(self.lib.cody_set_source_skip(self.h, int(bool(enable))))
 
     def set_graphs(self, enable: bool):
-        """Replay captured right-hand sides (True, the default) or launch every kernel."""
+        """True: replay captured graphs for single right-hand sides and propagator steps; False:
+        launch every kernel.  Default: propagator steps replayed, right-hand sides launched."""
         _check(self.lib.cody_set_graphs(self.h, int(bool(enable))))
 
     # ---------------------------------------------------------------- state

@@ -92,10 +92,12 @@ int cody_set_interaction_softyukawa(cody_ctx* ctx, double strength, double soft1
 int cody_set_laser(cody_ctx* ctx, int nK, const double* c, double T);
 /* Precision of the mean-field contraction (CODY_PREC_*). */
 int cody_set_precision(cody_ctx* ctx, int mode);
-/* Replay of captured right-hand sides (no counterpart in the reference, whose right-hand side is
- * a chain of host calls): with enable != 0 (the default) the device-pointer entry points and the
- * RK4 stages capture the launches of one right-hand side per (state, dState, mode) into a HIP
- * graph on first use and replay it; 0 launches every kernel individually. */
+/* Replay of captured launch sequences (no counterpart in the reference, whose right-hand side is
+ * a chain of host calls).  By default the propagators (cody_sil, cody_strang, cody_strang_cheb)
+ * capture a sub-step or step into a HIP graph on first use and replay it, and a single right-hand
+ * side (the device-pointer entry points, the stages of cody_rk4) is launched kernel by kernel:
+ * its few launches cost less than a graph launch.  enable != 0 also replays single right-hand
+ * sides as graphs, one per (state, dState, mode); 0 launches everything individually. */
 int cody_set_graphs(cody_ctx* ctx, int enable);
 /* Zero sources of the mean-field contraction: with enable != 0 (the default) the contraction skips
  * every 64-point (int8 form) or 16-point (FP64 form) block of the contraction index where all pair
