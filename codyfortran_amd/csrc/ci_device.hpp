@@ -58,15 +58,15 @@ __device__ __forceinline__ void ci_rdm_body(int bx, const double2* __restrict__ 
   }
 }
 
-// R = rho1 (rho1^2 + eps)^{-1}: the body of k_ci_reginv, run by the first nth >= 2 NO threads of
-// the workgroup (the others have left it).
+// R = rho1 (rho1^2 + eps)^{-1}: the body of k_ci_reginv.  The first nth >= 2 NO threads of the
+// workgroup do the work; the others only meet its barriers.
 template <int NO>
 __device__ __forceinline__ void reginv_body(const double2* __restrict__ rho1g, double reg,
                                             double2* __restrict__ reginv, int nth) {
   constexpr int n2 = NO * NO;
   __shared__ double2 s1[n2], sA0[n2], sM[2 * n2], sX[n2], sR[n2], sF[NO];
-  const int t = threadIdx.x;
-  for (int i = t; i < n2; i += 64) s1[i] = rho1g[i];
+  const int t = threadIdx.x < nth ? threadIdx.x : 1 << 20;  // (threads past nth only meet the barriers)
+  for (int i = t; i < n2; i += nth) s1[i] = rho1g[i];
   __syncthreads();
   for (int e = t; e < n2; e += nth) {
     const int i = e % NO, j = e / NO;
@@ -264,5 +264,6 @@ __device__ __forceinline__ void gram_reduce_energy_body(int bx, const double* __
     epart[bx] = t;
   }
 }
+
 
 }  // namespace cody

@@ -53,6 +53,36 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// NS sums over the 256 threads of a workgroup, each thread contributing v[s] to sum s, in rounds
+// of at most 8 sums: the round's values go to LDS (sBuf, 8 x 256 doubles) once, each group of 16
+// lanes adds 16 of one sum's values in order, and the 16 group partials meet in a 4-step
+// butterfly.  The result lands in res[s] (LDS).  About ten LDS latencies per round, against six
+// dependent cross-lane steps per sum for NS wave sums.  Fixed order (bit-reproducible).  Every
+// thread calls it.
+template <int NS>
+__device__ __forceinline__ void block_sums256(const double (&v)[NS], double* sBuf, double* res) {
+#pragma unroll
+  for (int s0 = 0; s0 < NS; s0 += 8) {
+    constexpr int kRound = 8;
+#pragma unroll
+    for (int s = 0; s < kRound; ++s)
+      if (s0 + s < NS) sBuf[s * 256 + threadIdx.x] = v[s0 + s];
+    __syncthreads();
+    const int n = NS - s0 < kRound ? NS - s0 : kRound;
+    if (threadIdx.x < n * 16) {
+      const int s = threadIdx.x >> 4, q = threadIdx.x & 15;
+      const double* b = sBuf + s * 256 + 16 * q;
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) acc += b[k];
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if (q == 0) res[s0 + s] = acc;
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ one-body operator at a point
 // FEDVR weak-form kinetic energy -(1/2m)(1/w_x) sum_e sum_b K(a,b)/h f_b
 // (src/Utils/M_Utils_DerivativeFedvr.f90:121-152): the point with global index g (numbered with
@@ -114,6 +144,37 @@ __device__ __forceinline__ double2 kinetic_at_lds(const double2* sF, int xlo, in
       const int j = base + b;
       if (j < 0 || j >= kp.nG) continue;
       const double kab = kp.kref[aa + b * kp.nLoc];
+      const double2 fj = sF[j - xlo];
+      acc.x = fma(kab, fj.x, acc.x);
+      acc.y = fma(kab, fj.y, acc.y);
+    }
+  }
+  return cscale(acc, kp.invh * rw * kp.fac);
+}
+
+// kinetic_at_lds with the reference-element matrix staged in LDS too: sK[a + b nLoc] = K(a, b).
+// Every operand of the stencil then comes from LDS (a global K(a, b) load per neighbour, each
+// waited for before the next, would dominate the one-body kernel's time).  Same operations in the
+// same order as kinetic_at.
+__device__ __forceinline__ double2 kinetic_at_lds_k(const double2* sF, int xlo, int x, const KinParams& kp,
+                                                    const double* sK, double rw) {
+  const int gi = x + kp.xminExcl;
+  const int per = kp.nLoc - 1;
+  double2 acc = cmk(0, 0);
+  int e = gi / per, a = gi % per;
+  for (int pass = 0; pass < 2; ++pass) {
+    int ee = e, aa = a;
+    if (pass == 1) {
+      if (a != 0 || e == 0) break;
+      ee = e - 1;
+      aa = per;
+    }
+    if (ee >= kp.nE) continue;
+    const int base = ee * per - kp.xminExcl;
+    for (int b = 0; b < kp.nLoc; ++b) {
+      const int j = base + b;
+      if (j < 0 || j >= kp.nG) continue;
+      const double kab = sK[aa + b * kp.nLoc];
       const double2 fj = sF[j - xlo];
       acc.x = fma(kab, fj.x, acc.x);
       acc.y = fma(kab, fj.y, acc.y);

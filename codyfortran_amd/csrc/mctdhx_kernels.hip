@@ -170,9 +170,10 @@ __device__ __forceinline__ void onebody_body(int bx, int j, const double2* __res
                                              const double* __restrict__ vext, KinParams kp,
                                              const double* __restrict__ efield, int nG, double2* __restrict__ hphi,
                                              double2* __restrict__ part1) {
-  __shared__ double2 sRed[4][NO];
+  __shared__ double sBuf[8 * 256];
+  __shared__ double sRes[2 * NO];
   __shared__ double2 sF[256 + 2 * kMaxStencilReach];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ double sK[(kMaxStencilReach + 1) * (kMaxStencilReach + 1)];
   const int x = bx * 256 + threadIdx.x;
   const bool in = x < nG;
   const int xc = in ? x : nG - 1;
@@ -183,6 +184,7 @@ __device__ __forceinline__ void onebody_body(int bx, int j, const double2* __res
     const int i = lo + t;
     sF[t] = (i >= 0 && i < nG) ? fj[i] : cmk(0, 0);
   }
+  for (int t = threadIdx.x; t < kp.nLoc * kp.nLoc; t += 256) sK[t] = kp.kref[t];
   double2 phi[NO];
 #pragma unroll
   for (int i = 0; i < NO; ++i) phi[i] = in ? orbs[(size_t)i * nG + xc] : cmk(0, 0);
@@ -191,26 +193,25 @@ __device__ __forceinline__ void onebody_body(int bx, int j, const double2* __res
   __syncthreads();
   double2 h = cmk(0, 0);
   if (in) {
-    h = kinetic_at_lds(sF, lo, xc, kp, 1.0 / wx);
+    h = kinetic_at_lds_k(sF, lo, xc, kp, sK, 1.0 / wx);
     const double2 pj = sF[xc - lo];
     h.x = fma(v, pj.x, h.x);
     h.y = fma(v, pj.y, h.y);
     hphi[(size_t)j * nG + x] = h;
   }
   const double2 hw = cscale(h, wx);
+  double v2[2 * NO];
 #pragma unroll
   for (int i = 0; i < NO; ++i) {
     double2 acc = cmk(0, 0);
     if (in) cfmac(acc, phi[i], hw);
-    acc.x = wave_sum(acc.x);
-    acc.y = wave_sum(acc.y);
-    if (lane == 0) sRed[wave][i] = acc;
+    v2[2 * i] = acc.x;
+    v2[2 * i + 1] = acc.y;
   }
-  __syncthreads();
+  block_sums256<2 * NO>(v2, sBuf, sRes);
   if (threadIdx.x < NO) {
     const int i = threadIdx.x;
-    const double2 acc = cadd(cadd(sRed[0][i], sRed[1][i]), cadd(sRed[2][i], sRed[3][i]));
-    part1[(size_t)bx * NO * NO + i + j * NO] = acc;
+    part1[(size_t)bx * NO * NO + i + j * NO] = cmk(sRes[2 * i], sRes[2 * i + 1]);
   }
 }
 
@@ -644,8 +645,8 @@ __global__ __launch_bounds__(256) void k_rhs_front(const double2* __restrict__ C
   pair_src_t_body(b, orbs, w, nG, nGp, NO, Pc, STt, smax, act);
 }
 
-// After the contraction: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0) and the k-split sum of W^T
-// (the others; none when the contraction wrote W^T itself).
+// After the contraction: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0, eliminated by its first wave)
+// and the k-split sum of W^T (the others; none when the contraction wrote W^T itself).
 template <int NO>
 __global__ __launch_bounds__(256) void k_rhs_wsum(const double2* __restrict__ rho1, double reg,
                                                   double2* __restrict__ reginv, const double* __restrict__ Wpart,

@@ -31,6 +31,12 @@ def build(native: bool = False, build_dir: str | None = None) -> str:
     return os.path.join(bdir, "libcody_oracle.so")
 
 
+def set_compaction(on: bool):
+    """Zero-source compaction of the mean-field sweep (the reference's convolution skips zero
+    sources); on by default.  Results are the same either way; off times the dense sweep."""
+    load().orc_set_compaction(int(bool(on)))
+
+
 def load(path: str | None = None):
     global _LIB
     if _LIB is not None and path is None:
@@ -65,6 +71,8 @@ def load(path: str | None = None):
         "orc_config_codes": (C.c_int, [vp, C.c_int, vp]),
         "orc_apply_single_body": (C.c_int, [vp, vp, C.c_double, vp, vp]),
         "orc_meanfield_w": (C.c_int, [vp, vp, vp]),
+        "orc_set_compaction": (None, [C.c_int]),
+        "orc_active_sources": (C.c_int, [vp, vp]),
         "orc_apply_correlation": (C.c_int, [vp, vp, vp, vp, vp, vp]),
         "orc_fill_h1": (C.c_int, [vp, vp, C.c_double, vp]),
         "orc_fill_h2": (C.c_int, [vp, vp, vp]),
@@ -271,6 +279,11 @@ class System:
         h1 = np.zeros((self.nO, self.nO), dtype=np.complex128, order="F")
         self.lib.orc_apply_single_body(self.h, _vp(o), t, _vp(d), _vp(h1))
         return d, h1
+
+    def active_sources(self, orbs):
+        """Source points the mean-field sweep visits for these orbitals (all of them with the
+        compaction off)."""
+        return int(self.lib.orc_active_sources(self.h, _vp(_c128(orbs))))
 
     def meanfield_w(self, orbs):
         o = _c128(orbs)

@@ -806,14 +806,42 @@ int orc_fill_h1(const orc_sys* s, const cplx* orbs, double t, cplx* h1) {
  * potential: src/Utils/M_Utils_ConvolutionIntegral.f90:71-107 (true distances, see header).
  * Output w is nG x P, column-major.  Pairs of different body types are computed too
  * (callers skip them). */
+/* Zero sources: ConvolutionIntegral_Apply (src/Utils/M_Utils_ConvolutionIntegral.f90:84-93) first
+ * collects the source points with |src(j)| > srcThreshold (default 0) and sums over those only.
+ * With compaction on (the default) the sweep below leaves out every point j where the sources of
+ * all pairs are exactly zero; the reference compacts each pair's source separately, which leaves
+ * out the same points when the orbitals share their support (as in every workload here).  The
+ * skipped terms are exact zeros, so W is the same to the last bit either way; off restores the
+ * dense sweep (for timing). */
+static int g_compact = 1;
+void orc_set_compaction(int on) { g_compact = on != 0; }
+
+/* Number of source points the sweep of orc_meanfield_w visits for these orbitals. */
+int orc_active_sources(const orc_sys* s, const cplx* orbs) {
+  int n = 0;
+  for (int g = 0; g < s->nG; ++g) {
+    int any = !g_compact;
+    for (int l = 0; l < s->nO && !any; ++l)
+      for (int k = l; k < s->nO && !any; ++k)
+        any = conj(orbs[(size_t)l * s->nG + g]) * orbs[(size_t)k * s->nG + g] * s->w[g] != 0.0;
+    n += any;
+  }
+  return n;
+}
+
 int orc_meanfield_w(const orc_sys* s, const cplx* orbs, cplx* W) {
   int nG = s->nG, nO = s->nO, P = nO * (nO + 1) / 2;
   cplx* src = (cplx*)malloc(sizeof(cplx) * (size_t)nG * P); /* [g][p] */
+  int* active = (int*)malloc(sizeof(int) * (size_t)nG);
+  int nAct = 0;
   for (int g = 0; g < nG; ++g) {
-    int p = 0;
+    int p = 0, any = !g_compact;
     for (int l = 0; l < nO; ++l)
-      for (int k = l; k < nO; ++k, ++p)
+      for (int k = l; k < nO; ++k, ++p) {
         src[(size_t)g * P + p] = conj(orbs[(size_t)l * nG + g]) * orbs[(size_t)k * nG + g] * s->w[g];
+        any |= src[(size_t)g * P + p] != 0.0;
+      }
+    if (any) active[nAct++] = g;
   }
 #pragma omp parallel
   {
@@ -822,7 +850,8 @@ int orc_meanfield_w(const orc_sys* s, const cplx* orbs, cplx* W) {
     for (int i = 0; i < nG; ++i) {
       for (int p = 0; p < P; ++p) acc[p] = 0.0;
       const double* row = s->vint + (size_t)i * nG;
-      for (int j = 0; j < nG; ++j) {
+      for (int jj = 0; jj < nAct; ++jj) {
+        const int j = active[jj];
         double v = row[j];
         const cplx* sj = src + (size_t)j * P;
         for (int p = 0; p < P; ++p) acc[p] += v * sj[p];
@@ -831,6 +860,7 @@ int orc_meanfield_w(const orc_sys* s, const cplx* orbs, cplx* W) {
     }
     free(acc);
   }
+  free(active);
   free(src);
   return 0;
 }

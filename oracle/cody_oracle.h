@@ -61,6 +61,10 @@ int orc_config_codes(const orc_sys* s, int bt, unsigned* codes);
 int orc_apply_single_body(const orc_sys* s, const double complex* orbs, double t,
                           double complex* dorbs /* += */, double complex* h1 /* may be NULL */);
 int orc_meanfield_w(const orc_sys* s, const double complex* orbs, double complex* w /* nG x P */);
+/* zero-source compaction of the mean-field sweep (on by default; off: every source point) and the
+ * number of source points the sweep visits for given orbitals */
+void orc_set_compaction(int on);
+int orc_active_sources(const orc_sys* s, const double complex* orbs);
 int orc_apply_correlation(const orc_sys* s, const double complex* orbs, const double complex* rdm1,
                           const double complex* rdm2, double complex* dorbs /* += */,
                           double complex* h2 /* may be NULL */);
