@@ -42,6 +42,8 @@ PREC_CODE = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "int8_sliced": 3}
 OPERAND_BYTES = {"bf16x3": 6.0, "int8_sliced": 5.0}
 KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_w"}
 TOL = 1e-9
+# forms whose contraction skips the k blocks of all-zero pair sources (cody_set_source_skip)
+SKIPPING = ("fp64", "int8_sliced")
 
 
 def parse():
@@ -284,7 +286,7 @@ def propagated_energy_by_form(args, wlm, wl, forms):
             "energy_rel_err_vs_fp64": {f: abs(e - ref) / abs(ref) for f, e in energy.items() if f != "fp64"}}
 
 
-def pmc_traffic(kname: str, workload: str):
+def pmc_traffic(kname: str, workload: str, skip: bool = False):
     """HBM bytes per launch of `kname` from the committed hardware-counter summary (rocprofv3
     --pmc FETCH_SIZE / WRITE_SIZE passes over `bench.py --pmc-probe`, scaled by a 256 MiB copy);
     None when no summary exists for this workload."""
@@ -293,8 +295,9 @@ def pmc_traffic(kname: str, workload: str):
         return None
     with open(path) as f:
         d = json.load(f)
+    key = "skip" if skip else "dense"
     for k, v in d["kernels"].items():
-        if k.split("<")[0] == kname:
+        if k.split("<")[0] == kname and v.get("mode", "dense") == key:
             return v["hbm_bytes"]
     return None
 
@@ -542,9 +545,18 @@ def main():
     assert sptr != 0
 
     flops = 2.0 * wl.nG ** 2 * 2 * wl.pairs  # pair symmetry exploited: 2 N_g^2 2P
+    nGp = 256 * ((wl.nG + 255) // 256)
+    Pc = 16 * ((2 * wl.pairs + 15) // 16)
+    # source points of the seeded state where some pair source conj(phi_l) phi_k w is non-zero:
+    # the only columns of V the zero-source skip contracts (the reference's convolution leaves the
+    # others out too)
+    _, orbs0 = wlm.split_state(state, wl.nC, wl.nG, wl.M)
+    src = np.abs(orbs0)[:, :, None] * np.abs(orbs0)[:, None, :] * w[:, None, None]
+    n_src = int(np.count_nonzero(src.reshape(wl.nG, -1).any(axis=1)))
 
-    def time_form(prec):
+    def time_form(prec, skip):
         ctx.set_precision(PREC_CODE[prec])
+        ctx.set_source_skip(skip)
         for _ in range(args.warmup):
             ctx.time_derivative_dev(d_state.data_ptr(), d_out.data_ptr(), 0.0, sptr)
         torch.cuda.synchronize()
@@ -559,34 +571,38 @@ def main():
         wall = time.perf_counter() - t0
         ms = start.elapsed_time(stop) / args.steps
         out = d_out.cpu().numpy().view(np.complex128).copy()
-        _, orbs = wlm.split_state(state, wl.nC, wl.nG, wl.M)
-        w_field = ctx.meanfield_w(orbs)
+        w_field = ctx.meanfield_w(orbs0)
         energy = ctx.energy(state)
-        ctx.upload_state(state)
+        ctx.upload_state(state)  # (the profiles below run on the context's own state)
         kernels = ctx.profile_rhs(args.profile_iters)
         kname = KERNEL[prec]
         kms = ctx.profile_contraction(args.profile_iters)
-        traffic = pmc_traffic(kname, wl.name)
+        cols = n_src if (skip and prec in SKIPPING) else nGp  # contraction columns the form needs
+        useful = 2.0 * wl.nG * (n_src if (skip and prec in SKIPPING) else wl.nG) * 2 * wl.pairs
         if BOUND[prec] == "mfma":
-            work, unit = flops, "TFLOP/s"
-            ach = flops / (kms * 1e-3) / 1e12 if kms else None
+            work, unit = useful, "TFLOP/s"
+            ach = useful / (kms * 1e-3) / 1e12 if kms else None
         else:
-            nGp = 256 * ((wl.nG + 255) // 256)
-            Pc = 16 * ((2 * wl.pairs + 15) // 16)
             eb = OPERAND_BYTES[prec]
-            work = eb * nGp * nGp + eb * Pc * nGp + 8.0 * Pc * nGp  # V and S operands once, W^T once
+            work = eb * nGp * cols + eb * Pc * cols + 8.0 * Pc * nGp  # V and S operands once, W^T once
             unit = "GB/s"
             ach = work / (kms * 1e-3) / 1e9 if kms else None
+        traffic = pmc_traffic(kname, wl.name, skip and prec in SKIPPING)
         roof = {"bound": BOUND[prec], "kernel": kname, "achieved": ach, "peak": PEAK[prec], "unit": unit,
                 "frac": ach / PEAK[prec] if ach else None, "traffic": traffic,
                 ("flop_per_launch" if BOUND[prec] == "mfma" else "algorithmic_bytes_per_launch"): work,
-                "useful_flop_per_launch": flops, "avg_launch_ms": kms,
-                "traffic_source": f"profiles/pmc_traffic_{wl.name}.json (HBM bytes per launch)" if traffic else None}
-        return {"precision": prec, "ms_per_step": ms, "value": 1000.0 / ms, "wall_s_timed": wall,
-                "roofline": roof, "kernels_ms": kernels, "dstate": out, "W": w_field, "energy": energy}
+                "useful_flop_per_launch": useful, "avg_launch_ms": kms,
+                "traffic_source": (f"profiles/pmc_traffic_{wl.name}.json (HBM bytes per launch)"
+                                   if traffic else None)}
+        return {"precision": prec, "zero_source_skip": bool(skip and prec in SKIPPING), "ms_per_step": ms,
+                "value": 1000.0 / ms, "wall_s_timed": wall, "roofline": roof, "kernels_ms": kernels,
+                "dstate": out, "W": w_field, "energy": energy}
 
     forms = ["fp64", "int8_sliced", "split_fp32", "bf16x3"] if args.precision == "auto" else [args.precision]
-    results = {f: time_form(f) for f in forms}
+    results = {f: time_form(f, True) for f in forms}
+    # the exact forms also without the zero-source skip (every source point contracted)
+    dense = {f + "_dense": time_form(f, False) for f in forms if f in SKIPPING}
+    ctx.set_source_skip(True)
     ctx.set_precision(0)
 
     errs = {}
@@ -600,9 +616,10 @@ def main():
         e_orc = s_orc.energy(state)
         errs = {f: {"rhs": state_rel(r["dstate"], d_orc, wl.nC, wts, wl.M),
                     "W": rel(r["W"], w_orc),
-                    "energy": abs(r["energy"] - e_orc) / abs(e_orc)} for f, r in results.items()}
+                    "energy": abs(r["energy"] - e_orc) / abs(e_orc)} for f, r in {**results, **dense}.items()}
     # a form qualifies when every reported quantity is within TOL of the FP64 oracle
-    ok = {f: bool(max(errs[f].values()) <= TOL) if f in errs else (f == "fp64") for f in forms}
+    ok = {f: bool(max(errs[f].values()) <= TOL) if f in errs else f.startswith("fp64")
+          for f in list(forms) + list(dense)}
     valid = [f for f in forms if ok[f]]
     best = max(valid, key=lambda f: results[f]["value"]) if valid else forms[0]
     b = results[best]
@@ -622,13 +639,24 @@ def main():
                 "the FEDVR weights; complex Gaussian CI vector, normalised",
         "config": {"workload": wl.name, "nG": wl.nG, "elements": wl.nE, "gll_points": wl.nLoc,
                    "N": wl.N, "M": wl.M, "nC": wl.nC, "pairs": wl.pairs, "precision": best,
-                   "regularization": wl.reg, "hip_graphs": True},
+                   "regularization": wl.reg, "zero_source_skip": b["zero_source_skip"],
+                   "launches": "kernel by kernel on one stream (HIP graphs for propagator steps)"},
+        "zero_source_skip": {
+            "what": "the contraction leaves out the k blocks where every pair source conj(phi_l) phi_k w is "
+                    "exactly zero, as the reference's convolution leaves out zero sources "
+                    "(M_Utils_ConvolutionIntegral.f90:84-93); the skipped products are exact zeros, so the "
+                    "results equal the dense contraction's up to summation order; cpu_baseline.value uses "
+                    "the same compaction",
+            "source_points_nonzero": n_src, "source_points": wl.nG,
+            "dense": {f: {"value": r["value"], "ms_per_step": r["ms_per_step"], "meets_1e-9": ok[f],
+                          "rel_err": errs.get(f), "roofline": r["roofline"]} for f, r in dense.items()},
+        },
         "roofline": b["roofline"],
         "cpu_baseline": cb,
         "rel_err_vs_fp64_oracle": errs,
         "forms": {f: {"value": r["value"], "ms_per_step": r["ms_per_step"], "rel_err": errs.get(f),
-                      "meets_1e-9": ok[f], "roofline": r["roofline"],
-                      "kernels_ms": r["kernels_ms"]} for f, r in results.items()},
+                      "meets_1e-9": ok[f], "zero_source_skip": r["zero_source_skip"], "roofline": r["roofline"],
+                      "kernels_ms": r["kernels_ms"]} for f, r in {**results, **dense}.items()},
     }
     if args.prop_steps > 0:
         result["propagation"] = propagation_measurement(args, wlm, wl, state, PREC_CODE[best])
