@@ -71,6 +71,9 @@ def parse():
     p.add_argument("--adjoint-only", action="store_true",
                    help="run only the laser-gradient measurement (no finite-difference checks unless "
                         "--adjoint-fd > 0): the command of the adjoint kernel profile")
+    p.add_argument("--conservation", action="store_true",
+                   help="run only the conservation study (energy and norm drift against the rho^-1 "
+                        "regularisation, dt and the SIL tolerance) and print it as one JSON line")
     p.add_argument("--pmc-probe", action="store_true",
                    help="short run for hardware-counter passes: a 256 MiB device copy (the unit "
                         "of the byte counters) and a few launches of the FP64 and int8 "
@@ -262,6 +265,55 @@ def adjoint_measurement(args, wlm, wl, checks=True):
         "full_tape_in_hbm": tape_info,
         "cpu_baseline": cpu,
     }
+
+
+def conservation_measurement(args, wlm, wl):
+    """Where norm and energy conservation to 1e-10 hold at the workload's size, from the smooth
+    seeded state over t = 2 (device-resident, FP64 contraction):
+    - RK4 on the full right-hand side for regularisations eps = 1e-8, 1e-10, 1e-12 (the rho^-1
+      regularisation of S_Method_Mb_OrbBased.f90:737-744 changes the equations of motion by
+      O(eps), so the energy is conserved only up to a floor set by eps) and steps dt = 0.002,
+      0.001, 0.0005 (below the explicit stability limit ~3.4e-3 of the 4096-point grid);
+    - the consistent Strang split (SIL Krylov 16, tolerance 1e-12) for dt = 0.04 ... 0.005
+      (second order in dt);
+    - the consistent split at dt = 0.02 for SIL tolerances 1e-8, 1e-10, 1e-12 (the SIL acts on
+      the orbitals only: its tolerance shows in their orthonormality, the CI norm is the RK4
+      part's)."""
+    import dataclasses
+
+    T = 2.0
+    out = {"t_end": T, "state": "smooth seeded state (seed 1234)", "rk4_full": [], "strang_consistent_dt": [],
+           "strang_consistent_tol": []}
+
+    def run(ctx, x, w, steps, dt, fn):
+        smooth = wlm.smooth_state(x, w, wl.N, wl.M, seed=1234)
+        return _propagate(ctx, wl, w, smooth, steps, dt, fn, chunk=max(50, steps // 4))
+
+    for eps in (1e-8, 1e-10, 1e-12):
+        ctx, x, w = wlm.make_context(dataclasses.replace(wl, reg=eps))
+        for dt in (0.002, 0.001, 0.0005):
+            def rk4(t0, t1):
+                ctx.rk4(t0, t1, full=True)
+                return True
+            r = run(ctx, x, w, int(round(T / dt)), dt, rk4)
+            out["rk4_full"].append({"eps": eps, **r})
+            print(f"conservation: rk4 eps {eps:g} dt {dt:g}: energy drift {r.get('energy_drift_rel')}", flush=True)
+        ctx.close()
+    ctx, x, w = wlm.make_context(wl)
+    ctx.set_split(True)
+    for dt in (0.04, 0.02, 0.01, 0.005):
+        r = run(ctx, x, w, int(round(T / dt)), dt,
+                lambda t0, t1: ctx.strang(t0, t1, krylov_dim=16, tol=1e-12, max_restarts=8, metric=True))
+        out["strang_consistent_dt"].append({"eps": wl.reg, "tol": 1e-12, **r})
+        print(f"conservation: consistent split dt {dt:g}: energy drift {r.get('energy_drift_rel')}", flush=True)
+    for tol in (1e-8, 1e-10, 1e-12):
+        r = run(ctx, x, w, int(round(T / 0.02)), 0.02,
+                lambda t0, t1: ctx.strang(t0, t1, krylov_dim=16, tol=tol, max_restarts=8, metric=True))
+        out["strang_consistent_tol"].append({"eps": wl.reg, "tol": tol, **r})
+        print(f"conservation: consistent split tol {tol:g}: orthonormality {r.get('orbital_orthonormality_error')}",
+              flush=True)
+    ctx.close()
+    return out
 
 
 def propagated_energy_by_form(args, wlm, wl, forms):
@@ -529,6 +581,9 @@ def main():
     wl = wlm.WORKLOADS[args.workload]
     if args.pmc_probe:
         pmc_probe(torch, wlm, wl)
+        return
+    if args.conservation:
+        print(json.dumps({"conservation": conservation_measurement(args, wlm, wl)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_measurement(args, wlm, wl, checks=args.adjoint_fd > 0)}), flush=True)
