@@ -43,12 +43,14 @@ def _compile(src: str) -> str:
 
 FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_SOURCES = ["M_Utils_Types.f90", "M_CodyHip.f90", "M_Method.f90", "M_Propagator.f90",
-                   "M_Grid.f90", "M_Method_Mb_OrbBased_MctdhxHip.f90",
-                   "S_Method_Mb_OrbBased_MctdhxHip.f90"]
+                   "M_Grid.f90", "M_Method_Mb_OrbBased.f90", "M_Method_Mb_OrbBased_MctdhxHip.f90",
+                   "S_Method_Mb_OrbBased_MctdhxHip.f90", "M_IntegratorList.f90", "M_IntegratorList_Hip.f90",
+                   "S_IntegratorList_Hip.f90", "S_IntegratorList.f90"]
 # test programs: (source, executable) -- each links the library and the objects above
 FORTRAN_DRIVERS = [
     (os.path.join(ROOT, "tests", "fortran", "T_MctdhxHip_Rhs.f90"), "t_mctdhx_hip"),
     (os.path.join(ROOT, "tests", "fortran", "T_StateFiles.f90"), "t_state_files"),
+    (os.path.join(ROOT, "tests", "fortran", "T_MctdhxHip_Split.f90"), "t_mctdhx_split"),
 ]
 BIN = os.path.join(HERE, "bin")
 FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")

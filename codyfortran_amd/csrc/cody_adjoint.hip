@@ -118,11 +118,6 @@ __global__ __launch_bounds__(256) void k_adj_ci_start(const double2* __restrict_
   }
 }
 
-__global__ void k_cadd(double2* __restrict__ y, const double2* __restrict__ x, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = cadd(y[i], x[i]);
-}
-
 // transition densities lin[code] = sum f conj(bra_jc) ket_ic through the density tables (the
 // layout of k_ci_rdm): h1b(i,j) at i + j nO, h2b unpacked to its four antisymmetric positions
 __global__ __launch_bounds__(256) void k_adj_transition(const double2* __restrict__ bra, const double2* __restrict__ ket,
@@ -1017,7 +1012,7 @@ int enqueue_vjp(cody_ctx* c, int part, const double2* state, const double2* lam,
   // CI projection and H-apply
   hipLaunchKernelGGL(k_adj_ci_start, dim3(1), dim3(256), 0, st, C, lam, c->d_dCraw, nC, a.D, grad);
   launch_ci_apply(a.D, nC, nO, c->Pc, c->d_G, c->d_h1, c->dci, a.tmpC, st);  // H^H D (H Hermitian)
-  hipLaunchKernelGGL(k_cadd, dim3(blocks(nC)), dim3(256), 0, st, grad, a.tmpC, nC);
+  launch_accumulate(grad, a.tmpC, nC, st);
   const int np = nO * (nO - 1) / 2, items = n2 + np * np;
   hipLaunchKernelGGL(k_adj_transition, dim3((items + 3) / 4), dim3(256), sizeof(double2) * 2 * nC, st, C, a.D, nC,
                      nO, c->dci.r1_ptr, c->dci.r1, c->dci.r2_ptr, c->dci.r2, a.h1b, a.h2b);

@@ -618,16 +618,27 @@ int cody_time_derivative_nonlin_dev(cody_ctx* c, const void* s, void* d, double 
   return rhs_dev(c, s, d, t, stream, kModeNonLin);
 }
 
-int cody_time_derivative(cody_ctx* c, const double* state, double* dstate, double t) {
-  if (!c || !state || !dstate) return fail(CODY_ERR_ARG, "cody_time_derivative: null argument");
+// One right-hand side of host arrays through the context's buffers (synchronous).
+static int rhs_host(cody_ctx* c, const double* state, double* dstate, double t, int mode, const char* who) {
+  if (!c || !state || !dstate) return fail(CODY_ERR_ARG, (std::string(who) + ": null argument").c_str());
   CHK(check_ready(c));
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpyAsync(c->d_tmp, state, sizeof(double2) * c->nS, hipMemcpyHostToDevice, c->stream));
-  CHK(enqueue_rhs(c, c->d_tmp, c->d_tmp2, t, kModeFull, c->stream, nullptr, nullptr));
+  CHK(enqueue_rhs(c, c->d_tmp, c->d_tmp2, t, mode, c->stream, nullptr, nullptr));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(dstate, c->d_tmp2, sizeof(double2) * c->nS, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return CODY_OK;
+}
+
+int cody_time_derivative(cody_ctx* c, const double* state, double* dstate, double t) {
+  return rhs_host(c, state, dstate, t, kModeFull, "cody_time_derivative");
+}
+int cody_time_derivative_orbs_lin(cody_ctx* c, const double* state, double* dstate, double t) {
+  return rhs_host(c, state, dstate, t, kModeLin, "cody_time_derivative_orbs_lin");
+}
+int cody_time_derivative_nonlin(cody_ctx* c, const double* state, double* dstate, double t) {
+  return rhs_host(c, state, dstate, t, kModeNonLin, "cody_time_derivative_nonlin");
 }
 
 int cody_energy(cody_ctx* c, const double* state, double t, double* energy) {
@@ -709,6 +720,31 @@ int cody_apply_single_body(cody_ctx* c, const double* orbs, double t, double* do
   return CODY_OK;
 }
 
+int cody_apply_single_body_acc(cody_ctx* c, const double* orbs, double t, double* dorbs, double* h1) {
+  if (!c || !orbs || !dorbs) return fail(CODY_ERR_ARG, "cody_apply_single_body_acc: null argument");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const int n2 = c->nO * c->nO, nOG = c->nO * c->nG;
+  double2* d_orbs = c->d_tmp + c->nC;
+  double2* d_in = c->d_tmp2 + c->nC;
+  HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * nOG, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_in, dorbs, sizeof(double2) * nOG, hipMemcpyHostToDevice, st));
+  if (c->nLaser > 0) launch_set_scalar(c->d_efield, laser_field(c, t), st);
+  launch_onebody(d_orbs, c->d_w, c->d_x, c->d_vext, c->kin, efield_src(c), c->nG, c->nO, c->d_draw, c->d_part1, st);
+  launch_accumulate(c->d_draw, d_in, nOG, st);
+  if (h1) {
+    // h1 from the accumulated result, as the reference forms it
+    launch_overlap_parts(d_orbs, c->d_w, c->d_draw, c->nG, c->nO, c->d_part1, st);
+    launch_sum_parts(c->d_part1, (c->nG + 255) / 256, n2, n2, c->d_h1, st);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dorbs, c->d_draw, sizeof(double2) * nOG, hipMemcpyDeviceToHost, st));
+  if (h1) HIPCHK(hipMemcpyAsync(h1, c->d_h1, sizeof(double2) * n2, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CODY_OK;
+}
+
 int cody_densities(cody_ctx* c, const double* coeffs, double* rdm1, double* rdm2, double* reginv) {
   if (!c || !coeffs) return fail(CODY_ERR_ARG, "cody_densities: null argument");
   HIPCHK(hipSetDevice(c->device));
@@ -724,6 +760,40 @@ int cody_densities(cody_ctx* c, const double* coeffs, double* rdm1, double* rdm2
   return CODY_OK;
 }
 
+// The buffers of the interaction-potential pieces: c->d_src (Pc / 2 sources) and the FP64
+// operand of V (kept current by the FP64 form; tiled here for the others).
+static int prepare_potential(cody_ctx* c, hipStream_t st) {
+  if (!c->d_src) CHK(ctx_alloc((void**)&c->d_src, sizeof(double2) * c->nG * (c->Pc / 2)));
+  if (c->precision != CODY_PREC_FP64 || !c->d_Vt) {
+    if (!c->d_Vt) CHK(ctx_alloc((void**)&c->d_Vt, sizeof(double) * c->nGp * c->nGp));
+    launch_tile_f64(c->d_V, c->nGp, c->d_Vt, st);
+  }
+  return CODY_OK;
+}
+
+int cody_apply_interaction(cody_ctx* c, const double* orbs, int i2, int j2, double t, double* dorbs) {
+  (void)t;  // the soft-Yukawa interaction does not depend on time
+  if (!c || !orbs || !dorbs) return fail(CODY_ERR_ARG, "cody_apply_interaction: null argument");
+  if (i2 < 0 || i2 >= c->nO || j2 < 0 || j2 >= c->nO)
+    return fail(CODY_ERR_ARG, "cody_apply_interaction: orbital index out of range");
+  CHK(check_ready(c));
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  CHK(prepare_potential(c, st));
+  const int nOG = c->nO * c->nG;
+  double2* d_orbs = c->d_tmp + c->nC;
+  HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * nOG, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->d_draw, dorbs, sizeof(double2) * nOG, hipMemcpyHostToDevice, st));
+  launch_pair_src_one(d_orbs, c->d_w, c->nG, i2, j2, c->d_src, st);
+  launch_tile_src(c->d_src, 1, c->nG, c->nGp, c->Pc, c->d_STt, st);
+  CHK(launch_mf_f64_w(c->d_Vt, c->d_STt, c->d_Wpart, c->d_WT, c->nGp, c->Pc, c->wsplit, st));
+  launch_apply_potential(c->d_WT, c->nGp, d_orbs, c->nG, c->nO, c->d_draw, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dorbs, c->d_draw, sizeof(double2) * nOG, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CODY_OK;
+}
+
 int cody_interaction_potential(cody_ctx* c, int nSrc, const double* src, double* pot) {
   if (!c || nSrc < 0 || (nSrc > 0 && (!src || !pot)))
     return fail(CODY_ERR_ARG, "cody_interaction_potential: bad arguments");
@@ -731,12 +801,7 @@ int cody_interaction_potential(cody_ctx* c, int nSrc, const double* src, double*
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
   const int chunk = c->Pc / 2;  // sources per contraction
-  if (!c->d_src) CHK(ctx_alloc((void**)&c->d_src, sizeof(double2) * c->nG * chunk));
-  // the FP64 operand of V (kept current by the FP64 form; tiled here for the others)
-  if (c->precision != CODY_PREC_FP64 || !c->d_Vt) {
-    if (!c->d_Vt) CHK(ctx_alloc((void**)&c->d_Vt, sizeof(double) * c->nGp * c->nGp));
-    launch_tile_f64(c->d_V, c->nGp, c->d_Vt, st);
-  }
+  CHK(prepare_potential(c, st));
   std::vector<double> wt((size_t)c->Pc * c->nGp);
   for (int s0 = 0; s0 < nSrc; s0 += chunk) {
     const int ns = std::min(chunk, nSrc - s0);

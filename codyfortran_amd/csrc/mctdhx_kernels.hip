@@ -506,6 +506,44 @@ void launch_zero(double2* p, int n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
 }
 
+// ------------------------------------------------------------------ operator pieces
+// y += x elementwise
+__global__ void k_accumulate(double2* __restrict__ y, const double2* __restrict__ x, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = cadd(y[i], x[i]);
+}
+
+void launch_accumulate(double2* y, const double2* x, int n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, st, y, x, n);
+}
+
+// One pair source src = conj(phi_a) phi_b w (FillInteractionSrc, S_SysInteraction_Linear.f90:57-72).
+__global__ void k_pair_src_one(const double2* __restrict__ orbs, const double* __restrict__ w, int nG, int a, int b,
+                               double2* __restrict__ src) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < nG) src[x] = cscale(cmul(cconj(orbs[(size_t)a * nG + x]), orbs[(size_t)b * nG + x]), w[x]);
+}
+
+void launch_pair_src_one(const double2* orbs, const double* w, int nG, int a, int b, double2* src, hipStream_t st) {
+  hipLaunchKernelGGL(k_pair_src_one, dim3((nG + 255) / 256), dim3(256), 0, st, orbs, w, nG, a, b, src);
+}
+
+// f(:, i) += pot * orbs(:, i) for every orbital i, pot = WT(0, :) + i WT(1, :), the potential of
+// the first source of a contraction (MultiplyPotentialOnOrbRange with MultiplyWithInteractionPotential,
+// S_Method_Mb_OrbBased.f90:545-572, S_SysInteraction_Linear.f90:79-88).
+__global__ void k_apply_potential(const double* __restrict__ WT, int nGp, const double2* __restrict__ orbs, int nG,
+                                  double2* __restrict__ f) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nG) return;
+  const size_t o = (size_t)blockIdx.y * nG + x;
+  f[o] = cadd(f[o], cmul(cmk(WT[x], WT[(size_t)nGp + x]), orbs[o]));
+}
+
+void launch_apply_potential(const double* WT, int nGp, const double2* orbs, int nG, int nO, double2* f,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_apply_potential, dim3((nG + 255) / 256, nO), dim3(256), 0, st, WT, nGp, orbs, nG, f);
+}
+
 void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a1, double a2,
                             double damp, double* V, hipStream_t st) {
   dim3 grid((nGp + 255) / 256, nGp);

@@ -55,6 +55,13 @@ struct KinLaunch {
 };
 
 void launch_zero(double2* p, int n, hipStream_t st);
+// y += x (n complex values)
+void launch_accumulate(double2* y, const double2* x, int n, hipStream_t st);
+// src = conj(phi_a) phi_b w: one pair source (nG values)
+void launch_pair_src_one(const double2* orbs, const double* w, int nG, int a, int b, double2* src, hipStream_t st);
+// f(:, i) += (WT(0, :) + i WT(1, :)) orbs(:, i) for the nO orbitals: the first source's potential applied
+void launch_apply_potential(const double* WT, int nGp, const double2* orbs, int nG, int nO, double2* f,
+                            hipStream_t st);
 void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a1, double a2,
                             double damp, double* V, hipStream_t st);
 void launch_ci_density(const double2* C, int nC, int nO, double reg, const DevCiTables& t,

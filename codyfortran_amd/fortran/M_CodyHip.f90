@@ -115,6 +115,25 @@ module M_CodyHip
       integer(c_int) :: r
     end function
 
+    !> The split-step vector fields TimeDerivativeOrbsLin and TimeDerivativeCoeffsPlusOrbsNonLin.
+    function cody_time_derivative_orbs_lin(ctx, state, dstate, t) &
+        bind(C, name="cody_time_derivative_orbs_lin") result(r)
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: state(*), dstate(*)
+      real(c_double), value :: t
+      integer(c_int) :: r
+    end function
+
+    function cody_time_derivative_nonlin(ctx, state, dstate, t) &
+        bind(C, name="cody_time_derivative_nonlin") result(r)
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: state(*), dstate(*)
+      real(c_double), value :: t
+      integer(c_int) :: r
+    end function
+
     function cody_energy(ctx, state, t, e) bind(C, name="cody_energy") result(r)
       import :: c_ptr, c_int, c_double, c_double_complex
       type(c_ptr), value :: ctx
@@ -188,6 +207,26 @@ module M_CodyHip
       integer(c_int) :: r
     end function
 
+    !> Short iterative Lanczos on the linear split part and RK4 on the nonlinear part (full /= 0:
+    !> on the full right-hand side), both on the context's device state.
+    function cody_sil(ctx, t0, t1, krylovDim, tol, maxRestarts, useMetric, converged) &
+        bind(C, name="cody_sil") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t0, t1, tol
+      integer(c_int), value :: krylovDim, maxRestarts, useMetric
+      integer(c_int) :: converged
+      integer(c_int) :: r
+    end function
+
+    function cody_rk4(ctx, t0, t1, full) bind(C, name="cody_rk4") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t0, t1
+      integer(c_int), value :: full
+      integer(c_int) :: r
+    end function
+
     !> Operator pieces (the reference's procedure-pointer semantics) and the CI densities.
     function cody_densities(ctx, coeffs, rdm1, rdm2, reginv) bind(C, name="cody_densities") result(r)
       import :: c_ptr, c_int, c_double_complex
@@ -215,6 +254,24 @@ module M_CodyHip
       import :: c_ptr, c_int, c_double_complex
       type(c_ptr), value :: ctx
       complex(c_double_complex) :: orbs(*), dorbs(*)
+      integer(c_int) :: r
+    end function
+
+    function cody_apply_single_body_acc(ctx, orbs, t, dorbs, h1) bind(C, name="cody_apply_single_body_acc") result(r)
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: orbs(*), dorbs(*), h1(*)
+      real(c_double), value :: t
+      integer(c_int) :: r
+    end function
+
+    !> i2, j2 are 0-based orbital indices
+    function cody_apply_interaction(ctx, orbs, i2, j2, t, dorbs) bind(C, name="cody_apply_interaction") result(r)
+      import :: c_ptr, c_int, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: orbs(*), dorbs(*)
+      integer(c_int), value :: i2, j2
+      real(c_double), value :: t
       integer(c_int) :: r
     end function
 

@@ -35,9 +35,48 @@ module M_Method_Mb_OrbBased_MctdhxHip
   real(R64) :: MctdhxHip_laserBound = 0.0_R64  !< sum |c_k| of the laser coefficients: a bound of |E(t)|
 
   interface
-    !> Binds Method_Setup, Method_TimeDerivative and Method_GetEnergy to the GPU leaf.
+    !> Binds Method_Setup, Method_TimeDerivative and Method_GetEnergy, and the orbital-based
+    !> method's pointers (the split-step vector fields Method_Mb_OrbBased_TimeDerivativeOrbsLin and
+    !> Method_Mb_OrbBased_TimeDerivativeCoeffsPlusOrbsNonLin, ApplySingleBodyOp, ApplyInteractionOp,
+    !> ApplyCorrelationOp) to the GPU leaf: the pointers S_Method_Mb_OrbBased.f90:85-109 binds.
     module subroutine Method_Mb_OrbBased_MctdhxHip_Fabricate(input)
       type(T_MctdhxHip_Input), intent(in) :: input
+    end subroutine
+    !> Method_TimeDerivative (S_Method_Mb_OrbBased_Mctdhx.f90:101-149) on the GPU, host arrays.
+    module subroutine MctdhxHip_TimeDerivative(dState, state, time)
+      complex(R64), intent(out), contiguous, target :: dState(:)
+      complex(R64), intent(in), contiguous, target :: state(:)
+      real(R64), intent(in) :: time
+    end subroutine
+    !> The split parts of the GPU leaf: TimeDerivativeOrbsLin (S_Method_Mb_OrbBased_Mctdhx.f90:152-177)
+    !> and TimeDerivativeCoeffsPlusOrbsNonLin (:180-226; with consistentSplitQ, the full right-hand
+    !> side minus the linear part).
+    module subroutine MctdhxHip_TimeDerivativeOrbsLin(dState, state, time)
+      complex(R64), intent(out), contiguous, target :: dState(:)
+      complex(R64), intent(in), contiguous, target :: state(:)
+      real(R64), intent(in) :: time
+    end subroutine
+    module subroutine MctdhxHip_TimeDerivativeCoeffsPlusOrbsNonLin(dState, state, time)
+      complex(R64), intent(out), contiguous, target :: dState(:)
+      complex(R64), intent(in), contiguous, target :: state(:)
+      real(R64), intent(in) :: time
+    end subroutine
+    !> ApplySingleBodyOp (S_Method_Mb_OrbBased.f90:976-1034): dOrbs += (T + V + E(t) x) orbs and
+    !> h1_(i, j) = <phi_i|dOrbs_j> of the accumulated dOrbs.
+    module subroutine MctdhxHip_ApplySingleBodyOp(dOrbs, orbs, time, h1_)
+      complex(R64), intent(inout), contiguous, target :: dOrbs(:, :)
+      complex(R64), intent(in), contiguous, target :: orbs(:, :)
+      real(R64), intent(in) :: time
+      complex(R64), intent(out), allocatable, optional :: h1_(:, :)
+    end subroutine
+    !> ApplyInteractionOp (S_Method_Mb_OrbBased.f90:491-542): dOrbs(:, i) += W orbs(:, i) for every
+    !> orbital i, W the interaction potential of conj(phi_i2) phi_j2 w.
+    module subroutine MctdhxHip_ApplyInteractionOp(dOrbs, orbs, i2, j2, time)
+      complex(R64), intent(inout), contiguous :: dOrbs(:, :)
+      complex(R64), intent(in), contiguous :: orbs(:, :)
+      integer(I32), intent(in) :: i2
+      integer(I32), intent(in) :: j2
+      real(R64), intent(in) :: time
     end subroutine
     !> Binds Propagator_Setup / Propagator_Propagate to the device split step order 2
     !> (SIL on the linear part, RK4 on the nonlinear part).

@@ -7,11 +7,17 @@ contains
 
   module subroutine Method_Mb_OrbBased_MctdhxHip_Fabricate(input)
     use M_Method
+    use M_Method_Mb_OrbBased
     type(T_MctdhxHip_Input), intent(in) :: input
     MctdhxHip_input = input
     Method_Setup => Setup
-    Method_TimeDerivative => TimeDerivative
+    Method_TimeDerivative => MctdhxHip_TimeDerivative
     Method_GetEnergy => GetEnergy
+    Method_Mb_OrbBased_TimeDerivativeOrbsLin => MctdhxHip_TimeDerivativeOrbsLin
+    Method_Mb_OrbBased_TimeDerivativeCoeffsPlusOrbsNonLin => MctdhxHip_TimeDerivativeCoeffsPlusOrbsNonLin
+    Method_Mb_OrbBased_ApplySingleBodyOp => MctdhxHip_ApplySingleBodyOp
+    Method_Mb_OrbBased_ApplyInteractionOp => MctdhxHip_ApplyInteractionOp
+    Method_Mb_OrbBased_ApplyCorrelationOp => MctdhxHip_ApplyCorrelationOp
   end subroutine
 
   module subroutine Propagator_SplitStepHip_Fabricate
@@ -63,14 +69,75 @@ contains
     Method_state(:) = (0.0_R64, 0.0_R64)
   end subroutine
 
-  !> Method_TimeDerivative (S_Method_Mb_OrbBased_Mctdhx.f90:101-149) on the GPU.
-  subroutine TimeDerivative(dState, state, time)
+  subroutine CheckStateSizes(dState, state, where)
+    complex(R64), intent(in) :: dState(:), state(:)
+    character(len=*), intent(in) :: where
+    if (.not. c_associated(MctdhxHip_ctx)) error stop "MctdhxHip: Method_Setup has not run"
+    if (size(state) /= MctdhxHip_nState .or. size(dState) /= MctdhxHip_nState) then
+      write (*, '(A)') "MctdhxHip "//where//": state size mismatch"
+      error stop "MI355X MCTDHX: state size mismatch"
+    end if
+  end subroutine
+
+  module subroutine MctdhxHip_TimeDerivative(dState, state, time)
     complex(R64), intent(out), contiguous, target :: dState(:)
     complex(R64), intent(in), contiguous, target :: state(:)
     real(R64), intent(in) :: time
-    if (size(state) /= MctdhxHip_nState .or. size(dState) /= MctdhxHip_nState) &
-      error stop "MctdhxHip TimeDerivative: state size mismatch"
+    call CheckStateSizes(dState, state, "TimeDerivative")
     call CodyHip_Check(cody_time_derivative(MctdhxHip_ctx, state, dState, time), "time_derivative")
+  end subroutine
+
+  module subroutine MctdhxHip_TimeDerivativeOrbsLin(dState, state, time)
+    complex(R64), intent(out), contiguous, target :: dState(:)
+    complex(R64), intent(in), contiguous, target :: state(:)
+    real(R64), intent(in) :: time
+    call CheckStateSizes(dState, state, "TimeDerivativeOrbsLin")
+    call CodyHip_Check(cody_time_derivative_orbs_lin(MctdhxHip_ctx, state, dState, time), "time_derivative_orbs_lin")
+  end subroutine
+
+  module subroutine MctdhxHip_TimeDerivativeCoeffsPlusOrbsNonLin(dState, state, time)
+    complex(R64), intent(out), contiguous, target :: dState(:)
+    complex(R64), intent(in), contiguous, target :: state(:)
+    real(R64), intent(in) :: time
+    call CheckStateSizes(dState, state, "TimeDerivativeCoeffsPlusOrbsNonLin")
+    call CodyHip_Check(cody_time_derivative_nonlin(MctdhxHip_ctx, state, dState, time), "time_derivative_nonlin")
+  end subroutine
+
+  subroutine CheckOrbShapes(dOrbs, orbs, where)
+    complex(R64), intent(in) :: dOrbs(:, :), orbs(:, :)
+    character(len=*), intent(in) :: where
+    if (size(orbs, 1) /= Grid_nPoints_of_ctx() .or. size(orbs, 2) /= MctdhxHip_input%nOrbs .or. &
+        any(shape(dOrbs) /= shape(orbs))) then
+      write (*, '(A)') "MctdhxHip "//where//": shapes must be nG x nOrbs"
+      error stop "MI355X MCTDHX: orbital shape mismatch"
+    end if
+  end subroutine
+
+  module subroutine MctdhxHip_ApplySingleBodyOp(dOrbs, orbs, time, h1_)
+    complex(R64), intent(inout), contiguous, target :: dOrbs(:, :)
+    complex(R64), intent(in), contiguous, target :: orbs(:, :)
+    real(R64), intent(in) :: time
+    complex(R64), intent(out), allocatable, optional :: h1_(:, :)
+    complex(c_double_complex) :: h1(MctdhxHip_input%nOrbs, MctdhxHip_input%nOrbs)
+    call CheckOrbShapes(dOrbs, orbs, "ApplySingleBodyOp")
+    call CodyHip_Check(cody_apply_single_body_acc(MctdhxHip_ctx, orbs, time, dOrbs, h1), "apply_single_body_acc")
+    if (present(h1_)) then
+      allocate (h1_(size(h1, 1), size(h1, 2)))
+      h1_ = h1
+    end if
+  end subroutine
+
+  module subroutine MctdhxHip_ApplyInteractionOp(dOrbs, orbs, i2, j2, time)
+    complex(R64), intent(inout), contiguous :: dOrbs(:, :)
+    complex(R64), intent(in), contiguous :: orbs(:, :)
+    integer(I32), intent(in) :: i2
+    integer(I32), intent(in) :: j2
+    real(R64), intent(in) :: time
+    call CheckOrbShapes(dOrbs, orbs, "ApplyInteractionOp")
+    if (i2 < 1 .or. i2 > size(orbs, 2) .or. j2 < 1 .or. j2 > size(orbs, 2)) &
+      error stop "MctdhxHip ApplyInteractionOp: orbital index out of range"
+    call CodyHip_Check(cody_apply_interaction(MctdhxHip_ctx, orbs, int(i2 - 1, c_int), int(j2 - 1, c_int), time, &
+                                              dOrbs), "apply_interaction")
   end subroutine
 
   !> Method_GetEnergy (S_Method_Mb_OrbBased.f90:132-161) on the GPU.

@@ -72,11 +72,15 @@ def load():
         "cody_time_derivative": (i, [vp, vp, vp, d]),
         "cody_time_derivative_orbs_lin_dev": (i, [vp, vp, vp, d, vp]),
         "cody_time_derivative_nonlin_dev": (i, [vp, vp, vp, d, vp]),
+        "cody_time_derivative_orbs_lin": (i, [vp, vp, vp, d]),
+        "cody_time_derivative_nonlin": (i, [vp, vp, vp, d]),
         "cody_energy": (i, [vp, vp, d, dp]),
         "cody_dipole": (i, [vp, vp, dp]),
         "cody_dipole_dev": (i, [vp, vp, vp, vp]),
         "cody_meanfield_w": (i, [vp, vp, vp]),
         "cody_apply_single_body": (i, [vp, vp, d, vp, vp]),
+        "cody_apply_single_body_acc": (i, [vp, vp, d, vp, vp]),
+        "cody_apply_interaction": (i, [vp, vp, i, i, d, vp]),
         "cody_densities": (i, [vp, vp, vp, vp, vp]),
         "cody_sil": (i, [vp, d, d, i, d, i, i, ip]),
         "cody_rk4": (i, [vp, d, d, i]),
@@ -289,10 +293,15 @@ class Context:
         return p.value
 
     # ---------------------------------------------------------------- method
-    def time_derivative(self, state, t=0.0):
+    def time_derivative(self, state, t=0.0, mode="full"):
+        """dState of a host state: the full right-hand side, or the split parts "lin"
+        (TimeDerivativeOrbsLin) and "nonlin" (TimeDerivativeCoeffsPlusOrbsNonLin)."""
+        fn = {"full": self.lib.cody_time_derivative,
+              "lin": self.lib.cody_time_derivative_orbs_lin,
+              "nonlin": self.lib.cody_time_derivative_nonlin}[mode]
         s = _c128(state)
         d = np.zeros(self.n_state, dtype=np.complex128)
-        _check(self.lib.cody_time_derivative(self.h, _vp(s), _vp(d), float(t)))
+        _check(fn(self.h, _vp(s), _vp(d), float(t)))
         return d
 
     def time_derivative_dev(self, d_state: int, d_dstate: int, t=0.0, stream: int = 0, mode="full"):
@@ -332,6 +341,21 @@ class Context:
         h1 = np.zeros((self.nO, self.nO), dtype=np.complex128, order="F")
         _check(self.lib.cody_apply_single_body(self.h, _vp(o), float(t), _vp(d), _vp(h1)))
         return d, h1
+
+    def apply_single_body_acc(self, orbs, dorbs, t=0.0):
+        """(dorbs + (T + V) orbs, h1 of the accumulated result): ApplySingleBodyOp's semantics."""
+        o = _c128(orbs)
+        d = _c128(dorbs).copy(order="F")
+        h1 = np.zeros((self.nO, self.nO), dtype=np.complex128, order="F")
+        _check(self.lib.cody_apply_single_body_acc(self.h, _vp(o), float(t), _vp(d), _vp(h1)))
+        return d, h1
+
+    def apply_interaction(self, orbs, i2, j2, dorbs, t=0.0):
+        """dorbs + W_(i2 j2) orbs (0-based pair indices): ApplyInteractionOp's semantics."""
+        o = _c128(orbs)
+        d = _c128(dorbs).copy(order="F")
+        _check(self.lib.cody_apply_interaction(self.h, _vp(o), int(i2), int(j2), float(t), _vp(d)))
+        return d
 
     def interaction_potential(self, src):
         """pot(:, s) = sum_j K(x_i - x_j) src(x_j, s) for sources holding the weights."""

@@ -156,6 +156,12 @@ int cody_time_derivative_orbs_lin_dev(cody_ctx* ctx, const void* d_state, void* 
                                       void* stream);
 int cody_time_derivative_nonlin_dev(cody_ctx* ctx, const void* d_state, void* d_dstate, double t,
                                     void* stream);
+/* The same two vector fields with host arrays (copies through the context's buffers;
+ * synchronous): the procedures the reference's Method_Mb_OrbBased_TimeDerivativeOrbsLin and
+ * Method_Mb_OrbBased_TimeDerivativeCoeffsPlusOrbsNonLin pointers
+ * (src/Method/Mb/OrbBased/M_Method_Mb_OrbBased.f90:272-300) bind to. */
+int cody_time_derivative_orbs_lin(cody_ctx* ctx, const double* state, double* dstate, double t);
+int cody_time_derivative_nonlin(cody_ctx* ctx, const double* state, double* dstate, double t);
 /* Energy E = Tr[rho1 h1] + Tr[rho2 h2] of a host state: replaces Method_GetEnergy
  * (src/Method/Mb/OrbBased/S_Method_Mb_OrbBased.f90:132-161). */
 int cody_energy(cody_ctx* ctx, const double* state, double t, double* energy);
@@ -172,6 +178,16 @@ int cody_dipole_dev(cody_ctx* ctx, const void* d_state, void* d_dipole, void* st
 int cody_meanfield_w(cody_ctx* ctx, const double* orbs, double* W);
 /* dOrbs = (T + V) orbs and h1 = <phi_i|(T+V) phi_j> (ApplySingleBodyOp, S_Method_Mb_OrbBased.f90:976-1034). */
 int cody_apply_single_body(cody_ctx* ctx, const double* orbs, double t, double* dorbs, double* h1);
+/* Method_Mb_OrbBased_ApplySingleBodyOp with its accumulate semantics (I_Method_Mb_OrbBased_ApplySingleBodyOp,
+ * M_Method_Mb_OrbBased.f90:253-269; S_Method_Mb_OrbBased.f90:976-1034): dorbs += (T + V + E(t) x) orbs,
+ * then h1 (nO x nO, may be NULL) = <phi_i|dorbs_j>_w of the accumulated dorbs. */
+int cody_apply_single_body_acc(cody_ctx* ctx, const double* orbs, double t, double* dorbs, double* h1);
+/* Method_Mb_OrbBased_ApplyInteractionOp (I_Method_Mb_OrbBased_ApplyInteractionOp,
+ * M_Method_Mb_OrbBased.f90:195-208; S_Method_Mb_OrbBased.f90:491-542): for the source pair (i2, j2)
+ * (0-based orbital indices), dorbs(:, i) += W * orbs(:, i) for every orbital i, with W the
+ * interaction potential of the source conj(phi_i2) phi_j2 w, on the FP64 MFMA contraction.
+ * Accumulate semantics; nG x nO complex128 column-major host arrays. */
+int cody_apply_interaction(cody_ctx* ctx, const double* orbs, int i2, int j2, double t, double* dorbs);
 /* rho1, rho2 of a CI vector (Coeffs_ApplyH1FillRdm1 / Coeffs_ApplyH2FillRdm2 in density mode,
  * S_Coeffs_Generic.f90:103-319 with the unpack of :325-434) and the regularised inverse
  * rho1 (rho1^2 + eps)^-1 (S_Method_Mb_OrbBased.f90:733-744). */

@@ -402,3 +402,38 @@ def test_operator_pieces(pair):
     vext = -w.pot_charge / (np.sqrt(x ** 2 + w.pot_soft1 ** 2) + w.pot_soft2)
     tk = s.apply_single_body(orbs, t=-1.0)[0] - vext[:, None] * orbs
     assert rel(ctx.apply_kinetic(orbs), tk) < 1e-12
+
+
+def test_accumulating_pieces_and_host_split_parts(pair):
+    """ApplySingleBodyOp and ApplyInteractionOp with the reference's accumulate semantics, and the
+    split-step vector fields through their host-array entry points.  h1 of ApplySingleBodyOp is
+    formed from the accumulated result (S_Method_Mb_OrbBased.f90:1010-1026); with a zero input it
+    is the oracle's h1.  The pair potential W[conj(phi_a) phi_b w] is checked against the dense
+    kernel matrix and against the oracle's mean field of the pair (conjugated for a > b)."""
+    import oracle
+
+    name, w, s, ctx, state, wts = pair
+    C, orbs = wl.split_state(state, w.nC, w.nG, w.M)
+    x = ctx.x
+    rng = np.random.default_rng(23)
+    d0 = np.asfortranarray(rng.standard_normal((w.nG, w.M)) + 1j * rng.standard_normal((w.nG, w.M)))
+    ho, h1o = s.apply_single_body(orbs)
+    dz, h1z = ctx.apply_single_body_acc(orbs, np.zeros_like(d0))
+    assert rel(dz, ho) < 1e-13
+    assert rel(h1z, h1o) < 1e-13
+    dg, h1g = ctx.apply_single_body_acc(orbs, d0)
+    acc = d0 + ho
+    assert rel(dg, acc) < 1e-13
+    assert rel(h1g, (np.conj(orbs) * wts[:, None]).T @ acc) < 1e-13
+    V = oracle.softyukawa(x[:, None] - x[None, :], w.int_strength, w.int_soft1, w.int_soft2, 0.0)
+    Wo = s.meanfield_w(orbs)
+    M = w.M
+    for a, b in [(0, 1), (2, 0), (1, 1), (M - 1, 0)]:
+        pot = V @ (np.conj(orbs[:, a]) * orbs[:, b] * wts)
+        lo, hi = min(a, b), max(a, b)
+        col = Wo[:, lo * M - lo * (lo - 1) // 2 + hi - lo]
+        assert rel(pot, col if a <= b else np.conj(col)) < 1e-12
+        got = ctx.apply_interaction(orbs, a, b, d0)
+        assert rel(got - d0, pot[:, None] * orbs) < 1e-12
+    assert state_rel(ctx.time_derivative(state, mode="lin"), s.time_derivative_orbs_lin(state), w.nC, wts, M) < 1e-12
+    assert state_rel(ctx.time_derivative(state, mode="nonlin"), s.time_derivative_nonlin(state), w.nC, wts, M) < 1e-12
