@@ -42,11 +42,15 @@ using namespace cody;
 
 namespace {
 
-constexpr int kLzBlocks = 128;     // blocks of every Lanczos vector launch (and partial-sum rows)
+constexpr int kLzMaxBlocks = 512;  // blocks of a Lanczos vector launch: one 256-entry tile each, at most this many
+                                   // (partial-sum rows); larger states take several tiles per block
 constexpr int kMaxKrylov = 32;     // maximal Krylov dimension
 constexpr int kPS = 64;            // partial-sum row stride (up to kMaxKrylov dots and one norm)
 constexpr double kZero = 1.0e-14;  // zeroThreshold of the reference
 constexpr int kChebMax = 200;      // Chebyshev terms of the subspace exponential (else Taylor)
+constexpr int kDctMax = 128;       // Chebyshev nodes of the coefficient transform (longer series: Bessel values)
+static_assert((kChebMax + 1) + (kDctMax + 256) + 2 * kDctMax <= 3 * kMaxKrylov * kMaxKrylov,
+              "the coefficient arrays fit the exponential workspace");
 
 // parameter slots (device doubles) written by the host before a replay
 enum {
@@ -61,22 +65,24 @@ __device__ __forceinline__ double metric_at(int i, int nC, int nG, const double*
   return (w == nullptr || i < nC) ? 1.0 : w[(i - nC) % nG];
 }
 
-// sOut[k] = sum_b partial[b][k] for k < nv <= kPS, fixed order (4 waves over the block rows, then
-// the waves in order).  Each wave's rows are loaded 16 at a time before they are added, so a
-// reduction costs two memory round trips rather than one per row (all 32 at once measured no
-// faster).
-__device__ void lz_reduce(const double2* __restrict__ partial, int nv, double2* sOut, double2 (*sTmp)[kPS]) {
-  static_assert(kLzBlocks % 64 == 0, "rows are loaded 16 per wave at a time");
+// sOut[k] = sum_b partial[b][k] for k < nv <= kPS over the nRows block rows, fixed order (wave q
+// takes rows q, q + 4, ..., then the waves in order).  Each wave's rows are loaded 24 at a time
+// before they are added, so up to 192 rows cost two memory round trips.
+__device__ void lz_reduce(const double2* __restrict__ partial, int nRows, int nv, double2* sOut,
+                          double2 (*sTmp)[kPS]) {
+  constexpr int kBatch = 24;
   const int t = threadIdx.x, k = t & 63, q = t >> 6;
   if (k < nv) {
     double2 acc = make_double2(0.0, 0.0);
+    for (int r0 = q; r0 < nRows; r0 += 4 * kBatch) {
+      double2 p[kBatch];
 #pragma unroll
-    for (int r0 = 0; r0 < kLzBlocks / 4; r0 += 16) {
-      double2 p[16];
+      for (int u = 0; u < kBatch; ++u) {
+        const int r = r0 + 4 * u;
+        p[u] = r < nRows ? partial[(size_t)r * kPS + k] : make_double2(0.0, 0.0);
+      }
 #pragma unroll
-      for (int u = 0; u < 16; ++u) p[u] = partial[(size_t)(q + 4 * (r0 + u)) * kPS + k];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
+      for (int u = 0; u < kBatch; ++u) {
         acc.x += p[u].x;
         acc.y += p[u].y;
       }
@@ -97,17 +103,17 @@ __device__ void lz_reduce(const double2* __restrict__ partial, int nv, double2* 
 
 // partial[block][k] = sum over this block's entries of m_i conj(V_k[i]) y[i] for k < nv, and with
 // withNorm also partial[block][nv] = sum m_i |y_i|^2.  The vectors are taken eight at a time so
-// that their loads are in flight together.
+// that their loads are in flight together; each batch's 16 block sums go through LDS
+// (block_sums256: sBuf 8 x 256 doubles, sRes 16 doubles), fixed order.
 __device__ void lz_dots(const double2* __restrict__ V, size_t ld, int nv, int withNorm, const double2* y, int n,
-                        int nC, int nG, const double* __restrict__ metric, double2* __restrict__ partial,
-                        double (*sr)[kPS], double (*si)[kPS]) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+                        int nb, int nC, int nG, const double* __restrict__ metric, double2* __restrict__ partial,
+                        double* sBuf, double* sRes) {
   const int nt = nv + (withNorm ? 1 : 0);
   for (int k0 = 0; k0 < nt; k0 += 8) {
-    double ar[8], ai[8];
+    double acc[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) ar[u] = ai[u] = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
+    for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += nb * 256) {
       const double m = metric_at(i, nC, nG, metric);
       const double2 b = y[i];
       double2 a[8];
@@ -118,25 +124,15 @@ __device__ void lz_dots(const double2* __restrict__ V, size_t ld, int nv, int wi
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        ar[u] = fma(m, a[u].x * b.x + a[u].y * b.y, ar[u]);
-        ai[u] = fma(m, a[u].x * b.y - a[u].y * b.x, ai[u]);
+        acc[u] = fma(m, a[u].x * b.x + a[u].y * b.y, acc[u]);
+        acc[8 + u] = fma(m, a[u].x * b.y - a[u].y * b.x, acc[8 + u]);
       }
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (k0 + u < nt) {  // uniform
-        const double r = wave_sum(ar[u]), im = wave_sum(ai[u]);
-        if (lane == 0) {
-          sr[wv][k0 + u] = r;
-          si[wv][k0 + u] = im;
-        }
-      }
-    }
+    block_sums256<16>(acc, sBuf, sRes);
+    for (int u = threadIdx.x; u < 8 && k0 + u < nt; u += 256)
+      partial[(size_t)blockIdx.x * kPS + k0 + u] = make_double2(sRes[u], sRes[8 + u]);
+    __syncthreads();  // sRes is reused by the next batch
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < nt; k += 256)
-    partial[(size_t)blockIdx.x * kPS + k] =
-        make_double2((sr[0][k] + sr[1][k]) + (sr[2][k] + sr[3][k]), (si[0][k] + si[1][k]) + (si[2][k] + si[3][k]));
 }
 
 // r -= sum_{k<nv} c_k v_k[i], eight vectors' loads at a time.
@@ -170,24 +166,24 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ a, dou
 }
 
 // Start of a sub-step: clear the loop control and form the partial sums of <in|in>_m.
-__global__ __launch_bounds__(256) void k_lz_norm(const double2* __restrict__ in, int n, int nC, int nG,
+__global__ __launch_bounds__(256) void k_lz_norm(const double2* __restrict__ in, int n, int nb, int nC, int nG,
                                                  const double* __restrict__ metric, double2* __restrict__ partial,
                                                  int* __restrict__ ctl) {
-  __shared__ double sr[4][kPS], si[4][kPS];
+  __shared__ double sBuf[8 * 256], sRes[16];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl[0] = 0;
     ctl[1] = 0;
     ctl[2] = 0;
   }
-  lz_dots(in, 0, 0, 1, in, n, nC, nG, metric, partial, sr, si);
+  lz_dots(in, 0, 0, 1, in, n, nb, nC, nG, metric, partial, sBuf, sRes);
 }
 
 // v_0 = in / |in|_m; a zero input ends the loop at once (the sub-step returns its input).
 __global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ partial, const double2* __restrict__ in,
-                                                  double2* __restrict__ v0, int n, double* __restrict__ nrm2,
+                                                  double2* __restrict__ v0, int n, int nb, double* __restrict__ nrm2,
                                                   int* __restrict__ ctl, double* __restrict__ flags) {
   __shared__ double2 sC[kPS], sTmp[4][kPS];
-  lz_reduce(partial, 1, sC, sTmp);
+  lz_reduce(partial, nb, 1, sC, sTmp);
   const double s2 = sC[0].x, nr = sqrt(s2);
   if (nr <= kZero) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -200,7 +196,7 @@ __global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ pa
     return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *nrm2 = s2;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256)
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += nb * 256)
     v0[i] = make_double2(in[i].x / nr, in[i].y / nr);
 }
 
@@ -215,15 +211,15 @@ __global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ ba
                                                   const double* __restrict__ beta, const int* __restrict__ ctl,
                                                   const double* __restrict__ w, const double* __restrict__ xg,
                                                   const double* __restrict__ vext, KinParams kp,
-                                                  const double* __restrict__ efield, int nC, int nG, int n,
+                                                  const double* __restrict__ efield, int nC, int nG, int n, int nb,
                                                   const double* __restrict__ metric, double2* __restrict__ wout,
                                                   double2* __restrict__ pout) {
-  __shared__ double sr[4][kPS], si[4][kPS];
+  __shared__ double sBuf[8 * 256], sRes[16];
   if (ctl[0]) return;
   const double E = *efield;
   const double bprev = j > 0 ? beta[j - 1] : 0.0;
   const double2* v = basis + (size_t)j * ld;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += nb * 256) {
     const double2 vi = v[i];
     double2 h = cmk(0, 0);
     if (i >= nC) {
@@ -240,24 +236,24 @@ __global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ ba
     }
     wout[i] = h;
   }
-  lz_dots(basis, ld, j + 1, 0, wout, n, nC, nG, metric, pout, sr, si);
+  lz_dots(basis, ld, j + 1, 0, wout, n, nb, nC, nG, metric, pout, sBuf, sRes);
 }
 
 // First re-orthogonalisation pass: w -= sum_{k<=j} c_k v_k (c from pin), alpha_j = Re c_j; then
 // the partial sums of the second pass, <v_k|w> for k <= j, and of <w|w>.
 __global__ __launch_bounds__(256) void k_lz_pass1(const int* __restrict__ ctl, const double2* __restrict__ pin,
                                                   int j, const double2* __restrict__ basis, size_t ld,
-                                                  double2* __restrict__ w, int n, int nC, int nG,
+                                                  double2* __restrict__ w, int n, int nb, int nC, int nG,
                                                   const double* __restrict__ metric, double* __restrict__ alpha,
                                                   double2* __restrict__ pout) {
   __shared__ double2 sC[kPS], sTmp[4][kPS];
-  __shared__ double sr[4][kPS], si[4][kPS];
+  __shared__ double sBuf[8 * 256], sRes[16];
   if (ctl[0]) return;
   const int nv = j + 1;
-  lz_reduce(pin, nv, sC, sTmp);
+  lz_reduce(pin, nb, nv, sC, sTmp);
   if (blockIdx.x == 0 && threadIdx.x == 0) alpha[j] = sC[j].x;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kLzBlocks * 256) w[i] = lz_subtract(w[i], basis, ld, nv, sC, i);
-  lz_dots(basis, ld, nv, 1, w, n, nC, nG, metric, pout, sr, si);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += nb * 256) w[i] = lz_subtract(w[i], basis, ld, nv, sC, i);
+  lz_dots(basis, ld, nv, 1, w, n, nb, nC, nG, metric, pout, sBuf, sRes);
 }
 
 // J_k(a), k = 0..K (0 < a), by Miller's backward recurrence J_{k-1} = (2k/a) J_k - J_{k+1} from
@@ -287,13 +283,43 @@ __device__ void bessel_j_miller(double a, int K, double* J) {
   for (int q = 0; q <= K; ++q) J[q] *= is;
 }
 
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// v of lane l - 1 (0 in lane 0) and of lane l + 1 (0 in lane 63): DPP wave_shr:1 / wave_shl:1
+__device__ __forceinline__ double from_lane_below(double v) { return dpp_move<0x138>(v); }
+__device__ __forceinline__ double from_lane_above(double v) { return dpp_move<0x130>(v); }
+
+__device__ __forceinline__ double readlane_double(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Number of Chebyshev terms for exp(-i a X), |X| <= 1: the first k > a with (a/2)^k / k! < 1e-18
 // (which bounds |J_k(a)| and the rest of the series); -1 when more than kChebMax would be needed.
-__device__ int cheb_terms(double a) {
-  double t = 1.0;
-  for (int k = 1; k < kChebMax; ++k) {
-    t *= 0.5 * a / k;
-    if (k > a && t < 1e-18) return k;
+// Whole wave 0: lane l tests k = k0 + l + 1, the products (a/2)^k / k! by a prefix product across
+// the lanes.  Every lane returns the same value.
+__device__ int cheb_terms_wave(double a) {
+  const int l = threadIdx.x & 63;
+  double base = 1.0;
+  for (int k0 = 0; k0 < kChebMax; k0 += 64) {
+    const int k = k0 + l + 1;
+    double p = 0.5 * a / k;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double o = __shfl_up(p, off, 64);
+      if (l >= off) p *= o;
+    }
+    p *= base;
+    const unsigned long long hit = __ballot(k < kChebMax && k > a && p < 1e-18);
+    if (hit) return k0 + __ffsll((long long)hit);
+    base = readlane_double(p, 63);
   }
   return -1;
 }
@@ -382,12 +408,18 @@ __device__ void tridiag_expm_taylor(const double* sA, const double* sB, int m, d
 
 // phi = exp(-i dt T) e1 into sPhi (LDS) for the Lanczos tridiagonal T: alpha sA[0..m-1], beta
 // sB[0..m-2].  The reference diagonalises T (EvaluateExpInSubspace, :277-310); here the
-// Chebyshev expansion exp(-i dt T) = e^{-i dt c} sum_k (2 - delta_k0) (-i)^k J_k(dt r) T_k(X),
-// X = (T - c)/r with c, r the centre and half-width of the Gershgorin interval (Tal-Ezer and
-// Kosloff, J. Chem. Phys. 81 (1984) 3967), runs on wave 0 with vector component l in lane l and
-// the tridiagonal product by lane shuffles.  Whole workgroup, one call per block.
+// Chebyshev expansion exp(-i dt T) = e^{-i dt c} sum_{k<=K} c_k T_k(X), X = (T - c)/r with c, r
+// the centre and half-width of the Gershgorin interval (Tal-Ezer and Kosloff, J. Chem. Phys. 81
+// (1984) 3967).  The coefficients of f(x) = exp(-i dt r x) are c_k = (2 - delta_k0) (-i)^k
+// J_k(dt r) (Jacobi-Anger).  For K < kDctMax they come from f at N > K Chebyshev nodes
+// x_j = cos theta_j, theta_j = pi (j + 1/2) / N, by the discrete cosine transform
+//   c_k = (2 - delta_k0) / N sum_j f(x_j) cos(k theta_j),
+// exact up to the aliased terms c_{2N-k}, c_{2N+k}, ... which the choice of K bounds by 1e-18
+// (all threads: nodes, sums over node ranges); for longer series from Bessel values by Miller's
+// recurrence.  The recurrence T_{k+1} = 2 X T_k - T_{k-1} on e1 runs on wave 0, vector component l
+// in lane l, neighbours by DPP lane moves.  Whole workgroup; ws holds 3 kMaxKrylov^2 double2.
 __device__ void subspace_exp(const double* sA, const double* sB, int m, double dt, double2* sPhi, double* sJ,
-                             double* sPar, double2* E, double2* Tm, double2* Pw, int* sSq) {
+                             double* sPar, double2* ws, int* sSq) {
   const int t = threadIdx.x, l = t & 63;
   if (t < 64) {
     const double al = l < m ? sA[l] : 0.0;
@@ -400,51 +432,99 @@ __device__ void subspace_exp(const double* sA, const double* sB, int m, double d
       lo = fmin(lo, __shfl_xor(lo, off, 64));
       hi = fmax(hi, __shfl_xor(hi, off, 64));
     }
+    const double c = 0.5 * (lo + hi), r = 0.5 * (hi - lo), a = fabs(dt) * r;
+    const int K = (r > 0.0 && dt != 0.0) ? cheb_terms_wave(a) : 0;
     if (l == 0) {
-      const double c = 0.5 * (lo + hi), r = 0.5 * (hi - lo), a = fabs(dt) * r;
       sPar[0] = c;
       sPar[1] = r;
       sPar[2] = a;
-      sPar[3] = (r > 0.0 && dt != 0.0) ? (double)cheb_terms(a) : 0.0;
+      sPar[3] = (double)K;
     }
   }
   __syncthreads();
   const int K = (int)sPar[3];
+  constexpr int K2 = kMaxKrylov * kMaxKrylov;
   if (K < 0) {
-    tridiag_expm_taylor(sA, sB, m, dt, sPhi, E, Tm, Pw, sSq);
+    tridiag_expm_taylor(sA, sB, m, dt, sPhi, ws, ws + K2, ws + 2 * K2, sSq);
     return;
   }
   const double c = sPar[0], r = sPar[1], a = sPar[2];
-  if (t == 0 && K > 0) bessel_j_miller(a, K, sJ);
-  __syncthreads();
+  // LDS layout in ws: coefficients, node values (and per-range partial sums), cosine table
+  double2* sCoef = ws;                      // kChebMax + 1
+  double2* sF = ws + (kChebMax + 1);        // kDctMax + blockDim.x
+  double* sCos = reinterpret_cast<double*>(sF + kDctMax + 256);  // 4 kDctMax
+  if (K > 0 && K < kDctMax) {
+    const int N = K < 64 ? 64 : kDctMax, msk = 4 * N - 1;
+    const double dr = dt * r;
+    for (int q = t; q < 4 * N; q += blockDim.x) sCos[q] = cos(M_PI * q / (2 * N));  // cos(pi q / 2N)
+    __syncthreads();
+    if (t < N) {
+      double sn, cs;
+      sincos(dr * sCos[2 * t + 1], &sn, &cs);  // f(x_t) = exp(-i dr x_t)
+      sF[t] = make_double2(cs, -sn);
+    }
+    __syncthreads();
+    // thread t: coefficient k = t % N over node range t / N of blockDim.x / N
+    const int k = t % N, part = t / N, nParts = blockDim.x / N, per = N / nParts;
+    double re = 0.0, im = 0.0;
+    if (k <= K) {
+      for (int j = part * per; j < (part + 1) * per; ++j) {
+        const double cw = sCos[(k * (2 * j + 1)) & msk];
+        const double2 f = sF[j];
+        re = fma(f.x, cw, re);
+        im = fma(f.y, cw, im);
+      }
+    }
+    sF[kDctMax + t] = make_double2(re, im);
+    __syncthreads();
+    if (t <= K) {
+      double2 acc = sF[kDctMax + t];
+      for (int q = 1; q < nParts; ++q) {
+        const double2 v = sF[kDctMax + t + q * N];
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      const double sc = (t == 0 ? 1.0 : 2.0) / N;
+      sCoef[t] = make_double2(acc.x * sc, acc.y * sc);
+    }
+    __syncthreads();
+  } else if (K > 0) {
+    if (t == 0) bessel_j_miller(a, K, sJ);
+    __syncthreads();
+    // (2 - delta_k0) (-i)^k J_k(a), conjugated for dt < 0 (the expansion is in exp(-i |dt| r X))
+    const double sg = dt < 0.0 ? -1.0 : 1.0;
+    for (int k = t; k <= K; k += blockDim.x) {
+      const double v = (k == 0 ? 1.0 : 2.0) * sJ[k];
+      const int q = k & 3;
+      sCoef[k] = q == 0 ? make_double2(v, 0.0) : q == 1 ? make_double2(0.0, -sg * v)
+               : q == 2 ? make_double2(-v, 0.0) : make_double2(0.0, sg * v);
+    }
+    __syncthreads();
+  }
   if (t < 64) {
     double sre = 0.0, sim = 0.0;
     if (K == 0) {
       sre = l == 0 ? 1.0 : 0.0;  // T = c I (or dt = 0)
     } else {
-      const double al = l < m ? sA[l] - c : 0.0;
-      const double bu = l < m - 1 ? sB[l] : 0.0;
-      const double bd = (l > 0 && l < m) ? sB[l - 1] : 0.0;
       const double ir = 1.0 / r;
-      // the expansion is in exp(-i |dt| r X); dt < 0 conjugates it
-      const double sg = dt < 0.0 ? -1.0 : 1.0;
+      const double al = l < m ? (sA[l] - c) * ir : 0.0;
+      const double bu = l < m - 1 ? sB[l] * ir : 0.0;
+      const double bd = (l > 0 && l < m) ? sB[l - 1] * ir : 0.0;
       double u0 = l == 0 ? 1.0 : 0.0;
-      auto applyX = [&](double u) {
-        const double um = __shfl_up(u, 1, 64), up = __shfl_down(u, 1, 64);
-        return (al * u + (l > 0 ? bd * um : 0.0) + bu * up) * ir;
-      };
-      double u1 = applyX(u0);
-      sre = sJ[0] * u0;
-      sim = -2.0 * sJ[1] * u1 * sg;
+      double u1 = al * u0 + bd * from_lane_below(u0) + bu * from_lane_above(u0);
+      const double2 c0 = sCoef[0], c1 = sCoef[1];
+      sre = c0.x * u0 + c1.x * u1;
+      sim = c0.y * u0 + c1.y * u1;
+      const double al2 = 2.0 * al, bu2 = 2.0 * bu, bd2 = 2.0 * bd;
+      double2 cn = K >= 2 ? sCoef[2] : make_double2(0.0, 0.0);
       for (int k = 2; k <= K; ++k) {
-        const double u2 = 2.0 * applyX(u1) - u0;
-        const double cf = 2.0 * sJ[k] * u2;
-        switch (k & 3) {
-          case 0: sre += cf; break;
-          case 1: sim -= cf * sg; break;
-          case 2: sre -= cf; break;
-          default: sim += cf * sg; break;
-        }
+        const double2 ck = cn;
+        if (k < K) cn = sCoef[k + 1];  // the next coefficient's load overlaps this step
+        double u2 = fma(al2, u1, -u0);
+        u2 = fma(bd2, from_lane_below(u1), u2);
+        u2 = fma(bu2, from_lane_above(u1), u2);
+        sre = fma(ck.x, u2, sre);
+        sim = fma(ck.y, u2, sim);
         u0 = u1;
         u1 = u2;
       }
@@ -473,8 +553,7 @@ __device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m
   }
   __syncthreads();
   const bool breakdown = bres <= kZero;
-  constexpr int K2 = kMaxKrylov * kMaxKrylov;
-  subspace_exp(sA, sB, m, *dtp, sPhi, sJ, sPar, ws, ws + K2, ws + 2 * K2, &sSq);
+  subspace_exp(sA, sB, m, *dtp, sPhi, sJ, sPar, ws, &sSq);
   for (int k = threadIdx.x; k < m; k += blockDim.x) phi[k] = sPhi[k];
   if (threadIdx.x == 0) {
     const double2 last = sPhi[m - 1];
@@ -495,8 +574,9 @@ __device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m
 // beta_j = |w2|_m = sqrt(|w|^2 - sum |c2_k|^2) from the second pass's sums (the v_k are
 // orthonormal in the metric and the second pass removes rounding-size components only); every
 // workgroup forms it from the same sums in the same order.
-__device__ double lz_beta(const double2* __restrict__ pin, int nv, double2* sC, double2 (*sTmp)[kPS], double* sBeta) {
-  lz_reduce(pin, nv + 1, sC, sTmp);
+__device__ double lz_beta(const double2* __restrict__ pin, int nRows, int nv, double2* sC, double2 (*sTmp)[kPS],
+                          double* sBeta) {
+  lz_reduce(pin, nRows, nv + 1, sC, sTmp);
   if (threadIdx.x == 0) {
     double b2 = sC[nv].x;
     for (int k = 0; k < nv; ++k) b2 -= sC[k].x * sC[k].x + sC[k].y * sC[k].y;
@@ -512,7 +592,7 @@ __device__ double lz_beta(const double2* __restrict__ pin, int nv, double2* sC, 
 // v_{j+1} is formed on each 256-entry tile and on `halo` = nLoc - 1 entries either side (the
 // reach of the FEDVR stencil) in LDS, so H needs no other workgroup's values; w' goes to the
 // other w buffer, so no workgroup overwrites the w that another one's halo reads.  Workgroup
-// kLzBlocks ends iteration j (lz_eval on T_{j+1}) while the others work; when it ends the loop
+// nb (the last) ends iteration j (lz_eval on T_{j+1}) while the others work; when it ends the loop
 // the launch's vector results are never used.  Block 0 records beta_j.
 __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __restrict__ pin,
                                                  const double* __restrict__ alpha, double* __restrict__ beta, int j,
@@ -522,28 +602,28 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
                                                  const double2* __restrict__ wcur, double2* __restrict__ wnext,
                                                  const double* __restrict__ wg, const double* __restrict__ xg,
                                                  const double* __restrict__ vext, KinParams kp,
-                                                 const double* __restrict__ efield, int nC, int nG, int n,
+                                                 const double* __restrict__ efield, int nC, int nG, int n, int nb,
                                                  const double* __restrict__ metric, double2* __restrict__ pout,
                                                  int halo) {
   __shared__ double2 sWork[3 * kMaxKrylov * kMaxKrylov];  // exponential workspace, or the v tile
   __shared__ double2 sC[kPS], sTmp[4][kPS];
-  __shared__ double sr[4][kPS], si[4][kPS];
+  __shared__ double sBuf[8 * 256], sRes[16];
   __shared__ double sBeta;
-  // (workgroup kLzBlocks of this launch may set ctl[0]: relaxed atomic load, see lz_eval)
+  // (workgroup nb of this launch may set ctl[0]: relaxed atomic load, see lz_eval)
   if (__hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const int nv = j + 1;
-  const double bj = lz_beta(pin, nv, sC, sTmp, &sBeta);
-  if (blockIdx.x == kLzBlocks) {
+  const double bj = lz_beta(pin, nb, nv, sC, sTmp, &sBeta);
+  if (blockIdx.x == nb) {
     lz_eval(ctl, alpha, beta, nv, bj, maxDim, dtp, tol, phi, flags, sWork);
     return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) beta[j] = bj;
-  if (bj <= kZero) return;  // breakdown: workgroup kLzBlocks ends the loop
+  if (bj <= kZero) return;  // breakdown: workgroup nb ends the loop
   const double E = *efield;
   double2* vn = basis + (size_t)nv * ld;
   const double2* vj = basis + (size_t)j * ld;
   double2* sV = sWork;
-  for (int tile = blockIdx.x; tile * 256 < n; tile += kLzBlocks) {
+  for (int tile = blockIdx.x; tile * 256 < n; tile += nb) {
     const int i0 = tile * 256, lo = i0 - halo;
     __syncthreads();  // the previous tile's reads of sV are done
     for (int t = threadIdx.x; t < 256 + 2 * halo; t += 256) {
@@ -574,7 +654,7 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
       wnext[i] = h;
     }
   }
-  lz_dots(basis, ld, nv + 1, 0, wnext, n, nC, nG, metric, pout, sr, si);
+  lz_dots(basis, ld, nv + 1, 0, wnext, n, nb, nC, nG, metric, pout, sBuf, sRes);
 }
 
 // The end of the last iteration j = maxDim - 1 (one workgroup): beta_j and lz_eval on T_maxDim,
@@ -582,12 +662,12 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
 __global__ __launch_bounds__(256) void k_lz_last(int* ctl, const double2* __restrict__ pin,
                                                  const double* __restrict__ alpha, double* __restrict__ beta, int j,
                                                  int maxDim, const double* __restrict__ dtp, double tol,
-                                                 double2* __restrict__ phi, double* __restrict__ flags) {
+                                                 double2* __restrict__ phi, double* __restrict__ flags, int nb) {
   __shared__ double2 sWork[3 * kMaxKrylov * kMaxKrylov];
   __shared__ double2 sC[kPS], sTmp[4][kPS];
   __shared__ double sBeta;
   if (ctl[0]) return;
-  const double bj = lz_beta(pin, j + 1, sC, sTmp, &sBeta);
+  const double bj = lz_beta(pin, nb, j + 1, sC, sTmp, &sBeta);
   if (threadIdx.x == 0) beta[j] = bj;
   lz_eval(ctl, alpha, beta, j + 1, bj, maxDim, dtp, tol, phi, flags, sWork);
 }
@@ -665,7 +745,7 @@ PropBuffers* buffers(cody_ctx* c, int kdim) {
   const size_t n = c->nS;
   if (ctx_alloc((void**)&b.basis, sizeof(double2) * n * kdim) || ctx_alloc((void**)&b.w, sizeof(double2) * n) ||
       ctx_alloc((void**)&b.w2, sizeof(double2) * n) ||
-      ctx_alloc((void**)&b.partial, sizeof(double2) * 2 * kLzBlocks * kPS) ||
+      ctx_alloc((void**)&b.partial, sizeof(double2) * 2 * kLzMaxBlocks * kPS) ||
       ctx_alloc((void**)&b.phi, sizeof(double2) * kMaxKrylov) ||
       ctx_alloc((void**)&b.trial, sizeof(double2) * n) || ctx_alloc((void**)&b.work, sizeof(double2) * n) ||
       ctx_alloc((void**)&b.backup, sizeof(double2) * n) ||
@@ -698,27 +778,28 @@ void enqueue_sil_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, in
   const size_t ld = (size_t)n;
   const double* metric = useMetric ? c->d_w : nullptr;
   double2* p0 = b->partial;
-  double2* p1 = b->partial + kLzBlocks * kPS;
-  const dim3 g(kLzBlocks), t(256);
-  hipLaunchKernelGGL(k_lz_norm, g, t, 0, st, in, n, c->nC, c->nG, metric, p0, b->ctl);
-  hipLaunchKernelGGL(k_lz_start, g, t, 0, st, p0, in, b->basis, n, b->nrm2, b->ctl, flags);
+  double2* p1 = b->partial + kLzMaxBlocks * kPS;
+  const int nb = std::min(kLzMaxBlocks, (n + 255) / 256);  // one 256-entry tile per block
+  const dim3 g(nb), t(256);
+  hipLaunchKernelGGL(k_lz_norm, g, t, 0, st, in, n, nb, c->nC, c->nG, metric, p0, b->ctl);
+  hipLaunchKernelGGL(k_lz_start, g, t, 0, st, p0, in, b->basis, n, nb, b->nrm2, b->ctl, flags);
   const KinParams kp{c->kin.kref, c->kin.nE, c->kin.nLoc, c->kin.xminExcl, c->nG, c->kin.invh, c->kin.fac};
   // iteration j: k_lz_pass1 on w[j % 2], then k_lz_next (the end of j and w[(j + 1) % 2] of
   // j + 1, one more workgroup for the end of j) or, after the last iteration, k_lz_last
   double2* wb[2] = {b->w, b->w2};
   const int halo = kp.nLoc - 1;
   hipLaunchKernelGGL(k_lz_apply, g, t, 0, st, b->basis, ld, 0, b->beta, b->ctl, c->d_w, c->d_x, c->d_vext, kp, efp,
-                     c->nC, c->nG, n, metric, wb[0], p0);
+                     c->nC, c->nG, n, nb, metric, wb[0], p0);
   for (int j = 0; j < maxDim; ++j) {
-    hipLaunchKernelGGL(k_lz_pass1, g, t, 0, st, b->ctl, p0, j, b->basis, ld, wb[j % 2], n, c->nC, c->nG, metric,
+    hipLaunchKernelGGL(k_lz_pass1, g, t, 0, st, b->ctl, p0, j, b->basis, ld, wb[j % 2], n, nb, c->nC, c->nG, metric,
                        b->alpha, p1);
     if (j + 1 < maxDim)
-      hipLaunchKernelGGL(k_lz_next, dim3(kLzBlocks + 1), t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, dtp, tol,
+      hipLaunchKernelGGL(k_lz_next, dim3(nb + 1), t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, dtp, tol,
                          b->phi, flags, b->basis, ld, wb[j % 2], wb[(j + 1) % 2], c->d_w, c->d_x, c->d_vext, kp, efp,
-                         c->nC, c->nG, n, metric, p0, halo);
+                         c->nC, c->nG, n, nb, metric, p0, halo);
     else
       hipLaunchKernelGGL(k_lz_last, dim3(1), t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, dtp, tol, b->phi,
-                         flags);
+                         flags, nb);
   }
   hipLaunchKernelGGL(k_lz_combine, blocks_for(n), t, 0, st, b->ctl, b->basis, ld, b->phi, b->nrm2, in, out, n);
 }
