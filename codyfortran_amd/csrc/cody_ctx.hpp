@@ -51,6 +51,17 @@ struct PropBuffers {
   int* ctl = nullptr;         // Lanczos loop control: done, Krylov dimension, zero input
   double2 *rk_inc = nullptr, *rk_tmp = nullptr, *rk_d = nullptr;
   std::vector<PropGraph> graphs;
+  // Krylov dimensions of the last Strang steps (the larger of the two SILs), to cap the next
+  // step's Lanczos loop
+  int dimHist[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int dimHistN = 0;
+  int krylov_cap(int maxDim) const {
+    if (dimHistN == 0) return maxDim;
+    int m = 0;
+    for (int i = 0; i < 8 && i < dimHistN; ++i) m = dimHist[i] > m ? dimHist[i] : m;
+    return m + 1 < maxDim ? m + 1 : maxDim;
+  }
+  void record_dim(int m) { dimHist[dimHistN++ % 8] = m; }
   void drop_graphs() {
     for (auto& g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();

@@ -192,6 +192,7 @@ __global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ pa
       ctl[2] = 1;
       flags[0] = 0.0;
       flags[1] = 1.0;
+      flags[2] = 0.0;
     }
     return;
   }
@@ -561,6 +562,7 @@ __device__ void lz_eval(int* ctl, const double* alpha, const double* beta, int m
     const bool conv = err <= tol;
     flags[0] = err;
     flags[1] = conv ? 1.0 : 0.0;
+    flags[2] = m;
     if (conv || breakdown || m == maxDim) {
       // The other workgroups of the same k_lz_next launch read ctl[0] at their entry: relaxed
       // atomic accesses, so a reader sees either value.  Both are correct: when this workgroup
@@ -771,7 +773,7 @@ void set_par(PropBuffers* b, int off, std::initializer_list<double> v, hipStream
 }
 
 // One SIL sub-step from `in` into `out` (PerformLanczosStep, :207-268): dt from *dtp, E of the
-// linear part from *efp, flags[0..1] = (error, converged).  Launches only.
+// linear part from *efp, flags[0..2] = (error, converged, Krylov dimension).  Launches only.
 void enqueue_sil_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, int useMetric, const double2* in,
                          double2* out, const double* dtp, const double* efp, double* flags, hipStream_t st) {
   const int n = c->nS;
@@ -939,8 +941,40 @@ int rk4_impl(cody_ctx* c, double t0, double t1, int mode) {
   return CODY_OK;
 }
 
+// One Strang step A(dt/2) B(dt) A(dt/2) as one graph, each SIL a single sub-step with at most
+// `dim` Lanczos iterations, after a backup of the state.  *ok: both SILs met the tolerance;
+// *mDim: the larger of their Krylov dimensions.  The parameter slots are set by the caller.
+int strang_graph_step(cody_ctx* c, PropBuffers* b, double t0, double t1, int dim, double tol, int useMetric,
+                      bool* ok, int* mDim) {
+  hipStream_t st = c->stream;
+  const int n = c->nS;
+  const PropGraph key{1, c->d_state, c->d_state, dim, useMetric, tol, t1 - t0, nullptr};
+  int r = run_prop_graph(c, b, key, [&](hipStream_t s) {
+    hipLaunchKernelGGL(k_copy, blocks_for(n), dim3(256), 0, s, c->d_state, b->backup, n);
+    enqueue_sil_substep(c, b, dim, tol, useMetric, c->d_state, c->d_state, b->par + kParSil1Dt, b->par + kParSil1E,
+                        b->flags, s);
+    int rr = enqueue_rk4(c, b, t0, t1, kModeNonLin, s, false, b->par + kParRk0);
+    if (rr) return rr;
+    enqueue_sil_substep(c, b, dim, tol, useMetric, c->d_state, c->d_state, b->par + kParSil2Dt, b->par + kParSil2E,
+                        b->flags + 3, s);
+    return CODY_OK;
+  });
+  if (r) return r;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->h_flags, b->flags, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *ok = b->h_flags[1] != 0.0 && b->h_flags[4] != 0.0;
+  *mDim = (int)std::max(b->h_flags[2], b->h_flags[5]);
+  return CODY_OK;
+}
+
 // A(dt/2) B(dt) A(dt/2): S_Propagator_SplitStep_Order2.f90:61-75.  Fast path: one graph with each
-// SIL as a single sub-step; if either misses the tolerance, restore and use host-driven halving.
+// SIL as a single sub-step, its Lanczos loop capped one iteration above the largest Krylov
+// dimension of the last eight steps (the launches after convergence return at once but still
+// cost their dispatch).  A SIL that stops at the cap without meeting the tolerance has not run
+// the reference's loop to its end: the state is restored and the step rerun with the full
+// dimension, so an accepted step is the uncapped one.  If a SIL misses the tolerance at the full
+// dimension, restore and use host-driven halving.
 int strang_impl(cody_ctx* c, double t0, double t1, int krylovDim, double tol, int maxRestarts, int useMetric,
                 int* converged) {
   const double dt = t1 - t0, t2 = t0 + 0.5 * dt;
@@ -954,22 +988,16 @@ int strang_impl(cody_ctx* c, double t0, double t1, int krylovDim, double tol, in
             {0.5 * dt, laser_field(c, t0 + 0.25 * dt), 0.5 * dt, laser_field(c, t2 + 0.25 * dt), laser_field(c, t0),
              laser_field(c, t0 + 0.5 * dt), laser_field(c, t0 + dt)},
             st);
-    const PropGraph key{1, c->d_state, c->d_state, maxDim, useMetric, tol, dt, nullptr};
-    int r = run_prop_graph(c, b, key, [&](hipStream_t s) {
-      hipLaunchKernelGGL(k_copy, blocks_for(n), dim3(256), 0, s, c->d_state, b->backup, n);
-      enqueue_sil_substep(c, b, maxDim, tol, useMetric, c->d_state, c->d_state, b->par + kParSil1Dt,
-                          b->par + kParSil1E, b->flags, s);
-      int rr = enqueue_rk4(c, b, t0, t1, kModeNonLin, s, false, b->par + kParRk0);
-      if (rr) return rr;
-      enqueue_sil_substep(c, b, maxDim, tol, useMetric, c->d_state, c->d_state, b->par + kParSil2Dt,
-                          b->par + kParSil2E, b->flags + 2, s);
-      return CODY_OK;
-    });
-    if (r) return r;
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(b->h_flags, b->flags, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (b->h_flags[1] != 0.0 && b->h_flags[3] != 0.0) {
+    const int cap = b->krylov_cap(maxDim);
+    bool ok = false;
+    int m = 0, r;
+    if ((r = strang_graph_step(c, b, t0, t1, cap, tol, useMetric, &ok, &m))) return r;
+    if (!ok && cap < maxDim) {
+      HIPCHK(hipMemcpyAsync(c->d_state, b->backup, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+      if ((r = strang_graph_step(c, b, t0, t1, maxDim, tol, useMetric, &ok, &m))) return r;
+    }
+    b->record_dim(m);
+    if (ok) {
       if (converged) *converged = 1;
       return CODY_OK;
     }
