@@ -135,25 +135,6 @@ __device__ void lz_dots(const double2* __restrict__ V, size_t ld, int nv, int wi
   }
 }
 
-// r -= sum_{k<nv} c_k v_k[i], eight vectors' loads at a time.
-__device__ __forceinline__ double2 lz_subtract(double2 r, const double2* __restrict__ basis, size_t ld, int nv,
-                                               const double2* c, int i) {
-  for (int k0 = 0; k0 < nv; k0 += 8) {
-    double2 v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = k0 + u < nv ? basis[(k0 + u) * ld + i] : make_double2(0.0, 0.0);
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (k0 + u < nv) {
-        const double2 cf = c[k0 + u];
-        r.x -= cf.x * v[u].x - cf.y * v[u].y;
-        r.y -= cf.x * v[u].y + cf.y * v[u].x;
-      }
-    }
-  }
-  return r;
-}
-
 __global__ void k_set_par(double* __restrict__ par, int off, int n, double v0, double v1, double v2, double v3,
                           double v4, double v5, double v6) {
   const double v[7] = {v0, v1, v2, v3, v4, v5, v6};
@@ -240,21 +221,62 @@ __global__ __launch_bounds__(256) void k_lz_apply(const double2* __restrict__ ba
   lz_dots(basis, ld, j + 1, 0, wout, n, nb, nC, nG, metric, pout, sBuf, sRes);
 }
 
+// The Lanczos vector launches keep a thread's basis entries v_k[i], k < nv <= KMAX, in registers:
+// they are loaded once, all together, and serve both the subtraction and the dot products.
+constexpr int lz_kmax(int nv) { return nv <= 8 ? 8 : nv <= 16 ? 16 : kMaxKrylov; }
+
+// Writes this block's nv + 1 partial sums from acc (slot 0: the extra sum, slots 2 + 2k: k < nv)
+// through the LDS block sums.
+template <int NS>
+__device__ __forceinline__ void lz_store_partials(const double (&acc)[NS], int nv, double* sBuf, double* sRes,
+                                                  double2* __restrict__ partial, int extraAt) {
+  block_sums256<NS>(acc, sBuf, sRes, 2 * (nv + 1));
+  for (int k = threadIdx.x; k <= nv; k += 256)
+    partial[(size_t)blockIdx.x * kPS + k] =
+        k == extraAt ? make_double2(sRes[0], sRes[1]) : make_double2(sRes[2 + 2 * k], sRes[3 + 2 * k]);
+}
+
 // First re-orthogonalisation pass: w -= sum_{k<=j} c_k v_k (c from pin), alpha_j = Re c_j; then
 // the partial sums of the second pass, <v_k|w> for k <= j, and of <w|w>.
+template <int KMAX>
 __global__ __launch_bounds__(256) void k_lz_pass1(const int* __restrict__ ctl, const double2* __restrict__ pin,
                                                   int j, const double2* __restrict__ basis, size_t ld,
                                                   double2* __restrict__ w, int n, int nb, int nC, int nG,
                                                   const double* __restrict__ metric, double* __restrict__ alpha,
                                                   double2* __restrict__ pout) {
+  constexpr int NS = 2 * (KMAX + 1);
   __shared__ double2 sC[kPS], sTmp[4][kPS];
-  __shared__ double sBuf[8 * 256], sRes[16];
+  __shared__ double sBuf[8 * 256], sRes[NS];
   if (ctl[0]) return;
   const int nv = j + 1;
   lz_reduce(pin, nb, nv, sC, sTmp);
   if (blockIdx.x == 0 && threadIdx.x == 0) alpha[j] = sC[j].x;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += nb * 256) w[i] = lz_subtract(w[i], basis, ld, nv, sC, i);
-  lz_dots(basis, ld, nv, 1, w, n, nb, nC, nG, metric, pout, sBuf, sRes);
+  double acc[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) acc[u] = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += nb * 256) {
+    double2 v[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) v[k] = k < nv ? basis[k * ld + i] : make_double2(0.0, 0.0);
+    double2 r = w[i];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      if (k < nv) {
+        const double2 cf = sC[k];
+        r.x -= cf.x * v[k].x - cf.y * v[k].y;
+        r.y -= cf.x * v[k].y + cf.y * v[k].x;
+      }
+    }
+    w[i] = r;
+    const double m = metric_at(i, nC, nG, metric);
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      acc[2 + 2 * k] = fma(m, v[k].x * r.x + v[k].y * r.y, acc[2 + 2 * k]);
+      acc[3 + 2 * k] = fma(m, v[k].x * r.y - v[k].y * r.x, acc[3 + 2 * k]);
+    }
+    acc[0] = fma(m, r.x * r.x + r.y * r.y, acc[0]);
+  }
+  lz_store_partials(acc, nv, sBuf, sRes, pout, nv);
 }
 
 // J_k(a), k = 0..K (0 < a), by Miller's backward recurrence J_{k-1} = (2k/a) J_k - J_{k+1} from
@@ -593,9 +615,12 @@ __device__ double lz_beta(const double2* __restrict__ pin, int nRows, int nv, do
 //   w' = H v_{j+1} - beta_j v_j (as k_lz_apply) and the partial sums of <v_k|w'>_m, k <= j + 1.
 // v_{j+1} is formed on each 256-entry tile and on `halo` = nLoc - 1 entries either side (the
 // reach of the FEDVR stencil) in LDS, so H needs no other workgroup's values; w' goes to the
-// other w buffer, so no workgroup overwrites the w that another one's halo reads.  Workgroup
-// nb (the last) ends iteration j (lz_eval on T_{j+1}) while the others work; when it ends the loop
-// the launch's vector results are never used.  Block 0 records beta_j.
+// other w buffer, so no workgroup overwrites the w that another one's halo reads.  Each thread
+// forms its tile entry and, for threads t < 2 halo, one halo entry, with the loads of both in
+// flight together; the tile entry's basis values stay in registers for the dot products.
+// Workgroup nb (the last) ends iteration j (lz_eval on T_{j+1}) while the others work; when it
+// ends the loop the launch's vector results are never used.  Block 0 records beta_j.
+template <int KMAX>
 __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __restrict__ pin,
                                                  const double* __restrict__ alpha, double* __restrict__ beta, int j,
                                                  int maxDim, const double* __restrict__ dtp, double tol,
@@ -607,9 +632,10 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
                                                  const double* __restrict__ efield, int nC, int nG, int n, int nb,
                                                  const double* __restrict__ metric, double2* __restrict__ pout,
                                                  int halo) {
+  constexpr int NS = 2 * (KMAX + 1);
   __shared__ double2 sWork[3 * kMaxKrylov * kMaxKrylov];  // exponential workspace, or the v tile
   __shared__ double2 sC[kPS], sTmp[4][kPS];
-  __shared__ double sBuf[8 * 256], sRes[16];
+  __shared__ double sBuf[8 * 256], sRes[NS];
   __shared__ double sBeta;
   // (workgroup nb of this launch may set ctl[0]: relaxed atomic load, see lz_eval)
   if (__hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
@@ -623,24 +649,43 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
   if (bj <= kZero) return;  // breakdown: workgroup nb ends the loop
   const double E = *efield;
   double2* vn = basis + (size_t)nv * ld;
-  const double2* vj = basis + (size_t)j * ld;
   double2* sV = sWork;
+  const int t = threadIdx.x;
+  const bool hasHalo = t < 2 * halo;
+  double acc[NS];
+#pragma unroll
+  for (int u = 0; u < NS; ++u) acc[u] = 0.0;
   for (int tile = blockIdx.x; tile * 256 < n; tile += nb) {
     const int i0 = tile * 256, lo = i0 - halo;
-    __syncthreads();  // the previous tile's reads of sV are done
-    for (int t = threadIdx.x; t < 256 + 2 * halo; t += 256) {
-      const int i = lo + t;
-      double2 v = make_double2(0.0, 0.0);
-      if (i >= 0 && i < n) {
-        const double2 r = lz_subtract(wcur[i], basis, ld, nv, sC, i);
-        v = make_double2(r.x / bj, r.y / bj);
-      }
-      sV[t] = v;
+    const int i = i0 + t;
+    // this thread's halo entry and its place in the tile (LDS slot p holds entry lo + p)
+    const int ih = t < halo ? lo + t : i0 + 256 + (t - halo);
+    const int ph = t < halo ? t : 256 + t;
+    const bool inMain = i < n, inHalo = hasHalo && ih >= 0 && ih < n;
+    double2 v[KMAX], vh[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      v[k] = (k < nv && inMain) ? basis[k * ld + i] : make_double2(0.0, 0.0);
+      vh[k] = (k < nv && inHalo) ? basis[k * ld + ih] : make_double2(0.0, 0.0);
     }
+    double2 r = inMain ? wcur[i] : make_double2(0.0, 0.0);
+    double2 rh = inHalo ? wcur[ih] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      if (k < nv) {
+        const double2 cf = sC[k];
+        r.x -= cf.x * v[k].x - cf.y * v[k].y;
+        r.y -= cf.x * v[k].y + cf.y * v[k].x;
+        rh.x -= cf.x * vh[k].x - cf.y * vh[k].y;
+        rh.y -= cf.x * vh[k].y + cf.y * vh[k].x;
+      }
+    }
+    const double2 vi = inMain ? make_double2(r.x / bj, r.y / bj) : make_double2(0.0, 0.0);
+    __syncthreads();  // the previous tile's reads of sV are done
+    sV[t + halo] = vi;
+    if (hasHalo) sV[ph] = inHalo ? make_double2(rh.x / bj, rh.y / bj) : make_double2(0.0, 0.0);
     __syncthreads();
-    const int i = i0 + threadIdx.x;
-    if (i < n) {
-      const double2 vi = sV[threadIdx.x + halo];
+    if (inMain) {
       vn[i] = vi;
       double2 h = cmk(0, 0);
       if (i >= nC) {
@@ -650,13 +695,24 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
         h.x = fma(pot, vi.x, h.x);
         h.y = fma(pot, vi.y, h.y);
       }
-      const double2 p = vj[i];
+      double2 p = make_double2(0.0, 0.0);  // v_j[i], selected without a dynamic register index
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k)
+        if (k == j) p = v[k];
       h.x -= bj * p.x;
       h.y -= bj * p.y;
       wnext[i] = h;
+      const double m = metric_at(i, nC, nG, metric);
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) {
+        acc[2 + 2 * k] = fma(m, v[k].x * h.x + v[k].y * h.y, acc[2 + 2 * k]);
+        acc[3 + 2 * k] = fma(m, v[k].x * h.y - v[k].y * h.x, acc[3 + 2 * k]);
+      }
+      acc[0] = fma(m, vi.x * h.x + vi.y * h.y, acc[0]);  // <v_{j+1}|w'>
+      acc[1] = fma(m, vi.x * h.y - vi.y * h.x, acc[1]);
     }
   }
-  lz_dots(basis, ld, nv + 1, 0, wnext, n, nb, nC, nG, metric, pout, sBuf, sRes);
+  lz_store_partials(acc, nv, sBuf, sRes, pout, nv);
 }
 
 // The end of the last iteration j = maxDim - 1 (one workgroup): beta_j and lz_eval on T_maxDim,
@@ -793,10 +849,13 @@ void enqueue_sil_substep(cody_ctx* c, PropBuffers* b, int maxDim, double tol, in
   hipLaunchKernelGGL(k_lz_apply, g, t, 0, st, b->basis, ld, 0, b->beta, b->ctl, c->d_w, c->d_x, c->d_vext, kp, efp,
                      c->nC, c->nG, n, nb, metric, wb[0], p0);
   for (int j = 0; j < maxDim; ++j) {
-    hipLaunchKernelGGL(k_lz_pass1, g, t, 0, st, b->ctl, p0, j, b->basis, ld, wb[j % 2], n, nb, c->nC, c->nG, metric,
+    const int km = lz_kmax(j + 1);
+    auto pass1 = km == 8 ? k_lz_pass1<8> : km == 16 ? k_lz_pass1<16> : k_lz_pass1<kMaxKrylov>;
+    hipLaunchKernelGGL(pass1, g, t, 0, st, b->ctl, p0, j, b->basis, ld, wb[j % 2], n, nb, c->nC, c->nG, metric,
                        b->alpha, p1);
+    auto next = km == 8 ? k_lz_next<8> : km == 16 ? k_lz_next<16> : k_lz_next<kMaxKrylov>;
     if (j + 1 < maxDim)
-      hipLaunchKernelGGL(k_lz_next, dim3(nb + 1), t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, dtp, tol,
+      hipLaunchKernelGGL(next, dim3(nb + 1), t, 0, st, b->ctl, p1, b->alpha, b->beta, j, maxDim, dtp, tol,
                          b->phi, flags, b->basis, ld, wb[j % 2], wb[(j + 1) % 2], c->d_w, c->d_x, c->d_vext, kp, efp,
                          c->nC, c->nG, n, nb, metric, p0, halo);
     else

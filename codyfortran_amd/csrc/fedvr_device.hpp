@@ -58,17 +58,19 @@ __device__ __forceinline__ double wave_sum(double v) {
 // lanes adds 16 of one sum's values in order, and the 16 group partials meet in a 4-step
 // butterfly.  The result lands in res[s] (LDS).  About ten LDS latencies per round, against six
 // dependent cross-lane steps per sum for NS wave sums.  Fixed order (bit-reproducible).  Every
-// thread calls it.
+// thread calls it.  Only the first nUsed (uniform, <= NS) sums are formed.
 template <int NS>
-__device__ __forceinline__ void block_sums256(const double (&v)[NS], double* sBuf, double* res) {
+__device__ __forceinline__ void block_sums256(const double (&v)[NS], double* sBuf, double* res, int nUsed = NS) {
+  const int ns = nUsed < NS ? nUsed : NS;
 #pragma unroll
   for (int s0 = 0; s0 < NS; s0 += 8) {
+    if (s0 >= ns) break;
     constexpr int kRound = 8;
 #pragma unroll
     for (int s = 0; s < kRound; ++s)
       if (s0 + s < NS) sBuf[s * 256 + threadIdx.x] = v[s0 + s];
     __syncthreads();
-    const int n = NS - s0 < kRound ? NS - s0 : kRound;
+    const int n = ns - s0 < kRound ? ns - s0 : kRound;
     if (threadIdx.x < n * 16) {
       const int s = threadIdx.x >> 4, q = threadIdx.x & 15;
       const double* b = sBuf + s * 256 + 16 * q;
