@@ -65,20 +65,23 @@ __device__ __forceinline__ double metric_at(int i, int nC, int nG, const double*
   return (w == nullptr || i < nC) ? 1.0 : w[(i - nC) % nG];
 }
 
-// sOut[k] = sum_b partial[b][k] for k < nv <= kPS over the nRows block rows, fixed order (wave q
-// takes rows q, q + 4, ..., then the waves in order).  Each wave's rows are loaded 24 at a time
-// before they are added, so up to 192 rows cost two memory round trips.
+// sOut[k] = sum_b partial[b][k] for k < nv <= kPS over the nRows block rows, fixed order: G row
+// groups (8 groups of 32 lanes for nv <= 32, else 4 of 64), group q taking rows q, q + G, ... in
+// order, then the groups in order.  A group's rows are loaded 17 at a time before they are added,
+// so up to 136 rows (8 groups) cost one memory round trip.  sTmp holds 8 x kPS values.
 __device__ void lz_reduce(const double2* __restrict__ partial, int nRows, int nv, double2* sOut,
                           double2 (*sTmp)[kPS]) {
-  constexpr int kBatch = 24;
-  const int t = threadIdx.x, k = t & 63, q = t >> 6;
+  constexpr int kBatch = 17;
+  const int t = threadIdx.x;
+  const int G = nv <= 32 ? 8 : 4, lanes = 256 / G;
+  const int k = t % lanes, q = t / lanes;
   if (k < nv) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int r0 = q; r0 < nRows; r0 += 4 * kBatch) {
+    for (int r0 = q; r0 < nRows; r0 += G * kBatch) {
       double2 p[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        const int r = r0 + 4 * u;
+        const int r = r0 + G * u;
         p[u] = r < nRows ? partial[(size_t)r * kPS + k] : make_double2(0.0, 0.0);
       }
 #pragma unroll
@@ -92,7 +95,7 @@ __device__ void lz_reduce(const double2* __restrict__ partial, int nRows, int nv
   __syncthreads();
   if (t < nv) {
     double2 acc = sTmp[0][t];
-    for (int g = 1; g < 4; ++g) {
+    for (int g = 1; g < G; ++g) {
       acc.x += sTmp[g][t].x;
       acc.y += sTmp[g][t].y;
     }
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(256) void k_lz_norm(const double2* __restrict__ in,
 __global__ __launch_bounds__(256) void k_lz_start(const double2* __restrict__ partial, const double2* __restrict__ in,
                                                   double2* __restrict__ v0, int n, int nb, double* __restrict__ nrm2,
                                                   int* __restrict__ ctl, double* __restrict__ flags) {
-  __shared__ double2 sC[kPS], sTmp[4][kPS];
+  __shared__ double2 sC[kPS], sTmp[8][kPS];
   lz_reduce(partial, nb, 1, sC, sTmp);
   const double s2 = sC[0].x, nr = sqrt(s2);
   if (nr <= kZero) {
@@ -245,11 +248,13 @@ __global__ __launch_bounds__(256) void k_lz_pass1(const int* __restrict__ ctl, c
                                                   const double* __restrict__ metric, double* __restrict__ alpha,
                                                   double2* __restrict__ pout) {
   constexpr int NS = 2 * (KMAX + 1);
-  __shared__ double2 sC[kPS], sTmp[4][kPS];
+  __shared__ double2 sC[kPS], sTmp[8][kPS];
   __shared__ double sBuf[8 * 256], sRes[NS];
-  if (ctl[0]) return;
   const int nv = j + 1;
+  // the loop-control flag is loaded with the partial sums (one round trip), tested after them
+  const int done = ctl[0];
   lz_reduce(pin, nb, nv, sC, sTmp);
+  if (done) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) alpha[j] = sC[j].x;
   double acc[NS];
 #pragma unroll
@@ -634,23 +639,35 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
                                                  int halo) {
   constexpr int NS = 2 * (KMAX + 1);
   __shared__ double2 sWork[3 * kMaxKrylov * kMaxKrylov];  // exponential workspace, or the v tile
-  __shared__ double2 sC[kPS], sTmp[4][kPS];
+  __shared__ double2 sC[kPS], sTmp[8][kPS];
   __shared__ double sBuf[8 * 256], sRes[NS];
   __shared__ double sBeta;
-  // (workgroup nb of this launch may set ctl[0]: relaxed atomic load, see lz_eval)
-  if (__hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  __shared__ double sK[(kMaxStencilReach + 1) * (kMaxStencilReach + 1)];  // K(a, b) of the reference element
+  // (workgroup nb of this launch may set ctl[0]: relaxed atomic load, see lz_eval; loaded with the
+  // partial sums of the beta reduction, tested after them)
+  const int done = __hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int nv = j + 1;
+  const int t = threadIdx.x;
+  // the element matrix goes to LDS (its loads in flight during the reduction): a global K(a, b)
+  // load per stencil neighbour, each waited for before the next, would dominate the apply
+  const int nK2 = kp.nLoc * kp.nLoc;
+  double kv[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) kv[u] = t + 256 * u < nK2 ? kp.kref[t + 256 * u] : 0.0;
+  const double E = *efield;
   const double bj = lz_beta(pin, nb, nv, sC, sTmp, &sBeta);
+  if (done) return;
   if (blockIdx.x == nb) {
     lz_eval(ctl, alpha, beta, nv, bj, maxDim, dtp, tol, phi, flags, sWork);
     return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) beta[j] = bj;
   if (bj <= kZero) return;  // breakdown: workgroup nb ends the loop
-  const double E = *efield;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (t + 256 * u < nK2) sK[t + 256 * u] = kv[u];  // read after the tile's first barrier
   double2* vn = basis + (size_t)nv * ld;
   double2* sV = sWork;
-  const int t = threadIdx.x;
   const bool hasHalo = t < 2 * halo;
   double acc[NS];
 #pragma unroll
@@ -662,6 +679,11 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
     const int ih = t < halo ? lo + t : i0 + 256 + (t - halo);
     const int ph = t < halo ? t : 256 + t;
     const bool inMain = i < n, inHalo = hasHalo && ih >= 0 && ih < n;
+    // orbital block and grid point of the tile entry, with their per-point data
+    const bool orbital = inMain && i >= nC;
+    const int o = orbital ? (i - nC) / nG : 0, g = orbital ? (i - nC) - o * nG : 0;
+    const double wgi = orbital ? wg[g] : 1.0, vxi = orbital ? vext[g] : 0.0, xgi = orbital ? xg[g] : 0.0;
+    const double m = metric_at(i, nC, nG, metric);
     double2 v[KMAX], vh[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
@@ -688,10 +710,9 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
     if (inMain) {
       vn[i] = vi;
       double2 h = cmk(0, 0);
-      if (i >= nC) {
-        const int o = (i - nC) / nG, g = (i - nC) - o * nG;
-        h = kinetic_at_lds(sV, lo - (nC + o * nG), g, kp, 1.0 / wg[g]);
-        const double pot = vext[g] + E * xg[g];
+      if (orbital) {
+        h = kinetic_at_lds_k(sV, lo - (nC + o * nG), g, kp, sK, 1.0 / wgi);
+        const double pot = vxi + E * xgi;
         h.x = fma(pot, vi.x, h.x);
         h.y = fma(pot, vi.y, h.y);
       }
@@ -702,7 +723,6 @@ __global__ __launch_bounds__(256) void k_lz_next(int* ctl, const double2* __rest
       h.x -= bj * p.x;
       h.y -= bj * p.y;
       wnext[i] = h;
-      const double m = metric_at(i, nC, nG, metric);
 #pragma unroll
       for (int k = 0; k < KMAX; ++k) {
         acc[2 + 2 * k] = fma(m, v[k].x * h.x + v[k].y * h.y, acc[2 + 2 * k]);
@@ -722,7 +742,7 @@ __global__ __launch_bounds__(256) void k_lz_last(int* ctl, const double2* __rest
                                                  int maxDim, const double* __restrict__ dtp, double tol,
                                                  double2* __restrict__ phi, double* __restrict__ flags, int nb) {
   __shared__ double2 sWork[3 * kMaxKrylov * kMaxKrylov];
-  __shared__ double2 sC[kPS], sTmp[4][kPS];
+  __shared__ double2 sC[kPS], sTmp[8][kPS];
   __shared__ double sBeta;
   if (ctl[0]) return;
   const double bj = lz_beta(pin, nb, j + 1, sC, sTmp, &sBeta);
