@@ -111,14 +111,21 @@ const int* act_flags(const cody_ctx* c) { return c->skip_zero ? c->d_act : nullp
 // Returns 0, or the launcher's error (nothing of the contraction launched) when the shape is
 // outside what the kernels take.
 int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark,
-                        int* partialSplit = nullptr) {
+                        int* partialSplit = nullptr, bool* reginvDone = nullptr) {
   const size_t ns = (size_t)c->Pc * c->nGp;
   if (partialSplit) *partialSplit = 0;
+  if (reginvDone) *reginvDone = false;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
       if (int8_tiled(c)) {  // 64-row kernel: S slices fragment-tiled, four k ranges
         if (int r = check_contraction_shape("k_mf_i8_w", c->nGp, c->Pc, kInt8Split, 5, c->nGp / 64)) return r;
-        launch_slice_s_tiled(c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
+        if (reginvDone) {  // R beside the slicing (the right-hand side's schedule)
+          launch_rhs_slice(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices,
+                           c->d_Si8, c->d_eS, st);
+          *reginvDone = true;
+        } else {
+          launch_slice_s_tiled(c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
+        }
         mark("k_slice_s_tiled");
         CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, partialSplit ? nullptr : c->d_WT,
                            c->nGp, c->Pc, kInt8Split, c->d_smax, st, act_flags(c)));
@@ -460,8 +467,10 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   mark("k_rhs_front");
   if (double* sr = plain_st(c)) launch_pair_src_rows(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, sr, st);
   int ps = 0;
-  CHK(enqueue_contraction(c, st, mark, &ps));
-  launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st);
+  bool reginvDone = false;
+  CHK(enqueue_contraction(c, st, mark, &ps, &reginvDone));
+  launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st,
+                  !reginvDone);
   mark("k_rhs_wsum");
   launch_rhs_mid(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, orbs, c->d_w, c->d_hphi, c->d_r2can,
                  c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st);

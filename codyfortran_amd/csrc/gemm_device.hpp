@@ -1,11 +1,13 @@
 // Device-side pieces of the MFMA contractions shared by meanfield_gemm.hip and the fused
 // right-hand-side launches of mctdhx_kernels.hip: vector types, the fixed-order LDS reduction of
-// per-wave tiles, and the bodies of the Gram kernel (G = S^T W split-K partials) and of the
-// k-split sum of W^T.  A body takes its workgroup indices as arguments; every thread of the
+// per-wave tiles, and the bodies of the Gram kernel (G = S^T W split-K partials), of the k-split
+// sum of W^T and of the int8 slicing of the tiled S^T.  A body takes its workgroup indices as arguments; every thread of the
 // workgroup calls it.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 #include "mctdhx_kernels.hpp"
 
@@ -177,6 +179,45 @@ __device__ __forceinline__ void wsum_body(int bx, const double* __restrict__ Wpa
   d2_t v = *reinterpret_cast<const d2_t*>(Wpart + i);
   for (int ks = 1; ks < nsplit; ++ks) v += *reinterpret_cast<const d2_t*>(Wpart + ks * plane + i);
   *reinterpret_cast<d2_t*>(WT + i) = v;
+}
+
+// The int8 slices of S^T straight from its FP64 fragment-tiled copy, four consecutive x of one
+// line per thread g (one 32-bit store per slice), in the tiled int8 layout of k_slice_rows; the
+// line's scale comes from smax (per-line max |S^T| from k_pair_src_t), so every thread knows its
+// exponent without a reduction.  The exponent rule and the slicing are those of k_slice_rows.
+__device__ __forceinline__ void slice_s_tiled_body(int g, const double* __restrict__ STt,
+                                                   const unsigned long long* __restrict__ smax, int nGp, int Pc,
+                                                   int ns, int8_t* __restrict__ out, int* __restrict__ expo) {
+  const int perLine = nGp >> 2;
+  if (g >= Pc * perLine) return;
+  const int c = g / perLine, x0 = 4 * (g % perLine);
+  const double m = __longlong_as_double((long long)smax[c]);
+  int e = 0;
+  if (m > 0.0) {
+    int ex;
+    const double fr = frexp(m, &ex);
+    e = (fr == 0.5 ? ex - 1 : ex) + 1;
+  }
+  if (x0 == 0) expo[c] = e;
+  const int nK = nGp >> 4, nK64 = nGp >> 6;
+  // x0..x0+3: (h, e2) = (0,0), (0,1), (1,0), (1,1) of lane 16 ((x0 % 16) / 4) + c % 16
+  const size_t f = ((size_t)(c >> 4) * nK + (x0 >> 4)) * 256 + 2 * (16 * ((x0 & 15) >> 2) + (c & 15));
+  double r[4] = {STt[f], STt[f + 1], STt[f + 128], STt[f + 129]};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) r[u] = ldexp(r[u], -e);
+  const int lane = 16 * ((x0 & 63) >> 4) + (c & 15);
+  const size_t o = ((size_t)(c >> 4) * nK64 + (x0 >> 6)) * ns * 1024 + 16 * lane + (x0 & 15);
+  for (int p = 1; p <= ns; ++p) {
+    unsigned word = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double t = r[u] * kSliceBase;
+      const double q = rint(t);
+      r[u] = t - q;
+      word |= (unsigned)(uint8_t)static_cast<int8_t>(q) << (8 * u);
+    }
+    *reinterpret_cast<unsigned*>(out + o + (size_t)(p - 1) * 1024) = word;
+  }
 }
 
 }  // namespace cody

@@ -683,19 +683,35 @@ __global__ __launch_bounds__(256) void k_rhs_front(const double2* __restrict__ C
   pair_src_t_body(b, orbs, w, nG, nGp, NO, Pc, STt, smax, act);
 }
 
-// After the contraction: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0, eliminated by its first wave)
-// and the k-split sum of W^T (the others; none when the contraction wrote W^T itself).
+// After the contraction: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0 when `first` is 1, eliminated
+// by its first wave) and the k-split sum of W^T (the others; none when the contraction wrote W^T
+// itself).
 template <int NO>
 __global__ __launch_bounds__(256) void k_rhs_wsum(const double2* __restrict__ rho1, double reg,
                                                   double2* __restrict__ reginv, const double* __restrict__ Wpart,
-                                                  int nsplit, size_t plane, double* __restrict__ WT) {
-  if (blockIdx.x == 0) {
+                                                  int nsplit, size_t plane, double* __restrict__ WT, int first) {
+  if (blockIdx.x < first) {
     // one wave: its barriers cost nothing once the other waves have left the workgroup
     if (threadIdx.x >= 64) return;
     reginv_body<NO>(rho1, reg, reginv, 64);
     return;
   }
-  wsum_body(blockIdx.x - 1, Wpart, nsplit, plane, WT);
+  wsum_body(blockIdx.x - first, Wpart, nsplit, plane, WT);
+}
+
+// Before the int8 contraction: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0, one wave) beside the
+// slicing of the tiled S^T (the others), both after the front launch.
+template <int NO>
+__global__ __launch_bounds__(256) void k_rhs_slice(const double2* __restrict__ rho1, double reg,
+                                                   double2* __restrict__ reginv, const double* __restrict__ STt,
+                                                   const unsigned long long* __restrict__ smax, int nGp, int Pc,
+                                                   int ns, int8_t* __restrict__ out, int* __restrict__ expo) {
+  if (blockIdx.x == 0) {
+    if (threadIdx.x >= 64) return;
+    reginv_body<NO>(rho1, reg, reginv, 64);
+    return;
+  }
+  slice_s_tiled_body((blockIdx.x - 1) * blockDim.x + threadIdx.x, STt, smax, nGp, Pc, ns, out, expo);
 }
 
 // After W: the Gram matrix's split-K partials (G = S^T W) and the orbital correlation with its
@@ -754,12 +770,26 @@ void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, do
 }
 
 void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, const double* Wpart, int nsplit,
-                     size_t plane, double* WT, hipStream_t st) {
+                     size_t plane, double* WT, hipStream_t st, bool withReginv) {
   const unsigned nSum = nsplit > 1 ? (unsigned)((plane / 2 + 255) / 256) : 0u;
-#define CALL_WS(NO) \
-  hipLaunchKernelGGL(k_rhs_wsum<NO>, dim3(1 + nSum), dim3(256), 0, st, rho1, reg, reginv, Wpart, nsplit, plane, WT)
+  const int first = withReginv ? 1 : 0;
+  if (nSum + first == 0) return;
+#define CALL_WS(NO)                                                                                        \
+  hipLaunchKernelGGL(k_rhs_wsum<NO>, dim3(first + nSum), dim3(256), 0, st, rho1, reg, reginv, Wpart, nsplit, \
+                     plane, WT, first)
   CODY_NO_SWITCH(nO, CALL_WS)
 #undef CALL_WS
+}
+
+void launch_rhs_slice(const double2* rho1, int nO, double reg, double2* reginv, const double* STt,
+                      const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out, int* expo,
+                      hipStream_t st) {
+  const int n = Pc * (nGp >> 2);
+#define CALL_SL(NO)                                                                                           \
+  hipLaunchKernelGGL(k_rhs_slice<NO>, dim3(1 + (n + 255) / 256), dim3(256), 0, st, rho1, reg, reginv, STt, smax, \
+                     nGp, Pc, ns, out, expo)
+  CODY_NO_SWITCH(nO, CALL_SL)
+#undef CALL_SL
 }
 
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,

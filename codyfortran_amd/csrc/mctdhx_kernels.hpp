@@ -93,7 +93,8 @@ void launch_ci_apply_project(const double2* C, int nC, int nO, int Pc, const dou
                              hipStream_t st);
 // Fused launches of the right-hand side (one queue):
 //   front: CI densities, one-body part (h phi, h1 partials), pair sources (+ maxima, flags);
-//   wsum:  R = rho1 (rho1^2 + eps)^-1 and the k-split sum of W^T (nsplit <= 1: none);
+//   wsum:  R = rho1 (rho1^2 + eps)^-1 (unless the int8 slicing launch formed it) and the k-split
+//          sum of W^T (nsplit <= 1: none);
 //   mid:   Gram partials (nxs x-ranges) and the orbital correlation with its overlap partials;
 //   post:  G and h1 from their partials with the energy partials epart (one per reduction
 //          workgroup, rhs_post_energy_parts of them), and the orbital projection (times -i).
@@ -102,7 +103,12 @@ void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, do
                       const KinLaunch& kl, const double* efield, int nG, double2* hphi, double2* part1, int nGp,
                       int Pc, double* STt, unsigned long long* smax, int* act, hipStream_t st);
 void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, const double* Wpart, int nsplit,
-                     size_t plane, double* WT, hipStream_t st);
+                     size_t plane, double* WT, hipStream_t st, bool withReginv = true);
+// the int8 slicing of the tiled S^T (launch_slice_s_tiled) with R = rho1 (rho1^2 + eps)^{-1} on
+// one more workgroup
+void launch_rhs_slice(const double2* rho1, int nO, double reg, double2* reginv, const double* STt,
+                      const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out, int* expo,
+                      hipStream_t st);
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
                     const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
                     const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st);

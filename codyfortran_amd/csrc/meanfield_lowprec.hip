@@ -14,6 +14,7 @@
 //    v_mfma_f32_16x16x32_bf16, FP32 accumulation over 64-wide k chunks flushed into FP64.
 #include <hip/hip_runtime.h>
 
+#include "gemm_device.hpp"
 #include "mctdhx_kernels.hpp"
 
 namespace cody {
@@ -77,45 +78,11 @@ __global__ __launch_bounds__(1024) void k_slice_rows(const double* __restrict__ 
 }
 
 // ------------------------------------------------------------------ slicing of the tiled S^T
-// The int8 slices of S^T straight from its FP64 fragment-tiled copy, four consecutive x of one
-// line per thread (one 32-bit store per slice), in the tiled int8 layout of k_slice_rows; the
-// line's scale comes from smax (per-line max |S^T| from k_pair_src_t), so every thread knows its
-// exponent without a reduction.  The exponent rule and the slicing are those of k_slice_rows.
 __global__ __launch_bounds__(256) void k_slice_s_tiled(const double* __restrict__ STt,
                                                        const unsigned long long* __restrict__ smax, int nGp,
                                                        int Pc, int ns, int8_t* __restrict__ out,
                                                        int* __restrict__ expo) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int perLine = nGp >> 2;
-  if (g >= Pc * perLine) return;
-  const int c = g / perLine, x0 = 4 * (g % perLine);
-  const double m = __longlong_as_double((long long)smax[c]);
-  int e = 0;
-  if (m > 0.0) {
-    int ex;
-    const double fr = frexp(m, &ex);
-    e = (fr == 0.5 ? ex - 1 : ex) + 1;
-  }
-  if (x0 == 0) expo[c] = e;
-  const int nK = nGp >> 4, nK64 = nGp >> 6;
-  // x0..x0+3: (h, e2) = (0,0), (0,1), (1,0), (1,1) of lane 16 ((x0 % 16) / 4) + c % 16
-  const size_t f = ((size_t)(c >> 4) * nK + (x0 >> 4)) * 256 + 2 * (16 * ((x0 & 15) >> 2) + (c & 15));
-  double r[4] = {STt[f], STt[f + 1], STt[f + 128], STt[f + 129]};
-#pragma unroll
-  for (int u = 0; u < 4; ++u) r[u] = ldexp(r[u], -e);
-  const int lane = 16 * ((x0 & 63) >> 4) + (c & 15);
-  const size_t o = ((size_t)(c >> 4) * nK64 + (x0 >> 6)) * ns * 1024 + 16 * lane + (x0 & 15);
-  for (int p = 1; p <= ns; ++p) {
-    unsigned word = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double t = r[u] * kSliceBase;
-      const double q = rint(t);
-      r[u] = t - q;
-      word |= (unsigned)(uint8_t)static_cast<int8_t>(q) << (8 * u);
-    }
-    *reinterpret_cast<unsigned*>(out + o + (size_t)(p - 1) * 1024) = word;
-  }
+  slice_s_tiled_body(blockIdx.x * blockDim.x + threadIdx.x, STt, smax, nGp, Pc, ns, out, expo);
 }
 
 // ------------------------------------------------------------------ FP64 tiling
