@@ -19,8 +19,20 @@ __device__ __forceinline__ void ci_rdm_body(int bx, const double2* __restrict__ 
                                                 double2* __restrict__ rho2) {
   const int n2 = nO * nO, np = nO * (nO - 1) / 2, nCan = np * np;
   const int item = bx * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63;
-  extern __shared__ double2 sC[];  // C staged once per workgroup
-  for (int i = threadIdx.x; i < nC; i += blockDim.x) sC[i] = C[i];
+  extern __shared__ double2 sC[];  // C staged once per workgroup, four loads per thread in flight
+  for (int i0 = threadIdx.x; i0 < nC; i0 += 4 * blockDim.x) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * blockDim.x;
+      v[u] = i < nC ? C[i] : cmk(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * blockDim.x;
+      if (i < nC) sC[i] = v[u];
+    }
+  }
   __syncthreads();
   if (item >= n2 + nCan) return;
   int code, i1 = 0, i2 = 0, j1 = 0, j2 = 0;
@@ -39,9 +51,17 @@ __device__ __forceinline__ void ci_rdm_body(int bx, const double2* __restrict__ 
     tab = r2;
   }
   double2 acc = cmk(0, 0);
-  for (int q = ptr[code] + ln; q < ptr[code + 1]; q += 64) {
-    const CiEntryDev en = tab[q];
-    cfma(acc, cscale(cconj(sC[en.jc]), en.f), sC[en.ic]);
+  const int q1 = ptr[code + 1];
+  for (int q0 = ptr[code] + ln; q0 < q1; q0 += 4 * 64) {  // four table entries' loads in flight
+    CiEntryDev en[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + 64 * u;
+      en[u] = q < q1 ? tab[q] : CiEntryDev{0, 0, 0, 0.0f};
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (q0 + 64 * u < q1) cfma(acc, cscale(cconj(sC[en[u].jc]), en[u].f), sC[en[u].ic]);
   }
   acc.x = wave_sum(acc.x);
   acc.y = wave_sum(acc.y);

@@ -82,12 +82,11 @@ __global__ __launch_bounds__(256) void k_ci_apply(const double2* __restrict__ C,
 // k_ci_apply) and dC = -i (dCraw - E C) (Coeffs_ProjectOnSubspace, src/Coeffs/S_Coeffs.f90:105-111),
 // with <C|dCraw> = <C|H C> = E taken from the nE energy partials of the G reduction (summed in a
 // fixed order by every wave) instead of a reduction over dCraw.
-__global__ __launch_bounds__(256) void k_ci_apply_project(const double2* __restrict__ C, int nC, int nO, int Pc,
-                                                          const double* __restrict__ G,
-                                                          const double2* __restrict__ h1, DevCiTables t,
-                                                          const double* __restrict__ epart, int nE,
-                                                          double2* __restrict__ dCraw, double2* __restrict__ dC) {
-  const int ic = blockIdx.x * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63;
+__device__ __forceinline__ void ci_apply_project_body(int bx, const double2* __restrict__ C, int nC, int nO, int Pc,
+                                                      const double* __restrict__ G, const double2* __restrict__ h1,
+                                                      const DevCiTables& t, const double* __restrict__ epart, int nE,
+                                                      double2* __restrict__ dCraw, double2* __restrict__ dC) {
+  const int ic = bx * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63;
   if (ic >= nC) return;
   // the energy partials are loaded before the configuration's sums and added after them
   double e = 0.0;
@@ -98,6 +97,22 @@ __global__ __launch_bounds__(256) void k_ci_apply_project(const double2* __restr
     dCraw[ic] = acc;
     dC[ic] = mul_minus_i(csub(acc, cscale(C[ic], e)));
   }
+}
+
+__global__ __launch_bounds__(256) void k_ci_apply_project(const double2* __restrict__ C, int nC, int nO, int Pc,
+                                                          const double* __restrict__ G,
+                                                          const double2* __restrict__ h1, DevCiTables t,
+                                                          const double* __restrict__ epart, int nE,
+                                                          double2* __restrict__ dCraw, double2* __restrict__ dC) {
+  ci_apply_project_body(blockIdx.x, C, nC, nO, Pc, G, h1, t, epart, nE, dCraw, dC);
+}
+
+// the CI H-apply and projection of the instances of a grouped right-hand side
+__global__ __launch_bounds__(256) void k_grp_apply(const GroupInstance* __restrict__ gi, const int2* __restrict__ map,
+                                                   int nO) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  ci_apply_project_body(m.y, a.C, a.nC, nO, a.Pc, a.G, a.h1, a.t, a.epart, a.nRed, a.dCraw, a.dC);
 }
 
 // the full h2(i1,a,j1,b) tensor from G (the h2_ output of ApplyCorrelationOp,
@@ -241,6 +256,10 @@ void launch_ci_apply_project(const double2* C, int nC, int nO, int Pc, const dou
                              hipStream_t st) {
   hipLaunchKernelGGL(k_ci_apply_project, dim3((nC + 3) / 4), dim3(256), 0, st, C, nC, nO, Pc, G, h1, t, epart, nE,
                      dCraw, dC);
+}
+
+void launch_grp_apply(const GroupInstance* gi, const int2* map, int nBlocks, int nO, hipStream_t st) {
+  hipLaunchKernelGGL(k_grp_apply, dim3(nBlocks), dim3(256), 0, st, gi, map, nO);
 }
 
 void launch_dipole(int nO, const double2* part1, int nPart, const double2* rho1, double* dipole, hipStream_t st) {

@@ -949,6 +949,149 @@ int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* nam
 }
 
 // ---------------------------------------------------------------- batches of instances
+namespace {
+
+// Device time of one right-hand side, for the assignment of work to chains: the contraction at
+// ~50 TFLOP/s plus ~50 us of latency-bound launches.
+double rhs_cost_estimate(const cody_ctx* c) { return 2.0 * c->nGp * (double)c->nGp * c->Pc / 50e6 + 50.0; }
+
+// Partitions the batch: instances on the FP64 or the tiled int8 contraction are grouped by
+// (orbital count, contraction), each group getting its argument table and block maps in device
+// memory; the others (the split-FP32 / BF16 forms, the untiled int8 kernel) keep their own
+// launches.  The maps repeat each instance's own launch grids (launch_rhs_front, the slicing,
+// launch_mf_f64_w / launch_mf_i8_w, launch_rhs_wsum, launch_rhs_mid, launch_rhs_post,
+// launch_ci_apply_project), so a grouped workgroup does what that instance's own would.
+int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dstates) {
+  b->release_groups();
+  const int n = static_cast<int>(b->ctx.size());
+  for (int i = 0; i < n; ++i) {
+    cody_ctx* c = b->ctx[i];
+    const bool fp64 = c->precision == CODY_PREC_FP64, i8 = int8_tiled(c);
+    if (!fp64 && !i8) {
+      b->singles.push_back(i);
+      continue;
+    }
+    BatchGroup* grp = nullptr;
+    for (auto& g : b->groups)
+      if (g.nO == c->nO && g.int8 == i8) grp = &g;
+    if (!grp) {
+      b->groups.emplace_back();
+      grp = &b->groups.back();
+      grp->nO = c->nO;
+      grp->Pc = c->Pc;
+      grp->int8 = i8;
+    }
+    grp->members.push_back(i);
+  }
+  for (auto& g : b->groups) {
+    std::vector<GroupInstance> gi;
+    std::vector<int2> maps[BatchGroup::kKinds];
+    const int nO = g.nO, n2 = nO * nO, np = nO * (nO - 1) / 2;
+    for (int mi = 0; mi < (int)g.members.size(); ++mi) {
+      const int i = g.members[mi];
+      cody_ctx* c = b->ctx[i];
+      GroupInstance a{};
+      a.C = static_cast<const double2*>(d_states[i]);
+      a.dC = static_cast<double2*>(d_dstates[i]);
+      a.nC = c->nC;
+      a.nG = c->nG;
+      a.nGp = c->nGp;
+      a.Pc = c->Pc;
+      a.nRdm = (n2 + np * np + 3) / 4;
+      a.nObx = (c->nG + 255) / 256;
+      a.nsplit = g.int8 ? kInt8Split : c->wsplit;
+      a.wsumFirst = g.int8 ? 0 : 1;  // (the int8 path forms R beside the slicing)
+      a.nxs = c->nxs;
+      a.nPart = c->nPart;
+      a.nPart1 = (c->nG + 255) / 256;
+      a.nRed = rhs_post_energy_parts(c->Pc, nO);
+      a.t = c->dci;
+      a.rho1 = c->d_rho1;
+      a.r2can = c->d_r2can;
+      a.rho2 = c->d_rho2;
+      a.hphi = c->d_hphi;
+      a.part1 = c->d_part1;
+      a.reginv = c->d_reginv;
+      a.draw = c->d_draw;
+      a.part = c->d_part;
+      a.h1 = c->d_h1;
+      a.dCraw = c->d_dCraw;
+      a.w = c->d_w;
+      a.x = c->d_x;
+      a.vext = c->d_vext;
+      a.efield = efield_src(c);
+      a.kin = c->kin;
+      a.reg = c->reg;
+      a.STt = c->d_STt;
+      a.Wpart = c->d_Wpart;
+      a.WT = c->d_WT;
+      a.Gpart = c->d_Gpart;
+      a.G = c->d_G;
+      a.epart = c->d_epart;
+      a.smax = st_max(c);
+      a.act = c->d_act;
+      a.actC = c->nGp / 16 > kSkipMaxBlocks16 ? nullptr : act_flags(c);
+      a.Vt = c->d_Vt;
+      a.Vi8 = c->d_Vi8;
+      a.eV = c->d_eV;
+      a.Si8 = c->d_Si8;
+      a.eS = c->d_eS;
+      if (g.int8) {
+        CHK(check_contraction_shape("k_mf_i8_w", c->nGp, c->Pc, kInt8Split, 5, c->nGp / 64));
+      } else {
+        const int kh = c->Pc / 16 <= 5 ? 2 : 1;
+        CHK(check_contraction_shape("k_mf_f64_w", c->nGp, c->Pc, a.nsplit, 10, c->nGp / 16 / kh));
+      }
+      gi.push_back(a);
+      g.maxC = std::max(g.maxC, c->nC);
+      g.cost += rhs_cost_estimate(c) - 40.0;  // (the launches' latency is shared by the group)
+      const size_t plane = (size_t)c->Pc * c->nGp;
+      const int cnt[BatchGroup::kKinds] = {
+          a.nRdm + a.nObx * nO + (int)(plane / 256),
+          g.int8 ? 1 + (c->Pc * (c->nGp >> 2) + 255) / 256 : 0,
+          c->nGp / 64 * a.nsplit,
+          a.wsumFirst + (a.nsplit > 1 ? (int)((plane / 2 + 255) / 256) : 0),
+          c->Pc / 16 * c->nxs + c->nGp / 64 * nO,
+          a.nRed + (c->nG + 255) / 256,
+          (c->nC + 3) / 4};
+      for (int k = 0; k < BatchGroup::kKinds; ++k)
+        for (int lb = 0; lb < cnt[k]; ++lb) maps[k].push_back(make_int2(mi, lb));
+    }
+    g.cost += 40.0;
+    std::vector<int2> all;
+    for (int k = 0; k < BatchGroup::kKinds; ++k) {
+      g.off[k] = static_cast<int>(all.size());
+      g.cnt[k] = static_cast<int>(maps[k].size());
+      all.insert(all.end(), maps[k].begin(), maps[k].end());
+    }
+    HIPCHK(hipMalloc((void**)&g.d_inst, sizeof(GroupInstance) * gi.size()));
+    HIPCHK(hipMalloc((void**)&g.d_map, sizeof(int2) * all.size()));
+    HIPCHK(hipMemcpy(g.d_inst, gi.data(), sizeof(GroupInstance) * gi.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.d_map, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
+  }
+  return CODY_OK;
+}
+
+// The right-hand sides of a group's instances: one launch per kernel type.
+int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
+  auto map = [&](int k) { return g.d_map + g.off[k]; };
+  launch_grp_front(g.nO, g.d_inst, map(BatchGroup::kFront), g.cnt[BatchGroup::kFront], g.maxC, st);
+  if (g.int8) {
+    launch_grp_slice(g.nO, g.d_inst, map(BatchGroup::kSlice), g.cnt[BatchGroup::kSlice], st);
+    CHK(launch_grp_mf_i8_w(g.Pc, g.d_inst, map(BatchGroup::kContract), g.cnt[BatchGroup::kContract], st));
+  } else {
+    CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(BatchGroup::kContract), g.cnt[BatchGroup::kContract], st));
+  }
+  launch_grp_wsum(g.nO, g.d_inst, map(BatchGroup::kWsum), g.cnt[BatchGroup::kWsum], st);
+  launch_grp_mid(g.nO, g.d_inst, map(BatchGroup::kMid), g.cnt[BatchGroup::kMid], st);
+  launch_grp_post(g.nO, g.d_inst, map(BatchGroup::kPost), g.cnt[BatchGroup::kPost], st);
+  launch_grp_apply(g.d_inst, map(BatchGroup::kApply), g.cnt[BatchGroup::kApply], g.nO, st);
+  HIPCHK(hipGetLastError());
+  return CODY_OK;
+}
+
+}  // namespace
+
 int cody_batch_create(cody_batch** out, int n, cody_ctx* const* ctxs) {
   if (!out || n < 1 || !ctxs) return fail(CODY_ERR_ARG, "cody_batch_create: bad arguments");
   for (int i = 0; i < n; ++i) {
@@ -1004,34 +1147,31 @@ int cody_batch_time_derivative_dev(cody_batch* b, void* const* d_states, void* c
   if (stale) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
-    // one graph: the instances' right-hand sides on kChains parallel chains, each instance with
-    // all of its launches on its chain's queue
-    // chains by longest-first greedy assignment of an estimated cost (the contraction at ~50
-    // TFLOP/s plus ~50 us of latency-bound launches per instance), largest instances first
-    std::vector<int> order(n), chainOf(n);
-    std::vector<double> cost(n), load(cody_batch::kChains, 0.0);
-    for (int i = 0; i < n; ++i) {
-      const cody_ctx* c = b->ctx[i];
-      cost[i] = 2.0 * c->nGp * (double)c->nGp * c->Pc / 50e6 + 50.0;
-      order[i] = i;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
-    for (int i : order) {
-      const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-      chainOf[i] = k;
-      load[k] += cost[i];
-    }
+    CHK(batch_build_groups(b, d_states, d_dstates));
+    // one graph: each group's grouped launches on one chain, the other instances' own launches on
+    // the chains too; groups and instances by longest-first greedy assignment of an estimated cost
+    std::vector<std::pair<double, int>> items;  // (cost, group index g >= 0, or -1 - instance)
+    for (int g = 0; g < (int)b->groups.size(); ++g) items.emplace_back(b->groups[g].cost, g);
+    for (int i : b->singles) items.emplace_back(rhs_cost_estimate(b->ctx[i]), -1 - i);
+    std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& c) { return a.first > c.first; });
+    std::vector<double> load(cody_batch::kChains, 0.0);
     HIPCHK(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
     hipEventRecord(b->fork, b->cs);
     for (int k = 0; k < cody_batch::kChains; ++k) hipStreamWaitEvent(b->chain[k], b->fork, 0);
     int r = CODY_OK;
-    for (int q = 0; q < n && r == CODY_OK; ++q) {
-      const int i = order[q];
-      cody_ctx* c = b->ctx[i];
-      c->capturing = true;
-      r = enqueue_rhs(c, static_cast<const double2*>(d_states[i]), static_cast<double2*>(d_dstates[i]), t,
-                      kModeFull, b->chain[chainOf[i]], nullptr, nullptr);
-      c->capturing = false;
+    for (const auto& it : items) {
+      if (r != CODY_OK) break;
+      const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      load[k] += it.first;
+      if (it.second >= 0) {
+        r = enqueue_group_rhs(b->groups[it.second], b->chain[k]);
+      } else {
+        cody_ctx* c = b->ctx[-1 - it.second];
+        c->capturing = true;
+        r = enqueue_rhs(c, static_cast<const double2*>(d_states[-1 - it.second]),
+                        static_cast<double2*>(d_dstates[-1 - it.second]), t, kModeFull, b->chain[k], nullptr, nullptr);
+        c->capturing = false;
+      }
     }
     for (int k = 0; k < cody_batch::kChains; ++k) {
       hipEventRecord(b->join[k], b->chain[k]);

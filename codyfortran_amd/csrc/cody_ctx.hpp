@@ -288,6 +288,19 @@ struct cody_ctx {
 };
 
 // Several contexts whose right-hand sides are replayed as one captured graph.
+// Instances of a batch with one orbital count and one contraction path, evaluated by one launch
+// per kernel type: their argument table and block maps in device memory.
+struct BatchGroup {
+  enum { kFront, kSlice, kContract, kWsum, kMid, kPost, kApply, kKinds };
+  int nO = 0, Pc = 0, maxC = 0;
+  bool int8 = false;
+  double cost = 0.0;  // estimated device time (for the assignment to chains)
+  std::vector<int> members;
+  cody::GroupInstance* d_inst = nullptr;
+  int2* d_map = nullptr;  // the kinds' maps back to back
+  int off[kKinds] = {}, cnt[kKinds] = {};
+};
+
 struct cody_batch {
   static constexpr int kChains = 4;  // parallel chains (the device exposes four hardware queues)
   std::vector<cody_ctx*> ctx;
@@ -298,7 +311,18 @@ struct cody_batch {
   hipEvent_t fork = nullptr;
   hipEvent_t join[kChains] = {};
   hipGraphExec_t exec = nullptr;
+  std::vector<BatchGroup> groups;  // grouped instances
+  std::vector<int> singles;        // instances evaluated by their own launches
+  void release_groups() {
+    for (auto& g : groups) {
+      if (g.d_inst) hipFree(g.d_inst);
+      if (g.d_map) hipFree(g.d_map);
+    }
+    groups.clear();
+    singles.clear();
+  }
   void release() {
+    release_groups();
     if (exec) hipGraphExecDestroy(exec);
     exec = nullptr;
     for (int k = 0; k < kChains; ++k) {

@@ -278,8 +278,22 @@ __device__ __forceinline__ void orbital_body(int bx, int j, const double2* __res
   __shared__ double2 sRed[NO];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x = bx * 64 + lane;
-  // canonical rho2 (i1 < i2, j1 < j2), column j of R and the pair table
-  for (int e = threadIdx.x; e < nCan; e += 256) s2[e] = r2can[e];
+  // canonical rho2 (i1 < i2, j1 < j2), column j of R and the pair table (each thread's loads of
+  // the canonical rho2 in flight together)
+  {
+    constexpr int kStage = nCan > 0 ? (nCan + 255) / 256 : 1;
+    double2 v[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      v[u] = e < nCan ? r2can[e] : cmk(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < nCan) s2[e] = v[u];
+    }
+  }
   for (int e = threadIdx.x; e < n2; e += 256) {
     const int a = e % NO, b = e / NO;
     sPair[e] = a == b ? kNone
@@ -752,6 +766,112 @@ __global__ __launch_bounds__(256) void k_rhs_post(const double* __restrict__ Gpa
     return;
   }
   project_body<NO>(b - nRed, false, orbs, draw, part, nPart, nG, 1, dorbs, nullptr, nullptr, 0, nullptr, hsub);
+}
+
+// ------------------------------------------------------------------ grouped launches
+// The fused launches above over several instances of one orbital count: workgroup b runs block
+// map[b].y of instance map[b].x with that instance's arguments, exactly as that instance's own
+// launch would (same bodies, same block decomposition, so the results are bit-identical).
+template <int NO>
+__global__ __launch_bounds__(256) void k_grp_front(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  int b = m.y;
+  if (b < a.nRdm) {
+    ci_rdm_body(b, a.C, a.nC, NO, a.t.r1_ptr, a.t.r1, a.t.r2_ptr, a.t.r2, a.rho1, a.r2can, a.rho2);
+    return;
+  }
+  b -= a.nRdm;
+  const double2* orbs = a.C + a.nC;
+  if (b < a.nObx * NO) {
+    const KinParams kp{a.kin.kref, a.kin.nE, a.kin.nLoc, a.kin.xminExcl, a.nG, a.kin.invh, a.kin.fac};
+    onebody_body<NO>(b % a.nObx, b / a.nObx, orbs, a.w, a.x, a.vext, kp, a.efield, a.nG, a.hphi, a.part1);
+    return;
+  }
+  b -= a.nObx * NO;
+  pair_src_t_body(b, orbs, a.w, a.nG, a.nGp, NO, a.Pc, a.STt, a.smax, a.act);
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_grp_slice(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  if (m.y == 0) {
+    if (threadIdx.x >= 64) return;
+    reginv_body<NO>(a.rho1, a.reg, a.reginv, 64);
+    return;
+  }
+  slice_s_tiled_body((m.y - 1) * blockDim.x + threadIdx.x, a.STt, a.smax, a.nGp, a.Pc, kInt8Slices, a.Si8, a.eS);
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_grp_wsum(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  if (m.y < a.wsumFirst) {
+    if (threadIdx.x >= 64) return;
+    reginv_body<NO>(a.rho1, a.reg, a.reginv, 64);
+    return;
+  }
+  wsum_body(m.y - a.wsumFirst, a.Wpart, a.nsplit, (size_t)a.Pc * a.nGp, a.WT);
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_grp_mid(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  constexpr int NT = (NO * (NO + 1) + 15) / 16;
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  const int nGram = NT * a.nxs;
+  int b = m.y;
+  if (b < nGram) {
+    gram_f64_body<NT>(b % NT, b / NT, a.STt, a.nGp >> 4, a.WT, a.nGp, a.Gpart, a.Pc, a.nGp / (4 * a.nxs),
+                      (size_t)a.Pc * a.Pc);
+    return;
+  }
+  b -= nGram;
+  const int nOrbX = a.nGp / 64;
+  orbital_body<NO>(b % nOrbX, b / nOrbX, a.C + a.nC, a.w, a.hphi, a.WT, a.r2can, a.reginv, a.nG, a.nGp, kModeFull,
+                   a.draw, a.part);
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_grp_post(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  if (m.y < a.nRed) {
+    gram_reduce_energy_body(m.y, a.Gpart, a.nxs, a.Pc, a.G, a.part1, a.nPart1, NO, a.h1, a.rho1, a.rho2, a.epart);
+    return;
+  }
+  project_body<NO>(m.y - a.nRed, false, a.C + a.nC, a.draw, a.part, a.nPart, a.nG, 1, a.dC + a.nC, nullptr, nullptr, 0,
+                   nullptr, nullptr);
+}
+
+void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlocks, int maxC, hipStream_t st) {
+#define CALL_GF(NO) \
+  hipLaunchKernelGGL(k_grp_front<NO>, dim3(nBlocks), dim3(256), sizeof(double2) * maxC, st, gi, map)
+  CODY_NO_SWITCH(nO, CALL_GF)
+#undef CALL_GF
+}
+void launch_grp_slice(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GS(NO) hipLaunchKernelGGL(k_grp_slice<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+  CODY_NO_SWITCH(nO, CALL_GS)
+#undef CALL_GS
+}
+void launch_grp_wsum(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+  if (nBlocks == 0) return;
+#define CALL_GW(NO) hipLaunchKernelGGL(k_grp_wsum<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+  CODY_NO_SWITCH(nO, CALL_GW)
+#undef CALL_GW
+}
+void launch_grp_mid(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GM(NO) hipLaunchKernelGGL(k_grp_mid<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+  CODY_NO_SWITCH(nO, CALL_GM)
+#undef CALL_GM
+}
+void launch_grp_post(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GP(NO) hipLaunchKernelGGL(k_grp_post<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+  CODY_NO_SWITCH(nO, CALL_GP)
+#undef CALL_GP
 }
 
 void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, double2* rho1, double2* r2can,

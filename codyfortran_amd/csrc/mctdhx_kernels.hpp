@@ -54,6 +54,45 @@ struct KinLaunch {
   double invh, fac;
 };
 
+// One instance of a grouped right-hand side: independent systems with one orbital count are
+// evaluated by one launch per kernel type, each workgroup taking its instance and its block index
+// within that instance's own launch from a block map (int2: instance, local block).  The fields
+// are the arguments of the fused launches of that instance.
+struct GroupInstance {
+  const double2* C;  // state: C, then the orbitals
+  double2* dC;       // dState, same layout
+  int nC, nG, nGp, Pc;
+  int nRdm, nObx;         // front roles: CI density blocks, one-body blocks per orbital
+  int nsplit;             // contraction k ranges (1: the contraction writes W^T itself)
+  int wsumFirst;          // 1: the k-split-sum launch also forms R (workgroup 0)
+  int nxs, nPart, nPart1, nRed;
+  DevCiTables t;
+  double2 *rho1, *r2can, *rho2, *hphi, *part1, *reginv, *draw, *part, *h1, *dCraw;
+  const double *w, *x, *vext, *efield;
+  KinLaunch kin;
+  double reg;
+  double *STt, *Wpart, *WT, *Gpart, *G, *epart;
+  unsigned long long* smax;  // int8 path: per-line maxima of S^T
+  int* act;                  // pair-source activity flags (written by the front launch)
+  const int* actC;           // the flags the contraction reads (null: no skipping)
+  const double* Vt;                          // FP64 contraction operand
+  const int8_t* Vi8;                         // int8 contraction operands
+  const int* eV;
+  int8_t* Si8;
+  int* eS;
+};
+
+// Grouped launches over the instances of one orbital count nO (the map holds nBlocks entries)
+void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlocks, int maxC, hipStream_t st);
+void launch_grp_slice(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+void launch_grp_wsum(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+void launch_grp_mid(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+void launch_grp_post(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+void launch_grp_apply(const GroupInstance* gi, const int2* map, int nBlocks, int nO, hipStream_t st);
+// the contractions, Pc = 16 NT for every instance of the group (CODY_ERR_ARG otherwise)
+int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+int launch_grp_mf_i8_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+
 void launch_zero(double2* p, int n, hipStream_t st);
 // y += x (n complex values)
 void launch_accumulate(double2* y, const double2* x, int n, hipStream_t st);

@@ -18,6 +18,7 @@
 
 #include <cstdio>
 
+#include "../../include/cody_mctdhx.h"
 #include "gemm_device.hpp"
 
 namespace cody {
@@ -58,10 +59,11 @@ __global__ __launch_bounds__(256) void k_gram_f64(const double* __restrict__ At,
 // kernel rather than by the last workgroup of a row group: the device-scope fences that a
 // last-arriver scheme needs write back and invalidate whole L2 caches on this GPU, which cost
 // more than the extra launch.
+// The body takes the workgroup's row group bx, k range by of nsplit (the grid of k_mf_f64_w).
 template <int NT>
-__global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
-    const double* __restrict__ Vt, const double* __restrict__ STt, int nGp, double* __restrict__ Wpart,
-    const int* __restrict__ act) {
+__device__ __forceinline__ void mf_f64_w_body(int bx, int by, int nsplit, const double* __restrict__ Vt,
+                                              const double* __restrict__ STt, int nGp,
+                                              double* __restrict__ Wpart, const int* __restrict__ act) {
   constexpr int KH = NT <= 5 ? 2 : 1, W = 4 * KH, D = 4, LD = NT * 16 + 1;
   constexpr int NCH = KH * NT * 2, CPW = (NCH + W - 1) / W;  // 1 KB S^T chunks per block, per wave
   constexpr int BUF = W * CPW * 128;  // doubles per LDS buffer (chunks NCH.. are never read)
@@ -74,12 +76,12 @@ __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
   const int nK = nGp >> 4;
   // k blocks of each (k range, sub-range): consecutive positions of the active-block list, or of
   // all blocks when act is null
-  const int nbw = (act ? active_blocks(act, NT, nK, 1, nK, gridDim.y * KH, sList, sWave) : nK) / (gridDim.y * KH);
+  const int nbw = (act ? active_blocks(act, NT, nK, 1, nK, nsplit * KH, sList, sWave) : nK) / (nsplit * KH);
   auto kblock = [&](int hh, int blk) {
-    const int j = (blockIdx.y * KH + hh) * nbw + min(blk, nbw - 1);
+    const int j = (by * KH + hh) * nbw + min(blk, nbw - 1);
     return act ? sList[j] : j;
   };
-  const double* ab = Vt + (size_t)(blockIdx.x * 4 + r) * nK * 256 + 2 * lane;
+  const double* ab = Vt + (size_t)(bx * 4 + r) * nK * 256 + 2 * lane;
   // this wave's S^T chunks: chunk c = wave + W m = (hh NT + t) 2 + half of every block (chunks
   // past NCH are clamped onto the last one for the load and not written)
   auto chunk_src = [&](int m, int blk) {
@@ -167,13 +169,20 @@ __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
   }
   __syncthreads();
   // store this k split's 64 x (16 NT) partial transposed: 64 consecutive rows per column
-  const int row0 = blockIdx.x * 64;
+  const int row0 = bx * 64;
   const size_t plane = (size_t)NT * 16 * nGp;
-  double* Wp = Wpart + blockIdx.y * plane;  // (the plane of a single k range is W^T itself)
+  double* Wp = Wpart + by * plane;  // (the plane of a single k range is W^T itself)
   for (int e = threadIdx.x; e < 64 * NT * 16; e += blockDim.x) {
     const int c = e >> 6, rl = e & 63;
     Wp[(size_t)c * nGp + row0 + rl] = lds[(rl >> 4) * 16 * LD + (rl & 15) * LD + c];
   }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_mf_f64_w(
+    const double* __restrict__ Vt, const double* __restrict__ STt, int nGp, double* __restrict__ Wpart,
+    const int* __restrict__ act) {
+  mf_f64_w_body<NT>(blockIdx.x, blockIdx.y, gridDim.y, Vt, STt, nGp, Wpart, act);
 }
 
 // W^T = sum of the nsplit partials, in fixed order (two doubles per thread).
@@ -194,16 +203,16 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, 
 // compile-time constants.  Products with p + q <= NS - 1 that share the scale 254^{-(p+q+2)}
 // accumulate exactly in one int32 tile; the diagonals are recombined in FP64 with the row and
 // column exponents, and the workgroup stores its FP64 partial (k_wsum adds the k ranges).
+// The body takes the workgroup's row group bx, k range by of nsplit (the grid of k_mf_i8_w).
 template <int NT>
-__global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, const int* __restrict__ eA,
-                                                 const int8_t* __restrict__ Bs, const int* __restrict__ fB,
-                                                 int nGp, double* __restrict__ Wpart,
-                                                 unsigned long long* __restrict__ smax,
-                                                 const int* __restrict__ act) {
+__device__ __forceinline__ void mf_i8_w_body(int bx, int by, int nsplit, const int8_t* __restrict__ As,
+                                             const int* __restrict__ eA, const int8_t* __restrict__ Bs,
+                                             const int* __restrict__ fB, int nGp, double* __restrict__ Wpart,
+                                             unsigned long long* __restrict__ smax, const int* __restrict__ act) {
   constexpr int NS = kInt8Slices, W = 4, D = 3, LD = NT * 16 + 1;
   // the per-line maxima of S^T were consumed by the slicing (earlier on this stream): clear them
   // for the next right-hand side
-  if (smax && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < NT * 16) smax[threadIdx.x] = 0ull;
+  if (smax && bx == 0 && by == 0 && threadIdx.x < NT * 16) smax[threadIdx.x] = 0ull;
   constexpr int NCH = NT * NS, CPW = (NCH + W - 1) / W;  // 1 KiB S chunks per block, per wave
   constexpr int BUF = W * CPW * 1024;                    // bytes per LDS buffer
   constexpr int LDSB = 2 * BUF > 4 * 16 * LD * 8 ? 2 * BUF : 4 * 16 * LD * 8;
@@ -214,12 +223,12 @@ __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, 
   const int nK64 = nGp >> 6;
   // k blocks of this k range: consecutive positions of the active-block list, or of all blocks
   // when act is null
-  const int nbw = (act ? active_blocks(act, NT, nGp >> 4, 4, nK64, gridDim.y, sList, sWave) : nK64) / gridDim.y;
+  const int nbw = (act ? active_blocks(act, NT, nGp >> 4, 4, nK64, nsplit, sList, sWave) : nK64) / nsplit;
   auto kblock = [&](int blk) {
-    const int j = blockIdx.y * nbw + min(blk, nbw - 1);
+    const int j = by * nbw + min(blk, nbw - 1);
     return act ? sList[j] : j;
   };
-  const int8_t* ab = As + (size_t)(blockIdx.x * 4 + wave) * nK64 * NS * 1024 + 16 * lane;
+  const int8_t* ab = As + (size_t)(bx * 4 + wave) * nK64 * NS * 1024 + 16 * lane;
   auto chunk_src = [&](int m, int K) {  // chunk c = t NS + p of this wave's share, k block K
     const int c = min(wave + W * m, NCH - 1);
     const int t = c / NS, p = c % NS;
@@ -291,7 +300,7 @@ __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, 
   // FP64 recombination; 32-bit MFMA results: element u of lane l is row 4(l/16) + u
   double* red = reinterpret_cast<double*>(lds);
   double* slot = red + wave * 16 * LD;
-  const int rowb = blockIdx.x * 64 + wave * 16;
+  const int rowb = bx * 64 + wave * 16;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int fc = fB[t * 16 + rr];
@@ -304,12 +313,21 @@ __global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, 
     }
   }
   __syncthreads();
-  const int row0 = blockIdx.x * 64;
-  double* Wp = Wpart + blockIdx.y * (size_t)NT * 16 * nGp;
+  const int row0 = bx * 64;
+  double* Wp = Wpart + by * (size_t)NT * 16 * nGp;
   for (int e = threadIdx.x; e < 64 * NT * 16; e += blockDim.x) {
     const int c = e >> 6, rl = e & 63;
     Wp[(size_t)c * nGp + row0 + rl] = red[(rl >> 4) * 16 * LD + (rl & 15) * LD + c];
   }
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, const int* __restrict__ eA,
+                                                 const int8_t* __restrict__ Bs, const int* __restrict__ fB,
+                                                 int nGp, double* __restrict__ Wpart,
+                                                 unsigned long long* __restrict__ smax,
+                                                 const int* __restrict__ act) {
+  mf_i8_w_body<NT>(blockIdx.x, blockIdx.y, gridDim.y, As, eA, Bs, fB, nGp, Wpart, smax, act);
 }
 
 // ---------------------------------------------------------------------------------------- int8 slices
@@ -575,6 +593,61 @@ int launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* 
   const size_t plane = (size_t)Pc * nGp;
   hipLaunchKernelGGL(k_wsum, dim3((unsigned)((plane / 2 + 255) / 256)), dim3(256), 0, st, Wpart, nsplit, plane,
                      WT);
+  return 0;
+}
+
+// ------------------------------------------------------------------ grouped contractions
+// Workgroup b runs block (bx, by) = (map[b].y mod nGp / 64, map[b].y div nGp / 64) of instance
+// map[b].x (see GroupInstance).
+template <int NT>
+__global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_grp_mf_f64_w(const GroupInstance* __restrict__ gi,
+                                                                      const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  const int nbx = a.nGp / 64;
+  mf_f64_w_body<NT>(m.y % nbx, m.y / nbx, a.nsplit, a.Vt, a.STt, a.nGp, a.nsplit == 1 ? a.WT : a.Wpart, a.actC);
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_grp_mf_i8_w(const GroupInstance* __restrict__ gi,
+                                                     const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  const int nbx = a.nGp / 64;
+  mf_i8_w_body<NT>(m.y % nbx, m.y / nbx, a.nsplit, a.Vi8, a.eV, a.Si8, a.eS, a.nGp, a.Wpart, a.smax, a.actC);
+}
+
+int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GW(NT) \
+  hipLaunchKernelGGL(k_grp_mf_f64_w<NT>, dim3(nBlocks), dim3((NT) <= 5 ? 512 : 256), 0, st, gi, map)
+  switch (Pc / 16) {
+    case 1: CALL_GW(1); break;
+    case 2: CALL_GW(2); break;
+    case 3: CALL_GW(3); break;
+    case 4: CALL_GW(4); break;
+    case 5: CALL_GW(5); break;
+    case 6: CALL_GW(6); break;
+    case 7: CALL_GW(7); break;
+    case 8: CALL_GW(8); break;
+    case 9: CALL_GW(9); break;
+    case 10: CALL_GW(10); break;
+    default: return set_error(CODY_ERR_ARG, "k_grp_mf_f64_w: Pc / 16 outside 1..10");
+  }
+#undef CALL_GW
+  return 0;
+}
+
+int launch_grp_mf_i8_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GI(NT) hipLaunchKernelGGL(k_grp_mf_i8_w<NT>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+  switch (Pc / 16) {
+    case 1: CALL_GI(1); break;
+    case 2: CALL_GI(2); break;
+    case 3: CALL_GI(3); break;
+    case 4: CALL_GI(4); break;
+    case 5: CALL_GI(5); break;
+    default: return set_error(CODY_ERR_ARG, "k_grp_mf_i8_w: Pc / 16 outside 1..5");
+  }
+#undef CALL_GI
   return 0;
 }
 

@@ -58,6 +58,47 @@ def test_batch_matches_instances_and_oracle(instances):
         assert err < 1e-12, w.name
 
 
+@pytest.mark.parametrize("forms", ["int8", "mixed"])
+def test_grouped_batch_bit_identical_to_instances(instances, forms):
+    """The batch groups instances by orbital count and contraction path (FP64, tiled int8) into one
+    launch per kernel type; instances on other paths (int8 at M = 12, untiled) keep their own
+    launches.  Every instance's right-hand side is bit-identical to its own launches, with all
+    instances on the int8-sliced contraction and with FP64 and int8 instances mixed."""
+    import torch
+
+    stream = torch.cuda.current_stream().cuda_stream
+    for k, (w, ctx, *_) in enumerate(instances):
+        int8 = forms == "int8" or k % 2 == 0
+        ctx.set_precision(hip.PREC_INT8_SLICED if int8 else hip.PREC_FP64)
+    try:
+        refs = []
+        for w, ctx, wts, state, d_s, d_o in instances:
+            ctx.set_graphs(False)
+            ctx.time_derivative_dev(d_s.data_ptr(), d_o.data_ptr(), 0.0, stream)
+            torch.cuda.synchronize()
+            refs.append(d_o.cpu().numpy().copy())
+            ctx.set_graphs(True)
+        b = hip.Batch([inst[1] for inst in instances])
+        try:
+            for rep in range(3):
+                for *_, d_o in instances:
+                    d_o.zero_()
+                b.time_derivative_dev([i[4].data_ptr() for i in instances],
+                                      [i[5].data_ptr() for i in instances], 0.0, stream)
+                torch.cuda.synchronize()
+                for (w, ctx, wts, state, d_s, d_o), ref in zip(instances, refs):
+                    assert np.array_equal(d_o.cpu().numpy(), ref), (forms, rep, w.name)
+        finally:
+            b.close()
+        for (w, ctx, wts, state, d_s, d_o), ref in zip(instances, refs):
+            s, _, _ = oracle_system(w)
+            err = state_rel(ref.view(np.complex128), s.time_derivative(state), w.nC, wts, w.M)
+            assert err < 1e-10, (forms, w.name, err)
+    finally:
+        for w, ctx, *_ in instances:
+            ctx.set_precision(hip.PREC_FP64)
+
+
 def test_autotune_keeps_parity(instances):
     w, ctx, wts, state, d_s, d_o = instances[-1]
     ctx.upload_state(state)
