@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcody_hip.so")
 
-SOURCES = ["host_setup.cpp", "state_files.cpp", "mctdhx_kernels.hip", "ci_kernels.hip", "meanfield_gemm.hip",
+SOURCES = ["host_setup.cpp", "state_files.cpp", "batch_plan.cpp", "mctdhx_kernels.hip", "ci_kernels.hip", "meanfield_gemm.hip",
            "meanfield_lowprec.hip",
            "cody_api.hip", "cody_propagate.hip", "cody_adjoint.hip", "noalloc.hip"]
 

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/cody_mctdhx.h"
+#include "batch_plan.hpp"
 #include "host_setup.hpp"
 #include "mctdhx_kernels.hpp"
 #include "cody_ctx.hpp"
@@ -949,6 +950,9 @@ int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* nam
 }
 
 // ---------------------------------------------------------------- batches of instances
+static_assert(kPlanMaxOrbs == kMaxOrbs && kPlanInt8Split == kInt8Split,
+              "the host planner and the device kernels agree on their limits");
+static_assert(sizeof(BlockRef) == sizeof(int2), "block references are uploaded as int2");
 namespace {
 
 // Device time of one right-hand side, for the assignment of work to chains: the contraction at
@@ -985,7 +989,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
   }
   for (auto& g : b->groups) {
     std::vector<GroupInstance> gi;
-    std::vector<int2> maps[BatchGroup::kKinds];
+    std::vector<PlanShape> shapes;
     const int nO = g.nO, n2 = nO * nO, np = nO * (nO - 1) / 2;
     for (int mi = 0; mi < (int)g.members.size(); ++mi) {
       const int i = g.members[mi];
@@ -1036,38 +1040,33 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.eV = c->d_eV;
       a.Si8 = c->d_Si8;
       a.eS = c->d_eS;
-      if (g.int8) {
-        CHK(check_contraction_shape("k_mf_i8_w", c->nGp, c->Pc, kInt8Split, 5, c->nGp / 64));
-      } else {
-        const int kh = c->Pc / 16 <= 5 ? 2 : 1;
-        CHK(check_contraction_shape("k_mf_f64_w", c->nGp, c->Pc, a.nsplit, 10, c->nGp / 16 / kh));
-      }
       gi.push_back(a);
+      PlanShape sh;
+      sh.nO = nO;
+      sh.nC = c->nC;
+      sh.nG = c->nG;
+      sh.nGp = c->nGp;
+      sh.Pc = c->Pc;
+      sh.nsplit = a.nsplit;
+      sh.nxs = c->nxs;
+      sh.int8 = g.int8;
+      shapes.push_back(sh);
       g.maxC = std::max(g.maxC, c->nC);
       g.cost += rhs_cost_estimate(c) - 40.0;  // (the launches' latency is shared by the group)
-      const size_t plane = (size_t)c->Pc * c->nGp;
-      const int cnt[BatchGroup::kKinds] = {
-          a.nRdm + a.nObx * nO + (int)(plane / 256),
-          g.int8 ? 1 + (c->Pc * (c->nGp >> 2) + 255) / 256 : 0,
-          c->nGp / 64 * a.nsplit,
-          a.wsumFirst + (a.nsplit > 1 ? (int)((plane / 2 + 255) / 256) : 0),
-          c->Pc / 16 * c->nxs + c->nGp / 64 * nO,
-          a.nRed + (c->nG + 255) / 256,
-          (c->nC + 3) / 4};
-      for (int k = 0; k < BatchGroup::kKinds; ++k)
-        for (int lb = 0; lb < cnt[k]; ++lb) maps[k].push_back(make_int2(mi, lb));
     }
     g.cost += 40.0;
-    std::vector<int2> all;
-    for (int k = 0; k < BatchGroup::kKinds; ++k) {
-      g.off[k] = static_cast<int>(all.size());
-      g.cnt[k] = static_cast<int>(maps[k].size());
-      all.insert(all.end(), maps[k].begin(), maps[k].end());
+    // the block maps, built and checked entry by entry on the host before any launch
+    GroupPlan plan;
+    const std::string why = build_group_plan(shapes, plan);
+    if (!why.empty()) return fail(CODY_ERR_ARG, ("cody_batch_time_derivative_dev: " + why).c_str());
+    for (int k = 0; k < kPlanKinds; ++k) {
+      g.off[k] = static_cast<int>(plan.off[k]);
+      g.cnt[k] = static_cast<int>(plan.cnt[k]);
     }
     HIPCHK(hipMalloc((void**)&g.d_inst, sizeof(GroupInstance) * gi.size()));
-    HIPCHK(hipMalloc((void**)&g.d_map, sizeof(int2) * all.size()));
+    HIPCHK(hipMalloc((void**)&g.d_map, sizeof(BlockRef) * plan.map.size()));
     HIPCHK(hipMemcpy(g.d_inst, gi.data(), sizeof(GroupInstance) * gi.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(g.d_map, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.d_map, plan.map.data(), sizeof(BlockRef) * plan.map.size(), hipMemcpyHostToDevice));
   }
   return CODY_OK;
 }
@@ -1075,17 +1074,17 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
 // The right-hand sides of a group's instances: one launch per kernel type.
 int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
   auto map = [&](int k) { return g.d_map + g.off[k]; };
-  launch_grp_front(g.nO, g.d_inst, map(BatchGroup::kFront), g.cnt[BatchGroup::kFront], g.maxC, st);
+  launch_grp_front(g.nO, g.d_inst, map(kPlanFront), g.cnt[kPlanFront], g.maxC, st);
   if (g.int8) {
-    launch_grp_slice(g.nO, g.d_inst, map(BatchGroup::kSlice), g.cnt[BatchGroup::kSlice], st);
-    CHK(launch_grp_mf_i8_w(g.Pc, g.d_inst, map(BatchGroup::kContract), g.cnt[BatchGroup::kContract], st));
+    launch_grp_slice(g.nO, g.d_inst, map(kPlanSlice), g.cnt[kPlanSlice], st);
+    CHK(launch_grp_mf_i8_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   } else {
-    CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(BatchGroup::kContract), g.cnt[BatchGroup::kContract], st));
+    CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   }
-  launch_grp_wsum(g.nO, g.d_inst, map(BatchGroup::kWsum), g.cnt[BatchGroup::kWsum], st);
-  launch_grp_mid(g.nO, g.d_inst, map(BatchGroup::kMid), g.cnt[BatchGroup::kMid], st);
-  launch_grp_post(g.nO, g.d_inst, map(BatchGroup::kPost), g.cnt[BatchGroup::kPost], st);
-  launch_grp_apply(g.d_inst, map(BatchGroup::kApply), g.cnt[BatchGroup::kApply], g.nO, st);
+  launch_grp_wsum(g.nO, g.d_inst, map(kPlanWsum), g.cnt[kPlanWsum], st);
+  launch_grp_mid(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
+  launch_grp_post(g.nO, g.d_inst, map(kPlanPost), g.cnt[kPlanPost], st);
+  launch_grp_apply(g.d_inst, map(kPlanApply), g.cnt[kPlanApply], g.nO, st);
   HIPCHK(hipGetLastError());
   return CODY_OK;
 }

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_plan.hpp"
 #include "host_setup.hpp"
 #include "mctdhx_kernels.hpp"
 
@@ -291,14 +292,13 @@ struct cody_ctx {
 // Instances of a batch with one orbital count and one contraction path, evaluated by one launch
 // per kernel type: their argument table and block maps in device memory.
 struct BatchGroup {
-  enum { kFront, kSlice, kContract, kWsum, kMid, kPost, kApply, kKinds };
   int nO = 0, Pc = 0, maxC = 0;
   bool int8 = false;
   double cost = 0.0;  // estimated device time (for the assignment to chains)
   std::vector<int> members;
   cody::GroupInstance* d_inst = nullptr;
   int2* d_map = nullptr;  // the kinds' maps back to back
-  int off[kKinds] = {}, cnt[kKinds] = {};
+  int off[cody::kPlanKinds] = {}, cnt[cody::kPlanKinds] = {};  // each kernel type's map
 };
 
 struct cody_batch {
