@@ -24,7 +24,11 @@ constexpr int kPlanInt8MaxTiles = 5;   // 16-column tiles of the int8 contractio
 constexpr int kPlanF64MaxTiles = 10;   // 16-column tiles of the FP64 contraction
 
 // The kernel types of a right-hand side, in launch order.
-enum PlanKind { kPlanFront, kPlanSlice, kPlanContract, kPlanWsum, kPlanMid, kPlanPost, kPlanApply, kPlanKinds };
+enum PlanKind { kPlanFront, kPlanSlice, kPlanContract, kPlanWsum, kPlanRho2p, kPlanMid, kPlanPost, kPlanApply, kPlanKinds };
+
+// Orbital counts from which a grouped right-hand side forms rho2' = rho2 x_3 R once per instance
+// (kPlanRho2p) instead of in every orbital-correlation workgroup.
+constexpr int kPlanRho2pMinOrbs = 10;
 
 // The sizes an instance's launches depend on.
 struct PlanShape {
@@ -40,8 +44,8 @@ struct PlanShape {
 
 // Workgroups of each kernel type of one instance's right-hand side: the grids of its own launches
 // (front: CI densities + one-body part + pair sources; the int8 slicing with R; the contraction;
-// the k-split sum (with R on the FP64 path); Gram + orbital correlation; G reduction + orbital
-// projection; CI apply).  The shape must have passed check_plan_shape.
+// the k-split sum (with R on the FP64 path); rho2' for nO >= kPlanRho2pMinOrbs; Gram + orbital
+// correlation; G reduction + orbital projection; CI apply).  The shape must have passed check_plan_shape.
 void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]);
 
 // Returns an empty string for a shape the grouped kernels accept, else the reason.

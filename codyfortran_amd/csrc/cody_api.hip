@@ -950,7 +950,7 @@ int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* nam
 }
 
 // ---------------------------------------------------------------- batches of instances
-static_assert(kPlanMaxOrbs == kMaxOrbs && kPlanInt8Split == kInt8Split,
+static_assert(kPlanMaxOrbs == kMaxOrbs && kPlanInt8Split == kInt8Split && kPlanRho2pMinOrbs == kRho2pMinOrbs,
               "the host planner and the device kernels agree on their limits");
 static_assert(sizeof(BlockRef) == sizeof(int2), "block references are uploaded as int2");
 namespace {
@@ -1063,6 +1063,11 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       g.off[k] = static_cast<int>(plan.off[k]);
       g.cnt[k] = static_cast<int>(plan.cnt[k]);
     }
+    if (nO >= kRho2pMinOrbs) {  // rho2' of every member, formed once per right-hand side
+      const size_t n4 = (size_t)n2 * n2;
+      HIPCHK(hipMalloc((void**)&g.d_r2p, sizeof(double2) * n4 * gi.size()));
+      for (size_t m = 0; m < gi.size(); ++m) gi[m].r2p = g.d_r2p + m * n4;
+    }
     HIPCHK(hipMalloc((void**)&g.d_inst, sizeof(GroupInstance) * gi.size()));
     HIPCHK(hipMalloc((void**)&g.d_map, sizeof(BlockRef) * plan.map.size()));
     HIPCHK(hipMemcpy(g.d_inst, gi.data(), sizeof(GroupInstance) * gi.size(), hipMemcpyHostToDevice));
@@ -1082,6 +1087,7 @@ int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
     CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   }
   launch_grp_wsum(g.nO, g.d_inst, map(kPlanWsum), g.cnt[kPlanWsum], st);
+  launch_grp_rho2p(g.nO, g.d_inst, map(kPlanRho2p), g.cnt[kPlanRho2p], st);
   launch_grp_mid(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
   launch_grp_post(g.nO, g.d_inst, map(kPlanPost), g.cnt[kPlanPost], st);
   launch_grp_apply(g.d_inst, map(kPlanApply), g.cnt[kPlanApply], g.nO, st);

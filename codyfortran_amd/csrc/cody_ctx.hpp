@@ -297,6 +297,7 @@ struct BatchGroup {
   double cost = 0.0;  // estimated device time (for the assignment to chains)
   std::vector<int> members;
   cody::GroupInstance* d_inst = nullptr;
+  double2* d_r2p = nullptr;  // rho2' of the members (orbital counts from kRho2pMinOrbs)
   int2* d_map = nullptr;  // the kinds' maps back to back
   int off[cody::kPlanKinds] = {}, cnt[cody::kPlanKinds] = {};  // each kernel type's map
 };
@@ -317,6 +318,7 @@ struct cody_batch {
     for (auto& g : groups) {
       if (g.d_inst) hipFree(g.d_inst);
       if (g.d_map) hipFree(g.d_map);
+      if (g.d_r2p) hipFree(g.d_r2p);
     }
     groups.clear();
     singles.clear();

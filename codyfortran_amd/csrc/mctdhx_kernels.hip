@@ -262,25 +262,76 @@ __global__ __launch_bounds__(256) void k_xmat(const double2* __restrict__ orbs, 
 // sums are split over the waves).  The slice rho2'(:,:,j,:) (NO^3 values) is staged in LDS and
 // read as broadcasts; each wave loads all of its W values before using them, and the four
 // correlation shares are added in fixed order.
+// The slice rho2'(:, :, j, :) of rho2'(k1, k, j, l) = sum_m rho2(k1, k, m, l) R(m, j) at
+// e = k1 + NO (k + NO l) (S_Method_Mb_OrbBased.f90:746-753), from the canonical rho2 s2 (np x np,
+// strict pairs) and column j of R: the same operations in the same order wherever it is formed.
 template <int NO>
+__device__ __forceinline__ double2 rho2p_entry(int e, const double2* s2, const double2* Rj) {
+  constexpr int n2 = NO * NO, np = NO * (NO - 1) / 2, kNone = -(n2 + 1);
+  auto pairc = [](int a, int b) {  // strict-pair index of {a, b}; -1 - index when a > b; kNone when a = b
+    return a == b ? kNone : (a < b ? a * NO - (a * (a + 1)) / 2 + (b - a - 1) : -1 - (b * NO - (b * (b + 1)) / 2 + (a - b - 1)));
+  };
+  const int ab = e % n2, l = e / n2;
+  const int pab = pairc(ab % NO, ab / NO);
+  double2 acc = cmk(0, 0);
+  if (pab != kNone) {
+    const int pa = pab >= 0 ? pab : -1 - pab;
+    const double sa = pab >= 0 ? 1.0 : -1.0;
+#pragma unroll
+    for (int m = 0; m < NO; ++m) {
+      const int pcd = pairc(m, l);
+      if (pcd == kNone) continue;
+      const int pc = pcd >= 0 ? pcd : -1 - pcd;
+      cfma(acc, cscale(s2[pa + np * pc], pcd >= 0 ? sa : -sa), Rj[m]);
+    }
+  }
+  return acc;
+}
+
+// With PRE the slice rho2'(:, :, j, :) is read from r2p (all slices, formed once per right-hand
+// side by k_grp_rho2p) instead of being formed from the canonical rho2 in every workgroup: the
+// workgroup then holds no copy of rho2 (for NO = 12 that is 70 KB of LDS and 20k products).
+template <int NO, bool PRE = false>
 __device__ __forceinline__ void orbital_body(int bx, int j, const double2* __restrict__ orbs, const double* __restrict__ w,
                                                  const double2* __restrict__ hphi, const double* __restrict__ WT,
                                                  const double2* __restrict__ r2can, const double2* __restrict__ reginv,
                                                  int nG, int nGp, int mode, double2* __restrict__ draw,
-                                                 double2* __restrict__ part) {
+                                                 double2* __restrict__ part, const double2* __restrict__ r2p = nullptr) {
   constexpr int n2 = NO * NO, n3 = n2 * NO, NP = NO * (NO + 1) / 2, PW = (NP + 3) / 4;
-  constexpr int np = NO * (NO - 1) / 2, nCan = np * np, kNone = -(n2 + 1);
-  __shared__ double2 s2[nCan > 0 ? nCan : 1];
+  constexpr int np = NO * (NO - 1) / 2, nCan = np * np;
+  __shared__ double2 s2[(nCan > 0 && !PRE) ? nCan : 1];
   __shared__ double2 sR[n3];
   __shared__ double2 sRj[NO];
-  __shared__ int sPair[n2];  // strict-pair index of {a, b}; -1 - index when a > b; kNone when a = b
   __shared__ double2 sCorr[4][64];
   __shared__ double2 sRed[NO];
+  __shared__ int sLK[NP];  // l + 256 k of pair p = l NO - l (l - 1) / 2 + k - l, l <= k
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x = bx * 64 + lane;
-  // canonical rho2 (i1 < i2, j1 < j2), column j of R and the pair table (each thread's loads of
-  // the canonical rho2 in flight together)
-  {
+  if (threadIdx.x < NP) {
+    int l = 0, r = threadIdx.x;
+    while (r >= NO - l) {
+      r -= NO - l;
+      ++l;
+    }
+    sLK[threadIdx.x] = l + 256 * (l + r);
+  }
+  if constexpr (PRE) {
+    constexpr int kStage = (n3 + 255) / 256;
+    const double2* src = r2p + (size_t)j * n3;
+    double2 v[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      v[u] = e < n3 ? src[e] : cmk(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < n3) sR[e] = v[u];
+    }
+  } else {
+    // canonical rho2 (i1 < i2, j1 < j2) and column j of R (each thread's loads of the canonical
+    // rho2 in flight together), then the slice
     constexpr int kStage = nCan > 0 ? (nCan + 255) / 256 : 1;
     double2 v[kStage];
 #pragma unroll
@@ -293,32 +344,9 @@ __device__ __forceinline__ void orbital_body(int bx, int j, const double2* __res
       const int e = threadIdx.x + 256 * u;
       if (e < nCan) s2[e] = v[u];
     }
-  }
-  for (int e = threadIdx.x; e < n2; e += 256) {
-    const int a = e % NO, b = e / NO;
-    sPair[e] = a == b ? kNone
-                      : (a < b ? a * NO - (a * (a + 1)) / 2 + (b - a - 1) : -1 - (b * NO - (b * (b + 1)) / 2 + (a - b - 1)));
-  }
-  if (threadIdx.x < NO) sRj[threadIdx.x] = reginv[threadIdx.x + j * NO];
-  __syncthreads();
-  // sR[k1 + NO (k + NO l)] = rho2'(k1, k, j, l) = sum_m rho2(k1, k, m, l) R(m, j)
-  // (S_Method_Mb_OrbBased.f90:746-753)
-  for (int e = threadIdx.x; e < n3; e += 256) {
-    const int ab = e % n2, l = e / n2;
-    const int pab = sPair[ab];
-    double2 acc = cmk(0, 0);
-    if (pab != kNone) {
-      const int pa = pab >= 0 ? pab : -1 - pab;
-      const double sa = pab >= 0 ? 1.0 : -1.0;
-#pragma unroll
-      for (int m = 0; m < NO; ++m) {
-        const int pcd = sPair[m + NO * l];
-        if (pcd == kNone) continue;
-        const int pc = pcd >= 0 ? pcd : -1 - pcd;
-        cfma(acc, cscale(s2[pa + np * pc], pcd >= 0 ? sa : -sa), sRj[m]);
-      }
-    }
-    sR[e] = acc;
+    if (threadIdx.x < NO) sRj[threadIdx.x] = reginv[threadIdx.x + j * NO];
+    __syncthreads();
+    for (int e = threadIdx.x; e < n3; e += 256) sR[e] = rho2p_entry<NO>(e, s2, sRj);
   }
   const bool in = x < nG;
   const int xc = in ? x : nG - 1;
@@ -339,25 +367,23 @@ __device__ __forceinline__ void orbital_body(int bx, int j, const double2* __res
   for (int m = 0; m < PW; ++m) {
     const int p = wave + 4 * m;  // uniform over the wave
     if (p >= NP) break;
-    int l = 0, r = p;
-    while (r >= NO - l) {
-      r -= NO - l;
-      ++l;
-    }
-    const int k = l + r;
-    // A = sum_k1 phi_k1 rho2'(k1,k,j,l); B = sum_k1 phi_k1 rho2'(k1,l,j,k)
+    const int lk = sLK[p], l = lk & 255, k = lk >> 8;
+    // A = sum_k1 phi_k1 rho2'(k1,k,j,l); B = sum_k1 phi_k1 rho2'(k1,l,j,k), each over even and
+    // odd k1 in two chains (shorter dependency chains), then added
     const double2* ra = sR + NO * (k + NO * l);
-    double2 A = cmk(0, 0);
+    const double2* rb = sR + NO * (l + NO * k);
+    double2 A0 = cmk(0, 0), A1 = cmk(0, 0), B0 = cmk(0, 0), B1 = cmk(0, 0);
 #pragma unroll
-    for (int k1 = 0; k1 < NO; ++k1) cfma(A, phi[k1], ra[k1]);
-    cfma(corr, Wv[m], A);
-    if (k != l) {
-      const double2* rb = sR + NO * (l + NO * k);
-      double2 B = cmk(0, 0);
-#pragma unroll
-      for (int k1 = 0; k1 < NO; ++k1) cfma(B, phi[k1], rb[k1]);
-      cfma(corr, cconj(Wv[m]), B);
+    for (int k1 = 0; k1 < NO; k1 += 2) {
+      cfma(A0, phi[k1], ra[k1]);
+      cfma(B0, phi[k1], rb[k1]);
+      if (k1 + 1 < NO) {
+        cfma(A1, phi[k1 + 1], ra[k1 + 1]);
+        cfma(B1, phi[k1 + 1], rb[k1 + 1]);
+      }
     }
+    cfma(corr, Wv[m], cadd(A0, A1));
+    if (k != l) cfma(corr, cconj(Wv[m]), cadd(B0, B1));
   }
   sCorr[wave][lane] = corr;
   __syncthreads();
@@ -816,6 +842,22 @@ __global__ __launch_bounds__(256) void k_grp_wsum(const GroupInstance* __restric
   wsum_body(m.y - a.wsumFirst, a.Wpart, a.nsplit, (size_t)a.Pc * a.nGp, a.WT);
 }
 
+// rho2'(k1, k, j, l) = sum_m rho2(k1, k, m, l) R(m, j) for all j, at j NO^3 + k1 + NO (k + NO l),
+// one entry per thread (after R): the slices the orbital correlation of nO >= 10 reads.
+template <int NO>
+__global__ __launch_bounds__(256) void k_grp_rho2p(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  constexpr int n3 = NO * NO * NO;
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  const int q = m.y * 256 + threadIdx.x;
+  if (q >= NO * n3) return;
+  const int j = q / n3, e = q % n3;
+  double2 Rj[NO];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) Rj[k] = a.reginv[k + j * NO];
+  a.r2p[q] = rho2p_entry<NO>(e, a.r2can, Rj);
+}
+
 template <int NO>
 __global__ __launch_bounds__(256) void k_grp_mid(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
   constexpr int NT = (NO * (NO + 1) + 15) / 16;
@@ -830,8 +872,9 @@ __global__ __launch_bounds__(256) void k_grp_mid(const GroupInstance* __restrict
   }
   b -= nGram;
   const int nOrbX = a.nGp / 64;
-  orbital_body<NO>(b % nOrbX, b / nOrbX, a.C + a.nC, a.w, a.hphi, a.WT, a.r2can, a.reginv, a.nG, a.nGp, kModeFull,
-                   a.draw, a.part);
+  constexpr bool kPre = NO >= kRho2pMinOrbs;
+  orbital_body<NO, kPre>(b % nOrbX, b / nOrbX, a.C + a.nC, a.w, a.hphi, a.WT, a.r2can, a.reginv, a.nG, a.nGp, kModeFull,
+                         a.draw, a.part, a.r2p);
 }
 
 template <int NO>
@@ -862,6 +905,12 @@ void launch_grp_wsum(int nO, const GroupInstance* gi, const int2* map, int nBloc
 #define CALL_GW(NO) hipLaunchKernelGGL(k_grp_wsum<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
   CODY_NO_SWITCH(nO, CALL_GW)
 #undef CALL_GW
+}
+void launch_grp_rho2p(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+  if (nBlocks == 0) return;
+#define CALL_GR(NO) hipLaunchKernelGGL(k_grp_rho2p<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+  CODY_NO_SWITCH(nO, CALL_GR)
+#undef CALL_GR
 }
 void launch_grp_mid(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
 #define CALL_GM(NO) hipLaunchKernelGGL(k_grp_mid<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)

@@ -75,6 +75,7 @@ struct GroupInstance {
   unsigned long long* smax;  // int8 path: per-line maxima of S^T
   int* act;                  // pair-source activity flags (written by the front launch)
   const int* actC;           // the flags the contraction reads (null: no skipping)
+  double2* r2p;              // rho2' (nO^4) for the orbital correlation, or null (formed in it)
   const double* Vt;                          // FP64 contraction operand
   const int8_t* Vi8;                         // int8 contraction operands
   const int* eV;
@@ -86,6 +87,7 @@ struct GroupInstance {
 void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlocks, int maxC, hipStream_t st);
 void launch_grp_slice(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_wsum(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+void launch_grp_rho2p(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_mid(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_post(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_apply(const GroupInstance* gi, const int2* map, int nBlocks, int nO, hipStream_t st);
@@ -178,6 +180,7 @@ void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream
 void launch_split_bf16x3(const double* X, size_t n, __bf16* out, hipStream_t st);
 constexpr int kMaxWSplit = 8;  // largest k split of the FP64 contraction (partials buffer depth)
 constexpr int kInt8Split = 4;  // k split of the 64-row int8 contraction
+constexpr int kRho2pMinOrbs = 10;  // grouped right-hand sides form rho2' once per instance from this many orbitals
 constexpr int kSkipMaxBlocks16 = 1024;  // longest grid (in 16-point blocks) whose zero k blocks are skipped
 // shape check of the k-split contraction launchers (0 or CODY_ERR_ARG, nothing launched)
 int check_contraction_shape(const char* who, int nGp, int Pc, int nsplit, int maxTiles, int kBlocks);

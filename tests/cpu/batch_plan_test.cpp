@@ -49,20 +49,24 @@ int main() {
   // 1. known answer: the 4096-point system with 8 orbitals, 4 fermions (70 configurations), FP64
   //    contraction split in 4, as its own launches size it (launch_rhs_front: 212 CI-density
   //    blocks + 16 x 8 one-body blocks + 80 x 4096 / 256 pair-source blocks; k_mf_f64_w: 64 row
-  //    groups x 4; k_rhs_wsum: 1 + 80 x 4096 / 512; k_rhs_mid: 5 x 32 Gram + 64 x 8 orbital;
+  //    groups x 4; k_rhs_wsum: 1 + 80 x 4096 / 512; no separate rho2'; k_rhs_mid: 5 x 32 Gram +
+  //    64 x 8 orbital;
   //    k_rhs_post: 26 reduction + 16 projection; k_ci_apply_project: 18)
   {
     const PlanShape s = shape(8, 70, 4096, 4, false);
     EXPECT(check_plan_shape(s).empty(), "headline shape refused: %s", check_plan_shape(s).c_str());
     int64_t c[kPlanKinds];
     plan_block_counts(s, c);
-    const int64_t want[kPlanKinds] = {212 + 128 + 1280, 0, 256, 641, 160 + 512, 26 + 16, 18};
+    const int64_t want[kPlanKinds] = {212 + 128 + 1280, 0, 256, 641, 0, 160 + 512, 26 + 16, 18};
     for (int k = 0; k < kPlanKinds; ++k) EXPECT(c[k] == want[k], "kind %d: %lld blocks, expected %lld", k, (long long)c[k], (long long)want[k]);
     // the int8 path: slicing with R (1 + 80 x 1024 / 256), four k ranges, no R in the sum launch
     const PlanShape s8 = shape(8, 70, 4096, 4, true);
     plan_block_counts(s8, c);
     EXPECT(c[kPlanSlice] == 1 + 320, "int8 slicing blocks %lld", (long long)c[kPlanSlice]);
     EXPECT(c[kPlanWsum] == 640, "int8 sum blocks %lld", (long long)c[kPlanWsum]);
+    // 12 orbitals: rho2' (12^4 entries, one per thread) is formed once per right-hand side
+    plan_block_counts(shape(12, 924, 4096, 4, false), c);
+    EXPECT(c[kPlanRho2p] == 81, "rho2' blocks %lld", (long long)c[kPlanRho2p]);
   }
   // 2. the batched workload's shapes (N_g in 511/1026/2046/4096, M in 4/6/8/12, N = M/2), every
   //    FP64 split and the int8 path: built plans pass the independent check, cover each
