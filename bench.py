@@ -40,7 +40,7 @@ PEAK = {"fp64": 78.6, "split_fp32": 157.3, "bf16x3": 8000.0, "int8_sliced": 8000
 PREC_CODE = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "int8_sliced": 3}
 # bytes per operand element of the HBM-bound forms: three BF16 terms; five int8 digits
 OPERAND_BYTES = {"bf16x3": 6.0, "int8_sliced": 5.0}
-KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_w"}
+KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_r16"}
 TOL = 1e-9
 # forms whose contraction skips the k blocks of all-zero pair sources (cody_set_source_skip)
 SKIPPING = ("fp64", "int8_sliced")
@@ -77,7 +77,12 @@ def parse():
     p.add_argument("--pmc-probe", action="store_true",
                    help="short run for hardware-counter passes: a 256 MiB device copy (the unit "
                         "of the byte counters) and a few launches of the FP64 and int8 "
-                        "contraction kernels; prints no JSON line")
+                        "contraction kernels, dense and with the zero-source skip; prints no JSON line")
+    p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                   help="turn the FETCH_SIZE and WRITE_SIZE passes over --pmc-probe (rocprofv3 output "
+                        "folders) into profiles/pmc_traffic_<workload>.json and .md; needs no GPU")
+    p.add_argument("--no-dense", action="store_true",
+                   help="time the forms only with the zero-source skip (no dense repeat)")
     return p.parse_args()
 
 
@@ -339,25 +344,31 @@ def propagated_energy_by_form(args, wlm, wl, forms):
 
 
 def pmc_traffic(kname: str, workload: str, skip: bool = False):
-    """HBM bytes per launch of `kname` from the committed hardware-counter summary (rocprofv3
-    --pmc FETCH_SIZE / WRITE_SIZE passes over `bench.py --pmc-probe`, scaled by a 256 MiB copy);
-    None when no summary exists for this workload."""
+    """HBM bytes per launch of `kname` (the contraction kernel of the form, dense or with the
+    zero-source skip) from the committed hardware-counter summary (rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE passes over `bench.py --pmc-probe`, scaled by a 256 MiB copy); None when no summary
+    exists for this workload and mode."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    key = "skip" if skip else "dense"
-    for k, v in d["kernels"].items():
-        if k.split("<")[0] == kname and v.get("mode", "dense") == key:
+    mode = "skip" if skip else "dense"
+    for v in d.get("launches", []):
+        if v["kernel"].split("<")[0] == kname and v["mode"] == mode and v["role"] == "contraction alone":
             return v["hbm_bytes"]
     return None
 
 
+# The probe's segments, in launch order; each starts with a 4 KiB device copy (the separator the
+# summary splits the counter rows on).  A segment holds the contraction kernel launched alone
+# (one warm-up with its slicing, then three) and three full right-hand sides.
+PMC_SEGMENTS = [("fp64", "dense"), ("fp64", "skip"), ("int8_sliced", "dense"), ("int8_sliced", "skip")]
+
+
 def pmc_probe(torch, wlm, wl):
-    """Launches counted by `rocprofv3 --pmc` runs: a plain copy of known size (the unit of the
-    byte counters), then, for the FP64 and the int8-sliced form, four launches of the contraction
-    kernel alone and three full right-hand sides (every kernel of the path)."""
+    """Launches counted by `rocprofv3 --pmc` runs: three plain copies of 256 MiB (the unit of the
+    byte counters), then the segments of PMC_SEGMENTS."""
     dev = torch.device("cuda:0")
     a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)  # 256 MiB
     b = torch.empty_like(a)
@@ -367,15 +378,100 @@ def pmc_probe(torch, wlm, wl):
     ctx, x, w = wlm.make_context(wl)
     st = wlm.seeded_state(x, w, wl.N, wl.M, seed=1234)
     ctx.upload_state(st)
-    for prec in (PREC_CODE["fp64"], PREC_CODE["int8_sliced"]):
-        ctx.set_precision(prec)
+    for prec, mode in PMC_SEGMENTS:
+        b[:512].copy_(a[:512])  # segment separator
+        torch.cuda.synchronize()
+        ctx.set_precision(PREC_CODE[prec])
+        ctx.set_source_skip(mode == "skip")
         ctx.profile_contraction(3)
         for _ in range(3):
             ctx.time_derivative(st)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
     ctx.close()
-    print("pmc probe: 3 copies of 256 MiB; per form 4 launches of the contraction kernel and 3 full "
-          "right-hand sides", flush=True)
+    print("pmc probe: 3 copies of 256 MiB; segments " + ", ".join(f"{p}/{m}" for p, m in PMC_SEGMENTS) +
+          ", each the contraction kernel alone (4 launches) and 3 full right-hand sides", flush=True)
+
+
+def _pmc_rows(folder: str, counter: str):
+    """(dispatch id, kernel name, grid size, value) of every dispatch in a rocprofv3 --pmc output
+    folder, in dispatch order."""
+    import csv
+    import glob
+    import re
+
+    rows = {}
+    for fn in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                name = re.sub(r"^void ", "", r["Kernel_Name"]).split("(")[0].strip()
+                key = int(r["Dispatch_Id"])
+                # (a counter summed over the dispatch's hardware instances)
+                prev = rows.get(key)
+                val = float(r["Counter_Value"]) + (prev[3] if prev else 0.0)
+                rows[key] = (key, name, int(r.get("Grid_Size", 0) or 0), val)
+    return [rows[k] for k in sorted(rows)]
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, wl) -> None:
+    """profiles/pmc_traffic_<workload>.json and .md from the FETCH_SIZE and WRITE_SIZE passes over
+    --pmc-probe: per segment and kernel the median counter per dispatch, in HBM bytes after the
+    calibration by the 256 MiB copy."""
+    per = {}
+    calib = {}
+    for counter, folder in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _pmc_rows(folder, counter)
+        copies = [r for r in rows if "copyBuffer" in r[1]]
+        big = [r for r in copies if r[2] >= 1 << 20]
+        if len(big) < 3:
+            raise SystemExit(f"pmc summary: {counter}: the three 256 MiB calibration copies are missing in {folder}")
+        calib[counter] = float(np.median([r[3] for r in big]))
+        seps = [r[0] for r in copies if r[2] < 1 << 20]
+        if len(seps) != len(PMC_SEGMENTS):
+            raise SystemExit(f"pmc summary: {counter}: {len(seps)} separators, expected {len(PMC_SEGMENTS)}")
+        bounds = seps + [1 << 62]
+        for i, (prec, mode) in enumerate(PMC_SEGMENTS):
+            seg = [r for r in rows if bounds[i] < r[0] < bounds[i + 1] and "copyBuffer" not in r[1]]
+            kern = KERNEL[prec]
+            names = []
+            for r in seg:
+                if r[1] not in names:
+                    names.append(r[1])
+            for name in names:
+                vals = [r[3] for r in seg if r[1] == name]
+                if name.split("<")[0] == kern:
+                    # 1 warm-up + 3 alone, then one per right-hand side
+                    roles = [("contraction alone", vals[1:4]), ("in the right-hand side", vals[4:])]
+                else:
+                    roles = [("in the right-hand side", vals[-3:])]
+                for role, v in roles:
+                    if v:
+                        per.setdefault((name, prec, mode, role), {})[counter] = float(np.median(v))
+    copy_kib = 256 * 1024.0
+    ff, wf = copy_kib / calib["FETCH_SIZE"], copy_kib / calib["WRITE_SIZE"]
+    nGp = 256 * ((wl.nG + 255) // 256)
+    Pc = 16 * ((2 * wl.pairs + 15) // 16)
+    launches = []
+    for (name, prec, mode, role), c in per.items():
+        fk, wk = c.get("FETCH_SIZE"), c.get("WRITE_SIZE")
+        hb = None if fk is None or wk is None else 1024.0 * (fk * ff + wk * wf)
+        launches.append({"kernel": name, "form": prec, "mode": mode, "role": role, "fetch_kib": fk,
+                         "write_kib": wk, "hbm_bytes": hb})
+    out = {"workload": wl.name, "copy_kib": copy_kib, "fetch_factor": ff, "write_factor": wf,
+           "nGp": nGp, "Pc": Pc, "launches": launches}
+    base = os.path.join(ROOT, "profiles", f"pmc_traffic_{wl.name}")
+    with open(base + ".json", "w") as f:
+        json.dump(out, f, indent=1)
+    lines = [f"| {v['kernel']} | {v['form']} | {v['mode']} | {v['role']} | {v['fetch_kib']:.0f} | "
+             f"{v['write_kib']:.0f} | {v['hbm_bytes']:,.0f} |" for v in launches if v["hbm_bytes"] is not None]
+    with open(base + ".table.md", "w") as f:
+        f.write(f"Calibration: 256 MiB copy reads FETCH_SIZE {calib['FETCH_SIZE']:.0f} KiB and WRITE_SIZE "
+                f"{calib['WRITE_SIZE']:.0f} KiB: read bytes = {ff:.3f} x FETCH_SIZE KiB x 1024, write bytes = "
+                f"{wf:.3f} x WRITE_SIZE KiB x 1024.\n\n"
+                "| kernel | form | mode | role | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM bytes per launch |\n"
+                "|---|---|---|---|---:|---:|---:|\n" + "\n".join(lines) + "\n")
+    print(f"wrote {base}.json and {base}.table.md ({len(launches)} entries)")
 
 
 def batched_measurement(args, torch, wlm, check: bool):
@@ -579,6 +675,9 @@ def main():
 
 
     wl = wlm.WORKLOADS[args.workload]
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], wl)
+        return
     if args.pmc_probe:
         pmc_probe(torch, wlm, wl)
         return
@@ -656,7 +755,7 @@ def main():
     forms = ["fp64", "int8_sliced", "split_fp32", "bf16x3"] if args.precision == "auto" else [args.precision]
     results = {f: time_form(f, True) for f in forms}
     # the exact forms also without the zero-source skip (every source point contracted)
-    dense = {f + "_dense": time_form(f, False) for f in forms if f in SKIPPING}
+    dense = {} if args.no_dense else {f + "_dense": time_form(f, False) for f in forms if f in SKIPPING}
     ctx.set_source_skip(True)
     ctx.set_precision(0)
 

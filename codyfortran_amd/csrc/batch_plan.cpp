@@ -30,8 +30,8 @@ std::string check_plan_shape(const PlanShape& s) {
     return fmt("Gram x ranges %d do not divide the %d 64-point blocks", s.nxs, k64);
   if (s.int8) {
     if (s.Pc / 16 > kPlanInt8MaxTiles) return fmt("%d pair columns exceed the int8 contraction's %d", s.Pc, 16 * kPlanInt8MaxTiles);
-    if (s.nsplit != kPlanInt8Split) return fmt("int8 k split %d (expected %d)", s.nsplit, kPlanInt8Split);
-    if (k64 % s.nsplit != 0) return fmt("int8 k split %d does not divide the %d k blocks", s.nsplit, k64);
+    if (s.nsplit != 1) return fmt("int8 k split %d (the int8 contraction takes one range)", s.nsplit);
+    if (k64 > kPlanInt8MaxBlocks64) return fmt("%d k blocks exceed the int8 contraction's %d", k64, kPlanInt8MaxBlocks64);
   } else {
     if (s.Pc / 16 > kPlanF64MaxTiles) return fmt("%d pair columns exceed the FP64 contraction's %d", s.Pc, 16 * kPlanF64MaxTiles);
     if (!is_pow2_upto(s.nsplit, 8)) return fmt("k split %d is not one of 1, 2, 4, 8", s.nsplit);
@@ -50,8 +50,10 @@ void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]) {
   const int64_t nRed = ((int64_t)s.Pc * s.Pc + n2 + 255) / 256;
   const int64_t wsumFirst = s.int8 ? 0 : 1;  // R beside the int8 slicing, else in the k-split-sum launch
   cnt[kPlanFront] = nRdm + nObx * s.nO + plane / 256;
-  cnt[kPlanSlice] = s.int8 ? 1 + ((int64_t)s.Pc * (s.nGp >> 2) + 255) / 256 : 0;
-  cnt[kPlanContract] = (int64_t)(s.nGp / 64) * s.nsplit;
+  // the int8 slicing: R, the active-block list, then the slices
+  cnt[kPlanSlice] = s.int8 ? 2 + ((int64_t)s.Pc * (s.nGp >> 2) + 255) / 256 : 0;
+  // the int8 contraction: one workgroup per 16-row tile; the FP64 one: 64-row groups x k ranges
+  cnt[kPlanContract] = s.int8 ? (int64_t)(s.nGp / 16) : (int64_t)(s.nGp / 64) * s.nsplit;
   cnt[kPlanWsum] = wsumFirst + (s.nsplit > 1 ? (plane / 2 + 255) / 256 : 0);
   cnt[kPlanRho2p] = s.nO >= kPlanRho2pMinOrbs ? (n2 * n2 + 255) / 256 : 0;
   cnt[kPlanMid] = (int64_t)(s.Pc / 16) * s.nxs + (int64_t)(s.nGp / 64) * s.nO;

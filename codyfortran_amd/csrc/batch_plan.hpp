@@ -15,13 +15,13 @@ struct BlockRef {
 };
 static_assert(sizeof(BlockRef) == 8, "a block reference is uploaded as int2");
 
-// Limits of the grouped kernels (the device code's kMaxOrbs, kInt8Split and contraction tiles;
-// cody_api.hip asserts that they agree).
-constexpr int kPlanMaxOrbs = 12;       // orbitals per instance
-constexpr int kPlanMaxConf = 1024;     // CI configurations per instance
-constexpr int kPlanInt8Split = 4;      // k ranges of the int8 contraction
-constexpr int kPlanInt8MaxTiles = 5;   // 16-column tiles of the int8 contraction
-constexpr int kPlanF64MaxTiles = 10;   // 16-column tiles of the FP64 contraction
+// Limits of the grouped kernels (the device code's kMaxOrbs, kR16MaxBlocks64 and contraction
+// tiles; cody_api.hip asserts that they agree).
+constexpr int kPlanMaxOrbs = 12;           // orbitals per instance
+constexpr int kPlanMaxConf = 1024;         // CI configurations per instance
+constexpr int kPlanInt8MaxTiles = 5;       // 16-column tiles of the int8 contraction
+constexpr int kPlanInt8MaxBlocks64 = 256;  // 64-point k blocks of the int8 contraction (one range)
+constexpr int kPlanF64MaxTiles = 10;       // 16-column tiles of the FP64 contraction
 
 // The kernel types of a right-hand side, in launch order.
 enum PlanKind { kPlanFront, kPlanSlice, kPlanContract, kPlanWsum, kPlanRho2p, kPlanMid, kPlanPost, kPlanApply, kPlanKinds };
@@ -37,15 +37,16 @@ struct PlanShape {
   int nG = 0;      // grid points
   int nGp = 0;     // padded grid (a multiple of 256)
   int Pc = 0;      // pair columns, 16 ceil(nO (nO + 1) / 16)
-  int nsplit = 1;  // contraction k ranges (the int8 path: 4)
+  int nsplit = 1;  // contraction k ranges (the int8 path: 1, the 16-row kernel writes W^T)
   int nxs = 1;     // x ranges of the Gram partials
   bool int8 = false;
 };
 
 // Workgroups of each kernel type of one instance's right-hand side: the grids of its own launches
-// (front: CI densities + one-body part + pair sources; the int8 slicing with R; the contraction;
-// the k-split sum (with R on the FP64 path); rho2' for nO >= kPlanRho2pMinOrbs; Gram + orbital
-// correlation; G reduction + orbital projection; CI apply).  The shape must have passed check_plan_shape.
+// (front: CI densities + one-body part + pair sources; the int8 slicing with R and the active-block
+// list; the contraction; the k-split sum (with R on the FP64 path); rho2' for nO >=
+// kPlanRho2pMinOrbs; Gram + orbital correlation; G reduction + orbital projection; CI apply).  The
+// shape must have passed check_plan_shape.
 void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]);
 
 // Returns an empty string for a shape the grouped kernels accept, else the reason.

@@ -100,7 +100,11 @@ int prepare_precision(cody_ctx* c) {
 
 // The int8 form with at most 80 pair columns slices the tiled S^T (per-line maxima from the
 // pair-source kernel); the other reduced-precision forms read row-major S^T.
-bool int8_tiled(const cody_ctx* c) { return c->precision == CODY_PREC_INT8_SLICED && c->Pc <= 80; }
+// the int8 form on the 16-row contraction over the active k blocks, S slices fragment-tiled
+// (Pc <= 80, grids up to kR16MaxBlocks64 64-point blocks); else the untiled int8 kernel
+bool int8_tiled(const cody_ctx* c) {
+  return c->precision == CODY_PREC_INT8_SLICED && c->Pc <= 80 && c->nGp / 64 <= kR16MaxBlocks64;
+}
 double* plain_st(cody_ctx* c) { return (c->precision == CODY_PREC_FP64 || int8_tiled(c)) ? nullptr : c->d_ST; }
 unsigned long long* st_max(cody_ctx* c) { return int8_tiled(c) ? c->d_smax : nullptr; }
 // activity flags of the pair sources for the contraction (null: every k block is contracted)
@@ -118,20 +122,16 @@ int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(co
   if (reginvDone) *reginvDone = false;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
-      if (int8_tiled(c)) {  // 64-row kernel: S slices fragment-tiled, four k ranges
-        if (int r = check_contraction_shape("k_mf_i8_w", c->nGp, c->Pc, kInt8Split, 5, c->nGp / 64)) return r;
-        if (reginvDone) {  // R beside the slicing (the right-hand side's schedule)
-          launch_rhs_slice(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices,
-                           c->d_Si8, c->d_eS, st);
-          *reginvDone = true;
-        } else {
-          launch_slice_s_tiled(c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices, c->d_Si8, c->d_eS, st);
-        }
-        mark("k_slice_s_tiled");
-        CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, partialSplit ? nullptr : c->d_WT,
-                           c->nGp, c->Pc, kInt8Split, c->d_smax, st, act_flags(c)));
-        if (partialSplit) *partialSplit = kInt8Split;
-        mark(partialSplit ? "k_mf_i8_w" : "k_mf_i8_w + k_wsum");
+      if (int8_tiled(c)) {  // 16-row kernel over the active k blocks, W^T written directly
+        const int* act = act_flags(c);
+        int* list = act ? c->d_blist : nullptr;
+        // R beside the slicing when the right-hand side asks for it, and the active-block list
+        launch_rhs_slice(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices,
+                         c->d_Si8, c->d_eS, st, reginvDone != nullptr, act, list);
+        if (reginvDone) *reginvDone = true;
+        mark("k_rhs_slice");
+        CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, c->d_smax, list, st));
+        mark("k_mf_i8_r16");
         return CODY_OK;
       }
       launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, 0, c->d_Si8, c->d_eS, st);
@@ -470,9 +470,11 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   int ps = 0;
   bool reginvDone = false;
   CHK(enqueue_contraction(c, st, mark, &ps, &reginvDone));
-  launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st,
-                  !reginvDone);
-  mark("k_rhs_wsum");
+  if (ps > 1 || !reginvDone) {  // (nothing to do when the contraction wrote W^T and R is formed)
+    launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st,
+                    !reginvDone);
+    mark("k_rhs_wsum");
+  }
   launch_rhs_mid(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, orbs, c->d_w, c->d_hphi, c->d_r2can,
                  c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st);
   mark("k_rhs_mid");
@@ -950,7 +952,7 @@ int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* nam
 }
 
 // ---------------------------------------------------------------- batches of instances
-static_assert(kPlanMaxOrbs == kMaxOrbs && kPlanInt8Split == kInt8Split && kPlanRho2pMinOrbs == kRho2pMinOrbs,
+static_assert(kPlanMaxOrbs == kMaxOrbs && kPlanInt8MaxBlocks64 == kR16MaxBlocks64 && kPlanRho2pMinOrbs == kRho2pMinOrbs,
               "the host planner and the device kernels agree on their limits");
 static_assert(sizeof(BlockRef) == sizeof(int2), "block references are uploaded as int2");
 namespace {
@@ -959,11 +961,11 @@ namespace {
 // ~50 TFLOP/s plus ~50 us of latency-bound launches.
 double rhs_cost_estimate(const cody_ctx* c) { return 2.0 * c->nGp * (double)c->nGp * c->Pc / 50e6 + 50.0; }
 
-// Partitions the batch: instances on the FP64 or the tiled int8 contraction are grouped by
+// Partitions the batch: instances on the FP64 or the 16-row int8 contraction are grouped by
 // (orbital count, contraction), each group getting its argument table and block maps in device
-// memory; the others (the split-FP32 / BF16 forms, the untiled int8 kernel) keep their own
-// launches.  The maps repeat each instance's own launch grids (launch_rhs_front, the slicing,
-// launch_mf_f64_w / launch_mf_i8_w, launch_rhs_wsum, launch_rhs_mid, launch_rhs_post,
+// memory; the others (the split-FP32 / BF16 forms, the other int8 kernels) keep their own
+// launches.  The maps repeat each instance's own launch grids (launch_rhs_front, launch_rhs_slice,
+// launch_mf_f64_w / launch_mf_i8_r16, launch_rhs_wsum, launch_rhs_mid, launch_rhs_post,
 // launch_ci_apply_project), so a grouped workgroup does what that instance's own would.
 int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dstates) {
   b->release_groups();
@@ -1003,7 +1005,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.Pc = c->Pc;
       a.nRdm = (n2 + np * np + 3) / 4;
       a.nObx = (c->nG + 255) / 256;
-      a.nsplit = g.int8 ? kInt8Split : c->wsplit;
+      a.nsplit = g.int8 ? 1 : c->wsplit;  // (the int8 contraction writes W^T itself)
       a.wsumFirst = g.int8 ? 0 : 1;  // (the int8 path forms R beside the slicing)
       a.nxs = c->nxs;
       a.nPart = c->nPart;
@@ -1035,6 +1037,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.smax = st_max(c);
       a.act = c->d_act;
       a.actC = c->nGp / 16 > kSkipMaxBlocks16 ? nullptr : act_flags(c);
+      a.blist = c->d_blist;
       a.Vt = c->d_Vt;
       a.Vi8 = c->d_Vi8;
       a.eV = c->d_eV;
@@ -1082,7 +1085,7 @@ int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
   launch_grp_front(g.nO, g.d_inst, map(kPlanFront), g.cnt[kPlanFront], g.maxC, st);
   if (g.int8) {
     launch_grp_slice(g.nO, g.d_inst, map(kPlanSlice), g.cnt[kPlanSlice], st);
-    CHK(launch_grp_mf_i8_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
+    CHK(launch_grp_mf_i8_r16(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   } else {
     CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   }
@@ -1216,8 +1219,8 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
     switch (c->precision) {
       case CODY_PREC_INT8_SLICED:  // the contraction kernel alone
         if (int8_tiled(c))
-          CHK(launch_mf_i8_w(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_Wpart, nullptr, c->nGp, c->Pc, kInt8Split,
-                             nullptr, st, act_flags(c)));
+          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, nullptr,
+                               act_flags(c) ? c->d_blist : nullptr, st));
         else
           launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
         break;

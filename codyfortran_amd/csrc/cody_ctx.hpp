@@ -151,6 +151,7 @@ struct cody_ctx {
   double *d_STt = nullptr, *d_Wpart = nullptr;  // fragment-tiled S^T, k-split partials of W^T
   unsigned long long* d_smax = nullptr;          // per-line max |S^T| (int8 form), cleared per use
   int* d_act = nullptr;        // per-fragment activity of S^T (k_pair_src_t), Pc/16 x nGp/16
+  int* d_blist = nullptr;      // active 64-point k blocks of S^T: count, then indices (nGp/64 + 1)
   bool skip_zero = true;       // contraction skips the k blocks whose pair sources are all zero
   double *d_Gpart = nullptr, *d_G = nullptr, *d_energy = nullptr;
   double* d_epart = nullptr;  // energy partials of the G reduction (the CI projection's overlap)
@@ -192,6 +193,7 @@ struct cody_ctx {
     ALLOC(d_STt, sizeof(double) * Pc * nGp);
     ALLOC(d_smax, sizeof(unsigned long long) * Pc);
     ALLOC(d_act, sizeof(int) * (Pc / 16) * (nGp / 16));
+    ALLOC(d_blist, sizeof(int) * (nGp / 64 + 1));
     ALLOC(d_Wpart, sizeof(double) * cody::kMaxWSplit * Pc * nGp);
     ALLOC(d_Gpart, sizeof(double) * nxs * Pc * Pc);
     ALLOC(d_G, sizeof(double) * Pc * Pc);
@@ -273,7 +275,7 @@ struct cody_ctx {
 
   void release() {
     drop_graphs();
-    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_smax, d_act,
+    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_smax, d_act, d_blist,
                     d_Gpart, d_G,  d_energy, d_epart, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
                     d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, d_zeros, d_src, d_h2, ci_blob};
     for (void* p : ptrs)

@@ -109,6 +109,28 @@ __device__ int active_blocks(const int* __restrict__ act, int nT, int nK16, int 
 }
 
 
+// The active 64-point k blocks of the fragment-tiled S^T, for the 16-row int8 contraction, by one
+// wave: list[0] = their count, list[1..] their indices in increasing order.  Block b is active when
+// some activity flag act[t nK16 + 4 b + q] (t < nT, q < 4) is set.
+__device__ __forceinline__ void block_list_wave(const int* __restrict__ act, int nT, int nK16, int nK64,
+                                                int* __restrict__ list) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int n = 0;
+  for (int base = 0; base < nK64; base += 64) {
+    const int b = base + lane;
+    int any = 0;
+    if (b < nK64)
+      for (int t = 0; t < nT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) any |= act[(size_t)t * nK16 + 4 * b + q];
+    const unsigned long long m = __ballot(any != 0);
+    if (any) list[1 + n + __popcll(m & below)] = b;
+    n += __popcll(m);
+  }
+  if (lane == 0) list[0] = n;
+}
+
 // Gram split-K partials Gpart[y][c2][c1] = sum_{x in range y} S^T[c1][x] W^T[c2][x] (G = S^T W,
 // used by h2): A = S^T fragment-tiled (as k_pair_src_t writes it), B = W^T row-major; one
 // workgroup per 16 rows of S^T (bx) and k range by, one quarter of it per wave.

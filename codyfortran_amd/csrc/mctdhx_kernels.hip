@@ -739,19 +739,32 @@ __global__ __launch_bounds__(256) void k_rhs_wsum(const double2* __restrict__ rh
   wsum_body(blockIdx.x - first, Wpart, nsplit, plane, WT);
 }
 
-// Before the int8 contraction: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0, one wave) beside the
-// slicing of the tiled S^T (the others), both after the front launch.
+// Before the int8 contraction, after the front launch: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0,
+// one wave; when withR), the active-block list of the 16-row contraction (the next workgroup, one
+// wave; when list is set) and the slicing of the tiled S^T (the others).
 template <int NO>
 __global__ __launch_bounds__(256) void k_rhs_slice(const double2* __restrict__ rho1, double reg,
                                                    double2* __restrict__ reginv, const double* __restrict__ STt,
                                                    const unsigned long long* __restrict__ smax, int nGp, int Pc,
-                                                   int ns, int8_t* __restrict__ out, int* __restrict__ expo) {
-  if (blockIdx.x == 0) {
-    if (threadIdx.x >= 64) return;
-    reginv_body<NO>(rho1, reg, reginv, 64);
-    return;
+                                                   int ns, int8_t* __restrict__ out, int* __restrict__ expo,
+                                                   int withR, const int* __restrict__ act, int* __restrict__ list) {
+  int b = blockIdx.x;
+  if (withR) {
+    if (b == 0) {
+      if (threadIdx.x >= 64) return;
+      reginv_body<NO>(rho1, reg, reginv, 64);
+      return;
+    }
+    --b;
   }
-  slice_s_tiled_body((blockIdx.x - 1) * blockDim.x + threadIdx.x, STt, smax, nGp, Pc, ns, out, expo);
+  if (list) {
+    if (b == 0) {
+      if (threadIdx.x < 64) block_list_wave(act, Pc / 16, nGp >> 4, nGp >> 6, list);
+      return;
+    }
+    --b;
+  }
+  slice_s_tiled_body(b * blockDim.x + threadIdx.x, STt, smax, nGp, Pc, ns, out, expo);
 }
 
 // After W: the Gram matrix's split-K partials (G = S^T W) and the orbital correlation with its
@@ -827,7 +840,11 @@ __global__ __launch_bounds__(256) void k_grp_slice(const GroupInstance* __restri
     reginv_body<NO>(a.rho1, a.reg, a.reginv, 64);
     return;
   }
-  slice_s_tiled_body((m.y - 1) * blockDim.x + threadIdx.x, a.STt, a.smax, a.nGp, a.Pc, kInt8Slices, a.Si8, a.eS);
+  if (m.y == 1) {  // the active-block list of the contraction (skipping instances)
+    if (a.actC && threadIdx.x < 64) block_list_wave(a.actC, a.Pc / 16, a.nGp >> 4, a.nGp >> 6, a.blist);
+    return;
+  }
+  slice_s_tiled_body((m.y - 2) * blockDim.x + threadIdx.x, a.STt, a.smax, a.nGp, a.Pc, kInt8Slices, a.Si8, a.eS);
 }
 
 template <int NO>
@@ -952,11 +969,12 @@ void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, c
 
 void launch_rhs_slice(const double2* rho1, int nO, double reg, double2* reginv, const double* STt,
                       const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out, int* expo,
-                      hipStream_t st) {
+                      hipStream_t st, bool withR, const int* act, int* list) {
   const int n = Pc * (nGp >> 2);
-#define CALL_SL(NO)                                                                                           \
-  hipLaunchKernelGGL(k_rhs_slice<NO>, dim3(1 + (n + 255) / 256), dim3(256), 0, st, rho1, reg, reginv, STt, smax, \
-                     nGp, Pc, ns, out, expo)
+  const int withR_ = withR ? 1 : 0, grid = withR_ + (list ? 1 : 0) + (n + 255) / 256;
+#define CALL_SL(NO)                                                                                              \
+  hipLaunchKernelGGL(k_rhs_slice<NO>, dim3(grid), dim3(256), 0, st, rho1, reg, reginv, STt, smax, nGp, Pc, ns, out, \
+                     expo, withR_, act, list)
   CODY_NO_SWITCH(nO, CALL_SL)
 #undef CALL_SL
 }

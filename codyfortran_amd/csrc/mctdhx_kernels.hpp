@@ -75,6 +75,7 @@ struct GroupInstance {
   unsigned long long* smax;  // int8 path: per-line maxima of S^T
   int* act;                  // pair-source activity flags (written by the front launch)
   const int* actC;           // the flags the contraction reads (null: no skipping)
+  int* blist;                // int8 path: the active-block list built from actC (when set)
   double2* r2p;              // rho2' (nO^4) for the orbital correlation, or null (formed in it)
   const double* Vt;                          // FP64 contraction operand
   const int8_t* Vi8;                         // int8 contraction operands
@@ -93,7 +94,7 @@ void launch_grp_post(int nO, const GroupInstance* gi, const int2* map, int nBloc
 void launch_grp_apply(const GroupInstance* gi, const int2* map, int nBlocks, int nO, hipStream_t st);
 // the contractions, Pc = 16 NT for every instance of the group (CODY_ERR_ARG otherwise)
 int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
-int launch_grp_mf_i8_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+int launch_grp_mf_i8_r16(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 
 void launch_zero(double2* p, int n, hipStream_t st);
 // y += x (n complex values)
@@ -145,11 +146,12 @@ void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, do
                       int Pc, double* STt, unsigned long long* smax, int* act, hipStream_t st);
 void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, const double* Wpart, int nsplit,
                      size_t plane, double* WT, hipStream_t st, bool withReginv = true);
-// the int8 slicing of the tiled S^T (launch_slice_s_tiled) with R = rho1 (rho1^2 + eps)^{-1} on
-// one more workgroup
+// the int8 slicing of the tiled S^T (slice_s_tiled_body) with, on one more workgroup each,
+// R = rho1 (rho1^2 + eps)^{-1} (withR) and the active-block list of the 16-row contraction from the
+// activity flags act (list: nGp / 64 + 1 ints; null for none)
 void launch_rhs_slice(const double2* rho1, int nO, double reg, double2* reginv, const double* STt,
                       const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out, int* expo,
-                      hipStream_t st);
+                      hipStream_t st, bool withR = true, const int* act = nullptr, int* list = nullptr);
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
                     const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
                     const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st);
@@ -179,7 +181,6 @@ void launch_slice_rows(const double* X, int len, int nlines, int ns, int tiled, 
 void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream_t st);
 void launch_split_bf16x3(const double* X, size_t n, __bf16* out, hipStream_t st);
 constexpr int kMaxWSplit = 8;  // largest k split of the FP64 contraction (partials buffer depth)
-constexpr int kInt8Split = 4;  // k split of the 64-row int8 contraction
 constexpr int kRho2pMinOrbs = 10;  // grouped right-hand sides form rho2' once per instance from this many orbitals
 constexpr int kSkipMaxBlocks16 = 1024;  // longest grid (in 16-point blocks) whose zero k blocks are skipped
 // shape check of the k-split contraction launchers (0 or CODY_ERR_ARG, nothing launched)
@@ -191,11 +192,10 @@ int launch_mf_f64_w(const double* Vt, const double* STt, double* Wpart, double* 
 void launch_tile_f64(const double* V, int nGp, double* Vt, hipStream_t st);
 void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp, int Pc, int nsplit,
                      hipStream_t st);
-int launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
-                   double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st,
-                   const int* act = nullptr);
-void launch_slice_s_tiled(const double* STt, const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out,
-                          int* expo, hipStream_t st);
+constexpr int kR16MaxBlocks64 = 256;  // longest grid (in 64-point blocks) of the 16-row int8 contraction
+constexpr int kR16Prefetch = 2;       // k blocks of operands in flight per wave of the 16-row int8 contraction
+int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int Pc,
+                     unsigned long long* smax, const int* list, hipStream_t st);
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,
                   int Pc, hipStream_t st);
 void launch_mf_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl, double* WT, int nGp,

@@ -191,143 +191,93 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, 
   wsum_body(blockIdx.x, Wpart, nsplit, plane, WT);
 }
 
-// ---------------------------------------------------------------------------------------- int8, 64-row
-// The int8-sliced contraction with the decomposition of k_mf_f64_w: a workgroup owns 64 rows
-// (one 16-row tile per wave) and one of gridDim.y contiguous k ranges.  Both operands are
-// fragment-tiled (k_slice_rows with tiled output): slice p of the 16-line x 64-k block (R, K) is
+// ---------------------------------------------------------------------------------------- int8, 16-row
+// The int8-sliced contraction for the right-hand side: one workgroup per 16-row tile of W (nGp / 16
+// workgroups) and one wave per 16-column tile t, all waves walking the same k blocks: the active-block
+// list (block_list_wave), or all blocks when the list is null.  Both operands are fragment-tiled
+// (k_slice_rows with tiled output, slice_s_tiled_body): slice p of the 16-line x 64-k block (R, K) is
 // 1 KiB at ((R nK64 + K) NS + p) 1024, lane l holding bytes k = 64K + 16(l/16) .. +15 of line
-// 16R + l%16.  Per 64-wide k block the S slices (NT x NS chunks of 1 KiB) are fetched once per
-// workgroup into LDS; each wave streams its own V slices.  Three register stages: every global
-// load is issued two blocks ahead of its MFMAs (loads complete in issue order).  The steps are
-// written out six at a time so that the register stage (mod 3) and the LDS buffer (mod 2) are
-// compile-time constants.  Products with p + q <= NS - 1 that share the scale 254^{-(p+q+2)}
-// accumulate exactly in one int32 tile; the diagonals are recombined in FP64 with the row and
-// column exponents, and the workgroup stores its FP64 partial (k_wsum adds the k ranges).
-// The body takes the workgroup's row group bx, k range by of nsplit (the grid of k_mf_i8_w).
-template <int NT>
-__device__ __forceinline__ void mf_i8_w_body(int bx, int by, int nsplit, const int8_t* __restrict__ As,
-                                             const int* __restrict__ eA, const int8_t* __restrict__ Bs,
-                                             const int* __restrict__ fB, int nGp, double* __restrict__ Wpart,
-                                             unsigned long long* __restrict__ smax, const int* __restrict__ act) {
-  constexpr int NS = kInt8Slices, W = 4, D = 3, LD = NT * 16 + 1;
+// 16R + l%16.  Per block a wave loads the five V slice fragments of its rows (the same for every
+// wave of the workgroup) and the five S slice fragments of its columns, PF - 1 blocks ahead (loads
+// past the last block are clamped onto it; loads complete in issue order).  Products with p + q <= NS - 1
+// that share the scale 254^{-(p+q+2)} accumulate exactly in one int32 tile per diagonal over the
+// whole k range (|sum| <= NS 127^2 nGp < 2^31), issued diagonal-interleaved so that consecutive
+// MFMAs write different tiles.  Each diagonal is recombined once in FP64 with the row and column
+// exponents and W^T is written directly: no k-split partials, no summing launch, no cross-wave
+// reduction.  (Measured at 4096 points with 18 of 64 blocks active: two blocks in flight beat three
+// or four, and sharing the V fragments through LDS with a barrier per block is slower.)
+// The body takes the workgroup's row tile rt.
+template <int NT, int PF>
+__device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
+                                               const int8_t* __restrict__ Bs, const int* __restrict__ fB, int nGp,
+                                               double* __restrict__ WT, unsigned long long* __restrict__ smax,
+                                               const int* __restrict__ list) {
+  constexpr int NS = kInt8Slices;
+  __shared__ int sList[kR16MaxBlocks64];
   // the per-line maxima of S^T were consumed by the slicing (earlier on this stream): clear them
   // for the next right-hand side
-  if (smax && bx == 0 && by == 0 && threadIdx.x < NT * 16) smax[threadIdx.x] = 0ull;
-  constexpr int NCH = NT * NS, CPW = (NCH + W - 1) / W;  // 1 KiB S chunks per block, per wave
-  constexpr int BUF = W * CPW * 1024;                    // bytes per LDS buffer
-  constexpr int LDSB = 2 * BUF > 4 * 16 * LD * 8 ? 2 * BUF : 4 * 16 * LD * 8;
-  __shared__ __align__(16) int8_t lds[LDSB];
-  __shared__ int sList[kSkipMaxBlocks16 / 4];
-  __shared__ int sWave[W + 1];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  if (smax && rt == 0 && threadIdx.x < NT * 16) smax[threadIdx.x] = 0ull;
+  const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
   const int nK64 = nGp >> 6;
-  // k blocks of this k range: consecutive positions of the active-block list, or of all blocks
-  // when act is null
-  const int nbw = (act ? active_blocks(act, NT, nGp >> 4, 4, nK64, nsplit, sList, sWave) : nK64) / nsplit;
-  auto kblock = [&](int blk) {
-    const int j = by * nbw + min(blk, nbw - 1);
-    return act ? sList[j] : j;
-  };
-  const int8_t* ab = As + (size_t)(bx * 4 + wave) * nK64 * NS * 1024 + 16 * lane;
-  auto chunk_src = [&](int m, int K) {  // chunk c = t NS + p of this wave's share, k block K
-    const int c = min(wave + W * m, NCH - 1);
-    const int t = c / NS, p = c % NS;
-    return Bs + (((size_t)t * nK64 + K) * NS + p) * 1024 + 16 * lane;
-  };
-  i4_t acc[NT][NS];
+  const int total = list ? __builtin_amdgcn_readfirstlane(list[0]) : nK64;
+  if (list) {
+    for (int j = threadIdx.x; j < total; j += NT * 64) sList[j] = list[1 + j];
+    __syncthreads();
+  }
+  const int8_t* ab = As + (size_t)rt * nK64 * NS * 1024 + 16 * lane;
+  const int8_t* bb = Bs + (size_t)t * nK64 * NS * 1024 + 16 * lane;
+  i4_t acc[NS];
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int d = 0; d < NS; ++d) acc[t][d] = (i4_t){0, 0, 0, 0};
-  i4_t a[D][NS];
-  i4_t stg[D][CPW];
-  auto gload = [&](int u, int blk) {
-    const int K = kblock(blk);
+  for (int d = 0; d < NS; ++d) acc[d] = (i4_t){0, 0, 0, 0};
+  i4_t a[PF][NS], b[PF][NS];
+  auto load = [&](int u, int j) {
+    j = min(j, total - 1);
+    const int K = list ? __builtin_amdgcn_readfirstlane(sList[j]) : j;
     const size_t o = (size_t)K * NS * 1024;
 #pragma unroll
     for (int p = 0; p < NS; ++p) a[u][p] = *reinterpret_cast<const i4_t*>(ab + o + p * 1024);
 #pragma unroll
-    for (int m = 0; m < CPW; ++m) stg[u][m] = *reinterpret_cast<const i4_t*>(chunk_src(m, K));
+    for (int p = 0; p < NS; ++p) b[u][p] = *reinterpret_cast<const i4_t*>(bb + o + p * 1024);
   };
-  auto lwrite = [&](int u, int buf) {
+  auto mfma = [&](int u) {
 #pragma unroll
-    for (int m = 0; m < CPW; ++m) *reinterpret_cast<i4_t*>(lds + buf * BUF + (wave + W * m) * 1024 + 16 * lane) = stg[u][m];
+    for (int p = 0; p < NS; ++p)
+#pragma unroll
+      for (int d = p; d < NS; ++d)
+        acc[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][p], b[u][d - p], acc[d], 0, 0, 0);
   };
-  auto step = [&](int u, int buf) {
+  if (total > 0) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      i4_t b[NS];
+    for (int u = 0; u < PF - 1; ++u) load(u, u);
+    for (int j = 0; j < total; j += PF) {
 #pragma unroll
-      for (int p = 0; p < NS; ++p) b[p] = *reinterpret_cast<const i4_t*>(lds + buf * BUF + (t * NS + p) * 1024 + 16 * lane);
-#pragma unroll
-      for (int d = 0; d < NS; ++d)
-#pragma unroll
-        for (int p = 0; p <= d; ++p)
-          acc[t][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][p], b[d - p], acc[t][d], 0, 0, 0);
-    }
-  };
-  // stage invariant at the top of an iteration (i a multiple of 6): register stages 0, 1 hold
-  // blocks i, i+1 and block i is in LDS buffer 0.  Loads past the last block are clamped onto it.
-#define CODY_I8_STEP(J)                          \
-  {                                              \
-    gload(((J) + 2) % D, i + (J) + 2);           \
-    lwrite(((J) + 1) % D, ((J) + 1) & 1);        \
-    step((J) % D, (J) & 1);                      \
-    __syncthreads();                             \
-  }
-  if (nbw > 0) {  // (no active block: the partial stays zero)
-    gload(0, 0);
-    gload(1, 1);
-    lwrite(0, 0);
-    __syncthreads();
-    int i = 0;
-    for (; i + 6 <= nbw; i += 6) {
-      CODY_I8_STEP(0)
-      CODY_I8_STEP(1)
-      CODY_I8_STEP(2)
-      CODY_I8_STEP(3)
-      CODY_I8_STEP(4)
-      CODY_I8_STEP(5)
-    }
-    // remaining nbw - i < 6 blocks (the conditions are uniform over the workgroup)
-    if (i < nbw) CODY_I8_STEP(0)
-    if (i + 1 < nbw) CODY_I8_STEP(1)
-    if (i + 2 < nbw) CODY_I8_STEP(2)
-    if (i + 3 < nbw) CODY_I8_STEP(3)
-    if (i + 4 < nbw) CODY_I8_STEP(4)
-  }
-#undef CODY_I8_STEP
-  // FP64 recombination; 32-bit MFMA results: element u of lane l is row 4(l/16) + u
-  double* red = reinterpret_cast<double*>(lds);
-  double* slot = red + wave * 16 * LD;
-  const int rowb = bx * 64 + wave * 16;
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int fc = fB[t * 16 + rr];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      double v = 0.0;
-#pragma unroll
-      for (int d = NS - 1; d >= 0; --d) v += static_cast<double>(acc[t][d][u]) * kSliceScale.v[d + 2];
-      slot[(4 * q + u) * LD + t * 16 + rr] = ldexp(v, eA[rowb + 4 * q + u] + fc);
+      for (int u = 0; u < PF; ++u) {
+        load((u + PF - 1) % PF, j + u + PF - 1);
+        if (j + u < total) mfma(u);
+      }
     }
   }
-  __syncthreads();
-  const int row0 = bx * 64;
-  double* Wp = Wpart + by * (size_t)NT * 16 * nGp;
-  for (int e = threadIdx.x; e < 64 * NT * 16; e += blockDim.x) {
-    const int c = e >> 6, rl = e & 63;
-    Wp[(size_t)c * nGp + row0 + rl] = red[(rl >> 4) * 16 * LD + (rl & 15) * LD + c];
-  }
+  // 32-bit MFMA results: element u of lane l is row 4(l/16) + u, column l%16
+  const int row0 = rt * 16 + 4 * q;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int d = NS - 1; d >= 0; --d)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] += static_cast<double>(acc[d][u]) * kSliceScale.v[d + 2];
+  const int fc = fB[t * 16 + rr];
+  double* dst = WT + (size_t)(t * 16 + rr) * nGp + row0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) dst[u] = ldexp(v[u], eA[row0 + u] + fc);
 }
 
 template <int NT>
-__global__ __launch_bounds__(256) void k_mf_i8_w(const int8_t* __restrict__ As, const int* __restrict__ eA,
-                                                 const int8_t* __restrict__ Bs, const int* __restrict__ fB,
-                                                 int nGp, double* __restrict__ Wpart,
-                                                 unsigned long long* __restrict__ smax,
-                                                 const int* __restrict__ act) {
-  mf_i8_w_body<NT>(blockIdx.x, blockIdx.y, gridDim.y, As, eA, Bs, fB, nGp, Wpart, smax, act);
+__global__ __launch_bounds__(NT * 64) void k_mf_i8_r16(const int8_t* __restrict__ As, const int* __restrict__ eA,
+                                                        const int8_t* __restrict__ Bs, const int* __restrict__ fB,
+                                                        int nGp, double* __restrict__ WT,
+                                                        unsigned long long* __restrict__ smax,
+                                                        const int* __restrict__ list) {
+  mf_i8_r16_body<NT, kR16Prefetch>(blockIdx.x, As, eA, Bs, fB, nGp, WT, smax, list);
 }
 
 // ---------------------------------------------------------------------------------------- int8 slices
@@ -609,12 +559,11 @@ __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_grp_mf_f64_w(const Grou
 }
 
 template <int NT>
-__global__ __launch_bounds__(256) void k_grp_mf_i8_w(const GroupInstance* __restrict__ gi,
-                                                     const int2* __restrict__ map) {
+__global__ __launch_bounds__(NT * 64) void k_grp_mf_i8_r16(const GroupInstance* __restrict__ gi,
+                                                            const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
-  const int nbx = a.nGp / 64;
-  mf_i8_w_body<NT>(m.y % nbx, m.y / nbx, a.nsplit, a.Vi8, a.eV, a.Si8, a.eS, a.nGp, a.Wpart, a.smax, a.actC);
+  mf_i8_r16_body<NT, kR16Prefetch>(m.y, a.Vi8, a.eV, a.Si8, a.eS, a.nGp, a.WT, a.smax, a.actC ? a.blist : nullptr);
 }
 
 int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
@@ -637,15 +586,15 @@ int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nB
   return 0;
 }
 
-int launch_grp_mf_i8_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
-#define CALL_GI(NT) hipLaunchKernelGGL(k_grp_mf_i8_w<NT>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+int launch_grp_mf_i8_r16(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GI(NT) hipLaunchKernelGGL(k_grp_mf_i8_r16<NT>, dim3(nBlocks), dim3((NT) * 64), 0, st, gi, map)
   switch (Pc / 16) {
     case 1: CALL_GI(1); break;
     case 2: CALL_GI(2); break;
     case 3: CALL_GI(3); break;
     case 4: CALL_GI(4); break;
     case 5: CALL_GI(5); break;
-    default: return set_error(CODY_ERR_ARG, "k_grp_mf_i8_w: Pc / 16 outside 1..5");
+    default: return set_error(CODY_ERR_ARG, "k_grp_mf_i8_r16: Pc / 16 outside 1..5");
   }
 #undef CALL_GI
   return 0;
@@ -662,30 +611,24 @@ void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp
 #undef CALL_G
 }
 
-// 64-row int8 contraction (Pc <= 80): Bs fragment-tiled; nsplit in {1, 2, 4, 8} dividing nGp / 64
-// (WT is written directly for nsplit = 1).  With WT null only the partials are formed.  Returns 0,
-// or CODY_ERR_ARG without launching anything when the shape is outside check_contraction_shape.
-int launch_mf_i8_w(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* Wpart,
-                   double* WT, int nGp, int Pc, int nsplit, unsigned long long* smax, hipStream_t st,
-                   const int* act) {
-  if (int r = check_contraction_shape("k_mf_i8_w", nGp, Pc, nsplit, 5, nGp / 64)) return r;
-  if (nGp / 16 > kSkipMaxBlocks16) act = nullptr;  // (the block list lives in LDS)
-  double* out = (nsplit == 1 && WT) ? WT : Wpart;
-#define CALL_I8(NT) \
-  hipLaunchKernelGGL(k_mf_i8_w<NT>, dim3(nGp / 64, nsplit), dim3(256), 0, st, As, eA, Bs, fB, nGp, out, smax, act)
+// 16-row int8 contraction of the right-hand side (Pc <= 80, nGp / 64 <= 256): W^T
+// written directly.  list: the active k blocks (block_list_wave) or null for all blocks.  Returns 0,
+// or CODY_ERR_ARG without launching anything when the shape is outside these limits.
+int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int Pc,
+                     unsigned long long* smax, const int* list, hipStream_t st) {
+  if (nGp < 64 || nGp % 64 || Pc < 16 || Pc > 80 || Pc % 16 || nGp / 64 > kR16MaxBlocks64)
+    return set_error(CODY_ERR_ARG, "k_mf_i8_r16: nGp must be a positive multiple of 64, at most 64 x 256, "
+                                   "and Pc one of 16, 32, 48, 64, 80");
+#define CALL_R16(NT) \
+  hipLaunchKernelGGL(k_mf_i8_r16<NT>, dim3(nGp / 16), dim3((NT) * 64), 0, st, As, eA, Bs, fB, nGp, WT, smax, list)
   switch (Pc / 16) {
-    case 1: CALL_I8(1); break;
-    case 2: CALL_I8(2); break;
-    case 3: CALL_I8(3); break;
-    case 4: CALL_I8(4); break;
-    case 5: CALL_I8(5); break;
-    default: break;  // (excluded by the shape check)
+    case 1: CALL_R16(1); break;
+    case 2: CALL_R16(2); break;
+    case 3: CALL_R16(3); break;
+    case 4: CALL_R16(4); break;
+    case 5: CALL_R16(5); break;
   }
-#undef CALL_I8
-  if (!WT || nsplit == 1) return 0;
-  const size_t plane = (size_t)Pc * nGp;
-  hipLaunchKernelGGL(k_wsum, dim3((unsigned)((plane / 2 + 255) / 256)), dim3(256), 0, st, Wpart, nsplit, plane,
-                     WT);
+#undef CALL_R16
   return 0;
 }
 

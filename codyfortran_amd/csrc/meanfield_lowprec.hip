@@ -77,14 +77,6 @@ __global__ __launch_bounds__(1024) void k_slice_rows(const double* __restrict__ 
   }
 }
 
-// ------------------------------------------------------------------ slicing of the tiled S^T
-__global__ __launch_bounds__(256) void k_slice_s_tiled(const double* __restrict__ STt,
-                                                       const unsigned long long* __restrict__ smax, int nGp,
-                                                       int Pc, int ns, int8_t* __restrict__ out,
-                                                       int* __restrict__ expo) {
-  slice_s_tiled_body(blockIdx.x * blockDim.x + threadIdx.x, STt, smax, nGp, Pc, ns, out, expo);
-}
-
 // ------------------------------------------------------------------ FP64 tiling
 // Vt[(R nK + K) 256 + 128 h + 2 l + e] = V[16 R + l % 16][16 K + 4 (l / 16) + 2 h + e]: every 16 x 16
 // block of V becomes 2 KB of contiguous MFMA fragments (two 1 KB loads per wave).
@@ -132,12 +124,6 @@ __global__ __launch_bounds__(256) void k_split_bf16x3(const double* __restrict__
 void launch_slice_rows(const double* X, int len, int nlines, int ns, int tiled, int8_t* out, int* expo,
                        hipStream_t st) {
   hipLaunchKernelGGL(k_slice_rows, dim3(nlines), dim3(1024), 0, st, X, len, nlines, ns, tiled, out, expo);
-}
-
-void launch_slice_s_tiled(const double* STt, const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out,
-                          int* expo, hipStream_t st) {
-  const int n = Pc * (nGp >> 2);
-  hipLaunchKernelGGL(k_slice_s_tiled, dim3((n + 255) / 256), dim3(256), 0, st, STt, smax, nGp, Pc, ns, out, expo);
 }
 
 void launch_tile_f64(const double* V, int nGp, double* Vt, hipStream_t st) {

@@ -59,11 +59,13 @@ int main() {
     plan_block_counts(s, c);
     const int64_t want[kPlanKinds] = {212 + 128 + 1280, 0, 256, 641, 0, 160 + 512, 26 + 16, 18};
     for (int k = 0; k < kPlanKinds; ++k) EXPECT(c[k] == want[k], "kind %d: %lld blocks, expected %lld", k, (long long)c[k], (long long)want[k]);
-    // the int8 path: slicing with R (1 + 80 x 1024 / 256), four k ranges, no R in the sum launch
-    const PlanShape s8 = shape(8, 70, 4096, 4, true);
+    // the int8 path: slicing with R and the active-block list (2 + 80 x 1024 / 256), one
+    // contraction workgroup per 16-row tile writing W^T (no k-split sum, R not in the sum launch)
+    const PlanShape s8 = shape(8, 70, 4096, 1, true);
     plan_block_counts(s8, c);
-    EXPECT(c[kPlanSlice] == 1 + 320, "int8 slicing blocks %lld", (long long)c[kPlanSlice]);
-    EXPECT(c[kPlanWsum] == 640, "int8 sum blocks %lld", (long long)c[kPlanWsum]);
+    EXPECT(c[kPlanSlice] == 2 + 320, "int8 slicing blocks %lld", (long long)c[kPlanSlice]);
+    EXPECT(c[kPlanContract] == 256, "int8 contraction blocks %lld", (long long)c[kPlanContract]);
+    EXPECT(c[kPlanWsum] == 0, "int8 sum blocks %lld", (long long)c[kPlanWsum]);
     // 12 orbitals: rho2' (12^4 entries, one per thread) is formed once per right-hand side
     plan_block_counts(shape(12, 924, 4096, 4, false), c);
     EXPECT(c[kPlanRho2p] == 81, "rho2' blocks %lld", (long long)c[kPlanRho2p]);
@@ -78,7 +80,7 @@ int main() {
       int k = 0;
       for (int nG : {511, 1026, 2046, 4096})
         for (int split : {1, 2, 4, 8}) {
-          if (int8 && split != kPlanInt8Split) continue;
+          if (int8 && split != 1) continue;
           members.push_back(shape(nO, (int)binom(nO, nO / 2), nG, split, int8));
           ++k;
         }
@@ -113,7 +115,8 @@ int main() {
     EXPECT(refused(shape(8, 70, 4096, 16, false)), "16-way split accepted");  // (the one that overran d_Wpart)
     EXPECT(refused(shape(8, 70, 4096, 3, false)), "3-way split accepted");
     EXPECT(refused(shape(12, 924, 4096, 4, true)), "int8 with 160 pair columns accepted");
-    EXPECT(refused(shape(8, 70, 4096, 2, true)), "int8 with a 2-way split accepted");
+    EXPECT(refused(shape(8, 70, 4096, 4, true)), "int8 with a 4-way split accepted");
+    EXPECT(refused(shape(8, 70, 16640, 1, true)), "int8 with 260 k blocks accepted");
     EXPECT(refused(shape(13, 70, 4096, 1, false)), "13 orbitals accepted");
     EXPECT(refused(shape(12, 1025, 4096, 1, false)), "1025 configurations accepted");
     PlanShape s = shape(8, 70, 4096, 4, false);
