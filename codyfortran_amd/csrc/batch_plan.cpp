@@ -56,7 +56,9 @@ void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]) {
   cnt[kPlanContract] = s.int8 ? (int64_t)(s.nGp / 16) : (int64_t)(s.nGp / 64) * s.nsplit;
   cnt[kPlanWsum] = wsumFirst + (s.nsplit > 1 ? (plane / 2 + 255) / 256 : 0);
   cnt[kPlanRho2p] = s.nO >= kPlanRho2pMinOrbs ? (n2 * n2 + 255) / 256 : 0;
-  cnt[kPlanMid] = (int64_t)(s.Pc / 16) * s.nxs + (int64_t)(s.nGp / 64) * s.nO;
+  // Gram + orbital correlation; the int8 path's contraction forms the correlation, and one
+  // workgroup sums its overlap partials beside the Gram
+  cnt[kPlanMid] = (int64_t)(s.Pc / 16) * s.nxs + (s.int8 ? 1 : (int64_t)(s.nGp / 64) * s.nO);
   cnt[kPlanPost] = nRed + nObx;
   cnt[kPlanApply] = (s.nC + 3) / 4;
 }

@@ -44,8 +44,9 @@ struct PlanShape {
 
 // Workgroups of each kernel type of one instance's right-hand side: the grids of its own launches
 // (front: CI densities + one-body part + pair sources; the int8 slicing with R and the active-block
-// list; the contraction; the k-split sum (with R on the FP64 path); rho2' for nO >=
-// kPlanRho2pMinOrbs; Gram + orbital correlation; G reduction + orbital projection; CI apply).  The
+// list; the contraction, with the orbital correlation on the int8 path; the k-split sum (with R on
+// the FP64 path); rho2' for nO >= kPlanRho2pMinOrbs; Gram + orbital correlation, or on the int8
+// path Gram + the sum of the overlap partials; G reduction + orbital projection; CI apply).  The
 // shape must have passed check_plan_shape.
 void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]);
 

@@ -116,7 +116,7 @@ const int* act_flags(const cody_ctx* c) { return c->skip_zero ? c->d_act : nullp
 // Returns 0, or the launcher's error (nothing of the contraction launched) when the shape is
 // outside what the kernels take.
 int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark,
-                        int* partialSplit = nullptr, bool* reginvDone = nullptr) {
+                        int* partialSplit = nullptr, bool* reginvDone = nullptr, const OrbCorrArgs* orb = nullptr) {
   const size_t ns = (size_t)c->Pc * c->nGp;
   if (partialSplit) *partialSplit = 0;
   if (reginvDone) *reginvDone = false;
@@ -125,13 +125,21 @@ int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(co
       if (int8_tiled(c)) {  // 16-row kernel over the active k blocks, W^T written directly
         const int* act = act_flags(c);
         int* list = act ? c->d_blist : nullptr;
-        // R beside the slicing when the right-hand side asks for it, and the active-block list
+        // R beside the slicing when the right-hand side asks for it (the fused correlation reads it),
+        // and the active-block list
+        if (orb && !reginvDone) return set_error(CODY_ERR_ARG, "enqueue_contraction: the fused correlation needs R");
         launch_rhs_slice(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices,
                          c->d_Si8, c->d_eS, st, reginvDone != nullptr, act, list);
         if (reginvDone) *reginvDone = true;
         mark("k_rhs_slice");
-        CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, c->d_smax, list, st));
-        mark("k_mf_i8_r16");
+        if (orb) {
+          CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->nO, c->d_smax, list, *orb,
+                               st));
+          mark("k_mf_i8_orb");
+        } else {
+          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, c->d_smax, list, st));
+          mark("k_mf_i8_r16");
+        }
         return CODY_OK;
       }
       launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, 0, c->d_Si8, c->d_eS, st);
@@ -469,17 +477,27 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   if (double* sr = plain_st(c)) launch_pair_src_rows(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, sr, st);
   int ps = 0;
   bool reginvDone = false;
-  CHK(enqueue_contraction(c, st, mark, &ps, &reginvDone));
+  // the int8 form's contraction also forms the orbital correlation of its points (with R formed
+  // beside its slicing); the other forms leave it to the Gram launch
+  const bool fused = int8_tiled(c);
+  const OrbCorrArgs oa{orbs, c->d_w, c->d_hphi, c->d_r2can, c->d_reginv, c->nG, mode, c->d_draw, c->d_part};
+  CHK(enqueue_contraction(c, st, mark, &ps, &reginvDone, fused ? &oa : nullptr));
   if (ps > 1 || !reginvDone) {  // (nothing to do when the contraction wrote W^T and R is formed)
     launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st,
                     !reginvDone);
     mark("k_rhs_wsum");
   }
-  launch_rhs_mid(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, orbs, c->d_w, c->d_hphi, c->d_r2can,
-                 c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st);
-  mark("k_rhs_mid");
+  if (fused) {
+    launch_rhs_gram(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, c->nO, c->d_part, c->nGp / 16, c->d_partr,
+                    st);
+    mark("k_rhs_gram");
+  } else {
+    launch_rhs_mid(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, orbs, c->d_w, c->d_hphi, c->d_r2can,
+                   c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st);
+    mark("k_rhs_mid");
+  }
   launch_rhs_post(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->d_part1, nPart1, c->d_h1, c->d_rho1, c->d_rho2, c->d_epart,
-                  orbs, c->d_draw, c->d_part, c->nPart, c->nG, c->nO, dorbs,
+                  orbs, c->d_draw, fused ? c->d_partr : c->d_part, fused ? 1 : c->nPart, c->nG, c->nO, dorbs,
                   mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st);
   mark("k_rhs_post");
   launch_ci_apply_project(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_epart,
@@ -1038,6 +1056,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.act = c->d_act;
       a.actC = c->nGp / 16 > kSkipMaxBlocks16 ? nullptr : act_flags(c);
       a.blist = c->d_blist;
+      a.partr = g.int8 ? c->d_partr : nullptr;
       a.Vt = c->d_Vt;
       a.Vi8 = c->d_Vi8;
       a.eV = c->d_eV;
@@ -1085,13 +1104,16 @@ int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
   launch_grp_front(g.nO, g.d_inst, map(kPlanFront), g.cnt[kPlanFront], g.maxC, st);
   if (g.int8) {
     launch_grp_slice(g.nO, g.d_inst, map(kPlanSlice), g.cnt[kPlanSlice], st);
-    CHK(launch_grp_mf_i8_r16(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
+    CHK(launch_grp_mf_i8_orb(g.nO, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   } else {
     CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   }
   launch_grp_wsum(g.nO, g.d_inst, map(kPlanWsum), g.cnt[kPlanWsum], st);
   launch_grp_rho2p(g.nO, g.d_inst, map(kPlanRho2p), g.cnt[kPlanRho2p], st);
-  launch_grp_mid(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
+  if (g.int8)
+    launch_grp_gram(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
+  else
+    launch_grp_mid(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
   launch_grp_post(g.nO, g.d_inst, map(kPlanPost), g.cnt[kPlanPost], st);
   launch_grp_apply(g.d_inst, map(kPlanApply), g.cnt[kPlanApply], g.nO, st);
   HIPCHK(hipGetLastError());

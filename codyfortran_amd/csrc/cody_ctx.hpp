@@ -161,6 +161,7 @@ struct cody_ctx {
   double2* d_r2can = nullptr;  // canonical rho2Lin (i1 < i2, j1 < j2)
   double2 *d_h1 = nullptr, *d_hphi = nullptr, *d_dCraw = nullptr;
   double2 *d_draw = nullptr, *d_part = nullptr, *d_part1 = nullptr;
+  double2* d_partr = nullptr;  // the overlap partials summed (the int8 path's projection reads these)
   double* d_zeros = nullptr;  // nG + 1 zeros: a zero potential and a zero field (the kinetic piece)
   double2* d_src = nullptr;   // interaction sources of cody_interaction_potential (nG x P, on use)
   double2* d_h2 = nullptr;    // full h2 tensor of cody_apply_correlation (nO^4, on use)
@@ -213,7 +214,10 @@ struct cody_ctx {
     ALLOC(d_hphi, sizeof(double2) * nG * nO);
     ALLOC(d_dCraw, sizeof(double2) * nC);
     ALLOC(d_draw, sizeof(double2) * nG * nO);
-    ALLOC(d_part, sizeof(double2) * nPart * n2);
+    // overlap partials: one per 64-point workgroup of the orbital kernel, or per 16-point tile of the
+    // fused int8 contraction
+    ALLOC(d_part, sizeof(double2) * std::max(nPart, nGp / 16) * n2);
+    ALLOC(d_partr, sizeof(double2) * n2);
     ALLOC(d_part1, sizeof(double2) * ((nG + 255) / 256) * n2);
     ALLOC(d_zeros, sizeof(double) * (nG + 1));
     // CI tables in one blob: ptr arrays then entry arrays
@@ -277,7 +281,8 @@ struct cody_ctx {
     drop_graphs();
     void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_smax, d_act, d_blist,
                     d_Gpart, d_G,  d_energy, d_epart, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
-                    d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, d_zeros, d_src, d_h2, ci_blob};
+                    d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, d_zeros, d_src, d_h2, ci_blob,
+                    d_partr};
     for (void* p : ptrs)
       if (p) hipFree(p);
     prop.release();

@@ -788,6 +788,33 @@ __global__ __launch_bounds__(256) void k_rhs_mid(const double* __restrict__ STt,
   orbital_body<NO>(b % nOrbX, b / nOrbX, orbs, w, hphi, WT, r2can, reginv, nG, nGp, mode, draw, part);
 }
 
+// After the fused int8 contraction (which formed the orbital correlation): the Gram matrix's split-K
+// partials (the first NT nxs workgroups) and, on the last workgroup, the fixed-order sum of the
+// contraction's nPart per-tile overlap partials into partr.
+template <int NO>
+__device__ __forceinline__ void gram_sum_body(int b, const double* __restrict__ STt, const double* __restrict__ WT,
+                                              int nGp, int Pc, double* __restrict__ Gpart, int nxs,
+                                              const double2* __restrict__ part, int nPart,
+                                              double2* __restrict__ partr) {
+  constexpr int NT = (NO * (NO + 1) + 15) / 16, n2 = NO * NO, G = kOverlapGroups<NO>;
+  if (b < NT * nxs) {
+    gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc);
+    return;
+  }
+  __shared__ double2 sG[G][n2];
+  __shared__ double2 sInp[n2];
+  sum_overlap_parts<NO>(part, nPart, sG, sInp);
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) partr[i] = sInp[i];
+}
+
+template <int NO>
+__global__ __launch_bounds__(256) void k_rhs_gram(const double* __restrict__ STt, const double* __restrict__ WT,
+                                                  int nGp, int Pc, double* __restrict__ Gpart, int nxs,
+                                                  const double2* __restrict__ part, int nPart,
+                                                  double2* __restrict__ partr) {
+  gram_sum_body<NO>(blockIdx.x, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, partr);
+}
+
 // Then: G and h1 from their partials with the energy partials (the first nRed workgroups), and the
 // orbital projection with the right-hand side's factor -i (the others).
 template <int NO>
@@ -895,6 +922,13 @@ __global__ __launch_bounds__(256) void k_grp_mid(const GroupInstance* __restrict
 }
 
 template <int NO>
+__global__ __launch_bounds__(256) void k_grp_gram(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  gram_sum_body<NO>(m.y, a.STt, a.WT, a.nGp, a.Pc, a.Gpart, a.nxs, a.part, a.nGp / 16, a.partr);
+}
+
+template <int NO>
 __global__ __launch_bounds__(256) void k_grp_post(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
@@ -902,8 +936,9 @@ __global__ __launch_bounds__(256) void k_grp_post(const GroupInstance* __restric
     gram_reduce_energy_body(m.y, a.Gpart, a.nxs, a.Pc, a.G, a.part1, a.nPart1, NO, a.h1, a.rho1, a.rho2, a.epart);
     return;
   }
-  project_body<NO>(m.y - a.nRed, false, a.C + a.nC, a.draw, a.part, a.nPart, a.nG, 1, a.dC + a.nC, nullptr, nullptr, 0,
-                   nullptr, nullptr);
+  // (the int8 path's projection reads the overlap partials summed after the fused contraction)
+  project_body<NO>(m.y - a.nRed, false, a.C + a.nC, a.draw, a.partr ? a.partr : a.part, a.partr ? 1 : a.nPart, a.nG,
+                   1, a.dC + a.nC, nullptr, nullptr, 0, nullptr, nullptr);
 }
 
 void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlocks, int maxC, hipStream_t st) {
@@ -933,6 +968,11 @@ void launch_grp_mid(int nO, const GroupInstance* gi, const int2* map, int nBlock
 #define CALL_GM(NO) hipLaunchKernelGGL(k_grp_mid<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
   CODY_NO_SWITCH(nO, CALL_GM)
 #undef CALL_GM
+}
+void launch_grp_gram(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GG(NO) hipLaunchKernelGGL(k_grp_gram<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+  CODY_NO_SWITCH(nO, CALL_GG)
+#undef CALL_GG
 }
 void launch_grp_post(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
 #define CALL_GP(NO) hipLaunchKernelGGL(k_grp_post<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
@@ -988,6 +1028,16 @@ void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double
                      reginv, nG, mode, draw, part)
   CODY_NO_SWITCH(nO, CALL_MI)
 #undef CALL_MI
+}
+
+void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs, int nO,
+                     const double2* part, int nPart, double2* partr, hipStream_t st) {
+  const unsigned grid = (unsigned)(Pc / 16 * nxs + 1);
+#define CALL_RG(NO)                                                                                           \
+  hipLaunchKernelGGL(k_rhs_gram<NO>, dim3(grid), dim3(256), 0, st, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, \
+                     partr)
+  CODY_NO_SWITCH(nO, CALL_RG)
+#undef CALL_RG
 }
 
 int rhs_post_energy_parts(int Pc, int nO) { return (Pc * Pc + nO * nO + 255) / 256; }

@@ -76,6 +76,7 @@ struct GroupInstance {
   int* act;                  // pair-source activity flags (written by the front launch)
   const int* actC;           // the flags the contraction reads (null: no skipping)
   int* blist;                // int8 path: the active-block list built from actC (when set)
+  double2* partr;            // int8 path: the summed overlap partials (the projection reads these)
   double2* r2p;              // rho2' (nO^4) for the orbital correlation, or null (formed in it)
   const double* Vt;                          // FP64 contraction operand
   const int8_t* Vi8;                         // int8 contraction operands
@@ -194,6 +195,25 @@ void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp
                      hipStream_t st);
 constexpr int kR16MaxBlocks64 = 256;  // longest grid (in 64-point blocks) of the 16-row int8 contraction
 constexpr int kR16Prefetch = 2;       // k blocks of operands in flight per wave of the 16-row int8 contraction
+// Orbital-correlation operands of the fused int8 contraction (launch_mf_i8_orb)
+struct OrbCorrArgs {
+  const double2* orbs;  // phi, nO x nG
+  const double* w;      // FEDVR weights
+  const double2* hphi;  // (T + V) phi
+  const double2* r2can;   // canonical rho2 (a < b, m < l), np x np
+  const double2* reginv;  // R = rho1 (rho1^2 + eps)^-1
+  int nG, mode;
+  double2* draw;        // output: h phi + correlation, nO x nG
+  double2* part;        // output: overlap partials, one nO^2 block per 16-point tile
+};
+int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int nO,
+                     unsigned long long* smax, const int* list, const OrbCorrArgs& o, hipStream_t st);
+// after launch_mf_i8_orb: the Gram partials (as launch_rhs_mid) and the fixed-order sum of the nPart
+// overlap partials into partr (nO^2) on one more workgroup
+void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs, int nO,
+                     const double2* part, int nPart, double2* partr, hipStream_t st);
+int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+void launch_grp_gram(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int Pc,
                      unsigned long long* smax, const int* list, hipStream_t st);
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,

@@ -19,6 +19,7 @@
 #include <cstdio>
 
 #include "../../include/cody_mctdhx.h"
+#include "fedvr_device.hpp"
 #include "gemm_device.hpp"
 
 namespace cody {
@@ -206,19 +207,19 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, 
 // exponents and W^T is written directly: no k-split partials, no summing launch, no cross-wave
 // reduction.  (Measured at 4096 points with 18 of 64 blocks active: two blocks in flight beat three
 // or four, and sharing the V fragments through LDS with a barrier per block is slower.)
-// The body takes the workgroup's row tile rt.
+// mf_i8_r16_acc: the k loop of the workgroup's row tile rt into this wave's five int32 diagonal
+// tiles; mf_i8_r16_w: their FP64 recombination, rows 4(l/16) + u of column l%16 of the wave's tile.
 template <int NT, int PF>
-__device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
-                                               const int8_t* __restrict__ Bs, const int* __restrict__ fB, int nGp,
-                                               double* __restrict__ WT, unsigned long long* __restrict__ smax,
-                                               const int* __restrict__ list) {
+__device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__ As, const int8_t* __restrict__ Bs,
+                                              int nGp, unsigned long long* __restrict__ smax,
+                                              const int* __restrict__ list, i4_t (&acc)[kInt8Slices]) {
   constexpr int NS = kInt8Slices;
   __shared__ int sList[kR16MaxBlocks64];
   // the per-line maxima of S^T were consumed by the slicing (earlier on this stream): clear them
   // for the next right-hand side
   if (smax && rt == 0 && threadIdx.x < NT * 16) smax[threadIdx.x] = 0ull;
   const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  const int lane = threadIdx.x & 63;
   const int nK64 = nGp >> 6;
   const int total = list ? __builtin_amdgcn_readfirstlane(list[0]) : nK64;
   if (list) {
@@ -227,7 +228,6 @@ __device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict_
   }
   const int8_t* ab = As + (size_t)rt * nK64 * NS * 1024 + 16 * lane;
   const int8_t* bb = Bs + (size_t)t * nK64 * NS * 1024 + 16 * lane;
-  i4_t acc[NS];
 #pragma unroll
   for (int d = 0; d < NS; ++d) acc[d] = (i4_t){0, 0, 0, 0};
   i4_t a[PF][NS], b[PF][NS];
@@ -258,17 +258,212 @@ __device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict_
       }
     }
   }
-  // 32-bit MFMA results: element u of lane l is row 4(l/16) + u, column l%16
+}
+
+template <int NT>
+__device__ __forceinline__ void mf_i8_r16_w(int rt, const i4_t (&acc)[kInt8Slices], const int* __restrict__ eA,
+                                            const int* __restrict__ fB, double (&v)[4]) {
+  constexpr int NS = kInt8Slices;
+  const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
   const int row0 = rt * 16 + 4 * q;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) v[u] = 0.0;
 #pragma unroll
   for (int d = NS - 1; d >= 0; --d)
 #pragma unroll
     for (int u = 0; u < 4; ++u) v[u] += static_cast<double>(acc[d][u]) * kSliceScale.v[d + 2];
   const int fc = fB[t * 16 + rr];
-  double* dst = WT + (size_t)(t * 16 + rr) * nGp + row0;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) dst[u] = ldexp(v[u], eA[row0 + u] + fc);
+  for (int u = 0; u < 4; ++u) v[u] = ldexp(v[u], eA[row0 + u] + fc);
+}
+
+// The body takes the workgroup's row tile rt.
+template <int NT, int PF>
+__device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
+                                               const int8_t* __restrict__ Bs, const int* __restrict__ fB, int nGp,
+                                               double* __restrict__ WT, unsigned long long* __restrict__ smax,
+                                               const int* __restrict__ list) {
+  i4_t acc[kInt8Slices];
+  mf_i8_r16_acc<NT, PF>(rt, As, Bs, nGp, smax, list, acc);
+  double v[4];
+  mf_i8_r16_w<NT>(rt, acc, eA, fB, v);
+  const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double* dst = WT + (size_t)(t * 16 + (lane & 15)) * nGp + rt * 16 + 4 * (lane >> 4);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) dst[u] = v[u];
+}
+
+// ------------------------------------------------------------------ int8 contraction + orbital correlation
+// The 16-row int8 contraction with the orbital correlation of its 16 points in the epilogue
+// (ApplyCorrelationOp, S_Method_Mb_OrbBased.f90:647-872): the workgroup holds W(x) for all pairs of
+// its points once the k loop ends, so the correlation needs no pass of its own over W^T.  With
+// rho2' = rho2 x_3 R, rho2 antisymmetric in its first index pair (a, b),
+//   corr_j(x) = sum_{k1,k,l} phi_k1(x) M_kl(x) rho2'(k1, k, j, l) = sum_m R(m, j) c_m(x),
+//   c_m(x)    = sum_{a<b, l} rho2(a, b, m, l) B_{ab,l}(x),   B_{ab,l} = phi_a M_bl - phi_b M_al,
+// where M_kl = W_p (pair p = {l <= k}) and M_kl = conj(W_p) for k < l.  The B terms (half the
+// products of the plain sum) are formed per (point, l) from registers; c = B^T rho2 is a 16-point x
+// (2 NO real) x (2 np NO real) product, run on FP64 MFMA 16x16x4 with the complex values split into
+// real and imaginary parts along k and along the columns, its k steps shared out over the waves and
+// the wave tiles added in LDS in fixed order.  rho2(a, b, m, l) = sign(m, l) rho2can[pair(a, b),
+// pair(m, l)] is a signed copy of the canonical rho2, so nothing waits for R but the last 8 products.
+// Then draw_j = h phi_j + corr_j (h phi only in the modes whose right-hand side has the one-body
+// part) and this tile's overlap partial inp(i, j) = sum_x conj(phi_i) w d_j over its points, at
+// part[rt NO^2 + i + NO j] (the partials are summed in fixed order by a later launch).
+template <int NO, int PF>
+__device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
+                                               const int8_t* __restrict__ Bs, const int* __restrict__ fB, int nGp,
+                                               double* __restrict__ WT, unsigned long long* __restrict__ smax,
+                                               const int* __restrict__ list, const OrbCorrArgs& o) {
+  constexpr int NT = (NO * (NO + 1) + 15) / 16, NTH = NT * 64, n2 = NO * NO, np = NO * (NO - 1) / 2;
+  constexpr int NE = np * NO;                    // B terms per point (a < b, l)
+  constexpr int NEP = NE > 0 ? NE : 1;
+  constexpr int NR = NE * NO;                    // rho2(a < b, m, l) entries
+  constexpr int KS = (2 * NE + 3) / 4;           // FP64 MFMA k steps (4 real k each)
+  constexpr int RPT = NR > 0 ? (NR + NTH - 1) / NTH : 1;  // rho2 entries staged per thread
+  __shared__ double sW[NT * 16][17];
+  __shared__ double sBre[NEP][16], sBim[NEP][16];  // B_{e}(x), e = pair(a, b) + np l
+  __shared__ double2 sRho[NR > 0 ? NR : 1];         // rho2(a < b, m, l) at e NO + m
+  __shared__ double sC[NT][16][17];                 // the waves' c tiles: row x, column 2m + re/im
+  __shared__ double2 sPhi[NO][16], sD[NO][16], sRm[n2];
+  __shared__ double sw[16];
+  __shared__ int sAB[np > 0 ? np : 1];              // a + 16 b of strict pair index pab
+  const int tid = threadIdx.x, x0 = rt * 16;
+  // epilogue operands, loaded ahead of the k loop (their latency hides behind it)
+  double2 rR[RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    const int e = tid + NTH * u;  // e = (pab + np l) NO + m
+    double2 r = cmk(0, 0);
+    if (e < NR) {
+      const int m = e % NO, ee = e / NO, pab = ee % np, l = ee / np;
+      if (m != l) {
+        r = o.r2can[pab + np * (m < l ? strict_pair(NO, m, l) : strict_pair(NO, l, m))];
+        if (m > l) r = cmk(-r.x, -r.y);
+      }
+    }
+    rR[u] = r;
+  }
+  const bool own = tid < NO * 16;  // (point tid % 16, orbital tid / 16)
+  const int px = tid & 15, pj = tid >> 4, gx = x0 + px;
+  const bool in = own && gx < o.nG;
+  const double2 phiL = in ? o.orbs[(size_t)pj * o.nG + gx] : cmk(0, 0);
+  const double2 hL = (in && (o.mode == kModeFull || o.mode == kModeNonLinConsistent))
+                         ? o.hphi[(size_t)pj * o.nG + gx] : cmk(0, 0);
+  const double wL = (tid < 16 && x0 + tid < o.nG) ? o.w[x0 + tid] : 0.0;
+  const double2 rmL = tid < n2 ? o.reginv[tid] : cmk(0, 0);
+  if (tid < np) {
+    int a = 0, r = tid;
+    while (r >= NO - 1 - a) {
+      r -= NO - 1 - a;
+      ++a;
+    }
+    sAB[tid] = a + 16 * (a + 1 + r);
+  }
+  i4_t acc[kInt8Slices];
+  mf_i8_r16_acc<NT, PF>(rt, As, Bs, nGp, smax, list, acc);
+  double v[4];
+  mf_i8_r16_w<NT>(rt, acc, eA, fB, v);
+  const int t = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, q = lane >> 4, rr = lane & 15;
+  double* dst = WT + (size_t)(t * 16 + rr) * nGp + x0 + 4 * q;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    dst[u] = v[u];
+    sW[t * 16 + rr][4 * q + u] = v[u];
+  }
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    const int e = tid + NTH * u;
+    if (e < NR) sRho[e] = rR[u];
+  }
+  if (own) {
+    sPhi[pj][px] = phiL;
+    sD[pj][px] = hL;
+  }
+  if (tid < 16) sw[tid] = wL;
+  if (tid < n2) sRm[tid] = rmL;
+  __syncthreads();
+  // B terms: thread (x, l) from phi(x) and the column l of M(x), both in registers
+  if constexpr (np > 0) {
+    if (own) {
+      const int x = px, l = pj;
+      double2 ph[NO], ml[NO];
+#pragma unroll
+      for (int k = 0; k < NO; ++k) {
+        ph[k] = sPhi[k][x];
+        const int p = l <= k ? pair_index(NO, l, k) : pair_index(NO, k, l);
+        const double2 wv = cmk(sW[2 * p][x], sW[2 * p + 1][x]);
+        ml[k] = l <= k ? wv : cconj(wv);
+      }
+#pragma unroll
+      for (int a = 0; a < NO; ++a)
+#pragma unroll
+        for (int b = a + 1; b < NO; ++b) {
+          const double2 bv = csub(cmul(ph[a], ml[b]), cmul(ph[b], ml[a]));
+          const int e = strict_pair(NO, a, b) + np * l;
+          sBre[e][x] = bv.x;
+          sBim[e][x] = bv.y;
+        }
+    }
+  }
+  __syncthreads();
+  // c = B^T rho2 on FP64 MFMA: step s takes real k = 4 s + lane / 16 (B term k / 2, part k % 2);
+  // lane l supplies A[x = l % 16][k] and rho2's real embedding at [k][n = l % 16], n = 2 m + part
+  {
+    d4_t cacc = (d4_t){0.0, 0.0, 0.0, 0.0};
+    const int n = rr, m = n >> 1, cn = n & 1;
+    for (int s = t; s < KS; s += NT) {
+      const int k = 4 * s + q, e = k >> 1, ck = k & 1;
+      double av = 0.0, bv = 0.0;
+      if (e < NE) {
+        av = ck ? sBim[e][rr] : sBre[e][rr];
+        if (m < NO) {
+          const double2 r = sRho[e * NO + m];
+          // real embedding: [re b, im b] x [[re r, im r], [-im r, re r]]
+          bv = ck ? (cn ? r.x : -r.y) : (cn ? r.y : r.x);
+        }
+      }
+      cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, cacc, 0, 0, 0);
+    }
+    // FP64 result layout: element i of lane l is row 4 i + l / 16, column l % 16
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sC[t][4 * i + q][rr] = cacc[i];
+  }
+  __syncthreads();
+  if (own) {
+    const int x = px, j = pj;
+    double2 corr = cmk(0, 0);
+#pragma unroll
+    for (int mm = 0; mm < NO; ++mm) {
+      double cr = sC[0][x][2 * mm], ci = sC[0][x][2 * mm + 1];
+      for (int w = 1; w < NT; ++w) {
+        cr += sC[w][x][2 * mm];
+        ci += sC[w][x][2 * mm + 1];
+      }
+      cfma(corr, sRm[mm + j * NO], cmk(cr, ci));
+    }
+    const double2 d = cadd(sD[j][x], corr);
+    sD[j][x] = d;
+    if (in) o.draw[(size_t)j * o.nG + gx] = d;
+  }
+  __syncthreads();
+  for (int e = tid; e < n2; e += NTH) {
+    const int i = e % NO, j = e / NO;
+    double2 s = cmk(0, 0);
+#pragma unroll
+    for (int x = 0; x < 16; ++x) cfmac(s, sPhi[i][x], cscale(sD[j][x], sw[x]));
+    o.part[(size_t)rt * n2 + e] = s;
+  }
+}
+
+template <int NO>
+__global__ __launch_bounds__(((NO * (NO + 1) + 15) / 16) * 64) void k_mf_i8_orb(
+    const int8_t* __restrict__ As, const int* __restrict__ eA, const int8_t* __restrict__ Bs,
+    const int* __restrict__ fB, int nGp, double* __restrict__ WT, unsigned long long* __restrict__ smax,
+    const int* __restrict__ list, OrbCorrArgs o) {
+  mf_i8_orb_body<NO, kR16Prefetch>(blockIdx.x, As, eA, Bs, fB, nGp, WT, smax, list, o);
 }
 
 template <int NT>
@@ -586,6 +781,33 @@ int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nB
   return 0;
 }
 
+template <int NO>
+__global__ __launch_bounds__(((NO * (NO + 1) + 15) / 16) * 64) void k_grp_mf_i8_orb(const GroupInstance* __restrict__ gi,
+                                                                                    const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  const OrbCorrArgs o{a.C + a.nC, a.w, a.hphi, a.r2can, a.reginv, a.nG, kModeFull, a.draw, a.part};
+  mf_i8_orb_body<NO, kR16Prefetch>(m.y, a.Vi8, a.eV, a.Si8, a.eS, a.nGp, a.WT, a.smax, a.actC ? a.blist : nullptr, o);
+}
+
+int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GO(NO) \
+  hipLaunchKernelGGL(k_grp_mf_i8_orb<NO>, dim3(nBlocks), dim3(((NO * (NO + 1) + 15) / 16) * 64), 0, st, gi, map)
+  switch (nO) {
+    case 1: CALL_GO(1); break;
+    case 2: CALL_GO(2); break;
+    case 3: CALL_GO(3); break;
+    case 4: CALL_GO(4); break;
+    case 5: CALL_GO(5); break;
+    case 6: CALL_GO(6); break;
+    case 7: CALL_GO(7); break;
+    case 8: CALL_GO(8); break;
+    default: return set_error(CODY_ERR_ARG, "k_grp_mf_i8_orb: nO outside 1..8");
+  }
+#undef CALL_GO
+  return 0;
+}
+
 int launch_grp_mf_i8_r16(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
 #define CALL_GI(NT) hipLaunchKernelGGL(k_grp_mf_i8_r16<NT>, dim3(nBlocks), dim3((NT) * 64), 0, st, gi, map)
   switch (Pc / 16) {
@@ -609,6 +831,31 @@ void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp
                      Pc, kq, (size_t)Pc * Pc)
   CODY_NT_SWITCH2(Pc, CALL_G)
 #undef CALL_G
+}
+
+// The 16-row int8 contraction with the orbital correlation of each tile (nO <= 8, nGp / 64 <= 256):
+// W^T, draw and one overlap partial per 16-point tile (part: nGp / 16 blocks of nO^2).  Returns 0,
+// or CODY_ERR_ARG without launching anything outside these limits.
+int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int nO,
+                     unsigned long long* smax, const int* list, const OrbCorrArgs& o, hipStream_t st) {
+  if (nGp < 64 || nGp % 64 || nGp / 64 > kR16MaxBlocks64 || nO < 1 || nO > 8 || o.nG > nGp)
+    return set_error(CODY_ERR_ARG, "k_mf_i8_orb: nGp must be a positive multiple of 64, at most 64 x 256, above "
+                                   "nG, and nO in 1..8");
+#define CALL_OC(NO)                                                                                        \
+  hipLaunchKernelGGL(k_mf_i8_orb<NO>, dim3(nGp / 16), dim3(((NO * (NO + 1) + 15) / 16) * 64), 0, st, As, eA, \
+                     Bs, fB, nGp, WT, smax, list, o)
+  switch (nO) {
+    case 1: CALL_OC(1); break;
+    case 2: CALL_OC(2); break;
+    case 3: CALL_OC(3); break;
+    case 4: CALL_OC(4); break;
+    case 5: CALL_OC(5); break;
+    case 6: CALL_OC(6); break;
+    case 7: CALL_OC(7); break;
+    case 8: CALL_OC(8); break;
+  }
+#undef CALL_OC
+  return 0;
 }
 
 // 16-row int8 contraction of the right-hand side (Pc <= 80, nGp / 64 <= 256): W^T

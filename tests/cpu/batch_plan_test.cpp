@@ -66,6 +66,8 @@ int main() {
     EXPECT(c[kPlanSlice] == 2 + 320, "int8 slicing blocks %lld", (long long)c[kPlanSlice]);
     EXPECT(c[kPlanContract] == 256, "int8 contraction blocks %lld", (long long)c[kPlanContract]);
     EXPECT(c[kPlanWsum] == 0, "int8 sum blocks %lld", (long long)c[kPlanWsum]);
+    // the correlation is formed by the contraction: 5 x 32 Gram + one partial-sum workgroup
+    EXPECT(c[kPlanMid] == 160 + 1, "int8 Gram blocks %lld", (long long)c[kPlanMid]);
     // 12 orbitals: rho2' (12^4 entries, one per thread) is formed once per right-hand side
     plan_block_counts(shape(12, 924, 4096, 4, false), c);
     EXPECT(c[kPlanRho2p] == 81, "rho2' blocks %lld", (long long)c[kPlanRho2p]);
