@@ -46,7 +46,8 @@ void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]) {
   const int64_t n2 = (int64_t)s.nO * s.nO, np = (int64_t)s.nO * (s.nO - 1) / 2;
   const int64_t plane = (int64_t)s.Pc * s.nGp;
   const int64_t nObx = (s.nG + 255) / 256;
-  const int64_t nRdm = (n2 + np * np + 3) / 4;
+  const int64_t per = 4 * (s.nC > kPlanRdmWideConf ? kPlanRdmWideItems : 1);
+  const int64_t nRdm = (n2 + np * np + per - 1) / per;
   const int64_t nRed = ((int64_t)s.Pc * s.Pc + n2 + 255) / 256;
   const int64_t wsumFirst = s.int8 ? 0 : 1;  // R beside the int8 slicing, else in the k-split-sum launch
   cnt[kPlanFront] = nRdm + nObx * s.nO + plane / 256;

@@ -22,6 +22,9 @@ constexpr int kPlanMaxConf = 1024;         // CI configurations per instance
 constexpr int kPlanInt8MaxTiles = 5;       // 16-column tiles of the int8 contraction
 constexpr int kPlanInt8MaxBlocks64 = 256;  // 64-point k blocks of the int8 contraction (one range)
 constexpr int kPlanF64MaxTiles = 10;       // 16-column tiles of the FP64 contraction
+// CI density role: entries per wave (the device's ci_rdm_items_per_wave): 1, or 4 above 256
+// configurations; a workgroup holds four waves
+constexpr int kPlanRdmWideConf = 256, kPlanRdmWideItems = 4;
 
 // The kernel types of a right-hand side, in launch order.
 enum PlanKind { kPlanFront, kPlanSlice, kPlanContract, kPlanWsum, kPlanRho2p, kPlanMid, kPlanPost, kPlanApply, kPlanKinds };

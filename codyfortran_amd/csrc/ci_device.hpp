@@ -17,8 +17,8 @@ __device__ __forceinline__ void ci_rdm_body(int bx, const double2* __restrict__ 
                                                 const CiEntryDev* __restrict__ r2,
                                                 double2* __restrict__ rho1, double2* __restrict__ r2can,
                                                 double2* __restrict__ rho2) {
-  const int n2 = nO * nO, np = nO * (nO - 1) / 2, nCan = np * np;
-  const int item = bx * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63;
+  const int n2 = nO * nO, np = nO * (nO - 1) / 2, nCan = np * np, ipw = ci_rdm_items_per_wave(nC);
+  const int ln = threadIdx.x & 63;
   extern __shared__ double2 sC[];  // C staged once per workgroup, four loads per thread in flight
   for (int i0 = threadIdx.x; i0 < nC; i0 += 4 * blockDim.x) {
     double2 v[4];
@@ -34,47 +34,51 @@ __device__ __forceinline__ void ci_rdm_body(int bx, const double2* __restrict__ 
     }
   }
   __syncthreads();
-  if (item >= n2 + nCan) return;
-  int code, i1 = 0, i2 = 0, j1 = 0, j2 = 0;
-  const int* ptr;
-  const CiEntryDev* tab;
-  if (item < n2) {
-    code = (item % nO) * nO + item / nO;
-    ptr = r1_ptr;
-    tab = r1;
-  } else {
-    const int e = item - n2;
-    strict_pair_decode(nO, e % np, i1, i2);
-    strict_pair_decode(nO, e / np, j1, j2);
-    code = ((i1 * nO + j1) * nO + i2) * nO + j2;
-    ptr = r2_ptr;
-    tab = r2;
-  }
-  double2 acc = cmk(0, 0);
-  const int q1 = ptr[code + 1];
-  for (int q0 = ptr[code] + ln; q0 < q1; q0 += 4 * 64) {  // four table entries' loads in flight
-    CiEntryDev en[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = q0 + 64 * u;
-      en[u] = q < q1 ? tab[q] : CiEntryDev{0, 0, 0, 0.0f};
+  for (int it = 0; it < ipw; ++it) {
+    const int item = (bx * 4 + (threadIdx.x >> 6)) * ipw + it;
+    if (item >= n2 + nCan) return;
+    int code, i1 = 0, i2 = 0, j1 = 0, j2 = 0;
+    const int* ptr;
+    const CiEntryDev* tab;
+    if (item < n2) {
+      code = (item % nO) * nO + item / nO;
+      ptr = r1_ptr;
+      tab = r1;
+    } else {
+      const int e = item - n2;
+      strict_pair_decode(nO, e % np, i1, i2);
+      strict_pair_decode(nO, e / np, j1, j2);
+      code = ((i1 * nO + j1) * nO + i2) * nO + j2;
+      ptr = r2_ptr;
+      tab = r2;
     }
+    double2 acc = cmk(0, 0);
+    const int q1 = ptr[code + 1];
+    for (int q0 = ptr[code] + ln; q0 < q1; q0 += 4 * 64) {  // four table entries' loads in flight
+      CiEntryDev en[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (q0 + 64 * u < q1) cfma(acc, cscale(cconj(sC[en[u].jc]), en[u].f), sC[en[u].ic]);
-  }
-  acc.x = wave_sum(acc.x);
-  acc.y = wave_sum(acc.y);
-  if (ln != 0) return;
-  if (item < n2) {
-    rho1[item] = acc;
-  } else {
-    r2can[item - n2] = acc;
-    const double2 m = cmk(-acc.x, -acc.y);
-    rho2[idx4(i1, i2, j1, j2, nO)] = acc;
-    rho2[idx4(i2, i1, j1, j2, nO)] = m;
-    rho2[idx4(i1, i2, j2, j1, nO)] = m;
-    rho2[idx4(i2, i1, j2, j1, nO)] = acc;
+      for (int u = 0; u < 4; ++u) {
+        const int q = q0 + 64 * u;
+        en[u] = q < q1 ? tab[q] : CiEntryDev{0, 0, 0, 0.0f};
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q0 + 64 * u < q1) cfma(acc, cscale(cconj(sC[en[u].jc]), en[u].f), sC[en[u].ic]);
+    }
+    acc.x = wave_sum(acc.x);
+    acc.y = wave_sum(acc.y);
+    if (ln == 0) {
+      if (item < n2) {
+        rho1[item] = acc;
+      } else {
+        r2can[item - n2] = acc;
+        const double2 m = cmk(-acc.x, -acc.y);
+        rho2[idx4(i1, i2, j1, j2, nO)] = acc;
+        rho2[idx4(i2, i1, j1, j2, nO)] = m;
+        rho2[idx4(i1, i2, j2, j1, nO)] = m;
+        rho2[idx4(i2, i1, j2, j1, nO)] = acc;
+      }
+    }
   }
 }
 

@@ -189,7 +189,8 @@ int ci_kernels_allow_lds(size_t bytes) {
 void launch_ci_density(const double2* C, int nC, int nO, double reg, const DevCiTables& t,
                        double2* rho1, double2* rho2, double2* r2can, double2* reginv, hipStream_t st) {
   const int n2 = nO * nO, np = nO * (nO - 1) / 2, items = n2 + np * np;
-  hipLaunchKernelGGL(k_ci_rdm, dim3((items + 3) / 4), dim3(256), sizeof(double2) * nC, st, C, nC, nO, t.r1_ptr,
+  const int per = 4 * ci_rdm_items_per_wave(nC);
+  hipLaunchKernelGGL(k_ci_rdm, dim3((items + per - 1) / per), dim3(256), sizeof(double2) * nC, st, C, nC, nO, t.r1_ptr,
                      t.r1, t.r2_ptr, t.r2, rho1, r2can, rho2);
 #define CALL_RI(NO) hipLaunchKernelGGL(k_ci_reginv<NO>, dim3(1), dim3(64), 0, st, rho1, reg, reginv)
   switch (nO) {

@@ -970,6 +970,7 @@ int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* nam
 }
 
 // ---------------------------------------------------------------- batches of instances
+static_assert(kPlanRdmWideConf == kRdmWideConf && kPlanRdmWideItems == kRdmWideItems, "CI density blocks agree");
 static_assert(kPlanMaxOrbs == kMaxOrbs && kPlanInt8MaxBlocks64 == kR16MaxBlocks64 && kPlanRho2pMinOrbs == kRho2pMinOrbs,
               "the host planner and the device kernels agree on their limits");
 static_assert(sizeof(BlockRef) == sizeof(int2), "block references are uploaded as int2");
@@ -1007,6 +1008,47 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
     }
     grp->members.push_back(i);
   }
+  // a group whose estimated cost exceeds a chain's share of the batch is split into two groups of
+  // about equal cost (members longest first, each to the lighter half), so that its launches run on
+  // two chains side by side instead of setting the batch's critical path alone
+  {
+    double total = 0.0;
+    for (const auto& g : b->groups)
+      for (int i : g.members) total += rhs_cost_estimate(b->ctx[i]);
+    for (int i : b->singles) total += rhs_cost_estimate(b->ctx[i]);
+    const double share = total / cody_batch::kChains;
+    for (size_t gi0 = 0; gi0 < b->groups.size() && b->groups.size() < 2 * cody_batch::kChains; ++gi0) {
+      BatchGroup& g = b->groups[gi0];
+      double cost = 0.0;
+      for (int i : g.members) cost += rhs_cost_estimate(b->ctx[i]);
+      if (cost <= share || g.members.size() < 2) continue;
+      std::vector<int> mem = g.members;
+      std::stable_sort(mem.begin(), mem.end(), [&](int a, int c) {
+        return rhs_cost_estimate(b->ctx[a]) > rhs_cost_estimate(b->ctx[c]);
+      });
+      BatchGroup h;
+      h.nO = g.nO;
+      h.Pc = g.Pc;
+      h.int8 = g.int8;
+      std::vector<int> keep;
+      double ck = 0.0, ch = 0.0;
+      for (int i : mem) {
+        const double ci = rhs_cost_estimate(b->ctx[i]);
+        if (ck <= ch) {
+          keep.push_back(i);
+          ck += ci;
+        } else {
+          h.members.push_back(i);
+          ch += ci;
+        }
+      }
+      std::sort(keep.begin(), keep.end());
+      std::sort(h.members.begin(), h.members.end());
+      g.members = keep;
+      b->groups.push_back(h);
+      --gi0;  // (look at the kept half again)
+    }
+  }
   for (auto& g : b->groups) {
     std::vector<GroupInstance> gi;
     std::vector<PlanShape> shapes;
@@ -1021,7 +1063,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.nG = c->nG;
       a.nGp = c->nGp;
       a.Pc = c->Pc;
-      a.nRdm = (n2 + np * np + 3) / 4;
+      a.nRdm = (n2 + np * np + 4 * ci_rdm_items_per_wave(c->nC) - 1) / (4 * ci_rdm_items_per_wave(c->nC));
       a.nObx = (c->nG + 255) / 256;
       a.nsplit = g.int8 ? 1 : c->wsplit;  // (the int8 contraction writes W^T itself)
       a.wsumFirst = g.int8 ? 0 : 1;  // (the int8 path forms R beside the slicing)

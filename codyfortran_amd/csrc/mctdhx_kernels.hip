@@ -984,7 +984,8 @@ void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, do
                       double2* rho2, const double2* orbs, const double* w, const double* x, const double* vext,
                       const KinLaunch& kl, const double* efield, int nG, double2* hphi, double2* part1, int nGp,
                       int Pc, double* STt, unsigned long long* smax, int* act, hipStream_t st) {
-  const int n2 = nO * nO, np = nO * (nO - 1) / 2, nRdm = (n2 + np * np + 3) / 4;
+  const int n2 = nO * nO, np = nO * (nO - 1) / 2, per = 4 * ci_rdm_items_per_wave(nC);
+  const int nRdm = (n2 + np * np + per - 1) / per;
   const int nObx = (nG + 255) / 256, nPair = (int)((size_t)Pc * nGp / 256);
   KinParams kp{kl.kref, kl.nE, kl.nLoc, kl.xminExcl, nG, kl.invh, kl.fac};
   const unsigned grid = (unsigned)(nRdm + nObx * nO + nPair);

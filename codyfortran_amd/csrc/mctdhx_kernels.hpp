@@ -12,6 +12,10 @@ int set_error(int code, const char* msg);  // records the library's last error, 
 constexpr int kMaxOrbs = 12;   // orbitals per body type handled by the kernels
 constexpr int kMaxStencilReach = 31;  // nLoc - 1 for the largest FEDVR element (nLoc <= 32)
 constexpr int kMaxPc = 160;    // 16*ceil(M(M+1)/16) for M <= 12
+// CI density role: each wave sums ci_rdm_items_per_wave(nC) entries of rho1 / canonical rho2, so
+// that a workgroup stages C into LDS once for 4 ipw entries (a large CI space takes 4 per wave)
+constexpr int kRdmWideConf = 256, kRdmWideItems = 4;
+__host__ __device__ constexpr int ci_rdm_items_per_wave(int nC) { return nC > kRdmWideConf ? kRdmWideItems : 1; }
 constexpr int kInt8Slices = 5; // balanced base-254 digits per operand of the int8-sliced contraction
 // Digit base of the int8 slices.  Values scaled into [-1/2, 1/2] have balanced base-254 digits
 // in [-127, 127] (254/2 = 127 fits int8 without a carry case); each digit carries log2 254 = 7.99 bits.

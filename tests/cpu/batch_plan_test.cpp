@@ -71,6 +71,9 @@ int main() {
     // 12 orbitals: rho2' (12^4 entries, one per thread) is formed once per right-hand side
     plan_block_counts(shape(12, 924, 4096, 4, false), c);
     EXPECT(c[kPlanRho2p] == 81, "rho2' blocks %lld", (long long)c[kPlanRho2p]);
+    // 924 configurations: four CI density entries per wave, 16 per workgroup: (144 + 66^2) / 16
+    // = 281.25 -> 282 blocks, then 16 x 12 one-body and 160 x 4096 / 256 pair-source blocks
+    EXPECT(c[kPlanFront] == 282 + 192 + 2560, "12-orbital front blocks %lld", (long long)c[kPlanFront]);
   }
   // 2. the batched workload's shapes (N_g in 511/1026/2046/4096, M in 4/6/8/12, N = M/2), every
   //    FP64 split and the int8 path: built plans pass the independent check, cover each
