@@ -40,7 +40,9 @@ PEAK = {"fp64": 78.6, "split_fp32": 157.3, "bf16x3": 8000.0, "int8_sliced": 8000
 PREC_CODE = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "int8_sliced": 3}
 # bytes per operand element of the HBM-bound forms: three BF16 terms; five int8 digits
 OPERAND_BYTES = {"bf16x3": 6.0, "int8_sliced": 5.0}
-KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_r16"}
+# (the int8-sliced form's right-hand side runs its contraction with the orbital correlation of each
+# 16-point tile in the epilogue: k_mf_i8_orb)
+KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_orb"}
 TOL = 1e-9
 # forms whose contraction skips the k blocks of all-zero pair sources (cody_set_source_skip)
 SKIPPING = ("fp64", "int8_sliced")
@@ -360,10 +362,14 @@ def pmc_traffic(kname: str, workload: str, skip: bool = False):
     return None
 
 
-# The probe's segments, in launch order; each starts with a 4 KiB device copy (the separator the
-# summary splits the counter rows on).  A segment holds the contraction kernel launched alone
-# (one warm-up with its slicing, then three) and three full right-hand sides.
+# The probe's segments, in launch order.  A segment holds the contraction kernel launched alone
+# (cody_profile_contraction: pair sources by k_pair_src_t, the kernel's operands, one warm-up, then
+# three launches) and three full right-hand sides; k_pair_src_t runs nowhere else in the probe, so
+# the summary splits the counter rows at its dispatches.
 PMC_SEGMENTS = [("fp64", "dense"), ("fp64", "skip"), ("int8_sliced", "dense"), ("int8_sliced", "skip")]
+# the launches of a right-hand side besides the contraction kernel
+RHS_LAUNCHES = ("k_rhs_front", "k_rhs_slice", "k_rhs_wsum", "k_rhs_mid", "k_rhs_gram", "k_rhs_post",
+                "k_ci_apply_project")
 
 
 def pmc_probe(torch, wlm, wl):
@@ -379,8 +385,6 @@ def pmc_probe(torch, wlm, wl):
     st = wlm.seeded_state(x, w, wl.N, wl.M, seed=1234)
     ctx.upload_state(st)
     for prec, mode in PMC_SEGMENTS:
-        b[:512].copy_(a[:512])  # segment separator
-        torch.cuda.synchronize()
         ctx.set_precision(PREC_CODE[prec])
         ctx.set_source_skip(mode == "skip")
         ctx.profile_contraction(3)
@@ -394,23 +398,32 @@ def pmc_probe(torch, wlm, wl):
 
 def _pmc_rows(folder: str, counter: str):
     """(dispatch id, kernel name, grid size, value) of every dispatch in a rocprofv3 --pmc output
-    folder, in dispatch order."""
+    folder (its rocpd SQLite database, or CSV output), in dispatch order."""
     import csv
     import glob
     import re
+    import sqlite3
 
     rows = {}
+
+    def add(did, kname, grid, val):
+        name = re.sub(r"^(void )?(cody::)?", "", kname).split("(")[0].strip()
+        prev = rows.get(did)  # (a counter summed over the dispatch's hardware instances)
+        rows[did] = (did, name, grid, val + (prev[3] if prev else 0.0))
+
+    for fn in glob.glob(os.path.join(folder, "**", "*.db"), recursive=True):
+        db = sqlite3.connect(fn)
+        for did, kname, grid, cname, val in db.execute(
+                "select dispatch_id, kernel_name, grid_size, counter_name, value from counters_collection"):
+            if cname == counter:
+                add(int(did), kname, int(grid or 0), float(val))
+        db.close()
     for fn in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
         with open(fn) as f:
             for r in csv.DictReader(f):
-                if r.get("Counter_Name") != counter:
-                    continue
-                name = re.sub(r"^void ", "", r["Kernel_Name"]).split("(")[0].strip()
-                key = int(r["Dispatch_Id"])
-                # (a counter summed over the dispatch's hardware instances)
-                prev = rows.get(key)
-                val = float(r["Counter_Value"]) + (prev[3] if prev else 0.0)
-                rows[key] = (key, name, int(r.get("Grid_Size", 0) or 0), val)
+                if r.get("Counter_Name") == counter:
+                    add(int(r["Dispatch_Id"]), r["Kernel_Name"], int(r.get("Grid_Size", 0) or 0),
+                        float(r["Counter_Value"]))
     return [rows[k] for k in sorted(rows)]
 
 
@@ -423,16 +436,17 @@ def pmc_summary(fetch_dir: str, write_dir: str, wl) -> None:
     for counter, folder in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         rows = _pmc_rows(folder, counter)
         copies = [r for r in rows if "copyBuffer" in r[1]]
-        big = [r for r in copies if r[2] >= 1 << 20]
-        if len(big) < 3:
+        gmax = max((r[2] for r in copies), default=0)
+        big = [r for r in copies if r[2] == gmax]  # the 256 MiB copies
+        if len(big) != 3:
             raise SystemExit(f"pmc summary: {counter}: the three 256 MiB calibration copies are missing in {folder}")
         calib[counter] = float(np.median([r[3] for r in big]))
-        seps = [r[0] for r in copies if r[2] < 1 << 20]
+        seps = [r[0] for r in rows if r[1] == "k_pair_src_t"]
         if len(seps) != len(PMC_SEGMENTS):
-            raise SystemExit(f"pmc summary: {counter}: {len(seps)} separators, expected {len(PMC_SEGMENTS)}")
+            raise SystemExit(f"pmc summary: {counter}: {len(seps)} segment starts, expected {len(PMC_SEGMENTS)}")
         bounds = seps + [1 << 62]
         for i, (prec, mode) in enumerate(PMC_SEGMENTS):
-            seg = [r for r in rows if bounds[i] < r[0] < bounds[i + 1] and "copyBuffer" not in r[1]]
+            seg = [r for r in rows if bounds[i] <= r[0] < bounds[i + 1] and "rocclr" not in r[1]]
             kern = KERNEL[prec]
             names = []
             for r in seg:
@@ -440,11 +454,14 @@ def pmc_summary(fetch_dir: str, write_dir: str, wl) -> None:
                     names.append(r[1])
             for name in names:
                 vals = [r[3] for r in seg if r[1] == name]
-                if name.split("<")[0] == kern:
-                    # 1 warm-up + 3 alone, then one per right-hand side
-                    roles = [("contraction alone", vals[1:4]), ("in the right-hand side", vals[4:])]
-                else:
+                base = name.split("<")[0]
+                if base == kern:
+                    # warm-ups, then 3 alone, then one per right-hand side
+                    roles = [("contraction alone", vals[-6:-3]), ("in the right-hand side", vals[-3:])]
+                elif base in RHS_LAUNCHES:
                     roles = [("in the right-hand side", vals[-3:])]
+                else:  # (operand set-up of the precision or the profile)
+                    roles = []
                 for role, v in roles:
                     if v:
                         per.setdefault((name, prec, mode, role), {})[counter] = float(np.median(v))
@@ -739,6 +756,10 @@ def main():
         else:
             eb = OPERAND_BYTES[prec]
             work = eb * nGp * cols + eb * Pc * cols + 8.0 * Pc * nGp  # V and S operands once, W^T once
+            if prec == "int8_sliced":  # the epilogue's correlation: phi, h phi read, draw written once,
+                # one overlap partial per 16-point tile, the canonical rho2 and R
+                npr = wl.M * (wl.M - 1) // 2
+                work += 3 * 16.0 * wl.M * wl.nG + 16.0 * wl.M ** 2 * (nGp // 16) + 16.0 * (npr * npr + wl.M ** 2)
             unit = "GB/s"
             ach = work / (kms * 1e-3) / 1e9 if kms else None
         traffic = pmc_traffic(kname, wl.name, skip and prec in SKIPPING)

@@ -1274,17 +1274,21 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   hipStream_t st = c->stream;
   launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
   auto none = [](const char*) {};
-  CHK(enqueue_contraction(c, st, none));  // warm-up
+  CHK(enqueue_contraction(c, st, none));  // operands (slices, block list) of the state
+  // the int8 form's right-hand side runs the contraction with the orbital correlation in its
+  // epilogue: that kernel is timed, with R, rho2 and h phi of the last right-hand side
+  const OrbCorrArgs oa{c->d_state + c->nC, c->d_w, c->d_hphi, c->d_r2can, c->d_reginv, c->nG, kModeFull, c->d_draw,
+                       c->d_part};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, st));
-  for (int i = 0; i < iters; ++i) {
+  for (int i = -1; i < iters; ++i) {  // (launch -1 is a warm-up)
+    if (i == 0) HIPCHK(hipEventRecord(e0, st));
     switch (c->precision) {
       case CODY_PREC_INT8_SLICED:  // the contraction kernel alone
         if (int8_tiled(c))
-          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, nullptr,
-                               act_flags(c) ? c->d_blist : nullptr, st));
+          CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->nO, nullptr,
+                               act_flags(c) ? c->d_blist : nullptr, oa, st));
         else
           launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
         break;

@@ -307,8 +307,10 @@ int cody_profile_rhs(cody_ctx* ctx, int iters, int nMax, double* avg_ms, char* n
                      int* nKernels);
 
 /* Average duration of one launch of the mean-field contraction kernel of the current precision
- * (the dominant kernel), from `iters` back-to-back launches on the context's stream bracketed by
- * two HIP events; the pair sources are formed once from the context's state beforehand. */
+ * (the dominant kernel; for the int8-sliced form the one with the orbital correlation in its
+ * epilogue, which reads R, rho2 and h phi of the last right-hand side), from `iters` back-to-back
+ * launches after one warm-up on the context's stream bracketed by two HIP events; the pair
+ * sources and the contraction operands are formed once from the context's state beforehand. */
 int cody_profile_contraction(cody_ctx* ctx, int iters, double* avg_ms);
 
 #ifdef __cplusplus
