@@ -363,12 +363,13 @@ def pmc_traffic(kname: str, workload: str, skip: bool = False):
 
 
 # The probe's segments, in launch order.  A segment holds the contraction kernel launched alone
-# (cody_profile_contraction: pair sources by k_pair_src_t, the kernel's operands, one warm-up, then
-# three launches) and three full right-hand sides; k_pair_src_t runs nowhere else in the probe, so
-# the summary splits the counter rows at its dispatches.
+# (cody_profile_contraction: pair sources by k_pair_src_t, or k_pair_src_i8 for the int8 form, the
+# kernel's operands, one warm-up, then three launches) and three full right-hand sides; those two
+# kernels run nowhere else in the probe (the right-hand side forms its pair sources in its front
+# launch), so the summary splits the counter rows at their dispatches.
 PMC_SEGMENTS = [("fp64", "dense"), ("fp64", "skip"), ("int8_sliced", "dense"), ("int8_sliced", "skip")]
 # the launches of a right-hand side besides the contraction kernel
-RHS_LAUNCHES = ("k_rhs_front", "k_rhs_slice", "k_rhs_wsum", "k_rhs_mid", "k_rhs_gram", "k_rhs_post",
+RHS_LAUNCHES = ("k_rhs_front", "k_rhs_wsum", "k_rhs_mid", "k_rhs_gram", "k_rhs_post",
                 "k_ci_apply_project")
 
 
@@ -441,7 +442,7 @@ def pmc_summary(fetch_dir: str, write_dir: str, wl) -> None:
         if len(big) != 3:
             raise SystemExit(f"pmc summary: {counter}: the three 256 MiB calibration copies are missing in {folder}")
         calib[counter] = float(np.median([r[3] for r in big]))
-        seps = [r[0] for r in rows if r[1] == "k_pair_src_t"]
+        seps = [r[0] for r in rows if r[1] in ("k_pair_src_t", "k_pair_src_i8")]
         if len(seps) != len(PMC_SEGMENTS):
             raise SystemExit(f"pmc summary: {counter}: {len(seps)} segment starts, expected {len(PMC_SEGMENTS)}")
         bounds = seps + [1 << 62]

@@ -49,17 +49,17 @@ void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]) {
   const int64_t per = 4 * (s.nC > kPlanRdmWideConf ? kPlanRdmWideItems : 1);
   const int64_t nRdm = (n2 + np * np + per - 1) / per;
   const int64_t nRed = ((int64_t)s.Pc * s.Pc + n2 + 255) / 256;
-  const int64_t wsumFirst = s.int8 ? 0 : 1;  // R beside the int8 slicing, else in the k-split-sum launch
-  cnt[kPlanFront] = nRdm + nObx * s.nO + plane / 256;
-  // the int8 slicing: R, the active-block list, then the slices
-  cnt[kPlanSlice] = s.int8 ? 2 + ((int64_t)s.Pc * (s.nGp >> 2) + 255) / 256 : 0;
+  const int64_t wsumFirst = s.int8 ? 0 : 1;  // R in the int8 path's Gram launch, else in the k-split sum's
+  // pair sources: one 16 x 16 fragment per workgroup, or for the int8 path (digits with a scale per
+  // line and 64-point block) one 16 x 64 block
+  cnt[kPlanFront] = nRdm + nObx * s.nO + plane / (s.int8 ? 1024 : 256);
   // the int8 contraction: one workgroup per 16-row tile; the FP64 one: 64-row groups x k ranges
   cnt[kPlanContract] = s.int8 ? (int64_t)(s.nGp / 16) : (int64_t)(s.nGp / 64) * s.nsplit;
   cnt[kPlanWsum] = wsumFirst + (s.nsplit > 1 ? (plane / 2 + 255) / 256 : 0);
   cnt[kPlanRho2p] = s.nO >= kPlanRho2pMinOrbs ? (n2 * n2 + 255) / 256 : 0;
-  // Gram + orbital correlation; the int8 path's contraction forms the correlation, and one
-  // workgroup sums its overlap partials beside the Gram
-  cnt[kPlanMid] = (int64_t)(s.Pc / 16) * s.nxs + (s.int8 ? 1 : (int64_t)(s.nGp / 64) * s.nO);
+  // Gram + orbital correlation; the int8 path's contraction forms the correlation, and beside the
+  // Gram two workgroups sum its two sets of overlap partials and one forms R
+  cnt[kPlanMid] = (int64_t)(s.Pc / 16) * s.nxs + (s.int8 ? 3 : (int64_t)(s.nGp / 64) * s.nO);
   cnt[kPlanPost] = nRed + nObx;
   cnt[kPlanApply] = (s.nC + 3) / 4;
 }

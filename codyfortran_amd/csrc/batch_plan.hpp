@@ -27,7 +27,7 @@ constexpr int kPlanF64MaxTiles = 10;       // 16-column tiles of the FP64 contra
 constexpr int kPlanRdmWideConf = 256, kPlanRdmWideItems = 4;
 
 // The kernel types of a right-hand side, in launch order.
-enum PlanKind { kPlanFront, kPlanSlice, kPlanContract, kPlanWsum, kPlanRho2p, kPlanMid, kPlanPost, kPlanApply, kPlanKinds };
+enum PlanKind { kPlanFront, kPlanContract, kPlanWsum, kPlanRho2p, kPlanMid, kPlanPost, kPlanApply, kPlanKinds };
 
 // Orbital counts from which a grouped right-hand side forms rho2' = rho2 x_3 R once per instance
 // (kPlanRho2p) instead of in every orbital-correlation workgroup.
@@ -46,11 +46,11 @@ struct PlanShape {
 };
 
 // Workgroups of each kernel type of one instance's right-hand side: the grids of its own launches
-// (front: CI densities + one-body part + pair sources; the int8 slicing with R and the active-block
-// list; the contraction, with the orbital correlation on the int8 path; the k-split sum (with R on
-// the FP64 path); rho2' for nO >= kPlanRho2pMinOrbs; Gram + orbital correlation, or on the int8
-// path Gram + the sum of the overlap partials; G reduction + orbital projection; CI apply).  The
-// shape must have passed check_plan_shape.
+// (front: CI densities + one-body part + pair sources, on the int8 path with S's digits; the
+// contraction, with the orbital correlation on the int8 path; the k-split sum with R (FP64 path);
+// rho2' for nO >= kPlanRho2pMinOrbs; Gram + orbital correlation, or on the int8 path Gram + the sums
+// of the overlap partials + R; G reduction + orbital projection; CI apply).  The shape must have
+// passed check_plan_shape.
 void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]);
 
 // Returns an empty string for a shape the grouped kernels accept, else the reason.

@@ -64,8 +64,12 @@ int prepare_precision(cody_ctx* c) {
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
       if (!c->d_Vi8) {
+        // (eS: per-line exponents of the untiled form; eSb, actK: per-line-and-block exponents and
+        // per-block flags of the tiled form)
+        const size_t nb = (size_t)(c->nGp / 64) * c->Pc;
         if ((r = ctx_alloc((void**)&c->d_Vi8, nv * kInt8Slices)) || (r = ctx_alloc((void**)&c->d_Si8, ns * kInt8Slices)) ||
-            (r = ctx_alloc((void**)&c->d_eV, sizeof(int) * c->nGp)) || (r = ctx_alloc((void**)&c->d_eS, sizeof(int) * c->Pc)))
+            (r = ctx_alloc((void**)&c->d_eV, sizeof(int) * c->nGp)) || (r = ctx_alloc((void**)&c->d_eS, sizeof(int) * c->Pc)) ||
+            (r = ctx_alloc((void**)&c->d_eSb, sizeof(int) * nb)) || (r = ctx_alloc((void**)&c->d_actK, sizeof(int) * nb / 16)))
           return r;
       }
       if (c->int_set && !c->vprec_ready)
@@ -98,17 +102,30 @@ int prepare_precision(cody_ctx* c) {
   return CODY_OK;
 }
 
-// The int8 form with at most 80 pair columns slices the tiled S^T (per-line maxima from the
-// pair-source kernel); the other reduced-precision forms read row-major S^T.
-// the int8 form on the 16-row contraction over the active k blocks, S slices fragment-tiled
-// (Pc <= 80, grids up to kR16MaxBlocks64 64-point blocks); else the untiled int8 kernel
+// The int8 form on the 16-row contraction over the active k blocks, S's digits fragment-tiled and
+// formed by the pair-source kernel (Pc <= 80, grids up to kR16MaxBlocks64 64-point blocks); else
+// the untiled int8 kernel, which like the other reduced-precision forms reads row-major S^T.
 bool int8_tiled(const cody_ctx* c) {
   return c->precision == CODY_PREC_INT8_SLICED && c->Pc <= 80 && c->nGp / 64 <= kR16MaxBlocks64;
 }
 double* plain_st(cody_ctx* c) { return (c->precision == CODY_PREC_FP64 || int8_tiled(c)) ? nullptr : c->d_ST; }
-unsigned long long* st_max(cody_ctx* c) { return int8_tiled(c) ? c->d_smax : nullptr; }
-// activity flags of the pair sources for the contraction (null: every k block is contracted)
-const int* act_flags(const cody_ctx* c) { return c->skip_zero ? c->d_act : nullptr; }
+// activity flags of the pair sources for the contraction (null: every k block is contracted):
+// per 16 x 16 fragment for the FP64 form, per 16 x 64 block for the tiled int8 form
+const int* act_flags(const cody_ctx* c) {
+  if (!c->skip_zero) return nullptr;
+  return int8_tiled(c) ? c->d_actK : c->d_act;
+}
+
+// The pair sources of orbs for the context's contraction: S^T fragment-tiled with its activity
+// flags, with the int8 digits where the tiled int8 contraction reads them, and row-major S^T for
+// the forms that read that.
+void enqueue_pair_src(cody_ctx* c, const double2* orbs, hipStream_t st) {
+  if (int8_tiled(c))
+    launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, nullptr, c->d_STt, nullptr, st, c->d_Si8, c->d_eSb,
+                    c->d_actK);
+  else
+    launch_pair_src(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, c->d_act, st);
+}
 
 // Enqueue W = V S for the context's precision (ST already formed).  With partialSplit null, W^T
 // is complete at the end; otherwise a k-split contraction leaves its partial planes in d_Wpart
@@ -116,28 +133,18 @@ const int* act_flags(const cody_ctx* c) { return c->skip_zero ? c->d_act : nullp
 // Returns 0, or the launcher's error (nothing of the contraction launched) when the shape is
 // outside what the kernels take.
 int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark,
-                        int* partialSplit = nullptr, bool* reginvDone = nullptr, const OrbCorrArgs* orb = nullptr) {
+                        int* partialSplit = nullptr, const OrbCorrArgs* orb = nullptr) {
   const size_t ns = (size_t)c->Pc * c->nGp;
   if (partialSplit) *partialSplit = 0;
-  if (reginvDone) *reginvDone = false;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
-      if (int8_tiled(c)) {  // 16-row kernel over the active k blocks, W^T written directly
-        const int* act = act_flags(c);
-        int* list = act ? c->d_blist : nullptr;
-        // R beside the slicing when the right-hand side asks for it (the fused correlation reads it),
-        // and the active-block list
-        if (orb && !reginvDone) return set_error(CODY_ERR_ARG, "enqueue_contraction: the fused correlation needs R");
-        launch_rhs_slice(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_STt, c->d_smax, c->nGp, c->Pc, kInt8Slices,
-                         c->d_Si8, c->d_eS, st, reginvDone != nullptr, act, list);
-        if (reginvDone) *reginvDone = true;
-        mark("k_rhs_slice");
+      if (int8_tiled(c)) {  // 16-row kernel over the active k blocks (S's digits from the pair sources)
         if (orb) {
-          CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->nO, c->d_smax, list, *orb,
+          CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->nO, *orb,
                                st));
           mark("k_mf_i8_orb");
         } else {
-          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, c->d_smax, list, st));
+          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->Pc, st));
           mark("k_mf_i8_r16");
         }
         return CODY_OK;
@@ -463,42 +470,43 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
     return CODY_OK;
   }
   // One queue, few launches: independent pieces share a launch as workgroup roles.
-  //   front  CI densities | one-body part and h1 partials | pair sources, maxima, activity flags
-  //   (S^T row-major for the forms that read it, the int8 slices of S)
+  //   front  CI densities | one-body part and h1 partials | pair sources and activity flags
+  //          (the tiled int8 form: with S's digits)
   //   W = V S, leaving its k-split partials
   //   wsum   R = rho1 (rho1^2 + eps)^-1 | k-split sum of W^T
   //   mid    Gram partials of G = S^T W | orbital correlation and projector-overlap partials
   //   post   G and h1 from their partials | orbital projection
   //   post   also the energy E = <C|H C> in partial sums
   //   final  CI H-apply and CI projection with E
+  // The tiled int8 form's contraction also forms the orbital correlation of its points before R
+  // (c_m(x) and its overlap partials); then one launch forms the Gram partials, the partial sums
+  // and R, and the projection applies R.
+  const bool fused = int8_tiled(c);
   launch_rhs_front(C, c->nC, c->nO, c->dci, c->d_rho1, c->d_r2can, c->d_rho2, orbs, c->d_w, c->d_x, c->d_vext, c->kin,
-                   efield_src(c), c->nG, c->d_hphi, c->d_part1, c->nGp, c->Pc, c->d_STt, st_max(c), c->d_act, st);
+                   efield_src(c), c->nG, c->d_hphi, c->d_part1, c->nGp, c->Pc, c->d_STt, c->d_act,
+                   fused ? c->d_Si8 : nullptr, c->d_eSb, c->d_actK, st);
   mark("k_rhs_front");
   if (double* sr = plain_st(c)) launch_pair_src_rows(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, sr, st);
   int ps = 0;
-  bool reginvDone = false;
-  // the int8 form's contraction also forms the orbital correlation of its points (with R formed
-  // beside its slicing); the other forms leave it to the Gram launch
-  const bool fused = int8_tiled(c);
-  const OrbCorrArgs oa{orbs, c->d_w, c->d_hphi, c->d_r2can, c->d_reginv, c->nG, mode, c->d_draw, c->d_part};
-  CHK(enqueue_contraction(c, st, mark, &ps, &reginvDone, fused ? &oa : nullptr));
-  if (ps > 1 || !reginvDone) {  // (nothing to do when the contraction wrote W^T and R is formed)
-    launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st,
-                    !reginvDone);
-    mark("k_rhs_wsum");
-  }
+  const OrbCorrArgs oa{orbs, c->d_w, c->d_hphi, c->d_r2can, c->nG, mode, c->d_draw, c->d_part};
+  CHK(enqueue_contraction(c, st, mark, &ps, fused ? &oa : nullptr));
   if (fused) {
     launch_rhs_gram(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, c->nO, c->d_part, c->nGp / 16, c->d_partr,
-                    st);
+                    c->d_rho1, c->reg, c->d_reginv, st);
     mark("k_rhs_gram");
   } else {
+    launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st);
+    mark("k_rhs_wsum");
     launch_rhs_mid(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, orbs, c->d_w, c->d_hphi, c->d_r2can,
                    c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st);
     mark("k_rhs_mid");
   }
+  // (the int8 projection: h phi where the mode has the one-body part, c_m in draw, the summed partials)
+  const bool hOn = mode == kModeFull || mode == kModeNonLinConsistent;
   launch_rhs_post(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->d_part1, nPart1, c->d_h1, c->d_rho1, c->d_rho2, c->d_epart,
-                  orbs, c->d_draw, fused ? c->d_partr : c->d_part, fused ? 1 : c->nPart, c->nG, c->nO, dorbs,
-                  mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st);
+                  orbs, fused ? (hOn ? c->d_hphi : nullptr) : c->d_draw, fused ? c->d_partr : c->d_part,
+                  fused ? 1 : c->nPart, c->nG, c->nO, dorbs, mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st,
+                  fused ? c->d_draw : nullptr, c->d_reginv);
   mark("k_rhs_post");
   launch_ci_apply_project(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_epart,
                           rhs_post_energy_parts(c->Pc, c->nO), c->d_dCraw, dC, st);
@@ -567,7 +575,7 @@ int cody_autotune(cody_ctx* c, int* wsplit) {
     return CODY_OK;
   }
   if (c->precision == CODY_PREC_FP64) {
-    launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
+    enqueue_pair_src(c, c->d_state + c->nC, st);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -718,7 +726,7 @@ int cody_meanfield_w(cody_ctx* c, const double* orbs, double* W) {
   HIPCHK(hipSetDevice(c->device));
   double2* d_orbs = c->d_tmp + c->nC;
   HIPCHK(hipMemcpyAsync(d_orbs, orbs, sizeof(double2) * c->nG * c->nO, hipMemcpyHostToDevice, c->stream));
-  launch_pair_src(d_orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, c->stream);
+  enqueue_pair_src(c, d_orbs, c->stream);
   CHK(enqueue_contraction(c, c->stream, [](const char*) {}));
   HIPCHK(hipGetLastError());
   std::vector<double> wt((size_t)c->Pc * c->nGp);
@@ -877,7 +885,7 @@ int cody_apply_correlation(cody_ctx* c, const double* orbs, const double* rdm1, 
   HIPCHK(hipMemcpyAsync(c->d_rho1, rdm1, sizeof(double2) * n2, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->d_r2can, can.data(), sizeof(double) * can.size(), hipMemcpyHostToDevice, st));
   launch_reginv(c->d_rho1, nO, c->reg, c->d_reginv, st);
-  launch_pair_src(d_orbs, c->d_w, nG, c->nGp, nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
+  enqueue_pair_src(c, d_orbs, st);
   CHK(enqueue_contraction(c, st, [](const char*) {}));
   launch_orbital(d_orbs, c->d_w, c->d_hphi, c->d_WT, c->d_r2can, c->d_reginv, nG, c->nGp, nO, kModeNonLin,
                  c->d_draw, c->d_part, st);
@@ -983,8 +991,8 @@ double rhs_cost_estimate(const cody_ctx* c) { return 2.0 * c->nGp * (double)c->n
 // Partitions the batch: instances on the FP64 or the 16-row int8 contraction are grouped by
 // (orbital count, contraction), each group getting its argument table and block maps in device
 // memory; the others (the split-FP32 / BF16 forms, the other int8 kernels) keep their own
-// launches.  The maps repeat each instance's own launch grids (launch_rhs_front, launch_rhs_slice,
-// launch_mf_f64_w / launch_mf_i8_r16, launch_rhs_wsum, launch_rhs_mid, launch_rhs_post,
+// launches.  The maps repeat each instance's own launch grids (launch_rhs_front, launch_mf_f64_w /
+// launch_mf_i8_orb, launch_rhs_wsum, launch_rhs_mid / launch_rhs_gram, launch_rhs_post,
 // launch_ci_apply_project), so a grouped workgroup does what that instance's own would.
 int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dstates) {
   b->release_groups();
@@ -1066,7 +1074,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.nRdm = (n2 + np * np + 4 * ci_rdm_items_per_wave(c->nC) - 1) / (4 * ci_rdm_items_per_wave(c->nC));
       a.nObx = (c->nG + 255) / 256;
       a.nsplit = g.int8 ? 1 : c->wsplit;  // (the int8 contraction writes W^T itself)
-      a.wsumFirst = g.int8 ? 0 : 1;  // (the int8 path forms R beside the slicing)
+      a.wsumFirst = g.int8 ? 0 : 1;  // (the int8 path forms R in its Gram launch)
       a.nxs = c->nxs;
       a.nPart = c->nPart;
       a.nPart1 = (c->nG + 255) / 256;
@@ -1094,16 +1102,16 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.Gpart = c->d_Gpart;
       a.G = c->d_G;
       a.epart = c->d_epart;
-      a.smax = st_max(c);
       a.act = c->d_act;
-      a.actC = c->nGp / 16 > kSkipMaxBlocks16 ? nullptr : act_flags(c);
-      a.blist = c->d_blist;
+      a.actK = c->d_actK;
+      // (the FP64 kernel keeps its block list in LDS, up to kSkipMaxBlocks16 fragments)
+      a.actC = (!g.int8 && c->nGp / 16 > kSkipMaxBlocks16) ? nullptr : act_flags(c);
       a.partr = g.int8 ? c->d_partr : nullptr;
       a.Vt = c->d_Vt;
       a.Vi8 = c->d_Vi8;
       a.eV = c->d_eV;
-      a.Si8 = c->d_Si8;
-      a.eS = c->d_eS;
+      a.Si8 = g.int8 ? c->d_Si8 : nullptr;  // (selects the front's pair-source role)
+      a.eSb = c->d_eSb;
       gi.push_back(a);
       PlanShape sh;
       sh.nO = nO;
@@ -1145,7 +1153,6 @@ int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
   auto map = [&](int k) { return g.d_map + g.off[k]; };
   launch_grp_front(g.nO, g.d_inst, map(kPlanFront), g.cnt[kPlanFront], g.maxC, st);
   if (g.int8) {
-    launch_grp_slice(g.nO, g.d_inst, map(kPlanSlice), g.cnt[kPlanSlice], st);
     CHK(launch_grp_mf_i8_orb(g.nO, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   } else {
     CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
@@ -1272,13 +1279,12 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
   CHK(check_ready(c));
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  launch_pair_src(c->d_state + c->nC, c->d_w, c->nG, c->nGp, c->nO, c->Pc, plain_st(c), c->d_STt, st_max(c), c->d_act, st);
+  enqueue_pair_src(c, c->d_state + c->nC, st);
   auto none = [](const char*) {};
   CHK(enqueue_contraction(c, st, none));  // operands (slices, block list) of the state
   // the int8 form's right-hand side runs the contraction with the orbital correlation in its
   // epilogue: that kernel is timed, with R, rho2 and h phi of the last right-hand side
-  const OrbCorrArgs oa{c->d_state + c->nC, c->d_w, c->d_hphi, c->d_r2can, c->d_reginv, c->nG, kModeFull, c->d_draw,
-                       c->d_part};
+  const OrbCorrArgs oa{c->d_state + c->nC, c->d_w, c->d_hphi, c->d_r2can, c->nG, kModeFull, c->d_draw, c->d_part};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -1287,8 +1293,7 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
     switch (c->precision) {
       case CODY_PREC_INT8_SLICED:  // the contraction kernel alone
         if (int8_tiled(c))
-          CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->nO, nullptr,
-                               act_flags(c) ? c->d_blist : nullptr, oa, st));
+          CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->nO, oa, st));
         else
           launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
         break;

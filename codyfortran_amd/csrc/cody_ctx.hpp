@@ -149,9 +149,7 @@ struct cody_ctx {
   double *d_x = nullptr, *d_w = nullptr, *d_vext = nullptr, *d_kref = nullptr;
   double *d_V = nullptr, *d_ST = nullptr, *d_WT = nullptr;
   double *d_STt = nullptr, *d_Wpart = nullptr;  // fragment-tiled S^T, k-split partials of W^T
-  unsigned long long* d_smax = nullptr;          // per-line max |S^T| (int8 form), cleared per use
   int* d_act = nullptr;        // per-fragment activity of S^T (k_pair_src_t), Pc/16 x nGp/16
-  int* d_blist = nullptr;      // active 64-point k blocks of S^T: count, then indices (nGp/64 + 1)
   bool skip_zero = true;       // contraction skips the k blocks whose pair sources are all zero
   double *d_Gpart = nullptr, *d_G = nullptr, *d_energy = nullptr;
   double* d_epart = nullptr;  // energy partials of the G reduction (the CI projection's overlap)
@@ -161,7 +159,7 @@ struct cody_ctx {
   double2* d_r2can = nullptr;  // canonical rho2Lin (i1 < i2, j1 < j2)
   double2 *d_h1 = nullptr, *d_hphi = nullptr, *d_dCraw = nullptr;
   double2 *d_draw = nullptr, *d_part = nullptr, *d_part1 = nullptr;
-  double2* d_partr = nullptr;  // the overlap partials summed (the int8 path's projection reads these)
+  double2* d_partr = nullptr;  // the int8 path's two sets of overlap partials summed (2 nO^2)
   double* d_zeros = nullptr;  // nG + 1 zeros: a zero potential and a zero field (the kinetic piece)
   double2* d_src = nullptr;   // interaction sources of cody_interaction_potential (nG x P, on use)
   double2* d_h2 = nullptr;    // full h2 tensor of cody_apply_correlation (nO^4, on use)
@@ -172,6 +170,7 @@ struct cody_ctx {
   double* d_Vt = nullptr;
   int8_t *d_Vi8 = nullptr, *d_Si8 = nullptr;
   int *d_eV = nullptr, *d_eS = nullptr;
+  int *d_eSb = nullptr, *d_actK = nullptr;  // tiled int8 form: S's block exponents, block flags
   float *d_Vhi = nullptr, *d_Vlo = nullptr, *d_Shi = nullptr, *d_Slo = nullptr;
   __bf16 *d_Vbf = nullptr, *d_Sbf = nullptr;
   bool vprec_ready = false;  // V operands of the current precision are up to date
@@ -192,9 +191,7 @@ struct cody_ctx {
     ALLOC(d_ST, sizeof(double) * Pc * nGp);
     ALLOC(d_WT, sizeof(double) * Pc * nGp);
     ALLOC(d_STt, sizeof(double) * Pc * nGp);
-    ALLOC(d_smax, sizeof(unsigned long long) * Pc);
     ALLOC(d_act, sizeof(int) * (Pc / 16) * (nGp / 16));
-    ALLOC(d_blist, sizeof(int) * (nGp / 64 + 1));
     ALLOC(d_Wpart, sizeof(double) * cody::kMaxWSplit * Pc * nGp);
     ALLOC(d_Gpart, sizeof(double) * nxs * Pc * Pc);
     ALLOC(d_G, sizeof(double) * Pc * Pc);
@@ -214,10 +211,10 @@ struct cody_ctx {
     ALLOC(d_hphi, sizeof(double2) * nG * nO);
     ALLOC(d_dCraw, sizeof(double2) * nC);
     ALLOC(d_draw, sizeof(double2) * nG * nO);
-    // overlap partials: one per 64-point workgroup of the orbital kernel, or per 16-point tile of the
-    // fused int8 contraction
-    ALLOC(d_part, sizeof(double2) * std::max(nPart, nGp / 16) * n2);
-    ALLOC(d_partr, sizeof(double2) * n2);
+    // overlap partials: one per 64-point workgroup of the orbital kernel, or two per 16-point tile of
+    // the fused int8 contraction
+    ALLOC(d_part, sizeof(double2) * std::max(nPart, 2 * (nGp / 16)) * n2);
+    ALLOC(d_partr, sizeof(double2) * 2 * n2);
     ALLOC(d_part1, sizeof(double2) * ((nG + 255) / 256) * n2);
     ALLOC(d_zeros, sizeof(double) * (nG + 1));
     // CI tables in one blob: ptr arrays then entry arrays
@@ -279,7 +276,7 @@ struct cody_ctx {
 
   void release() {
     drop_graphs();
-    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_smax, d_act, d_blist,
+    void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_act,
                     d_Gpart, d_G,  d_energy, d_epart, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
                     d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, d_zeros, d_src, d_h2, ci_blob,
                     d_partr};
@@ -287,7 +284,7 @@ struct cody_ctx {
       if (p) hipFree(p);
     prop.release();
     adj.release();
-    void* lp[] = {d_Vt, d_Vi8, d_Si8, d_eV, d_eS, d_Vhi, d_Vlo, d_Shi, d_Slo, d_Vbf, d_Sbf};
+    void* lp[] = {d_Vt, d_Vi8, d_Si8, d_eV, d_eS, d_eSb, d_actK, d_Vhi, d_Vlo, d_Shi, d_Slo, d_Vbf, d_Sbf};
     for (void* p : lp)
       if (p) hipFree(p);
     if (stream) hipStreamDestroy(stream);

@@ -109,28 +109,6 @@ __device__ int active_blocks(const int* __restrict__ act, int nT, int nK16, int 
 }
 
 
-// The active 64-point k blocks of the fragment-tiled S^T, for the 16-row int8 contraction, by one
-// wave: list[0] = their count, list[1..] their indices in increasing order.  Block b is active when
-// some activity flag act[t nK16 + 4 b + q] (t < nT, q < 4) is set.
-__device__ __forceinline__ void block_list_wave(const int* __restrict__ act, int nT, int nK16, int nK64,
-                                                int* __restrict__ list) {
-  const int lane = threadIdx.x & 63;
-  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  int n = 0;
-  for (int base = 0; base < nK64; base += 64) {
-    const int b = base + lane;
-    int any = 0;
-    if (b < nK64)
-      for (int t = 0; t < nT; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) any |= act[(size_t)t * nK16 + 4 * b + q];
-    const unsigned long long m = __ballot(any != 0);
-    if (any) list[1 + n + __popcll(m & below)] = b;
-    n += __popcll(m);
-  }
-  if (lane == 0) list[0] = n;
-}
-
 // Gram split-K partials Gpart[y][c2][c1] = sum_{x in range y} S^T[c1][x] W^T[c2][x] (G = S^T W,
 // used by h2): A = S^T fragment-tiled (as k_pair_src_t writes it), B = W^T row-major; one
 // workgroup per 16 rows of S^T (bx) and k range by, one quarter of it per wave.
@@ -201,45 +179,6 @@ __device__ __forceinline__ void wsum_body(int bx, const double* __restrict__ Wpa
   d2_t v = *reinterpret_cast<const d2_t*>(Wpart + i);
   for (int ks = 1; ks < nsplit; ++ks) v += *reinterpret_cast<const d2_t*>(Wpart + ks * plane + i);
   *reinterpret_cast<d2_t*>(WT + i) = v;
-}
-
-// The int8 slices of S^T straight from its FP64 fragment-tiled copy, four consecutive x of one
-// line per thread g (one 32-bit store per slice), in the tiled int8 layout of k_slice_rows; the
-// line's scale comes from smax (per-line max |S^T| from k_pair_src_t), so every thread knows its
-// exponent without a reduction.  The exponent rule and the slicing are those of k_slice_rows.
-__device__ __forceinline__ void slice_s_tiled_body(int g, const double* __restrict__ STt,
-                                                   const unsigned long long* __restrict__ smax, int nGp, int Pc,
-                                                   int ns, int8_t* __restrict__ out, int* __restrict__ expo) {
-  const int perLine = nGp >> 2;
-  if (g >= Pc * perLine) return;
-  const int c = g / perLine, x0 = 4 * (g % perLine);
-  const double m = __longlong_as_double((long long)smax[c]);
-  int e = 0;
-  if (m > 0.0) {
-    int ex;
-    const double fr = frexp(m, &ex);
-    e = (fr == 0.5 ? ex - 1 : ex) + 1;
-  }
-  if (x0 == 0) expo[c] = e;
-  const int nK = nGp >> 4, nK64 = nGp >> 6;
-  // x0..x0+3: (h, e2) = (0,0), (0,1), (1,0), (1,1) of lane 16 ((x0 % 16) / 4) + c % 16
-  const size_t f = ((size_t)(c >> 4) * nK + (x0 >> 4)) * 256 + 2 * (16 * ((x0 & 15) >> 2) + (c & 15));
-  double r[4] = {STt[f], STt[f + 1], STt[f + 128], STt[f + 129]};
-#pragma unroll
-  for (int u = 0; u < 4; ++u) r[u] = ldexp(r[u], -e);
-  const int lane = 16 * ((x0 & 63) >> 4) + (c & 15);
-  const size_t o = ((size_t)(c >> 4) * nK64 + (x0 >> 6)) * ns * 1024 + 16 * lane + (x0 & 15);
-  for (int p = 1; p <= ns; ++p) {
-    unsigned word = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double t = r[u] * kSliceBase;
-      const double q = rint(t);
-      r[u] = t - q;
-      word |= (unsigned)(uint8_t)static_cast<int8_t>(q) << (8 * u);
-    }
-    *reinterpret_cast<unsigned*>(out + o + (size_t)(p - 1) * 1024) = word;
-  }
 }
 
 }  // namespace cody

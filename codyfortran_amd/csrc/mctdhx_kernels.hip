@@ -61,18 +61,14 @@ __global__ __launch_bounds__(256) void k_pair_src(const double2* __restrict__ or
 
 // The same S^T fragment-tiled for the FP64 kernels, one output element per thread (coalesced):
 // STt[(t nK + K) 256 + 128 h + 2 l + e] = S^T[c = 16 t + l % 16][x = 16 K + 4 (l / 16) + 2 h + e].
-// A workgroup covers one 16 x 16 fragment; with smax non-null it also raises smax[c] to the
-// largest |S^T[c][x]| of its fragment (bit patterns of non-negative doubles order as integers),
-// which the int8 slicing needs as the per-line scale.  With act non-null it writes act[fragment]
+// A workgroup covers one 16 x 16 fragment.  With act non-null it writes act[fragment]
 // = 1 if any of its values is non-zero, else 0 (a plain store every call: nothing to clear); the
 // contraction kernels skip the k blocks whose fragments are all zero, the sources the reference
 // leaves out of its convolution sums (src/Utils/M_Utils_ConvolutionIntegral.f90:84-93).
 __device__ __forceinline__ void pair_src_t_body(int bx, const double2* __restrict__ orbs,
                                                     const double* __restrict__ w, int nG, int nGp, int nO,
                                                     int Pc, double* __restrict__ STt,
-                                                    unsigned long long* __restrict__ smax,
                                                     int* __restrict__ act) {
-  __shared__ unsigned long long sm[16];
   const size_t o = (size_t)bx * blockDim.x + threadIdx.x;  // grid = Pc nGp / 256 exactly
   const int nK = nGp >> 4;
   const int e = o & 1, h = (o >> 7) & 1, l = (o >> 1) & 63;
@@ -91,25 +87,101 @@ __device__ __forceinline__ void pair_src_t_body(int bx, const double2* __restric
     v = (c & 1) ? s.y : s.x;
   }
   STt[o] = v;
-  if (smax) {
-    if (threadIdx.x < 16) sm[threadIdx.x] = 0ull;
-    __syncthreads();
-    atomicMax(&sm[l & 15], (unsigned long long)__double_as_longlong(fabs(v)));
-    __syncthreads();
-    if (threadIdx.x < 16 && sm[threadIdx.x] != 0ull) atomicMax(&smax[16 * t + threadIdx.x], sm[threadIdx.x]);
-  }
   if (act) {
     const int any = __syncthreads_or(v != 0.0);
     if (threadIdx.x == 0) act[blk] = any ? 1 : 0;
   }
 }
 
+// The pair sources of one 16-line x 64-point block (t, K64) for the int8 form: S^T in the FP64
+// fragment order of pair_src_t_body, and its five balanced base-254 digits in the int8 fragment
+// order (digit p of block (t, K64) is 1 KiB at ((t nK64 + K64) NS + p) 1024, lane 16 (K16 % 4) +
+// line % 16 holding the 16 points of K16), under a scale 2^e per line and block: e = the exponent
+// of the block's largest |S| on that line plus one (|S| 2^-e <= 1/2), at eSb[K64 Pc + line].  So
+// the digits need no grid-wide maximum and are formed where S is.  act[t nK64 + K64] = 1 when a
+// value of the block is non-zero.  Wave w takes K16 = 4 K64 + w, each thread four points of one line.
+__device__ __forceinline__ void pair_src_i8_body(int bx, const double2* __restrict__ orbs,
+                                                 const double* __restrict__ w, int nG, int nGp, int nO, int Pc,
+                                                 double* __restrict__ STt, int8_t* __restrict__ Si8,
+                                                 int* __restrict__ eSb, int* __restrict__ act) {
+  constexpr int NS = kInt8Slices;
+  __shared__ double sM[4][64];
+  __shared__ int sE[16];
+  const int nK16 = nGp >> 4, nK64 = nGp >> 6;
+  const int t = bx / nK64, K64 = bx % nK64;
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, K16 = 4 * K64 + wv;
+  const int c = 16 * t + (l & 15), p = c >> 1;
+  const bool pin = p < nO * (nO + 1) / 2;
+  int a = 0, r = p;
+  if (pin)
+    while (r >= nO - a) {
+      r -= nO - a;
+      ++a;
+    }
+  double v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {  // u = 2 h + e of the FP64 fragment
+    const int x = 16 * K16 + 4 * (l >> 4) + u;
+    double s = 0.0;
+    if (pin && x < nG) {
+      const double2 z = cscale(cmul(cconj(orbs[(size_t)a * nG + x]), orbs[(size_t)(a + r) * nG + x]), w[x]);
+      s = (c & 1) ? z.y : z.x;
+    }
+    v[u] = s;
+  }
+  double* dst = STt + ((size_t)t * nK16 + K16) * 256 + 2 * l;
+  *reinterpret_cast<double2*>(dst) = make_double2(v[0], v[1]);
+  *reinterpret_cast<double2*>(dst + 128) = make_double2(v[2], v[3]);
+  sM[wv][l] = fmax(fmax(fabs(v[0]), fabs(v[1])), fmax(fabs(v[2]), fabs(v[3])));
+  const int any = __syncthreads_or(v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0);
+  if (threadIdx.x < 16) {
+    double m = 0.0;
+#pragma unroll
+    for (int w2 = 0; w2 < 4; ++w2)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) m = fmax(m, sM[w2][16 * q + threadIdx.x]);
+    int e = 0;
+    if (m > 0.0) {
+      int ex;
+      const double fr = frexp(m, &ex);
+      e = (fr == 0.5 ? ex - 1 : ex) + 1;
+    }
+    sE[threadIdx.x] = e;
+    eSb[(size_t)K64 * Pc + 16 * t + threadIdx.x] = e;
+  }
+  if (threadIdx.x == 0) act[(size_t)t * nK64 + K64] = any ? 1 : 0;
+  __syncthreads();
+  const int e = sE[l & 15];
+  double rr[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) rr[u] = ldexp(v[u], -e);
+  int8_t* o = Si8 + ((size_t)t * nK64 + K64) * NS * 1024 + 16 * (16 * wv + (l & 15)) + 4 * (l >> 4);
+#pragma unroll
+  for (int pp = 0; pp < NS; ++pp) {
+    unsigned word = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double tt = rr[u] * kSliceBase;
+      const double q = rint(tt);
+      rr[u] = tt - q;
+      word |= (unsigned)(uint8_t)static_cast<int8_t>(q) << (8 * u);
+    }
+    *reinterpret_cast<unsigned*>(o + (size_t)pp * 1024) = word;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pair_src_i8(const double2* __restrict__ orbs, const double* __restrict__ w,
+                                                     int nG, int nGp, int nO, int Pc, double* __restrict__ STt,
+                                                     int8_t* __restrict__ Si8, int* __restrict__ eSb,
+                                                     int* __restrict__ act) {
+  pair_src_i8_body(blockIdx.x, orbs, w, nG, nGp, nO, Pc, STt, Si8, eSb, act);
+}
+
 __global__ __launch_bounds__(256) void k_pair_src_t(const double2* __restrict__ orbs,
                                                     const double* __restrict__ w, int nG, int nGp, int nO,
                                                     int Pc, double* __restrict__ STt,
-                                                    unsigned long long* __restrict__ smax,
                                                     int* __restrict__ act) {
-  pair_src_t_body(blockIdx.x, orbs, w, nG, nGp, nO, Pc, STt, smax, act);
+  pair_src_t_body(blockIdx.x, orbs, w, nG, nGp, nO, Pc, STt, act);
 }
 
 // Arbitrary complex sources (ns columns of nG points, column-major) in the fragment-tiled S^T
@@ -516,6 +588,58 @@ __device__ __forceinline__ void project_body(int bx, bool ciBlock, const double2
   }
 }
 
+// The projection of the int8 path, whose contraction left c_m(x) (the correlation before R) and
+// the overlap sums A(i, j) = <phi_i|h_j>_w, B(i, m) = <phi_i|c_m>_w (partr: A, then B; h = h phi,
+// or null where the right-hand side has no one-body part):
+//   d_j = h_j + sum_m R(m, j) c_m,   <phi_i|d_j>_w = A(i, j) + sum_m B(i, m) R(m, j),
+//   dphi_j = -i [d_j - sum_i phi_i <phi_i|d_j>_w - hsub_j]  (hsub optional).
+template <int NO>
+__device__ __forceinline__ void project_corr_body(int bx, const double2* __restrict__ orbs,
+                                                  const double2* __restrict__ h, const double2* __restrict__ corrc,
+                                                  const double2* __restrict__ reginv,
+                                                  const double2* __restrict__ partr, int nG,
+                                                  double2* __restrict__ dorbs, const double2* __restrict__ hsub) {
+  constexpr int n2 = NO * NO;
+  static_assert(n2 <= 256, "one overlap entry per thread");
+  __shared__ double2 sR[n2], sB[n2], sInp[n2];
+  // every global load first (one latency): the point's orbitals, c and h, then R and the sums
+  const int x = bx * blockDim.x + threadIdx.x;
+  const bool in = x < nG;
+  double2 phi[NO], cm[NO], hv[NO];
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
+    phi[i] = in ? orbs[(size_t)i * nG + x] : cmk(0, 0);
+    cm[i] = in ? corrc[(size_t)i * nG + x] : cmk(0, 0);
+    hv[i] = (in && h) ? h[(size_t)i * nG + x] : cmk(0, 0);
+  }
+  const int e = threadIdx.x;
+  double2 av = cmk(0, 0);
+  if (e < n2) {
+    av = partr[e];
+    sR[e] = reginv[e];
+    sB[e] = partr[n2 + e];
+  }
+  __syncthreads();
+  if (e < n2) {
+    const int i = e % NO, j = e / NO;
+#pragma unroll
+    for (int m = 0; m < NO; ++m) cfma(av, sB[i + NO * m], sR[m + NO * j]);
+    sInp[e] = av;
+  }
+  __syncthreads();
+  if (!in) return;
+#pragma unroll
+  for (int j = 0; j < NO; ++j) {
+    double2 d = hv[j];
+#pragma unroll
+    for (int m = 0; m < NO; ++m) cfma(d, sR[m + NO * j], cm[m]);
+#pragma unroll
+    for (int i = 0; i < NO; ++i) d = csub(d, cmul(phi[i], sInp[i + j * NO]));
+    if (hsub) d = csub(d, hsub[(size_t)j * nG + x]);
+    dorbs[(size_t)j * nG + x] = mul_minus_i(d);
+  }
+}
+
 template <int NO>
 __global__ __launch_bounds__(256) void k_project(const double2* __restrict__ orbs,
                                                  const double2* __restrict__ draw,
@@ -607,13 +731,17 @@ void launch_fill_softyukawa(const double* x, int nG, int nGp, double s, double a
     default: break;                 \
   }
 
-// STt (fragment-tiled, always; with the per-line maxima when smax is non-null) and, when ST is
+// STt (fragment-tiled, always; with the int8 digits when Si8 is non-null) and, when ST is
 // non-null, S^T row-major.
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
-                     double* ST, double* STt, unsigned long long* smax, int* act, hipStream_t st) {
+                     double* ST, double* STt, int* act, hipStream_t st, int8_t* Si8, int* eSb, int* actK) {
   const size_t n = (size_t)Pc * nGp;
-  hipLaunchKernelGGL(k_pair_src_t, dim3((unsigned)(n / 256)), dim3(256), 0, st, orbs, w, nG, nGp, nO, Pc, STt,
-                     smax, act);
+  if (Si8)  // the int8 form: S^T with its digits (per line and 64-point block scales)
+    hipLaunchKernelGGL(k_pair_src_i8, dim3((unsigned)(n / 1024)), dim3(256), 0, st, orbs, w, nG, nGp, nO, Pc, STt,
+                       Si8, eSb, actK);
+  else
+    hipLaunchKernelGGL(k_pair_src_t, dim3((unsigned)(n / 256)), dim3(256), 0, st, orbs, w, nG, nGp, nO, Pc, STt,
+                       act);
   if (!ST) return;
 #define CALL_PS(NO)                                                                            \
   hipLaunchKernelGGL(k_pair_src<NO>, dim3((nGp + 255) / 256), dim3(256), 0, st, orbs, w, nG, nGp, \
@@ -698,7 +826,8 @@ void launch_project(const double2* orbs, const double2* draw, const double2* par
 // indices), so a latency-bound piece costs no launch of its own and no cross-queue dependency.
 
 // Front: from the state alone, the CI densities (rho1, rho2, canonical rho2), the one-body part
-// (h phi, h1 partials) and the pair sources (S^T tiled, line maxima, activity flags).
+// (h phi, h1 partials) and the pair sources (S^T tiled with activity flags, or for the int8 form
+// with its digits, block exponents and per-block flags when Si8 is set).
 template <int NO>
 __global__ __launch_bounds__(256) void k_rhs_front(const double2* __restrict__ C, int nC, DevCiTables t,
                                                    double2* __restrict__ rho1, double2* __restrict__ r2can,
@@ -708,7 +837,8 @@ __global__ __launch_bounds__(256) void k_rhs_front(const double2* __restrict__ C
                                                    KinParams kp, const double* __restrict__ efield, int nG,
                                                    double2* __restrict__ hphi, double2* __restrict__ part1, int nObx,
                                                    int nGp, int Pc, double* __restrict__ STt,
-                                                   unsigned long long* __restrict__ smax, int* __restrict__ act) {
+                                                   int* __restrict__ act, int8_t* __restrict__ Si8,
+                                                   int* __restrict__ eSb, int* __restrict__ actK) {
   int b = blockIdx.x;
   if (b < nRdm) {
     ci_rdm_body(b, C, nC, NO, t.r1_ptr, t.r1, t.r2_ptr, t.r2, rho1, r2can, rho2);
@@ -720,7 +850,10 @@ __global__ __launch_bounds__(256) void k_rhs_front(const double2* __restrict__ C
     return;
   }
   b -= nObx * NO;
-  pair_src_t_body(b, orbs, w, nG, nGp, NO, Pc, STt, smax, act);
+  if (Si8)
+    pair_src_i8_body(b, orbs, w, nG, nGp, NO, Pc, STt, Si8, eSb, actK);
+  else
+    pair_src_t_body(b, orbs, w, nG, nGp, NO, Pc, STt, act);
 }
 
 // After the contraction: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0 when `first` is 1, eliminated
@@ -737,34 +870,6 @@ __global__ __launch_bounds__(256) void k_rhs_wsum(const double2* __restrict__ rh
     return;
   }
   wsum_body(blockIdx.x - first, Wpart, nsplit, plane, WT);
-}
-
-// Before the int8 contraction, after the front launch: R = rho1 (rho1^2 + eps)^{-1} (workgroup 0,
-// one wave; when withR), the active-block list of the 16-row contraction (the next workgroup, one
-// wave; when list is set) and the slicing of the tiled S^T (the others).
-template <int NO>
-__global__ __launch_bounds__(256) void k_rhs_slice(const double2* __restrict__ rho1, double reg,
-                                                   double2* __restrict__ reginv, const double* __restrict__ STt,
-                                                   const unsigned long long* __restrict__ smax, int nGp, int Pc,
-                                                   int ns, int8_t* __restrict__ out, int* __restrict__ expo,
-                                                   int withR, const int* __restrict__ act, int* __restrict__ list) {
-  int b = blockIdx.x;
-  if (withR) {
-    if (b == 0) {
-      if (threadIdx.x >= 64) return;
-      reginv_body<NO>(rho1, reg, reginv, 64);
-      return;
-    }
-    --b;
-  }
-  if (list) {
-    if (b == 0) {
-      if (threadIdx.x < 64) block_list_wave(act, Pc / 16, nGp >> 4, nGp >> 6, list);
-      return;
-    }
-    --b;
-  }
-  slice_s_tiled_body(b * blockDim.x + threadIdx.x, STt, smax, nGp, Pc, ns, out, expo);
 }
 
 // After W: the Gram matrix's split-K partials (G = S^T W) and the orbital correlation with its
@@ -788,35 +893,46 @@ __global__ __launch_bounds__(256) void k_rhs_mid(const double* __restrict__ STt,
   orbital_body<NO>(b % nOrbX, b / nOrbX, orbs, w, hphi, WT, r2can, reginv, nG, nGp, mode, draw, part);
 }
 
-// After the fused int8 contraction (which formed the orbital correlation): the Gram matrix's split-K
-// partials (the first NT nxs workgroups) and, on the last workgroup, the fixed-order sum of the
-// contraction's nPart per-tile overlap partials into partr.
+// After the fused int8 contraction (which formed c_m(x) and its overlap partials): the Gram
+// matrix's split-K partials (the first NT nxs workgroups), the fixed-order sums of the
+// contraction's nPart overlap partials A(i, j) and of its nPart partials B(i, m) into partr[0, n2)
+// and partr[n2, 2 n2) (one workgroup each), and R = rho1 (rho1^2 + eps)^{-1} (the last workgroup,
+// one wave).
 template <int NO>
 __device__ __forceinline__ void gram_sum_body(int b, const double* __restrict__ STt, const double* __restrict__ WT,
                                               int nGp, int Pc, double* __restrict__ Gpart, int nxs,
                                               const double2* __restrict__ part, int nPart,
-                                              double2* __restrict__ partr) {
+                                              double2* __restrict__ partr, const double2* __restrict__ rho1,
+                                              double reg, double2* __restrict__ reginv) {
   constexpr int NT = (NO * (NO + 1) + 15) / 16, n2 = NO * NO, G = kOverlapGroups<NO>;
   if (b < NT * nxs) {
     gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc);
     return;
   }
+  b -= NT * nxs;
+  if (b == 2) {
+    if (threadIdx.x >= 64) return;
+    reginv_body<NO>(rho1, reg, reginv, 64);
+    return;
+  }
   __shared__ double2 sG[G][n2];
   __shared__ double2 sInp[n2];
-  sum_overlap_parts<NO>(part, nPart, sG, sInp);
-  for (int i = threadIdx.x; i < n2; i += blockDim.x) partr[i] = sInp[i];
+  sum_overlap_parts<NO>(part + (size_t)b * nPart * n2, nPart, sG, sInp);
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) partr[b * n2 + i] = sInp[i];
 }
 
 template <int NO>
 __global__ __launch_bounds__(256) void k_rhs_gram(const double* __restrict__ STt, const double* __restrict__ WT,
                                                   int nGp, int Pc, double* __restrict__ Gpart, int nxs,
                                                   const double2* __restrict__ part, int nPart,
-                                                  double2* __restrict__ partr) {
-  gram_sum_body<NO>(blockIdx.x, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, partr);
+                                                  double2* __restrict__ partr, const double2* __restrict__ rho1,
+                                                  double reg, double2* __restrict__ reginv) {
+  gram_sum_body<NO>(blockIdx.x, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, partr, rho1, reg, reginv);
 }
 
 // Then: G and h1 from their partials with the energy partials (the first nRed workgroups), and the
-// orbital projection with the right-hand side's factor -i (the others).
+// orbital projection with the right-hand side's factor -i (the others; with corrc set, the int8
+// path's: draw is then h phi or null and part the summed partials, see project_corr_body).
 template <int NO>
 __global__ __launch_bounds__(256) void k_rhs_post(const double* __restrict__ Gpart, int nxs, int Pc,
                                                   double* __restrict__ G, const double2* __restrict__ part1,
@@ -825,13 +941,18 @@ __global__ __launch_bounds__(256) void k_rhs_post(const double* __restrict__ Gpa
                                                   double* __restrict__ epart, int nRed,
                                                   const double2* __restrict__ orbs, const double2* __restrict__ draw,
                                                   const double2* __restrict__ part, int nPart, int nG,
-                                                  double2* __restrict__ dorbs, const double2* __restrict__ hsub) {
+                                                  double2* __restrict__ dorbs, const double2* __restrict__ hsub,
+                                                  const double2* __restrict__ corrc,
+                                                  const double2* __restrict__ reginv) {
   int b = blockIdx.x;
   if (b < nRed) {
     gram_reduce_energy_body(b, Gpart, nxs, Pc, G, part1, nPart1, NO, h1, rho1, rho2, epart);
     return;
   }
-  project_body<NO>(b - nRed, false, orbs, draw, part, nPart, nG, 1, dorbs, nullptr, nullptr, 0, nullptr, hsub);
+  if (corrc)
+    project_corr_body<NO>(b - nRed, orbs, draw, corrc, reginv, part, nG, dorbs, hsub);
+  else
+    project_body<NO>(b - nRed, false, orbs, draw, part, nPart, nG, 1, dorbs, nullptr, nullptr, 0, nullptr, hsub);
 }
 
 // ------------------------------------------------------------------ grouped launches
@@ -855,23 +976,10 @@ __global__ __launch_bounds__(256) void k_grp_front(const GroupInstance* __restri
     return;
   }
   b -= a.nObx * NO;
-  pair_src_t_body(b, orbs, a.w, a.nG, a.nGp, NO, a.Pc, a.STt, a.smax, a.act);
-}
-
-template <int NO>
-__global__ __launch_bounds__(256) void k_grp_slice(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
-  const int2 m = map[blockIdx.x];
-  const GroupInstance& a = gi[m.x];
-  if (m.y == 0) {
-    if (threadIdx.x >= 64) return;
-    reginv_body<NO>(a.rho1, a.reg, a.reginv, 64);
-    return;
-  }
-  if (m.y == 1) {  // the active-block list of the contraction (skipping instances)
-    if (a.actC && threadIdx.x < 64) block_list_wave(a.actC, a.Pc / 16, a.nGp >> 4, a.nGp >> 6, a.blist);
-    return;
-  }
-  slice_s_tiled_body((m.y - 2) * blockDim.x + threadIdx.x, a.STt, a.smax, a.nGp, a.Pc, kInt8Slices, a.Si8, a.eS);
+  if (a.Si8)
+    pair_src_i8_body(b, orbs, a.w, a.nG, a.nGp, NO, a.Pc, a.STt, a.Si8, a.eSb, a.actK);
+  else
+    pair_src_t_body(b, orbs, a.w, a.nG, a.nGp, NO, a.Pc, a.STt, a.act);
 }
 
 template <int NO>
@@ -925,7 +1033,8 @@ template <int NO>
 __global__ __launch_bounds__(256) void k_grp_gram(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
-  gram_sum_body<NO>(m.y, a.STt, a.WT, a.nGp, a.Pc, a.Gpart, a.nxs, a.part, a.nGp / 16, a.partr);
+  gram_sum_body<NO>(m.y, a.STt, a.WT, a.nGp, a.Pc, a.Gpart, a.nxs, a.part, a.nGp / 16, a.partr, a.rho1, a.reg,
+                    a.reginv);
 }
 
 template <int NO>
@@ -936,9 +1045,11 @@ __global__ __launch_bounds__(256) void k_grp_post(const GroupInstance* __restric
     gram_reduce_energy_body(m.y, a.Gpart, a.nxs, a.Pc, a.G, a.part1, a.nPart1, NO, a.h1, a.rho1, a.rho2, a.epart);
     return;
   }
-  // (the int8 path's projection reads the overlap partials summed after the fused contraction)
-  project_body<NO>(m.y - a.nRed, false, a.C + a.nC, a.draw, a.partr ? a.partr : a.part, a.partr ? 1 : a.nPart, a.nG,
-                   1, a.dC + a.nC, nullptr, nullptr, 0, nullptr, nullptr);
+  if (a.partr)  // the int8 path: c_m(x) and the summed overlap partials of the fused contraction
+    project_corr_body<NO>(m.y - a.nRed, a.C + a.nC, a.hphi, a.draw, a.reginv, a.partr, a.nG, a.dC + a.nC, nullptr);
+  else
+    project_body<NO>(m.y - a.nRed, false, a.C + a.nC, a.draw, a.part, a.nPart, a.nG, 1, a.dC + a.nC, nullptr, nullptr,
+                     0, nullptr, nullptr);
 }
 
 void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlocks, int maxC, hipStream_t st) {
@@ -946,11 +1057,6 @@ void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlo
   hipLaunchKernelGGL(k_grp_front<NO>, dim3(nBlocks), dim3(256), sizeof(double2) * maxC, st, gi, map)
   CODY_NO_SWITCH(nO, CALL_GF)
 #undef CALL_GF
-}
-void launch_grp_slice(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
-#define CALL_GS(NO) hipLaunchKernelGGL(k_grp_slice<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
-  CODY_NO_SWITCH(nO, CALL_GS)
-#undef CALL_GS
 }
 void launch_grp_wsum(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
   if (nBlocks == 0) return;
@@ -983,15 +1089,15 @@ void launch_grp_post(int nO, const GroupInstance* gi, const int2* map, int nBloc
 void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, double2* rho1, double2* r2can,
                       double2* rho2, const double2* orbs, const double* w, const double* x, const double* vext,
                       const KinLaunch& kl, const double* efield, int nG, double2* hphi, double2* part1, int nGp,
-                      int Pc, double* STt, unsigned long long* smax, int* act, hipStream_t st) {
+                      int Pc, double* STt, int* act, int8_t* Si8, int* eSb, int* actK, hipStream_t st) {
   const int n2 = nO * nO, np = nO * (nO - 1) / 2, per = 4 * ci_rdm_items_per_wave(nC);
   const int nRdm = (n2 + np * np + per - 1) / per;
-  const int nObx = (nG + 255) / 256, nPair = (int)((size_t)Pc * nGp / 256);
+  const int nObx = (nG + 255) / 256, nPair = (int)((size_t)Pc * nGp / (Si8 ? 1024 : 256));
   KinParams kp{kl.kref, kl.nE, kl.nLoc, kl.xminExcl, nG, kl.invh, kl.fac};
   const unsigned grid = (unsigned)(nRdm + nObx * nO + nPair);
 #define CALL_FR(NO)                                                                                             \
   hipLaunchKernelGGL(k_rhs_front<NO>, dim3(grid), dim3(256), sizeof(double2) * nC, st, C, nC, t, rho1, r2can, rho2, \
-                     nRdm, orbs, w, x, vext, kp, efield, nG, hphi, part1, nObx, nGp, Pc, STt, smax, act)
+                     nRdm, orbs, w, x, vext, kp, efield, nG, hphi, part1, nObx, nGp, Pc, STt, act, Si8, eSb, actK)
   CODY_NO_SWITCH(nO, CALL_FR)
 #undef CALL_FR
 }
@@ -1008,18 +1114,6 @@ void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, c
 #undef CALL_WS
 }
 
-void launch_rhs_slice(const double2* rho1, int nO, double reg, double2* reginv, const double* STt,
-                      const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out, int* expo,
-                      hipStream_t st, bool withR, const int* act, int* list) {
-  const int n = Pc * (nGp >> 2);
-  const int withR_ = withR ? 1 : 0, grid = withR_ + (list ? 1 : 0) + (n + 255) / 256;
-#define CALL_SL(NO)                                                                                              \
-  hipLaunchKernelGGL(k_rhs_slice<NO>, dim3(grid), dim3(256), 0, st, rho1, reg, reginv, STt, smax, nGp, Pc, ns, out, \
-                     expo, withR_, act, list)
-  CODY_NO_SWITCH(nO, CALL_SL)
-#undef CALL_SL
-}
-
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
                     const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
                     const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st) {
@@ -1032,11 +1126,12 @@ void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double
 }
 
 void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs, int nO,
-                     const double2* part, int nPart, double2* partr, hipStream_t st) {
-  const unsigned grid = (unsigned)(Pc / 16 * nxs + 1);
+                     const double2* part, int nPart, double2* partr, const double2* rho1, double reg,
+                     double2* reginv, hipStream_t st) {
+  const unsigned grid = (unsigned)(Pc / 16 * nxs + 3);
 #define CALL_RG(NO)                                                                                           \
   hipLaunchKernelGGL(k_rhs_gram<NO>, dim3(grid), dim3(256), 0, st, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, \
-                     partr)
+                     partr, rho1, reg, reginv)
   CODY_NO_SWITCH(nO, CALL_RG)
 #undef CALL_RG
 }
@@ -1046,12 +1141,12 @@ int rhs_post_energy_parts(int Pc, int nO) { return (Pc * Pc + nO * nO + 255) / 2
 void launch_rhs_post(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
                      double2* h1, const double2* rho1, const double2* rho2, double* epart, const double2* orbs,
                      const double2* draw, const double2* part, int nPart, int nG, int nO, double2* dorbs,
-                     const double2* hsub, hipStream_t st) {
+                     const double2* hsub, hipStream_t st, const double2* corrc, const double2* reginv) {
   const int nRed = rhs_post_energy_parts(Pc, nO);
   const unsigned grid = (unsigned)(nRed + (nG + 255) / 256);
 #define CALL_PO(NO)                                                                                                \
   hipLaunchKernelGGL(k_rhs_post<NO>, dim3(grid), dim3(256), 0, st, Gpart, nxs, Pc, G, part1, nPart1, h1, rho1, rho2, \
-                     epart, nRed, orbs, draw, part, nPart, nG, dorbs, hsub)
+                     epart, nRed, orbs, draw, part, nPart, nG, dorbs, hsub, corrc, reginv)
   CODY_NO_SWITCH(nO, CALL_PO)
 #undef CALL_PO
 }

@@ -76,22 +76,20 @@ struct GroupInstance {
   KinLaunch kin;
   double reg;
   double *STt, *Wpart, *WT, *Gpart, *G, *epart;
-  unsigned long long* smax;  // int8 path: per-line maxima of S^T
-  int* act;                  // pair-source activity flags (written by the front launch)
+  int* act;                  // pair-source activity flags (written by the front launch; FP64 path)
+  int* actK;                 // int8 path: per-block activity flags (written by the front launch)
   const int* actC;           // the flags the contraction reads (null: no skipping)
-  int* blist;                // int8 path: the active-block list built from actC (when set)
   double2* partr;            // int8 path: the summed overlap partials (the projection reads these)
   double2* r2p;              // rho2' (nO^4) for the orbital correlation, or null (formed in it)
   const double* Vt;                          // FP64 contraction operand
   const int8_t* Vi8;                         // int8 contraction operands
   const int* eV;
-  int8_t* Si8;
-  int* eS;
+  int8_t* Si8;  // int8 path: digits of S (written by the front launch), null on the FP64 path
+  int* eSb;     // int8 path: their per-line, per-64-point-block exponents
 };
 
 // Grouped launches over the instances of one orbital count nO (the map holds nBlocks entries)
 void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlocks, int maxC, hipStream_t st);
-void launch_grp_slice(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_wsum(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_rho2p(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_mid(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
@@ -139,24 +137,19 @@ void launch_ci_apply_project(const double2* C, int nC, int nO, int Pc, const dou
                              const DevCiTables& t, const double* epart, int nE, double2* dCraw, double2* dC,
                              hipStream_t st);
 // Fused launches of the right-hand side (one queue):
-//   front: CI densities, one-body part (h phi, h1 partials), pair sources (+ maxima, flags);
-//   wsum:  R = rho1 (rho1^2 + eps)^-1 (unless the int8 slicing launch formed it) and the k-split
-//          sum of W^T (nsplit <= 1: none);
+//   front: CI densities, one-body part (h phi, h1 partials), pair sources (+ flags; with Si8 set
+//          the int8 digits of S, their block exponents eSb and per-block flags actK instead of act);
+//   wsum:  R = rho1 (rho1^2 + eps)^-1 and the k-split sum of W^T (nsplit <= 1: none);
 //   mid:   Gram partials (nxs x-ranges) and the orbital correlation with its overlap partials;
 //   post:  G and h1 from their partials with the energy partials epart (one per reduction
-//          workgroup, rhs_post_energy_parts of them), and the orbital projection (times -i).
+//          workgroup, rhs_post_energy_parts of them), and the orbital projection (times -i); with
+//          corrc set, the int8 path's projection (draw: h phi or null, part: the summed partials).
 void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, double2* rho1, double2* r2can,
                       double2* rho2, const double2* orbs, const double* w, const double* x, const double* vext,
                       const KinLaunch& kl, const double* efield, int nG, double2* hphi, double2* part1, int nGp,
-                      int Pc, double* STt, unsigned long long* smax, int* act, hipStream_t st);
+                      int Pc, double* STt, int* act, int8_t* Si8, int* eSb, int* actK, hipStream_t st);
 void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, const double* Wpart, int nsplit,
                      size_t plane, double* WT, hipStream_t st, bool withReginv = true);
-// the int8 slicing of the tiled S^T (slice_s_tiled_body) with, on one more workgroup each,
-// R = rho1 (rho1^2 + eps)^{-1} (withR) and the active-block list of the 16-row contraction from the
-// activity flags act (list: nGp / 64 + 1 ints; null for none)
-void launch_rhs_slice(const double2* rho1, int nO, double reg, double2* reginv, const double* STt,
-                      const unsigned long long* smax, int nGp, int Pc, int ns, int8_t* out, int* expo,
-                      hipStream_t st, bool withR = true, const int* act = nullptr, int* list = nullptr);
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
                     const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
                     const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st);
@@ -164,13 +157,17 @@ int rhs_post_energy_parts(int Pc, int nO);
 void launch_rhs_post(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
                      double2* h1, const double2* rho1, const double2* rho2, double* epart, const double2* orbs,
                      const double2* draw, const double2* part, int nPart, int nG, int nO, double2* dorbs,
-                     const double2* hsub, hipStream_t st);
+                     const double2* hsub, hipStream_t st, const double2* corrc = nullptr,
+                     const double2* reginv = nullptr);
 size_t ci_density_lds_bytes(int nC, int nO);
 int ci_kernels_allow_lds(size_t bytes);
-// pair sources S^T (fragment-tiled STt, optionally row-major ST), per-line maxima (smax, optional)
-// and per-fragment activity flags (act, optional: Pc/16 x nGp/16 ints, 1 = a non-zero value)
+// pair sources S^T (fragment-tiled STt, optionally row-major ST) and per-fragment activity flags
+// (act, optional: Pc/16 x nGp/16 ints, 1 = a non-zero value); with
+// Si8 set, the int8 form's instead: S^T, its digits Si8, the per-line and 64-point-block exponents
+// eSb (nGp/64 x Pc) and the per-block activity flags actK (Pc/16 x nGp/64)
 void launch_pair_src(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc,
-                     double* ST, double* STt, unsigned long long* smax, int* act, hipStream_t st);
+                     double* ST, double* STt, int* act, hipStream_t st,
+                     int8_t* Si8 = nullptr, int* eSb = nullptr, int* actK = nullptr);
 // row-major S^T alone (the operand of the split-FP32, BF16 and untiled int8 forms)
 void launch_pair_src_rows(const double2* orbs, const double* w, int nG, int nGp, int nO, int Pc, double* ST,
                           hipStream_t st);
@@ -204,22 +201,23 @@ struct OrbCorrArgs {
   const double2* orbs;  // phi, nO x nG
   const double* w;      // FEDVR weights
   const double2* hphi;  // (T + V) phi
-  const double2* r2can;   // canonical rho2 (a < b, m < l), np x np
-  const double2* reginv;  // R = rho1 (rho1^2 + eps)^-1
+  const double2* r2can;  // canonical rho2 (a < b, m < l), np x np
   int nG, mode;
-  double2* draw;        // output: h phi + correlation, nO x nG
-  double2* part;        // output: overlap partials, one nO^2 block per 16-point tile
+  double2* corrc;  // output: c_m(x) (the correlation before R), nO x nG
+  double2* part;   // output: overlap partials, two nO^2 blocks per 16-point tile (h phi, then c)
 };
-int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int nO,
-                     unsigned long long* smax, const int* list, const OrbCorrArgs& o, hipStream_t st);
-// after launch_mf_i8_orb: the Gram partials (as launch_rhs_mid) and the fixed-order sum of the nPart
-// overlap partials into partr (nO^2) on one more workgroup
+int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
+                     int nGp, int nO, const OrbCorrArgs& o, hipStream_t st);
+// after launch_mf_i8_orb: the Gram partials (as launch_rhs_mid), the fixed-order sums of the
+// contraction's two sets of nPart overlap partials into partr (2 nO^2; one workgroup each) and
+// R = rho1 (rho1^2 + eps)^-1 (one wave)
 void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs, int nO,
-                     const double2* part, int nPart, double2* partr, hipStream_t st);
+                     const double2* part, int nPart, double2* partr, const double2* rho1, double reg,
+                     double2* reginv, hipStream_t st);
 int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_gram(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
-int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int Pc,
-                     unsigned long long* smax, const int* list, hipStream_t st);
+int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
+                     int nGp, int Pc, hipStream_t st);
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,
                   int Pc, hipStream_t st);
 void launch_mf_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl, double* WT, int nGp,

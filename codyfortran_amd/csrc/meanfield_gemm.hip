@@ -194,100 +194,119 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, 
 
 // ---------------------------------------------------------------------------------------- int8, 16-row
 // The int8-sliced contraction for the right-hand side: one workgroup per 16-row tile of W (nGp / 16
-// workgroups) and one wave per 16-column tile t, all waves walking the same k blocks: the active-block
-// list (block_list_wave), or all blocks when the list is null.  Both operands are fragment-tiled
-// (k_slice_rows with tiled output, slice_s_tiled_body): slice p of the 16-line x 64-k block (R, K) is
-// 1 KiB at ((R nK64 + K) NS + p) 1024, lane l holding bytes k = 64K + 16(l/16) .. +15 of line
-// 16R + l%16.  Per block a wave loads the five V slice fragments of its rows (the same for every
-// wave of the workgroup) and the five S slice fragments of its columns, PF - 1 blocks ahead (loads
-// past the last block are clamped onto it; loads complete in issue order).  Products with p + q <= NS - 1
-// that share the scale 254^{-(p+q+2)} accumulate exactly in one int32 tile per diagonal over the
-// whole k range (|sum| <= NS 127^2 nGp < 2^31), issued diagonal-interleaved so that consecutive
-// MFMAs write different tiles.  Each diagonal is recombined once in FP64 with the row and column
-// exponents and W^T is written directly: no k-split partials, no summing launch, no cross-wave
-// reduction.  (Measured at 4096 points with 18 of 64 blocks active: two blocks in flight beat three
-// or four, and sharing the V fragments through LDS with a barrier per block is slower.)
-// mf_i8_r16_acc: the k loop of the workgroup's row tile rt into this wave's five int32 diagonal
-// tiles; mf_i8_r16_w: their FP64 recombination, rows 4(l/16) + u of column l%16 of the wave's tile.
+// workgroups) and one wave per 16-column tile t, all waves walking the same k blocks of 64 points:
+// the active ones (a block is active when some line's pair source in it is non-zero, per-block
+// flags from pair_src_i8_body), or all blocks when the flags are null.  Both operands are
+// fragment-tiled: digit p of the 16-line x 64-k block (R, K) is 1 KiB at ((R nK64 + K) NS + p) 1024,
+// lane l holding bytes k = 64K + 16(l/16) .. +15 of line 16R + l%16.  V's digits carry one scale per
+// row (eA), S's one per line and block (eSb[K Pc + line]).  Per block a wave loads the five V digit
+// fragments of its rows (the same for every wave of the workgroup) and the five S digit fragments
+// of its columns, one block ahead (loads complete in issue order; loads past the last block are
+// clamped onto it).  Products with p + q <= NS - 1 that share the scale 254^{-(p+q+2)} accumulate
+// exactly in one int32 tile per diagonal over the block (|sum| <= NS 127^2 64); the block's
+// diagonals are recombined in FP64 with the block's scale while the next block's MFMAs run (two
+// tile sets), and the blocks are added in list order.  (Measured at 4096 points with 18 of 64
+// blocks active: two blocks in flight beat three or four; sharing the V fragments through LDS with
+// a barrier per block is slower.)
+//
+// mf_i8_r16_acc: that loop for the workgroup's row tile rt; v[u] is then W at row 4(l/16) + u,
+// column l%16 of the wave's tile.
 template <int NT, int PF>
-__device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__ As, const int8_t* __restrict__ Bs,
-                                              int nGp, unsigned long long* __restrict__ smax,
-                                              const int* __restrict__ list, i4_t (&acc)[kInt8Slices]) {
+__device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
+                                              const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
+                                              const int* __restrict__ actK, int nGp, int Pc, double (&v)[4]) {
+  static_assert(PF == 2, "the loop below keeps one block in flight");
   constexpr int NS = kInt8Slices;
   __shared__ int sList[kR16MaxBlocks64];
-  // the per-line maxima of S^T were consumed by the slicing (earlier on this stream): clear them
-  // for the next right-hand side
-  if (smax && rt == 0 && threadIdx.x < NT * 16) smax[threadIdx.x] = 0ull;
+  __shared__ int sCount;
   const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
   const int nK64 = nGp >> 6;
-  const int total = list ? __builtin_amdgcn_readfirstlane(list[0]) : nK64;
-  if (list) {
-    for (int j = threadIdx.x; j < total; j += NT * 64) sList[j] = list[1 + j];
+  if (actK) {  // the active blocks, in increasing order (wave 0)
+    if (threadIdx.x < 64) {
+      const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+      int n = 0;
+      for (int base = 0; base < nK64; base += 64) {
+        const int b = base + lane;
+        int any = 0;
+        if (b < nK64)
+          for (int tt = 0; tt < NT; ++tt) any |= actK[(size_t)tt * nK64 + b];
+        const unsigned long long m = __ballot(any != 0);
+        if (any) sList[n + __popcll(m & below)] = b;
+        n += __popcll(m);
+      }
+      if (lane == 0) sCount = n;
+    }
     __syncthreads();
   }
+  const int total = actK ? __builtin_amdgcn_readfirstlane(sCount) : nK64;
   const int8_t* ab = As + (size_t)rt * nK64 * NS * 1024 + 16 * lane;
   const int8_t* bb = Bs + (size_t)t * nK64 * NS * 1024 + 16 * lane;
+  const int line = t * 16 + rr;
 #pragma unroll
-  for (int d = 0; d < NS; ++d) acc[d] = (i4_t){0, 0, 0, 0};
-  i4_t a[PF][NS], b[PF][NS];
+  for (int u = 0; u < 4; ++u) v[u] = 0.0;
+  i4_t a[2][NS], b[2][NS], acc[2][NS];
+  int es[2];
+  double sc[2];
   auto load = [&](int u, int j) {
     j = min(j, total - 1);
-    const int K = list ? __builtin_amdgcn_readfirstlane(sList[j]) : j;
+    const int K = actK ? __builtin_amdgcn_readfirstlane(sList[j]) : j;
     const size_t o = (size_t)K * NS * 1024;
 #pragma unroll
     for (int p = 0; p < NS; ++p) a[u][p] = *reinterpret_cast<const i4_t*>(ab + o + p * 1024);
 #pragma unroll
     for (int p = 0; p < NS; ++p) b[u][p] = *reinterpret_cast<const i4_t*>(bb + o + p * 1024);
+    es[u] = eSb[(size_t)K * Pc + line];
   };
   auto mfma = [&](int u) {
+#pragma unroll
+    for (int d = 0; d < NS; ++d) acc[u][d] = (i4_t){0, 0, 0, 0};
 #pragma unroll
     for (int p = 0; p < NS; ++p)
 #pragma unroll
       for (int d = p; d < NS; ++d)
-        acc[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][p], b[u][d - p], acc[d], 0, 0, 0);
+        acc[u][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][p], b[u][d - p], acc[u][d], 0, 0, 0);
+    sc[u] = ldexp(1.0, es[u]);
+  };
+  auto flush = [&](int u) {  // the block in tile set u, scaled by its line-and-block factor
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double bs = 0.0;
+#pragma unroll
+      for (int d = NS - 1; d >= 0; --d) bs += static_cast<double>(acc[u][d][k]) * kSliceScale.v[d + 2];
+      v[k] = fma(bs, sc[u], v[k]);
+    }
   };
   if (total > 0) {
-#pragma unroll
-    for (int u = 0; u < PF - 1; ++u) load(u, u);
-    for (int j = 0; j < total; j += PF) {
-#pragma unroll
-      for (int u = 0; u < PF; ++u) {
-        load((u + PF - 1) % PF, j + u + PF - 1);
-        if (j + u < total) mfma(u);
+    load(0, 0);
+    for (int j = 0; j < total; j += 2) {
+      load(1, j + 1);
+      mfma(0);              // block j
+      if (j > 0) flush(1);  // block j - 1
+      if (j + 1 < total) {
+        load(0, j + 2);
+        mfma(1);   // block j + 1
+        flush(0);  // block j
       }
     }
+    if ((total - 1) & 1)  // the last block (tile sets indexed by constants only: no scratch copy)
+      flush(1);
+    else
+      flush(0);
   }
-}
-
-template <int NT>
-__device__ __forceinline__ void mf_i8_r16_w(int rt, const i4_t (&acc)[kInt8Slices], const int* __restrict__ eA,
-                                            const int* __restrict__ fB, double (&v)[4]) {
-  constexpr int NS = kInt8Slices;
-  const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  // 32-bit MFMA results: element u of lane l is row 4(l/16) + u, column l%16
   const int row0 = rt * 16 + 4 * q;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) v[u] = 0.0;
-#pragma unroll
-  for (int d = NS - 1; d >= 0; --d)
-#pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] += static_cast<double>(acc[d][u]) * kSliceScale.v[d + 2];
-  const int fc = fB[t * 16 + rr];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) v[u] = ldexp(v[u], eA[row0 + u] + fc);
+  for (int u = 0; u < 4; ++u) v[u] = ldexp(v[u], eA[row0 + u]);
 }
 
 // The body takes the workgroup's row tile rt.
 template <int NT, int PF>
 __device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
-                                               const int8_t* __restrict__ Bs, const int* __restrict__ fB, int nGp,
-                                               double* __restrict__ WT, unsigned long long* __restrict__ smax,
-                                               const int* __restrict__ list) {
-  i4_t acc[kInt8Slices];
-  mf_i8_r16_acc<NT, PF>(rt, As, Bs, nGp, smax, list, acc);
+                                               const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
+                                               const int* __restrict__ actK, int nGp, int Pc, double* __restrict__ WT) {
   double v[4];
-  mf_i8_r16_w<NT>(rt, acc, eA, fB, v);
+  mf_i8_r16_acc<NT, PF>(rt, As, eA, Bs, eSb, actK, nGp, Pc, v);
   const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double* dst = WT + (size_t)(t * 16 + (lane & 15)) * nGp + rt * 16 + 4 * (lane >> 4);
@@ -307,15 +326,16 @@ __device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict_
 // (2 NO real) x (2 np NO real) product, run on FP64 MFMA 16x16x4 with the complex values split into
 // real and imaginary parts along k and along the columns, its k steps shared out over the waves and
 // the wave tiles added in LDS in fixed order.  rho2(a, b, m, l) = sign(m, l) rho2can[pair(a, b),
-// pair(m, l)] is a signed copy of the canonical rho2, so nothing waits for R but the last 8 products.
-// Then draw_j = h phi_j + corr_j (h phi only in the modes whose right-hand side has the one-body
-// part) and this tile's overlap partial inp(i, j) = sum_x conj(phi_i) w d_j over its points, at
-// part[rt NO^2 + i + NO j] (the partials are summed in fixed order by a later launch).
+// pair(m, l)] is a signed copy of the canonical rho2.  R is applied later (by the projection,
+// project_corr_body), so this kernel does not wait for it: it writes c_m(x) and this tile's two
+// overlap partials, A(i, j) = sum_x conj(phi_i) w h_j and B(i, m) = sum_x conj(phi_i) w c_m, at
+// part[rt NO^2 + i + NO j] and part[(nGp / 16 + rt) NO^2 + i + NO m] (h phi only in the modes whose
+// right-hand side has the one-body part).
 template <int NO, int PF>
 __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
-                                               const int8_t* __restrict__ Bs, const int* __restrict__ fB, int nGp,
-                                               double* __restrict__ WT, unsigned long long* __restrict__ smax,
-                                               const int* __restrict__ list, const OrbCorrArgs& o) {
+                                               const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
+                                               const int* __restrict__ actK, int nGp, double* __restrict__ WT,
+                                               const OrbCorrArgs& o) {
   constexpr int NT = (NO * (NO + 1) + 15) / 16, NTH = NT * 64, n2 = NO * NO, np = NO * (NO - 1) / 2;
   constexpr int NE = np * NO;                    // B terms per point (a < b, l)
   constexpr int NEP = NE > 0 ? NE : 1;
@@ -326,7 +346,7 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
   __shared__ double sBre[NEP][16], sBim[NEP][16];  // B_{e}(x), e = pair(a, b) + np l
   __shared__ double2 sRho[NR > 0 ? NR : 1];         // rho2(a < b, m, l) at e NO + m
   __shared__ double sC[NT][16][17];                 // the waves' c tiles: row x, column 2m + re/im
-  __shared__ double2 sPhi[NO][16], sD[NO][16], sRm[n2];
+  __shared__ double2 sPhi[NO][16], sH[NO][16], sCm[NO][16];
   __shared__ double sw[16];
   __shared__ int sAB[np > 0 ? np : 1];              // a + 16 b of strict pair index pab
   const int tid = threadIdx.x, x0 = rt * 16;
@@ -352,7 +372,6 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
   const double2 hL = (in && (o.mode == kModeFull || o.mode == kModeNonLinConsistent))
                          ? o.hphi[(size_t)pj * o.nG + gx] : cmk(0, 0);
   const double wL = (tid < 16 && x0 + tid < o.nG) ? o.w[x0 + tid] : 0.0;
-  const double2 rmL = tid < n2 ? o.reginv[tid] : cmk(0, 0);
   if (tid < np) {
     int a = 0, r = tid;
     while (r >= NO - 1 - a) {
@@ -361,10 +380,8 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
     }
     sAB[tid] = a + 16 * (a + 1 + r);
   }
-  i4_t acc[kInt8Slices];
-  mf_i8_r16_acc<NT, PF>(rt, As, Bs, nGp, smax, list, acc);
   double v[4];
-  mf_i8_r16_w<NT>(rt, acc, eA, fB, v);
+  mf_i8_r16_acc<NT, PF>(rt, As, eA, Bs, eSb, actK, nGp, NT * 16, v);
   const int t = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63, q = lane >> 4, rr = lane & 15;
   double* dst = WT + (size_t)(t * 16 + rr) * nGp + x0 + 4 * q;
@@ -380,10 +397,9 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
   }
   if (own) {
     sPhi[pj][px] = phiL;
-    sD[pj][px] = hL;
+    sH[pj][px] = hL;
   }
   if (tid < 16) sw[tid] = wL;
-  if (tid < n2) sRm[tid] = rmL;
   __syncthreads();
   // B terms: thread (x, l) from phi(x) and the column l of M(x), both in registers
   if constexpr (np > 0) {
@@ -393,8 +409,8 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
 #pragma unroll
       for (int k = 0; k < NO; ++k) {
         ph[k] = sPhi[k][x];
-        const int p = l <= k ? pair_index(NO, l, k) : pair_index(NO, k, l);
-        const double2 wv = cmk(sW[2 * p][x], sW[2 * p + 1][x]);
+        const int pp = l <= k ? pair_index(NO, l, k) : pair_index(NO, k, l);
+        const double2 wv = cmk(sW[2 * pp][x], sW[2 * pp + 1][x]);
         ml[k] = l <= k ? wv : cconj(wv);
       }
 #pragma unroll
@@ -432,52 +448,44 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
     for (int i = 0; i < 4; ++i) sC[t][4 * i + q][rr] = cacc[i];
   }
   __syncthreads();
-  if (own) {
-    const int x = px, j = pj;
-    double2 corr = cmk(0, 0);
-#pragma unroll
-    for (int mm = 0; mm < NO; ++mm) {
-      double cr = sC[0][x][2 * mm], ci = sC[0][x][2 * mm + 1];
-      for (int w = 1; w < NT; ++w) {
-        cr += sC[w][x][2 * mm];
-        ci += sC[w][x][2 * mm + 1];
-      }
-      cfma(corr, sRm[mm + j * NO], cmk(cr, ci));
+  if (own) {  // c_m(x), m = pj, summed over the waves in order
+    const int x = px, mm = pj;
+    double cr = sC[0][x][2 * mm], ci = sC[0][x][2 * mm + 1];
+    for (int w2 = 1; w2 < NT; ++w2) {
+      cr += sC[w2][x][2 * mm];
+      ci += sC[w2][x][2 * mm + 1];
     }
-    const double2 d = cadd(sD[j][x], corr);
-    sD[j][x] = d;
-    if (in) o.draw[(size_t)j * o.nG + gx] = d;
+    const double2 cv = cmk(cr, ci);
+    sCm[mm][x] = cv;
+    if (in) o.corrc[(size_t)mm * o.nG + gx] = cv;
   }
   __syncthreads();
-  for (int e = tid; e < n2; e += NTH) {
-    const int i = e % NO, j = e / NO;
-    double2 s = cmk(0, 0);
+  for (int e = tid; e < 2 * n2; e += NTH) {
+    const int half = e / n2, f = e % n2, i = f % NO, j = f / NO;
+    double2 sacc = cmk(0, 0);
 #pragma unroll
-    for (int x = 0; x < 16; ++x) cfmac(s, sPhi[i][x], cscale(sD[j][x], sw[x]));
-    o.part[(size_t)rt * n2 + e] = s;
+    for (int x = 0; x < 16; ++x) cfmac(sacc, sPhi[i][x], cscale(half ? sCm[j][x] : sH[j][x], sw[x]));
+    o.part[((size_t)half * (nGp >> 4) + rt) * n2 + f] = sacc;
   }
 }
 
 template <int NO>
 __global__ __launch_bounds__(((NO * (NO + 1) + 15) / 16) * 64) void k_mf_i8_orb(
     const int8_t* __restrict__ As, const int* __restrict__ eA, const int8_t* __restrict__ Bs,
-    const int* __restrict__ fB, int nGp, double* __restrict__ WT, unsigned long long* __restrict__ smax,
-    const int* __restrict__ list, OrbCorrArgs o) {
-  mf_i8_orb_body<NO, kR16Prefetch>(blockIdx.x, As, eA, Bs, fB, nGp, WT, smax, list, o);
+    const int* __restrict__ eSb, const int* __restrict__ actK, int nGp, double* __restrict__ WT, OrbCorrArgs o) {
+  mf_i8_orb_body<NO, kR16Prefetch>(blockIdx.x, As, eA, Bs, eSb, actK, nGp, WT, o);
 }
 
 template <int NT>
 __global__ __launch_bounds__(NT * 64) void k_mf_i8_r16(const int8_t* __restrict__ As, const int* __restrict__ eA,
-                                                        const int8_t* __restrict__ Bs, const int* __restrict__ fB,
-                                                        int nGp, double* __restrict__ WT,
-                                                        unsigned long long* __restrict__ smax,
-                                                        const int* __restrict__ list) {
-  mf_i8_r16_body<NT, kR16Prefetch>(blockIdx.x, As, eA, Bs, fB, nGp, WT, smax, list);
+                                                        const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
+                                                        const int* __restrict__ actK, int nGp, int Pc,
+                                                        double* __restrict__ WT) {
+  mf_i8_r16_body<NT, kR16Prefetch>(blockIdx.x, As, eA, Bs, eSb, actK, nGp, Pc, WT);
 }
 
 // ---------------------------------------------------------------------------------------- int8 slices
-// NS slices per operand; all slice products with p + q <= NS - 1 (0-based) that share the scale
-// 254^{-(p+q+2)} accumulate into one int32 tile (exact: |sum| <= NS 127^2 K < 2^31 for K <= 4096).
+// (the untiled form for more than 80 pair columns)
 template <int NT, int NS>
 __global__ __launch_bounds__(256) void k_mf_i8(const int8_t* __restrict__ As, const int* __restrict__ eA,
                                                const int8_t* __restrict__ Bs, const int* __restrict__ fB,
@@ -758,7 +766,7 @@ __global__ __launch_bounds__(NT * 64) void k_grp_mf_i8_r16(const GroupInstance* 
                                                             const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
-  mf_i8_r16_body<NT, kR16Prefetch>(m.y, a.Vi8, a.eV, a.Si8, a.eS, a.nGp, a.WT, a.smax, a.actC ? a.blist : nullptr);
+  mf_i8_r16_body<NT, kR16Prefetch>(m.y, a.Vi8, a.eV, a.Si8, a.eSb, a.actC, a.nGp, a.Pc, a.WT);
 }
 
 int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
@@ -786,8 +794,8 @@ __global__ __launch_bounds__(((NO * (NO + 1) + 15) / 16) * 64) void k_grp_mf_i8_
                                                                                     const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
-  const OrbCorrArgs o{a.C + a.nC, a.w, a.hphi, a.r2can, a.reginv, a.nG, kModeFull, a.draw, a.part};
-  mf_i8_orb_body<NO, kR16Prefetch>(m.y, a.Vi8, a.eV, a.Si8, a.eS, a.nGp, a.WT, a.smax, a.actC ? a.blist : nullptr, o);
+  const OrbCorrArgs o{a.C + a.nC, a.w, a.hphi, a.r2can, a.nG, kModeFull, a.draw, a.part};
+  mf_i8_orb_body<NO, kR16Prefetch>(m.y, a.Vi8, a.eV, a.Si8, a.eSb, a.actC, a.nGp, a.WT, o);
 }
 
 int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
@@ -834,16 +842,16 @@ void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp
 }
 
 // The 16-row int8 contraction with the orbital correlation of each tile (nO <= 8, nGp / 64 <= 256):
-// W^T, draw and one overlap partial per 16-point tile (part: nGp / 16 blocks of nO^2).  Returns 0,
-// or CODY_ERR_ARG without launching anything outside these limits.
-int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int nO,
-                     unsigned long long* smax, const int* list, const OrbCorrArgs& o, hipStream_t st) {
+// W^T, c_m(x) and two overlap partials per 16-point tile (part: 2 nGp / 16 blocks of nO^2).  Returns
+// 0, or CODY_ERR_ARG without launching anything outside these limits.
+int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
+                     int nGp, int nO, const OrbCorrArgs& o, hipStream_t st) {
   if (nGp < 64 || nGp % 64 || nGp / 64 > kR16MaxBlocks64 || nO < 1 || nO > 8 || o.nG > nGp)
     return set_error(CODY_ERR_ARG, "k_mf_i8_orb: nGp must be a positive multiple of 64, at most 64 x 256, above "
                                    "nG, and nO in 1..8");
 #define CALL_OC(NO)                                                                                        \
   hipLaunchKernelGGL(k_mf_i8_orb<NO>, dim3(nGp / 16), dim3(((NO * (NO + 1) + 15) / 16) * 64), 0, st, As, eA, \
-                     Bs, fB, nGp, WT, smax, list, o)
+                     Bs, eSb, actK, nGp, WT, o)
   switch (nO) {
     case 1: CALL_OC(1); break;
     case 2: CALL_OC(2); break;
@@ -858,16 +866,16 @@ int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const in
   return 0;
 }
 
-// 16-row int8 contraction of the right-hand side (Pc <= 80, nGp / 64 <= 256): W^T
-// written directly.  list: the active k blocks (block_list_wave) or null for all blocks.  Returns 0,
-// or CODY_ERR_ARG without launching anything when the shape is outside these limits.
-int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp, int Pc,
-                     unsigned long long* smax, const int* list, hipStream_t st) {
+// 16-row int8 contraction (Pc <= 80, nGp / 64 <= 256): W^T written directly.  actK: the per-block
+// activity flags of S (pair_src_i8_body), or null to contract every k block.  Returns 0, or
+// CODY_ERR_ARG without launching anything when the shape is outside these limits.
+int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
+                     int nGp, int Pc, hipStream_t st) {
   if (nGp < 64 || nGp % 64 || Pc < 16 || Pc > 80 || Pc % 16 || nGp / 64 > kR16MaxBlocks64)
     return set_error(CODY_ERR_ARG, "k_mf_i8_r16: nGp must be a positive multiple of 64, at most 64 x 256, "
                                    "and Pc one of 16, 32, 48, 64, 80");
 #define CALL_R16(NT) \
-  hipLaunchKernelGGL(k_mf_i8_r16<NT>, dim3(nGp / 16), dim3((NT) * 64), 0, st, As, eA, Bs, fB, nGp, WT, smax, list)
+  hipLaunchKernelGGL(k_mf_i8_r16<NT>, dim3(nGp / 16), dim3((NT) * 64), 0, st, As, eA, Bs, eSb, actK, nGp, Pc, WT)
   switch (Pc / 16) {
     case 1: CALL_R16(1); break;
     case 2: CALL_R16(2); break;

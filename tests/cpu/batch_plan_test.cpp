@@ -57,17 +57,17 @@ int main() {
     EXPECT(check_plan_shape(s).empty(), "headline shape refused: %s", check_plan_shape(s).c_str());
     int64_t c[kPlanKinds];
     plan_block_counts(s, c);
-    const int64_t want[kPlanKinds] = {212 + 128 + 1280, 0, 256, 641, 0, 160 + 512, 26 + 16, 18};
+    const int64_t want[kPlanKinds] = {212 + 128 + 1280, 256, 641, 0, 160 + 512, 26 + 16, 18};
     for (int k = 0; k < kPlanKinds; ++k) EXPECT(c[k] == want[k], "kind %d: %lld blocks, expected %lld", k, (long long)c[k], (long long)want[k]);
-    // the int8 path: slicing with R and the active-block list (2 + 80 x 1024 / 256), one
-    // contraction workgroup per 16-row tile writing W^T (no k-split sum, R not in the sum launch)
+    // the int8 path: pair sources with their digits per 16 x 64 block (80 x 4096 / 1024), one
+    // contraction workgroup per 16-row tile writing W^T (no k-split sum, R in the Gram launch)
     const PlanShape s8 = shape(8, 70, 4096, 1, true);
     plan_block_counts(s8, c);
-    EXPECT(c[kPlanSlice] == 2 + 320, "int8 slicing blocks %lld", (long long)c[kPlanSlice]);
+    EXPECT(c[kPlanFront] == 212 + 128 + 320, "int8 front blocks %lld", (long long)c[kPlanFront]);
     EXPECT(c[kPlanContract] == 256, "int8 contraction blocks %lld", (long long)c[kPlanContract]);
     EXPECT(c[kPlanWsum] == 0, "int8 sum blocks %lld", (long long)c[kPlanWsum]);
-    // the correlation is formed by the contraction: 5 x 32 Gram + one partial-sum workgroup
-    EXPECT(c[kPlanMid] == 160 + 1, "int8 Gram blocks %lld", (long long)c[kPlanMid]);
+    // the correlation is formed by the contraction: 5 x 32 Gram + two partial-sum workgroups + R
+    EXPECT(c[kPlanMid] == 160 + 3, "int8 Gram blocks %lld", (long long)c[kPlanMid]);
     // 12 orbitals: rho2' (12^4 entries, one per thread) is formed once per right-hand side
     plan_block_counts(shape(12, 924, 4096, 4, false), c);
     EXPECT(c[kPlanRho2p] == 81, "rho2' blocks %lld", (long long)c[kPlanRho2p]);
