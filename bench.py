@@ -757,10 +757,12 @@ def main():
         else:
             eb = OPERAND_BYTES[prec]
             work = eb * nGp * cols + eb * Pc * cols + 8.0 * Pc * nGp  # V and S operands once, W^T once
-            if prec == "int8_sliced":  # the epilogue's correlation: phi, h phi read, draw written once,
-                # one overlap partial per 16-point tile, the canonical rho2 and R
+            if prec == "int8_sliced":  # S's exponents (one per line and 64-point block); the epilogue's
+                # correlation: phi, h phi read, c written once, two overlap partials per 16-point tile,
+                # the canonical rho2
                 npr = wl.M * (wl.M - 1) // 2
-                work += 3 * 16.0 * wl.M * wl.nG + 16.0 * wl.M ** 2 * (nGp // 16) + 16.0 * (npr * npr + wl.M ** 2)
+                work += 4.0 * Pc * cols / 64
+                work += 3 * 16.0 * wl.M * wl.nG + 2 * 16.0 * wl.M ** 2 * (nGp // 16) + 16.0 * npr * npr
             unit = "GB/s"
             ach = work / (kms * 1e-3) / 1e9 if kms else None
         traffic = pmc_traffic(kname, wl.name, skip and prec in SKIPPING)
