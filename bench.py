@@ -542,20 +542,31 @@ def batched_measurement(args, torch, wlm, check: bool):
         return max(state_rel(o.cpu().numpy().view(np.complex128), r, w.nC, wts, w.M)
                    for (ctx, wts), w, o, r in zip(ctxs, ws, d_o, refs))
 
+    ms_default = timed()
+    pick, cand_ms = b.autotune(ps, po)
     ms = timed()
     flops = sum(2.0 * w.nG ** 2 * 2 * w.pairs for w in ws)
     out = {"instances": len(ws), "value": len(ws) * 1000.0 / ms, "unit": "instance RHS evals/s",
            "ms_per_batched_rhs": ms, "contraction_flop_per_batched_rhs": flops,
            "contraction_tflops_aggregate": flops / (ms * 1e-3) / 1e12, "dtype": "f64",
-           "k_split_by_shape": splits, "schedule": "one HIP graph, 4 parallel chains"}
+           "k_split_by_shape": splits,
+           "schedule": {"form": "one HIP graph, 4 parallel chains, grouped launches per orbital count",
+                        "ms_before_schedule_autotune": ms_default,
+                        "autotuned": {"chains": hip.Batch.SCHEDULES[pick][0],
+                                      "split_threshold": hip.Batch.SCHEDULES[pick][1],
+                                      "candidates_ms": {f"{c}x{t}": round(m, 4) for (c, t), m
+                                                        in zip(hip.Batch.SCHEDULES, cand_ms)}}}}
     if check:
         out["max_rhs_rel_err_vs_fp64_oracle"] = worst_err()
     # the same batch with the int8-sliced contraction in every instance
     for ctx, _ in ctxs:
         ctx.set_precision(hip.PREC_INT8_SLICED)
+    pick8, cand8 = b.autotune(ps, po)
     ms8 = timed()
     out["int8_sliced"] = {"value": len(ws) * 1000.0 / ms8, "ms_per_batched_rhs": ms8,
-                          "dtype": "f64 (int8_sliced contraction)"}
+                          "dtype": "f64 (int8_sliced contraction)",
+                          "schedule": {"chains": hip.Batch.SCHEDULES[pick8][0],
+                                       "split_threshold": hip.Batch.SCHEDULES[pick8][1]}}
     if check:
         out["int8_sliced"]["max_rhs_rel_err_vs_fp64_oracle"] = worst_err()
     b.close()

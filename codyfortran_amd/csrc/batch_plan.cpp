@@ -56,10 +56,14 @@ void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]) {
   // the int8 contraction: one workgroup per 16-row tile; the FP64 one: 64-row groups x k ranges
   cnt[kPlanContract] = s.int8 ? (int64_t)(s.nGp / 16) : (int64_t)(s.nGp / 64) * s.nsplit;
   cnt[kPlanWsum] = wsumFirst + (s.nsplit > 1 ? (plane / 2 + 255) / 256 : 0);
-  cnt[kPlanRho2p] = s.nO >= kPlanRho2pMinOrbs ? (n2 * n2 + 255) / 256 : 0;
-  // Gram + orbital correlation; the int8 path's contraction forms the correlation, and beside the
-  // Gram two workgroups sum its two sets of overlap partials and one forms R
-  cnt[kPlanMid] = (int64_t)(s.Pc / 16) * s.nxs + (s.int8 ? 3 : (int64_t)(s.nGp / 64) * s.nO);
+  const bool corrQ = !s.int8 && s.nO >= kPlanCorrQMinOrbs;
+  const int64_t nGram = (int64_t)(s.Pc / 16) * s.nxs;
+  // Q (four doubles per thread) beside the Gram partials
+  cnt[kPlanQform] = corrQ ? (plan_corr_q_doubles(s.nO) / 4 + 255) / 256 + nGram : 0;
+  // Gram + orbital correlation per 64 points and orbital, or the correlation on FP64 MFMA per 64
+  // points and orbital group; the int8 path's contraction forms the correlation, and beside the Gram
+  // two workgroups sum its two sets of overlap partials and one forms R
+  cnt[kPlanMid] = s.int8 ? nGram + 3 : (corrQ ? (int64_t)(s.nGp / 64) * kPlanCorrQGroups : nGram + (int64_t)(s.nGp / 64) * s.nO);
   cnt[kPlanPost] = nRed + nObx;
   cnt[kPlanApply] = (s.nC + 3) / 4;
 }

@@ -27,11 +27,14 @@ constexpr int kPlanF64MaxTiles = 10;       // 16-column tiles of the FP64 contra
 constexpr int kPlanRdmWideConf = 256, kPlanRdmWideItems = 4;
 
 // The kernel types of a right-hand side, in launch order.
-enum PlanKind { kPlanFront, kPlanContract, kPlanWsum, kPlanRho2p, kPlanMid, kPlanPost, kPlanApply, kPlanKinds };
+enum PlanKind { kPlanFront, kPlanContract, kPlanWsum, kPlanQform, kPlanMid, kPlanPost, kPlanApply, kPlanKinds };
 
-// Orbital counts from which a grouped right-hand side forms rho2' = rho2 x_3 R once per instance
-// (kPlanRho2p) instead of in every orbital-correlation workgroup.
-constexpr int kPlanRho2pMinOrbs = 10;
+// Orbital counts from which the FP64 path runs the orbital correlation on FP64 MFMA: one workgroup
+// per 64 points and each of kPlanCorrQGroups orbital groups, after a launch (kPlanQform) that forms
+// the rho2'-weighted matrix Q of plan_corr_q_doubles(nO) doubles, four per thread, beside the Gram
+// partials.
+constexpr int kPlanCorrQMinOrbs = 8, kPlanCorrQGroups = 4;
+constexpr int plan_corr_q_doubles(int nO) { return nO * ((2 * nO + 15) / 16) * ((nO * (nO + 1) + 3) / 4) * 64; }
 
 // The sizes an instance's launches depend on.
 struct PlanShape {
@@ -48,7 +51,7 @@ struct PlanShape {
 // Workgroups of each kernel type of one instance's right-hand side: the grids of its own launches
 // (front: CI densities + one-body part + pair sources, on the int8 path with S's digits; the
 // contraction, with the orbital correlation on the int8 path; the k-split sum with R (FP64 path);
-// rho2' for nO >= kPlanRho2pMinOrbs; Gram + orbital correlation, or on the int8 path Gram + the sums
+// Q for nO >= kPlanCorrQMinOrbs (FP64 path); Gram + orbital correlation, or on the int8 path Gram + the sums
 // of the overlap partials + R; G reduction + orbital projection; CI apply).  The shape must have
 // passed check_plan_shape.
 void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]);

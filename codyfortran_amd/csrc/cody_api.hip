@@ -239,11 +239,12 @@ int cody_create(cody_ctx** out, int device, int nG, int nBodies, int nOrbs, doub
       c->nsplit = s;
       break;
     }
-  // split-K factor of the Gram matrix G = S^T W (nGp / (64 nxs) must be an integer)
-  const int k64 = c->nGp / 64;
+  // split-K factor of the Gram matrix G = S^T W (nGp / (64 nxs) must be an integer); at most 8
+  // above 80 pair columns, where each workgroup's 16 x Pc tile has enough MFMA work per k range
+  const int k64 = c->nGp / 64, maxXs = c->Pc > 80 ? 8 : 32;
   c->nxs = 1;
   for (int s : {32, 16, 8, 4, 2, 1})
-    if (k64 % s == 0) {
+    if (s <= maxXs && k64 % s == 0) {
       c->nxs = s;
       break;
     }
@@ -497,8 +498,13 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   } else {
     launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st);
     mark("k_rhs_wsum");
+    if (c->nO >= kCorrQMinOrbs) {
+      launch_rhs_qform(c->nO, c->d_r2can, c->d_reginv, c->d_qf, c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs,
+                       st);
+      mark("k_rhs_qform");
+    }
     launch_rhs_mid(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, orbs, c->d_w, c->d_hphi, c->d_r2can,
-                   c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st);
+                   c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st, c->d_qf);
     mark("k_rhs_mid");
   }
   // (the int8 projection: h phi where the mode has the one-body part, c_m in draw, the summed partials)
@@ -979,14 +985,25 @@ int cody_profile_rhs(cody_ctx* c, int iters, int nMax, double* avg_ms, char* nam
 
 // ---------------------------------------------------------------- batches of instances
 static_assert(kPlanRdmWideConf == kRdmWideConf && kPlanRdmWideItems == kRdmWideItems, "CI density blocks agree");
-static_assert(kPlanMaxOrbs == kMaxOrbs && kPlanInt8MaxBlocks64 == kR16MaxBlocks64 && kPlanRho2pMinOrbs == kRho2pMinOrbs,
+static_assert(kPlanMaxOrbs == kMaxOrbs && kPlanInt8MaxBlocks64 == kR16MaxBlocks64 && kPlanCorrQMinOrbs == kCorrQMinOrbs &&
+                  kPlanCorrQGroups == kCorrQGroups,
               "the host planner and the device kernels agree on their limits");
 static_assert(sizeof(BlockRef) == sizeof(int2), "block references are uploaded as int2");
+static_assert(plan_corr_q_doubles(8) == corr_q_doubles(8) && plan_corr_q_doubles(9) == corr_q_doubles(9) &&
+                  plan_corr_q_doubles(10) == corr_q_doubles(10) && plan_corr_q_doubles(11) == corr_q_doubles(11) &&
+                  plan_corr_q_doubles(12) == corr_q_doubles(12),
+              "the host planner and the device kernels agree on the size of Q");
 namespace {
 
 // Device time of one right-hand side, for the assignment of work to chains: the contraction at
 // ~50 TFLOP/s plus ~50 us of latency-bound launches.
 double rhs_cost_estimate(const cody_ctx* c) { return 2.0 * c->nGp * (double)c->nGp * c->Pc / 50e6 + 50.0; }
+
+// Cost of instance i for the schedule: its measured right-hand side (cody_batch_autotune) or the
+// estimate.
+double batch_cost(const cody_batch* b, int i) {
+  return i < (int)b->cost.size() ? b->cost[i] : rhs_cost_estimate(b->ctx[i]);
+}
 
 // Partitions the batch: instances on the FP64 or the 16-row int8 contraction are grouped by
 // (orbital count, contraction), each group getting its argument table and block maps in device
@@ -1016,24 +1033,22 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
     }
     grp->members.push_back(i);
   }
-  // a group whose estimated cost exceeds a chain's share of the batch is split into two groups of
-  // about equal cost (members longest first, each to the lighter half), so that its launches run on
-  // two chains side by side instead of setting the batch's critical path alone
+  // a group whose cost exceeds split_share times a chain's share of the batch is split into two
+  // groups of about equal cost (members longest first, each to the lighter half), so that its
+  // launches run on two chains side by side instead of setting the batch's critical path alone
   {
     double total = 0.0;
     for (const auto& g : b->groups)
-      for (int i : g.members) total += rhs_cost_estimate(b->ctx[i]);
-    for (int i : b->singles) total += rhs_cost_estimate(b->ctx[i]);
-    const double share = total / cody_batch::kChains;
-    for (size_t gi0 = 0; gi0 < b->groups.size() && b->groups.size() < 2 * cody_batch::kChains; ++gi0) {
+      for (int i : g.members) total += batch_cost(b, i);
+    for (int i : b->singles) total += batch_cost(b, i);
+    const double share = b->split_share * total / b->nchains;
+    for (size_t gi0 = 0; gi0 < b->groups.size() && b->groups.size() < 3 * cody_batch::kChains; ++gi0) {
       BatchGroup& g = b->groups[gi0];
       double cost = 0.0;
-      for (int i : g.members) cost += rhs_cost_estimate(b->ctx[i]);
+      for (int i : g.members) cost += batch_cost(b, i);
       if (cost <= share || g.members.size() < 2) continue;
       std::vector<int> mem = g.members;
-      std::stable_sort(mem.begin(), mem.end(), [&](int a, int c) {
-        return rhs_cost_estimate(b->ctx[a]) > rhs_cost_estimate(b->ctx[c]);
-      });
+      std::stable_sort(mem.begin(), mem.end(), [&](int a, int c) { return batch_cost(b, a) > batch_cost(b, c); });
       BatchGroup h;
       h.nO = g.nO;
       h.Pc = g.Pc;
@@ -1041,7 +1056,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       std::vector<int> keep;
       double ck = 0.0, ch = 0.0;
       for (int i : mem) {
-        const double ci = rhs_cost_estimate(b->ctx[i]);
+        const double ci = batch_cost(b, i);
         if (ck <= ch) {
           keep.push_back(i);
           ck += ci;
@@ -1124,9 +1139,8 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       sh.int8 = g.int8;
       shapes.push_back(sh);
       g.maxC = std::max(g.maxC, c->nC);
-      g.cost += rhs_cost_estimate(c) - 40.0;  // (the launches' latency is shared by the group)
+      g.cost += batch_cost(b, i);
     }
-    g.cost += 40.0;
     // the block maps, built and checked entry by entry on the host before any launch
     GroupPlan plan;
     const std::string why = build_group_plan(shapes, plan);
@@ -1135,10 +1149,10 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       g.off[k] = static_cast<int>(plan.off[k]);
       g.cnt[k] = static_cast<int>(plan.cnt[k]);
     }
-    if (nO >= kRho2pMinOrbs) {  // rho2' of every member, formed once per right-hand side
-      const size_t n4 = (size_t)n2 * n2;
-      HIPCHK(hipMalloc((void**)&g.d_r2p, sizeof(double2) * n4 * gi.size()));
-      for (size_t m = 0; m < gi.size(); ++m) gi[m].r2p = g.d_r2p + m * n4;
+    if (nO >= kCorrQMinOrbs && !g.int8) {  // Q of every member's orbital correlation
+      const size_t nq = (size_t)corr_q_doubles(nO);
+      HIPCHK(hipMalloc((void**)&g.d_qf, sizeof(double) * nq * gi.size()));
+      for (size_t m = 0; m < gi.size(); ++m) gi[m].qf = g.d_qf + m * nq;
     }
     HIPCHK(hipMalloc((void**)&g.d_inst, sizeof(GroupInstance) * gi.size()));
     HIPCHK(hipMalloc((void**)&g.d_map, sizeof(BlockRef) * plan.map.size()));
@@ -1158,7 +1172,7 @@ int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
     CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   }
   launch_grp_wsum(g.nO, g.d_inst, map(kPlanWsum), g.cnt[kPlanWsum], st);
-  launch_grp_rho2p(g.nO, g.d_inst, map(kPlanRho2p), g.cnt[kPlanRho2p], st);
+  launch_grp_qform(g.nO, g.d_inst, map(kPlanQform), g.cnt[kPlanQform], st);
   if (g.int8)
     launch_grp_gram(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
   else
@@ -1231,9 +1245,9 @@ int cody_batch_time_derivative_dev(cody_batch* b, void* const* d_states, void* c
     // the chains too; groups and instances by longest-first greedy assignment of an estimated cost
     std::vector<std::pair<double, int>> items;  // (cost, group index g >= 0, or -1 - instance)
     for (int g = 0; g < (int)b->groups.size(); ++g) items.emplace_back(b->groups[g].cost, g);
-    for (int i : b->singles) items.emplace_back(rhs_cost_estimate(b->ctx[i]), -1 - i);
+    for (int i : b->singles) items.emplace_back(batch_cost(b, i), -1 - i);
     std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& c) { return a.first > c.first; });
-    std::vector<double> load(cody_batch::kChains, 0.0);
+    std::vector<double> load(b->nchains, 0.0);
     HIPCHK(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
     hipEventRecord(b->fork, b->cs);
     for (int k = 0; k < cody_batch::kChains; ++k) hipStreamWaitEvent(b->chain[k], b->fork, 0);
@@ -1271,6 +1285,84 @@ int cody_batch_time_derivative_dev(cody_batch* b, void* const* d_states, void* c
   for (int i = 0; i < n; ++i)
     if (b->ctx[i]->nLaser > 0) launch_set_scalar(b->ctx[i]->d_efield, laser_field(b->ctx[i], t), st);
   HIPCHK(hipGraphLaunch(b->exec, st));
+  return CODY_OK;
+}
+
+int cody_batch_autotune(cody_batch* b, void* const* d_states, void* const* d_dstates, double* cand_ms,
+                        int max_cand, int* pick_out) {
+  if (!b) return fail(CODY_ERR_ARG, "cody_batch_autotune: null batch");
+  // validates the arguments and evaluates once with the current schedule
+  CHK(cody_batch_time_derivative_dev(b, d_states, d_dstates, 0.0, nullptr));
+  const int n = static_cast<int>(b->ctx.size());
+  hipStream_t st = b->cs;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) hipEventDestroy(e0);
+    return fail(CODY_ERR_HIP, "cody_batch_autotune: event creation failed");
+  }
+  auto timed = [&](auto&& body, int reps, double* ms_out) -> int {
+    CHK(body());  // warm-up
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) CHK(body());
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_out = ms / reps;
+    return CODY_OK;
+  };
+  int r = CODY_OK;
+  // each instance's own right-hand side, in microseconds
+  std::vector<double> cost(n, 0.0);
+  for (int i = 0; i < n && r == CODY_OK; ++i) {
+    cody_ctx* c = b->ctx[i];
+    double ms = 0.0;
+    r = timed([&] { return run_rhs(c, static_cast<const double2*>(d_states[i]), static_cast<double2*>(d_dstates[i]), 0.0,
+                                   kModeFull, st); }, 3, &ms);
+    cost[i] = 1000.0 * ms;
+  }
+  struct Cand {
+    int chains;
+    double share;
+  };
+  const Cand cand[] = {{4, 1.0}, {4, 0.7}, {4, 0.5}, {2, 1.0}, {2, 0.5}, {1, 1.0}};
+  static_assert(cody_batch::kChains >= 4, "the candidates use up to four chains");
+  constexpr int nCand = sizeof(cand) / sizeof(cand[0]);
+  double best = 1e300;
+  int pick = 0;
+  if (r == CODY_OK) b->cost = cost;
+  for (int k = 0; k < nCand && r == CODY_OK; ++k) {
+    b->nchains = cand[k].chains;
+    b->split_share = cand[k].share;
+    b->key.clear();  // (rebuilds the groups and the graph on the next call)
+    double ms = 0.0;
+    r = timed([&] { return cody_batch_time_derivative_dev(b, d_states, d_dstates, 0.0, st); }, 5, &ms);
+    if (r == CODY_OK && cand_ms && k < max_cand) cand_ms[k] = ms;
+    if (r == CODY_OK && ms < best) {
+      best = ms;
+      pick = k;
+    }
+  }
+  if (r == CODY_OK) {
+    b->nchains = cand[pick].chains;
+    b->split_share = cand[pick].share;
+    b->key.clear();
+    r = cody_batch_time_derivative_dev(b, d_states, d_dstates, 0.0, st);
+    if (r == CODY_OK && hipStreamSynchronize(st) != hipSuccess) r = fail(CODY_ERR_HIP, "cody_batch_autotune: sync failed");
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (r != CODY_OK) return r;
+  if (pick_out) *pick_out = pick;
+  return CODY_OK;
+}
+
+int cody_batch_set_schedule(cody_batch* b, int chains, double split_threshold) {
+  if (!b || chains < 1 || chains > cody_batch::kChains || !(split_threshold > 0.0))
+    return fail(CODY_ERR_ARG, "cody_batch_set_schedule: bad arguments");
+  b->nchains = chains;
+  b->split_share = split_threshold;
+  b->key.clear();
   return CODY_OK;
 }
 

@@ -159,6 +159,7 @@ struct cody_ctx {
   double2* d_r2can = nullptr;  // canonical rho2Lin (i1 < i2, j1 < j2)
   double2 *d_h1 = nullptr, *d_hphi = nullptr, *d_dCraw = nullptr;
   double2 *d_draw = nullptr, *d_part = nullptr, *d_part1 = nullptr;
+  double* d_qf = nullptr;  // Q of the orbital correlation (nO >= kCorrQMinOrbs)
   double2* d_partr = nullptr;  // the int8 path's two sets of overlap partials summed (2 nO^2)
   double* d_zeros = nullptr;  // nG + 1 zeros: a zero potential and a zero field (the kinetic piece)
   double2* d_src = nullptr;   // interaction sources of cody_interaction_potential (nG x P, on use)
@@ -215,6 +216,7 @@ struct cody_ctx {
     // the fused int8 contraction
     ALLOC(d_part, sizeof(double2) * std::max(nPart, 2 * (nGp / 16)) * n2);
     ALLOC(d_partr, sizeof(double2) * 2 * n2);
+    if (nO >= cody::kCorrQMinOrbs) ALLOC(d_qf, sizeof(double) * cody::corr_q_doubles(nO));
     ALLOC(d_part1, sizeof(double2) * ((nG + 255) / 256) * n2);
     ALLOC(d_zeros, sizeof(double) * (nG + 1));
     // CI tables in one blob: ptr arrays then entry arrays
@@ -279,7 +281,7 @@ struct cody_ctx {
     void* ptrs[] = {d_x,   d_w,    d_vext,  d_kref, d_V,     d_ST,    d_WT, d_STt, d_Wpart, d_act,
                     d_Gpart, d_G,  d_energy, d_epart, d_efield, d_state, d_tmp, d_tmp2, d_rho1, d_rho2,
                     d_reginv, d_r2can, d_h1, d_hphi, d_dCraw, d_draw, d_part, d_part1, d_zeros, d_src, d_h2, ci_blob,
-                    d_partr};
+                    d_partr, d_qf};
     for (void* p : ptrs)
       if (p) hipFree(p);
     prop.release();
@@ -301,7 +303,7 @@ struct BatchGroup {
   double cost = 0.0;  // estimated device time (for the assignment to chains)
   std::vector<int> members;
   cody::GroupInstance* d_inst = nullptr;
-  double2* d_r2p = nullptr;  // rho2' of the members (orbital counts from kRho2pMinOrbs)
+  double* d_qf = nullptr;  // Q of the members' orbital correlation (orbital counts from kCorrQMinOrbs)
   int2* d_map = nullptr;  // the kinds' maps back to back
   int off[cody::kPlanKinds] = {}, cnt[cody::kPlanKinds] = {};  // each kernel type's map
 };
@@ -318,11 +320,16 @@ struct cody_batch {
   hipGraphExec_t exec = nullptr;
   std::vector<BatchGroup> groups;  // grouped instances
   std::vector<int> singles;        // instances evaluated by their own launches
+  // schedule: measured device time of each instance's own right-hand side (empty: the estimate
+  // is used) and the split threshold, in chain shares, above which a group is split further
+  std::vector<double> cost;
+  double split_share = 1.0;
+  int nchains = kChains;  // chains the schedule uses (1..kChains)
   void release_groups() {
     for (auto& g : groups) {
       if (g.d_inst) hipFree(g.d_inst);
       if (g.d_map) hipFree(g.d_map);
-      if (g.d_r2p) hipFree(g.d_r2p);
+      if (g.d_qf) hipFree(g.d_qf);
     }
     groups.clear();
     singles.clear();

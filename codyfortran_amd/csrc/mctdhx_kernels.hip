@@ -360,18 +360,15 @@ __device__ __forceinline__ double2 rho2p_entry(int e, const double2* s2, const d
   return acc;
 }
 
-// With PRE the slice rho2'(:, :, j, :) is read from r2p (all slices, formed once per right-hand
-// side by k_grp_rho2p) instead of being formed from the canonical rho2 in every workgroup: the
-// workgroup then holds no copy of rho2 (for NO = 12 that is 70 KB of LDS and 20k products).
-template <int NO, bool PRE = false>
+template <int NO>
 __device__ __forceinline__ void orbital_body(int bx, int j, const double2* __restrict__ orbs, const double* __restrict__ w,
                                                  const double2* __restrict__ hphi, const double* __restrict__ WT,
                                                  const double2* __restrict__ r2can, const double2* __restrict__ reginv,
                                                  int nG, int nGp, int mode, double2* __restrict__ draw,
-                                                 double2* __restrict__ part, const double2* __restrict__ r2p = nullptr) {
+                                                 double2* __restrict__ part) {
   constexpr int n2 = NO * NO, n3 = n2 * NO, NP = NO * (NO + 1) / 2, PW = (NP + 3) / 4;
   constexpr int np = NO * (NO - 1) / 2, nCan = np * np;
-  __shared__ double2 s2[(nCan > 0 && !PRE) ? nCan : 1];
+  __shared__ double2 s2[nCan > 0 ? nCan : 1];
   __shared__ double2 sR[n3];
   __shared__ double2 sRj[NO];
   __shared__ double2 sCorr[4][64];
@@ -387,21 +384,7 @@ __device__ __forceinline__ void orbital_body(int bx, int j, const double2* __res
     }
     sLK[threadIdx.x] = l + 256 * (l + r);
   }
-  if constexpr (PRE) {
-    constexpr int kStage = (n3 + 255) / 256;
-    const double2* src = r2p + (size_t)j * n3;
-    double2 v[kStage];
-#pragma unroll
-    for (int u = 0; u < kStage; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      v[u] = e < n3 ? src[e] : cmk(0, 0);
-    }
-#pragma unroll
-    for (int u = 0; u < kStage; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      if (e < n3) sR[e] = v[u];
-    }
-  } else {
+  {
     // canonical rho2 (i1 < i2, j1 < j2) and column j of R (each thread's loads of the canonical
     // rho2 in flight together), then the slice
     constexpr int kStage = nCan > 0 ? (nCan + 255) / 256 : 1;
@@ -485,6 +468,172 @@ __global__ __launch_bounds__(256) void k_orbital(const double2* __restrict__ orb
                                                  double2* __restrict__ part) {
   orbital_body<NO>(blockIdx.x, blockIdx.y, orbs, w, hphi, WT, r2can, reginv, nG, nGp, mode, draw, part);
 }
+
+// ------------------------------------------------------------------ orbital correlation on FP64 MFMA
+// From kCorrQMinOrbs orbitals the correlation of orbital_body (nO^4 products per point, each reading
+// an LDS broadcast) runs as a product on FP64 MFMA followed by a short contraction with phi:
+//   corr_j(x)   = sum_k1 phi_k1(x) O_{k1 j}(x),
+//   O_{k1 j}(x) = sum_p a_p(x) (X + Y) + i b_p(x) (X - Y),
+// where W_p = a_p + i b_p for the pair p = {l <= k}, X = rho2'(k1, k, j, l) and Y = rho2'(k1, l, j, k)
+// (zero for k = l), since W_p X + conj(W_p) Y = a_p (X + Y) + i b_p (X - Y) (ApplyCorrelationOp,
+// S_Method_Mb_OrbBased.f90:762-838).  O is the real product of the 16-point block of W^T (rows
+// 2p + re/im, as the contraction writes them) with Q (2P x 2 nO^2 real): row 2p + 0 holds the re/im
+// parts of X + Y, row 2p + 1 those of i (X - Y).  Q's columns come in 16-column tiles, tile
+// t = j TPJ + th holding re/im of O_{k1 j} for k1 = 8 th .. 8 th + 7; Q is stored in the order the
+// MFMA's B operand is read (lane l of step s of tile t: row 4 s + l / 16, column l % 16, at
+// ((t KS + s) 64 + l)), formed once per right-hand side from the canonical rho2 and R.
+// One thread per (pair slot p < 2 KS, k1 < 8 TPJ, j): X and Y once, the four entries of rows 2p, 2p + 1
+// and columns re/im of (k1, j) written as two pairs of neighbours (zeros for the padding slots).
+template <int NO>
+__device__ __forceinline__ void qform_body(int blk, const double2* __restrict__ r2can,
+                                           const double2* __restrict__ reginv, double* __restrict__ qf) {
+  constexpr int KS = corr_q_steps(NO), TPJ = corr_q_tiles_per_orb(NO), NP = NO * (NO + 1) / 2, K1 = 8 * TPJ;
+  const int e = blk * 256 + threadIdx.x;
+  if (e >= 2 * KS * K1 * NO) return;
+  const int p = e % (2 * KS), k1 = (e / (2 * KS)) % K1, j = e / (2 * KS * K1);
+  double2 X = cmk(0, 0), Y = cmk(0, 0);
+  if (p < NP && k1 < NO) {
+    int l = 0, r = p;  // pair p -> (l <= k)
+    while (r >= NO - l) {
+      r -= NO - l;
+      ++l;
+    }
+    const int k = l + r;
+    double2 Rj[NO];
+#pragma unroll
+    for (int m = 0; m < NO; ++m) Rj[m] = reginv[m + j * NO];
+    X = rho2p_entry<NO>(k1 + NO * (k + NO * l), r2can, Rj);
+    if (k != l) Y = rho2p_entry<NO>(k1 + NO * (l + NO * k), r2can, Rj);
+  }
+  // row 2p (a_p): re/im of X + Y; row 2p + 1 (b_p): re/im of i (X - Y)
+  const int t = j * TPJ + (k1 >> 3), s = p >> 1, n = 2 * (k1 & 7);
+  double* base = qf + ((size_t)t * KS + s) * 64 + 16 * (2 * (p & 1)) + n;
+  *reinterpret_cast<double2*>(base) = cmk(X.x + Y.x, X.y + Y.y);
+  *reinterpret_cast<double2*>(base + 16) = cmk(-(X.y - Y.y), X.x - Y.x);
+}
+
+// One workgroup per 64 points (bx) and group g of the orbitals j (j = g, g + G, ..., G =
+// kCorrQGroups), one wave per 16 of the points: the wave holds its W^T block as the MFMA A operand
+// in registers (KS values per lane) and walks its orbitals' Q tiles with Q's fragments loaded one
+// tile ahead (loads complete in issue order, so every wait covers one tile of MFMAs).  The result
+// of tile t (element i of lane l: point 4 i + l / 16, column l % 16) is contracted with phi_k1 of
+// its column in registers and the 16 columns of a lane group are added by a fixed butterfly.  Then
+// draw_j = h phi_j + corr_j (h phi in the modes with the one-body part) and the per-workgroup
+// partial sums part[bx][i + j nO] = sum_x conj(phi_i) w draw_j of its orbitals j, as orbital_body
+// leaves them.
+template <int NO>
+__device__ __forceinline__ void corr_q_body(int bx, int g, const double2* __restrict__ orbs,
+                                            const double* __restrict__ w, const double2* __restrict__ hphi,
+                                            const double* __restrict__ WT, const double* __restrict__ qf, int nG,
+                                            int nGp, int mode, double2* __restrict__ draw,
+                                            double2* __restrict__ part) {
+  constexpr int n2 = NO * NO, NP = NO * (NO + 1) / 2, G = kCorrQGroups;
+  constexpr int KS = corr_q_steps(NO), TPJ = corr_q_tiles_per_orb(NO), JPG = (NO + G - 1) / G;
+  __shared__ double2 sCorr[JPG][64];
+  __shared__ double2 sPhi[NO][64];
+  __shared__ double sw[64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = lane >> 4, n = lane & 15;
+  const int x0 = bx * 64, xw = x0 + wave * 16;
+  const int nj = (NO - g + G - 1) / G;  // orbitals j = g + G u of this group, u < nj
+  double a[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int kk = 4 * s + q;
+    a[s] = kk < 2 * NP ? WT[(size_t)kk * nGp + xw + n] : 0.0;
+  }
+  const double* qp = qf + lane;
+  double qb[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) qb[s] = qp[((size_t)g * TPJ * KS + s) * 64];
+  double pr[TPJ][4], pim[TPJ][4];
+#pragma unroll
+  for (int th = 0; th < TPJ; ++th)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k1 = 8 * th + (n >> 1), gx = xw + 4 * i + q;
+      const double2 v = (k1 < NO && gx < nG) ? orbs[(size_t)k1 * nG + gx] : cmk(0, 0);
+      pr[th][i] = v.x;
+      pim[th][i] = v.y;
+    }
+  for (int e = tid; e < NO * 64; e += 256) {
+    const int i = e >> 6, xx = e & 63;
+    sPhi[i][xx] = x0 + xx < nG ? orbs[(size_t)i * nG + x0 + xx] : cmk(0, 0);
+  }
+  if (tid < 64) sw[tid] = x0 + tid < nG ? w[x0 + tid] : 0.0;
+  const bool reCol = (n & 1) == 0;
+  for (int u = 0; u < nj; ++u) {
+    const int j = g + G * u;
+    double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int th = 0; th < TPJ; ++th) {
+      // the tile after this one in the wave's walk (the last one reloads itself, unused)
+      const int tn = th + 1 < TPJ ? j * TPJ + th + 1 : (u + 1 < nj ? (j + G) * TPJ : j * TPJ + th);
+      d4_t acc = (d4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const double qv = qb[s];
+        qb[s] = qp[((size_t)tn * KS + s) * 64];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], qv, acc, 0, 0, 0);
+      }
+      // phi_k1 (re + i im) at this lane's column: the re column adds phi re, the im column i phi im
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        re[i] = fma(reCol ? pr[th][i] : -pim[th][i], acc[i], re[i]);
+        im[i] = fma(reCol ? pim[th][i] : pr[th][i], acc[i], im[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        re[i] += __shfl_xor(re[i], off, 16);
+        im[i] += __shfl_xor(im[i], off, 16);
+      }
+    if (n < 4) {
+      const double rv = n == 0 ? re[0] : n == 1 ? re[1] : n == 2 ? re[2] : re[3];
+      const double iv = n == 0 ? im[0] : n == 1 ? im[1] : n == 2 ? im[2] : im[3];
+      sCorr[u][wave * 16 + 4 * n + q] = cmk(rv, iv);
+    }
+  }
+  __syncthreads();
+  const int xx = tid & 63, gx = x0 + xx;
+  const bool in = gx < nG, hOn = mode == kModeFull || mode == kModeNonLinConsistent;
+  for (int u = tid >> 6; u < nj; u += 4) {
+    const int j = g + G * u;
+    const double2 hj = (in && hOn) ? hphi[(size_t)j * nG + gx] : cmk(0, 0);
+    const double2 d = cadd(hj, sCorr[u][xx]);
+    if (in) draw[(size_t)j * nG + gx] = d;
+    sCorr[u][xx] = cscale(d, sw[xx]);
+  }
+  __syncthreads();
+  for (int f = tid; f < NO * nj; f += 256) {
+    const int i = f % NO, u = f / NO;
+    double2 acc = cmk(0, 0);
+#pragma unroll 8
+    for (int x = 0; x < 64; ++x) cfmac(acc, sPhi[i][x], sCorr[u][x]);
+    part[(size_t)bx * n2 + i + NO * (g + G * u)] = acc;
+  }
+}
+
+// Q (the first corr_q_blocks workgroups) beside the Gram matrix's split-K partials (G = S^T W, the
+// others), which the correlation launch then leaves out.
+template <int NO>
+__global__ __launch_bounds__(256) void k_rhs_qform(const double2* __restrict__ r2can, const double2* __restrict__ reginv,
+                                                   double* __restrict__ qf, const double* __restrict__ STt,
+                                                   const double* __restrict__ WT, int nGp, int Pc,
+                                                   double* __restrict__ Gpart, int nxs) {
+  if constexpr (NO >= kCorrQMinOrbs) {
+    constexpr int NT = (NO * (NO + 1) + 15) / 16, nQ = corr_q_blocks(NO);
+    const int b = blockIdx.x;
+    if (b < nQ) {
+      qform_body<NO>(b, r2can, reginv, qf);
+      return;
+    }
+    gram_f64_body<NT>((b - nQ) % NT, (b - nQ) / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs),
+                      (size_t)Pc * Pc);
+  }
+}
+
 
 // ------------------------------------------------------------------ overlap sums
 // inp(i, j) = sum_b part[b][i + j NO] over nPart per-workgroup partials, into sInp (LDS): G groups
@@ -797,6 +946,16 @@ void launch_xmat(const double2* orbs, const double* w, const double* x, int nG, 
 #undef CALL_XM
 }
 
+void launch_rhs_qform(int nO, const double2* r2can, const double2* reginv, double* qf, const double* STt,
+                      const double* WT, int nGp, int Pc, double* Gpart, int nxs, hipStream_t st) {
+  const unsigned grid = (unsigned)(corr_q_blocks(nO) + Pc / 16 * nxs);
+#define CALL_QF(NO)                                                                                          \
+  hipLaunchKernelGGL(k_rhs_qform<NO>, dim3(grid), dim3(256), 0, st, r2can, reginv, qf, STt, WT, nGp, Pc, Gpart, \
+                     nxs)
+  CODY_NO_SWITCH(nO, CALL_QF)
+#undef CALL_QF
+}
+
 void launch_orbital(const double2* orbs, const double* w, const double2* hphi, const double* WT,
                     const double2* r2can, const double2* reginv, int nG, int nGp, int nO, int mode,
                     double2* draw, double2* part, hipStream_t st) {
@@ -880,17 +1039,23 @@ __global__ __launch_bounds__(256) void k_rhs_mid(const double* __restrict__ STt,
                                                  const double2* __restrict__ orbs, const double* __restrict__ w,
                                                  const double2* __restrict__ hphi, const double2* __restrict__ r2can,
                                                  const double2* __restrict__ reginv, int nG, int mode,
-                                                 double2* __restrict__ draw, double2* __restrict__ part) {
+                                                 double2* __restrict__ draw, double2* __restrict__ part,
+                                                 const double* __restrict__ qf) {
   constexpr int NT = (NO * (NO + 1) + 15) / 16;  // = Pc / 16
-  const int nGram = NT * nxs;
   int b = blockIdx.x;
-  if (b < nGram) {
-    gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc);
-    return;
+  if constexpr (NO >= kCorrQMinOrbs) {  // (the Gram partials come from the Q launch)
+    const int nOrbX = nGp / 64;
+    corr_q_body<NO>(b % nOrbX, b / nOrbX, orbs, w, hphi, WT, qf, nG, nGp, mode, draw, part);
+  } else {
+    const int nGram = NT * nxs;
+    if (b < nGram) {
+      gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc);
+      return;
+    }
+    b -= nGram;
+    const int nOrbX = nGp / 64;
+    orbital_body<NO>(b % nOrbX, b / nOrbX, orbs, w, hphi, WT, r2can, reginv, nG, nGp, mode, draw, part);
   }
-  b -= nGram;
-  const int nOrbX = nGp / 64;
-  orbital_body<NO>(b % nOrbX, b / nOrbX, orbs, w, hphi, WT, r2can, reginv, nG, nGp, mode, draw, part);
 }
 
 // After the fused int8 contraction (which formed c_m(x) and its overlap partials): the Gram
@@ -994,20 +1159,21 @@ __global__ __launch_bounds__(256) void k_grp_wsum(const GroupInstance* __restric
   wsum_body(m.y - a.wsumFirst, a.Wpart, a.nsplit, (size_t)a.Pc * a.nGp, a.WT);
 }
 
-// rho2'(k1, k, j, l) = sum_m rho2(k1, k, m, l) R(m, j) for all j, at j NO^3 + k1 + NO (k + NO l),
-// one entry per thread (after R): the slices the orbital correlation of nO >= 10 reads.
+// Q of the orbital correlation (nO >= kCorrQMinOrbs), after R: as k_rhs_qform.
 template <int NO>
-__global__ __launch_bounds__(256) void k_grp_rho2p(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
-  constexpr int n3 = NO * NO * NO;
-  const int2 m = map[blockIdx.x];
-  const GroupInstance& a = gi[m.x];
-  const int q = m.y * 256 + threadIdx.x;
-  if (q >= NO * n3) return;
-  const int j = q / n3, e = q % n3;
-  double2 Rj[NO];
-#pragma unroll
-  for (int k = 0; k < NO; ++k) Rj[k] = a.reginv[k + j * NO];
-  a.r2p[q] = rho2p_entry<NO>(e, a.r2can, Rj);
+__global__ __launch_bounds__(256) void k_grp_qform(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  if constexpr (NO >= kCorrQMinOrbs) {
+    constexpr int NT = (NO * (NO + 1) + 15) / 16, nQ = corr_q_blocks(NO);
+    const int2 m = map[blockIdx.x];
+    const GroupInstance& a = gi[m.x];
+    if (m.y < nQ) {
+      qform_body<NO>(m.y, a.r2can, a.reginv, a.qf);
+      return;
+    }
+    const int b = m.y - nQ;
+    gram_f64_body<NT>(b % NT, b / NT, a.STt, a.nGp >> 4, a.WT, a.nGp, a.Gpart, a.Pc, a.nGp / (4 * a.nxs),
+                      (size_t)a.Pc * a.Pc);
+  }
 }
 
 template <int NO>
@@ -1015,18 +1181,22 @@ __global__ __launch_bounds__(256) void k_grp_mid(const GroupInstance* __restrict
   constexpr int NT = (NO * (NO + 1) + 15) / 16;
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
-  const int nGram = NT * a.nxs;
   int b = m.y;
-  if (b < nGram) {
-    gram_f64_body<NT>(b % NT, b / NT, a.STt, a.nGp >> 4, a.WT, a.nGp, a.Gpart, a.Pc, a.nGp / (4 * a.nxs),
-                      (size_t)a.Pc * a.Pc);
-    return;
+  if constexpr (NO >= kCorrQMinOrbs) {  // (the Gram partials come from the Q launch)
+    const int nOrbX = a.nGp / 64;
+    corr_q_body<NO>(b % nOrbX, b / nOrbX, a.C + a.nC, a.w, a.hphi, a.WT, a.qf, a.nG, a.nGp, kModeFull, a.draw, a.part);
+  } else {
+    const int nGram = NT * a.nxs;
+    if (b < nGram) {
+      gram_f64_body<NT>(b % NT, b / NT, a.STt, a.nGp >> 4, a.WT, a.nGp, a.Gpart, a.Pc, a.nGp / (4 * a.nxs),
+                        (size_t)a.Pc * a.Pc);
+      return;
+    }
+    b -= nGram;
+    const int nOrbX = a.nGp / 64;
+    orbital_body<NO>(b % nOrbX, b / nOrbX, a.C + a.nC, a.w, a.hphi, a.WT, a.r2can, a.reginv, a.nG, a.nGp, kModeFull,
+                     a.draw, a.part);
   }
-  b -= nGram;
-  const int nOrbX = a.nGp / 64;
-  constexpr bool kPre = NO >= kRho2pMinOrbs;
-  orbital_body<NO, kPre>(b % nOrbX, b / nOrbX, a.C + a.nC, a.w, a.hphi, a.WT, a.r2can, a.reginv, a.nG, a.nGp, kModeFull,
-                         a.draw, a.part, a.r2p);
 }
 
 template <int NO>
@@ -1064,9 +1234,9 @@ void launch_grp_wsum(int nO, const GroupInstance* gi, const int2* map, int nBloc
   CODY_NO_SWITCH(nO, CALL_GW)
 #undef CALL_GW
 }
-void launch_grp_rho2p(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+void launch_grp_qform(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
   if (nBlocks == 0) return;
-#define CALL_GR(NO) hipLaunchKernelGGL(k_grp_rho2p<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+#define CALL_GR(NO) hipLaunchKernelGGL(k_grp_qform<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
   CODY_NO_SWITCH(nO, CALL_GR)
 #undef CALL_GR
 }
@@ -1116,11 +1286,12 @@ void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, c
 
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
                     const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
-                    const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st) {
-  const unsigned grid = (unsigned)(Pc / 16 * nxs + nGp / 64 * nO);
+                    const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st,
+                    const double* qf) {
+  const unsigned grid = (unsigned)(nO >= kCorrQMinOrbs ? nGp / 64 * kCorrQGroups : Pc / 16 * nxs + nGp / 64 * nO);
 #define CALL_MI(NO)                                                                                             \
   hipLaunchKernelGGL(k_rhs_mid<NO>, dim3(grid), dim3(256), 0, st, STt, WT, nGp, Pc, Gpart, nxs, orbs, w, hphi, r2can, \
-                     reginv, nG, mode, draw, part)
+                     reginv, nG, mode, draw, part, qf)
   CODY_NO_SWITCH(nO, CALL_MI)
 #undef CALL_MI
 }

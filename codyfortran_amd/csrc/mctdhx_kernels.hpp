@@ -80,7 +80,7 @@ struct GroupInstance {
   int* actK;                 // int8 path: per-block activity flags (written by the front launch)
   const int* actC;           // the flags the contraction reads (null: no skipping)
   double2* partr;            // int8 path: the summed overlap partials (the projection reads these)
-  double2* r2p;              // rho2' (nO^4) for the orbital correlation, or null (formed in it)
+  double* qf;                // Q of the orbital correlation (nO >= kCorrQMinOrbs, FP64 path), or null
   const double* Vt;                          // FP64 contraction operand
   const int8_t* Vi8;                         // int8 contraction operands
   const int* eV;
@@ -91,7 +91,7 @@ struct GroupInstance {
 // Grouped launches over the instances of one orbital count nO (the map holds nBlocks entries)
 void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlocks, int maxC, hipStream_t st);
 void launch_grp_wsum(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
-void launch_grp_rho2p(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+void launch_grp_qform(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_mid(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_post(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_apply(const GroupInstance* gi, const int2* map, int nBlocks, int nO, hipStream_t st);
@@ -152,7 +152,8 @@ void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, c
                      size_t plane, double* WT, hipStream_t st, bool withReginv = true);
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
                     const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
-                    const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st);
+                    const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st,
+                    const double* qf = nullptr);
 int rhs_post_energy_parts(int Pc, int nO);
 void launch_rhs_post(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
                      double2* h1, const double2* rho1, const double2* rho2, double* epart, const double2* orbs,
@@ -183,7 +184,22 @@ void launch_slice_rows(const double* X, int len, int nlines, int ns, int tiled, 
 void launch_split_f32(const double* X, size_t n, float* hi, float* lo, hipStream_t st);
 void launch_split_bf16x3(const double* X, size_t n, __bf16* out, hipStream_t st);
 constexpr int kMaxWSplit = 8;  // largest k split of the FP64 contraction (partials buffer depth)
-constexpr int kRho2pMinOrbs = 10;  // grouped right-hand sides form rho2' once per instance from this many orbitals
+// From this many orbitals the FP64 right-hand side runs the orbital correlation as an FP64 MFMA
+// product of W (16 points x 2P real) with the rho2'-weighted matrix Q (2P x 2 nO^2 real), Q formed
+// once per right-hand side in MFMA fragment order (corr_q_body, qform_body).
+constexpr int kCorrQMinOrbs = 8;
+constexpr int kCorrQGroups = 4;  // orbital groups of the correlation: workgroups per 64 points
+// Q's k steps of 4 real rows (2P = nO (nO + 1) rows), its 16-column tiles per orbital j (columns
+// re/im of k1 = 0..7, then 8..15) and its size in doubles: tile t = j tiles-per-j + th, step s,
+// lane l at ((t KS + s) 64 + l)
+constexpr int corr_q_steps(int nO) { return (nO * (nO + 1) + 3) / 4; }
+constexpr int corr_q_tiles_per_orb(int nO) { return (2 * nO + 15) / 16; }
+constexpr int corr_q_doubles(int nO) { return nO * corr_q_tiles_per_orb(nO) * corr_q_steps(nO) * 64; }
+constexpr int corr_q_blocks(int nO) { return (corr_q_doubles(nO) / 4 + 255) / 256; }  // four doubles per thread
+// Q beside the Gram partials (which the correlation launch then leaves out): the launch between the
+// k-split sum (R) and launch_rhs_mid from kCorrQMinOrbs orbitals
+void launch_rhs_qform(int nO, const double2* r2can, const double2* reginv, double* qf, const double* STt,
+                      const double* WT, int nGp, int Pc, double* Gpart, int nxs, hipStream_t st);
 constexpr int kSkipMaxBlocks16 = 1024;  // longest grid (in 16-point blocks) whose zero k blocks are skipped
 // shape check of the k-split contraction launchers (0 or CODY_ERR_ARG, nothing launched)
 int check_contraction_shape(const char* who, int nGp, int Pc, int nsplit, int maxTiles, int kBlocks);

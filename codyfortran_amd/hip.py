@@ -65,6 +65,8 @@ def load():
         "cody_batch_create": (i, [C.POINTER(vp), i, C.POINTER(vp)]),
         "cody_batch_destroy": (i, [vp]),
         "cody_batch_time_derivative_dev": (i, [vp, C.POINTER(vp), C.POINTER(vp), d, vp]),
+        "cody_batch_autotune": (i, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(d), i, C.POINTER(C.c_int)]),
+        "cody_batch_set_schedule": (i, [vp, i, d]),
         "cody_upload_state": (i, [vp, vp]),
         "cody_download_state": (i, [vp, vp]),
         "cody_state_device_ptr": (i, [vp, C.POINTER(vp)]),
@@ -472,6 +474,26 @@ class Batch:
         s = (C.c_void_p * n)(*d_states)
         d = (C.c_void_p * n)(*d_dstates)
         _check(self.lib.cody_batch_time_derivative_dev(self.h, s, d, float(t), C.c_void_p(stream or None)))
+
+    # the candidate schedules of cody_batch_autotune: (parallel chains, split threshold)
+    SCHEDULES = [(4, 1.0), (4, 0.7), (4, 0.5), (2, 1.0), (2, 0.5), (1, 1.0)]
+
+    def autotune(self, d_states, d_dstates):
+        """Times the candidate schedules for these state pointers and keeps the fastest; returns
+        (index of the chosen schedule in SCHEDULES, ms per batched right-hand side of each
+        candidate).  d_dstates are overwritten with a right-hand side at t = 0."""
+        n = len(self.contexts)
+        assert len(d_states) == n and len(d_dstates) == n
+        s = (C.c_void_p * n)(*d_states)
+        d = (C.c_void_p * n)(*d_dstates)
+        k = len(self.SCHEDULES)
+        ms = (C.c_double * k)()
+        pick = C.c_int()
+        _check(self.lib.cody_batch_autotune(self.h, s, d, ms, k, C.byref(pick)))
+        return pick.value, list(ms)
+
+    def set_schedule(self, chains: int, split_threshold: float):
+        _check(self.lib.cody_batch_set_schedule(self.h, int(chains), float(split_threshold)))
 
     def close(self):
         if self.h:

@@ -298,6 +298,21 @@ int cody_batch_destroy(cody_batch* batch);
  * (0: the batch's own stream). */
 int cody_batch_time_derivative_dev(cody_batch* batch, void* const* d_states, void* const* d_dstates,
                                    double t, void* stream);
+/* Schedule autotuner of a batch (no counterpart in the reference): times each instance's own
+ * right-hand side once, then builds and times the batched graph for each candidate schedule and
+ * keeps the fastest for these state pointers.  A schedule is (parallel chains, split threshold):
+ * groups of one orbital count are split while a group's measured cost exceeds the threshold times
+ * a chain's share, and groups and the other instances go to the chains longest first.  Candidates,
+ * in order: (4, 1.0), (4, 0.7), (4, 0.5), (2, 1.0), (2, 0.5), (1, 1.0).  cand_ms (optional, max_cand
+ * entries) receives each candidate's time per batched right-hand side, *pick (optional) the index
+ * of the chosen one.  d_dstates are overwritten (one right-hand side at t = 0).  Results are
+ * bit-identical under every schedule (the grouped launches run the instances' own bodies). */
+int cody_batch_autotune(cody_batch* batch, void* const* d_states, void* const* d_dstates, double* cand_ms,
+                        int max_cand, int* pick);
+/* A fixed schedule instead of the autotuned one: parallel chains (1..4) and split threshold (> 0),
+ * with the measured costs of the last cody_batch_autotune (else estimates).  Takes effect on the
+ * next batched call. */
+int cody_batch_set_schedule(cody_batch* batch, int chains, double split_threshold);
 
 /* ---------------------------------------------------------------- measurement */
 /* Runs `iters` right-hand sides of the context's state and returns the average duration of each

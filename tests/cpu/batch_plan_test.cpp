@@ -49,15 +49,15 @@ int main() {
   // 1. known answer: the 4096-point system with 8 orbitals, 4 fermions (70 configurations), FP64
   //    contraction split in 4, as its own launches size it (launch_rhs_front: 212 CI-density
   //    blocks + 16 x 8 one-body blocks + 80 x 4096 / 256 pair-source blocks; k_mf_f64_w: 64 row
-  //    groups x 4; k_rhs_wsum: 1 + 80 x 4096 / 512; no separate rho2'; k_rhs_mid: 5 x 32 Gram +
-  //    64 x 8 orbital;
+  //    groups x 4; k_rhs_wsum: 1 + 80 x 4096 / 512; k_rhs_qform: Q (8 x 18 x 64 / 4 doubles / 256) +
+  //    5 x 32 Gram; k_rhs_mid: 64 x 4 correlation;
   //    k_rhs_post: 26 reduction + 16 projection; k_ci_apply_project: 18)
   {
     const PlanShape s = shape(8, 70, 4096, 4, false);
     EXPECT(check_plan_shape(s).empty(), "headline shape refused: %s", check_plan_shape(s).c_str());
     int64_t c[kPlanKinds];
     plan_block_counts(s, c);
-    const int64_t want[kPlanKinds] = {212 + 128 + 1280, 256, 641, 0, 160 + 512, 26 + 16, 18};
+    const int64_t want[kPlanKinds] = {212 + 128 + 1280, 256, 641, 9 + 160, 64 * 4, 26 + 16, 18};
     for (int k = 0; k < kPlanKinds; ++k) EXPECT(c[k] == want[k], "kind %d: %lld blocks, expected %lld", k, (long long)c[k], (long long)want[k]);
     // the int8 path: pair sources with their digits per 16 x 64 block (80 x 4096 / 1024), one
     // contraction workgroup per 16-row tile writing W^T (no k-split sum, R in the Gram launch)
@@ -68,9 +68,12 @@ int main() {
     EXPECT(c[kPlanWsum] == 0, "int8 sum blocks %lld", (long long)c[kPlanWsum]);
     // the correlation is formed by the contraction: 5 x 32 Gram + two partial-sum workgroups + R
     EXPECT(c[kPlanMid] == 160 + 3, "int8 Gram blocks %lld", (long long)c[kPlanMid]);
-    // 12 orbitals: rho2' (12^4 entries, one per thread) is formed once per right-hand side
+    // 12 orbitals: Q (24 tiles x 39 k steps x 64 doubles, four per thread) is formed once per
+    // right-hand side beside the Gram partials; the correlation takes one workgroup per 64 points and
+    // group of 3 orbitals
     plan_block_counts(shape(12, 924, 4096, 4, false), c);
-    EXPECT(c[kPlanRho2p] == 81, "rho2' blocks %lld", (long long)c[kPlanRho2p]);
+    EXPECT(c[kPlanQform] == 59 + 10 * 32, "Q + Gram blocks %lld", (long long)c[kPlanQform]);
+    EXPECT(c[kPlanMid] == 64 * 4, "12-orbital correlation blocks %lld", (long long)c[kPlanMid]);
     // 924 configurations: four CI density entries per wave, 16 per workgroup: (144 + 66^2) / 16
     // = 281.25 -> 282 blocks, then 16 x 12 one-body and 160 x 4096 / 256 pair-source blocks
     EXPECT(c[kPlanFront] == 282 + 192 + 2560, "12-orbital front blocks %lld", (long long)c[kPlanFront]);

@@ -132,9 +132,23 @@ def test_full_batch_matches_oracle():
             d_o.append(torch.zeros_like(d_s[-1]))
         b = hip.Batch([c for c, _ in ctxs])
         stream = torch.cuda.current_stream().cuda_stream
+        ps, po = [t.data_ptr() for t in d_s], [t.data_ptr() for t in d_o]
         for _ in range(2):
-            b.time_derivative_dev([t.data_ptr() for t in d_s], [t.data_ptr() for t in d_o], 0.0, stream)
+            b.time_derivative_dev(ps, po, 0.0, stream)
         torch.cuda.synchronize()
+        first = [o.cpu().numpy().copy() for o in d_o]
+        # the schedule autotuner keeps the results bit for bit (the grouped launches run the
+        # instances' own bodies, whichever chain or group they land in)
+        pick, ms = b.autotune(ps, po)
+        print(f"  batch autotune: schedule {hip.Batch.SCHEDULES[pick]}, ms per batched RHS by schedule "
+              + ", ".join(f"{m:.3f}" for m in ms))
+        assert 0 <= pick < len(ms) and all(m > 0 for m in ms) and ms[pick] == min(ms)
+        for o in d_o:
+            o.zero_()
+        b.time_derivative_dev(ps, po, 0.0, stream)
+        torch.cuda.synchronize()
+        for w, o, f in zip(ws, d_o, first):
+            assert np.array_equal(o.cpu().numpy(), f), w.name
         b.close()
         worst = 0.0
         for (ctx, wts), w, st, o in zip(ctxs, ws, states, d_o):
