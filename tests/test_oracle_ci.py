@@ -50,12 +50,15 @@ def test_rdm1_rdm2_match_jordan_wigner(nb, no):
         for j in range(nO):
             ref = np.vdot(psi, ad[i] @ a[j] @ psi) if bt[i] == bt[j] else 0.0
             assert abs(r1[i, j] - ref) < 1e-13
+    # <Psi| a+_i1 a+_i2 a_j2 a_j1 |Psi> = <a_i2 a_i1 Psi | a_j2 a_j1 Psi>, from the vectors
+    # a_j2 a_j1 Psi (matrix-vector products only)
+    aa = [[a[j2] @ (a[j1] @ psi) for j2 in range(nO)] for j1 in range(nO)]
     for i1 in range(nO):
         for i2 in range(nO):
             for j1 in range(nO):
                 for j2 in range(nO):
                     ok = (bt[i1] == bt[j1] and bt[i2] == bt[j2]) or (bt[i1] == bt[j2] and bt[i2] == bt[j1])
-                    ref = np.vdot(psi, ad[i1] @ ad[i2] @ a[j2] @ a[j1] @ psi) if ok else 0.0
+                    ref = np.vdot(aa[i1][i2], aa[j1][j2]) if ok else 0.0
                     assert abs(r2[i1, i2, j1, j2] - ref) < 1e-13, (i1, i2, j1, j2)
     # invariants: Tr rho1 = N, sum rho2_{ijij} = N(N-1)   (src/Utils/M_Utils_PrinterObservable.f90:52-62)
     N = sum(nb)
@@ -76,11 +79,17 @@ def test_h_apply_matches_jordan_wigner(nb, no):
     mask = same1[:, None, :, None] & same1[None, :, None, :]
     h2 = h2 * mask
     ad = [m.T for m in a]
-    H1 = sum(h1[q, p] * ad[p] @ a[q] for p in range(nO) for q in range(nO))
-    H2 = sum(h2[i1, i2, j1, j2] * ad[j1] @ ad[j2] @ a[i2] @ a[i1]
-             for i1 in range(nO) for i2 in range(nO) for j1 in range(nO) for j2 in range(nO))
-    ref1 = fock.restrict(H1 @ psi, codes, shifts)
-    ref2 = fock.restrict(H2 @ psi, codes, shifts)
+    # H1 Psi and H2 Psi by matrix-vector products: a_q Psi, a_i2 a_i1 Psi once, then the creators
+    aq = [a[q] @ psi for q in range(nO)]
+    h1psi = sum(h1[q, p] * (ad[p] @ aq[q]) for p in range(nO) for q in range(nO))
+    aa = [[a[i2] @ aq[i1] for i2 in range(nO)] for i1 in range(nO)]
+    h2psi = np.zeros_like(psi)
+    for j1 in range(nO):
+        for j2 in range(nO):
+            v = sum(h2[i1, i2, j1, j2] * aa[i1][i2] for i1 in range(nO) for i2 in range(nO))
+            h2psi = h2psi + ad[j1] @ (ad[j2] @ v)
+    ref1 = fock.restrict(h1psi, codes, shifts)
+    ref2 = fock.restrict(h2psi, codes, shifts)
     np.testing.assert_allclose(s.apply_h1(C, h1), ref1, atol=1e-12)
     np.testing.assert_allclose(s.apply_h2(C, h2), ref2, atol=1e-11)
 
