@@ -35,17 +35,20 @@ METRIC = "MCTDHF RHS evals/s (M=8, 4096 pts): split-FP32 MFMA vs FP64, rel err â
 # HBM delivers their operand bytes (int8: 15 digit products of 2 N_g^2 Pc ops take ~8 us at
 # 5.0 POP/s, the 84 MB of V digits ~10.5 us at 8 TB/s), so they are HBM-bound: algorithmic bytes
 # = V and S operands read once plus W^T written once, against 8 TB/s.
-BOUND = {"fp64": "mfma", "split_fp32": "mfma", "bf16x3": "hbm", "int8_sliced": "hbm"}
-PEAK = {"fp64": 78.6, "split_fp32": 157.3, "bf16x3": 8000.0, "int8_sliced": 8000.0}  # TFLOP/s or GB/s
-PREC_CODE = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "int8_sliced": 3}
-# bytes per operand element of the HBM-bound forms: three BF16 terms; five int8 digits
-OPERAND_BYTES = {"bf16x3": 6.0, "int8_sliced": 5.0}
+BOUND = {"fp64": "mfma", "split_fp32": "mfma", "bf16x3": "hbm", "int8_sliced": "hbm", "bf16_sliced": "hbm"}
+PEAK = {"fp64": 78.6, "split_fp32": 157.3, "bf16x3": 8000.0, "int8_sliced": 8000.0,
+        "bf16_sliced": 8000.0}  # TFLOP/s or GB/s
+PREC_CODE = {"fp64": 0, "split_fp32": 1, "bf16x3": 2, "int8_sliced": 3, "bf16_sliced": 4}
+# bytes per operand element of the HBM-bound forms: three BF16 terms; five int8 digits (the
+# BF16-sliced form reads the same digits and widens them to BF16 in registers)
+OPERAND_BYTES = {"bf16x3": 6.0, "int8_sliced": 5.0, "bf16_sliced": 5.0}
 # (the int8-sliced form's right-hand side runs its contraction with the orbital correlation of each
 # 16-point tile in the epilogue: k_mf_i8_orb)
-KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_orb"}
+KERNEL = {"fp64": "k_mf_f64_w", "split_fp32": "k_mf_f32x3", "bf16x3": "k_mf_bf16x3", "int8_sliced": "k_mf_i8_orb",
+          "bf16_sliced": "k_mf_i8_orb<bf16>"}
 TOL = 1e-9
 # forms whose contraction skips the k blocks of all-zero pair sources (cody_set_source_skip)
-SKIPPING = ("fp64", "int8_sliced")
+SKIPPING = ("fp64", "int8_sliced", "bf16_sliced")
 
 
 def parse():
@@ -55,7 +58,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--workload", default="mctdhf_1d_4096")
     p.add_argument("--precision", default="auto",
-                   choices=["auto", "fp64", "int8_sliced", "split_fp32", "bf16x3"],
+                   choices=["auto", "fp64", "int8_sliced", "bf16_sliced", "split_fp32", "bf16x3"],
                    help="auto: time every form, report the fastest whose W, RHS and energy errors "
                         "vs the FP64 oracle are all <= 1e-9")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
@@ -351,7 +354,7 @@ def pmc_traffic(kname: str, workload: str, skip: bool = False):
     WRITE_SIZE passes over `bench.py --pmc-probe`, scaled by a 256 MiB copy); None when no summary
     exists for this workload and mode."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
-    if not os.path.exists(path):
+    if not os.path.exists(path) or "<bf16>" in kname:  # (the counters were taken of the int8 kernel only)
         return None
     with open(path) as f:
         d = json.load(f)
@@ -768,7 +771,7 @@ def main():
         else:
             eb = OPERAND_BYTES[prec]
             work = eb * nGp * cols + eb * Pc * cols + 8.0 * Pc * nGp  # V and S operands once, W^T once
-            if prec == "int8_sliced":  # S's exponents (one per line and 64-point block); the epilogue's
+            if prec in ("int8_sliced", "bf16_sliced"):  # S's exponents (one per line and 64-point block); the epilogue's
                 # correlation: phi, h phi read, c written once, two overlap partials per 16-point tile,
                 # the canonical rho2
                 npr = wl.M * (wl.M - 1) // 2
@@ -787,7 +790,8 @@ def main():
                 "value": 1000.0 / ms, "wall_s_timed": wall, "roofline": roof, "kernels_ms": kernels,
                 "dstate": out, "W": w_field, "energy": energy}
 
-    forms = ["fp64", "int8_sliced", "split_fp32", "bf16x3"] if args.precision == "auto" else [args.precision]
+    forms = (["fp64", "int8_sliced", "bf16_sliced", "split_fp32", "bf16x3"] if args.precision == "auto"
+             else [args.precision])
     results = {f: time_form(f, True) for f in forms}
     # the exact forms also without the zero-source skip (every source point contracted)
     dense = {} if args.no_dense else {f + "_dense": time_form(f, False) for f in forms if f in SKIPPING}

@@ -63,6 +63,7 @@ int prepare_precision(cody_ctx* c) {
   int r = 0;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
+    case CODY_PREC_BF16_SLICED:
       if (!c->d_Vi8) {
         // (eS: per-line exponents of the untiled form; eSb, actK: per-line-and-block exponents and
         // per-block flags of the tiled form)
@@ -106,7 +107,8 @@ int prepare_precision(cody_ctx* c) {
 // formed by the pair-source kernel (Pc <= 80, grids up to kR16MaxBlocks64 64-point blocks); else
 // the untiled int8 kernel, which like the other reduced-precision forms reads row-major S^T.
 bool int8_tiled(const cody_ctx* c) {
-  return c->precision == CODY_PREC_INT8_SLICED && c->Pc <= 80 && c->nGp / 64 <= kR16MaxBlocks64;
+  return (c->precision == CODY_PREC_INT8_SLICED || c->precision == CODY_PREC_BF16_SLICED) && c->Pc <= 80 &&
+         c->nGp / 64 <= kR16MaxBlocks64;
 }
 double* plain_st(cody_ctx* c) { return (c->precision == CODY_PREC_FP64 || int8_tiled(c)) ? nullptr : c->d_ST; }
 // activity flags of the pair sources for the contraction (null: every k block is contracted):
@@ -136,19 +138,22 @@ int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(co
                         int* partialSplit = nullptr, const OrbCorrArgs* orb = nullptr) {
   const size_t ns = (size_t)c->Pc * c->nGp;
   if (partialSplit) *partialSplit = 0;
+  const bool bf = c->precision == CODY_PREC_BF16_SLICED;
   switch (c->precision) {
     case CODY_PREC_INT8_SLICED:
+    case CODY_PREC_BF16_SLICED:
       if (int8_tiled(c)) {  // 16-row kernel over the active k blocks (S's digits from the pair sources)
         if (orb) {
           CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->nO, *orb,
-                               st));
-          mark("k_mf_i8_orb");
+                               st, bf));
+          mark(bf ? "k_mf_i8_orb<bf16>" : "k_mf_i8_orb");
         } else {
-          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->Pc, st));
-          mark("k_mf_i8_r16");
+          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->Pc, st, bf));
+          mark(bf ? "k_mf_i8_r16<bf16>" : "k_mf_i8_r16");
         }
         return CODY_OK;
       }
+      if (bf) return set_error(CODY_ERR_UNSUPPORTED, "the BF16-sliced contraction takes at most 80 pair columns");
       launch_slice_rows(c->d_ST, c->nGp, c->Pc, kInt8Slices, 0, c->d_Si8, c->d_eS, st);
       mark("k_slice_rows");
       launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
@@ -380,8 +385,11 @@ int cody_set_laser(cody_ctx* c, int nK, const double* coef, double T) {
 
 int cody_set_precision(cody_ctx* c, int mode) {
   if (!c) return fail(CODY_ERR_ARG, "cody_set_precision: null context");
-  if (mode < CODY_PREC_FP64 || mode > CODY_PREC_INT8_SLICED)
+  if (mode < CODY_PREC_FP64 || mode > CODY_PREC_BF16_SLICED)
     return fail(CODY_ERR_ARG, "cody_set_precision: unknown mode");
+  if (mode == CODY_PREC_BF16_SLICED && (c->Pc > 80 || c->nGp / 64 > kR16MaxBlocks64))
+    return fail(CODY_ERR_UNSUPPORTED, "cody_set_precision: the BF16-sliced contraction takes at most 80 pair "
+                                      "columns (8 orbitals) and 16384 padded grid points");
   HIPCHK(hipSetDevice(c->device));
   c->precision = mode;
   c->vprec_ready = false;
@@ -1016,7 +1024,8 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
   const int n = static_cast<int>(b->ctx.size());
   for (int i = 0; i < n; ++i) {
     cody_ctx* c = b->ctx[i];
-    const bool fp64 = c->precision == CODY_PREC_FP64, i8 = int8_tiled(c);
+    // (the BF16-sliced form keeps its own launches)
+    const bool fp64 = c->precision == CODY_PREC_FP64, i8 = c->precision == CODY_PREC_INT8_SLICED && int8_tiled(c);
     if (!fp64 && !i8) {
       b->singles.push_back(i);
       continue;
@@ -1384,8 +1393,10 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
     if (i == 0) HIPCHK(hipEventRecord(e0, st));
     switch (c->precision) {
       case CODY_PREC_INT8_SLICED:  // the contraction kernel alone
+      case CODY_PREC_BF16_SLICED:
         if (int8_tiled(c))
-          CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->nO, oa, st));
+          CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->nO, oa, st,
+                               c->precision == CODY_PREC_BF16_SLICED));
         else
           launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);
         break;

@@ -223,7 +223,8 @@ struct OrbCorrArgs {
   double2* part;   // output: overlap partials, two nO^2 blocks per 16-point tile (h phi, then c)
 };
 int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
-                     int nGp, int nO, const OrbCorrArgs& o, hipStream_t st);
+                     int nGp, int nO, const OrbCorrArgs& o, hipStream_t st,
+                     bool bf16 = false);
 // after launch_mf_i8_orb: the Gram partials (as launch_rhs_mid), the fixed-order sums of the
 // contraction's two sets of nPart overlap partials into partr (2 nO^2; one workgroup each) and
 // R = rho1 (rho1^2 + eps)^-1 (one wave)
@@ -233,7 +234,8 @@ void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, doubl
 int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_gram(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
-                     int nGp, int Pc, hipStream_t st);
+                     int nGp, int Pc, hipStream_t st,
+                     bool bf16 = false);
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,
                   int Pc, hipStream_t st);
 void launch_mf_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl, double* WT, int nGp,

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <type_traits>
 
 #include "../../include/cody_mctdhx.h"
 #include "fedvr_device.hpp"
@@ -192,6 +193,17 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, 
   wsum_body(blockIdx.x, Wpart, nsplit, plane, WT);
 }
 
+// Eight signed digit bytes (two words, lowest byte first) as eight BF16 values, exactly.
+__device__ __forceinline__ bf8_t digits_bf16(int w0, int w1) {
+  bf8_t r;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    r[j] = static_cast<__bf16>(static_cast<float>((w0 << (24 - 8 * j)) >> 24));
+    r[4 + j] = static_cast<__bf16>(static_cast<float>((w1 << (24 - 8 * j)) >> 24));
+  }
+  return r;
+}
+
 // ---------------------------------------------------------------------------------------- int8, 16-row
 // The int8-sliced contraction for the right-hand side: one workgroup per 16-row tile of W (nGp / 16
 // workgroups) and one wave per 16-column tile t, all waves walking the same k blocks of 64 points:
@@ -211,11 +223,17 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wpart, 
 //
 // mf_i8_r16_acc: that loop for the workgroup's row tile rt; v[u] is then W at row 4(l/16) + u,
 // column l%16 of the wave's tile.
-template <int NT, int PF>
+//
+// With BF the same digits run on BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation: a
+// digit (|d| <= 127) is exact in BF16, a product of two exact in FP32, and a diagonal's sum over
+// one block (|sum| <= 5 127^2 64 < 2^24) exact too, so the block sums, and W, equal the int8
+// path's bit for bit.  The lane's 16 digit bytes (k = 16 (l/16) .. +15) feed two MFMAs of eight.
+template <int NT, int PF, bool BF = false>
 __device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
                                               const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
                                               const int* __restrict__ actK, int nGp, int Pc, double (&v)[4]) {
   static_assert(PF == 2, "the loop below keeps one block in flight");
+  using acc_t = typename std::conditional<BF, f4_t, i4_t>::type;
   constexpr int NS = kInt8Slices;
   __shared__ int sList[kR16MaxBlocks64];
   __shared__ int sCount;
@@ -245,7 +263,8 @@ __device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__
   const int line = t * 16 + rr;
 #pragma unroll
   for (int u = 0; u < 4; ++u) v[u] = 0.0;
-  i4_t a[2][NS], b[2][NS], acc[2][NS];
+  i4_t a[2][NS], b[2][NS];
+  acc_t acc[2][NS];
   int es[2];
   double sc[2];
   auto load = [&](int u, int j) {
@@ -260,12 +279,30 @@ __device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__
   };
   auto mfma = [&](int u) {
 #pragma unroll
-    for (int d = 0; d < NS; ++d) acc[u][d] = (i4_t){0, 0, 0, 0};
+    for (int d = 0; d < NS; ++d) acc[u][d] = (acc_t){0, 0, 0, 0};
+    if constexpr (BF) {
+      bf8_t ah[NS][2], bh[NS][2];
 #pragma unroll
-    for (int p = 0; p < NS; ++p)
+      for (int p = 0; p < NS; ++p) {
+        ah[p][0] = digits_bf16(a[u][p][0], a[u][p][1]);
+        ah[p][1] = digits_bf16(a[u][p][2], a[u][p][3]);
+        bh[p][0] = digits_bf16(b[u][p][0], b[u][p][1]);
+        bh[p][1] = digits_bf16(b[u][p][2], b[u][p][3]);
+      }
 #pragma unroll
-      for (int d = p; d < NS; ++d)
-        acc[u][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][p], b[u][d - p], acc[u][d], 0, 0, 0);
+      for (int p = 0; p < NS; ++p)
+#pragma unroll
+        for (int d = p; d < NS; ++d)
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+            acc[u][d] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[p][h], bh[d - p][h], acc[u][d], 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int p = 0; p < NS; ++p)
+#pragma unroll
+        for (int d = p; d < NS; ++d)
+          acc[u][d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][p], b[u][d - p], acc[u][d], 0, 0, 0);
+    }
     sc[u] = ldexp(1.0, es[u]);
   };
   auto flush = [&](int u) {  // the block in tile set u, scaled by its line-and-block factor
@@ -301,12 +338,12 @@ __device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__
 }
 
 // The body takes the workgroup's row tile rt.
-template <int NT, int PF>
+template <int NT, int PF, bool BF = false>
 __device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
                                                const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
                                                const int* __restrict__ actK, int nGp, int Pc, double* __restrict__ WT) {
   double v[4];
-  mf_i8_r16_acc<NT, PF>(rt, As, eA, Bs, eSb, actK, nGp, Pc, v);
+  mf_i8_r16_acc<NT, PF, BF>(rt, As, eA, Bs, eSb, actK, nGp, Pc, v);
   const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double* dst = WT + (size_t)(t * 16 + (lane & 15)) * nGp + rt * 16 + 4 * (lane >> 4);
@@ -331,7 +368,7 @@ __device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict_
 // overlap partials, A(i, j) = sum_x conj(phi_i) w h_j and B(i, m) = sum_x conj(phi_i) w c_m, at
 // part[rt NO^2 + i + NO j] and part[(nGp / 16 + rt) NO^2 + i + NO m] (h phi only in the modes whose
 // right-hand side has the one-body part).
-template <int NO, int PF>
+template <int NO, int PF, bool BF = false>
 __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
                                                const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
                                                const int* __restrict__ actK, int nGp, double* __restrict__ WT,
@@ -381,7 +418,7 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
     sAB[tid] = a + 16 * (a + 1 + r);
   }
   double v[4];
-  mf_i8_r16_acc<NT, PF>(rt, As, eA, Bs, eSb, actK, nGp, NT * 16, v);
+  mf_i8_r16_acc<NT, PF, BF>(rt, As, eA, Bs, eSb, actK, nGp, NT * 16, v);
   const int t = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63, q = lane >> 4, rr = lane & 15;
   double* dst = WT + (size_t)(t * 16 + rr) * nGp + x0 + 4 * q;
@@ -469,19 +506,19 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
   }
 }
 
-template <int NO>
+template <int NO, bool BF = false>
 __global__ __launch_bounds__(((NO * (NO + 1) + 15) / 16) * 64) void k_mf_i8_orb(
     const int8_t* __restrict__ As, const int* __restrict__ eA, const int8_t* __restrict__ Bs,
     const int* __restrict__ eSb, const int* __restrict__ actK, int nGp, double* __restrict__ WT, OrbCorrArgs o) {
-  mf_i8_orb_body<NO, kR16Prefetch>(blockIdx.x, As, eA, Bs, eSb, actK, nGp, WT, o);
+  mf_i8_orb_body<NO, kR16Prefetch, BF>(blockIdx.x, As, eA, Bs, eSb, actK, nGp, WT, o);
 }
 
-template <int NT>
+template <int NT, bool BF = false>
 __global__ __launch_bounds__(NT * 64) void k_mf_i8_r16(const int8_t* __restrict__ As, const int* __restrict__ eA,
                                                         const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
                                                         const int* __restrict__ actK, int nGp, int Pc,
                                                         double* __restrict__ WT) {
-  mf_i8_r16_body<NT, kR16Prefetch>(blockIdx.x, As, eA, Bs, eSb, actK, nGp, Pc, WT);
+  mf_i8_r16_body<NT, kR16Prefetch, BF>(blockIdx.x, As, eA, Bs, eSb, actK, nGp, Pc, WT);
 }
 
 // ---------------------------------------------------------------------------------------- int8 slices
@@ -845,13 +882,19 @@ void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp
 // W^T, c_m(x) and two overlap partials per 16-point tile (part: 2 nGp / 16 blocks of nO^2).  Returns
 // 0, or CODY_ERR_ARG without launching anything outside these limits.
 int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
-                     int nGp, int nO, const OrbCorrArgs& o, hipStream_t st) {
+                     int nGp, int nO, const OrbCorrArgs& o, hipStream_t st, bool bf16) {
   if (nGp < 64 || nGp % 64 || nGp / 64 > kR16MaxBlocks64 || nO < 1 || nO > 8 || o.nG > nGp)
     return set_error(CODY_ERR_ARG, "k_mf_i8_orb: nGp must be a positive multiple of 64, at most 64 x 256, above "
                                    "nG, and nO in 1..8");
-#define CALL_OC(NO)                                                                                        \
-  hipLaunchKernelGGL(k_mf_i8_orb<NO>, dim3(nGp / 16), dim3(((NO * (NO + 1) + 15) / 16) * 64), 0, st, As, eA, \
-                     Bs, eSb, actK, nGp, WT, o)
+#define CALL_OC(NO)                                                                                               \
+  do {                                                                                                            \
+    if (bf16)                                                                                                     \
+      hipLaunchKernelGGL((k_mf_i8_orb<NO, true>), dim3(nGp / 16), dim3(((NO * (NO + 1) + 15) / 16) * 64), 0, st, \
+                         As, eA, Bs, eSb, actK, nGp, WT, o);                                                      \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_mf_i8_orb<NO, false>), dim3(nGp / 16), dim3(((NO * (NO + 1) + 15) / 16) * 64), 0,    \
+                         st, As, eA, Bs, eSb, actK, nGp, WT, o);                                                  \
+  } while (0)
   switch (nO) {
     case 1: CALL_OC(1); break;
     case 2: CALL_OC(2); break;
@@ -870,12 +913,19 @@ int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const in
 // activity flags of S (pair_src_i8_body), or null to contract every k block.  Returns 0, or
 // CODY_ERR_ARG without launching anything when the shape is outside these limits.
 int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
-                     int nGp, int Pc, hipStream_t st) {
+                     int nGp, int Pc, hipStream_t st, bool bf16) {
   if (nGp < 64 || nGp % 64 || Pc < 16 || Pc > 80 || Pc % 16 || nGp / 64 > kR16MaxBlocks64)
     return set_error(CODY_ERR_ARG, "k_mf_i8_r16: nGp must be a positive multiple of 64, at most 64 x 256, "
                                    "and Pc one of 16, 32, 48, 64, 80");
-#define CALL_R16(NT) \
-  hipLaunchKernelGGL(k_mf_i8_r16<NT>, dim3(nGp / 16), dim3((NT) * 64), 0, st, As, eA, Bs, eSb, actK, nGp, Pc, WT)
+#define CALL_R16(NT)                                                                                            \
+  do {                                                                                                          \
+    if (bf16)                                                                                                   \
+      hipLaunchKernelGGL((k_mf_i8_r16<NT, true>), dim3(nGp / 16), dim3((NT) * 64), 0, st, As, eA, Bs, eSb, actK, \
+                         nGp, Pc, WT);                                                                          \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_mf_i8_r16<NT, false>), dim3(nGp / 16), dim3((NT) * 64), 0, st, As, eA, Bs, eSb,     \
+                         actK, nGp, Pc, WT);                                                                    \
+  } while (0)
   switch (Pc / 16) {
     case 1: CALL_R16(1); break;
     case 2: CALL_R16(2); break;

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcody_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "cody_mctdhx.h")
 
-PREC_FP64, PREC_SPLIT_FP32, PREC_BF16X3, PREC_INT8_SLICED = 0, 1, 2, 3
+PREC_FP64, PREC_SPLIT_FP32, PREC_BF16X3, PREC_INT8_SLICED, PREC_BF16_SLICED = 0, 1, 2, 3, 4
 SPLIT_REFERENCE, SPLIT_CONSISTENT = 0, 1
 PART_FULL, PART_LIN, PART_NONLIN = 0, 1, 2
 

@@ -38,6 +38,10 @@ extern "C" {
 #define CODY_PREC_INT8_SLICED 3 /* V and S scaled and split into five balanced base-254 digits;
                                    exact int8-MFMA products with int32 accumulation, FP64
                                    recombination */
+#define CODY_PREC_BF16_SLICED 4 /* the same digits as BF16 operands of BF16 MFMA with FP32
+                                   accumulation (exact per 64-point block), FP64 recombination:
+                                   W equal to CODY_PREC_INT8_SLICED's bit for bit; at most 80 pair
+                                   columns (8 orbitals) and 16384 padded grid points */
 
 typedef struct cody_ctx cody_ctx;
 typedef struct cody_batch cody_batch;

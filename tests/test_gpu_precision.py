@@ -6,6 +6,9 @@ each relative to the FP64 oracle.
 
 - int8-sliced (five balanced base-254 digits, exact int8-MFMA products, FP64 recombination)
   meets the 1e-9 bar on all three quantities; W within 1e-10.
+- BF16-sliced (the same digits as BF16 operands of BF16 MFMA, FP32 accumulation exact within each
+  64-point block, FP64 recombination) meets the same bar, and its W and right-hand side equal the
+  int8-sliced form's bit for bit.
 - Split-FP32 and 3xBF16 accumulate their products in FP32 inside the MFMA, which rounds every
   partial sum at 2^-24 ~ 6e-8; their W errors sit there (measured 6.6e-8 and 5.7e-8).  They are
   kept as named regression bounds at those errors, and their 1e-9 test is skipped with that
@@ -18,7 +21,7 @@ import pytest
 
 from codyfortran_amd import hip
 from codyfortran_amd import workloads as wl
-from tests.common import oracle_system, rel, state_rel
+from tests.common import oracle_system, rel, small, state_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -30,6 +33,7 @@ SPLIT_FP32_REGRESSION = (1e-7, BAR, BAR)
 BF16X3_REGRESSION = (1e-7, BAR, BAR)
 MODES = {
     "int8_sliced": (hip.PREC_INT8_SLICED, INT8_BOUNDS),
+    "bf16_sliced": (hip.PREC_BF16_SLICED, INT8_BOUNDS),
     "split_fp32": (hip.PREC_SPLIT_FP32, SPLIT_FP32_REGRESSION),
     "bf16x3": (hip.PREC_BF16X3, BF16X3_REGRESSION),
 }
@@ -80,3 +84,31 @@ def test_fp32_accumulating_forms_meet_the_bar(full_size, mode):
     code, _ = MODES[mode]
     ew, _, ed, ee = _errors(full_size, code)
     assert max(ew, ed, ee) < BAR
+
+
+def test_bf16_sliced_equals_int8_sliced_bit_for_bit(full_size):
+    """Every digit product and every block sum of a diagonal is an integer below 2^24, exact in the
+    BF16 MFMA's FP32 accumulator as in the int8 MFMA's int32 one, and both forms recombine the
+    blocks in the same FP64 order."""
+    w, ctx, state, orbs, *_ = full_size
+    out = {}
+    for code in (hip.PREC_INT8_SLICED, hip.PREC_BF16_SLICED):
+        ctx.set_precision(code)
+        out[code] = (ctx.meanfield_w(orbs), ctx.time_derivative(state), ctx.energy(state))
+    ctx.set_precision(hip.PREC_FP64)
+    a, b = out[hip.PREC_INT8_SLICED], out[hip.PREC_BF16_SLICED]
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+    assert a[2] == b[2]
+
+
+def test_bf16_sliced_refuses_more_than_80_pair_columns():
+    """12 orbitals (160 pair columns) are outside the BF16-sliced kernel: refused on the host."""
+    w = small("m12_refuse", -10.0, 10.0, 32, 5, 6, 12)
+    ctx, x, wts = wl.make_context(w)
+    try:
+        with pytest.raises(hip.CodyError, match="80 pair"):
+            ctx.set_precision(hip.PREC_BF16_SLICED)
+        ctx.set_precision(hip.PREC_INT8_SLICED)  # (the int8 form takes them on its untiled kernel)
+    finally:
+        ctx.close()
