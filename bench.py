@@ -572,6 +572,18 @@ def batched_measurement(args, torch, wlm, check: bool):
                                        "split_threshold": hip.Batch.SCHEDULES[pick8][1]}}
     if check:
         out["int8_sliced"]["max_rhs_rel_err_vs_fp64_oracle"] = worst_err()
+    # the batch's figure: the faster form whose every instance is within 1e-9 of the FP64 oracle
+    # (without the oracle check, the FP64 form)
+    fp64 = {k: out[k] for k in ("value", "ms_per_batched_rhs", "dtype", "max_rhs_rel_err_vs_fp64_oracle") if k in out}
+    out["fp64"] = fp64
+    i8 = out["int8_sliced"]
+    if check and i8["max_rhs_rel_err_vs_fp64_oracle"] <= TOL and i8["value"] > fp64["value"]:
+        out.update(value=i8["value"], ms_per_batched_rhs=i8["ms_per_batched_rhs"], dtype=i8["dtype"],
+                   max_rhs_rel_err_vs_fp64_oracle=i8["max_rhs_rel_err_vs_fp64_oracle"],
+                   contraction_tflops_aggregate=flops / (i8["ms_per_batched_rhs"] * 1e-3) / 1e12)
+        out["form"] = "int8_sliced"
+    else:
+        out["form"] = "fp64"
     b.close()
     for c, _ in ctxs:
         c.close()

@@ -28,6 +28,8 @@ std::string check_plan_shape(const PlanShape& s) {
   const int k64 = s.nGp / 64;
   if (!is_pow2_upto(s.nxs, 32) || k64 % s.nxs != 0)
     return fmt("Gram x ranges %d do not divide the %d 64-point blocks", s.nxs, k64);
+  if (s.fused != (s.int8 && s.nO <= kPlanFusedMaxOrbs))
+    return fmt("the fused int8 path is for int8 instances of at most %d orbitals (%d orbitals)", kPlanFusedMaxOrbs, s.nO);
   if (s.int8) {
     if (s.Pc / 16 > kPlanInt8MaxTiles) return fmt("%d pair columns exceed the int8 contraction's %d", s.Pc, 16 * kPlanInt8MaxTiles);
     if (s.nsplit != 1) return fmt("int8 k split %d (the int8 contraction takes one range)", s.nsplit);
@@ -49,21 +51,21 @@ void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]) {
   const int64_t per = 4 * (s.nC > kPlanRdmWideConf ? kPlanRdmWideItems : 1);
   const int64_t nRdm = (n2 + np * np + per - 1) / per;
   const int64_t nRed = ((int64_t)s.Pc * s.Pc + n2 + 255) / 256;
-  const int64_t wsumFirst = s.int8 ? 0 : 1;  // R in the int8 path's Gram launch, else in the k-split sum's
+  const int64_t wsumFirst = s.fused ? 0 : 1;  // R in the fused int8 path's Gram launch, else in the k-split sum's
   // pair sources: one 16 x 16 fragment per workgroup, or for the int8 path (digits with a scale per
   // line and 64-point block) one 16 x 64 block
   cnt[kPlanFront] = nRdm + nObx * s.nO + plane / (s.int8 ? 1024 : 256);
   // the int8 contraction: one workgroup per 16-row tile; the FP64 one: 64-row groups x k ranges
   cnt[kPlanContract] = s.int8 ? (int64_t)(s.nGp / 16) : (int64_t)(s.nGp / 64) * s.nsplit;
   cnt[kPlanWsum] = wsumFirst + (s.nsplit > 1 ? (plane / 2 + 255) / 256 : 0);
-  const bool corrQ = !s.int8 && s.nO >= kPlanCorrQMinOrbs;
+  const bool corrQ = !s.fused && s.nO >= kPlanCorrQMinOrbs;
   const int64_t nGram = (int64_t)(s.Pc / 16) * s.nxs;
   // Q (four doubles per thread) beside the Gram partials
   cnt[kPlanQform] = corrQ ? (plan_corr_q_doubles(s.nO) / 4 + 255) / 256 + nGram : 0;
   // Gram + orbital correlation per 64 points and orbital, or the correlation on FP64 MFMA per 64
   // points and orbital group; the int8 path's contraction forms the correlation, and beside the Gram
   // two workgroups sum its two sets of overlap partials and one forms R
-  cnt[kPlanMid] = s.int8 ? nGram + 3 : (corrQ ? (int64_t)(s.nGp / 64) * kPlanCorrQGroups : nGram + (int64_t)(s.nGp / 64) * s.nO);
+  cnt[kPlanMid] = s.fused ? nGram + 3 : (corrQ ? (int64_t)(s.nGp / 64) * kPlanCorrQGroups : nGram + (int64_t)(s.nGp / 64) * s.nO);
   cnt[kPlanPost] = nRed + nObx;
   cnt[kPlanApply] = (s.nC + 3) / 4;
 }
@@ -77,7 +79,8 @@ std::string build_group_plan(const std::vector<PlanShape>& members, GroupPlan& o
     const PlanShape& s = members[m];
     const std::string why = check_plan_shape(s);
     if (!why.empty()) return fmt("instance %d: ", m) + why;
-    if (s.nO != members[0].nO || s.int8 != members[0].int8) return fmt("instance %d: another orbital count or path", m);
+    if (s.nO != members[0].nO || s.int8 != members[0].int8 || s.fused != members[0].fused)
+      return fmt("instance %d: another orbital count or path", m);
     plan_block_counts(s, &counts[(size_t)m * kPlanKinds]);
   }
   int64_t total = 0;

@@ -19,7 +19,8 @@ static_assert(sizeof(BlockRef) == 8, "a block reference is uploaded as int2");
 // tiles; cody_api.hip asserts that they agree).
 constexpr int kPlanMaxOrbs = 12;           // orbitals per instance
 constexpr int kPlanMaxConf = 1024;         // CI configurations per instance
-constexpr int kPlanInt8MaxTiles = 5;       // 16-column tiles of the int8 contraction
+constexpr int kPlanInt8MaxTiles = 10;      // 16-column tiles of the int8 contraction
+constexpr int kPlanFusedMaxOrbs = 8;       // orbitals of the int8 contraction with the correlation epilogue
 constexpr int kPlanInt8MaxBlocks64 = 256;  // 64-point k blocks of the int8 contraction (one range)
 constexpr int kPlanF64MaxTiles = 10;       // 16-column tiles of the FP64 contraction
 // CI density role: entries per wave (the device's ci_rdm_items_per_wave): 1, or 4 above 256
@@ -46,6 +47,7 @@ struct PlanShape {
   int nsplit = 1;  // contraction k ranges (the int8 path: 1, the 16-row kernel writes W^T)
   int nxs = 1;     // x ranges of the Gram partials
   bool int8 = false;
+  bool fused = false;  // int8 with the orbital correlation in the contraction (exactly: int8, nO <= 8)
 };
 
 // Workgroups of each kernel type of one instance's right-hand side: the grids of its own launches

@@ -104,12 +104,15 @@ int prepare_precision(cody_ctx* c) {
 }
 
 // The int8 form on the 16-row contraction over the active k blocks, S's digits fragment-tiled and
-// formed by the pair-source kernel (Pc <= 80, grids up to kR16MaxBlocks64 64-point blocks); else
+// formed by the pair-source kernel (Pc <= 160, grids up to kR16MaxBlocks64 64-point blocks); else
 // the untiled int8 kernel, which like the other reduced-precision forms reads row-major S^T.
 bool int8_tiled(const cody_ctx* c) {
-  return (c->precision == CODY_PREC_INT8_SLICED || c->precision == CODY_PREC_BF16_SLICED) && c->Pc <= 80 &&
-         c->nGp / 64 <= kR16MaxBlocks64;
+  return (c->precision == CODY_PREC_INT8_SLICED || c->precision == CODY_PREC_BF16_SLICED) &&
+         c->Pc <= 16 * kR16MaxTiles && c->nGp / 64 <= kR16MaxBlocks64;
 }
+// ... with the orbital correlation in the contraction's epilogue (up to 8 orbitals, 80 pair columns);
+// above that the tiled contraction writes W^T and the FP64 form's correlation follows.
+bool int8_fused(const cody_ctx* c) { return int8_tiled(c) && c->nO <= 8; }
 double* plain_st(cody_ctx* c) { return (c->precision == CODY_PREC_FP64 || int8_tiled(c)) ? nullptr : c->d_ST; }
 // activity flags of the pair sources for the contraction (null: every k block is contracted):
 // per 16 x 16 fragment for the FP64 form, per 16 x 64 block for the tiled int8 form
@@ -490,10 +493,10 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   // The tiled int8 form's contraction also forms the orbital correlation of its points before R
   // (c_m(x) and its overlap partials); then one launch forms the Gram partials, the partial sums
   // and R, and the projection applies R.
-  const bool fused = int8_tiled(c);
+  const bool fused = int8_fused(c);
   launch_rhs_front(C, c->nC, c->nO, c->dci, c->d_rho1, c->d_r2can, c->d_rho2, orbs, c->d_w, c->d_x, c->d_vext, c->kin,
                    efield_src(c), c->nG, c->d_hphi, c->d_part1, c->nGp, c->Pc, c->d_STt, c->d_act,
-                   fused ? c->d_Si8 : nullptr, c->d_eSb, c->d_actK, st);
+                   int8_tiled(c) ? c->d_Si8 : nullptr, c->d_eSb, c->d_actK, st);
   mark("k_rhs_front");
   if (double* sr = plain_st(c)) launch_pair_src_rows(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, sr, st);
   int ps = 0;
@@ -1039,6 +1042,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       grp->nO = c->nO;
       grp->Pc = c->Pc;
       grp->int8 = i8;
+      grp->fused = i8 && int8_fused(c);
     }
     grp->members.push_back(i);
   }
@@ -1062,6 +1066,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       h.nO = g.nO;
       h.Pc = g.Pc;
       h.int8 = g.int8;
+      h.fused = g.fused;
       std::vector<int> keep;
       double ck = 0.0, ch = 0.0;
       for (int i : mem) {
@@ -1098,7 +1103,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.nRdm = (n2 + np * np + 4 * ci_rdm_items_per_wave(c->nC) - 1) / (4 * ci_rdm_items_per_wave(c->nC));
       a.nObx = (c->nG + 255) / 256;
       a.nsplit = g.int8 ? 1 : c->wsplit;  // (the int8 contraction writes W^T itself)
-      a.wsumFirst = g.int8 ? 0 : 1;  // (the int8 path forms R in its Gram launch)
+      a.wsumFirst = g.fused ? 0 : 1;  // (the fused int8 path forms R in its Gram launch)
       a.nxs = c->nxs;
       a.nPart = c->nPart;
       a.nPart1 = (c->nG + 255) / 256;
@@ -1130,7 +1135,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.actK = c->d_actK;
       // (the FP64 kernel keeps its block list in LDS, up to kSkipMaxBlocks16 fragments)
       a.actC = (!g.int8 && c->nGp / 16 > kSkipMaxBlocks16) ? nullptr : act_flags(c);
-      a.partr = g.int8 ? c->d_partr : nullptr;
+      a.partr = g.fused ? c->d_partr : nullptr;
       a.Vt = c->d_Vt;
       a.Vi8 = c->d_Vi8;
       a.eV = c->d_eV;
@@ -1146,6 +1151,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       sh.nsplit = a.nsplit;
       sh.nxs = c->nxs;
       sh.int8 = g.int8;
+      sh.fused = g.fused;
       shapes.push_back(sh);
       g.maxC = std::max(g.maxC, c->nC);
       g.cost += batch_cost(b, i);
@@ -1158,7 +1164,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       g.off[k] = static_cast<int>(plan.off[k]);
       g.cnt[k] = static_cast<int>(plan.cnt[k]);
     }
-    if (nO >= kCorrQMinOrbs && !g.int8) {  // Q of every member's orbital correlation
+    if (nO >= kCorrQMinOrbs && !g.fused) {  // Q of every member's orbital correlation
       const size_t nq = (size_t)corr_q_doubles(nO);
       HIPCHK(hipMalloc((void**)&g.d_qf, sizeof(double) * nq * gi.size()));
       for (size_t m = 0; m < gi.size(); ++m) gi[m].qf = g.d_qf + m * nq;
@@ -1175,14 +1181,16 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
 int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
   auto map = [&](int k) { return g.d_map + g.off[k]; };
   launch_grp_front(g.nO, g.d_inst, map(kPlanFront), g.cnt[kPlanFront], g.maxC, st);
-  if (g.int8) {
+  if (g.fused) {
     CHK(launch_grp_mf_i8_orb(g.nO, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
+  } else if (g.int8) {
+    CHK(launch_grp_mf_i8_r16(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   } else {
     CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   }
   launch_grp_wsum(g.nO, g.d_inst, map(kPlanWsum), g.cnt[kPlanWsum], st);
   launch_grp_qform(g.nO, g.d_inst, map(kPlanQform), g.cnt[kPlanQform], st);
-  if (g.int8)
+  if (g.fused)
     launch_grp_gram(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
   else
     launch_grp_mid(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
@@ -1394,8 +1402,11 @@ int cody_profile_contraction(cody_ctx* c, int iters, double* avg_ms) {
     switch (c->precision) {
       case CODY_PREC_INT8_SLICED:  // the contraction kernel alone
       case CODY_PREC_BF16_SLICED:
-        if (int8_tiled(c))
+        if (int8_fused(c))
           CHK(launch_mf_i8_orb(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->nO, oa, st,
+                               c->precision == CODY_PREC_BF16_SLICED));
+        else if (int8_tiled(c))
+          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->Pc, st,
                                c->precision == CODY_PREC_BF16_SLICED));
         else
           launch_mf_i8(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eS, c->d_WT, c->nGp, c->Pc, st);

@@ -300,6 +300,7 @@ struct cody_ctx {
 struct BatchGroup {
   int nO = 0, Pc = 0, maxC = 0;
   bool int8 = false;
+  bool fused = false;  // int8 with the orbital correlation in the contraction (up to 8 orbitals)
   double cost = 0.0;  // estimated device time (for the assignment to chains)
   std::vector<int> members;
   cody::GroupInstance* d_inst = nullptr;

@@ -211,6 +211,7 @@ void launch_tile_f64(const double* V, int nGp, double* Vt, hipStream_t st);
 void launch_gram_f64(const double* STt, const double* WT, double* Gpart, int nGp, int Pc, int nsplit,
                      hipStream_t st);
 constexpr int kR16MaxBlocks64 = 256;  // longest grid (in 64-point blocks) of the 16-row int8 contraction
+constexpr int kR16MaxTiles = 10;      // 16-column tiles of the 16-row int8 contraction (with the orbital epilogue: 5)
 constexpr int kR16Prefetch = 2;       // k blocks of operands in flight per wave of the 16-row int8 contraction
 // Orbital-correlation operands of the fused int8 contraction (launch_mf_i8_orb)
 struct OrbCorrArgs {

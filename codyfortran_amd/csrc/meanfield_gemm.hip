@@ -861,7 +861,12 @@ int launch_grp_mf_i8_r16(int Pc, const GroupInstance* gi, const int2* map, int n
     case 3: CALL_GI(3); break;
     case 4: CALL_GI(4); break;
     case 5: CALL_GI(5); break;
-    default: return set_error(CODY_ERR_ARG, "k_grp_mf_i8_r16: Pc / 16 outside 1..5");
+    case 6: CALL_GI(6); break;
+    case 7: CALL_GI(7); break;
+    case 8: CALL_GI(8); break;
+    case 9: CALL_GI(9); break;
+    case 10: CALL_GI(10); break;
+    default: return set_error(CODY_ERR_ARG, "k_grp_mf_i8_r16: Pc / 16 outside 1..10");
   }
 #undef CALL_GI
   return 0;
@@ -914,9 +919,9 @@ int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const in
 // CODY_ERR_ARG without launching anything when the shape is outside these limits.
 int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
                      int nGp, int Pc, hipStream_t st, bool bf16) {
-  if (nGp < 64 || nGp % 64 || Pc < 16 || Pc > 80 || Pc % 16 || nGp / 64 > kR16MaxBlocks64)
+  if (nGp < 64 || nGp % 64 || Pc < 16 || Pc > 16 * kR16MaxTiles || Pc % 16 || nGp / 64 > kR16MaxBlocks64)
     return set_error(CODY_ERR_ARG, "k_mf_i8_r16: nGp must be a positive multiple of 64, at most 64 x 256, "
-                                   "and Pc one of 16, 32, 48, 64, 80");
+                                   "and Pc a multiple of 16 up to 160");
 #define CALL_R16(NT)                                                                                            \
   do {                                                                                                          \
     if (bf16)                                                                                                   \
@@ -932,6 +937,11 @@ int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const in
     case 3: CALL_R16(3); break;
     case 4: CALL_R16(4); break;
     case 5: CALL_R16(5); break;
+    case 6: CALL_R16(6); break;
+    case 7: CALL_R16(7); break;
+    case 8: CALL_R16(8); break;
+    case 9: CALL_R16(9); break;
+    case 10: CALL_R16(10); break;
   }
 #undef CALL_R16
   return 0;

@@ -36,6 +36,7 @@ static PlanShape shape(int nO, int nC, int nG, int nsplit, bool int8) {
       break;
     }
   s.int8 = int8;
+  s.fused = int8 && nO <= kPlanFusedMaxOrbs;
   return s;
 }
 
@@ -74,6 +75,18 @@ int main() {
     plan_block_counts(shape(12, 924, 4096, 4, false), c);
     EXPECT(c[kPlanQform] == 59 + 10 * 32, "Q + Gram blocks %lld", (long long)c[kPlanQform]);
     EXPECT(c[kPlanMid] == 64 * 4, "12-orbital correlation blocks %lld", (long long)c[kPlanMid]);
+    // 12 orbitals on the int8 path: the 16-row contraction (no correlation epilogue) writes W^T;
+    // R in the k-split-sum launch, Q + Gram, the MFMA correlation, as the FP64 path
+    {
+      int64_t d[kPlanKinds];
+      const PlanShape u = shape(12, 924, 4096, 1, true);
+      EXPECT(check_plan_shape(u).empty(), "unfused int8 shape refused: %s", check_plan_shape(u).c_str());
+      plan_block_counts(u, d);
+      EXPECT(d[kPlanContract] == 256 && d[kPlanWsum] == 1 && d[kPlanQform] == 59 + 10 * 32 && d[kPlanMid] == 64 * 4,
+             "unfused int8 blocks %lld %lld %lld %lld", (long long)d[kPlanContract], (long long)d[kPlanWsum],
+             (long long)d[kPlanQform], (long long)d[kPlanMid]);
+      EXPECT(d[kPlanFront] == 282 + 192 + 640, "unfused int8 front blocks %lld", (long long)d[kPlanFront]);
+    }
     // 924 configurations: four CI density entries per wave, 16 per workgroup: (144 + 66^2) / 16
     // = 281.25 -> 282 blocks, then 16 x 12 one-body and 160 x 4096 / 256 pair-source blocks
     EXPECT(c[kPlanFront] == 282 + 192 + 2560, "12-orbital front blocks %lld", (long long)c[kPlanFront]);
@@ -83,7 +96,6 @@ int main() {
   //    instance's blocks exactly once and in order
   for (bool int8 : {false, true})
     for (int nO : {4, 6, 8, 12}) {
-      if (int8 && nO > 8) continue;
       std::vector<PlanShape> members;
       int k = 0;
       for (int nG : {511, 1026, 2046, 4096})
@@ -122,7 +134,12 @@ int main() {
     auto refused = [](PlanShape s) { return !check_plan_shape(s).empty(); };
     EXPECT(refused(shape(8, 70, 4096, 16, false)), "16-way split accepted");  // (the one that overran d_Wpart)
     EXPECT(refused(shape(8, 70, 4096, 3, false)), "3-way split accepted");
-    EXPECT(refused(shape(12, 924, 4096, 4, true)), "int8 with 160 pair columns accepted");
+    EXPECT(refused(shape(12, 924, 4096, 4, true)), "int8 with a 4-way split accepted (12 orbitals)");
+    {
+      PlanShape f = shape(12, 924, 4096, 1, true);
+      f.fused = true;
+      EXPECT(refused(f), "the fused int8 path accepted for 12 orbitals");
+    }
     EXPECT(refused(shape(8, 70, 4096, 4, true)), "int8 with a 4-way split accepted");
     EXPECT(refused(shape(8, 70, 16640, 1, true)), "int8 with 260 k blocks accepted");
     EXPECT(refused(shape(13, 70, 4096, 1, false)), "13 orbitals accepted");

@@ -109,6 +109,26 @@ def test_bf16_sliced_refuses_more_than_80_pair_columns():
     try:
         with pytest.raises(hip.CodyError, match="80 pair"):
             ctx.set_precision(hip.PREC_BF16_SLICED)
-        ctx.set_precision(hip.PREC_INT8_SLICED)  # (the int8 form takes them on its untiled kernel)
+        ctx.set_precision(hip.PREC_INT8_SLICED)  # (the int8 form takes them, without the epilogue)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("M", [10, 12])
+def test_int8_above_eight_orbitals_matches_oracle(M):
+    """Above 8 orbitals (up to 160 pair columns) the int8 form runs the 16-row contraction without
+    the correlation epilogue, then the FP64 form's correlation on FP64 MFMA."""
+    w = small(f"m{M}_int8", -10.0, 10.0, 32, 5, M // 2, M)
+    s, x, wts = oracle_system(w)
+    ctx, xg, wg = wl.make_context(w)
+    state = wl.seeded_state(xg, wg, w.N, w.M, seed=77)
+    _, orbs = wl.split_state(state, w.nC, w.nG, w.M)
+    try:
+        ctx.set_precision(hip.PREC_INT8_SLICED)
+        ew = rel(ctx.meanfield_w(orbs), s.meanfield_w(orbs))
+        ed = state_rel(ctx.time_derivative(state), s.time_derivative(state), w.nC, wts, w.M)
+        ee = abs(ctx.energy(state) - s.energy(state)) / abs(s.energy(state))
+    finally:
+        ctx.close()
+    print(f"\nint8, M = {M}: W rel err {ew:.3e}, RHS rel err {ed:.3e}, energy rel err {ee:.3e}")
+    assert ew < 1e-10 and ed < BAR and ee < BAR
