@@ -120,6 +120,12 @@ const int* act_flags(const cody_ctx* c) {
   if (!c->skip_zero) return nullptr;
   return int8_tiled(c) ? c->d_actK : c->d_act;
 }
+// the flags the Gram partials skip by (as the grouped launches take them: the FP64 contraction's,
+// which it leaves out above kSkipMaxBlocks16 16-point blocks) and their block size (2^shift points)
+const int* gram_flags(const cody_ctx* c) {
+  return (!int8_tiled(c) && c->nGp / 16 > kSkipMaxBlocks16) ? nullptr : act_flags(c);
+}
+int gram_flag_shift(const cody_ctx* c) { return int8_tiled(c) ? 6 : 4; }
 
 // The pair sources of orbs for the context's contraction: S^T fragment-tiled with its activity
 // flags, with the int8 digits where the tiled int8 contraction reads them, and row-major S^T for
@@ -504,18 +510,19 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   CHK(enqueue_contraction(c, st, mark, &ps, fused ? &oa : nullptr));
   if (fused) {
     launch_rhs_gram(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, c->nO, c->d_part, c->nGp / 16, c->d_partr,
-                    c->d_rho1, c->reg, c->d_reginv, st);
+                    c->d_rho1, c->reg, c->d_reginv, st, gram_flags(c));
     mark("k_rhs_gram");
   } else {
     launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st);
     mark("k_rhs_wsum");
     if (c->nO >= kCorrQMinOrbs) {
       launch_rhs_qform(c->nO, c->d_r2can, c->d_reginv, c->d_qf, c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs,
-                       st);
+                       st, gram_flags(c), gram_flag_shift(c));
       mark("k_rhs_qform");
     }
     launch_rhs_mid(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, orbs, c->d_w, c->d_hphi, c->d_r2can,
-                   c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st, c->d_qf);
+                   c->d_reginv, c->nG, c->nO, mode, c->d_draw, c->d_part, st, c->d_qf, gram_flags(c),
+                   gram_flag_shift(c));
     mark("k_rhs_mid");
   }
   // (the int8 projection: h phi where the mode has the one-body part, c_m in draw, the summed partials)
@@ -1134,7 +1141,7 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.act = c->d_act;
       a.actK = c->d_actK;
       // (the FP64 kernel keeps its block list in LDS, up to kSkipMaxBlocks16 fragments)
-      a.actC = (!g.int8 && c->nGp / 16 > kSkipMaxBlocks16) ? nullptr : act_flags(c);
+      a.actC = gram_flags(c);
       a.partr = g.fused ? c->d_partr : nullptr;
       a.Vt = c->d_Vt;
       a.Vi8 = c->d_Vi8;

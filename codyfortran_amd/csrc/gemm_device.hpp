@@ -112,11 +112,15 @@ __device__ int active_blocks(const int* __restrict__ act, int nT, int nK16, int 
 // Gram split-K partials Gpart[y][c2][c1] = sum_{x in range y} S^T[c1][x] W^T[c2][x] (G = S^T W,
 // used by h2): A = S^T fragment-tiled (as k_pair_src_t writes it), B = W^T row-major; one
 // workgroup per 16 rows of S^T (bx) and k range by, one quarter of it per wave.
+// act (optional): the pair sources' activity flags, act[t (ldb >> actShift) + block] for column
+// tile t of S^T and block of 2^actShift points; a wave whose k range holds no active block of its
+// row tile bx skips its loads and MFMAs (those products are exact zeros).
 template <int NT>
 __device__ __forceinline__ void gram_f64_body(int bx, int by, const double* __restrict__ At, int nK,
                                                   const double* __restrict__ B, int ldb,
                                                   double* __restrict__ Cout, int ldc, int kq,
-                                                  size_t cSplitStride) {
+                                                  size_t cSplitStride, const int* __restrict__ act = nullptr,
+                                                  int actShift = 4) {
   __shared__ double red[2 * 16 * (NT * 16 + 1)];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
   const int row0 = bx * 16;
@@ -149,20 +153,30 @@ __device__ __forceinline__ void gram_f64_body(int bx, int by, const double* __re
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][s], b[u][t][s], acc[t], 0, 0, 0);
   };
-#pragma unroll
-  for (int j = 0; j < NBUF - 1; ++j) load(j, j);
-  int i = 0;
-  for (; i + NBUF <= nb; i += NBUF) {
-#pragma unroll
-    for (int j = 0; j < NBUF; ++j) {
-      load((j + NBUF - 1) % NBUF, i + j + NBUF - 1);
-      mma(j);
-    }
+  bool live = true;
+  if (act) {
+    const int b0 = kw >> actShift, nbw = ((kw + kq - 1) >> actShift) - b0 + 1;
+    const int* fl = act + (size_t)bx * (ldb >> actShift) + b0;
+    int any = 0;
+    for (int i = lane; i < nbw; i += 64) any |= fl[i];
+    live = __ballot(any != 0) != 0ull;
   }
-  // the remaining nb - i < NBUF blocks are already in stages 0 .. nb - i - 1
+  if (live) {
 #pragma unroll
-  for (int j = 0; j < NBUF - 1; ++j)
-    if (j < nb - i) mma(j);
+    for (int j = 0; j < NBUF - 1; ++j) load(j, j);
+    int i = 0;
+    for (; i + NBUF <= nb; i += NBUF) {
+#pragma unroll
+      for (int j = 0; j < NBUF; ++j) {
+        load((j + NBUF - 1) % NBUF, i + j + NBUF - 1);
+        mma(j);
+      }
+    }
+    // the remaining nb - i < NBUF blocks are already in stages 0 .. nb - i - 1
+#pragma unroll
+    for (int j = 0; j < NBUF - 1; ++j)
+      if (j < nb - i) mma(j);
+  }
   double vals[NT][4];
 #pragma unroll
   for (int t = 0; t < NT; ++t)

@@ -621,7 +621,8 @@ template <int NO>
 __global__ __launch_bounds__(256) void k_rhs_qform(const double2* __restrict__ r2can, const double2* __restrict__ reginv,
                                                    double* __restrict__ qf, const double* __restrict__ STt,
                                                    const double* __restrict__ WT, int nGp, int Pc,
-                                                   double* __restrict__ Gpart, int nxs) {
+                                                   double* __restrict__ Gpart, int nxs, const int* __restrict__ act,
+                                                   int actShift) {
   if constexpr (NO >= kCorrQMinOrbs) {
     constexpr int NT = (NO * (NO + 1) + 15) / 16, nQ = corr_q_blocks(NO);
     const int b = blockIdx.x;
@@ -630,7 +631,7 @@ __global__ __launch_bounds__(256) void k_rhs_qform(const double2* __restrict__ r
       return;
     }
     gram_f64_body<NT>((b - nQ) % NT, (b - nQ) / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs),
-                      (size_t)Pc * Pc);
+                      (size_t)Pc * Pc, act, actShift);
   }
 }
 
@@ -947,11 +948,12 @@ void launch_xmat(const double2* orbs, const double* w, const double* x, int nG, 
 }
 
 void launch_rhs_qform(int nO, const double2* r2can, const double2* reginv, double* qf, const double* STt,
-                      const double* WT, int nGp, int Pc, double* Gpart, int nxs, hipStream_t st) {
+                      const double* WT, int nGp, int Pc, double* Gpart, int nxs, hipStream_t st, const int* act,
+                      int actShift) {
   const unsigned grid = (unsigned)(corr_q_blocks(nO) + Pc / 16 * nxs);
 #define CALL_QF(NO)                                                                                          \
   hipLaunchKernelGGL(k_rhs_qform<NO>, dim3(grid), dim3(256), 0, st, r2can, reginv, qf, STt, WT, nGp, Pc, Gpart, \
-                     nxs)
+                     nxs, act, actShift)
   CODY_NO_SWITCH(nO, CALL_QF)
 #undef CALL_QF
 }
@@ -1040,7 +1042,8 @@ __global__ __launch_bounds__(256) void k_rhs_mid(const double* __restrict__ STt,
                                                  const double2* __restrict__ hphi, const double2* __restrict__ r2can,
                                                  const double2* __restrict__ reginv, int nG, int mode,
                                                  double2* __restrict__ draw, double2* __restrict__ part,
-                                                 const double* __restrict__ qf) {
+                                                 const double* __restrict__ qf, const int* __restrict__ act,
+                                                 int actShift) {
   constexpr int NT = (NO * (NO + 1) + 15) / 16;  // = Pc / 16
   int b = blockIdx.x;
   if constexpr (NO >= kCorrQMinOrbs) {  // (the Gram partials come from the Q launch)
@@ -1049,7 +1052,8 @@ __global__ __launch_bounds__(256) void k_rhs_mid(const double* __restrict__ STt,
   } else {
     const int nGram = NT * nxs;
     if (b < nGram) {
-      gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc);
+      gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc, act,
+                        actShift);
       return;
     }
     b -= nGram;
@@ -1068,10 +1072,10 @@ __device__ __forceinline__ void gram_sum_body(int b, const double* __restrict__ 
                                               int nGp, int Pc, double* __restrict__ Gpart, int nxs,
                                               const double2* __restrict__ part, int nPart,
                                               double2* __restrict__ partr, const double2* __restrict__ rho1,
-                                              double reg, double2* __restrict__ reginv) {
+                                              double reg, double2* __restrict__ reginv, const int* __restrict__ actK) {
   constexpr int NT = (NO * (NO + 1) + 15) / 16, n2 = NO * NO, G = kOverlapGroups<NO>;
   if (b < NT * nxs) {
-    gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc);
+    gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc, actK, 6);
     return;
   }
   b -= NT * nxs;
@@ -1091,8 +1095,9 @@ __global__ __launch_bounds__(256) void k_rhs_gram(const double* __restrict__ STt
                                                   int nGp, int Pc, double* __restrict__ Gpart, int nxs,
                                                   const double2* __restrict__ part, int nPart,
                                                   double2* __restrict__ partr, const double2* __restrict__ rho1,
-                                                  double reg, double2* __restrict__ reginv) {
-  gram_sum_body<NO>(blockIdx.x, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, partr, rho1, reg, reginv);
+                                                  double reg, double2* __restrict__ reginv,
+                                                  const int* __restrict__ actK) {
+  gram_sum_body<NO>(blockIdx.x, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, partr, rho1, reg, reginv, actK);
 }
 
 // Then: G and h1 from their partials with the energy partials (the first nRed workgroups), and the
@@ -1172,7 +1177,7 @@ __global__ __launch_bounds__(256) void k_grp_qform(const GroupInstance* __restri
     }
     const int b = m.y - nQ;
     gram_f64_body<NT>(b % NT, b / NT, a.STt, a.nGp >> 4, a.WT, a.nGp, a.Gpart, a.Pc, a.nGp / (4 * a.nxs),
-                      (size_t)a.Pc * a.Pc);
+                      (size_t)a.Pc * a.Pc, a.actC, a.Si8 ? 6 : 4);
   }
 }
 
@@ -1189,7 +1194,7 @@ __global__ __launch_bounds__(256) void k_grp_mid(const GroupInstance* __restrict
     const int nGram = NT * a.nxs;
     if (b < nGram) {
       gram_f64_body<NT>(b % NT, b / NT, a.STt, a.nGp >> 4, a.WT, a.nGp, a.Gpart, a.Pc, a.nGp / (4 * a.nxs),
-                        (size_t)a.Pc * a.Pc);
+                        (size_t)a.Pc * a.Pc, a.actC, a.Si8 ? 6 : 4);
       return;
     }
     b -= nGram;
@@ -1204,7 +1209,7 @@ __global__ __launch_bounds__(256) void k_grp_gram(const GroupInstance* __restric
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
   gram_sum_body<NO>(m.y, a.STt, a.WT, a.nGp, a.Pc, a.Gpart, a.nxs, a.part, a.nGp / 16, a.partr, a.rho1, a.reg,
-                    a.reginv);
+                    a.reginv, a.actC);
 }
 
 template <int NO>
@@ -1287,22 +1292,22 @@ void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, c
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
                     const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
                     const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st,
-                    const double* qf) {
+                    const double* qf, const int* act, int actShift) {
   const unsigned grid = (unsigned)(nO >= kCorrQMinOrbs ? nGp / 64 * kCorrQGroups : Pc / 16 * nxs + nGp / 64 * nO);
 #define CALL_MI(NO)                                                                                             \
   hipLaunchKernelGGL(k_rhs_mid<NO>, dim3(grid), dim3(256), 0, st, STt, WT, nGp, Pc, Gpart, nxs, orbs, w, hphi, r2can, \
-                     reginv, nG, mode, draw, part, qf)
+                     reginv, nG, mode, draw, part, qf, act, actShift)
   CODY_NO_SWITCH(nO, CALL_MI)
 #undef CALL_MI
 }
 
 void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs, int nO,
                      const double2* part, int nPart, double2* partr, const double2* rho1, double reg,
-                     double2* reginv, hipStream_t st) {
+                     double2* reginv, hipStream_t st, const int* actK) {
   const unsigned grid = (unsigned)(Pc / 16 * nxs + 3);
 #define CALL_RG(NO)                                                                                           \
   hipLaunchKernelGGL(k_rhs_gram<NO>, dim3(grid), dim3(256), 0, st, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, \
-                     partr, rho1, reg, reginv)
+                     partr, rho1, reg, reginv, actK)
   CODY_NO_SWITCH(nO, CALL_RG)
 #undef CALL_RG
 }

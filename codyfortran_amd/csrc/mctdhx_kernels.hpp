@@ -153,7 +153,7 @@ void launch_rhs_wsum(const double2* rho1, int nO, double reg, double2* reginv, c
 void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs,
                     const double2* orbs, const double* w, const double2* hphi, const double2* r2can,
                     const double2* reginv, int nG, int nO, int mode, double2* draw, double2* part, hipStream_t st,
-                    const double* qf = nullptr);
+                    const double* qf = nullptr, const int* act = nullptr, int actShift = 4);
 int rhs_post_energy_parts(int Pc, int nO);
 void launch_rhs_post(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
                      double2* h1, const double2* rho1, const double2* rho2, double* epart, const double2* orbs,
@@ -199,7 +199,8 @@ constexpr int corr_q_blocks(int nO) { return (corr_q_doubles(nO) / 4 + 255) / 25
 // Q beside the Gram partials (which the correlation launch then leaves out): the launch between the
 // k-split sum (R) and launch_rhs_mid from kCorrQMinOrbs orbitals
 void launch_rhs_qform(int nO, const double2* r2can, const double2* reginv, double* qf, const double* STt,
-                      const double* WT, int nGp, int Pc, double* Gpart, int nxs, hipStream_t st);
+                      const double* WT, int nGp, int Pc, double* Gpart, int nxs, hipStream_t st,
+                      const int* act = nullptr, int actShift = 4);
 constexpr int kSkipMaxBlocks16 = 1024;  // longest grid (in 16-point blocks) whose zero k blocks are skipped
 // shape check of the k-split contraction launchers (0 or CODY_ERR_ARG, nothing launched)
 int check_contraction_shape(const char* who, int nGp, int Pc, int nsplit, int maxTiles, int kBlocks);
@@ -231,7 +232,7 @@ int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const in
 // R = rho1 (rho1^2 + eps)^-1 (one wave)
 void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs, int nO,
                      const double2* part, int nPart, double2* partr, const double2* rho1, double reg,
-                     double2* reginv, hipStream_t st);
+                     double2* reginv, hipStream_t st, const int* actK = nullptr);
 int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_gram(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
