@@ -372,7 +372,7 @@ def pmc_traffic(kname: str, workload: str, skip: bool = False):
 # launch), so the summary splits the counter rows at their dispatches.
 PMC_SEGMENTS = [("fp64", "dense"), ("fp64", "skip"), ("int8_sliced", "dense"), ("int8_sliced", "skip")]
 # the launches of a right-hand side besides the contraction kernel
-RHS_LAUNCHES = ("k_rhs_front", "k_rhs_wsum", "k_rhs_mid", "k_rhs_gram", "k_rhs_post",
+RHS_LAUNCHES = ("k_rhs_front", "k_rhs_wsum", "k_rhs_qform", "k_rhs_mid", "k_rhs_gram", "k_rhs_post",
                 "k_ci_apply_project")
 
 
