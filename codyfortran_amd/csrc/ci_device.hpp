@@ -192,22 +192,35 @@ __device__ __forceinline__ double2 ci_config_sum(int ic, int ln, const double2* 
                                                  const int* __restrict__ s_ptr, const CiEntryDev* __restrict__ sing,
                                                  const int* __restrict__ d_ptr, const CiEntryDev* __restrict__ dbl) {
   double2 acc = cmk(0, 0);
-  for (int q = s_ptr[ic] + ln; q < s_ptr[ic + 1]; q += 64) {
+  const int s0 = s_ptr[ic], s1 = s_ptr[ic + 1], d0 = d_ptr[ic], d1 = d_ptr[ic + 1];
+  for (int q = s0 + ln; q < s1; q += 64) {
     const CiEntryDev en = sing[q];
     const int i = en.code / nO, j = en.code % nO;
     cfma(acc, cscale(h1[i + j * nO], en.f), C[en.jc]);
   }
-  for (int q = d_ptr[ic] + ln; q < d_ptr[ic + 1]; q += 64) {
-    const CiEntryDev en = dbl[q];
-    int code = en.code;
-    const int j2 = code % nO;
-    code /= nO;
-    const int i2 = code % nO;
-    code /= nO;
-    const int j1 = code % nO;
-    const int i1 = code / nO;
-    if (i1 == i2 || j1 == j2) continue;
-    cfma(acc, cscale(h2lin_from_G(G, Pc, nO, i1, i2, j1, j2), en.f), C[en.jc]);
+  // the doubles in rounds of R entries per lane: a round's table entries are loaded together, then
+  // its G and C gathers, so a round costs two memory round trips (each lane still adds its entries
+  // in the order q = ln, ln + 64, ...)
+  constexpr int R = 4;
+  for (int qb = d0 + ln; qb < d1; qb += R * 64) {
+    CiEntryDev en[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int q = qb + 64 * u;
+      en[u] = q < d1 ? dbl[q] : CiEntryDev{0, 0, 0, 0.0f};
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      int code = en[u].code;
+      const int j2 = code % nO;
+      code /= nO;
+      const int i2 = code % nO;
+      code /= nO;
+      const int j1 = code % nO;
+      const int i1 = code / nO;
+      if (qb + 64 * u >= d1 || i1 == i2 || j1 == j2) continue;
+      cfma(acc, cscale(h2lin_from_G(G, Pc, nO, i1, i2, j1, j2), en[u].f), C[en[u].jc]);
+    }
   }
   acc.x = wave_sum(acc.x);
   acc.y = wave_sum(acc.y);
