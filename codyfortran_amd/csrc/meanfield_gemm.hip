@@ -228,10 +228,14 @@ __device__ __forceinline__ bf8_t digits_bf16(int w0, int w1) {
 // digit (|d| <= 127) is exact in BF16, a product of two exact in FP32, and a diagonal's sum over
 // one block (|sum| <= 5 127^2 64 < 2^24) exact too, so the block sums, and W, equal the int8
 // path's bit for bit.  The lane's 16 digit bytes (k = 16 (l/16) .. +15) feed two MFMAs of eight.
+//
+// pred >= 0: every thread's vote whether the tile is needed, taken at the loop's first barrier; when
+// no thread votes 1 the function returns false at once (v unset).  pred < 0: no vote (true).
 template <int NT, int PF, bool BF = false>
-__device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
+__device__ __forceinline__ bool mf_i8_r16_acc(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
                                               const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
-                                              const int* __restrict__ actK, int nGp, int Pc, double (&v)[4]) {
+                                              const int* __restrict__ actK, int nGp, int Pc, double (&v)[4],
+                                              int pred = -1) {
   static_assert(PF == 2, "the loop below keeps one block in flight");
   using acc_t = typename std::conditional<BF, f4_t, i4_t>::type;
   constexpr int NS = kInt8Slices;
@@ -255,7 +259,13 @@ __device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__
       }
       if (lane == 0) sCount = n;
     }
-    __syncthreads();
+    if (pred >= 0) {
+      if (!__syncthreads_or(pred)) return false;
+    } else {
+      __syncthreads();
+    }
+  } else if (pred >= 0 && !__syncthreads_or(pred)) {
+    return false;
   }
   const int total = actK ? __builtin_amdgcn_readfirstlane(sCount) : nK64;
   const int8_t* ab = As + (size_t)rt * nK64 * NS * 1024 + 16 * lane;
@@ -335,6 +345,7 @@ __device__ __forceinline__ void mf_i8_r16_acc(int rt, const int8_t* __restrict__
   const int row0 = rt * 16 + 4 * q;
 #pragma unroll
   for (int u = 0; u < 4; ++u) v[u] = ldexp(v[u], eA[row0 + u]);
+  return true;
 }
 
 // The body takes the workgroup's row tile rt.
@@ -417,11 +428,26 @@ __device__ __forceinline__ void mf_i8_orb_body(int rt, const int8_t* __restrict_
     }
     sAB[tid] = a + 16 * (a + 1 + r);
   }
+  // A tile whose 16 points carry no orbital value at all (every phi exactly zero there) needs no
+  // W: its correlation terms are products with phi and its Gram terms products with the pair
+  // sources, both exact zeros.  It writes zeros for its rows of W^T, its c and its partials (the
+  // test rides on the contraction's first barrier).
   double v[4];
-  mf_i8_r16_acc<NT, PF, BF>(rt, As, eA, Bs, eSb, actK, nGp, NT * 16, v);
+  const bool live = mf_i8_r16_acc<NT, PF, BF>(rt, As, eA, Bs, eSb, actK, nGp, NT * 16, v,
+                                              in && (phiL.x != 0.0 || phiL.y != 0.0) ? 1 : 0);
   const int t = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63, q = lane >> 4, rr = lane & 15;
   double* dst = WT + (size_t)(t * 16 + rr) * nGp + x0 + 4 * q;
+  if (!live) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) dst[u] = 0.0;
+    if (in) o.corrc[(size_t)pj * o.nG + gx] = cmk(0, 0);
+    for (int e = tid; e < 2 * n2; e += NTH) {
+      const int half = e / n2, f = e % n2;
+      o.part[((size_t)half * (nGp >> 4) + rt) * n2 + f] = cmk(0, 0);
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     dst[u] = v[u];

@@ -143,12 +143,18 @@ def test_full_batch_matches_oracle():
         print(f"  batch autotune: schedule {hip.Batch.SCHEDULES[pick]}, ms per batched RHS by schedule "
               + ", ".join(f"{m:.3f}" for m in ms))
         assert 0 <= pick < len(ms) and all(m > 0 for m in ms) and ms[pick] == min(ms)
-        for o in d_o:
-            o.zero_()
-        b.time_derivative_dev(ps, po, 0.0, stream)
-        torch.cuda.synchronize()
-        for w, o, f in zip(ws, d_o, first):
-            assert np.array_equal(o.cpu().numpy(), f), w.name
+        # the autotuned schedule, then fixed ones: one chain, and four with every group split
+        for sched in (None, (1, 1.0), (4, 0.25)):
+            if sched:
+                b.set_schedule(*sched)
+            for o in d_o:
+                o.zero_()
+            b.time_derivative_dev(ps, po, 0.0, stream)
+            torch.cuda.synchronize()
+            for w, o, f in zip(ws, d_o, first):
+                assert np.array_equal(o.cpu().numpy(), f), (sched, w.name)
+        with pytest.raises(hip.CodyError):
+            b.set_schedule(5, 1.0)  # (four chains at most)
         b.close()
         worst = 0.0
         for (ctx, wts), w, st, o in zip(ctxs, ws, states, d_o):
