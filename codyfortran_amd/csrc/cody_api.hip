@@ -144,7 +144,8 @@ void enqueue_pair_src(cody_ctx* c, const double2* orbs, hipStream_t st) {
 // Returns 0, or the launcher's error (nothing of the contraction launched) when the shape is
 // outside what the kernels take.
 int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(const char*)>& mark,
-                        int* partialSplit = nullptr, const OrbCorrArgs* orb = nullptr) {
+                        int* partialSplit = nullptr, const OrbCorrArgs* orb = nullptr,
+                        const double2* rhsOrbs = nullptr) {
   const size_t ns = (size_t)c->Pc * c->nGp;
   if (partialSplit) *partialSplit = 0;
   const bool bf = c->precision == CODY_PREC_BF16_SLICED;
@@ -157,7 +158,9 @@ int enqueue_contraction(cody_ctx* c, hipStream_t st, const std::function<void(co
                                st, bf));
           mark(bf ? "k_mf_i8_orb<bf16>" : "k_mf_i8_orb");
         } else {
-          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->Pc, st, bf));
+          // (in a right-hand side, tiles with no orbital value skip W)
+          CHK(launch_mf_i8_r16(c->d_Vi8, c->d_eV, c->d_Si8, c->d_eSb, act_flags(c), c->d_WT, c->nGp, c->Pc, st, bf,
+                               rhsOrbs, c->nG, c->nO));
           mark(bf ? "k_mf_i8_r16<bf16>" : "k_mf_i8_r16");
         }
         return CODY_OK;
@@ -507,7 +510,7 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   if (double* sr = plain_st(c)) launch_pair_src_rows(orbs, c->d_w, c->nG, c->nGp, c->nO, c->Pc, sr, st);
   int ps = 0;
   const OrbCorrArgs oa{orbs, c->d_w, c->d_hphi, c->d_r2can, c->nG, mode, c->d_draw, c->d_part};
-  CHK(enqueue_contraction(c, st, mark, &ps, fused ? &oa : nullptr));
+  CHK(enqueue_contraction(c, st, mark, &ps, fused ? &oa : nullptr, orbs));
   if (fused) {
     launch_rhs_gram(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, c->nO, c->d_part, c->nGp / 16, c->d_partr,
                     c->d_rho1, c->reg, c->d_reginv, st, gram_flags(c));
@@ -1191,7 +1194,7 @@ int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
   if (g.fused) {
     CHK(launch_grp_mf_i8_orb(g.nO, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   } else if (g.int8) {
-    CHK(launch_grp_mf_i8_r16(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
+    CHK(launch_grp_mf_i8_r16(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st, g.nO));
   } else {
     CHK(launch_grp_mf_f64_w(g.Pc, g.d_inst, map(kPlanContract), g.cnt[kPlanContract], st));
   }

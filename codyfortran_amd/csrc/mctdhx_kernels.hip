@@ -555,11 +555,27 @@ __device__ __forceinline__ void corr_q_body(int bx, int g, const double2* __rest
       pr[th][i] = v.x;
       pim[th][i] = v.y;
     }
+  int any = 0;
   for (int e = tid; e < NO * 64; e += 256) {
     const int i = e >> 6, xx = e & 63;
-    sPhi[i][xx] = x0 + xx < nG ? orbs[(size_t)i * nG + x0 + xx] : cmk(0, 0);
+    const double2 v = x0 + xx < nG ? orbs[(size_t)i * nG + x0 + xx] : cmk(0, 0);
+    sPhi[i][xx] = v;
+    any |= (v.x != 0.0 || v.y != 0.0);
   }
   if (tid < 64) sw[tid] = x0 + tid < nG ? w[x0 + tid] : 0.0;
+  // 64 points with no orbital value: the correlation (sum_k1 phi_k1 O_k1j) and the overlap partials
+  // (products with conj(phi_i)) are exact zeros, so draw_j = h phi_j without the MFMA work
+  if (!__syncthreads_or(any)) {
+    const int xx = tid & 63, gx = x0 + xx;
+    const bool in = gx < nG, hOn = mode == kModeFull || mode == kModeNonLinConsistent;
+    for (int u = tid >> 6; u < nj; u += 4) {
+      const int j = g + G * u;
+      const double2 hj = (in && hOn) ? hphi[(size_t)j * nG + gx] : cmk(0, 0);
+      if (in) draw[(size_t)j * nG + gx] = cadd(hj, cmk(0, 0));
+    }
+    for (int f = tid; f < NO * nj; f += 256) part[(size_t)bx * n2 + f % NO + NO * (g + G * (f / NO))] = cmk(0, 0);
+    return;
+  }
   const bool reCol = (n & 1) == 0;
   for (int u = 0; u < nj; ++u) {
     const int j = g + G * u;

@@ -97,7 +97,7 @@ void launch_grp_post(int nO, const GroupInstance* gi, const int2* map, int nBloc
 void launch_grp_apply(const GroupInstance* gi, const int2* map, int nBlocks, int nO, hipStream_t st);
 // the contractions, Pc = 16 NT for every instance of the group (CODY_ERR_ARG otherwise)
 int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
-int launch_grp_mf_i8_r16(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
+int launch_grp_mf_i8_r16(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st, int nO);
 
 void launch_zero(double2* p, int n, hipStream_t st);
 // y += x (n complex values)
@@ -237,7 +237,8 @@ int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int n
 void launch_grp_gram(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
                      int nGp, int Pc, hipStream_t st,
-                     bool bf16 = false);
+                     bool bf16 = false,
+                     const double2* orbs = nullptr, int nG = 0, int nO = 0);
 void launch_mf_i8(const int8_t* As, const int* eA, const int8_t* Bs, const int* fB, double* WT, int nGp,
                   int Pc, hipStream_t st);
 void launch_mf_f32x3(const float* Ah, const float* Al, const float* Bh, const float* Bl, double* WT, int nGp,

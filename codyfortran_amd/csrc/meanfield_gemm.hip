@@ -349,17 +349,27 @@ __device__ __forceinline__ bool mf_i8_r16_acc(int rt, const int8_t* __restrict__
 }
 
 // The body takes the workgroup's row tile rt.
+// With orbs set (a right-hand side, whose W is only multiplied by phi and by the pair sources), a
+// tile whose 16 points carry no orbital value writes zero rows of W^T instead (the vote rides on
+// the contraction's first barrier); without (the mean fields themselves) every row is formed.
 template <int NT, int PF, bool BF = false>
 __device__ __forceinline__ void mf_i8_r16_body(int rt, const int8_t* __restrict__ As, const int* __restrict__ eA,
                                                const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
-                                               const int* __restrict__ actK, int nGp, int Pc, double* __restrict__ WT) {
+                                               const int* __restrict__ actK, int nGp, int Pc, double* __restrict__ WT,
+                                               const double2* __restrict__ orbs = nullptr, int nG = 0, int nO = 0) {
+  int pred = -1;
+  if (orbs) {  // (NT 64 >= 16 nO threads for every orbital count)
+    const int px = threadIdx.x & 15, pj = threadIdx.x >> 4, gx = rt * 16 + px;
+    const double2 ph = (pj < nO && gx < nG) ? orbs[(size_t)pj * nG + gx] : make_double2(0.0, 0.0);
+    pred = (ph.x != 0.0 || ph.y != 0.0) ? 1 : 0;
+  }
   double v[4];
-  mf_i8_r16_acc<NT, PF, BF>(rt, As, eA, Bs, eSb, actK, nGp, Pc, v);
+  const bool live = mf_i8_r16_acc<NT, PF, BF>(rt, As, eA, Bs, eSb, actK, nGp, Pc, v, pred);
   const int t = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double* dst = WT + (size_t)(t * 16 + (lane & 15)) * nGp + rt * 16 + 4 * (lane >> 4);
 #pragma unroll
-  for (int u = 0; u < 4; ++u) dst[u] = v[u];
+  for (int u = 0; u < 4; ++u) dst[u] = live ? v[u] : 0.0;
 }
 
 // ------------------------------------------------------------------ int8 contraction + orbital correlation
@@ -543,8 +553,9 @@ template <int NT, bool BF = false>
 __global__ __launch_bounds__(NT * 64) void k_mf_i8_r16(const int8_t* __restrict__ As, const int* __restrict__ eA,
                                                         const int8_t* __restrict__ Bs, const int* __restrict__ eSb,
                                                         const int* __restrict__ actK, int nGp, int Pc,
-                                                        double* __restrict__ WT) {
-  mf_i8_r16_body<NT, kR16Prefetch, BF>(blockIdx.x, As, eA, Bs, eSb, actK, nGp, Pc, WT);
+                                                        double* __restrict__ WT, const double2* __restrict__ orbs,
+                                                        int nG, int nO) {
+  mf_i8_r16_body<NT, kR16Prefetch, BF>(blockIdx.x, As, eA, Bs, eSb, actK, nGp, Pc, WT, orbs, nG, nO);
 }
 
 // ---------------------------------------------------------------------------------------- int8 slices
@@ -826,10 +837,10 @@ __global__ __launch_bounds__(NT <= 5 ? 512 : 256) void k_grp_mf_f64_w(const Grou
 
 template <int NT>
 __global__ __launch_bounds__(NT * 64) void k_grp_mf_i8_r16(const GroupInstance* __restrict__ gi,
-                                                            const int2* __restrict__ map) {
+                                                            const int2* __restrict__ map, int nO) {
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
-  mf_i8_r16_body<NT, kR16Prefetch>(m.y, a.Vi8, a.eV, a.Si8, a.eSb, a.actC, a.nGp, a.Pc, a.WT);
+  mf_i8_r16_body<NT, kR16Prefetch>(m.y, a.Vi8, a.eV, a.Si8, a.eSb, a.actC, a.nGp, a.Pc, a.WT, a.C + a.nC, a.nG, nO);
 }
 
 int launch_grp_mf_f64_w(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
@@ -879,8 +890,8 @@ int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int n
   return 0;
 }
 
-int launch_grp_mf_i8_r16(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
-#define CALL_GI(NT) hipLaunchKernelGGL(k_grp_mf_i8_r16<NT>, dim3(nBlocks), dim3((NT) * 64), 0, st, gi, map)
+int launch_grp_mf_i8_r16(int Pc, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st, int nO) {
+#define CALL_GI(NT) hipLaunchKernelGGL(k_grp_mf_i8_r16<NT>, dim3(nBlocks), dim3((NT) * 64), 0, st, gi, map, nO)
   switch (Pc / 16) {
     case 1: CALL_GI(1); break;
     case 2: CALL_GI(2); break;
@@ -944,7 +955,7 @@ int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const in
 // activity flags of S (pair_src_i8_body), or null to contract every k block.  Returns 0, or
 // CODY_ERR_ARG without launching anything when the shape is outside these limits.
 int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
-                     int nGp, int Pc, hipStream_t st, bool bf16) {
+                     int nGp, int Pc, hipStream_t st, bool bf16, const double2* orbs, int nG, int nO) {
   if (nGp < 64 || nGp % 64 || Pc < 16 || Pc > 16 * kR16MaxTiles || Pc % 16 || nGp / 64 > kR16MaxBlocks64)
     return set_error(CODY_ERR_ARG, "k_mf_i8_r16: nGp must be a positive multiple of 64, at most 64 x 256, "
                                    "and Pc a multiple of 16 up to 160");
@@ -952,10 +963,10 @@ int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const in
   do {                                                                                                          \
     if (bf16)                                                                                                   \
       hipLaunchKernelGGL((k_mf_i8_r16<NT, true>), dim3(nGp / 16), dim3((NT) * 64), 0, st, As, eA, Bs, eSb, actK, \
-                         nGp, Pc, WT);                                                                          \
+                         nGp, Pc, WT, orbs, nG, nO);                                                            \
     else                                                                                                        \
       hipLaunchKernelGGL((k_mf_i8_r16<NT, false>), dim3(nGp / 16), dim3((NT) * 64), 0, st, As, eA, Bs, eSb,     \
-                         actK, nGp, Pc, WT);                                                                    \
+                         actK, nGp, Pc, WT, orbs, nG, nO);                                                      \
   } while (0)
   switch (Pc / 16) {
     case 1: CALL_R16(1); break;
