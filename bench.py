@@ -67,6 +67,8 @@ def parse():
     p.add_argument("--batch", type=int, default=64,
                    help="independent instances of the batched measurement (0: skip it)")
     p.add_argument("--batch-steps", type=int, default=20)
+    p.add_argument("--batch-int8-only", action="store_true",
+                   help="time the batch with the int8-sliced contraction only (a profiling run of that form)")
     p.add_argument("--prop-steps", type=int, default=1000,
                    help="steps of each propagation measurement (0: skip them)")
     p.add_argument("--adjoint-steps", type=int, default=1000,
@@ -545,6 +547,18 @@ def batched_measurement(args, torch, wlm, check: bool):
         return max(state_rel(o.cpu().numpy().view(np.complex128), r, w.nC, wts, w.M)
                    for (ctx, wts), w, o, r in zip(ctxs, ws, d_o, refs))
 
+    if args.batch_int8_only:
+        for ctx, _ in ctxs:
+            ctx.set_precision(hip.PREC_INT8_SLICED)
+        pick8, _ = b.autotune(ps, po)
+        ms8 = timed()
+        b.close()
+        for c, _ in ctxs:
+            c.close()
+        return {"instances": len(ws), "value": len(ws) * 1000.0 / ms8, "unit": "instance RHS evals/s",
+                "ms_per_batched_rhs": ms8, "form": "int8_sliced", "dtype": "f64 (int8_sliced contraction)",
+                "schedule": {"chains": hip.Batch.SCHEDULES[pick8][0],
+                             "split_threshold": hip.Batch.SCHEDULES[pick8][1]}}
     ms_default = timed()
     pick, cand_ms = b.autotune(ps, po)
     ms = timed()
