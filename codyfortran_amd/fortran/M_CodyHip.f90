@@ -7,7 +7,8 @@ module M_CodyHip
   implicit none
 
   integer(c_int), parameter :: CODY_OK = 0
-  integer(c_int), parameter :: CODY_PREC_FP64 = 0
+  integer(c_int), parameter :: CODY_PREC_FP64 = 0, CODY_PREC_SPLIT_FP32 = 1, CODY_PREC_BF16X3 = 2
+  integer(c_int), parameter :: CODY_PREC_INT8_SLICED = 3, CODY_PREC_BF16_SLICED = 4
 
   interface
     function cody_last_error() bind(C, name="cody_last_error") result(p)
@@ -148,6 +149,13 @@ module M_CodyHip
       type(c_ptr), value :: ctx
       complex(c_double_complex) :: state(*)
       real(c_double) :: d
+      integer(c_int) :: r
+    end function
+
+    function cody_set_precision(ctx, mode) bind(C, name="cody_set_precision") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
       integer(c_int) :: r
     end function
 

@@ -25,6 +25,9 @@ module M_Method_Mb_OrbBased_MctdhxHip
     !> .true.: the split step's nonlinear part is the full right-hand side minus the linear part
     !> (second order in dt); .false. (default): the reference's TimeDerivativeCoeffsPlusOrbsNonLin
     logical :: consistentSplitQ = .false.
+    !> precision of the mean-field contraction (CODY_PREC_* of M_CodyHip): FP64 by default; the
+    !> int8-sliced and BF16-sliced forms give W within 1e-10 of FP64 (the adjoint needs FP64)
+    integer(I32) :: contractionPrecision = 0
     integer(I32) :: device = 0
   end type
 

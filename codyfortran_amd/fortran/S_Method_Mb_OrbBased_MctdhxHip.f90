@@ -64,6 +64,8 @@ contains
                                                        in%intSoftening2, in%intDampening), &
                        "set_interaction_softyukawa")
     if (in%consistentSplitQ) call CodyHip_Check(cody_set_split(MctdhxHip_ctx, 1_c_int), "set_split")
+    if (in%contractionPrecision /= CODY_PREC_FP64) &
+      call CodyHip_Check(cody_set_precision(MctdhxHip_ctx, int(in%contractionPrecision, c_int)), "set_precision")
     if (allocated(Method_state)) deallocate (Method_state)
     allocate (Method_state(nS))
     Method_state(:) = (0.0_R64, 0.0_R64)

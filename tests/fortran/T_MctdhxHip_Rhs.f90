@@ -12,7 +12,7 @@ program T_MctdhxHip_Rhs
   implicit none
 
   type(T_MctdhxHip_Input) :: input
-  character(len=1024) :: inFile, outDir
+  character(len=1024) :: inFile, outDir, precArg
   complex(R64), allocatable :: dState(:)
   real(R64) :: t, dt, e0, e1, gJ, grad(3), chebLo, chebHi
   integer :: io, iStep
@@ -20,7 +20,8 @@ program T_MctdhxHip_Rhs
 
   call get_command_argument(1, inFile)
   call get_command_argument(2, outDir)
-  if (len_trim(inFile) == 0 .or. len_trim(outDir) == 0) error stop "usage: t_mctdhx_hip <state> <outdir>"
+  call get_command_argument(3, precArg)
+  if (len_trim(inFile) == 0 .or. len_trim(outDir) == 0) error stop "usage: t_mctdhx_hip <state> <outdir> [precision]"
 
   ! 1D FEDVR MCTDHF: 100 elements x 5 GLL points on [-100, 100], N=2 fermions in M=4 orbitals
   input%xmin = -100.0_R64
@@ -38,6 +39,7 @@ program T_MctdhxHip_Rhs
   input%regularizationParameter = 1.0e-8_R64
   input%krylovDim = 16
   input%tolerance = 1.0e-12_R64
+  if (len_trim(precArg) > 0) read (precArg, *) input%contractionPrecision
 
   call Method_Mb_OrbBased_MctdhxHip_Fabricate(input)
   call Propagator_SplitStepHip_Fabricate
@@ -49,7 +51,7 @@ program T_MctdhxHip_Rhs
   close (io)
 
   allocate (dState(size(Method_state)))
-  call WritePieces
+  if (input%contractionPrecision == 0) call WritePieces
   call Method_TimeDerivative(dState, Method_state, 0.0_R64)
   open (newunit=io, file=trim(outDir)//"/dstate.bin", access="stream", form="unformatted", status="replace")
   write (io) dState
@@ -57,6 +59,10 @@ program T_MctdhxHip_Rhs
 
   e0 = Method_GetEnergy(0.0_R64)
   write (*, '(A,ES24.16)') "energy0 ", e0
+  if (input%contractionPrecision /= 0) then  ! (the reduced-precision run: the right-hand side and energy)
+    call Method_Mb_OrbBased_MctdhxHip_Finalize
+    stop
+  end if
 
   t = 0.0_R64
   dt = 0.01_R64

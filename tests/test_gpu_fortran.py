@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from codyfortran_amd import build as cbuild
+from codyfortran_amd import hip
 from codyfortran_amd import workloads as wl
 from tests.common import oracle_system, state_rel
 
@@ -143,3 +144,26 @@ def test_fortran_split_chain_matches_oracle():
     mw = s.metric_weights()
     drift_o = abs(np.sqrt(np.sum(mw * np.abs(so) ** 2)) - np.sqrt(np.sum(mw * np.abs(state) ** 2)))
     assert abs(float(vals["normDrift"]) - drift_o) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [hip.PREC_INT8_SLICED, hip.PREC_BF16_SLICED])
+def test_fortran_driver_with_sliced_contraction(precision):
+    """The fabrication input's contractionPrecision selects the contraction form through the
+    Fortran leaf: the right-hand side and the energy within 1e-9 of the FP64 oracle."""
+    w = wl.WORKLOADS["mctdhf_1d_401"]
+    s, x, wts = oracle_system(w)
+    state = wl.seeded_state(x, wts, w.N, w.M, seed=11)
+    with tempfile.TemporaryDirectory() as d:
+        inp = os.path.join(d, "state.in")
+        state.astype(np.complex128).tofile(inp)
+        r = subprocess.run([EXE, inp, d, str(precision)], capture_output=True, text=True, timeout=120)
+        print(r.stdout, r.stderr)
+        assert r.returncode == 0
+        dstate = np.fromfile(os.path.join(d, "dstate.bin"), dtype=np.complex128)
+    vals = dict(line.split() for line in r.stdout.splitlines() if len(line.split()) == 2)
+    err = state_rel(dstate, s.time_derivative(state), w.nC, wts, w.M)
+    print(f"precision {precision}: RHS rel err {err:.3e}")
+    assert err < 1e-9
+    e0 = s.energy(state)
+    assert abs(float(vals["energy0"]) - e0) < 1e-9 * abs(e0)
