@@ -375,7 +375,7 @@ def pmc_traffic(kname: str, workload: str, skip: bool = False):
 PMC_SEGMENTS = [("fp64", "dense"), ("fp64", "skip"), ("int8_sliced", "dense"), ("int8_sliced", "skip")]
 # the launches of a right-hand side besides the contraction kernel
 RHS_LAUNCHES = ("k_rhs_front", "k_rhs_wsum", "k_rhs_qform", "k_rhs_mid", "k_rhs_gram", "k_rhs_post",
-                "k_ci_apply_project")
+                "k_rhs_final", "k_ci_apply_project")
 
 
 def pmc_probe(torch, wlm, wl):

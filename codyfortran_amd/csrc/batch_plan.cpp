@@ -65,9 +65,13 @@ void plan_block_counts(const PlanShape& s, int64_t cnt[kPlanKinds]) {
   // Gram + orbital correlation per 64 points and orbital, or the correlation on FP64 MFMA per 64
   // points and orbital group; the int8 path's contraction forms the correlation, and beside the Gram
   // two workgroups sum its two sets of overlap partials and one forms R
-  cnt[kPlanMid] = s.fused ? nGram + 3 : (corrQ ? (int64_t)(s.nGp / 64) * kPlanCorrQGroups : nGram + (int64_t)(s.nGp / 64) * s.nO);
-  cnt[kPlanPost] = nRed + nObx;
-  cnt[kPlanApply] = (s.nC + 3) / 4;
+  // (the fused int8 path: G directly, one 1024-thread workgroup per 16 x 16 tile, then h1, the two
+  // overlap sums and R; no separate G reduction and projection launch: the projection runs beside
+  // the CI apply)
+  const int64_t nt = s.Pc / 16;
+  cnt[kPlanMid] = s.fused ? nt * nt + 4 : (corrQ ? (int64_t)(s.nGp / 64) * kPlanCorrQGroups : nGram + (int64_t)(s.nGp / 64) * s.nO);
+  cnt[kPlanPost] = s.fused ? 0 : nRed + nObx;
+  cnt[kPlanApply] = (s.nC + 3) / 4 + (s.fused ? nObx : 0);
 }
 
 std::string build_group_plan(const std::vector<PlanShape>& members, GroupPlan& out) {

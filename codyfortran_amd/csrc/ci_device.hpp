@@ -303,4 +303,26 @@ __device__ __forceinline__ void gram_reduce_energy_body(int bx, const double* __
 }
 
 
+// CI H-apply and CI projection in one launch, one wave per configuration: dCraw = H C (as
+// k_ci_apply) and dC = -i (dCraw - E C) (Coeffs_ProjectOnSubspace, src/Coeffs/S_Coeffs.f90:105-111),
+// with <C|dCraw> = <C|H C> = E taken from the nE energy partials of the G reduction (summed in a
+// fixed order by every wave) instead of a reduction over dCraw.
+__device__ __forceinline__ void ci_apply_project_body(int bx, const double2* __restrict__ C, int nC, int nO, int Pc,
+                                                      const double* __restrict__ G, const double2* __restrict__ h1,
+                                                      const DevCiTables& t, const double* __restrict__ epart, int nE,
+                                                      double2* __restrict__ dCraw, double2* __restrict__ dC) {
+  const int ic = bx * 4 + (threadIdx.x >> 6), ln = threadIdx.x & 63;
+  if (ic >= nC) return;
+  // the energy partials are loaded before the configuration's sums and added after them
+  double e = 0.0;
+  for (int b = ln; b < nE; b += 64) e += epart[b];
+  const double2 acc = ci_config_sum(ic, ln, C, nO, Pc, G, h1, t.s_ptr, t.singles, t.d_ptr, t.doubles);
+  e = wave_sum(e);
+  if (ln == 0) {
+    dCraw[ic] = acc;
+    dC[ic] = mul_minus_i(csub(acc, cscale(C[ic], e)));
+  }
+}
+
+
 }  // namespace cody

@@ -512,9 +512,18 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
   const OrbCorrArgs oa{orbs, c->d_w, c->d_hphi, c->d_r2can, c->nG, mode, c->d_draw, c->d_part};
   CHK(enqueue_contraction(c, st, mark, &ps, fused ? &oa : nullptr, orbs));
   if (fused) {
-    launch_rhs_gram(c->d_STt, c->d_WT, c->nGp, c->Pc, c->d_Gpart, c->nxs, c->nO, c->d_part, c->nGp / 16, c->d_partr,
-                    c->d_rho1, c->reg, c->d_reginv, st, gram_flags(c));
+    // G directly with its energy shares | h1 | the overlap-partial sums | R; then the CI apply beside
+    // the orbital projection (which does not depend on G)
+    launch_rhs_gram(c->d_STt, c->d_WT, c->nGp, c->Pc, c->nO, gram_flags(c), c->d_rho2, c->d_G, c->d_epart, c->d_part1,
+                    nPart1, c->d_h1, c->d_part, c->nGp / 16, c->d_partr, c->d_rho1, c->reg, c->d_reginv, st);
     mark("k_rhs_gram");
+    const bool hOn = mode == kModeFull || mode == kModeNonLinConsistent;
+    launch_rhs_final(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_epart, gram_direct_energy_parts(c->Pc),
+                     c->d_dCraw, dC, orbs, hOn ? c->d_hphi : nullptr, c->d_draw, c->d_reginv, c->d_partr, c->nG,
+                     dorbs, mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st);
+    mark("k_rhs_final");
+    if (energy) launch_ci_energy(c->nO, c->Pc, c->d_G, c->d_h1, c->d_rho1, c->d_rho2, energy, st);
+    return CODY_OK;
   } else {
     launch_rhs_wsum(c->d_rho1, c->nO, c->reg, c->d_reginv, c->d_Wpart, ps, (size_t)c->Pc * c->nGp, c->d_WT, st);
     mark("k_rhs_wsum");
@@ -528,12 +537,9 @@ int enqueue_rhs(cody_ctx* c, const double2* state, double2* dstate, double t, in
                    gram_flag_shift(c));
     mark("k_rhs_mid");
   }
-  // (the int8 projection: h phi where the mode has the one-body part, c_m in draw, the summed partials)
-  const bool hOn = mode == kModeFull || mode == kModeNonLinConsistent;
   launch_rhs_post(c->d_Gpart, c->nxs, c->Pc, c->d_G, c->d_part1, nPart1, c->d_h1, c->d_rho1, c->d_rho2, c->d_epart,
-                  orbs, fused ? (hOn ? c->d_hphi : nullptr) : c->d_draw, fused ? c->d_partr : c->d_part,
-                  fused ? 1 : c->nPart, c->nG, c->nO, dorbs, mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st,
-                  fused ? c->d_draw : nullptr, c->d_reginv);
+                  orbs, c->d_draw, c->d_part, c->nPart, c->nG, c->nO, dorbs,
+                  mode == kModeNonLinConsistent ? c->d_hphi : nullptr, st);
   mark("k_rhs_post");
   launch_ci_apply_project(C, c->nC, c->nO, c->Pc, c->d_G, c->d_h1, c->dci, c->d_epart,
                           rhs_post_energy_parts(c->Pc, c->nO), c->d_dCraw, dC, st);
@@ -1117,7 +1123,8 @@ int batch_build_groups(cody_batch* b, void* const* d_states, void* const* d_dsta
       a.nxs = c->nxs;
       a.nPart = c->nPart;
       a.nPart1 = (c->nG + 255) / 256;
-      a.nRed = rhs_post_energy_parts(c->Pc, nO);
+      // energy shares the CI apply sums: the G reduction's, or on the fused path the direct G's
+      a.nRed = g.fused ? gram_direct_energy_parts(c->Pc) : rhs_post_energy_parts(c->Pc, nO);
       a.t = c->dci;
       a.rho1 = c->d_rho1;
       a.r2can = c->d_r2can;
@@ -1200,10 +1207,13 @@ int enqueue_group_rhs(const BatchGroup& g, hipStream_t st) {
   }
   launch_grp_wsum(g.nO, g.d_inst, map(kPlanWsum), g.cnt[kPlanWsum], st);
   launch_grp_qform(g.nO, g.d_inst, map(kPlanQform), g.cnt[kPlanQform], st);
-  if (g.fused)
+  if (g.fused) {  // G directly, h1, the overlap sums, R; then the CI apply beside the projection
     launch_grp_gram(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
-  else
-    launch_grp_mid(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
+    launch_grp_final(g.nO, g.d_inst, map(kPlanApply), g.cnt[kPlanApply], st);
+    HIPCHK(hipGetLastError());
+    return CODY_OK;
+  }
+  launch_grp_mid(g.nO, g.d_inst, map(kPlanMid), g.cnt[kPlanMid], st);
   launch_grp_post(g.nO, g.d_inst, map(kPlanPost), g.cnt[kPlanPost], st);
   launch_grp_apply(g.d_inst, map(kPlanApply), g.cnt[kPlanApply], g.nO, st);
   HIPCHK(hipGetLastError());

@@ -659,10 +659,10 @@ __global__ __launch_bounds__(256) void k_rhs_qform(const double2* __restrict__ r
 template <int NO>
 constexpr int kOverlapGroups = NO * NO >= 256 ? 1 : 256 / (NO * NO);
 
-template <int NO>
+template <int NO, int G = kOverlapGroups<NO>>
 __device__ __forceinline__ void sum_overlap_parts(const double2* __restrict__ part, int nPart,
                                                   double2 (*sG)[NO * NO], double2* sInp) {
-  constexpr int n2 = NO * NO, G = kOverlapGroups<NO>;
+  constexpr int n2 = NO * NO;
   for (int e = threadIdx.x; e < G * n2; e += blockDim.x) {
     const int g = e / n2, i = e % n2;
     double2 acc = cmk(0, 0);
@@ -1078,47 +1078,184 @@ __global__ __launch_bounds__(256) void k_rhs_mid(const double* __restrict__ STt,
   }
 }
 
-// After the fused int8 contraction (which formed c_m(x) and its overlap partials): the Gram
-// matrix's split-K partials (the first NT nxs workgroups), the fixed-order sums of the
-// contraction's nPart overlap partials A(i, j) and of its nPart partials B(i, m) into partr[0, n2)
-// and partr[n2, 2 n2) (one workgroup each), and R = rho1 (rho1^2 + eps)^{-1} (the last workgroup,
-// one wave).
+// After the fused int8 contraction (which formed c_m(x) and its overlap partials), with 1024
+// threads per workgroup:
+//   - G = S^T W directly, one 16 x 16 tile (R, T) per workgroup (the first NT^2): 16 waves take
+//     every 16th 16-point block of the active 64-point blocks of S's row tile R (actK, or every
+//     block), four blocks' loads issued before their MFMAs; the wave tiles are added in LDS in
+//     fixed order, G[c1 Pc + c2] written, and the tile's energy share sum G(c1, c2) Q(c1, c2) into
+//     epart[tile] (Q = energy_g_coef);
+//   - h1 from its partials and its energy share sum h1(k, l) rho1(l, k) into epart[NT^2];
+//   - the fixed-order sums of the contraction's nPart overlap partials A(i, j) and B(i, m) into
+//     partr[0, n2) and partr[n2, 2 n2) (one workgroup each);
+//   - R = rho1 (rho1^2 + eps)^{-1} (one wave).
+// E = sum of the NT^2 + 1 energy shares (gram_direct_energy_parts); the CI apply reads them.
 template <int NO>
-__device__ __forceinline__ void gram_sum_body(int b, const double* __restrict__ STt, const double* __restrict__ WT,
-                                              int nGp, int Pc, double* __restrict__ Gpart, int nxs,
-                                              const double2* __restrict__ part, int nPart,
-                                              double2* __restrict__ partr, const double2* __restrict__ rho1,
-                                              double reg, double2* __restrict__ reginv, const int* __restrict__ actK) {
-  constexpr int NT = (NO * (NO + 1) + 15) / 16, n2 = NO * NO, G = kOverlapGroups<NO>;
-  if (b < NT * nxs) {
-    gram_f64_body<NT>(b % NT, b / NT, STt, nGp >> 4, WT, nGp, Gpart, Pc, nGp / (4 * nxs), (size_t)Pc * Pc, actK, 6);
+__device__ __forceinline__ void gram_tile_body(int tile, const double* __restrict__ STt, const double* __restrict__ WT,
+                                               int nGp, int Pc, const int* __restrict__ actK,
+                                               const double2* __restrict__ rho2, double* __restrict__ G,
+                                               double* __restrict__ epart) {
+  constexpr int NT = (NO * (NO + 1) + 15) / 16, NW = 16, U = 4;
+  __shared__ int sList[kR16MaxBlocks64];
+  __shared__ int sCount;
+  __shared__ double sAcc[NW][16][17];
+  __shared__ double sRed[4];
+  const int R = tile / NT, T = tile % NT;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, rr = lane & 15;
+  const int nK64 = nGp >> 6, nK = nGp >> 4;
+  if (actK) {  // the active 64-point blocks of row tile R of S^T, in increasing order (wave 0)
+    if (wave == 0) {
+      const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+      int n = 0;
+      for (int base = 0; base < nK64; base += 64) {
+        const int b = base + lane;
+        const int any = b < nK64 ? actK[(size_t)R * nK64 + b] : 0;
+        const unsigned long long m = __ballot(any != 0);
+        if (any) sList[n + __popcll(m & below)] = b;
+        n += __popcll(m);
+      }
+      if (lane == 0) sCount = n;
+    }
+    __syncthreads();
+  }
+  const int nSub = 4 * (actK ? sCount : nK64);  // 16-point blocks to contract
+  const double* ap = STt + (size_t)R * nK * 256 + 2 * lane;
+  const double* bp = WT + (size_t)(16 * T + rr) * nGp + 4 * q;
+  d4_t acc = (d4_t){0.0, 0.0, 0.0, 0.0};
+  for (int j0 = wave; j0 < nSub; j0 += NW * U) {
+    d4_t a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + NW * u;
+      if (j < nSub) {
+        const int k16 = 4 * (actK ? sList[j >> 2] : (j >> 2)) + (j & 3);
+        const d2_t lo = *reinterpret_cast<const d2_t*>(ap + (size_t)k16 * 256);
+        const d2_t hi = *reinterpret_cast<const d2_t*>(ap + (size_t)k16 * 256 + 128);
+        a[u] = (d4_t){lo[0], lo[1], hi[0], hi[1]};
+        b[u] = *reinterpret_cast<const d4_t*>(bp + 16 * k16);
+      } else {
+        a[u] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        b[u] = a[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][s], b[u][s], acc, 0, 0, 0);
+  }
+  // FP64 result layout: element i of lane l is row 4 i + l / 16 (c1), column l % 16 (c2)
+#pragma unroll
+  for (int i = 0; i < 4; ++i) sAcc[wave][4 * i + q][rr] = acc[i];
+  __syncthreads();
+  double e = 0.0;
+  if (threadIdx.x < 256) {
+    const int row = threadIdx.x >> 4, col = threadIdx.x & 15;
+    double v = sAcc[0][row][col];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) v += sAcc[w][row][col];
+    const int c1 = 16 * R + row, c2 = 16 * T + col;
+    G[c1 * Pc + c2] = v;
+    e = v * energy_g_coef(c1, c2, NO, rho2);
+    e = wave_sum(e);
+    if (lane == 0) sRed[wave] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) epart[tile] = (sRed[0] + sRed[1]) + (sRed[2] + sRed[3]);
+}
+
+// h1 from its nPart1 partials and its energy share sum h1(k, l) rho1(l, k), in fixed order.
+template <int NO>
+__device__ __forceinline__ void h1_energy_body(const double2* __restrict__ part1, int nPart1, double2* __restrict__ h1,
+                                               const double2* __restrict__ rho1, double* __restrict__ eslot) {
+  constexpr int n2 = NO * NO;
+  static_assert(n2 <= 256, "one entry per thread of the first four waves");
+  __shared__ double sRed[4];
+  double e = 0.0;
+  if (threadIdx.x < 256) {
+    if (threadIdx.x < n2) {
+      const int q = threadIdx.x;
+      double2 acc = cmk(0, 0);
+#pragma unroll 8
+      for (int b = 0; b < nPart1; ++b) acc = cadd(acc, part1[(size_t)b * n2 + q]);
+      h1[q] = acc;
+      const int k = q % NO, l = q / NO;
+      const double2 r = rho1[l + k * NO];
+      e = acc.x * r.x - acc.y * r.y;
+    }
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) sRed[threadIdx.x >> 6] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *eslot = (sRed[0] + sRed[1]) + (sRed[2] + sRed[3]);
+}
+
+template <int NO>
+__device__ __forceinline__ void gram_direct_body(int b, const double* __restrict__ STt, const double* __restrict__ WT,
+                                                 int nGp, int Pc, const int* __restrict__ actK,
+                                                 const double2* __restrict__ rho2, double* __restrict__ G,
+                                                 double* __restrict__ epart, const double2* __restrict__ part1,
+                                                 int nPart1, double2* __restrict__ h1,
+                                                 const double2* __restrict__ part, int nPart,
+                                                 double2* __restrict__ partr, const double2* __restrict__ rho1,
+                                                 double reg, double2* __restrict__ reginv) {
+  // (the overlap sums take all 1024 threads: 1024 / nO^2 groups of threads)
+  constexpr int NT = (NO * (NO + 1) + 15) / 16, n2 = NO * NO, G2 = n2 >= 1024 ? 1 : 1024 / n2;
+  if (b < NT * NT) {
+    gram_tile_body<NO>(b, STt, WT, nGp, Pc, actK, rho2, G, epart);
     return;
   }
-  b -= NT * nxs;
-  if (b == 2) {
+  b -= NT * NT;
+  if (b == 0) {
+    h1_energy_body<NO>(part1, nPart1, h1, rho1, epart + NT * NT);
+    return;
+  }
+  if (b == 3) {
     if (threadIdx.x >= 64) return;
     reginv_body<NO>(rho1, reg, reginv, 64);
     return;
   }
-  __shared__ double2 sG[G][n2];
+  __shared__ double2 sG[G2][n2];
   __shared__ double2 sInp[n2];
-  sum_overlap_parts<NO>(part + (size_t)b * nPart * n2, nPart, sG, sInp);
-  for (int i = threadIdx.x; i < n2; i += blockDim.x) partr[b * n2 + i] = sInp[i];
+  sum_overlap_parts<NO, G2>(part + (size_t)(b - 1) * nPart * n2, nPart, sG, sInp);
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) partr[(b - 1) * n2 + i] = sInp[i];
 }
 
 template <int NO>
-__global__ __launch_bounds__(256) void k_rhs_gram(const double* __restrict__ STt, const double* __restrict__ WT,
-                                                  int nGp, int Pc, double* __restrict__ Gpart, int nxs,
-                                                  const double2* __restrict__ part, int nPart,
-                                                  double2* __restrict__ partr, const double2* __restrict__ rho1,
-                                                  double reg, double2* __restrict__ reginv,
-                                                  const int* __restrict__ actK) {
-  gram_sum_body<NO>(blockIdx.x, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, partr, rho1, reg, reginv, actK);
+__global__ __launch_bounds__(1024) void k_rhs_gram(const double* __restrict__ STt, const double* __restrict__ WT,
+                                                   int nGp, int Pc, const int* __restrict__ actK,
+                                                   const double2* __restrict__ rho2, double* __restrict__ G,
+                                                   double* __restrict__ epart, const double2* __restrict__ part1,
+                                                   int nPart1, double2* __restrict__ h1,
+                                                   const double2* __restrict__ part, int nPart,
+                                                   double2* __restrict__ partr, const double2* __restrict__ rho1,
+                                                   double reg, double2* __restrict__ reginv) {
+  gram_direct_body<NO>(blockIdx.x, STt, WT, nGp, Pc, actK, rho2, G, epart, part1, nPart1, h1, part, nPart, partr,
+                       rho1, reg, reginv);
+}
+
+// The fused int8 path's last launch: the CI apply and projection (the first nCi workgroups, with
+// E from the nE energy shares) beside the orbital projection with the R mixing
+// (project_corr_body), which does not depend on G.
+template <int NO>
+__global__ __launch_bounds__(256) void k_rhs_final(const double2* __restrict__ C, int nC, int Pc,
+                                                   const double* __restrict__ G, const double2* __restrict__ h1,
+                                                   DevCiTables t, const double* __restrict__ epart, int nE,
+                                                   double2* __restrict__ dCraw, double2* __restrict__ dC, int nCi,
+                                                   const double2* __restrict__ orbs, const double2* __restrict__ h,
+                                                   const double2* __restrict__ corrc,
+                                                   const double2* __restrict__ reginv,
+                                                   const double2* __restrict__ partr, int nG,
+                                                   double2* __restrict__ dorbs, const double2* __restrict__ hsub) {
+  const int b = blockIdx.x;
+  if (b < nCi) {
+    ci_apply_project_body(b, C, nC, NO, Pc, G, h1, t, epart, nE, dCraw, dC);
+    return;
+  }
+  project_corr_body<NO>(b - nCi, orbs, h, corrc, reginv, partr, nG, dorbs, hsub);
 }
 
 // Then: G and h1 from their partials with the energy partials (the first nRed workgroups), and the
-// orbital projection with the right-hand side's factor -i (the others; with corrc set, the int8
-// path's: draw is then h phi or null and part the summed partials, see project_corr_body).
+// orbital projection with the right-hand side's factor -i (the others).
 template <int NO>
 __global__ __launch_bounds__(256) void k_rhs_post(const double* __restrict__ Gpart, int nxs, int Pc,
                                                   double* __restrict__ G, const double2* __restrict__ part1,
@@ -1127,18 +1264,13 @@ __global__ __launch_bounds__(256) void k_rhs_post(const double* __restrict__ Gpa
                                                   double* __restrict__ epart, int nRed,
                                                   const double2* __restrict__ orbs, const double2* __restrict__ draw,
                                                   const double2* __restrict__ part, int nPart, int nG,
-                                                  double2* __restrict__ dorbs, const double2* __restrict__ hsub,
-                                                  const double2* __restrict__ corrc,
-                                                  const double2* __restrict__ reginv) {
+                                                  double2* __restrict__ dorbs, const double2* __restrict__ hsub) {
   int b = blockIdx.x;
   if (b < nRed) {
     gram_reduce_energy_body(b, Gpart, nxs, Pc, G, part1, nPart1, NO, h1, rho1, rho2, epart);
     return;
   }
-  if (corrc)
-    project_corr_body<NO>(b - nRed, orbs, draw, corrc, reginv, part, nG, dorbs, hsub);
-  else
-    project_body<NO>(b - nRed, false, orbs, draw, part, nPart, nG, 1, dorbs, nullptr, nullptr, 0, nullptr, hsub);
+  project_body<NO>(b - nRed, false, orbs, draw, part, nPart, nG, 1, dorbs, nullptr, nullptr, 0, nullptr, hsub);
 }
 
 // ------------------------------------------------------------------ grouped launches
@@ -1221,11 +1353,24 @@ __global__ __launch_bounds__(256) void k_grp_mid(const GroupInstance* __restrict
 }
 
 template <int NO>
-__global__ __launch_bounds__(256) void k_grp_gram(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+__global__ __launch_bounds__(1024) void k_grp_gram(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
-  gram_sum_body<NO>(m.y, a.STt, a.WT, a.nGp, a.Pc, a.Gpart, a.nxs, a.part, a.nGp / 16, a.partr, a.rho1, a.reg,
-                    a.reginv, a.actC);
+  gram_direct_body<NO>(m.y, a.STt, a.WT, a.nGp, a.Pc, a.actC, a.rho2, a.G, a.epart, a.part1, a.nPart1, a.h1, a.part,
+                       a.nGp / 16, a.partr, a.rho1, a.reg, a.reginv);
+}
+
+// the fused int8 groups' last launch (as k_rhs_final)
+template <int NO>
+__global__ __launch_bounds__(256) void k_grp_final(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const GroupInstance& a = gi[m.x];
+  const int nCi = (a.nC + 3) / 4;
+  if (m.y < nCi) {
+    ci_apply_project_body(m.y, a.C, a.nC, NO, a.Pc, a.G, a.h1, a.t, a.epart, a.nRed, a.dCraw, a.dC);
+    return;
+  }
+  project_corr_body<NO>(m.y - nCi, a.C + a.nC, a.hphi, a.draw, a.reginv, a.partr, a.nG, a.dC + a.nC, nullptr);
 }
 
 template <int NO>
@@ -1236,11 +1381,8 @@ __global__ __launch_bounds__(256) void k_grp_post(const GroupInstance* __restric
     gram_reduce_energy_body(m.y, a.Gpart, a.nxs, a.Pc, a.G, a.part1, a.nPart1, NO, a.h1, a.rho1, a.rho2, a.epart);
     return;
   }
-  if (a.partr)  // the int8 path: c_m(x) and the summed overlap partials of the fused contraction
-    project_corr_body<NO>(m.y - a.nRed, a.C + a.nC, a.hphi, a.draw, a.reginv, a.partr, a.nG, a.dC + a.nC, nullptr);
-  else
-    project_body<NO>(m.y - a.nRed, false, a.C + a.nC, a.draw, a.part, a.nPart, a.nG, 1, a.dC + a.nC, nullptr, nullptr,
-                     0, nullptr, nullptr);
+  project_body<NO>(m.y - a.nRed, false, a.C + a.nC, a.draw, a.part, a.nPart, a.nG, 1, a.dC + a.nC, nullptr, nullptr, 0,
+                   nullptr, nullptr);
 }
 
 void launch_grp_front(int nO, const GroupInstance* gi, const int2* map, int nBlocks, int maxC, hipStream_t st) {
@@ -1266,8 +1408,13 @@ void launch_grp_mid(int nO, const GroupInstance* gi, const int2* map, int nBlock
   CODY_NO_SWITCH(nO, CALL_GM)
 #undef CALL_GM
 }
+void launch_grp_final(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
+#define CALL_GFI(NO) hipLaunchKernelGGL(k_grp_final<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+  CODY_NO_SWITCH(nO, CALL_GFI)
+#undef CALL_GFI
+}
 void launch_grp_gram(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st) {
-#define CALL_GG(NO) hipLaunchKernelGGL(k_grp_gram<NO>, dim3(nBlocks), dim3(256), 0, st, gi, map)
+#define CALL_GG(NO) hipLaunchKernelGGL(k_grp_gram<NO>, dim3(nBlocks), dim3(1024), 0, st, gi, map)
   CODY_NO_SWITCH(nO, CALL_GG)
 #undef CALL_GG
 }
@@ -1317,15 +1464,31 @@ void launch_rhs_mid(const double* STt, const double* WT, int nGp, int Pc, double
 #undef CALL_MI
 }
 
-void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs, int nO,
+int gram_direct_energy_parts(int Pc) { return (Pc / 16) * (Pc / 16) + 1; }
+
+void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, int nO, const int* actK,
+                     const double2* rho2, double* G, double* epart, const double2* part1, int nPart1, double2* h1,
                      const double2* part, int nPart, double2* partr, const double2* rho1, double reg,
-                     double2* reginv, hipStream_t st, const int* actK) {
-  const unsigned grid = (unsigned)(Pc / 16 * nxs + 3);
-#define CALL_RG(NO)                                                                                           \
-  hipLaunchKernelGGL(k_rhs_gram<NO>, dim3(grid), dim3(256), 0, st, STt, WT, nGp, Pc, Gpart, nxs, part, nPart, \
-                     partr, rho1, reg, reginv, actK)
+                     double2* reginv, hipStream_t st) {
+  const unsigned grid = (unsigned)((Pc / 16) * (Pc / 16) + 4);
+#define CALL_RG(NO)                                                                                          \
+  hipLaunchKernelGGL(k_rhs_gram<NO>, dim3(grid), dim3(1024), 0, st, STt, WT, nGp, Pc, actK, rho2, G, epart, part1, \
+                     nPart1, h1, part, nPart, partr, rho1, reg, reginv)
   CODY_NO_SWITCH(nO, CALL_RG)
 #undef CALL_RG
+}
+
+void launch_rhs_final(const double2* C, int nC, int nO, int Pc, const double* G, const double2* h1,
+                      const DevCiTables& t, const double* epart, int nE, double2* dCraw, double2* dC,
+                      const double2* orbs, const double2* h, const double2* corrc, const double2* reginv,
+                      const double2* partr, int nG, double2* dorbs, const double2* hsub, hipStream_t st) {
+  const int nCi = (nC + 3) / 4;
+  const unsigned grid = (unsigned)(nCi + (nG + 255) / 256);
+#define CALL_FI(NO)                                                                                          \
+  hipLaunchKernelGGL(k_rhs_final<NO>, dim3(grid), dim3(256), 0, st, C, nC, Pc, G, h1, t, epart, nE, dCraw, dC, nCi, \
+                     orbs, h, corrc, reginv, partr, nG, dorbs, hsub)
+  CODY_NO_SWITCH(nO, CALL_FI)
+#undef CALL_FI
 }
 
 int rhs_post_energy_parts(int Pc, int nO) { return (Pc * Pc + nO * nO + 255) / 256; }
@@ -1333,12 +1496,12 @@ int rhs_post_energy_parts(int Pc, int nO) { return (Pc * Pc + nO * nO + 255) / 2
 void launch_rhs_post(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
                      double2* h1, const double2* rho1, const double2* rho2, double* epart, const double2* orbs,
                      const double2* draw, const double2* part, int nPart, int nG, int nO, double2* dorbs,
-                     const double2* hsub, hipStream_t st, const double2* corrc, const double2* reginv) {
+                     const double2* hsub, hipStream_t st) {
   const int nRed = rhs_post_energy_parts(Pc, nO);
   const unsigned grid = (unsigned)(nRed + (nG + 255) / 256);
 #define CALL_PO(NO)                                                                                                \
   hipLaunchKernelGGL(k_rhs_post<NO>, dim3(grid), dim3(256), 0, st, Gpart, nxs, Pc, G, part1, nPart1, h1, rho1, rho2, \
-                     epart, nRed, orbs, draw, part, nPart, nG, dorbs, hsub, corrc, reginv)
+                     epart, nRed, orbs, draw, part, nPart, nG, dorbs, hsub)
   CODY_NO_SWITCH(nO, CALL_PO)
 #undef CALL_PO
 }

@@ -142,8 +142,7 @@ void launch_ci_apply_project(const double2* C, int nC, int nO, int Pc, const dou
 //   wsum:  R = rho1 (rho1^2 + eps)^-1 and the k-split sum of W^T (nsplit <= 1: none);
 //   mid:   Gram partials (nxs x-ranges) and the orbital correlation with its overlap partials;
 //   post:  G and h1 from their partials with the energy partials epart (one per reduction
-//          workgroup, rhs_post_energy_parts of them), and the orbital projection (times -i); with
-//          corrc set, the int8 path's projection (draw: h phi or null, part: the summed partials).
+//          workgroup, rhs_post_energy_parts of them), and the orbital projection (times -i).
 void launch_rhs_front(const double2* C, int nC, int nO, const DevCiTables& t, double2* rho1, double2* r2can,
                       double2* rho2, const double2* orbs, const double* w, const double* x, const double* vext,
                       const KinLaunch& kl, const double* efield, int nG, double2* hphi, double2* part1, int nGp,
@@ -158,8 +157,7 @@ int rhs_post_energy_parts(int Pc, int nO);
 void launch_rhs_post(const double* Gpart, int nxs, int Pc, double* G, const double2* part1, int nPart1,
                      double2* h1, const double2* rho1, const double2* rho2, double* epart, const double2* orbs,
                      const double2* draw, const double2* part, int nPart, int nG, int nO, double2* dorbs,
-                     const double2* hsub, hipStream_t st, const double2* corrc = nullptr,
-                     const double2* reginv = nullptr);
+                     const double2* hsub, hipStream_t st);
 size_t ci_density_lds_bytes(int nC, int nO);
 int ci_kernels_allow_lds(size_t bytes);
 // pair sources S^T (fragment-tiled STt, optionally row-major ST) and per-fragment activity flags
@@ -227,12 +225,22 @@ struct OrbCorrArgs {
 int launch_mf_i8_orb(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,
                      int nGp, int nO, const OrbCorrArgs& o, hipStream_t st,
                      bool bf16 = false);
-// after launch_mf_i8_orb: the Gram partials (as launch_rhs_mid), the fixed-order sums of the
-// contraction's two sets of nPart overlap partials into partr (2 nO^2; one workgroup each) and
-// R = rho1 (rho1^2 + eps)^-1 (one wave)
-void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, double* Gpart, int nxs, int nO,
+// after launch_mf_i8_orb: G = S^T W directly (a 1024-thread workgroup per 16 x 16 tile, over the
+// active blocks of actK or every block) with its energy shares, h1 and its share, the fixed-order
+// sums of the contraction's two sets of nPart overlap partials into partr (2 nO^2; one workgroup
+// each) and R = rho1 (rho1^2 + eps)^-1 (one wave)
+void launch_rhs_gram(const double* STt, const double* WT, int nGp, int Pc, int nO, const int* actK,
+                     const double2* rho2, double* G, double* epart, const double2* part1, int nPart1, double2* h1,
                      const double2* part, int nPart, double2* partr, const double2* rho1, double reg,
-                     double2* reginv, hipStream_t st, const int* actK = nullptr);
+                     double2* reginv, hipStream_t st);
+// the energy shares the direct-G launch leaves (one per G tile, one for h1)
+int gram_direct_energy_parts(int Pc);
+// the fused int8 path's last launch: CI apply and projection | orbital projection with the R mixing
+void launch_rhs_final(const double2* C, int nC, int nO, int Pc, const double* G, const double2* h1,
+                      const DevCiTables& t, const double* epart, int nE, double2* dCraw, double2* dC,
+                      const double2* orbs, const double2* h, const double2* corrc, const double2* reginv,
+                      const double2* partr, int nG, double2* dorbs, const double2* hsub, hipStream_t st);
+void launch_grp_final(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 int launch_grp_mf_i8_orb(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 void launch_grp_gram(int nO, const GroupInstance* gi, const int2* map, int nBlocks, hipStream_t st);
 int launch_mf_i8_r16(const int8_t* As, const int* eA, const int8_t* Bs, const int* eSb, const int* actK, double* WT,

@@ -67,8 +67,11 @@ int main() {
     EXPECT(c[kPlanFront] == 212 + 128 + 320, "int8 front blocks %lld", (long long)c[kPlanFront]);
     EXPECT(c[kPlanContract] == 256, "int8 contraction blocks %lld", (long long)c[kPlanContract]);
     EXPECT(c[kPlanWsum] == 0, "int8 sum blocks %lld", (long long)c[kPlanWsum]);
-    // the correlation is formed by the contraction: 5 x 32 Gram + two partial-sum workgroups + R
-    EXPECT(c[kPlanMid] == 160 + 3, "int8 Gram blocks %lld", (long long)c[kPlanMid]);
+    // the correlation is formed by the contraction: 5 x 5 direct G tiles + h1 + two partial-sum
+    // workgroups + R; no G-reduction launch; the projection (16 workgroups) beside the CI apply
+    EXPECT(c[kPlanMid] == 25 + 4, "int8 Gram blocks %lld", (long long)c[kPlanMid]);
+    EXPECT(c[kPlanPost] == 0, "int8 post blocks %lld", (long long)c[kPlanPost]);
+    EXPECT(c[kPlanApply] == 18 + 16, "int8 final blocks %lld", (long long)c[kPlanApply]);
     // 12 orbitals: Q (24 tiles x 39 k steps x 64 doubles, four per thread) is formed once per
     // right-hand side beside the Gram partials; the correlation takes one workgroup per 64 points and
     // group of 3 orbitals
