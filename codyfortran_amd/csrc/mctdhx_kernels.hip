@@ -1210,6 +1210,8 @@ __device__ __forceinline__ void gram_direct_body(int b, const double* __restrict
     return;
   }
   if (b == 3) {
+    // one wave: a wave that has ended no longer counts toward the workgroup barrier on CDNA, so
+    // the remaining wave's barriers wait for nobody else
     if (threadIdx.x >= 64) return;
     reginv_body<NO>(rho1, reg, reginv, 64);
     return;
