@@ -1041,9 +1041,9 @@ __global__ __launch_bounds__(256) void k_rhs_wsum(const double2* __restrict__ rh
                                                   double2* __restrict__ reginv, const double* __restrict__ Wpart,
                                                   int nsplit, size_t plane, double* __restrict__ WT, int first) {
   if (blockIdx.x < first) {
-    // one wave: its barriers cost nothing once the other waves have left the workgroup
-    if (threadIdx.x >= 64) return;
-    reginv_body<NO>(rho1, reg, reginv, 64);
+    // every thread of the workgroup: the elimination's row updates (2 nO^2 entries per pivot) take
+    // one round instead of up to five on one wave (the same arithmetic per entry either way)
+    reginv_body<NO>(rho1, reg, reginv, blockDim.x);
     return;
   }
   wsum_body(blockIdx.x - first, Wpart, nsplit, plane, WT);
@@ -1306,9 +1306,8 @@ template <int NO>
 __global__ __launch_bounds__(256) void k_grp_wsum(const GroupInstance* __restrict__ gi, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const GroupInstance& a = gi[m.x];
-  if (m.y < a.wsumFirst) {
-    if (threadIdx.x >= 64) return;
-    reginv_body<NO>(a.rho1, a.reg, a.reginv, 64);
+  if (m.y < a.wsumFirst) {  // (every thread, as in k_rhs_wsum)
+    reginv_body<NO>(a.rho1, a.reg, a.reginv, blockDim.x);
     return;
   }
   wsum_body(m.y - a.wsumFirst, a.Wpart, a.nsplit, (size_t)a.Pc * a.nGp, a.WT);
