@@ -62,8 +62,9 @@ def test_dense_h_is_self_adjoint_in_the_weights(fedvr401):
 
 @pytest.mark.parametrize("krylov_dim,tol,bar", [(32, 1e-13, 1e-12), (12, 1e-10, 1e-10)])
 def test_sil_matches_expm_with_metric(fedvr401, krylov_dim, tol, bar):
-    """FEDVR 401-point grid (100 elements x 5 points), weighted metric; dt = 0.05 (the configs[0]
-    step). The second case is configs[0]'s own Krylov dimension and tolerance."""
+    """FEDVR 401-point grid (100 elements x 5 points), weighted metric; dt = 0.05 (the step of the
+    401-point propagation workload). The second case is that workload's own Krylov dimension (12)
+    and tolerance (1e-10)."""
     w, s, wts, st, H = fedvr401
     dt = 0.05
     U = scipy.linalg.expm(-1j * dt * H)
