@@ -132,3 +132,27 @@ def test_int8_above_eight_orbitals_matches_oracle(M):
         ctx.close()
     print(f"\nint8, M = {M}: W rel err {ew:.3e}, RHS rel err {ed:.3e}, energy rel err {ee:.3e}")
     assert ew < 1e-10 and ed < BAR and ee < BAR
+
+
+@pytest.mark.parametrize("form", ["int8_sliced", "bf16_sliced"])
+def test_sliced_forms_on_a_grid_beyond_64_blocks(form):
+    """8201 points (1640 elements of the 4096-point grid's length on [-400, 400]): 132 k blocks of
+    64 points, so the active-block lists of the contraction and of the direct G tiles walk three
+    words of 64 flags, and the active blocks (|x| < 55) straddle the first two.  Against the
+    oracle, to the same bars as the 4096-point system."""
+    w = small("long_8201", -400.0, 400.0, 1640, 6, 4, 8)
+    assert w.nG == 8201
+    s, x, wts = oracle_system(w)
+    ctx, xg, wg = wl.make_context(w)
+    state = wl.seeded_state(xg, wg, w.N, w.M, seed=5)
+    _, orbs = wl.split_state(state, w.nC, w.nG, w.M)
+    print(f"\n[{form}: nG={w.nG} N={w.N} M={w.M} nC={w.nC}]", flush=True)
+    try:
+        ctx.set_precision(hip.PREC_INT8_SLICED if form == "int8_sliced" else hip.PREC_BF16_SLICED)
+        ew = rel(ctx.meanfield_w(orbs), s.meanfield_w(orbs))
+        ed = state_rel(ctx.time_derivative(state), s.time_derivative(state), w.nC, wts, w.M)
+        ee = abs(ctx.energy(state) - s.energy(state)) / abs(s.energy(state))
+    finally:
+        ctx.close()
+    print(f"{form}, 8201 points: W rel err {ew:.3e}, RHS rel err {ed:.3e}, energy rel err {ee:.3e}")
+    assert ew < 1e-10 and ed < BAR and ee < BAR
